@@ -1,0 +1,206 @@
+// C++ drop-in layer: libfusion's host classes (DenseMapping, DeviceImage,
+// DenseTracking, DenseOdometry, System) re-implemented over the C-ABI of
+// libfusion_amd.h.  Method names and meanings follow the reference headers
+// (include/voxel_hashing/voxel_hashing.h:12-51, include/tracking/device_image.h:18-80,
+// include/tracking/icp_tracker.h:13-88, include/tracking/rgbd_odometry.h:13-41,
+// include/system.h:19-92).  Types the reference takes from third-party
+// libraries change because those libraries are not part of this build:
+//   cv::cuda::GpuMat -> fusion::GpuImage  (owning dense device buffer)
+//   Sophus::SE3d     -> fusion::SE3d      (FP64 rigid transform)
+//   cv::Mat input    -> device pointers to uint16 depth / uint8x3 RGB
+// Every method enqueues work on the owning object's HIP stream and returns
+// without waiting, except the ones documented as synchronising.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <vector>
+
+#include "libfusion_amd.h"
+
+struct LfTrackState;
+
+namespace fusion {
+
+// IntrinsicMatrix (intrinsic_matrix.h:12-47).
+struct IntrinsicMatrix : lf_intrinsics {
+    IntrinsicMatrix();
+    IntrinsicMatrix(int cols, int rows, float fx, float fy, float cx, float cy);
+    IntrinsicMatrix pyr_down() const;
+};
+void BuildIntrinsicPyramid(IntrinsicMatrix base, std::vector<IntrinsicMatrix> &pyr, int num_pyr);
+
+// FP64 rigid transform in place of Sophus::SE3d.
+struct SE3d {
+    double R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    double t[3] = {0, 0, 0};
+    static SE3d exp(const double *xi);  // tangent [upsilon; omega]
+    void log(double *xi) const;
+    SE3d inverse() const;
+    SE3d operator*(const SE3d &o) const;
+    void matrix3x4(double *out12) const;
+    static SE3d from3x4(const double *in12);
+};
+
+// Owning dense device buffer in place of cv::cuda::GpuMat.
+class GpuImage {
+public:
+    GpuImage() = default;
+    GpuImage(int cols, int rows, int elem_bytes) { create(cols, rows, elem_bytes); }
+    ~GpuImage();
+    GpuImage(const GpuImage &) = delete;
+    GpuImage &operator=(const GpuImage &) = delete;
+    GpuImage(GpuImage &&o) noexcept;
+    GpuImage &operator=(GpuImage &&o) noexcept;
+    void create(int cols, int rows, int elem_bytes);
+    bool empty() const { return data_ == nullptr; }
+    template <class T> T *ptr() const { return static_cast<T *>(data_); }
+    int cols = 0, rows = 0, elem = 0;
+
+private:
+    void *data_ = nullptr;
+};
+
+// DeviceImage (device_image.h:18-80): one RGB-D frame and its pyramids.
+class DeviceImage {
+public:
+    DeviceImage(const IntrinsicMatrix &K, int num_pyr, hipStream_t stream);
+    // DeviceImage::upload (device_image.cpp:146-181) from device-resident input.
+    int upload(const uint16_t *depth_u16, const unsigned char *rgb, float depth_scale);
+    // DeviceImage::resize_device_map (device_image.cpp:188-199).
+    int resize_device_map();
+    const GpuImage &get_raw_depth() const { return depth_raw_; }
+    const GpuImage &get_image() const { return image_; }
+    GpuImage &get_depth(int l = 0) { return depth_[l]; }
+    GpuImage &get_vmap(int l = 0) { return vmap_[l]; }
+    GpuImage &get_nmap(int l = 0) { return nmap_[l]; }
+    GpuImage &get_intensity(int l = 0) { return intensity_[l]; }
+    GpuImage &get_intensity_dx(int l = 0) { return dx_[l]; }
+    GpuImage &get_intensity_dy(int l = 0) { return dy_[l]; }
+    const std::vector<IntrinsicMatrix> &intrinsics() const { return K_; }
+    int levels() const { return (int)K_.size(); }
+
+private:
+    std::vector<IntrinsicMatrix> K_;
+    hipStream_t stream_;
+    GpuImage depth_raw_, image_;
+    std::vector<GpuImage> depth_, intensity_, dx_, dy_, vmap_, nmap_;
+};
+
+// DenseMapping (voxel_hashing.h:12-51).
+class DenseMapping {
+public:
+    DenseMapping(const IntrinsicMatrix &K, const lf_map_params &params, hipStream_t stream);
+    ~DenseMapping();
+    DenseMapping(const DenseMapping &) = delete;
+    DenseMapping &operator=(const DenseMapping &) = delete;
+    // update(RgbdImagePtr) (voxel_hashing.cpp:44-65): raw depth, level-0 normals, colour.
+    int update(DeviceImage &frame, const float *pose24_dev, bool colour, hipEvent_t e0 = nullptr,
+               hipEvent_t e1 = nullptr);
+    // update(depth, image, pose) (voxel_hashing.cpp:67-85): weight 1/d^2, no normals.
+    int update(const float *depth, const unsigned char *rgb, const float *pose24_dev);
+    // raycast (voxel_hashing.cpp:87-119) into a float4 vertex map.
+    int raycast(float *vmap, const float *pose24_dev, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+    int reset_mapping();
+    // fetch_mesh_* (voxel_hashing.cpp:142-187); synchronising; return triangle counts.
+    size_t fetch_mesh_vertex_only(float *vertex, int max_triangles);
+    size_t fetch_mesh_with_normal(float *vertex, float *normal, int max_triangles);
+    size_t fetch_mesh_with_colour(float *vertex, unsigned char *colour, int max_triangles);
+    const lf_map_params &params() const { return params_; }
+    const lf_map_storage &storage() const { return storage_; }
+
+private:
+    IntrinsicMatrix K_;
+    lf_map_params params_;
+    lf_map_storage storage_;
+    hipStream_t stream_;
+};
+
+struct TrackingResult {  // icp_tracker.h:13-18 (read back with DenseTracking::result)
+    bool sucess = false;
+    float icp_error = 0;
+    SE3d update;
+};
+
+struct TrackingContext {  // icp_tracker.h:20-26
+    bool use_initial_guess_ = true;
+    std::vector<IntrinsicMatrix> intrinsics_pyr_;
+    std::vector<int> max_iterations_;
+    SE3d initial_estimate_;
+};
+
+// DenseTracking (icp_tracker.h:28-88).  The Gauss-Newton loop runs on the
+// device; compute_transform only enqueues it.
+class DenseTracking {
+public:
+    explicit DenseTracking(hipStream_t stream);
+    ~DenseTracking();
+    int compute_transform(DeviceImage &reference, DeviceImage &current, const TrackingContext &c);
+    LfTrackState *state() const { return state_; }
+    // Synchronising read of the last result (update = estimate^-1).
+    TrackingResult result();
+    float icp_weight = 1e6f;  // icp_tracker.cpp:89-90
+
+private:
+    hipStream_t stream_;
+    LfTrackState *state_ = nullptr;
+    float *scratch_ = nullptr;
+};
+
+// DenseOdometry (rgbd_odometry.h:13-41).
+class DenseOdometry {
+public:
+    DenseOdometry(const IntrinsicMatrix &base, int num_pyr, std::vector<int> iterations, hipStream_t stream);
+    // trackFrame (rgbd_odometry.cpp:18-48) on device-resident input.
+    int trackFrame(const uint16_t *depth, const unsigned char *rgb, float depth_scale, const SE3d &initial_pose);
+    void reset();
+    DeviceImage &get_current_image() { return *curr_; }
+    DeviceImage &get_reference_image() { return *ref_; }
+    DenseTracking &tracker() { return tracker_; }
+    const float *pose24() const;
+    bool trackingLost = false;
+
+private:
+    std::vector<IntrinsicMatrix> K_;
+    std::vector<int> iterations_;
+    hipStream_t stream_;
+    std::unique_ptr<DeviceImage> curr_, ref_;
+    DenseTracking tracker_;
+    bool initialized_ = false;
+};
+
+// System (system.h:19-92).
+class System {
+public:
+    System(const lf_system_config &cfg);
+    ~System();
+    int process_images(const uint16_t *depth, const unsigned char *rgb);
+    int restart();
+    size_t fetch_mesh_vertex_only(float *vertex, int max_triangles);
+    size_t fetch_mesh_with_normal(float *vertex, float *normal, int max_triangles);
+    size_t fetch_mesh_with_colour(float *vertex, unsigned char *colour, int max_triangles);
+    int get_camera_pose(double *out16);  // synchronising
+    hipStream_t stream() const { return stream_; }
+    DenseMapping &mapping() { return *mapping_; }
+    DenseOdometry &odometry() { return *odometry_; }
+    // Event timing of stages: 0 integrate kernel, 1 raycast kernel, 2 tracking,
+    // 3 image pipeline, 4 whole frame.
+    void set_timing(bool on);
+    int read_timing(int stage, int *launches, double *total_ms);
+    bool mapping_enabled = true;
+
+private:
+    lf_system_config cfg_;
+    hipStream_t stream_ = nullptr;
+    std::unique_ptr<DenseMapping> mapping_;
+    std::unique_ptr<DenseOdometry> odometry_;
+    SE3d initial_pose_;
+    bool timing_ = false;
+    struct Ev { int stage; hipEvent_t a, b; };
+    std::vector<Ev> events_;
+    std::vector<hipEvent_t> pool_;
+    hipEvent_t take_event();
+};
+
+}  // namespace fusion

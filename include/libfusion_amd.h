@@ -1,0 +1,280 @@
+/*
+ * libfusion_amd — C-ABI of the MI355X (gfx950) dense-reconstruction hot path.
+ *
+ * Every entry point takes plain device pointers, sizes, POD parameter structs
+ * and an explicit hipStream_t (passed as void* so this header needs no HIP
+ * include), and returns an int status (LF_OK = 0).  Nothing here allocates
+ * per call except the functions whose name ends in _create.
+ *
+ * The cut line is the set of free functions libfusion's host classes call
+ * (reference: include/voxel_hashing/map_proc.h:14-102,
+ * include/tracking/pose_estimator.h:13-39, include/tracking/m_estimator.h:11-24,
+ * include/tracking/cuda_imgproc.h:11-39).  Each declaration below names the
+ * reference function it replaces.
+ *
+ * Conventions shared by all entry points
+ *  - Images are dense row-major (pitch == width * element size).
+ *  - float4 maps (vmap / nmap) are 4 consecutive floats per pixel.
+ *  - RGB images are 3 consecutive uint8 per pixel, channel 0 = R.
+ *  - A device pose block ("pose24") is 24 floats: T_world_camera as a row-major
+ *    3x4 matrix followed by T_camera_world (its inverse) as a row-major 3x4.
+ *    The reference passes Sophus::SE3d by value and casts inside
+ *    (depth_fusion.cu:462,471); here the caller supplies both float matrices
+ *    on the device so a frame step never needs a host round trip.
+ */
+#ifndef LIBFUSION_AMD_H
+#define LIBFUSION_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LF_OK 0
+#define LF_ERR_ARG 1         /* bad argument (null pointer, bad size) */
+#define LF_ERR_HIP 2         /* a HIP runtime call failed */
+#define LF_ERR_CAPACITY 3    /* an explicit capacity was exceeded */
+#define LF_ERR_NOT_READY 4   /* state missing (e.g. raycast before update) */
+
+/* Voxel block edge (voxels) — reference BLOCK_SIZE (map_struct.h:9). */
+#define LF_BLOCK 8
+#define LF_BLOCK3 512
+
+/* Overflow bits reported in counters[LF_CNT_OVERFLOW]. */
+#define LF_OVF_TABLE 1u      /* no free hash slot on a probe path */
+#define LF_OVF_POOL 2u       /* voxel-block pool exhausted */
+#define LF_OVF_TRIANGLES 4u  /* mesh triangle capacity exceeded */
+
+/* Indices into the device counter array of a map. */
+#define LF_CNT_ALLOCATED 0   /* blocks taken from the pool (may exceed pool on overflow) */
+#define LF_CNT_VISIBLE 1     /* visible blocks after the last update */
+#define LF_CNT_OVERFLOW 2    /* LF_OVF_* bits, sticky until reset */
+#define LF_CNT_TRIANGLES 3   /* triangles requested by the last mesh extraction */
+#define LF_CNT_VIS_SUM_LO 4  /* running sum of visible counts (stats), low word */
+#define LF_CNT_VIS_SUM_HI 5  /* high word */
+#define LF_CNT_UPDATES 6     /* number of updates since reset */
+#define LF_CNT_RENDER_BLOCKS 7 /* visible blocks with a non-empty tile rectangle, last raycast */
+#define LF_NUM_COUNTERS 16
+
+/* Camera intrinsics — mirrors fusion::IntrinsicMatrix (intrinsic_matrix.h:12-37).
+ * invfx/invfy must equal 1.0f/fx and 1.0f/fy (lf_intrinsics_make fills them). */
+typedef struct lf_intrinsics {
+    int width, height;
+    float fx, fy, cx, cy, invfx, invfy;
+} lf_intrinsics;
+
+/* Map configuration — mirrors fusion::MapState (map_struct.h:17-51) with the
+ * truncation distance made a parameter (the reference hard-codes 5 voxels,
+ * map_struct.cu:120-123). */
+typedef struct lf_map_params {
+    int num_buckets;          /* home buckets of computeHash (map_struct.cu:143-148) */
+    int num_entries;          /* total hash slots (buckets + excess), >= num_buckets */
+    int num_blocks;           /* voxel-block pool capacity */
+    int max_triangles;        /* mesh capacity, reference num_max_mesh_triangles_ */
+    float zmin_raycast, zmax_raycast;
+    float zmin_update, zmax_update;
+    float voxel_size;         /* metres */
+    float truncation;         /* metres */
+} lf_map_params;
+
+/* Device storage of one voxel-hash map — the counterpart of fusion::MapStorage
+ * (map_struct.h:69-77).  Layout (all arrays in HBM):
+ *   keys[num_entries]        uint64 packed block coordinate, ~0 = empty slot
+ *   slot_block[num_entries]  pool block index of the slot, -1 = none
+ *   slot_flag[num_entries]   1 if the slot's block was touched by the current update
+ *   block_pos[4*num_blocks]  {x, y, z, slot} of every pool block in allocation order
+ *   sdf[512*num_blocks]      int16 TSDF, value = sdf/32767 truncation units
+ *   weight[512*num_blocks]   float32 fusion weight, clamped to 255
+ *   rgb[3*512*num_blocks]    uint8 colour, planar per block: R[512] G[512] B[512]
+ *   visible[num_blocks]      pool block indices visible in the last update
+ *   counters[LF_NUM_COUNTERS]
+ * Voxel i of block b lives at b*512 + z*64 + y*8 + x (map_struct.cu:657-660). */
+typedef struct lf_map_storage {
+    unsigned long long *keys;
+    int *slot_block;
+    unsigned char *slot_flag;
+    int *block_pos;
+    short *sdf;
+    float *weight;
+    unsigned char *rgb;
+    int *visible;
+    unsigned int *counters;
+    float *zrange;            /* raycast z-range tiles, 2 * tiles_x * tiles_y floats */
+    int tiles_x, tiles_y;
+} lf_map_storage;
+
+/* ---------------------------------------------------------------- general */
+
+/* Library version string and the gfx target the kernels were built for. */
+const char *lf_version(void);
+/* Fill an lf_intrinsics the way IntrinsicMatrix's constructor does
+ * (intrinsic_matrix.h:27-31). */
+void lf_intrinsics_make(int width, int height, float fx, float fy, float cx, float cy,
+                        lf_intrinsics *out);
+/* IntrinsicMatrix::pyr_down (intrinsic_matrix.h:33-37). */
+void lf_intrinsics_pyr_down(const lf_intrinsics *in, lf_intrinsics *out);
+/* computeHash (map_struct.cu:143-148): ((x*73856093)^(y*19349669)^(z*83492791)) mod n,
+ * int32 wrap-around, result in [0, n). */
+int lf_compute_hash(int x, int y, int z, int num_buckets);
+/* Copy the marching-cubes tables compiled into the library
+ * (edge[256], tri[256*16] int32) — reference device_tables.h:9-280. */
+void lf_mc_tables(int *edge_out, int *tri_out);
+/* Number of HIP devices visible (0 on a machine without a GPU). */
+int lf_device_count(void);
+/* Last error message of this thread ("" if none). */
+const char *lf_last_error(void);
+
+/* -------------------------------------------------------------- voxel map */
+
+/* MapStruct<true>::create (map_struct.cu:391-414): allocate device storage for
+ * params, z-range tiles for a (width x height) raycast. */
+int lf_map_create(const lf_map_params *p, int width, int height, lf_map_storage *out);
+/* MapStruct<true>::release (map_struct.cu:424-449). */
+int lf_map_release(lf_map_storage *s);
+/* MapStruct<true>::reset (map_struct.cu:74-93): empty table, empty pool,
+ * zero voxels and counters. */
+int lf_map_reset(const lf_map_params *p, const lf_map_storage *s, void *stream);
+/* Copy the device counters to host memory (synchronises the stream). */
+int lf_map_read_counters(const lf_map_storage *s, unsigned int *out16, void *stream);
+
+/* fusion::cuda::update_weighted (depth_fusion.cu:502-575): block allocation along
+ * the truncation band of every valid pixel (create_blocks_kernel, :121-228),
+ * visible-block selection (check_visibility_flag_kernel, :49-83, plus the
+ * compaction :85-93,549-557) and TSDF + colour fusion with weight
+ * |sin(n_z)|/d^2 (update_map_weighted_kernel, :356-428).
+ *   depth  float metres [rows*cols], nmap float4 [rows*cols], rgb uint8x3.
+ *   pose24 device pose block of the frame.
+ * The visible count stays on the device (counters[LF_CNT_VISIBLE]). */
+int lf_update_weighted(const lf_map_params *p, const lf_map_storage *s,
+                       const float *depth, const float *nmap, const unsigned char *rgb,
+                       int cols, int rows, const float *pose24, const lf_intrinsics *K,
+                       void *stream);
+/* fusion::cuda::update (depth_fusion.cu:430-500): same allocation and visibility,
+ * TSDF + colour fusion with weight 1/d^2 and no normal (update_map_with_colour_kernel,
+ * :288-354). */
+int lf_update(const lf_map_params *p, const lf_map_storage *s,
+              const float *depth, const unsigned char *rgb,
+              int cols, int rows, const float *pose24, const lf_intrinsics *K, void *stream);
+
+/* fusion::cuda::create_rendering_blocks + raycast_with_colour (ray_tracing.cu:223-277,
+ * 478-564,611-645) when check_colour != 0, or + raycast (ray_tracing.cu:346-425,
+ * 577-609) when check_colour == 0.  Casts rays of a (cols x rows) camera at pose24
+ * against the visible blocks of the last update, writing vmap float4 (camera
+ * frame, NaN where nothing is hit) and, if rgb_out is non-null, the nearest
+ * voxel colour (255 where nothing is hit). */
+int lf_raycast(const lf_map_params *p, const lf_map_storage *s,
+               float *vmap, unsigned char *rgb_out, int cols, int rows,
+               const float *pose24, const lf_intrinsics *K, int check_colour, void *stream);
+
+/* fusion::cuda::create_mesh_vertex_only / _with_normal / _with_colour
+ * (scene_mesh.cu:260-301,308-351,600-643): marching cubes over every allocated
+ * block.  vertex_out: 9 floats per triangle (metres); normal_out: 9 floats per
+ * triangle (flat normal repeated), may be null; colour_out: 9 uint8 per triangle,
+ * may be null.  Triangle order is unspecified (the reference appends with an
+ * atomic counter).  *triangles_out receives min(count, max_triangles); the
+ * return is LF_ERR_CAPACITY if triangles were dropped. */
+int lf_create_mesh(const lf_map_params *p, const lf_map_storage *s,
+                   float *vertex_out, float *normal_out, unsigned char *colour_out,
+                   int max_triangles, unsigned int *triangles_out, void *stream);
+
+/* --------------------------------------------------------- image pipeline */
+
+/* Depth conversion of System::process_images (system.cpp:43) with the scale as a
+ * parameter: out = in * scale. */
+int lf_depth_to_float(const uint16_t *in, float *out, int cols, int rows, float scale,
+                      void *stream);
+/* filterDepthBilateral (cuda_imgproc.cu:197-200): cv::cuda::bilateralFilter with
+ * ksize 5, sigma_colour 1, sigma_space 1 on metres. */
+int lf_bilateral_filter(const float *in, float *out, int cols, int rows, void *stream);
+/* pyrDownDepth (cuda_imgproc.cu:202-205): resize x0.5, linear, 2x2 mean. */
+int lf_pyr_down_depth(const float *in, float *out, int cols, int rows, void *stream);
+/* pyrDownVMap (cuda_imgproc.cu:212-215): same on float4 maps. */
+int lf_pyr_down_vmap(const float *in, float *out, int cols, int rows, void *stream);
+/* pyrDownImage (cuda_imgproc.cu:207-210): 5x5 Gaussian, keep even pixels,
+ * output ((cols+1)/2, (rows+1)/2). */
+int lf_pyr_down_image(const float *in, float *out, int cols, int rows, void *stream);
+/* cvtColor(RGB2GRAY) on the float image (device_image.cpp:159-162). */
+int lf_rgb_to_intensity(const unsigned char *rgb, float *out, int cols, int rows,
+                        void *stream);
+/* computeDerivative (cuda_imgproc.cu:217-247); last row/column written as 0. */
+int lf_compute_derivative(const float *image, float *dx, float *dy, int cols, int rows,
+                          void *stream);
+/* backProjectDepth (cuda_imgproc.cu:249-274). */
+int lf_back_project_depth(const float *depth, float *vmap, int cols, int rows,
+                          const lf_intrinsics *K, void *stream);
+/* computeNMap (cuda_imgproc.cu:276-305); last row/column written as NaN. */
+int lf_compute_nmap(const float *vmap, float *nmap, int cols, int rows, void *stream);
+
+/* ------------------------------------------------------------- tracking */
+
+/* Scratch for the reductions: lf_reduce_scratch_floats() floats of device memory. */
+size_t lf_reduce_scratch_floats(void);
+/* icp_reduce (pose_estimator.cu:267-307): point-to-plane ICP normal equations of
+ * the reference frame (last_*) against the current frame (curr_*) under the
+ * float 3x4 pose (12 floats, device) mapping reference-camera points into the
+ * current camera.  Writes 29 floats to sum29 (device): the 28 upper-triangle
+ * products of [J|r] followed by the correspondence count. */
+int lf_icp_reduce(const float *curr_vmap, const float *curr_nmap,
+                  const float *last_vmap, const float *last_nmap,
+                  int cols, int rows, const float *pose12, const lf_intrinsics *K,
+                  float *scratch, float *sum29, void *stream);
+/* rgb_step (m_estimator.cu:437-482): Huber-weighted photometric normal equations
+ * with stddev_estimate; same output layout as lf_icp_reduce. */
+int lf_rgb_step(const float *curr_intensity, const float *last_intensity,
+                const float *last_vmap, const float *intensity_dx, const float *intensity_dy,
+                int cols, int rows, float stddev_estimate, const float *pose12,
+                const lf_intrinsics *K, float *scratch, float *sum29, void *stream);
+/* create_jtjjtr<6,7> (reduce_sum.h:48-61) plus residual[2] = {sum r^2, count}:
+ * unpack 29 sums into a symmetric 6x6 (row-major == column-major) and a 6-vector. */
+void lf_unpack_jtjjtr(const float *sum29, float *jtj36, float *jtr6, float *residual2);
+
+/* ------------------------------------------------- pipeline (System facade) */
+
+typedef struct lf_system lf_system;
+
+typedef struct lf_system_config {
+    lf_intrinsics K;          /* level-0 intrinsics */
+    lf_map_params map;
+    int num_pyr;              /* pyramid levels (reference System(K, NUM_PYR)) */
+    int iterations[8];        /* Gauss-Newton iterations per level, [0] = finest */
+    float depth_scale;        /* metres per raw depth unit */
+    int integrate_colour;     /* fuse RGB into the volume */
+} lf_system_config;
+
+/* System::System (system.cpp:12-29). */
+int lf_system_create(const lf_system_config *cfg, lf_system **out);
+int lf_system_destroy(lf_system *sys);
+/* System::process_images (system.cpp:40-114) on device-resident input:
+ * depth uint16 [rows*cols] and rgb uint8x3.  Enqueues the whole frame on the
+ * system's stream without host synchronisation. */
+int lf_system_process_images(lf_system *sys, const uint16_t *depth, const unsigned char *rgb);
+/* Wait for all enqueued frames. */
+int lf_system_synchronize(lf_system *sys);
+/* Camera pose of the last processed frame, row-major 4x4 doubles (synchronises). */
+int lf_system_get_pose(lf_system *sys, double *out16);
+/* Tracking result flags of the last frame: success, icp correspondences. */
+int lf_system_get_tracking(lf_system *sys, int *success, float *icp_error);
+/* System::restart (system.cpp:148-157). */
+int lf_system_restart(lf_system *sys);
+/* System::fetch_mesh_* (system.cpp:168-181); normal_out / colour_out may be null. */
+int lf_system_fetch_mesh(lf_system *sys, float *vertex_out, float *normal_out,
+                         unsigned char *colour_out, int max_triangles,
+                         unsigned int *triangles_out);
+/* Access to the system's map (for inspection and checkpointing). */
+int lf_system_map(lf_system *sys, lf_map_params *p, lf_map_storage *s);
+/* Device pointers of the reference frame's level-l vmap / nmap after the frame. */
+int lf_system_maps(lf_system *sys, int level, const float **vmap, const float **nmap);
+/* The system's HIP stream (as void*). */
+void *lf_system_stream(lf_system *sys);
+/* Kernel timing of the integrate kernel: enable (1) / disable (0) event
+ * bracketing, and read {launches, total_ms}. */
+int lf_system_set_timing(lf_system *sys, int enable);
+int lf_system_read_timing(lf_system *sys, int stage, int *launches, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIBFUSION_AMD_H */

@@ -1,0 +1,341 @@
+"""ctypes binding of libfusion_amd.so (the C-ABI in include/libfusion_amd.h).
+
+torch is used only as the device allocator: every call passes raw device
+pointers (tensor.data_ptr()) and the current HIP stream.  Nothing here falls
+back to a CPU path: if the library is missing, load() raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfusion_amd.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "libfusion_amd.h")
+
+LF_OK, LF_ERR_ARG, LF_ERR_HIP, LF_ERR_CAPACITY, LF_ERR_NOT_READY = 0, 1, 2, 3, 4
+CNT_ALLOCATED, CNT_VISIBLE, CNT_OVERFLOW, CNT_TRIANGLES = 0, 1, 2, 3
+CNT_VIS_SUM_LO, CNT_VIS_SUM_HI, CNT_UPDATES, CNT_RENDER_BLOCKS = 4, 5, 6, 7
+OVF_TABLE, OVF_POOL, OVF_TRIANGLES = 1, 2, 4
+
+
+class LfIntrinsics(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("fx", ctypes.c_float), ("fy", ctypes.c_float), ("cx", ctypes.c_float),
+                ("cy", ctypes.c_float), ("invfx", ctypes.c_float), ("invfy", ctypes.c_float)]
+
+
+class LfMapParams(ctypes.Structure):
+    _fields_ = [("num_buckets", ctypes.c_int), ("num_entries", ctypes.c_int),
+                ("num_blocks", ctypes.c_int), ("max_triangles", ctypes.c_int),
+                ("zmin_raycast", ctypes.c_float), ("zmax_raycast", ctypes.c_float),
+                ("zmin_update", ctypes.c_float), ("zmax_update", ctypes.c_float),
+                ("voxel_size", ctypes.c_float), ("truncation", ctypes.c_float)]
+
+
+class LfMapStorage(ctypes.Structure):
+    _fields_ = [("keys", ctypes.c_void_p), ("slot_block", ctypes.c_void_p), ("slot_flag", ctypes.c_void_p),
+                ("block_pos", ctypes.c_void_p), ("sdf", ctypes.c_void_p), ("weight", ctypes.c_void_p),
+                ("rgb", ctypes.c_void_p), ("visible", ctypes.c_void_p), ("counters", ctypes.c_void_p),
+                ("zrange", ctypes.c_void_p), ("tiles_x", ctypes.c_int), ("tiles_y", ctypes.c_int)]
+
+
+class LfSystemConfig(ctypes.Structure):
+    _fields_ = [("K", LfIntrinsics), ("map", LfMapParams), ("num_pyr", ctypes.c_int),
+                ("iterations", ctypes.c_int * 8), ("depth_scale", ctypes.c_float),
+                ("integrate_colour", ctypes.c_int)]
+
+
+# (name, restype, argtypes) of every exported function; the CPU tests check
+# that this list covers every declaration of the header.
+P, I, F, U = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_uint
+SIGNATURES = [
+    ("lf_version", ctypes.c_char_p, []),
+    ("lf_intrinsics_make", None, [I, I, F, F, F, F, P]),
+    ("lf_intrinsics_pyr_down", None, [P, P]),
+    ("lf_compute_hash", I, [I, I, I, I]),
+    ("lf_mc_tables", None, [P, P]),
+    ("lf_device_count", I, []),
+    ("lf_last_error", ctypes.c_char_p, []),
+    ("lf_map_create", I, [P, I, I, P]),
+    ("lf_map_release", I, [P]),
+    ("lf_map_reset", I, [P, P, P]),
+    ("lf_map_read_counters", I, [P, P, P]),
+    ("lf_update_weighted", I, [P, P, P, P, P, I, I, P, P, P]),
+    ("lf_update", I, [P, P, P, P, I, I, P, P, P]),
+    ("lf_raycast", I, [P, P, P, P, I, I, P, P, I, P]),
+    ("lf_create_mesh", I, [P, P, P, P, P, I, P, P]),
+    ("lf_depth_to_float", I, [P, P, I, I, F, P]),
+    ("lf_bilateral_filter", I, [P, P, I, I, P]),
+    ("lf_pyr_down_depth", I, [P, P, I, I, P]),
+    ("lf_pyr_down_vmap", I, [P, P, I, I, P]),
+    ("lf_pyr_down_image", I, [P, P, I, I, P]),
+    ("lf_rgb_to_intensity", I, [P, P, I, I, P]),
+    ("lf_compute_derivative", I, [P, P, P, I, I, P]),
+    ("lf_back_project_depth", I, [P, P, I, I, P, P]),
+    ("lf_compute_nmap", I, [P, P, I, I, P]),
+    ("lf_reduce_scratch_floats", ctypes.c_size_t, []),
+    ("lf_icp_reduce", I, [P, P, P, P, I, I, P, P, P, P, P]),
+    ("lf_rgb_step", I, [P, P, P, P, P, I, I, F, P, P, P, P, P]),
+    ("lf_unpack_jtjjtr", None, [P, P, P, P]),
+    ("lf_system_create", I, [P, P]),
+    ("lf_system_destroy", I, [P]),
+    ("lf_system_process_images", I, [P, P, P]),
+    ("lf_system_synchronize", I, [P]),
+    ("lf_system_get_pose", I, [P, P]),
+    ("lf_system_get_tracking", I, [P, P, P]),
+    ("lf_system_restart", I, [P]),
+    ("lf_system_fetch_mesh", I, [P, P, P, P, I, P]),
+    ("lf_system_map", I, [P, P, P]),
+    ("lf_system_maps", I, [P, I, P, P]),
+    ("lf_system_stream", P, [P]),
+    ("lf_system_set_timing", I, [P, I]),
+    ("lf_system_read_timing", I, [P, I, P, P]),
+]
+
+_lib = None
+
+
+def load():
+    """Load libfusion_amd.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libfusion_amd.so is not built (run __graft_entry__.build()); "
+                               "there is no fallback path")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+class LfError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != LF_OK:
+        msg = load().lf_last_error().decode(errors="replace")
+        raise LfError("libfusion_amd error %d: %s" % (rc, msg))
+    return rc
+
+
+def intrinsics(width, height, fx, fy, cx, cy):
+    K = LfIntrinsics()
+    load().lf_intrinsics_make(int(width), int(height), float(fx), float(fy), float(cx), float(cy), ctypes.byref(K))
+    return K
+
+
+def pyr_down(K):
+    out = LfIntrinsics()
+    load().lf_intrinsics_pyr_down(ctypes.byref(K), ctypes.byref(out))
+    return out
+
+
+def map_params(num_buckets, num_blocks, voxel_size, truncation, excess_ratio=0.25, zmin=0.3, zmax=5.0,
+               max_triangles=20_000_000):
+    return LfMapParams(int(num_buckets), int(num_buckets + int(num_buckets * excess_ratio)), int(num_blocks),
+                       int(max_triangles), float(zmin), float(zmax), float(zmin), float(zmax),
+                       float(voxel_size), float(truncation))
+
+
+def mc_tables():
+    e = np.zeros(256, np.int32)
+    t = np.zeros((256, 16), np.int32)
+    load().lf_mc_tables(e.ctypes.data_as(P), t.ctypes.data_as(P))
+    return e, t
+
+
+def pose24(T):
+    """Float pose block (T_world_camera 3x4, then its inverse) from a 4x4 FP64 pose."""
+    T = np.asarray(T, np.float64)
+    Ti = np.eye(4)
+    Ti[:3, :3] = T[:3, :3].T
+    Ti[:3, 3] = -T[:3, :3].T @ T[:3, 3]
+    return np.concatenate([T[:3, :].ravel(), Ti[:3, :].ravel()]).astype(np.float32)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def stream_ptr():
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+class Map:
+    """A device voxel-hash map whose arrays are torch tensors (layout of lf_map_storage)."""
+
+    def __init__(self, params, width, height, device="cuda"):
+        torch = _torch()
+        self.params = params
+        E, B = params.num_entries, params.num_blocks
+        tx, ty = width // 8, height // 8
+        self.t = {
+            "keys": torch.empty(E, dtype=torch.int64, device=device),
+            "slot_block": torch.empty(E, dtype=torch.int32, device=device),
+            "slot_flag": torch.empty(E, dtype=torch.uint8, device=device),
+            "block_pos": torch.empty(B * 4, dtype=torch.int32, device=device),
+            "sdf": torch.empty(B * 512, dtype=torch.int16, device=device),
+            "weight": torch.empty(B * 512, dtype=torch.float32, device=device),
+            "rgb": torch.empty(B * 1536, dtype=torch.uint8, device=device),
+            "visible": torch.empty(B, dtype=torch.int32, device=device),
+            "counters": torch.zeros(16, dtype=torch.int32, device=device),
+            "zrange": torch.empty(2 * tx * ty, dtype=torch.float32, device=device),
+        }
+        s = LfMapStorage()
+        for k, v in self.t.items():
+            setattr(s, k, v.data_ptr())
+        s.tiles_x, s.tiles_y = tx, ty
+        self.storage = s
+        self.reset()
+
+    def reset(self):
+        check(load().lf_map_reset(ctypes.byref(self.params), ctypes.byref(self.storage), stream_ptr()))
+
+    def counters(self):
+        return self.t["counters"].cpu().numpy().view(np.uint32).copy()
+
+    def update_weighted(self, depth, nmap, rgb, pose24_dev, K):
+        h, w = depth.shape[:2]
+        check(load().lf_update_weighted(ctypes.byref(self.params), ctypes.byref(self.storage), ptr(depth), ptr(nmap),
+                                        ptr(rgb), w, h, ptr(pose24_dev), ctypes.byref(K), stream_ptr()))
+
+    def update(self, depth, rgb, pose24_dev, K):
+        h, w = depth.shape[:2]
+        check(load().lf_update(ctypes.byref(self.params), ctypes.byref(self.storage), ptr(depth), ptr(rgb), w, h,
+                               ptr(pose24_dev), ctypes.byref(K), stream_ptr()))
+
+    def raycast(self, vmap, pose24_dev, K, check_colour=True, rgb_out=None):
+        h, w = vmap.shape[:2]
+        check(load().lf_raycast(ctypes.byref(self.params), ctypes.byref(self.storage), ptr(vmap), ptr(rgb_out),
+                                w, h, ptr(pose24_dev), ctypes.byref(K), 1 if check_colour else 0, stream_ptr()))
+
+    def mesh(self, mode=1, max_triangles=None):
+        """mode 0 vertices, 1 + normals, 2 + colours -> (tri[n,3,3], extra, count)."""
+        torch = _torch()
+        cap = int(max_triangles if max_triangles is not None else self.params.max_triangles)
+        vtx = torch.empty((max(cap, 1), 3, 3), dtype=torch.float32, device="cuda")
+        nrm = torch.empty((max(cap, 1), 3, 3), dtype=torch.float32, device="cuda") if mode == 1 else None
+        col = torch.empty((max(cap, 1), 3, 3), dtype=torch.uint8, device="cuda") if mode == 2 else None
+        n = ctypes.c_uint(0)
+        rc = load().lf_create_mesh(ctypes.byref(self.params), ctypes.byref(self.storage), ptr(vtx), ptr(nrm),
+                                   ptr(col), cap, ctypes.byref(n), stream_ptr())
+        if rc not in (LF_OK, LF_ERR_CAPACITY):
+            check(rc)
+        n = n.value
+        extra = nrm if mode == 1 else col
+        return vtx[:n], (extra[:n] if extra is not None else None), int(self.counters()[CNT_TRIANGLES])
+
+    def blocks(self):
+        """Allocated block coordinates (n,3) in pool order."""
+        c = self.counters()
+        n = min(int(c[CNT_ALLOCATED]), self.params.num_blocks)
+        return self.t["block_pos"][: 4 * n].view(n, 4)[:, :3].cpu().numpy()
+
+    def visible_blocks(self):
+        c = self.counters()
+        n = int(c[CNT_VISIBLE])
+        idx = self.t["visible"][:n].long()
+        return self.t["block_pos"].view(-1, 4)[idx, :3].cpu().numpy()
+
+    def voxels(self):
+        """(blocks[n,3], sdf int16[n,512], weight[n,512], rgb uint8[n,3,512]) in pool order."""
+        c = self.counters()
+        n = min(int(c[CNT_ALLOCATED]), self.params.num_blocks)
+        return (self.blocks(), self.t["sdf"][: 512 * n].view(n, 512).cpu().numpy(),
+                self.t["weight"][: 512 * n].view(n, 512).cpu().numpy(),
+                self.t["rgb"][: 1536 * n].view(n, 3, 512).cpu().numpy())
+
+
+class System:
+    """lf_system_* (the System facade) over device-resident frames."""
+
+    def __init__(self, K, params, num_pyr, iterations, depth_scale, integrate_colour=True):
+        cfg = LfSystemConfig()
+        cfg.K = K
+        cfg.map = params
+        cfg.num_pyr = int(num_pyr)
+        for i, v in enumerate(iterations):
+            cfg.iterations[i] = int(v)
+        cfg.depth_scale = float(depth_scale)
+        cfg.integrate_colour = 1 if integrate_colour else 0
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        check(load().lf_system_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            load().lf_system_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def process(self, depth_dev, rgb_dev):
+        check(load().lf_system_process_images(self.h, ptr(depth_dev), ptr(rgb_dev)))
+
+    def synchronize(self):
+        check(load().lf_system_synchronize(self.h))
+
+    def pose(self):
+        T = np.zeros((4, 4))
+        check(load().lf_system_get_pose(self.h, T.ctypes.data_as(P)))
+        return T
+
+    def tracking(self):
+        s = ctypes.c_int(0)
+        e = ctypes.c_float(0)
+        check(load().lf_system_get_tracking(self.h, ctypes.byref(s), ctypes.byref(e)))
+        return bool(s.value), float(e.value)
+
+    def map_storage(self):
+        p = LfMapParams()
+        s = LfMapStorage()
+        check(load().lf_system_map(self.h, ctypes.byref(p), ctypes.byref(s)))
+        return p, s
+
+    def counters(self):
+        p, s = self.map_storage()
+        out = np.zeros(16, np.uint32)
+        check(load().lf_map_read_counters(ctypes.byref(s), out.ctypes.data_as(P), load().lf_system_stream(self.h)))
+        return out
+
+    def ref_maps(self, level):
+        v = ctypes.c_void_p()
+        n = ctypes.c_void_p()
+        check(load().lf_system_maps(self.h, int(level), ctypes.byref(v), ctypes.byref(n)))
+        return v.value, n.value
+
+    def mesh(self, vtx, nrm=None, col=None, cap=None):
+        n = ctypes.c_uint(0)
+        rc = load().lf_system_fetch_mesh(self.h, ptr(vtx), ptr(nrm), ptr(col), int(cap or vtx.shape[0]),
+                                         ctypes.byref(n))
+        if rc not in (LF_OK, LF_ERR_CAPACITY):
+            check(rc)
+        return n.value
+
+    def set_timing(self, on):
+        check(load().lf_system_set_timing(self.h, 1 if on else 0))
+
+    def read_timing(self, stage):
+        n = ctypes.c_int(0)
+        t = ctypes.c_double(0)
+        check(load().lf_system_read_timing(self.h, int(stage), ctypes.byref(n), ctypes.byref(t)))
+        return n.value, t.value
+
+    def restart(self):
+        check(load().lf_system_restart(self.h))

@@ -1,0 +1,150 @@
+// Library-wide entry points: errors, intrinsics, hash, tables, map storage.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "lf_common.h"
+#include "mc_tables.h"
+
+static thread_local char g_err[512] = "";
+
+int lf_set_error(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+extern "C" const char *lf_last_error(void) { return g_err; }
+
+extern "C" const char *lf_version(void) { return "libfusion_amd 0.1 (gfx950, wave64)"; }
+
+extern "C" void lf_intrinsics_make(int width, int height, float fx, float fy, float cx, float cy,
+                                   lf_intrinsics *out) {
+    out->width = width;
+    out->height = height;
+    out->fx = fx;
+    out->fy = fy;
+    out->cx = cx;
+    out->cy = cy;
+    out->invfx = 1.0f / fx;
+    out->invfy = 1.0f / fy;
+}
+
+extern "C" void lf_intrinsics_pyr_down(const lf_intrinsics *in, lf_intrinsics *out) {
+    const float s = 0.5f;
+    lf_intrinsics_make((int)(s * (float)in->width), (int)(s * (float)in->height), s * in->fx, s * in->fy,
+                       s * in->cx, s * in->cy, out);
+}
+
+extern "C" int lf_compute_hash(int x, int y, int z, int num_buckets) {
+    return lf::compute_hash(x, y, z, num_buckets);
+}
+
+extern "C" void lf_mc_tables(int *edge_out, int *tri_out) {
+    for (int i = 0; i < 256; ++i) {
+        edge_out[i] = lf_mc_edge_table[i];
+        for (int j = 0; j < 16; ++j) tri_out[i * 16 + j] = lf_mc_tri_table[i][j];
+    }
+}
+
+extern "C" int lf_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+// ------------------------------------------------------------ map storage
+
+static int check_params(const lf_map_params *p) {
+    if (!p) return lf_set_error(LF_ERR_ARG, "null params");
+    if (p->num_buckets <= 0 || p->num_entries < p->num_buckets || p->num_blocks <= 0)
+        return lf_set_error(LF_ERR_ARG, "bad map capacities (buckets %d, entries %d, blocks %d)", p->num_buckets,
+                            p->num_entries, p->num_blocks);
+    if ((long long)p->num_blocks * LF_BLOCK3 > (long long)INT_MAX)
+        return lf_set_error(LF_ERR_ARG, "pool of %d blocks exceeds 32-bit voxel indexing", p->num_blocks);
+    if (!(p->voxel_size > 0) || !(p->truncation > 0) || !(p->zmin_raycast > 0) || !(p->zmin_update >= 0))
+        return lf_set_error(LF_ERR_ARG, "bad voxel size / truncation / z ranges");
+    return LF_OK;
+}
+
+extern "C" int lf_map_create(const lf_map_params *p, int width, int height, lf_map_storage *s) {
+    int rc = check_params(p);
+    if (rc) return rc;
+    if (!s || width <= 0 || height <= 0) return lf_set_error(LF_ERR_ARG, "bad map storage / size");
+    memset(s, 0, sizeof(*s));
+    const size_t E = (size_t)p->num_entries, B = (size_t)p->num_blocks;
+    s->tiles_x = width / 8;
+    s->tiles_y = height / 8;
+    if (s->tiles_x <= 0 || s->tiles_y <= 0) return lf_set_error(LF_ERR_ARG, "raycast size below 8x8");
+    LF_HIP_CHECK(hipMalloc(&s->keys, E * sizeof(unsigned long long)));
+    LF_HIP_CHECK(hipMalloc(&s->slot_block, E * sizeof(int)));
+    LF_HIP_CHECK(hipMalloc(&s->slot_flag, E));
+    LF_HIP_CHECK(hipMalloc(&s->block_pos, B * 4 * sizeof(int)));
+    LF_HIP_CHECK(hipMalloc(&s->sdf, B * LF_BLOCK3 * sizeof(short)));
+    LF_HIP_CHECK(hipMalloc(&s->weight, B * LF_BLOCK3 * sizeof(float)));
+    LF_HIP_CHECK(hipMalloc(&s->rgb, B * LF_BLOCK3 * 3));
+    LF_HIP_CHECK(hipMalloc(&s->visible, B * sizeof(int)));
+    LF_HIP_CHECK(hipMalloc(&s->counters, LF_NUM_COUNTERS * sizeof(unsigned int)));
+    LF_HIP_CHECK(hipMalloc(&s->zrange, 2 * (size_t)s->tiles_x * s->tiles_y * sizeof(float)));
+    return LF_OK;
+}
+
+extern "C" int lf_map_release(lf_map_storage *s) {
+    if (!s) return LF_OK;
+    void *ptrs[] = {s->keys, s->slot_block, s->slot_flag, s->block_pos, s->sdf, s->weight,
+                    s->rgb, s->visible, s->counters, s->zrange};
+    for (void *q : ptrs)
+        if (q) LF_HIP_CHECK(hipFree(q));
+    memset(s, 0, sizeof(*s));
+    return LF_OK;
+}
+
+__global__ void k_reset_table(unsigned long long *keys, int *slot_block, unsigned char *slot_flag, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        keys[i] = lf::kEmptyKey;
+        slot_block[i] = -1;
+        slot_flag[i] = 0;
+    }
+}
+
+__global__ void k_zero_u32(unsigned int *p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+__global__ void k_zero_u128(uint4 *p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = make_uint4(0, 0, 0, 0);
+}
+
+// MapStruct::reset (map_struct.cu:74-93).  Zero-fill kernels rather than
+// memset nodes keep the reset capturable in a HIP graph.
+extern "C" int lf_map_reset(const lf_map_params *p, const lf_map_storage *s, void *stream) {
+    int rc = check_params(p);
+    if (rc) return rc;
+    hipStream_t st = lf_stream(stream);
+    const int E = p->num_entries;
+    k_reset_table<<<std::min(4096, (E + 255) / 256), 256, 0, st>>>(s->keys, s->slot_block, s->slot_flag, E);
+    LF_LAUNCH_CHECK();
+    k_zero_u32<<<1, 64, 0, st>>>(s->counters, LF_NUM_COUNTERS);
+    LF_LAUNCH_CHECK();
+    const size_t B = (size_t)p->num_blocks;
+    // sdf (2 B), weight (4 B), rgb (3 B) per voxel; every array is a multiple of 512 bytes.
+    struct { void *ptr; size_t bytes; } arrs[] = {{s->sdf, B * 1024}, {s->weight, B * 2048}, {s->rgb, B * 1536}};
+    for (auto &a : arrs) {
+        size_t n16 = a.bytes / 16;
+        int grid = (int)std::min<size_t>(8192, (n16 + 255) / 256);
+        k_zero_u128<<<grid, 256, 0, st>>>(reinterpret_cast<uint4 *>(a.ptr), n16);
+        LF_LAUNCH_CHECK();
+    }
+    return LF_OK;
+}
+
+extern "C" int lf_map_read_counters(const lf_map_storage *s, unsigned int *out16, void *stream) {
+    hipStream_t st = lf_stream(stream);
+    LF_HIP_CHECK(hipMemcpyAsync(out16, s->counters, LF_NUM_COUNTERS * sizeof(unsigned int),
+                                hipMemcpyDeviceToHost, st));
+    LF_HIP_CHECK(hipStreamSynchronize(st));
+    return LF_OK;
+}

@@ -1,0 +1,111 @@
+// Shared device helpers of the gfx950 kernels.
+//
+// Float arithmetic is written in the operation order of libfusion's POD math
+// (include/math/vector_type.h, include/math/matrix_type.h) and every HIP file
+// is compiled with -ffp-contract=off, so each product and sum is rounded on its
+// own exactly as on the CPU: block coordinates, projections and visibility
+// tests agree bit for bit with a CPU restatement on the same inputs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <climits>
+
+#include "libfusion_amd.h"
+
+#define LF_DEV __device__ __forceinline__
+#define LF_HD __host__ __device__ __forceinline__
+
+namespace lf {
+
+constexpr unsigned long long kEmptyKey = ~0ull;
+constexpr int kBad = INT_MIN;
+constexpr int kWave = 64;
+
+struct f3 { float x, y, z; };
+LF_HD f3 add(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+LF_HD f3 sub(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+LF_HD f3 mul(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+LF_HD f3 dvd(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+LF_HD float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+LF_HD f3 cross(f3 a, f3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+LF_HD float norm(f3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+LF_HD f3 normalised(f3 a) { return dvd(a, norm(a)); }
+
+// Matrix3x4 applied to a point / rotated (matrix_type.h:86-98).
+LF_HD f3 xform(const float *m, f3 v) {
+    return {m[0] * v.x + m[1] * v.y + m[2] * v.z + m[3], m[4] * v.x + m[5] * v.y + m[6] * v.z + m[7],
+            m[8] * v.x + m[9] * v.y + m[10] * v.z + m[11]};
+}
+LF_HD f3 rotate(const float *m, f3 v) {
+    return {m[0] * v.x + m[1] * v.y + m[2] * v.z, m[4] * v.x + m[5] * v.y + m[6] * v.z,
+            m[8] * v.x + m[9] * v.y + m[10] * v.z};
+}
+
+struct i3 { int x, y, z; };
+
+// ToVector3i (vector_type.h:218-225) with NaN / out-of-range sent to kBad.
+LF_HD int floor_i(float v) {
+    if (!(v > -2147483520.0f && v < 2147483520.0f)) return kBad;
+    int b = (int)v;
+    return (float)b > v ? b - 1 : b;
+}
+LF_HD i3 to_i3(f3 v) { return {floor_i(v.x), floor_i(v.y), floor_i(v.z)}; }
+// __float2int_rd of a projected coordinate; non-finite / huge values leave every image.
+LF_HD int proj_floor(float t) {
+    if (!(t > -1e9f && t < 1e9f)) return kBad;
+    return (int)floorf(t);
+}
+// voxelPosToBlockPos (map_struct.cu:626-636): floor division by 8.
+LF_HD int blk(int v) { return v == kBad ? kBad : (v >> 3); }
+LF_HD int lidx(int x, int y, int z) { return ((z & 7) << 6) | ((y & 7) << 3) | (x & 7); }
+
+// computeHash (map_struct.cu:143-148).
+LF_HD int compute_hash(int x, int y, int z, int n) {
+    uint32_t a = (uint32_t)x * 73856093u, b = (uint32_t)y * 19349669u, c = (uint32_t)z * 83492791u;
+    int res = (int)(a ^ b ^ c);
+    res %= n;
+    return res < 0 ? res + n : res;
+}
+
+// Block coordinate packed into 63 bits (21 bits per axis, two's complement).
+LF_HD unsigned long long pack_key(int x, int y, int z) {
+    return ((unsigned long long)(x & 0x1FFFFF) << 42) | ((unsigned long long)(y & 0x1FFFFF) << 21) |
+           (unsigned long long)(z & 0x1FFFFF);
+}
+
+LF_HD float unpack_sdf(short v) { return (float)v / 32767.0f; }
+LF_HD short pack_sdf(float v) { return (short)(v * 32767.0f); }
+
+// Linear-probing lookup from the computeHash home bucket; returns the pool
+// block of (bx, by, bz) or -1.  The probe is bounded by the table size.
+LF_DEV int find_block(const unsigned long long *__restrict__ keys, const int *__restrict__ slot_block,
+                      int bx, int by, int bz, int num_buckets, int num_entries) {
+    if (bx == kBad || by == kBad || bz == kBad) return -1;
+    const unsigned long long key = pack_key(bx, by, bz);
+    int slot = compute_hash(bx, by, bz, num_buckets);
+    for (int i = 0; i < num_entries; ++i) {
+        unsigned long long k = keys[slot];
+        if (k == key) return slot_block[slot];
+        if (k == kEmptyKey) return -1;
+        if (++slot == num_entries) slot = 0;
+    }
+    return -1;
+}
+
+LF_DEV int lane_id() { return __lane_id(); }
+
+}  // namespace lf
+
+// Error reporting (common.hip).
+int lf_set_error(int code, const char *fmt, ...);
+#define LF_HIP_CHECK(call)                                                               \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return lf_set_error(LF_ERR_HIP, "%s failed: %s (%s:%d)", #call,              \
+                                hipGetErrorString(e_), __FILE__, __LINE__);              \
+    } while (0)
+#define LF_LAUNCH_CHECK() LF_HIP_CHECK(hipGetLastError())
+
+static inline hipStream_t lf_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }

@@ -1,0 +1,347 @@
+// Voxel-hash block allocation, visible-block selection and TSDF integration.
+//
+// Reference: src/voxel_hashing/depth_fusion.cu (create_blocks_kernel :121-228,
+// check_visibility_flag_kernel :49-83, copy_visible_block_kernel :85-93,
+// update_map_kernel / update_map_with_colour_kernel / update_map_weighted_kernel
+// :230-428, host update / update_weighted :430-575) and the hash-map primitives
+// of src/data_struct/map_struct.cu (computeHash :143-148, createBlock :280-325,
+// findEntry :365-381).
+//
+// MI355X design (not a translation of the reference's scheme):
+//  * The table is open addressing with linear probing from computeHash's home
+//    bucket over num_entries slots.  A slot holds a 63-bit packed block
+//    coordinate claimed with one 64-bit CAS — no bucket locks, so no request is
+//    ever dropped (the reference drops an allocation when a bucket lock is
+//    contended, map_struct.cu:303-324).
+//  * Pool blocks are handed out by an atomic counter and their coordinates are
+//    recorded in allocation order (block_pos), so visibility runs over the
+//    allocated blocks, O(allocated), instead of scanning every table entry.
+//  * A 16x16-pixel workgroup first deduplicates its requests in an LDS hash set
+//    (ds_cmpst_b64): neighbouring pixels request the same few blocks, so only
+//    unique keys reach HBM.
+//  * The visible list is appended with one atomic per wavefront (ballot + popc).
+//  * Integration walks the visible list with a fixed grid (grid-stride), one
+//    256-thread workgroup per 8^3 block: each lane owns two voxels of the
+//    block's contiguous 512-voxel run, so every voxel plane is read and written
+//    with fully coalesced 128-byte wavefront accesses.
+#include "lf_common.h"
+#include "lf_internal.h"
+
+using namespace lf;
+
+namespace {
+
+struct MapDev {
+    unsigned long long *keys;
+    int *slot_block;
+    unsigned char *slot_flag;
+    int *block_pos;
+    short *sdf;
+    float *weight;
+    unsigned char *rgb;
+    int *visible;
+    unsigned int *counters;
+    int num_buckets, num_entries, num_blocks;
+    float voxel_size, truncation, zmin_update, zmax_update;
+};
+
+MapDev make_dev(const lf_map_params *p, const lf_map_storage *s) {
+    MapDev d;
+    d.keys = s->keys; d.slot_block = s->slot_block; d.slot_flag = s->slot_flag; d.block_pos = s->block_pos;
+    d.sdf = s->sdf; d.weight = s->weight; d.rgb = s->rgb; d.visible = s->visible; d.counters = s->counters;
+    d.num_buckets = p->num_buckets; d.num_entries = p->num_entries; d.num_blocks = p->num_blocks;
+    d.voxel_size = p->voxel_size; d.truncation = p->truncation;
+    d.zmin_update = p->zmin_update; d.zmax_update = p->zmax_update;
+    return d;
+}
+
+// createBlock (map_struct.cu:280-325) as a lock-free insert.  Marks the slot as
+// touched by this update (the reference's flag = 2, depth_fusion.cu:157).
+LF_DEV void insert_block(const MapDev &m, int bx, int by, int bz) {
+    const unsigned long long key = pack_key(bx, by, bz);
+    int slot = compute_hash(bx, by, bz, m.num_buckets);
+    for (int i = 0; i < m.num_entries; ++i) {
+        // A plain load may see a stale empty slot (another XCD's L2); the CAS is
+        // authoritative.  A non-empty key never changes until the next reset.
+        unsigned long long k = m.keys[slot];
+        bool claimed = false;
+        if (k == kEmptyKey) {
+            k = atomicCAS(&m.keys[slot], kEmptyKey, key);
+            claimed = (k == kEmptyKey);
+        }
+        if (claimed) {
+            unsigned int b = atomicAdd(&m.counters[LF_CNT_ALLOCATED], 1u);
+            if (b < (unsigned int)m.num_blocks) {
+                m.slot_block[slot] = (int)b;
+                reinterpret_cast<int4 *>(m.block_pos)[b] = make_int4(bx, by, bz, slot);
+            } else {
+                atomicOr(&m.counters[LF_CNT_OVERFLOW], LF_OVF_POOL);
+            }
+            m.slot_flag[slot] = 1;
+            return;
+        }
+        if (k == key) {
+            m.slot_flag[slot] = 1;
+            return;
+        }
+        if (++slot == m.num_entries) slot = 0;
+    }
+    atomicOr(&m.counters[LF_CNT_OVERFLOW], LF_OVF_TABLE);
+}
+
+// One pixel of create_blocks_kernel (depth_fusion.cu:121-228): the 3-D
+// Bresenham walk over the blocks between the near and far ends of the
+// truncation band around the measured depth.
+template <class F>
+LF_DEV void band_blocks(int x, int y, float z, const float *Twc, const lf_intrinsics &K, const MapDev &m, F emit) {
+    const float zmin = m.zmin_update, zmax = m.zmax_update;
+    if (isnan(z) || z < zmin || z > zmax) return;
+    const float half = m.truncation * 0.5f;
+    const float zn = fmaxf(zmin, z - half);
+    const float zf = fminf(zmax, z + half);
+    if (zn >= zf) return;
+    const float vs = m.voxel_size;
+    const float ux = K.invfx * ((float)x - K.cx), uy = K.invfy * ((float)y - K.cy);
+    f3 pn = xform(Twc, f3{ux * zn, uy * zn, zn});
+    f3 pf = xform(Twc, f3{ux * zf, uy * zf, zf});
+    i3 vn = to_i3(dvd(pn, vs)), vf = to_i3(dvd(pf, vs));
+    if (vn.x == kBad || vn.y == kBad || vn.z == kBad || vf.x == kBad || vf.y == kBad || vf.z == kBad) return;
+    i3 a = {blk(vn.x), blk(vn.y), blk(vn.z)}, b = {blk(vf.x), blk(vf.y), blk(vf.z)};
+    const int dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+    const int sx = dx < 0 ? -1 : 1, sy = dy < 0 ? -1 : 1, sz = dz < 0 ? -1 : 1;
+    const int ax = abs(dx), ay = abs(dy), az = abs(dz);
+    const int ex = ax << 1, ey = ay << 1, ez = az << 1;
+    emit(a.x, a.y, a.z);
+    if (ax >= ay && ax >= az) {
+        int e1 = ey - 1, e2 = ez - 1;
+        for (int i = 0; i < ax; ++i) {
+            if (e1 > 0) { a.y += sy; e1 -= ex; }
+            if (e2 > 0) { a.z += sz; e2 -= ex; }
+            e1 += ey; e2 += ez; a.x += sx;
+            emit(a.x, a.y, a.z);
+        }
+    } else if (ay >= ax && ay >= az) {
+        int e1 = ex - 1, e2 = ez - 1;
+        for (int i = 0; i < ay; ++i) {
+            if (e1 > 0) { a.x += sx; e1 -= ey; }
+            if (e2 > 0) { a.z += sz; e2 -= ey; }
+            e1 += ex; e2 += ez; a.y += sy;
+            emit(a.x, a.y, a.z);
+        }
+    } else {
+        int e1 = ey - 1, e2 = ex - 1;
+        for (int i = 0; i < az; ++i) {
+            if (e1 > 0) { a.y += sy; e1 -= ez; }
+            if (e2 > 0) { a.x += sx; e2 -= ez; }
+            e1 += ey; e2 += ex; a.z += sz;
+            emit(a.x, a.y, a.z);
+        }
+    }
+}
+
+constexpr int kDedup = 1024;  // LDS request set per 16x16 tile (8 KiB)
+
+LF_DEV unsigned int key_mix(unsigned long long k) {
+    k ^= k >> 29;
+    k *= 0xBF58476D1CE4E5B9ull;
+    k ^= k >> 32;
+    return (unsigned int)k;
+}
+
+__global__ __launch_bounds__(256) void k_alloc(MapDev m, const float *__restrict__ depth, int cols, int rows,
+                                               const float *__restrict__ pose24, lf_intrinsics K) {
+    __shared__ unsigned long long s_set[kDedup];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < kDedup; i += 256) s_set[i] = kEmptyKey;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) m.counters[LF_CNT_VISIBLE] = 0u;
+    float Twc[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) Twc[i] = pose24[i];
+    __syncthreads();
+    const int x = blockIdx.x * 16 + (tid & 15), y = blockIdx.y * 16 + (tid >> 4);
+    if (x >= cols || y >= rows) return;
+    band_blocks(x, y, depth[(size_t)y * cols + x], Twc, K, m, [&](int bx, int by, int bz) {
+        const unsigned long long key = pack_key(bx, by, bz);
+        unsigned int h = key_mix(key) & (kDedup - 1);
+        for (int probe = 0; probe < kDedup; ++probe) {
+            unsigned long long old = atomicCAS(&s_set[h], kEmptyKey, key);
+            if (old == kEmptyKey) { insert_block(m, bx, by, bz); return; }
+            if (old == key) return;  // another lane of this tile owns the request
+            h = (h + 1) & (kDedup - 1);
+        }
+        insert_block(m, bx, by, bz);  // LDS set full: go straight to the table
+    });
+}
+
+// is_block_visible (depth_fusion.cu:27-47): any of the 8 block corners inside
+// the image with z in [zmin, zmax] of the update.
+LF_DEV bool block_visible(int bx, int by, int bz, const float *Tcw, const lf_intrinsics &K, int cols, int rows,
+                          float zmin, float zmax, float scale) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const float wx = (float)(bx + (c & 1)) * scale, wy = (float)(by + ((c >> 1) & 1)) * scale,
+                    wz = (float)(bz + ((c >> 2) & 1)) * scale;
+        f3 pt = xform(Tcw, f3{wx, wy, wz});
+        float u = K.fx * pt.x / pt.z + K.cx, v = K.fy * pt.y / pt.z + K.cy;
+        if (!(u < 0.0f || v < 0.0f || u > (float)(cols - 1) || v > (float)(rows - 1) || pt.z < zmin || pt.z > zmax))
+            return true;
+    }
+    return false;
+}
+
+// check_visibility_flag_kernel + exclusive scan + copy (depth_fusion.cu:49-93,
+// 549-557) over the allocated blocks: touched-this-update, or frustum-visible.
+__global__ __launch_bounds__(256) void k_visible(MapDev m, const float *__restrict__ pose24, lf_intrinsics K,
+                                                 int cols, int rows) {
+    float Tcw[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) Tcw[i] = pose24[12 + i];
+    const unsigned int n = min(m.counters[LF_CNT_ALLOCATED], (unsigned int)m.num_blocks);
+    const float scale = 8.0f * m.voxel_size;
+    const unsigned int stride = gridDim.x * blockDim.x;
+    const unsigned int n_round = (n + kWave - 1) / kWave * kWave;
+    const int lane = lane_id();
+    for (unsigned int b = blockIdx.x * blockDim.x + threadIdx.x; b < n_round; b += stride) {
+        bool vis = false;
+        if (b < n) {
+            const int4 p = reinterpret_cast<const int4 *>(m.block_pos)[b];
+            if (m.slot_flag[p.w]) {
+                vis = true;
+                m.slot_flag[p.w] = 0;
+            } else {
+                vis = block_visible(p.x, p.y, p.z, Tcw, K, cols, rows, m.zmin_update, m.zmax_update, scale);
+            }
+        }
+        const unsigned long long ballot = __ballot(vis);
+        if (ballot == 0ull) continue;
+        const int leader = __ffsll((long long)ballot) - 1;
+        unsigned int base = 0;
+        if (lane == leader) base = atomicAdd(&m.counters[LF_CNT_VISIBLE], (unsigned int)__popcll(ballot));
+        base = __shfl(base, leader);
+        if (vis) m.visible[base + __popcll(ballot & ((1ull << lane) - 1ull))] = (int)b;
+    }
+}
+
+// update_map_weighted_kernel (kWeighted, depth_fusion.cu:356-428) and
+// update_map_with_colour_kernel (!kWeighted, :288-354).
+template <bool kWeighted, bool kColour>
+__global__ __launch_bounds__(256) void k_integrate(MapDev m, const float *__restrict__ depth,
+                                                   const float *__restrict__ nmap,
+                                                   const unsigned char *__restrict__ rgb, int cols, int rows,
+                                                   const float *__restrict__ pose24, lf_intrinsics K) {
+    float Tcw[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) Tcw[i] = pose24[12 + i];
+    const int nvis = (int)m.counters[LF_CNT_VISIBLE];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(&m.counters[LF_CNT_UPDATES], 1u);
+        atomicAdd(reinterpret_cast<unsigned long long *>(&m.counters[LF_CNT_VIS_SUM_LO]), (unsigned long long)nvis);
+    }
+    const float vs = m.voxel_size, trunc = m.truncation, inv_trunc = 1.0f / trunc;
+    const float zmin = m.zmin_update, zmax = m.zmax_update;
+    const int t = threadIdx.x;
+    const int lx = t & 7, ly = (t >> 3) & 7, lz0 = t >> 6;
+    for (int i = blockIdx.x; i < nvis; i += gridDim.x) {
+        const int b = m.visible[i];
+        const int4 p = reinterpret_cast<const int4 *>(m.block_pos)[b];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int lz = lz0 + 4 * h;
+            const float wx = (float)(p.x * 8 + lx) * vs, wy = (float)(p.y * 8 + ly) * vs, wz = (float)(p.z * 8 + lz) * vs;
+            const f3 pt = xform(Tcw, f3{wx, wy, wz});
+            const int u = proj_floor(K.fx * pt.x / pt.z + K.cx + 0.5f);
+            const int v = proj_floor(K.fy * pt.y / pt.z + K.cy + 0.5f);
+            if (u < 0 || v < 0 || u > cols - 1 || v > rows - 1) continue;
+            const size_t pix = (size_t)v * cols + u;
+            const float dist = depth[pix];
+            float nz = 0.0f;
+            if (kWeighted) {
+                const float nx = nmap[4 * pix];
+                nz = nmap[4 * pix + 2];
+                if (isnan(dist) || isnan(nx) || dist > zmax || dist < zmin) continue;
+            } else {
+                if (isnan(dist) || (double)dist < 1e-2 || dist > zmax || dist < zmin) continue;
+            }
+            float sdf = dist - pt.z;
+            if (sdf < -trunc) continue;
+            sdf = fminf(1.0f, sdf * inv_trunc);
+            const size_t vi = (size_t)b * LF_BLOCK3 + (size_t)(t + 256 * h);
+            const float wp = m.weight[vi];
+            const float wt = kWeighted ? fabsf(sinf(nz)) / (dist * dist) : 1.0f / (dist * dist);
+            unsigned char *c = m.rgb + (size_t)b * (3 * LF_BLOCK3) + (size_t)(t + 256 * h);
+            if (wp == 0.0f) {
+                m.sdf[vi] = pack_sdf(sdf);
+                m.weight[vi] = wt > 255.0f ? 255.0f : wt;
+                if (kColour) {
+                    c[0] = rgb[3 * pix]; c[LF_BLOCK3] = rgb[3 * pix + 1]; c[2 * LF_BLOCK3] = rgb[3 * pix + 2];
+                }
+                continue;
+            }
+            const float sp = unpack_sdf(m.sdf[vi]);
+            const float fused = (sp * wp + sdf * wt) / (wp + wt);
+            m.sdf[vi] = pack_sdf(fused);
+            const float wn = wp + wt;
+            m.weight[vi] = wn > 255.0f ? 255.0f : wn;
+            if (kColour) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const float mixed = ((float)c[k * LF_BLOCK3] * wp + (float)rgb[3 * pix + k] * wt) / (wp + wt);
+                    c[k * LF_BLOCK3] = (unsigned char)(int)mixed;
+                }
+            }
+        }
+    }
+}
+
+int integrate_grid() { return 4096; }
+
+int check_update_args(const lf_map_params *p, const lf_map_storage *s, const float *depth, int cols, int rows,
+                      const float *pose24, const lf_intrinsics *K) {
+    if (!p || !s || !depth || !pose24 || !K) return lf_set_error(LF_ERR_ARG, "null argument to update");
+    if (cols <= 0 || rows <= 0) return lf_set_error(LF_ERR_ARG, "bad image size %dx%d", cols, rows);
+    return LF_OK;
+}
+
+}  // namespace
+
+int lf_update_ev(const lf_map_params *p, const lf_map_storage *s, const float *depth, const float *nmap,
+                 const unsigned char *rgb, int cols, int rows, const float *pose24, const lf_intrinsics *K,
+                 hipStream_t st, bool weighted, hipEvent_t e0, hipEvent_t e1) {
+    const MapDev m = make_dev(p, s);
+    dim3 g((cols + 15) / 16, (rows + 15) / 16);
+    k_alloc<<<g, 256, 0, st>>>(m, depth, cols, rows, pose24, *K);
+    LF_LAUNCH_CHECK();
+    int nb = std::min(2048, std::max(1, (p->num_blocks + 255) / 256));
+    k_visible<<<nb, 256, 0, st>>>(m, pose24, *K, cols, rows);
+    LF_LAUNCH_CHECK();
+    const int G = integrate_grid();
+    if (e0) LF_HIP_CHECK(hipEventRecord(e0, st));
+    if (weighted) {
+        if (rgb) k_integrate<true, true><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K);
+        else k_integrate<true, false><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K);
+    } else {
+        if (rgb) k_integrate<false, true><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K);
+        else k_integrate<false, false><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K);
+    }
+    LF_LAUNCH_CHECK();
+    if (e1) LF_HIP_CHECK(hipEventRecord(e1, st));
+    return LF_OK;
+}
+
+extern "C" int lf_update_weighted(const lf_map_params *p, const lf_map_storage *s, const float *depth,
+                                  const float *nmap, const unsigned char *rgb, int cols, int rows,
+                                  const float *pose24, const lf_intrinsics *K, void *stream) {
+    int rc = check_update_args(p, s, depth, cols, rows, pose24, K);
+    if (rc) return rc;
+    if (!nmap) return lf_set_error(LF_ERR_ARG, "update_weighted needs a normal map");
+    return lf_update_ev(p, s, depth, nmap, rgb, cols, rows, pose24, K, lf_stream(stream), true, nullptr, nullptr);
+}
+
+extern "C" int lf_update(const lf_map_params *p, const lf_map_storage *s, const float *depth,
+                         const unsigned char *rgb, int cols, int rows, const float *pose24, const lf_intrinsics *K,
+                         void *stream) {
+    int rc = check_update_args(p, s, depth, cols, rows, pose24, K);
+    if (rc) return rc;
+    return lf_update_ev(p, s, depth, nullptr, rgb, cols, rows, pose24, K, lf_stream(stream), false, nullptr,
+                        nullptr);
+}

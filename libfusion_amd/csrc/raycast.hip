@@ -1,0 +1,260 @@
+// Raycasting of the hashed TSDF volume.
+//
+// Reference: src/voxel_hashing/ray_tracing.cu — create_rendering_blocks
+// (RenderingBlockDelegate :18-211, host :223-277), MapRenderingDelegate
+// (:279-564: read_sdf :291-305, read_sdf_interped :307-339, operator() :346-425,
+// raycast_with_colour :478-564), launchers :577-645.
+//
+// MI355X design:
+//  * z-range tiles (one per 8x8 pixels) take the min / max depth of every
+//    visible block's projected rectangle.  All depths are positive, so IEEE
+//    ordering equals signed-integer ordering and plain 32-bit atomicMin/Max
+//    replace the reference's CAS loops; the rectangle is filled directly (the
+//    reference's 16x16 sub-rectangle list covers exactly the same tiles and only
+//    bounds the work at 100k entries).
+//  * One wavefront marches the 64 rays of one 8x8 tile: those rays share the
+//    tile's z-range and neighbouring directions, so their loop trip counts and
+//    the voxel blocks they touch are nearly the same (little divergence, good
+//    L2 locality).
+//  * Each lane caches the last block it resolved: consecutive samples step at
+//    most a few voxels, so most samples and most trilinear taps skip the hash
+//    probe entirely.
+#include "lf_common.h"
+#include "lf_internal.h"
+
+using namespace lf;
+
+namespace {
+
+struct RayDev {
+    const unsigned long long *keys;
+    const int *slot_block;
+    const short *sdf;
+    const float *weight;
+    const unsigned char *rgb;
+    const int *visible;
+    const int *block_pos;
+    unsigned int *counters;
+    float *zx, *zy;
+    int tiles_x, tiles_y;
+    int num_buckets, num_entries;
+    float voxel_size, truncation, zmin_raycast, zmax_raycast;
+};
+
+__global__ void k_zrange_clear(RayDev r) {
+    const int n = r.tiles_x * r.tiles_y;
+    if (blockIdx.x == 0 && threadIdx.x == 0) r.counters[LF_CNT_RENDER_BLOCKS] = 0u;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        r.zx[i] = 100.0f;
+        r.zy[i] = 0.0f;
+    }
+}
+
+LF_DEV int sat16(float v) {
+    if (!(v == v)) return 0;
+    if (v > 32767.0f) return 32767;
+    if (v < -32768.0f) return -32768;
+    return (int)v;
+}
+
+// RenderingBlockDelegate::create_rendering_block (ray_tracing.cu:63-126) and
+// the tile fill (:187-210).
+__global__ __launch_bounds__(256) void k_zrange_fill(RayDev r, const float *__restrict__ pose24, lf_intrinsics K) {
+    float Tcw[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) Tcw[i] = pose24[12 + i];
+    const int nvis = (int)r.counters[LF_CNT_VISIBLE];
+    const float scale = 8.0f * r.voxel_size;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nvis; i += gridDim.x * blockDim.x) {
+        const int4 bp = reinterpret_cast<const int4 *>(r.block_pos)[r.visible[i]];
+        int ulx = r.tiles_x, uly = r.tiles_y, lrx = -1, lry = -1;
+        float zr0 = r.zmax_raycast, zr1 = r.zmin_raycast;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const float wx = (float)(bp.x + (c & 1)) * scale, wy = (float)(bp.y + ((c >> 1) & 1)) * scale,
+                        wz = (float)(bp.z + ((c >> 2) & 1)) * scale;
+            const f3 pt = xform(Tcw, f3{wx, wy, wz});
+            const float u = (K.fx * pt.x / pt.z + K.cx) / 8.0f, v = (K.fy * pt.y / pt.z + K.cy) / 8.0f;
+            const float fu = floorf(u), cu = ceilf(u), fv = floorf(v), cv = ceilf(v);
+            if ((float)ulx > fu) ulx = sat16(fu);
+            if ((float)lrx < cu) lrx = sat16(cu);
+            if ((float)uly > fv) uly = sat16(fv);
+            if ((float)lry < cv) lry = sat16(cv);
+            if (zr0 > pt.z) zr0 = pt.z;
+            if (zr1 < pt.z) zr1 = pt.z;
+        }
+        if (ulx < 0) ulx = 0;
+        if (uly < 0) uly = 0;
+        if (lrx >= r.tiles_x) lrx = r.tiles_x - 1;
+        if (lry >= r.tiles_y) lry = r.tiles_y - 1;
+        if (ulx > lrx || uly > lry) continue;
+        if (zr0 < r.zmin_raycast) zr0 = r.zmin_raycast;
+        if (zr1 < r.zmin_raycast) continue;
+        atomicAdd(&r.counters[LF_CNT_RENDER_BLOCKS], 1u);
+        const int zi0 = __float_as_int(zr0), zi1 = __float_as_int(zr1);
+        for (int y = uly; y <= lry; ++y)
+            for (int x = ulx; x <= lrx; ++x) {
+                const int t = y * r.tiles_x + x;
+                atomicMin(reinterpret_cast<int *>(r.zx) + t, zi0);
+                atomicMax(reinterpret_cast<int *>(r.zy) + t, zi1);
+            }
+    }
+}
+
+struct BlockCache {
+    int bx, by, bz, b;
+};
+
+// findVoxel + read_sdf (ray_tracing.cu:291-305) with a one-block cache.
+LF_DEV float read_sdf(const RayDev &r, BlockCache &bc, f3 p, bool &valid, int *voxel_out = nullptr) {
+    const i3 v = to_i3(p);
+    const int bx = blk(v.x), by = blk(v.y), bz = blk(v.z);
+    if (bx != bc.bx || by != bc.by || bz != bc.bz) {
+        bc.b = find_block(r.keys, r.slot_block, bx, by, bz, r.num_buckets, r.num_entries);
+        bc.bx = bx; bc.by = by; bc.bz = bz;
+    }
+    if (bc.b >= 0) {
+        const int vi = bc.b * LF_BLOCK3 + lidx(v.x, v.y, v.z);
+        if (voxel_out) *voxel_out = vi;
+        if (r.weight[vi] != 0.0f) {
+            valid = true;
+            return unpack_sdf(r.sdf[vi]);
+        }
+    }
+    valid = false;
+    return __int_as_float(0x7fc00000);
+}
+
+// read_sdf_interped (ray_tracing.cu:307-339).
+LF_DEV float read_interp(const RayDev &r, BlockCache &bc, f3 p, bool &valid) {
+    const f3 xyz = {p.x - floorf(p.x), p.y - floorf(p.y), p.z - floorf(p.z)};
+    bool vp, v;
+    float s0 = read_sdf(r, bc, p, vp);
+    float s1 = read_sdf(r, bc, add(p, f3{1.0f, 0.0f, 0.0f}), v); vp &= v;
+    float r0 = (1.0f - xyz.x) * s0 + xyz.x * s1;
+    s0 = read_sdf(r, bc, add(p, f3{0.0f, 1.0f, 0.0f}), v); vp &= v;
+    s1 = read_sdf(r, bc, add(p, f3{1.0f, 1.0f, 0.0f}), v); vp &= v;
+    float r1 = (1.0f - xyz.x) * s0 + xyz.x * s1;
+    const float r2 = (1.0f - xyz.y) * r0 + xyz.y * r1;
+    s0 = read_sdf(r, bc, add(p, f3{0.0f, 0.0f, 1.0f}), v); vp &= v;
+    s1 = read_sdf(r, bc, add(p, f3{1.0f, 0.0f, 1.0f}), v); vp &= v;
+    r0 = (1.0f - xyz.x) * s0 + xyz.x * s1;
+    s0 = read_sdf(r, bc, add(p, f3{0.0f, 1.0f, 1.0f}), v); vp &= v;
+    s1 = read_sdf(r, bc, add(p, f3{1.0f, 1.0f, 1.0f}), v); vp &= v;
+    r1 = (1.0f - xyz.x) * s0 + xyz.x * s1;
+    const float r3 = (1.0f - xyz.y) * r0 + xyz.y * r1;
+    valid = vp;
+    return (1.0f - xyz.z) * r2 + xyz.z * r3;
+}
+
+// MapRenderingDelegate::operator() (check_colour == false) and
+// ::raycast_with_colour (check_colour == true): ray_tracing.cu:346-425,478-564.
+template <bool kCheckColour>
+__global__ __launch_bounds__(256) void k_raycast(RayDev r, float *__restrict__ vmap, unsigned char *__restrict__ rgb_out,
+                                                 int cols, int rows, const float *__restrict__ pose24,
+                                                 lf_intrinsics K) {
+    // DenseMapping::raycast leaves the map untouched when nothing is visible
+    // (voxel_hashing.cpp:92-93) or no rendering block was made (:105).
+    if (r.counters[LF_CNT_VISIBLE] == 0u || r.counters[LF_CNT_RENDER_BLOCKS] == 0u) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tile_x = blockIdx.x * 2 + (wave & 1), tile_y = blockIdx.y * 2 + (wave >> 1);
+    const int x = tile_x * 8 + (lane & 7), y = tile_y * 8 + (lane >> 3);
+    if (x >= cols || y >= rows) return;
+    float4 *out = reinterpret_cast<float4 *>(vmap) + (size_t)y * cols + x;
+    const float qnan = __int_as_float(0x7fc00000);
+    float4 res4 = make_float4(qnan, qnan, qnan, qnan);
+    unsigned char col[3] = {255, 255, 255};
+    do {
+        if (tile_x >= r.tiles_x || tile_y >= r.tiles_y) break;
+        const int t = tile_y * r.tiles_x + tile_x;
+        const float z0 = r.zx[t], z1 = r.zy[t];
+        if (z1 < 1e-3f || z0 < 1e-3f || isnan(z0) || isnan(z1)) break;
+        const float *Twc = pose24;
+        const float inv_vs = 1.0f / r.voxel_size;
+        const float step_scale = r.truncation * inv_vs;
+        f3 pt = {((float)x - K.cx) * K.invfx * z0, ((float)y - K.cy) * K.invfy * z0, z0};
+        float dist_s = norm(pt) * inv_vs;
+        const f3 bs = mul(xform(Twc, pt), inv_vs);
+        pt = f3{((float)x - K.cx) * K.invfx * z1, ((float)y - K.cy) * K.invfy * z1, z1};
+        const float dist_e = norm(pt) * inv_vs;
+        const f3 be = mul(xform(Twc, pt), inv_vs);
+        const f3 dir = normalised(sub(be, bs));
+        f3 res = bs;
+        BlockCache bc = {kBad, kBad, kBad, -1};
+        float sdf = 1.0f, last_sdf, step;
+        bool valid = false, back_face = false;
+        while (dist_s < dist_e) {
+            last_sdf = sdf;
+            sdf = read_sdf(r, bc, res, valid);
+            if (sdf <= 0.5f && sdf >= -0.5f) sdf = read_interp(r, bc, res, valid);
+            if (sdf <= 0.0f) break;
+            if (sdf >= 0.0f && last_sdf < 0.0f) { back_face = true; break; }
+            step = valid ? fmaxf(sdf * step_scale, 1.0f) : 2.0f;
+            res = add(res, mul(dir, step));
+            dist_s += step;
+        }
+        if (back_face || !(sdf <= 0.0f)) break;
+        step = sdf * step_scale;
+        res = add(res, mul(dir, step));
+        sdf = read_interp(r, bc, res, valid);
+        step = sdf * step_scale;
+        res = add(res, mul(dir, step));
+        if (!valid) break;
+        if (kCheckColour) {  // read_colour (ray_tracing.cu:427-441)
+            bool cv;
+            int vi = -1;
+            read_sdf(r, bc, res, cv, &vi);
+            if (!cv) break;
+            const int b = vi / LF_BLOCK3, l = vi % LF_BLOCK3;
+            const unsigned char *c = r.rgb + (size_t)b * (3 * LF_BLOCK3) + l;
+            col[0] = c[0]; col[1] = c[LF_BLOCK3]; col[2] = c[2 * LF_BLOCK3];
+        }
+        const float *Tcw = pose24 + 12;
+        const f3 q = xform(Tcw, mul(res, r.voxel_size));
+        res4 = make_float4(q.x, q.y, q.z, 1.0f);
+    } while (false);
+    *out = res4;
+    if (rgb_out) {
+        unsigned char *o = rgb_out + 3 * ((size_t)y * cols + x);
+        o[0] = col[0]; o[1] = col[1]; o[2] = col[2];
+    }
+}
+
+}  // namespace
+
+int lf_raycast_ev(const lf_map_params *p, const lf_map_storage *s, float *vmap, unsigned char *rgb_out, int cols,
+                  int rows, const float *pose24, const lf_intrinsics *K, int check_colour, hipStream_t st,
+                  hipEvent_t e0, hipEvent_t e1) {
+    if (!p || !s || !vmap || !pose24 || !K) return lf_set_error(LF_ERR_ARG, "null argument to raycast");
+    if (cols / 8 != s->tiles_x || rows / 8 != s->tiles_y)
+        return lf_set_error(LF_ERR_ARG, "raycast size %dx%d does not match the map's %dx%d tiles", cols, rows,
+                            s->tiles_x, s->tiles_y);
+    RayDev r;
+    r.keys = s->keys; r.slot_block = s->slot_block; r.sdf = s->sdf; r.weight = s->weight; r.rgb = s->rgb;
+    r.visible = s->visible; r.block_pos = s->block_pos; r.counters = s->counters;
+    r.zx = s->zrange; r.zy = s->zrange + (size_t)s->tiles_x * s->tiles_y;
+    r.tiles_x = s->tiles_x; r.tiles_y = s->tiles_y;
+    r.num_buckets = p->num_buckets; r.num_entries = p->num_entries;
+    r.voxel_size = p->voxel_size; r.truncation = p->truncation;
+    r.zmin_raycast = p->zmin_raycast; r.zmax_raycast = p->zmax_raycast;
+    const int T = s->tiles_x * s->tiles_y;
+    k_zrange_clear<<<std::max(1, std::min(1024, (T + 255) / 256)), 256, 0, st>>>(r);
+    LF_LAUNCH_CHECK();
+    k_zrange_fill<<<std::max(1, std::min(2048, (p->num_blocks + 255) / 256)), 256, 0, st>>>(r, pose24, *K);
+    LF_LAUNCH_CHECK();
+    const int tx = (cols + 7) / 8, ty = (rows + 7) / 8;
+    dim3 g((tx + 1) / 2, (ty + 1) / 2);
+    if (e0) LF_HIP_CHECK(hipEventRecord(e0, st));
+    if (check_colour) k_raycast<true><<<g, 256, 0, st>>>(r, vmap, rgb_out, cols, rows, pose24, *K);
+    else k_raycast<false><<<g, 256, 0, st>>>(r, vmap, rgb_out, cols, rows, pose24, *K);
+    LF_LAUNCH_CHECK();
+    if (e1) LF_HIP_CHECK(hipEventRecord(e1, st));
+    return LF_OK;
+}
+
+extern "C" int lf_raycast(const lf_map_params *p, const lf_map_storage *s, float *vmap, unsigned char *rgb_out,
+                          int cols, int rows, const float *pose24, const lf_intrinsics *K, int check_colour,
+                          void *stream) {
+    return lf_raycast_ev(p, s, vmap, rgb_out, cols, rows, pose24, K, check_colour, lf_stream(stream), nullptr,
+                         nullptr);
+}

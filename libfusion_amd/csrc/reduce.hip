@@ -1,0 +1,534 @@
+// ICP and photometric normal-equation reductions and the on-device
+// Gauss-Newton step.
+//
+// Reference: src/tracking/pose_estimator.cu (ICPReduction :166-260,
+// icp_reduce :267-307), src/tracking/m_estimator.cu (RgbReduction2 :313-430,
+// rgb_step :437-482), include/tracking/reduce_sum.h (WarpReduce / BlockReduce
+// :11-46, create_jtjjtr :48-61), src/tracking/icp_tracker.cpp
+// (compute_transform :23-156).
+//
+// MI355X design:
+//  * One kernel evaluates both the ICP row and the RGB row of every reference
+//    pixel (they share the projected vertex), accumulating 2 x 29 sums per lane.
+//  * Wavefront reduction uses 64-lane shuffles, then 4 wavefront partials meet
+//    in LDS; each workgroup writes one partial row.
+//  * The last workgroup to finish (atomic ticket, agent-scope fences) sums the
+//    partial rows in a fixed order (deterministic), then one lane builds
+//    A = 1e6 H_icp + H_rgb, solves it with an FP64 LDL^T, applies
+//    T <- exp(dx) T in FP64 and runs the reference's divergence / revert logic.
+//    The next iteration reads the updated pose from device memory: a whole
+//    frame of tracking is a fixed sequence of launches with no host round trip
+//    (the reference downloads the sums and solves on the host twice per
+//    iteration).
+#include <cfloat>
+
+#include "lf_common.h"
+#include "lf_track.h"
+
+using namespace lf;
+
+namespace {
+
+constexpr int kGrid = 256;     // workgroups of a reduction
+constexpr int kThreads = 256;  // lanes per workgroup
+constexpr int kVals = 58;      // 29 ICP + 29 RGB sums
+
+struct ReduceArgs {
+    LfTrackLevel lv;
+    const float *pose12;   // float 3x4 (modes 1/2); mode 3 reads the tracker state
+    float stddev;          // modes 1/2
+    float angle, dthr;
+    float *partials;       // kGrid * kVals floats
+    unsigned int *ticket;
+    float *out29;          // modes 1/2
+    LfTrackState *st;      // mode 3
+    int first_iter;
+    float icp_weight;
+};
+
+// ICPReduction::searchPoint + getRow (pose_estimator.cu:168-222).
+LF_DEV bool icp_row(const ReduceArgs &a, const float *P, int i, float *row) {
+    const LfTrackLevel &L = a.lv;
+    const float4 lv = reinterpret_cast<const float4 *>(L.last_vmap)[i];
+    if (isnan(lv.x)) return false;
+    const f3 vg = xform(P, f3{lv.x, lv.y, lv.z});
+    const float invz = 1.0f / vg.z;
+    const int u = proj_floor(vg.x * invz * L.K.fx + L.K.cx + 0.5f);
+    const int v = proj_floor(vg.y * invz * L.K.fy + L.K.cy + 0.5f);
+    if (u < 0 || v < 0 || u >= L.cols || v >= L.rows) return false;
+    const size_t j = (size_t)v * L.cols + u;
+    const float4 cv = reinterpret_cast<const float4 *>(L.curr_vmap)[j];
+    const float4 ln = reinterpret_cast<const float4 *>(L.last_nmap)[i];
+    const float4 cn = reinterpret_cast<const float4 *>(L.curr_nmap)[j];
+    const f3 vc = {cv.x, cv.y, cv.z};
+    const f3 ng = rotate(P, f3{ln.x, ln.y, ln.z});
+    const f3 nc = {cn.x, cn.y, cn.z};
+    const float dist = norm(sub(vg, vc));
+    const float sine = norm(cross(nc, ng));
+    if (!(sine < a.angle && dist <= a.dthr && !isnan(nc.x) && !isnan(ng.x))) return false;
+    const f3 c = cross(vg, ng);
+    row[0] = ng.x; row[1] = ng.y; row[2] = ng.z;
+    row[3] = c.x; row[4] = c.y; row[5] = c.z;
+    row[6] = dot(ng, sub(vc, vg));
+    return true;
+}
+
+LF_DEV float interp2(const float *img, int w, float x, float y) {  // m_estimator.cu:340-346
+    const int u = (int)floorf(x), v = (int)floorf(y);
+    const float cx = x - (float)u, cy = y - (float)v;
+    const float *r0 = img + (size_t)v * w + u, *r1 = r0 + w;
+    return (r0[0] * (1.0f - cx) + r0[1] * cx) * (1.0f - cy) + (r1[0] * (1.0f - cx) + r1[1] * cx) * cy;
+}
+
+// RgbReduction2::find_corresp + compute_jacobian (m_estimator.cu:315-391).
+LF_DEV bool rgb_row(const ReduceArgs &a, const float *P, float stddev, float th, int i, float *row) {
+    const LfTrackLevel &L = a.lv;
+    const float4 pt = reinterpret_cast<const float4 *>(L.last_vmap)[i];
+    if (pt.w < 0.0f || isnan(pt.x)) return false;
+    const float il = L.last_I[i];
+    if (!isfinite(il)) return false;
+    const f3 p = xform(P, f3{pt.x, pt.y, pt.z});
+    const float u0 = p.x / p.z * L.K.fx + L.K.cx, v0 = p.y / p.z * L.K.fy + L.K.cy;
+    if (!(u0 >= 2.0f && u0 < (float)(L.cols - 2) && v0 >= 2.0f && v0 < (float)(L.rows - 2))) return false;
+    const float ic = interp2(L.curr_I, L.cols, u0, v0);
+    const float dx = interp2(L.dIdx, L.cols, u0, v0);
+    const float dy = interp2(L.dIdy, L.cols, u0, v0);
+    if (!((dx > 2.0f || dy > 2.0f) && isfinite(ic) && isfinite(dx) && isfinite(dy))) return false;
+    const float zi = 1.0f / p.z;
+    f3 l;
+    l.x = dx * L.K.fx * zi;
+    l.y = dy * L.K.fy * zi;
+    l.z = -(l.x * p.x + l.y * p.y) * zi;
+    float r = ic - il;
+    const double sd = (double)stddev;
+    if (sd > 10e-5) r /= stddev;
+    float w = 1.0f;
+    if (fabsf(r) > th && sd > 10e-6) w = sqrtf(th / fabsf(r));
+    const f3 c = cross(p, l);
+    row[6] = w * (-r);
+    row[0] = l.x * w; row[1] = l.y * w; row[2] = l.z * w;
+    row[3] = c.x * w; row[4] = c.y * w; row[5] = c.z * w;
+    return true;
+}
+
+LF_DEV void accumulate(float *s, const float *row) {
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < 7; ++i)
+#pragma unroll
+        for (int j = i; j < 7; ++j) s[k++] += row[i] * row[j];
+    s[28] += 1.0f;
+}
+
+// ---------------------------------------------------------- FP64 SE(3) helpers
+LF_DEV void m34_identity(double *T) {
+    for (int i = 0; i < 12; ++i) T[i] = (i == 0 || i == 5 || i == 10) ? 1.0 : 0.0;
+}
+LF_DEV void m34_mul(const double *A, const double *B, double *C) {
+    double t[12];
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) t[i * 4 + j] = A[i * 4] * B[j] + A[i * 4 + 1] * B[4 + j] + A[i * 4 + 2] * B[8 + j];
+        t[i * 4 + 3] = A[i * 4] * B[3] + A[i * 4 + 1] * B[7] + A[i * 4 + 2] * B[11] + A[i * 4 + 3];
+    }
+    for (int i = 0; i < 12; ++i) C[i] = t[i];
+}
+LF_DEV void m34_inv(const double *A, double *C) {
+    double t[12];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) t[i * 4 + j] = A[j * 4 + i];
+    for (int i = 0; i < 3; ++i) t[i * 4 + 3] = -(t[i * 4] * A[3] + t[i * 4 + 1] * A[7] + t[i * 4 + 2] * A[11]);
+    for (int i = 0; i < 12; ++i) C[i] = t[i];
+}
+// SE(3) exponential, tangent [upsilon; omega] (Sophus order); Barfoot sec. 7.1.
+LF_DEV void se3_exp(const double *xi, double *T) {
+    const double *u = xi, *w = xi + 3;
+    const double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2], th = sqrt(th2);
+    const double W[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+    double W2[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) W2[i * 3 + j] = W[i * 3] * W[j] + W[i * 3 + 1] * W[3 + j] + W[i * 3 + 2] * W[6 + j];
+    double A, B, C;
+    if (th < 1e-8) { A = 1 - th2 / 6; B = 0.5 - th2 / 24; C = 1.0 / 6 - th2 / 120; }
+    else { A = sin(th) / th; B = (1 - cos(th)) / th2; C = (th - sin(th)) / (th2 * th); }
+    double V[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const double I = i == j ? 1.0 : 0.0;
+            T[i * 4 + j] = I + A * W[i * 3 + j] + B * W2[i * 3 + j];
+            V[i * 3 + j] = I + B * W[i * 3 + j] + C * W2[i * 3 + j];
+        }
+    for (int i = 0; i < 3; ++i) T[i * 4 + 3] = V[i * 3] * u[0] + V[i * 3 + 1] * u[1] + V[i * 3 + 2] * u[2];
+}
+// Norm of the SE(3) logarithm (Barfoot sec. 7.1.3).
+LF_DEV double se3_log_norm(const double *T) {
+    double c = (T[0] + T[5] + T[10] - 1) * 0.5;
+    c = fmax(-1.0, fmin(1.0, c));
+    double th = acos(c);
+    const double v[3] = {T[9] - T[6], T[2] - T[8], T[4] - T[1]};
+    double w[3];
+    if (th < 1e-8) {
+        for (int i = 0; i < 3; ++i) w[i] = 0.5 * v[i];
+    } else if (M_PI - th < 1e-6) {
+        int k = 0;
+        for (int i = 1; i < 3; ++i)
+            if (T[i * 5] > T[k * 5]) k = i;
+        double ax[3];
+        ax[k] = sqrt(fmax(0.0, (T[k * 5] + 1) * 0.5));
+        for (int i = 0; i < 3; ++i)
+            if (i != k) ax[i] = T[k * 4 + i] / (2 * ax[k]);
+        for (int i = 0; i < 3; ++i) w[i] = ax[i] * th;
+    } else {
+        const double s = th / (2 * sin(th));
+        for (int i = 0; i < 3; ++i) w[i] = s * v[i];
+    }
+    const double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    th = sqrt(th2);
+    const double W[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+    double W2[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) W2[i * 3 + j] = W[i * 3] * W[j] + W[i * 3 + 1] * W[3 + j] + W[i * 3 + 2] * W[6 + j];
+    const double D = th < 1e-8 ? 1.0 / 12 : (1 - th * sin(th) / (2 * (1 - cos(th)))) / th2;
+    double n2 = th2;
+    for (int i = 0; i < 3; ++i) {
+        double s = 0;
+        for (int j = 0; j < 3; ++j) {
+            const double Vi = (i == j ? 1.0 : 0.0) - 0.5 * W[i * 3 + j] + D * W2[i * 3 + j];
+            s += Vi * T[j * 4 + 3];
+        }
+        n2 += s * s;
+    }
+    return sqrt(n2);
+}
+// LDL^T solve of a symmetric positive definite 6x6 (no pivoting).
+LF_DEV void ldlt6(const double *A, const double *b, double *x) {
+    double L[36], D[6];
+    for (int i = 0; i < 36; ++i) L[i] = 0.0;
+    for (int j = 0; j < 6; ++j) {
+        double d = A[j * 6 + j];
+        for (int k = 0; k < j; ++k) d -= L[j * 6 + k] * L[j * 6 + k] * D[k];
+        D[j] = d;
+        L[j * 6 + j] = 1.0;
+        for (int i = j + 1; i < 6; ++i) {
+            double s = A[i * 6 + j];
+            for (int k = 0; k < j; ++k) s -= L[i * 6 + k] * L[j * 6 + k] * D[k];
+            L[i * 6 + j] = d != 0.0 ? s / d : 0.0;
+        }
+    }
+    double y[6];
+    for (int i = 0; i < 6; ++i) {
+        double s = b[i];
+        for (int k = 0; k < i; ++k) s -= L[i * 6 + k] * y[k];
+        y[i] = s;
+    }
+    for (int i = 0; i < 6; ++i) y[i] = D[i] != 0.0 ? y[i] / D[i] : 0.0;
+    for (int i = 5; i >= 0; --i) {
+        double s = y[i];
+        for (int k = i + 1; k < 6; ++k) s -= L[k * 6 + i] * x[k];
+        x[i] = s;
+    }
+}
+// create_jtjjtr<6,7> (reduce_sum.h:48-61), scaled as A = w * H (float) + ...
+LF_DEV void unpack29(const float *s, float *A, float *b) {
+    int k = 0;
+    for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 7; ++j) {
+            const float v = s[k++];
+            if (j == 6) b[i] = v;
+            else A[j * 6 + i] = A[i * 6 + j] = v;
+        }
+}
+
+// One Gauss-Newton step of compute_transform (icp_tracker.cpp:50-139), run by
+// a single lane after the sums are final.
+LF_DEV void gn_step(LfTrackState &st, const float *si, const float *sr, int first_iter, float icp_weight) {
+    if (first_iter) {
+        st.icp_error = FLT_MAX;
+        st.rgb_error = FLT_MAX;
+        st.icp_count = st.rgb_count = 0;
+        st.active = 1;
+    }
+    for (int k = 0; k < 29; ++k) { st.last_icp_sums[k] = si[k]; st.last_rgb_sums[k] = sr[k]; }
+    double last[12];
+    for (int k = 0; k < 12; ++k) last[k] = st.est[k];
+    const float last_icp = st.icp_error, last_rgb = st.rgb_error;
+    st.stddev = sqrtf(sr[27] / (sr[28] - 6.0f));
+    float Ai[36], bi[6], Ar[36], br[6];
+    unpack29(si, Ai, bi);
+    unpack29(sr, Ar, br);
+    double A[36], b[6], x[6];
+    for (int k = 0; k < 36; ++k) A[k] = (double)(icp_weight * Ai[k] + Ar[k]);
+    for (int k = 0; k < 6; ++k) b[k] = (double)(icp_weight * bi[k] + br[k]);
+    ldlt6(A, b, x);
+    double E[12];
+    se3_exp(x, E);
+    for (int k = 0; k < 12; ++k) st.prev[k] = st.est[k];
+    m34_mul(E, last, st.est);
+    st.iterations++;
+    bool stop = false;
+    float icp_error = sqrtf(si[27]) / si[28];
+    if (isnan(icp_error)) {
+        st.invalid = 1;
+        stop = true;
+    } else if (icp_error > last_icp) {
+        if (st.icp_count >= 2) {
+            for (int k = 0; k < 12; ++k) st.est[k] = st.prev[k];
+            stop = true;
+        } else {
+            st.icp_count++;
+            icp_error = last_icp;
+        }
+    } else {
+        st.icp_count = 0;
+    }
+    st.icp_error = icp_error;
+    if (!stop) {
+        float rgb_error = sqrtf(sr[27]) / sr[28];
+        if (isnan(rgb_error)) {
+            st.invalid = 1;
+            stop = true;
+        } else if (rgb_error > last_rgb) {
+            if (st.rgb_count >= 2) {
+                for (int k = 0; k < 12; ++k) st.est[k] = st.prev[k];
+                stop = true;
+            } else {
+                st.rgb_count++;
+                rgb_error = last_rgb;
+            }
+        } else {
+            st.rgb_count = 0;
+        }
+        st.rgb_error = rgb_error;
+    }
+    st.active = stop ? 0 : 1;
+    for (int k = 0; k < 12; ++k) st.pose_f[k] = (float)st.est[k];
+}
+
+// kMode 1: ICP sums -> out29; 2: RGB sums -> out29; 3: both + Gauss-Newton step.
+template <int kMode>
+__global__ __launch_bounds__(kThreads) void k_reduce(ReduceArgs a) {
+    __shared__ float s_wave[4][kVals];
+    __shared__ int s_last;
+    __shared__ float s_part[4][64];
+    __shared__ float s_sum[kVals];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    float P[12];
+    float stddev = a.stddev;
+    if (kMode == 3) {
+        const LfTrackState &st = *a.st;
+        if (st.invalid) return;                                  // the frame already failed
+        if (!a.first_iter && !st.active) return;                 // the level already broke out
+#pragma unroll
+        for (int k = 0; k < 12; ++k) P[k] = st.pose_f[k];
+        stddev = a.first_iter ? 0.0f : st.stddev;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) P[k] = a.pose12[k];
+    }
+    const float th = (float)(1.345 * (double)stddev);
+    constexpr bool kIcp = kMode == 1 || kMode == 3;
+    constexpr bool kRgb = kMode == 2 || kMode == 3;
+    float si[29], sr[29];
+#pragma unroll
+    for (int k = 0; k < 29; ++k) { si[k] = 0.0f; sr[k] = 0.0f; }
+    const int n = a.lv.cols * a.lv.rows;
+    for (int i = blockIdx.x * kThreads + tid; i < n; i += gridDim.x * kThreads) {
+        float row[7];
+        if (kIcp && icp_row(a, P, i, row)) accumulate(si, row);
+        if (kRgb && rgb_row(a, P, stddev, th, i, row)) accumulate(sr, row);
+    }
+    // wavefront reduction
+#pragma unroll
+    for (int k = 0; k < 29; ++k) {
+        float vi = si[k], vr = sr[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            if (kIcp) vi += __shfl_xor(vi, off);
+            if (kRgb) vr += __shfl_xor(vr, off);
+        }
+        if (lane == 0) {
+            s_wave[wave][k] = vi;
+            s_wave[wave][29 + k] = vr;
+        }
+    }
+    __syncthreads();
+    if (tid < kVals)
+        a.partials[(size_t)blockIdx.x * kVals + tid] = (s_wave[0][tid] + s_wave[1][tid]) + (s_wave[2][tid] + s_wave[3][tid]);
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) s_last = atomicAdd(a.ticket, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    // fixed-order final sum: part p of value v sums workgroups g = p, p+4, ...
+    {
+        const int v = tid & 63, p = tid >> 6;
+        float s = 0.0f;
+        if (v < kVals) {
+            const volatile float *part = a.partials;
+            for (int g = p; g < (int)gridDim.x; g += 4) s += part[(size_t)g * kVals + v];
+        }
+        s_part[p][v] = s;
+    }
+    __syncthreads();
+    if (tid < kVals) s_sum[tid] = (s_part[0][tid] + s_part[1][tid]) + (s_part[2][tid] + s_part[3][tid]);
+    __syncthreads();
+    if (tid == 0) {
+        *a.ticket = 0u;
+        if (kMode == 1) for (int k = 0; k < 29; ++k) a.out29[k] = s_sum[k];
+        if (kMode == 2) for (int k = 0; k < 29; ++k) a.out29[k] = s_sum[29 + k];
+        if (kMode == 3) gn_step(*a.st, s_sum, s_sum + 29, a.first_iter, a.icp_weight);
+    }
+}
+
+// End of DenseTracking::compute_transform (icp_tracker.cpp:143-155) and the
+// odometry chain of DenseOdometry::trackFrame (rgbd_odometry.cpp:39).
+__global__ void k_track_finish(LfTrackState *st) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double inv_est[12], D[12];
+    m34_inv(st->est, inv_est);
+    m34_mul(inv_est, st->init, D);
+    const bool ok = !st->invalid && !(se3_log_norm(D) > 0.1);
+    st->success = ok ? 1 : 0;
+    double upd[12];
+    if (ok) for (int k = 0; k < 12; ++k) upd[k] = inv_est[k];
+    else m34_identity(upd);
+    double P[12];
+    m34_mul(st->pose, upd, P);
+    double Pi[12];
+    m34_inv(P, Pi);
+    for (int k = 0; k < 12; ++k) {
+        st->pose[k] = P[k];
+        st->pose24[k] = (float)P[k];
+        st->pose24[12 + k] = (float)Pi[k];
+    }
+}
+
+__global__ void k_track_begin(LfTrackState *st, double t0, double t1, double t2, double t3, double t4, double t5,
+                              double t6, double t7, double t8, double t9, double t10, double t11) {
+    const double T[12] = {t0, t1, t2, t3, t4, t5, t6, t7, t8, t9, t10, t11};
+    for (int k = 0; k < 12; ++k) {
+        st->est[k] = T[k];
+        st->prev[k] = T[k];
+        st->init[k] = T[k];
+        st->pose_f[k] = (float)T[k];
+    }
+    st->invalid = 0;
+    st->active = 1;
+    st->iterations = 0;
+}
+
+__global__ void k_track_set_pose(LfTrackState *st, double t0, double t1, double t2, double t3, double t4, double t5,
+                                 double t6, double t7, double t8, double t9, double t10, double t11) {
+    const double T[12] = {t0, t1, t2, t3, t4, t5, t6, t7, t8, t9, t10, t11};
+    double Ti[12];
+    m34_inv(T, Ti);
+    for (int k = 0; k < 12; ++k) {
+        st->pose[k] = T[k];
+        st->pose24[k] = (float)T[k];
+        st->pose24[12 + k] = (float)Ti[k];
+    }
+    st->success = 1;
+    st->ticket = 0u;
+}
+
+int check_level(const LfTrackLevel &lv) {
+    if (lv.cols <= 0 || lv.rows <= 0) return lf_set_error(LF_ERR_ARG, "bad level size %dx%d", lv.cols, lv.rows);
+    return LF_OK;
+}
+
+float icp_angle() { return (float)cos(30 * 3.14 / 180); }
+
+}  // namespace
+
+size_t lf_reduce_partials(void) { return (size_t)kGrid * kVals + 64; }
+
+extern "C" size_t lf_reduce_scratch_floats(void) { return lf_reduce_partials(); }
+
+int lf_track_begin(LfTrackState *st, const double *T, hipStream_t stream) {
+    k_track_begin<<<1, 1, 0, stream>>>(st, T[0], T[1], T[2], T[3], T[4], T[5], T[6], T[7], T[8], T[9], T[10], T[11]);
+    LF_LAUNCH_CHECK();
+    return LF_OK;
+}
+
+int lf_track_iteration(const LfTrackLevel &lv, float *scratch, LfTrackState *st, int first_iter_of_level,
+                       float icp_weight, hipStream_t stream) {
+    if (int rc = check_level(lv)) return rc;
+    ReduceArgs a = {};
+    a.lv = lv;
+    a.angle = icp_angle();
+    a.dthr = 0.01f;
+    a.partials = scratch;
+    a.ticket = &st->ticket;
+    a.st = st;
+    a.first_iter = first_iter_of_level;
+    a.icp_weight = icp_weight;
+    k_reduce<3><<<kGrid, kThreads, 0, stream>>>(a);
+    LF_LAUNCH_CHECK();
+    return LF_OK;
+}
+
+int lf_track_finish(LfTrackState *st, hipStream_t stream) {
+    k_track_finish<<<1, 64, 0, stream>>>(st);
+    LF_LAUNCH_CHECK();
+    return LF_OK;
+}
+
+int lf_track_set_pose(LfTrackState *st, const double *T, hipStream_t stream) {
+    k_track_set_pose<<<1, 1, 0, stream>>>(st, T[0], T[1], T[2], T[3], T[4], T[5], T[6], T[7], T[8], T[9], T[10], T[11]);
+    LF_LAUNCH_CHECK();
+    return LF_OK;
+}
+
+extern "C" int lf_icp_reduce(const float *curr_vmap, const float *curr_nmap, const float *last_vmap,
+                             const float *last_nmap, int cols, int rows, const float *pose12,
+                             const lf_intrinsics *K, float *scratch, float *sum29, void *stream) {
+    if (!curr_vmap || !curr_nmap || !last_vmap || !last_nmap || !pose12 || !K || !scratch || !sum29)
+        return lf_set_error(LF_ERR_ARG, "null argument to icp_reduce");
+    ReduceArgs a = {};
+    a.lv.curr_vmap = curr_vmap; a.lv.curr_nmap = curr_nmap; a.lv.last_vmap = last_vmap; a.lv.last_nmap = last_nmap;
+    a.lv.cols = cols; a.lv.rows = rows; a.lv.K = *K;
+    if (int rc = check_level(a.lv)) return rc;
+    a.pose12 = pose12;
+    a.angle = icp_angle();
+    a.dthr = 0.01f;
+    a.partials = scratch;
+    a.ticket = reinterpret_cast<unsigned int *>(scratch + (size_t)kGrid * kVals);
+    a.out29 = sum29;
+    k_reduce<1><<<kGrid, kThreads, 0, lf_stream(stream)>>>(a);
+    LF_LAUNCH_CHECK();
+    return LF_OK;
+}
+
+extern "C" int lf_rgb_step(const float *curr_intensity, const float *last_intensity, const float *last_vmap,
+                           const float *intensity_dx, const float *intensity_dy, int cols, int rows,
+                           float stddev_estimate, const float *pose12, const lf_intrinsics *K, float *scratch,
+                           float *sum29, void *stream) {
+    if (!curr_intensity || !last_intensity || !last_vmap || !intensity_dx || !intensity_dy || !pose12 || !K ||
+        !scratch || !sum29)
+        return lf_set_error(LF_ERR_ARG, "null argument to rgb_step");
+    ReduceArgs a = {};
+    a.lv.last_vmap = last_vmap; a.lv.curr_I = curr_intensity; a.lv.last_I = last_intensity;
+    a.lv.dIdx = intensity_dx; a.lv.dIdy = intensity_dy;
+    a.lv.cols = cols; a.lv.rows = rows; a.lv.K = *K;
+    if (int rc = check_level(a.lv)) return rc;
+    a.pose12 = pose12;
+    a.stddev = stddev_estimate;
+    a.partials = scratch;
+    a.ticket = reinterpret_cast<unsigned int *>(scratch + (size_t)kGrid * kVals);
+    a.out29 = sum29;
+    k_reduce<2><<<kGrid, kThreads, 0, lf_stream(stream)>>>(a);
+    LF_LAUNCH_CHECK();
+    return LF_OK;
+}
+
+extern "C" void lf_unpack_jtjjtr(const float *sum29, float *jtj36, float *jtr6, float *residual2) {
+    int k = 0;
+    for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 7; ++j) {
+            const float v = sum29[k++];
+            if (j == 6) jtr6[i] = v;
+            else jtj36[j * 6 + i] = jtj36[i * 6 + j] = v;
+        }
+    residual2[0] = sum29[27];
+    residual2[1] = sum29[28];
+}

@@ -1,0 +1,528 @@
+// Host classes of the drop-in C++ layer and the lf_system_* C-ABI.
+//
+// Reference: src/tracking/device_image.cpp, src/voxel_hashing/voxel_hashing.cpp,
+// src/tracking/icp_tracker.cpp, src/tracking/rgbd_odometry.cpp, src/system.cpp.
+// Each method below enqueues kernels on its object's stream; nothing waits for
+// the GPU inside a frame (the reference synchronises twice per Gauss-Newton
+// iteration and once per integrate / raycast).
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "fusion_amd.hpp"
+#include "lf_common.h"
+#include "lf_internal.h"
+#include "lf_track.h"
+
+#define LF_RC(call)                 \
+    do {                            \
+        int rc_ = (call);           \
+        if (rc_ != LF_OK) return rc_; \
+    } while (0)
+
+namespace fusion {
+
+// ------------------------------------------------------------ small types
+
+IntrinsicMatrix::IntrinsicMatrix() { lf_intrinsics_make(0, 0, 1, 1, 0, 0, this); invfx = invfy = 0; fx = fy = 0; }
+IntrinsicMatrix::IntrinsicMatrix(int cols, int rows, float fx_, float fy_, float cx_, float cy_) {
+    lf_intrinsics_make(cols, rows, fx_, fy_, cx_, cy_, this);
+}
+IntrinsicMatrix IntrinsicMatrix::pyr_down() const {
+    IntrinsicMatrix k;
+    lf_intrinsics_pyr_down(this, &k);
+    return k;
+}
+void BuildIntrinsicPyramid(IntrinsicMatrix base, std::vector<IntrinsicMatrix> &pyr, int n) {
+    pyr.clear();
+    pyr.push_back(base);
+    for (int i = 0; i < n - 1; ++i) pyr.push_back(pyr[i].pyr_down());
+}
+
+SE3d SE3d::exp(const double *xi) {
+    const double *u = xi, *w = xi + 3;
+    const double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2], th = std::sqrt(th2);
+    const double W[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+    double W2[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) W2[i * 3 + j] = W[i * 3] * W[j] + W[i * 3 + 1] * W[3 + j] + W[i * 3 + 2] * W[6 + j];
+    double A, B, C;
+    if (th < 1e-8) { A = 1 - th2 / 6; B = 0.5 - th2 / 24; C = 1.0 / 6 - th2 / 120; }
+    else { A = std::sin(th) / th; B = (1 - std::cos(th)) / th2; C = (th - std::sin(th)) / (th2 * th); }
+    SE3d T;
+    double V[9];
+    for (int i = 0; i < 9; ++i) {
+        const double I = (i % 4 == 0) ? 1 : 0;
+        T.R[i] = I + A * W[i] + B * W2[i];
+        V[i] = I + B * W[i] + C * W2[i];
+    }
+    for (int i = 0; i < 3; ++i) T.t[i] = V[i * 3] * u[0] + V[i * 3 + 1] * u[1] + V[i * 3 + 2] * u[2];
+    return T;
+}
+
+void SE3d::log(double *xi) const {
+    double c = std::max(-1.0, std::min(1.0, (R[0] + R[4] + R[8] - 1) * 0.5));
+    double th = std::acos(c);
+    const double v[3] = {R[7] - R[5], R[2] - R[6], R[3] - R[1]};
+    double w[3];
+    if (th < 1e-8) {
+        for (int i = 0; i < 3; ++i) w[i] = 0.5 * v[i];
+    } else if (M_PI - th < 1e-6) {
+        int k = 0;
+        for (int i = 1; i < 3; ++i)
+            if (R[i * 4] > R[k * 4]) k = i;
+        double ax[3];
+        ax[k] = std::sqrt(std::max(0.0, (R[k * 4] + 1) * 0.5));
+        for (int i = 0; i < 3; ++i)
+            if (i != k) ax[i] = R[k * 3 + i] / (2 * ax[k]);
+        for (int i = 0; i < 3; ++i) w[i] = ax[i] * th;
+    } else {
+        const double s = th / (2 * std::sin(th));
+        for (int i = 0; i < 3; ++i) w[i] = s * v[i];
+    }
+    const double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    th = std::sqrt(th2);
+    const double W[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+    double W2[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) W2[i * 3 + j] = W[i * 3] * W[j] + W[i * 3 + 1] * W[3 + j] + W[i * 3 + 2] * W[6 + j];
+    const double D = th < 1e-8 ? 1.0 / 12 : (1 - th * std::sin(th) / (2 * (1 - std::cos(th)))) / th2;
+    for (int i = 0; i < 3; ++i) {
+        double s = 0;
+        for (int j = 0; j < 3; ++j) s += ((i == j ? 1.0 : 0.0) - 0.5 * W[i * 3 + j] + D * W2[i * 3 + j]) * t[j];
+        xi[i] = s;
+        xi[3 + i] = w[i];
+    }
+}
+
+SE3d SE3d::inverse() const {
+    SE3d o;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) o.R[i * 3 + j] = R[j * 3 + i];
+    for (int i = 0; i < 3; ++i) o.t[i] = -(o.R[i * 3] * t[0] + o.R[i * 3 + 1] * t[1] + o.R[i * 3 + 2] * t[2]);
+    return o;
+}
+
+SE3d SE3d::operator*(const SE3d &b) const {
+    SE3d c;
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) c.R[i * 3 + j] = R[i * 3] * b.R[j] + R[i * 3 + 1] * b.R[3 + j] + R[i * 3 + 2] * b.R[6 + j];
+        c.t[i] = R[i * 3] * b.t[0] + R[i * 3 + 1] * b.t[1] + R[i * 3 + 2] * b.t[2] + t[i];
+    }
+    return c;
+}
+
+void SE3d::matrix3x4(double *o) const {
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) o[i * 4 + j] = R[i * 3 + j];
+        o[i * 4 + 3] = t[i];
+    }
+}
+
+SE3d SE3d::from3x4(const double *m) {
+    SE3d T;
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) T.R[i * 3 + j] = m[i * 4 + j];
+        T.t[i] = m[i * 4 + 3];
+    }
+    return T;
+}
+
+GpuImage::~GpuImage() {
+    if (data_) (void)hipFree(data_);
+}
+GpuImage::GpuImage(GpuImage &&o) noexcept : cols(o.cols), rows(o.rows), elem(o.elem), data_(o.data_) { o.data_ = nullptr; }
+GpuImage &GpuImage::operator=(GpuImage &&o) noexcept {
+    if (this != &o) {
+        if (data_) (void)hipFree(data_);
+        data_ = o.data_; cols = o.cols; rows = o.rows; elem = o.elem;
+        o.data_ = nullptr;
+    }
+    return *this;
+}
+void GpuImage::create(int c, int r, int e) {
+    if (data_ && c == cols && r == rows && e == elem) return;
+    if (data_) (void)hipFree(data_);
+    data_ = nullptr;
+    cols = c; rows = r; elem = e;
+    if (hipMalloc(&data_, (size_t)c * r * e) != hipSuccess) {
+        data_ = nullptr;
+        throw std::bad_alloc();
+    }
+}
+
+// ------------------------------------------------------------ DeviceImage
+
+DeviceImage::DeviceImage(const IntrinsicMatrix &K, int num_pyr, hipStream_t stream) : stream_(stream) {
+    BuildIntrinsicPyramid(K, K_, num_pyr);
+    depth_raw_.create(K.width, K.height, 4);
+    image_.create(K.width, K.height, 3);
+    depth_.resize(num_pyr); intensity_.resize(num_pyr); dx_.resize(num_pyr); dy_.resize(num_pyr);
+    vmap_.resize(num_pyr); nmap_.resize(num_pyr);
+    for (int l = 0; l < num_pyr; ++l) {
+        const int w = K_[l].width, h = K_[l].height;
+        depth_[l].create(w, h, 4); intensity_[l].create(w, h, 4); dx_[l].create(w, h, 4); dy_[l].create(w, h, 4);
+        vmap_[l].create(w, h, 16); nmap_[l].create(w, h, 16);
+    }
+}
+
+int DeviceImage::upload(const uint16_t *depth_u16, const unsigned char *rgb, float depth_scale) {
+    const int W = K_[0].width, H = K_[0].height;
+    void *st = stream_;
+    LF_HIP_CHECK(hipMemcpyAsync(image_.ptr<void>(), rgb, (size_t)W * H * 3, hipMemcpyDeviceToDevice, stream_));
+    LF_RC(lf_depth_to_float(depth_u16, depth_raw_.ptr<float>(), W, H, depth_scale, st));
+    LF_RC(lf_rgb_to_intensity(rgb, intensity_[0].ptr<float>(), W, H, st));
+    LF_RC(lf_bilateral_filter(depth_raw_.ptr<float>(), depth_[0].ptr<float>(), W, H, st));
+    for (int l = 1; l < levels(); ++l) {
+        LF_RC(lf_pyr_down_depth(depth_[l - 1].ptr<float>(), depth_[l].ptr<float>(), K_[l - 1].width, K_[l - 1].height, st));
+        LF_RC(lf_pyr_down_image(intensity_[l - 1].ptr<float>(), intensity_[l].ptr<float>(), K_[l - 1].width,
+                                K_[l - 1].height, st));
+    }
+    for (int l = 0; l < levels(); ++l) {
+        const int w = K_[l].width, h = K_[l].height;
+        LF_RC(lf_compute_derivative(intensity_[l].ptr<float>(), dx_[l].ptr<float>(), dy_[l].ptr<float>(), w, h, st));
+        LF_RC(lf_back_project_depth(depth_[l].ptr<float>(), vmap_[l].ptr<float>(), w, h, &K_[l], st));
+        LF_RC(lf_compute_nmap(vmap_[l].ptr<float>(), nmap_[l].ptr<float>(), w, h, st));
+    }
+    return LF_OK;
+}
+
+int DeviceImage::resize_device_map() {
+    void *st = stream_;
+    for (int l = 1; l < levels(); ++l)
+        LF_RC(lf_pyr_down_vmap(vmap_[l - 1].ptr<float>(), vmap_[l].ptr<float>(), K_[l - 1].width, K_[l - 1].height, st));
+    for (int l = 0; l < levels(); ++l)
+        LF_RC(lf_compute_nmap(vmap_[l].ptr<float>(), nmap_[l].ptr<float>(), K_[l].width, K_[l].height, st));
+    return LF_OK;
+}
+
+// ------------------------------------------------------------ DenseMapping
+
+DenseMapping::DenseMapping(const IntrinsicMatrix &K, const lf_map_params &params, hipStream_t stream)
+    : K_(K), params_(params), stream_(stream) {
+    memset(&storage_, 0, sizeof(storage_));
+    if (lf_map_create(&params_, K.width, K.height, &storage_) != LF_OK) throw std::bad_alloc();
+    reset_mapping();
+}
+
+DenseMapping::~DenseMapping() { lf_map_release(&storage_); }
+
+int DenseMapping::update(DeviceImage &frame, const float *pose24, bool colour, hipEvent_t e0, hipEvent_t e1) {
+    return lf_update_ev(&params_, &storage_, frame.get_raw_depth().ptr<float>(), frame.get_nmap(0).ptr<float>(),
+                        colour ? frame.get_image().ptr<unsigned char>() : nullptr, K_.width, K_.height, pose24, &K_,
+                        stream_, true, e0, e1);
+}
+
+int DenseMapping::update(const float *depth, const unsigned char *rgb, const float *pose24) {
+    return lf_update_ev(&params_, &storage_, depth, nullptr, rgb, K_.width, K_.height, pose24, &K_, stream_, false,
+                        nullptr, nullptr);
+}
+
+int DenseMapping::raycast(float *vmap, const float *pose24, hipEvent_t e0, hipEvent_t e1) {
+    return lf_raycast_ev(&params_, &storage_, vmap, nullptr, K_.width, K_.height, pose24, &K_, 1, stream_, e0, e1);
+}
+
+int DenseMapping::reset_mapping() { return lf_map_reset(&params_, &storage_, stream_); }
+
+size_t DenseMapping::fetch_mesh_vertex_only(float *vertex, int max_tri) {
+    unsigned int n = 0;
+    lf_create_mesh(&params_, &storage_, vertex, nullptr, nullptr, max_tri, &n, stream_);
+    return n;
+}
+size_t DenseMapping::fetch_mesh_with_normal(float *vertex, float *normal, int max_tri) {
+    unsigned int n = 0;
+    lf_create_mesh(&params_, &storage_, vertex, normal, nullptr, max_tri, &n, stream_);
+    return n;
+}
+size_t DenseMapping::fetch_mesh_with_colour(float *vertex, unsigned char *colour, int max_tri) {
+    unsigned int n = 0;
+    lf_create_mesh(&params_, &storage_, vertex, nullptr, colour, max_tri, &n, stream_);
+    return n;
+}
+
+// ------------------------------------------------------------ DenseTracking
+
+DenseTracking::DenseTracking(hipStream_t stream) : stream_(stream) {
+    if (hipMalloc(&state_, sizeof(LfTrackState)) != hipSuccess) throw std::bad_alloc();
+    if (hipMalloc(&scratch_, lf_reduce_partials() * sizeof(float)) != hipSuccess) throw std::bad_alloc();
+    const double I[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    lf_track_set_pose(state_, I, stream_);
+}
+
+DenseTracking::~DenseTracking() {
+    (void)hipFree(state_);
+    (void)hipFree(scratch_);
+}
+
+// compute_transform(ref, cur, ctx) (icp_tracker.cpp:23-156): levels coarse to
+// fine, max_iterations_[level] Gauss-Newton steps each, all on the device.
+int DenseTracking::compute_transform(DeviceImage &ref, DeviceImage &cur, const TrackingContext &c) {
+    double init[12];
+    (c.use_initial_guess_ ? c.initial_estimate_ : SE3d()).matrix3x4(init);
+    LF_RC(lf_track_begin(state_, init, stream_));
+    for (int level = (int)c.max_iterations_.size() - 1; level >= 0; --level) {
+        if (level >= ref.levels()) continue;
+        LfTrackLevel lv;
+        lv.curr_vmap = cur.get_vmap(level).ptr<float>();
+        lv.curr_nmap = cur.get_nmap(level).ptr<float>();
+        lv.last_vmap = ref.get_vmap(level).ptr<float>();
+        lv.last_nmap = ref.get_nmap(level).ptr<float>();
+        lv.curr_I = cur.get_intensity(level).ptr<float>();
+        lv.last_I = ref.get_intensity(level).ptr<float>();
+        lv.dIdx = cur.get_intensity_dx(level).ptr<float>();
+        lv.dIdy = cur.get_intensity_dy(level).ptr<float>();
+        const IntrinsicMatrix &K = c.intrinsics_pyr_.empty() ? ref.intrinsics()[level] : c.intrinsics_pyr_[level];
+        lv.cols = K.width;
+        lv.rows = K.height;
+        lv.K = K;
+        for (int it = 0; it < c.max_iterations_[level]; ++it)
+            LF_RC(lf_track_iteration(lv, scratch_, state_, it == 0 ? 1 : 0, icp_weight, stream_));
+    }
+    return LF_OK;
+}
+
+TrackingResult DenseTracking::result() {
+    LfTrackState h;
+    TrackingResult r;
+    if (hipMemcpyAsync(&h, state_, sizeof(h), hipMemcpyDeviceToHost, stream_) != hipSuccess) return r;
+    if (hipStreamSynchronize(stream_) != hipSuccess) return r;
+    r.sucess = h.success != 0;
+    r.icp_error = h.icp_error;
+    r.update = r.sucess ? SE3d::from3x4(h.est).inverse() : SE3d();
+    return r;
+}
+
+// ------------------------------------------------------------ DenseOdometry
+
+DenseOdometry::DenseOdometry(const IntrinsicMatrix &base, int num_pyr, std::vector<int> iterations,
+                             hipStream_t stream)
+    : iterations_(std::move(iterations)), stream_(stream), tracker_(stream) {
+    BuildIntrinsicPyramid(base, K_, num_pyr);
+    curr_.reset(new DeviceImage(base, num_pyr, stream));
+    ref_.reset(new DeviceImage(base, num_pyr, stream));
+}
+
+const float *DenseOdometry::pose24() const { return tracker_.state()->pose24; }
+
+int DenseOdometry::trackFrame(const uint16_t *depth, const unsigned char *rgb, float depth_scale,
+                              const SE3d &initial_pose) {
+    LF_RC(curr_->upload(depth, rgb, depth_scale));
+    if (!initialized_) {
+        double T[12];
+        initial_pose.matrix3x4(T);
+        LF_RC(lf_track_set_pose(tracker_.state(), T, stream_));
+        std::swap(curr_, ref_);
+        initialized_ = true;
+        return LF_OK;
+    }
+    TrackingContext c;
+    c.use_initial_guess_ = true;
+    c.initial_estimate_ = SE3d();
+    c.intrinsics_pyr_ = K_;
+    c.max_iterations_ = iterations_;
+    LF_RC(tracker_.compute_transform(*ref_, *curr_, c));
+    LF_RC(lf_track_finish(tracker_.state(), stream_));
+    std::swap(curr_, ref_);
+    trackingLost = false;
+    return LF_OK;
+}
+
+void DenseOdometry::reset() {
+    initialized_ = false;
+    trackingLost = false;
+}
+
+// ------------------------------------------------------------ System
+
+System::System(const lf_system_config &cfg) : cfg_(cfg) {
+    if (hipStreamCreate(&stream_) != hipSuccess) throw std::bad_alloc();
+    IntrinsicMatrix K(cfg.K.width, cfg.K.height, cfg.K.fx, cfg.K.fy, cfg.K.cx, cfg.K.cy);
+    mapping_.reset(new DenseMapping(K, cfg.map, stream_));
+    std::vector<int> its(cfg.iterations, cfg.iterations + std::max(1, std::min(8, cfg.num_pyr)));
+    odometry_.reset(new DenseOdometry(K, cfg.num_pyr, its, stream_));
+}
+
+System::~System() {
+    mapping_.reset();
+    odometry_.reset();
+    for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+    for (auto &e : events_) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+hipEvent_t System::take_event() {
+    hipEvent_t e = nullptr;
+    if (!pool_.empty()) {
+        e = pool_.back();
+        pool_.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        e = nullptr;
+    }
+    return e;
+}
+
+void System::set_timing(bool on) {
+    timing_ = on;
+    for (auto &e : events_) { pool_.push_back(e.a); pool_.push_back(e.b); }
+    events_.clear();
+}
+
+int System::read_timing(int stage, int *launches, double *total_ms) {
+    LF_HIP_CHECK(hipStreamSynchronize(stream_));
+    int n = 0;
+    double t = 0;
+    for (auto &e : events_) {
+        if (e.stage != stage) continue;
+        float ms = 0;
+        LF_HIP_CHECK(hipEventElapsedTime(&ms, e.a, e.b));
+        t += ms;
+        ++n;
+    }
+    *launches = n;
+    *total_ms = t;
+    return LF_OK;
+}
+
+// process_images (system.cpp:40-114): track, fuse, raycast, re-pyramid.
+int System::process_images(const uint16_t *depth, const unsigned char *rgb) {
+    Ev fr{4, nullptr, nullptr}, ig{0, nullptr, nullptr}, rc{1, nullptr, nullptr}, tr{2, nullptr, nullptr};
+    if (timing_) {
+        fr.a = take_event(); fr.b = take_event(); ig.a = take_event(); ig.b = take_event();
+        rc.a = take_event(); rc.b = take_event(); tr.a = take_event(); tr.b = take_event();
+        LF_HIP_CHECK(hipEventRecord(fr.a, stream_));
+    }
+    DenseOdometry &od = *odometry_;
+    if (!od.trackingLost) {
+        if (timing_) LF_HIP_CHECK(hipEventRecord(tr.a, stream_));
+        LF_RC(od.trackFrame(depth, rgb, cfg_.depth_scale, initial_pose_));
+        if (timing_) LF_HIP_CHECK(hipEventRecord(tr.b, stream_));
+    }
+    DeviceImage &ref = od.get_reference_image();
+    if (mapping_enabled) LF_RC(mapping_->update(ref, od.pose24(), cfg_.integrate_colour != 0, ig.a, ig.b));
+    LF_RC(mapping_->raycast(ref.get_vmap(0).ptr<float>(), od.pose24(), rc.a, rc.b));
+    LF_RC(ref.resize_device_map());
+    if (timing_) {
+        LF_HIP_CHECK(hipEventRecord(fr.b, stream_));
+        events_.push_back(fr);
+        events_.push_back(tr);
+        if (mapping_enabled) events_.push_back(ig);
+        else { pool_.push_back(ig.a); pool_.push_back(ig.b); }
+        events_.push_back(rc);
+    }
+    return LF_OK;
+}
+
+int System::restart() {  // system.cpp:148-157
+    double T[16];
+    LF_RC(get_camera_pose(T));
+    double T12[12];
+    for (int i = 0; i < 12; ++i) T12[i] = T[i];
+    initial_pose_ = SE3d::from3x4(T12);
+    LF_RC(mapping_->reset_mapping());
+    odometry_->reset();
+    return LF_OK;
+}
+
+size_t System::fetch_mesh_vertex_only(float *v, int max_tri) { return mapping_->fetch_mesh_vertex_only(v, max_tri); }
+size_t System::fetch_mesh_with_normal(float *v, float *n, int max_tri) {
+    return mapping_->fetch_mesh_with_normal(v, n, max_tri);
+}
+size_t System::fetch_mesh_with_colour(float *v, unsigned char *c, int max_tri) {
+    return mapping_->fetch_mesh_with_colour(v, c, max_tri);
+}
+
+int System::get_camera_pose(double *out16) {
+    LfTrackState h;
+    LF_HIP_CHECK(hipMemcpyAsync(&h, odometry_->tracker().state(), sizeof(h), hipMemcpyDeviceToHost, stream_));
+    LF_HIP_CHECK(hipStreamSynchronize(stream_));
+    for (int i = 0; i < 12; ++i) out16[i] = h.pose[i];
+    out16[12] = out16[13] = out16[14] = 0;
+    out16[15] = 1;
+    return LF_OK;
+}
+
+}  // namespace fusion
+
+// ------------------------------------------------------------ C-ABI
+
+struct lf_system {
+    fusion::System *sys;
+};
+
+extern "C" int lf_system_create(const lf_system_config *cfg, lf_system **out) {
+    if (!cfg || !out) return lf_set_error(LF_ERR_ARG, "null argument to lf_system_create");
+    if (cfg->num_pyr < 1 || cfg->num_pyr > 8) return lf_set_error(LF_ERR_ARG, "num_pyr must be 1..8");
+    for (int l = 0; l < cfg->num_pyr; ++l) {
+        const int sh = 1 << l;
+        if (cfg->K.width % (2 * sh) != 0 && l + 1 < cfg->num_pyr)
+            return lf_set_error(LF_ERR_ARG, "width %d does not halve evenly over %d levels", cfg->K.width, cfg->num_pyr);
+        if (cfg->K.height % (2 * sh) != 0 && l + 1 < cfg->num_pyr)
+            return lf_set_error(LF_ERR_ARG, "height %d does not halve evenly over %d levels", cfg->K.height, cfg->num_pyr);
+    }
+    try {
+        *out = new lf_system{new fusion::System(*cfg)};
+    } catch (const std::bad_alloc &) {
+        return lf_set_error(LF_ERR_HIP, "device allocation failed in lf_system_create: %s", lf_last_error());
+    }
+    return LF_OK;
+}
+
+extern "C" int lf_system_destroy(lf_system *s) {
+    if (!s) return LF_OK;
+    delete s->sys;
+    delete s;
+    return LF_OK;
+}
+
+extern "C" int lf_system_process_images(lf_system *s, const uint16_t *depth, const unsigned char *rgb) {
+    if (!s || !depth || !rgb) return lf_set_error(LF_ERR_ARG, "null argument to process_images");
+    return s->sys->process_images(depth, rgb);
+}
+
+extern "C" int lf_system_synchronize(lf_system *s) {
+    LF_HIP_CHECK(hipStreamSynchronize(s->sys->stream()));
+    return LF_OK;
+}
+
+extern "C" int lf_system_get_pose(lf_system *s, double *out16) { return s->sys->get_camera_pose(out16); }
+
+extern "C" int lf_system_get_tracking(lf_system *s, int *success, float *icp_error) {
+    fusion::TrackingResult r = s->sys->odometry().tracker().result();
+    if (success) *success = r.sucess ? 1 : 0;
+    if (icp_error) *icp_error = r.icp_error;
+    return LF_OK;
+}
+
+extern "C" int lf_system_restart(lf_system *s) { return s->sys->restart(); }
+
+extern "C" int lf_system_fetch_mesh(lf_system *s, float *vertex_out, float *normal_out, unsigned char *colour_out,
+                                    int max_triangles, unsigned int *triangles_out) {
+    const fusion::DenseMapping &m = s->sys->mapping();
+    return lf_create_mesh(&m.params(), &m.storage(), vertex_out, normal_out, colour_out, max_triangles,
+                          triangles_out, s->sys->stream());
+}
+
+extern "C" int lf_system_map(lf_system *s, lf_map_params *p, lf_map_storage *st) {
+    if (p) *p = s->sys->mapping().params();
+    if (st) *st = s->sys->mapping().storage();
+    return LF_OK;
+}
+
+extern "C" int lf_system_maps(lf_system *s, int level, const float **vmap, const float **nmap) {
+    fusion::DeviceImage &ref = s->sys->odometry().get_reference_image();
+    if (level < 0 || level >= ref.levels()) return lf_set_error(LF_ERR_ARG, "bad level %d", level);
+    if (vmap) *vmap = ref.get_vmap(level).ptr<float>();
+    if (nmap) *nmap = ref.get_nmap(level).ptr<float>();
+    return LF_OK;
+}
+
+extern "C" void *lf_system_stream(lf_system *s) { return s->sys->stream(); }
+
+extern "C" int lf_system_set_timing(lf_system *s, int enable) {
+    s->sys->set_timing(enable != 0);
+    return LF_OK;
+}
+
+extern "C" int lf_system_read_timing(lf_system *s, int stage, int *launches, double *total_ms) {
+    return s->sys->read_timing(stage, launches, total_ms);
+}

@@ -1,0 +1,327 @@
+"""GPU parity: every stage of the HIP path against the CPU oracle on identical inputs.
+
+Bars (from the path's specification):
+  * allocated and visible block sets: bit-exact
+  * TSDF: within 1e-3 truncation units (|d sdf_int16| <= 32); weights rel. 1e-5
+  * raycast and marching cubes on an identical volume: bit-exact vertex maps,
+    identical triangle topology
+  * poses of a tracked sequence vs the FP64 oracle: <= 0.5 mm and <= 0.05 deg
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from libfusion_amd import capi, configs
+from libfusion_amd.synth import Scene
+from oracle import oracle as O
+from tests.helpers import canonical_triangles, pose_err, sort_blocks, vmap_compare
+
+pytestmark = pytest.mark.gpu
+
+SDF_TOL = 32          # int16 units: 1e-3 truncation units = 32.767
+W_RTOL = 1e-5
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def o_params(p):
+    return O.Params(p.num_buckets, p.num_entries, p.num_blocks, p.max_triangles, p.zmin_raycast, p.zmax_raycast,
+                    p.zmin_update, p.zmax_update, p.voxel_size, p.truncation)
+
+
+def o_K(K):
+    return O.Intr(K.width, K.height, K.fx, K.fy, K.cx, K.cy, K.invfx, K.invfy)
+
+
+def half_res(sc):
+    """Intrinsics of the scene's camera at half resolution (pixel-centre aligned)."""
+    return capi.intrinsics(sc.width // 2, sc.height // 2, sc.fx / 2, sc.fy / 2, (sc.cx + 0.5) / 2 - 0.5,
+                           (sc.cy + 0.5) / 2 - 0.5)
+
+
+def frame_inputs(sc, T, K, frame_id):
+    """Depth (float m), oracle FP32 normal map and RGB of one rendered view."""
+    d16, rgb = sc.render(T, K.width, K.height, K.fx, K.fy, K.cx, K.cy, noise=True, frame_id=frame_id)
+    depth = d16.astype(np.float32) * np.float32(sc.depth_scale)
+    filt = O.bilateral(depth)
+    nmap = O.nmap(O.back_project(filt, o_K(K)))
+    return depth, nmap, rgb
+
+
+def compare_maps(gmap, omap):
+    gb, gsdf, gw, grgb = gmap.voxels()
+    ob, osdf, ow, orgb = omap.voxels()
+    go, gbs = sort_blocks(gb)
+    oo, obs = sort_blocks(ob)
+    assert len(gbs) == len(obs), (len(gbs), len(obs))
+    assert np.array_equal(gbs, obs), "allocated block sets differ"
+    dsdf = np.abs(gsdf[go].astype(np.int32) - osdf[oo].astype(np.int32))
+    assert dsdf.max() <= SDF_TOL, dsdf.max()
+    assert np.allclose(gw[go], ow[oo], rtol=W_RTOL, atol=0), np.max(np.abs(gw[go] - ow[oo]))
+    assert np.abs(grgb[go].astype(np.int32) - orgb[oo].astype(np.int32)).max() <= 1
+    return float(np.mean(dsdf == 0)), len(gbs)
+
+
+@pytest.fixture(scope="module")
+def c0():
+    sc, K, p = configs.c0()
+    T = sc.pose(0)
+    depth, nmap, rgb = frame_inputs(sc, T, K, 0)
+    return sc, K, p, T, depth, nmap, rgb
+
+
+def test_pose_block_matches_oracle():
+    T = Scene(1, 1).pose(5)
+    assert np.array_equal(capi.pose24(T), O.pose24(T, np.float32))
+
+
+def test_update_c0_block_set_and_tsdf(c0):
+    sc, K, p, T, depth, nmap, rgb = c0
+    gmap = capi.Map(p, K.width, K.height)
+    gmap.update_weighted(dev(depth), dev(nmap), dev(rgb), dev(capi.pose24(T)), K)
+    torch.cuda.synchronize()
+    omap = O.Map(o_params(p), np.float32)
+    omap.update_weighted(depth, nmap, rgb, T, o_K(K))
+    c = gmap.counters()
+    assert c[capi.CNT_OVERFLOW] == 0
+    exact, n = compare_maps(gmap, omap)
+    assert n > 1000 and exact > 0.99
+    _, gv = sort_blocks(gmap.visible_blocks())
+    _, ov = sort_blocks(omap.visible())
+    assert np.array_equal(gv, ov), "visible sets differ"
+
+
+def test_update_sequence_fusion_half_res():
+    """Five views fused on both sides (exercises the running weighted average)."""
+    sc = Scene(1, 1)
+    K = half_res(sc)
+    p = capi.map_params(2 ** 18, 1 << 17, 0.01, 0.03)
+    gmap = capi.Map(p, K.width, K.height)
+    omap = O.Map(o_params(p), np.float32)
+    for f in range(0, 25, 5):
+        T = sc.pose(f)
+        depth, nmap, rgb = frame_inputs(sc, T, K, f)
+        gmap.update_weighted(dev(depth), dev(nmap), dev(rgb), dev(capi.pose24(T)), K)
+        omap.update_weighted(depth, nmap, rgb, T, o_K(K))
+        torch.cuda.synchronize()
+        compare_maps(gmap, omap)
+        _, gv = sort_blocks(gmap.visible_blocks())
+        _, ov = sort_blocks(omap.visible())
+        assert np.array_equal(gv, ov)
+
+
+def test_update_full_resolution_block_set():
+    """640x480, 5 mm voxels: the exact allocated set on a full-size frame."""
+    sc, K, p = configs.c1(num_blocks=1 << 18)
+    T = sc.pose(0)
+    depth, nmap, rgb = frame_inputs(sc, T, K, 0)
+    gmap = capi.Map(p, K.width, K.height)
+    gmap.update_weighted(dev(depth), dev(nmap), dev(rgb), dev(capi.pose24(T)), K)
+    torch.cuda.synchronize()
+    omap = O.Map(o_params(p), np.float32)
+    omap.update_weighted(depth, nmap, rgb, T, o_K(K))
+    _, n = compare_maps(gmap, omap)
+    assert n > 10000
+
+
+def load_gpu_volume_into_oracle(gmap, omap):
+    b, sdf, w, rgb = gmap.voxels()
+    omap.set_blocks(b, sdf, w, rgb)
+
+
+def test_raycast_bit_exact_on_identical_volume(c0):
+    sc, K, p, T, depth, nmap, rgb = c0
+    gmap = capi.Map(p, K.width, K.height)
+    pose = dev(capi.pose24(T))
+    gmap.update_weighted(dev(depth), dev(nmap), dev(rgb), pose, K)
+    omap = O.Map(o_params(p), np.float32)
+    omap.update_weighted(depth, nmap, rgb, T, o_K(K))
+    load_gpu_volume_into_oracle(gmap, omap)
+    for check_colour in (True, False):
+        vm = torch.full((K.height, K.width, 4), float("nan"), device="cuda")
+        rgb_out = torch.zeros((K.height, K.width, 3), dtype=torch.uint8, device="cuda")
+        gmap.raycast(vm, pose, K, check_colour=check_colour, rgb_out=rgb_out)
+        torch.cuda.synchronize()
+        ovm, orgb = omap.raycast(T, o_K(K), K.width, K.height, check_colour=check_colour, want_rgb=True)
+        g = vm.cpu().numpy()
+        zx = gmap.t["zrange"].cpu().numpy()
+        tiles = (K.height // 8) * (K.width // 8)
+        ozx, ozy = omap.zrange(T, o_K(K), K.width, K.height)
+        assert np.array_equal(zx[:tiles], ozx.ravel()) and np.array_equal(zx[tiles:], ozy.ravel())
+        mism, dmax, nboth = vmap_compare(g, ovm)
+        assert nboth > 0.5 * K.width * K.height
+        assert mism == 0.0 and dmax == 0.0, (mism, dmax)
+        if check_colour:
+            assert np.array_equal(rgb_out.cpu().numpy(), orgb)
+
+
+def test_raycast_close_to_surface(c0):
+    """Raycast hits of a noise-free view lie on the analytic surface (independent
+    of the oracle): within half a voxel for almost every ray."""
+    sc, K, p, T, _, _, _ = c0
+    d16, rgb = sc.render(T, noise=False)
+    depth = d16.astype(np.float32) * np.float32(sc.depth_scale)
+    nmap = O.nmap(O.back_project(O.bilateral(depth), o_K(K)))
+    gmap = capi.Map(p, K.width, K.height)
+    pose = dev(capi.pose24(T))
+    gmap.update_weighted(dev(depth), dev(nmap), dev(rgb), pose, K)
+    vm = torch.full((K.height, K.width, 4), float("nan"), device="cuda")
+    gmap.raycast(vm, pose, K)
+    v = vm.cpu().numpy().reshape(-1, 4)
+    v = v[~np.isnan(v[:, 0])]
+    pw = (T[:3, :3] @ v[:, :3].T.astype(np.float64)).T + T[:3, 3]
+    d = np.abs(sc.sdf(pw))
+    assert np.median(d) < 0.002 and np.mean(d < 0.005) > 0.98, (np.median(d), np.mean(d < 0.005))
+
+
+def test_mesh_topology_on_identical_volume(c0):
+    sc, K, p, T, depth, nmap, rgb = c0
+    gmap = capi.Map(p, K.width, K.height)
+    gmap.update_weighted(dev(depth), dev(nmap), dev(rgb), dev(capi.pose24(T)), K)
+    omap = O.Map(o_params(p), np.float32)
+    load_gpu_volume_into_oracle(gmap, omap)
+    for mode in (0, 1, 2):
+        gtri, gx, gcount = gmap.mesh(mode=mode, max_triangles=2_000_000)
+        otri, ox, ocount = omap.mesh(mode=mode)
+        assert gcount == ocount and gcount > 1000
+        gt, gxe = canonical_triangles(gtri.cpu().numpy(), None if gx is None else gx.cpu().numpy())
+        ot, oxe = canonical_triangles(otri, ox)
+        assert np.array_equal(gt, ot)
+        if mode == 1:
+            assert np.allclose(gxe, oxe, atol=1e-6, equal_nan=True)
+        if mode == 2:
+            assert np.array_equal(gxe, oxe)
+
+
+def test_mesh_capacity_overflow_reported(c0):
+    sc, K, p, T, depth, nmap, rgb = c0
+    gmap = capi.Map(p, K.width, K.height)
+    gmap.update_weighted(dev(depth), dev(nmap), dev(rgb), dev(capi.pose24(T)), K)
+    tri, _, count = gmap.mesh(mode=0, max_triangles=100)
+    assert len(tri) == 100 and count > 100
+
+
+def test_image_pipeline_matches_oracle():
+    sc = Scene(1, 1)
+    d16, rgb = sc.frame(3)
+    W, H = sc.width, sc.height
+    K = capi.intrinsics(W, H, sc.fx, sc.fy, sc.cx, sc.cy)
+    lib = capi.load()
+    st = capi.stream_ptr()
+    P = capi.ptr
+    d16_t, rgb_t = dev(d16), dev(rgb)
+    raw = torch.empty((H, W), device="cuda")
+    capi.check(lib.lf_depth_to_float(P(d16_t), P(raw), W, H, float(np.float32(sc.depth_scale)), st))
+    depth = d16.astype(np.float32) * np.float32(sc.depth_scale)
+    assert np.array_equal(raw.cpu().numpy(), depth)
+    filt = torch.empty_like(raw)
+    capi.check(lib.lf_bilateral_filter(P(raw), P(filt), W, H, st))
+    ofilt = O.bilateral(depth)
+    assert np.allclose(filt.cpu().numpy(), ofilt, rtol=2e-6, atol=1e-7)
+    ofilt = filt.cpu().numpy()  # continue on identical inputs
+    half = torch.empty((H // 2, W // 2), device="cuda")
+    capi.check(lib.lf_pyr_down_depth(P(filt), P(half), W, H, st))
+    assert np.array_equal(half.cpu().numpy(), O.resize_half(ofilt))
+    grey = torch.empty((H, W), device="cuda")
+    capi.check(lib.lf_rgb_to_intensity(P(rgb_t), P(grey), W, H, st))
+    assert np.array_equal(grey.cpu().numpy(), O.grey(rgb))
+    gd = torch.empty(((H + 1) // 2, (W + 1) // 2), device="cuda")
+    capi.check(lib.lf_pyr_down_image(P(grey), P(gd), W, H, st))
+    assert np.array_equal(gd.cpu().numpy(), O.pyr_down(O.grey(rgb)))
+    dx, dy = torch.empty_like(grey), torch.empty_like(grey)
+    capi.check(lib.lf_compute_derivative(P(grey), P(dx), P(dy), W, H, st))
+    odx, ody = O.derivative(O.grey(rgb))
+    assert np.array_equal(dx.cpu().numpy(), odx) and np.array_equal(dy.cpu().numpy(), ody)
+    vm = torch.empty((H, W, 4), device="cuda")
+    capi.check(lib.lf_back_project_depth(P(filt), P(vm), W, H, ctypes.byref(K), st))
+    ovm = O.back_project(ofilt, o_K(K))
+    assert np.array_equal(vm.cpu().numpy(), ovm, equal_nan=True)
+    nm = torch.empty_like(vm)
+    capi.check(lib.lf_compute_nmap(P(vm), P(nm), W, H, st))
+    assert np.array_equal(nm.cpu().numpy(), O.nmap(ovm), equal_nan=True)
+    vh = torch.empty((H // 2, W // 2, 4), device="cuda")
+    capi.check(lib.lf_pyr_down_vmap(P(vm), P(vh), W, H, st))
+    assert np.array_equal(vh.cpu().numpy(), O.resize_half(ovm), equal_nan=True)
+
+
+def level_maps(sc, frame, K):
+    d16, rgb = sc.render(sc.pose(frame), K.width, K.height, K.fx, K.fy, K.cx, K.cy, noise=True, frame_id=frame)
+    depth = d16.astype(np.float32) * np.float32(sc.depth_scale)
+    vm = O.back_project(O.bilateral(depth), o_K(K))
+    nm = O.nmap(vm)
+    I = O.grey(rgb)
+    dx, dy = O.derivative(I)
+    return vm, nm, I, dx, dy
+
+
+@pytest.mark.parametrize("stddev", [0.0, 7.5])
+def test_reductions_match_oracle(stddev):
+    sc = Scene(1, 1)
+    K = capi.intrinsics(sc.width, sc.height, sc.fx, sc.fy, sc.cx, sc.cy)
+    cv, cn, cI, cdx, cdy = level_maps(sc, 1, K)
+    lv, ln, lI, _, _ = level_maps(sc, 0, K)
+    Trel = np.linalg.inv(sc.pose(1)) @ sc.pose(0)
+    pose12 = Trel[:3, :].astype(np.float32).ravel()
+    lib = capi.load()
+    scratch = torch.zeros(int(lib.lf_reduce_scratch_floats()), device="cuda")
+    out = torch.zeros(29, device="cuda")
+    P = capi.ptr
+    st = capi.stream_ptr()
+    # keep every device input alive until the kernels have read it
+    t_cv, t_cn, t_lv, t_ln, t_pose = dev(cv), dev(cn), dev(lv), dev(ln), dev(pose12)
+    t_cI, t_lI, t_dx, t_dy = dev(cI), dev(lI), dev(cdx), dev(cdy)
+    for _ in range(2):  # the second call checks the ticket reset
+        capi.check(lib.lf_icp_reduce(P(t_cv), P(t_cn), P(t_lv), P(t_ln), K.width, K.height,
+                                     P(t_pose), ctypes.byref(K), P(scratch), P(out), st))
+        g = out.cpu().numpy().astype(np.float64)
+        o32 = O.icp_reduce(cv, cn, lv, ln, pose12, o_K(K), np.float32).astype(np.float64)
+        o64 = O.icp_reduce(cv, cn, lv, ln, pose12, o_K(K), np.float64)
+        # same FP32 gating decisions as the oracle's float instantiation ...
+        assert g[28] == o32[28] and g[28] > 1000
+        # ... sums equal up to FP32 reassociation, and close to the FP64 sums
+        assert np.allclose(g, o32, rtol=1e-4, atol=1e-6 * np.abs(o32).max())
+        assert np.allclose(g, o64, rtol=1e-3, atol=1e-4 * np.abs(o64).max())
+    capi.check(lib.lf_rgb_step(P(t_cI), P(t_lI), P(t_lv), P(t_dx), P(t_dy), K.width, K.height,
+                               float(stddev), P(t_pose), ctypes.byref(K), P(scratch), P(out), st))
+    g = out.cpu().numpy().astype(np.float64)
+    o32 = O.rgb_step(cI, lI, lv, cdx, cdy, np.float32(stddev), pose12, o_K(K), np.float32).astype(np.float64)
+    o64 = O.rgb_step(cI, lI, lv, cdx, cdy, np.float32(stddev), pose12, o_K(K), np.float64)
+    assert g[28] == o32[28] and g[28] > 1000
+    assert np.allclose(g, o32, rtol=1e-4, atol=1e-6 * np.abs(o32).max())
+    assert np.allclose(g, o64, rtol=1e-3, atol=1e-4 * np.abs(o64).max())
+
+
+def run_sequence(sc, K, p, levels, iters, frames, scale_depth=None):
+    gsys = capi.System(K, p, levels, iters, sc.depth_scale)
+    osys = O.System(o_params(p), o_K(K), levels, iters, sc.depth_scale, np.float64)
+    res = []
+    for f in frames:
+        d16, rgb = sc.render(sc.pose(f), K.width, K.height, K.fx, K.fy, K.cx, K.cy, noise=True, frame_id=f)
+        t_d, t_rgb = dev(d16), dev(rgb)
+        torch.cuda.synchronize()  # the system runs on its own stream
+        gsys.process(t_d, t_rgb)
+        gsys.synchronize()
+        osys.process(d16, rgb)
+        res.append((gsys.pose(), osys.pose(), gsys.tracking()[0], osys.success()))
+    return gsys, osys, res
+
+
+def test_tracked_sequence_vs_fp64_oracle():
+    """Half-resolution 12-frame orbit: per-frame pose vs FP64 <= 0.5 mm, 0.05 deg."""
+    sc = Scene(1, 1)
+    K = half_res(sc)
+    p = capi.map_params(2 ** 18, 1 << 17, 0.01, 0.03)
+    gsys, osys, res = run_sequence(sc, K, p, 3, (10, 5, 3), range(12))
+    for f, (Tg, To, sg, so) in enumerate(res):
+        dt, dr = pose_err(Tg, To)
+        assert sg == so, f
+        assert dt <= 5e-4 and dr <= 0.05, (f, dt, dr)
+    # the reconstruction tracks the real trajectory too
+    T0 = sc.pose(0)
+    Tgt = np.linalg.inv(T0) @ sc.pose(11)
+    dt, dr = pose_err(res[-1][0], Tgt)
+    assert dt < 0.05 and dr < 2.0, (dt, dr)

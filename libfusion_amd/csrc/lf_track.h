@@ -22,8 +22,7 @@ struct LfTrackState {
     // odometry chain (DenseOdometry::trackFrame, rgbd_odometry.cpp:18-48)
     double pose[12];     // T_world_camera of the current frame
     float pose24[24];    // float T_world_camera and T_camera_world for the mapping kernels
-    unsigned int ticket; // last-workgroup election of the fused reduction
-    int pad[3];
+    int pad[4];
 };
 
 // Per-level inputs of one tracking iteration.

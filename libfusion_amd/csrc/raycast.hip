@@ -39,6 +39,7 @@ struct RayDev {
     int tiles_x, tiles_y;
     int num_buckets, num_entries;
     float voxel_size, truncation, zmin_raycast, zmax_raycast;
+    int pool_fits_lds;  // pool indices fit the 24-bit field of the LDS cache
 };
 
 __global__ void k_zrange_clear(RayDev r) {
@@ -101,47 +102,100 @@ __global__ __launch_bounds__(256) void k_zrange_fill(RayDev r, const float *__re
     }
 }
 
+// Workgroup-shared, direct-mapped cache of block lookups in LDS, negative
+// results included.  An entry is one 64-bit word: valid bit, three 13-bit
+// block coordinates and a 24-bit pool index (all ones = not allocated), so a
+// single LDS store / load keeps it consistent without atomics.  The 4 tiles of
+// a workgroup march through nearly the same blocks, and most samples of a ray
+// lie in unallocated blocks, so most global hash probes disappear.
+constexpr int kLdsCache = 512;
+constexpr unsigned long long kNoBlock = 0xFFFFFFull;
+
 struct BlockCache {
     int bx, by, bz, b;
+    unsigned long long *lds;  // null: no shared cache (coordinates / pool out of range)
 };
 
-// findVoxel + read_sdf (ray_tracing.cu:291-305) with a one-block cache.
-LF_DEV float read_sdf(const RayDev &r, BlockCache &bc, f3 p, bool &valid, int *voxel_out = nullptr) {
-    const i3 v = to_i3(p);
-    const int bx = blk(v.x), by = blk(v.y), bz = blk(v.z);
-    if (bx != bc.bx || by != bc.by || bz != bc.bz) {
-        bc.b = find_block(r.keys, r.slot_block, bx, by, bz, r.num_buckets, r.num_entries);
-        bc.bx = bx; bc.by = by; bc.bz = bz;
-    }
-    if (bc.b >= 0) {
-        const int vi = bc.b * LF_BLOCK3 + lidx(v.x, v.y, v.z);
-        if (voxel_out) *voxel_out = vi;
-        if (r.weight[vi] != 0.0f) {
-            valid = true;
-            return unpack_sdf(r.sdf[vi]);
-        }
-    }
-    valid = false;
-    return __int_as_float(0x7fc00000);
+LF_DEV bool fits13(int v) { return v >= -4096 && v < 4096; }
+
+LF_DEV unsigned long long lds_tag(int bx, int by, int bz) {
+    return (1ull << 63) | ((unsigned long long)(bx & 0x1FFF) << 50) | ((unsigned long long)(by & 0x1FFF) << 37) |
+           ((unsigned long long)(bz & 0x1FFF) << 24);
 }
 
-// read_sdf_interped (ray_tracing.cu:307-339).
+// Resolve the pool block of voxel coordinate v: the lane's last block, then the
+// workgroup's LDS cache, then the hash table.
+LF_DEV int resolve(const RayDev &r, BlockCache &bc, const i3 &v) {
+    const int bx = blk(v.x), by = blk(v.y), bz = blk(v.z);
+    if (bx == bc.bx && by == bc.by && bz == bc.bz) return bc.b;
+    bc.bx = bx; bc.by = by; bc.bz = bz;
+    if (bc.lds && fits13(bx) && fits13(by) && fits13(bz)) {
+        const unsigned long long tag = lds_tag(bx, by, bz);
+        const unsigned int h = ((unsigned int)bx * 73856093u ^ (unsigned int)by * 19349669u ^
+                                (unsigned int)bz * 83492791u) & (kLdsCache - 1);
+        const unsigned long long e = bc.lds[h];
+        if ((e & ~kNoBlock) == tag) {
+            const unsigned long long idx = e & kNoBlock;
+            bc.b = idx == kNoBlock ? -1 : (int)idx;
+            return bc.b;
+        }
+        bc.b = find_block(r.keys, r.slot_block, bx, by, bz, r.num_buckets, r.num_entries);
+        bc.lds[h] = tag | (bc.b < 0 ? kNoBlock : (unsigned long long)bc.b);
+        return bc.b;
+    }
+    bc.b = find_block(r.keys, r.slot_block, bx, by, bz, r.num_buckets, r.num_entries);
+    return bc.b;
+}
+
+// findVoxel + read_sdf (ray_tracing.cu:291-305).  The weight and the sdf are
+// loaded together (one memory round trip), then the weight decides validity.
+LF_DEV float read_sdf(const RayDev &r, BlockCache &bc, f3 p, bool &valid, int *voxel_out = nullptr) {
+    const i3 v = to_i3(p);
+    const int b = resolve(r, bc, v);
+    valid = false;
+    if (b < 0) return __int_as_float(0x7fc00000);
+    const int vi = b * LF_BLOCK3 + lidx(v.x, v.y, v.z);
+    const float w = r.weight[vi];
+    const short sd = r.sdf[vi];
+    if (voxel_out) *voxel_out = vi;
+    if (w == 0.0f) return __int_as_float(0x7fc00000);
+    valid = true;
+    return unpack_sdf(sd);
+}
+
+// read_sdf_interped (ray_tracing.cu:307-339).  The 8 corner voxels are
+// resolved first, then all 16 weight / sdf loads are issued in one round trip;
+// the arithmetic is the reference's order.
 LF_DEV float read_interp(const RayDev &r, BlockCache &bc, f3 p, bool &valid) {
     const f3 xyz = {p.x - floorf(p.x), p.y - floorf(p.y), p.z - floorf(p.z)};
-    bool vp, v;
-    float s0 = read_sdf(r, bc, p, vp);
-    float s1 = read_sdf(r, bc, add(p, f3{1.0f, 0.0f, 0.0f}), v); vp &= v;
-    float r0 = (1.0f - xyz.x) * s0 + xyz.x * s1;
-    s0 = read_sdf(r, bc, add(p, f3{0.0f, 1.0f, 0.0f}), v); vp &= v;
-    s1 = read_sdf(r, bc, add(p, f3{1.0f, 1.0f, 0.0f}), v); vp &= v;
-    float r1 = (1.0f - xyz.x) * s0 + xyz.x * s1;
+    // corner order of the reference: (0,0,0) (1,0,0) (0,1,0) (1,1,0) (0,0,1) (1,0,1) (0,1,1) (1,1,1)
+    int vi[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const f3 q = add(p, f3{(float)(k & 1), (float)((k >> 1) & 1), (float)((k >> 2) & 1)});
+        const i3 v = to_i3(q);
+        const int b = resolve(r, bc, v);
+        vi[k] = b < 0 ? -1 : b * LF_BLOCK3 + lidx(v.x, v.y, v.z);
+    }
+    float w[8], s[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int i = vi[k] < 0 ? 0 : vi[k];
+        w[k] = r.weight[i];
+        s[k] = unpack_sdf(r.sdf[i]);
+    }
+    bool vp = true;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const bool ok = vi[k] >= 0 && w[k] != 0.0f;
+        vp = vp && ok;
+        if (!ok) s[k] = __int_as_float(0x7fc00000);
+    }
+    float r0 = (1.0f - xyz.x) * s[0] + xyz.x * s[1];
+    float r1 = (1.0f - xyz.x) * s[2] + xyz.x * s[3];
     const float r2 = (1.0f - xyz.y) * r0 + xyz.y * r1;
-    s0 = read_sdf(r, bc, add(p, f3{0.0f, 0.0f, 1.0f}), v); vp &= v;
-    s1 = read_sdf(r, bc, add(p, f3{1.0f, 0.0f, 1.0f}), v); vp &= v;
-    r0 = (1.0f - xyz.x) * s0 + xyz.x * s1;
-    s0 = read_sdf(r, bc, add(p, f3{0.0f, 1.0f, 1.0f}), v); vp &= v;
-    s1 = read_sdf(r, bc, add(p, f3{1.0f, 1.0f, 1.0f}), v); vp &= v;
-    r1 = (1.0f - xyz.x) * s0 + xyz.x * s1;
+    r0 = (1.0f - xyz.x) * s[4] + xyz.x * s[5];
+    r1 = (1.0f - xyz.x) * s[6] + xyz.x * s[7];
     const float r3 = (1.0f - xyz.y) * r0 + xyz.y * r1;
     valid = vp;
     return (1.0f - xyz.z) * r2 + xyz.z * r3;
@@ -156,6 +210,9 @@ __global__ __launch_bounds__(256) void k_raycast(RayDev r, float *__restrict__ v
     // DenseMapping::raycast leaves the map untouched when nothing is visible
     // (voxel_hashing.cpp:92-93) or no rendering block was made (:105).
     if (r.counters[LF_CNT_VISIBLE] == 0u || r.counters[LF_CNT_RENDER_BLOCKS] == 0u) return;
+    __shared__ unsigned long long s_cache[kLdsCache];
+    for (int i = threadIdx.x; i < kLdsCache; i += blockDim.x) s_cache[i] = 0ull;
+    __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tile_x = blockIdx.x * 2 + (wave & 1), tile_y = blockIdx.y * 2 + (wave >> 1);
     const int x = tile_x * 8 + (lane & 7), y = tile_y * 8 + (lane >> 3);
@@ -180,7 +237,7 @@ __global__ __launch_bounds__(256) void k_raycast(RayDev r, float *__restrict__ v
         const f3 be = mul(xform(Twc, pt), inv_vs);
         const f3 dir = normalised(sub(be, bs));
         f3 res = bs;
-        BlockCache bc = {kBad, kBad, kBad, -1};
+        BlockCache bc = {kBad, kBad, kBad, -1, r.pool_fits_lds ? s_cache : nullptr};
         float sdf = 1.0f, last_sdf, step;
         bool valid = false, back_face = false;
         while (dist_s < dist_e) {
@@ -237,6 +294,7 @@ int lf_raycast_ev(const lf_map_params *p, const lf_map_storage *s, float *vmap, 
     r.num_buckets = p->num_buckets; r.num_entries = p->num_entries;
     r.voxel_size = p->voxel_size; r.truncation = p->truncation;
     r.zmin_raycast = p->zmin_raycast; r.zmax_raycast = p->zmax_raycast;
+    r.pool_fits_lds = p->num_blocks < (int)kNoBlock ? 1 : 0;
     const int T = s->tiles_x * s->tiles_y;
     k_zrange_clear<<<std::max(1, std::min(1024, (T + 255) / 256)), 256, 0, st>>>(r);
     LF_LAUNCH_CHECK();

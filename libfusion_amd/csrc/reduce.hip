@@ -12,10 +12,15 @@
 //    pixel (they share the projected vertex), accumulating 2 x 29 sums per lane.
 //  * Wavefront reduction uses 64-lane shuffles, then 4 wavefront partials meet
 //    in LDS; each workgroup writes one partial row.
-//  * The last workgroup to finish (atomic ticket, agent-scope fences) sums the
-//    partial rows in a fixed order (deterministic), then one lane builds
-//    A = 1e6 H_icp + H_rgb, solves it with an FP64 LDL^T, applies
-//    T <- exp(dx) T in FP64 and runs the reference's divergence / revert logic.
+//  * One lane per reference pixel, so a whole pyramid level is in flight at
+//    once (the pass is bound by memory latency, not bandwidth); each pixel's
+//    loads are issued in two rounds.
+//  * A second one-workgroup kernel sums the partial rows in a fixed order
+//    (deterministic), then one lane builds A = 1e6 H_icp + H_rgb, solves it
+//    with an FP64 LDL^T, applies T <- exp(dx) T in FP64 and runs the
+//    reference's divergence / revert logic.  (A last-workgroup ticket in one
+//    kernel needs an agent-scope release per workgroup, i.e. an L2 write-back
+//    per workgroup on a multi-XCD part; the kernel boundary is cheaper.)
 //    The next iteration reads the updated pose from device memory: a whole
 //    frame of tracking is a fixed sequence of launches with no host round trip
 //    (the reference downloads the sums and solves on the host twice per
@@ -29,8 +34,9 @@ using namespace lf;
 
 namespace {
 
-constexpr int kGrid = 256;     // workgroups of a reduction
-constexpr int kThreads = 256;  // lanes per workgroup
+constexpr int kGrid = 1024;    // maximum workgroups of a reduction
+constexpr int kThreads = 512;  // lanes per workgroup (8 wavefronts)
+constexpr int kWaves = kThreads / 64;
 constexpr int kVals = 58;      // 29 ICP + 29 RGB sums
 
 struct ReduceArgs {
@@ -39,76 +45,104 @@ struct ReduceArgs {
     float stddev;          // modes 1/2
     float angle, dthr;
     float *partials;       // kGrid * kVals floats
-    unsigned int *ticket;
     float *out29;          // modes 1/2
     LfTrackState *st;      // mode 3
     int first_iter;
     float icp_weight;
 };
 
-// ICPReduction::searchPoint + getRow (pose_estimator.cu:168-222).
-LF_DEV bool icp_row(const ReduceArgs &a, const float *P, int i, float *row) {
+LF_DEV void accumulate(float *s, const float *row);
+
+// Both rows of one reference pixel — ICPReduction::searchPoint + getRow
+// (pose_estimator.cu:168-222) and RgbReduction2::find_corresp +
+// compute_jacobian with interp2 (m_estimator.cu:315-391) — with all loads in
+// two rounds: the pixel's own vertex / normal / intensity first, then every
+// gather (the ICP correspondence and the 12 bilinear taps of the photometric
+// term) together.  The arithmetic follows the reference's operation order.
+template <bool kIcp, bool kRgb>
+LF_DEV void pixel_rows(const ReduceArgs &a, const float *P, float stddev, float th, int i, float *si, float *sr) {
     const LfTrackLevel &L = a.lv;
     const float4 lv = reinterpret_cast<const float4 *>(L.last_vmap)[i];
-    if (isnan(lv.x)) return false;
+    float4 ln = make_float4(0.f, 0.f, 0.f, 0.f);
+    float il = 0.0f;
+    if (kIcp) ln = reinterpret_cast<const float4 *>(L.last_nmap)[i];
+    if (kRgb) il = L.last_I[i];
     const f3 vg = xform(P, f3{lv.x, lv.y, lv.z});
-    const float invz = 1.0f / vg.z;
-    const int u = proj_floor(vg.x * invz * L.K.fx + L.K.cx + 0.5f);
-    const int v = proj_floor(vg.y * invz * L.K.fy + L.K.cy + 0.5f);
-    if (u < 0 || v < 0 || u >= L.cols || v >= L.rows) return false;
-    const size_t j = (size_t)v * L.cols + u;
-    const float4 cv = reinterpret_cast<const float4 *>(L.curr_vmap)[j];
-    const float4 ln = reinterpret_cast<const float4 *>(L.last_nmap)[i];
-    const float4 cn = reinterpret_cast<const float4 *>(L.curr_nmap)[j];
-    const f3 vc = {cv.x, cv.y, cv.z};
-    const f3 ng = rotate(P, f3{ln.x, ln.y, ln.z});
-    const f3 nc = {cn.x, cn.y, cn.z};
-    const float dist = norm(sub(vg, vc));
-    const float sine = norm(cross(nc, ng));
-    if (!(sine < a.angle && dist <= a.dthr && !isnan(nc.x) && !isnan(ng.x))) return false;
-    const f3 c = cross(vg, ng);
-    row[0] = ng.x; row[1] = ng.y; row[2] = ng.z;
-    row[3] = c.x; row[4] = c.y; row[5] = c.z;
-    row[6] = dot(ng, sub(vc, vg));
-    return true;
-}
-
-LF_DEV float interp2(const float *img, int w, float x, float y) {  // m_estimator.cu:340-346
-    const int u = (int)floorf(x), v = (int)floorf(y);
-    const float cx = x - (float)u, cy = y - (float)v;
-    const float *r0 = img + (size_t)v * w + u, *r1 = r0 + w;
-    return (r0[0] * (1.0f - cx) + r0[1] * cx) * (1.0f - cy) + (r1[0] * (1.0f - cx) + r1[1] * cx) * cy;
-}
-
-// RgbReduction2::find_corresp + compute_jacobian (m_estimator.cu:315-391).
-LF_DEV bool rgb_row(const ReduceArgs &a, const float *P, float stddev, float th, int i, float *row) {
-    const LfTrackLevel &L = a.lv;
-    const float4 pt = reinterpret_cast<const float4 *>(L.last_vmap)[i];
-    if (pt.w < 0.0f || isnan(pt.x)) return false;
-    const float il = L.last_I[i];
-    if (!isfinite(il)) return false;
-    const f3 p = xform(P, f3{pt.x, pt.y, pt.z});
-    const float u0 = p.x / p.z * L.K.fx + L.K.cx, v0 = p.y / p.z * L.K.fy + L.K.cy;
-    if (!(u0 >= 2.0f && u0 < (float)(L.cols - 2) && v0 >= 2.0f && v0 < (float)(L.rows - 2))) return false;
-    const float ic = interp2(L.curr_I, L.cols, u0, v0);
-    const float dx = interp2(L.dIdx, L.cols, u0, v0);
-    const float dy = interp2(L.dIdy, L.cols, u0, v0);
-    if (!((dx > 2.0f || dy > 2.0f) && isfinite(ic) && isfinite(dx) && isfinite(dy))) return false;
-    const float zi = 1.0f / p.z;
-    f3 l;
-    l.x = dx * L.K.fx * zi;
-    l.y = dy * L.K.fy * zi;
-    l.z = -(l.x * p.x + l.y * p.y) * zi;
-    float r = ic - il;
-    const double sd = (double)stddev;
-    if (sd > 10e-5) r /= stddev;
-    float w = 1.0f;
-    if (fabsf(r) > th && sd > 10e-6) w = sqrtf(th / fabsf(r));
-    const f3 c = cross(p, l);
-    row[6] = w * (-r);
-    row[0] = l.x * w; row[1] = l.y * w; row[2] = l.z * w;
-    row[3] = c.x * w; row[4] = c.y * w; row[5] = c.z * w;
-    return true;
+    // ICP correspondence (pose_estimator.cu:168-193)
+    bool icp_ok = false;
+    size_t j = 0;
+    if (kIcp) {
+        const float invz = 1.0f / vg.z;
+        const int u = proj_floor(vg.x * invz * L.K.fx + L.K.cx + 0.5f);
+        const int v = proj_floor(vg.y * invz * L.K.fy + L.K.cy + 0.5f);
+        icp_ok = !isnan(lv.x) && u >= 0 && v >= 0 && u < L.cols && v < L.rows;
+        if (icp_ok) j = (size_t)v * L.cols + u;
+    }
+    // photometric correspondence (m_estimator.cu:315-338)
+    bool rgb_ok = false;
+    float u0 = 0.0f, v0 = 0.0f;
+    size_t t0 = 0;
+    if (kRgb) {
+        u0 = vg.x / vg.z * L.K.fx + L.K.cx;
+        v0 = vg.y / vg.z * L.K.fy + L.K.cy;
+        rgb_ok = !(lv.w < 0.0f || isnan(lv.x)) && isfinite(il) && u0 >= 2.0f && u0 < (float)(L.cols - 2) &&
+                 v0 >= 2.0f && v0 < (float)(L.rows - 2);
+        if (rgb_ok) t0 = (size_t)(int)floorf(v0) * L.cols + (int)floorf(u0);
+    }
+    // gathers
+    float4 cv = make_float4(0.f, 0.f, 0.f, 0.f), cn = cv;
+    if (kIcp) {
+        cv = reinterpret_cast<const float4 *>(L.curr_vmap)[j];
+        cn = reinterpret_cast<const float4 *>(L.curr_nmap)[j];
+    }
+    float tI[4], tx[4], ty[4];
+    if (kRgb) {
+        const size_t o[4] = {t0, t0 + 1, t0 + L.cols, t0 + L.cols + 1};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            tI[k] = L.curr_I[o[k]];
+            tx[k] = L.dIdx[o[k]];
+            ty[k] = L.dIdy[o[k]];
+        }
+    }
+    float row[7];
+    if (kIcp && icp_ok) {
+        const f3 vc = {cv.x, cv.y, cv.z};
+        const f3 ng = rotate(P, f3{ln.x, ln.y, ln.z});
+        const f3 nc = {cn.x, cn.y, cn.z};
+        const float dist = norm(sub(vg, vc));
+        const float sine = norm(cross(nc, ng));
+        if (sine < a.angle && dist <= a.dthr && !isnan(nc.x) && !isnan(ng.x)) {
+            const f3 c = cross(vg, ng);
+            row[0] = ng.x; row[1] = ng.y; row[2] = ng.z;
+            row[3] = c.x; row[4] = c.y; row[5] = c.z;
+            row[6] = dot(ng, sub(vc, vg));
+            accumulate(si, row);
+        }
+    }
+    if (kRgb && rgb_ok) {
+        const float cx = u0 - floorf(u0), cy = v0 - floorf(v0);
+        const float ic = (tI[0] * (1.0f - cx) + tI[1] * cx) * (1.0f - cy) + (tI[2] * (1.0f - cx) + tI[3] * cx) * cy;
+        const float dx = (tx[0] * (1.0f - cx) + tx[1] * cx) * (1.0f - cy) + (tx[2] * (1.0f - cx) + tx[3] * cx) * cy;
+        const float dy = (ty[0] * (1.0f - cx) + ty[1] * cx) * (1.0f - cy) + (ty[2] * (1.0f - cx) + ty[3] * cx) * cy;
+        if ((dx > 2.0f || dy > 2.0f) && isfinite(ic) && isfinite(dx) && isfinite(dy)) {
+            const float zi = 1.0f / vg.z;
+            f3 l;
+            l.x = dx * L.K.fx * zi;
+            l.y = dy * L.K.fy * zi;
+            l.z = -(l.x * vg.x + l.y * vg.y) * zi;
+            float r = ic - il;
+            const double sd = (double)stddev;
+            if (sd > 10e-5) r /= stddev;
+            float w = 1.0f;
+            if (fabsf(r) > th && sd > 10e-6) w = sqrtf(th / fabsf(r));
+            const f3 c = cross(vg, l);
+            row[6] = w * (-r);
+            row[0] = l.x * w; row[1] = l.y * w; row[2] = l.z * w;
+            row[3] = c.x * w; row[4] = c.y * w; row[5] = c.z * w;
+            accumulate(sr, row);
+        }
+    }
 }
 
 LF_DEV void accumulate(float *s, const float *row) {
@@ -303,20 +337,25 @@ LF_DEV void gn_step(LfTrackState &st, const float *si, const float *sr, int firs
     for (int k = 0; k < 12; ++k) st.pose_f[k] = (float)st.est[k];
 }
 
-// kMode 1: ICP sums -> out29; 2: RGB sums -> out29; 3: both + Gauss-Newton step.
+// Whether a mode-3 launch has work: the frame has not failed and the level has
+// not broken out of its loop (the first iteration of a level always runs).
+LF_DEV bool track_active(const ReduceArgs &a) {
+    const LfTrackState &st = *a.st;
+    if (st.invalid) return false;
+    return a.first_iter || st.active;
+}
+
+// Partial sums.  kMode 1: ICP rows, 2: RGB rows, 3: both.  Each workgroup
+// writes its kVals partials value-major (value v of workgroup g at v * G + g).
 template <int kMode>
 __global__ __launch_bounds__(kThreads) void k_reduce(ReduceArgs a) {
-    __shared__ float s_wave[4][kVals];
-    __shared__ int s_last;
-    __shared__ float s_part[4][64];
-    __shared__ float s_sum[kVals];
+    __shared__ float s_wave[kWaves][kVals];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     float P[12];
     float stddev = a.stddev;
     if (kMode == 3) {
+        if (!track_active(a)) return;
         const LfTrackState &st = *a.st;
-        if (st.invalid) return;                                  // the frame already failed
-        if (!a.first_iter && !st.active) return;                 // the level already broke out
 #pragma unroll
         for (int k = 0; k < 12; ++k) P[k] = st.pose_f[k];
         stddev = a.first_iter ? 0.0f : st.stddev;
@@ -331,12 +370,9 @@ __global__ __launch_bounds__(kThreads) void k_reduce(ReduceArgs a) {
 #pragma unroll
     for (int k = 0; k < 29; ++k) { si[k] = 0.0f; sr[k] = 0.0f; }
     const int n = a.lv.cols * a.lv.rows;
-    for (int i = blockIdx.x * kThreads + tid; i < n; i += gridDim.x * kThreads) {
-        float row[7];
-        if (kIcp && icp_row(a, P, i, row)) accumulate(si, row);
-        if (kRgb && rgb_row(a, P, stddev, th, i, row)) accumulate(sr, row);
-    }
-    // wavefront reduction
+    for (int i = blockIdx.x * kThreads + tid; i < n; i += gridDim.x * kThreads)
+        pixel_rows<kIcp, kRgb>(a, P, stddev, th, i, si, sr);
+    // wavefront reduction, then the workgroup's wavefronts in LDS
 #pragma unroll
     for (int k = 0; k < 29; ++k) {
         float vi = si[k], vr = sr[k];
@@ -351,29 +387,53 @@ __global__ __launch_bounds__(kThreads) void k_reduce(ReduceArgs a) {
         }
     }
     __syncthreads();
-    if (tid < kVals)
-        a.partials[(size_t)blockIdx.x * kVals + tid] = (s_wave[0][tid] + s_wave[1][tid]) + (s_wave[2][tid] + s_wave[3][tid]);
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) s_last = atomicAdd(a.ticket, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    // fixed-order final sum: part p of value v sums workgroups g = p, p+4, ...
-    {
-        const int v = tid & 63, p = tid >> 6;
-        float s = 0.0f;
-        if (v < kVals) {
-            const volatile float *part = a.partials;
-            for (int g = p; g < (int)gridDim.x; g += 4) s += part[(size_t)g * kVals + v];
+    if (tid < kVals) {
+        float t = 0.0f;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) t += s_wave[w][tid];
+        a.partials[(size_t)tid * gridDim.x + blockIdx.x] = t;
+    }
+}
+
+// Final sum of the G partial rows in a fixed order (deterministic): wavefront w
+// reduces values w, w+kWaves, ...; lane l adds partials l, l+64, ... of a value
+// (independent loads, in flight together), then a 64-lane tree.  Mode 3 then
+// runs the Gauss-Newton step on one lane.
+template <int kMode>
+__global__ __launch_bounds__(kThreads) void k_reduce_final(ReduceArgs a, int G) {
+    __shared__ float s_sum[kVals];
+    if (kMode == 3 && !track_active(a)) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int kPer = (kVals + kWaves - 1) / kWaves;
+    constexpr int kChunks = kGrid / 64;
+    float acc[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int v = wave + kWaves * j;
+        const float *src = a.partials + (size_t)(v < kVals ? v : 0) * G;
+        // every load of the value first (compile-time bound, predicated), then
+        // the fixed-order sum: the loads overlap instead of waiting in turn
+        float x[kChunks];
+#pragma unroll
+        for (int c = 0; c < kChunks; ++c) {
+            const int g = c * 64 + lane;
+            x[c] = (v < kVals && g < G) ? src[g] : 0.0f;
         }
-        s_part[p][v] = s;
+        float s = 0.0f;
+#pragma unroll
+        for (int c = 0; c < kChunks; ++c) s += x[c];
+        acc[j] = s;
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        float s = acc[j];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        const int v = wave + kWaves * j;
+        if (lane == 0 && v < kVals) s_sum[v] = s;
     }
     __syncthreads();
-    if (tid < kVals) s_sum[tid] = (s_part[0][tid] + s_part[1][tid]) + (s_part[2][tid] + s_part[3][tid]);
-    __syncthreads();
     if (tid == 0) {
-        *a.ticket = 0u;
         if (kMode == 1) for (int k = 0; k < 29; ++k) a.out29[k] = s_sum[k];
         if (kMode == 2) for (int k = 0; k < 29; ++k) a.out29[k] = s_sum[29 + k];
         if (kMode == 3) gn_step(*a.st, s_sum, s_sum + 29, a.first_iter, a.icp_weight);
@@ -428,7 +488,6 @@ __global__ void k_track_set_pose(LfTrackState *st, double t0, double t1, double 
         st->pose24[12 + k] = (float)Ti[k];
     }
     st->success = 1;
-    st->ticket = 0u;
 }
 
 int check_level(const LfTrackLevel &lv) {
@@ -437,6 +496,13 @@ int check_level(const LfTrackLevel &lv) {
 }
 
 float icp_angle() { return (float)cos(30 * 3.14 / 180); }
+
+// Workgroups of a reduction: one pixel per lane (every pixel of a level is in
+// flight at once), at most kGrid workgroups.
+int grid_for(const LfTrackLevel &lv) {
+    const long n = (long)lv.cols * lv.rows;
+    return (int)std::max(1L, std::min((long)kGrid, (n + kThreads - 1) / kThreads));
+}
 
 }  // namespace
 
@@ -458,11 +524,13 @@ int lf_track_iteration(const LfTrackLevel &lv, float *scratch, LfTrackState *st,
     a.angle = icp_angle();
     a.dthr = 0.01f;
     a.partials = scratch;
-    a.ticket = &st->ticket;
     a.st = st;
     a.first_iter = first_iter_of_level;
     a.icp_weight = icp_weight;
-    k_reduce<3><<<kGrid, kThreads, 0, stream>>>(a);
+    const int G = grid_for(lv);
+    k_reduce<3><<<G, kThreads, 0, stream>>>(a);
+    LF_LAUNCH_CHECK();
+    k_reduce_final<3><<<1, kThreads, 0, stream>>>(a, G);
     LF_LAUNCH_CHECK();
     return LF_OK;
 }
@@ -492,9 +560,11 @@ extern "C" int lf_icp_reduce(const float *curr_vmap, const float *curr_nmap, con
     a.angle = icp_angle();
     a.dthr = 0.01f;
     a.partials = scratch;
-    a.ticket = reinterpret_cast<unsigned int *>(scratch + (size_t)kGrid * kVals);
     a.out29 = sum29;
-    k_reduce<1><<<kGrid, kThreads, 0, lf_stream(stream)>>>(a);
+    const int G = grid_for(a.lv);
+    k_reduce<1><<<G, kThreads, 0, lf_stream(stream)>>>(a);
+    LF_LAUNCH_CHECK();
+    k_reduce_final<1><<<1, kThreads, 0, lf_stream(stream)>>>(a, G);
     LF_LAUNCH_CHECK();
     return LF_OK;
 }
@@ -514,9 +584,11 @@ extern "C" int lf_rgb_step(const float *curr_intensity, const float *last_intens
     a.pose12 = pose12;
     a.stddev = stddev_estimate;
     a.partials = scratch;
-    a.ticket = reinterpret_cast<unsigned int *>(scratch + (size_t)kGrid * kVals);
     a.out29 = sum29;
-    k_reduce<2><<<kGrid, kThreads, 0, lf_stream(stream)>>>(a);
+    const int G = grid_for(a.lv);
+    k_reduce<2><<<G, kThreads, 0, lf_stream(stream)>>>(a);
+    LF_LAUNCH_CHECK();
+    k_reduce_final<2><<<1, kThreads, 0, lf_stream(stream)>>>(a, G);
     LF_LAUNCH_CHECK();
     return LF_OK;
 }

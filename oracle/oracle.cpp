@@ -142,6 +142,9 @@ template <class R> struct Map {
     unsigned overflow = 0;
     // instrumentation for the bytes model
     std::vector<uint8_t> ray_touched;
+    // raycast instrumentation: samples, samples in unallocated blocks, trilinear
+    // reads, block changes along rays, rays, hits
+    long long ray_stats[6] = {0, 0, 0, 0, 0, 0};
 
     explicit Map(const Params &pp) : p(pp) { reset(); }
     static uint64_t key(I3 b) {
@@ -451,7 +454,10 @@ template <class R> void raycast(Map<R> &m, R *vmap, uint8_t *rgb_out, int cols, 
     // visible or no rendering block was made (voxel_hashing.cpp:92-93,105).
     if (m.visible.empty()) return;
     if (zrange_tiles(m, inv, K, tx, ty, zx, zy) == 0) return;
-    if (instrument) m.ray_touched.assign(m.pos.size(), 0);
+    if (instrument) {
+        m.ray_touched.assign(m.pos.size(), 0);
+        for (auto &v : m.ray_stats) v = 0;
+    }
     Ray<R> ray{m, instrument};
     const R inv_vs = (R)1 / (R)p.voxel_size;
     const R step_scale = (R)p.truncation * inv_vs;
@@ -480,10 +486,22 @@ template <class R> void raycast(Map<R> &m, R *vmap, uint8_t *rgb_out, int cols, 
             bool valid = false, found = false;
             R step;
             bool back_face = false;
+            I3 lastb = {kBad, kBad, kBad};
+            if (instrument) m.ray_stats[4]++;
             while (dist_s < dist_e) {
                 last_sdf = sdf;
                 sdf = ray.read_sdf(res, valid);
-                if (sdf <= (R)0.5 && sdf >= (R)-0.5) sdf = ray.read_interp(res, valid);
+                if (instrument) {
+                    I3 bb = voxel_to_block(to_i3(res));
+                    m.ray_stats[0]++;
+                    if (m.find(bb) < 0) m.ray_stats[1]++;
+                    if (bb.x != lastb.x || bb.y != lastb.y || bb.z != lastb.z) m.ray_stats[3]++;
+                    lastb = bb;
+                }
+                if (sdf <= (R)0.5 && sdf >= (R)-0.5) {
+                    sdf = ray.read_interp(res, valid);
+                    if (instrument) m.ray_stats[2]++;
+                }
                 if (sdf <= (R)0) break;
                 if (sdf >= (R)0 && last_sdf < (R)0) { back_face = true; break; }
                 if (valid) step = std::max(sdf * step_scale, (R)1);
@@ -511,6 +529,7 @@ template <class R> void raycast(Map<R> &m, R *vmap, uint8_t *rgb_out, int cols, 
                 const uint8_t *c = &m.rgb[(size_t)b * 1536 + (size_t)local_idx(v)];
                 col[0] = c[0]; col[1] = c[512]; col[2] = c[1024];
             }
+            if (instrument) m.ray_stats[5]++;
             V3<R> q = inv(mul(res, vs));
             out[0] = q.x; out[1] = q.y; out[2] = q.z; out[3] = 1;
             if (rgb_out) {
@@ -1161,6 +1180,9 @@ using namespace orc;
                                     const Intr *K, int check_colour, int instrument) {                        \
         raycast(*static_cast<Map<R> *>(m), vmap, rgb, cols, rows, m34_from(pose24), m34_from(pose24 + 12),     \
                 K_<R>(*K), check_colour, instrument != 0);                                                     \
+    }                                                                                                          \
+    extern "C" void orc_ray_stats_##S(void *m, long long *out6) {                                              \
+        for (int k = 0; k < 6; ++k) out6[k] = static_cast<Map<R> *>(m)->ray_stats[k];                          \
     }                                                                                                          \
     extern "C" int orc_ray_blocks_touched_##S(void *m) {                                                       \
         auto &M = *static_cast<Map<R> *>(m);                                                                   \

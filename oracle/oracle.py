@@ -69,6 +69,7 @@ def lib():
             sig("orc_update_weighted_%s", None, [P, P, P, P, I, I, P, P])
             sig("orc_raycast_%s", None, [P, P, P, I, I, P, P, I, I])
             sig("orc_ray_blocks_touched_%s", I, [P])
+            sig("orc_ray_stats_%s", None, [P, P])
             sig("orc_zrange_%s", None, [P, P, P, I, I, P, P])
             sig("orc_mesh_%s", ctypes.c_long, [P, I, P, P, P, ctypes.c_long])
             sig("orc_bilateral_%s", None, [P, P, I, I])
@@ -220,6 +221,14 @@ class Map:
         self._f("orc_raycast")(self.h, _p(vmap), _p(rgb), cols, rows, _p(pose24(T, self.dtype)),
                                ctypes.byref(K), 1 if check_colour else 0, 1 if instrument else 0)
         return (vmap, rgb) if want_rgb else vmap
+
+    def ray_stats(self):
+        """Counters of the last instrumented raycast: samples, samples in unallocated
+        blocks, trilinear reads, block changes along rays, rays started, hits."""
+        out = np.zeros(6, np.int64)
+        self._f("orc_ray_stats")(self.h, _p(out))
+        return dict(zip(("samples", "samples_unallocated", "trilinear", "block_changes", "rays", "hits"),
+                        out.tolist()))
 
     def ray_blocks_touched(self):
         return int(self._f("orc_ray_blocks_touched")(self.h))

@@ -64,6 +64,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-frames", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 256 MiB device-to-device (a plain copy kernel of known size, "
+                         "to scale rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the path runs one process per GPU with no "
@@ -75,6 +78,12 @@ def main():
     from libfusion_amd import capi, configs
 
     capi.load()
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     sc, K, params = configs.c1()
     n_total = args.warmup + args.steps
     if n_total > sc.num_frames:
@@ -120,8 +129,12 @@ def main():
     bytes_per_launch = n_vis * (2 * 512 * 9 + 20) + P * 15
     avg_s = stages["integrate_kernel"]["avg_ms"] * 1e-3
     achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "integrate_c1_traffic.json")
+    if os.path.exists(tf):  # HBM bytes per launch measured with rocprofv3 --pmc (see that file)
+        traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_integrate<true,true>", "algorithmic_bytes_per_launch": int(bytes_per_launch),
                 "avg_launch_ms": round(stages["integrate_kernel"]["avg_ms"], 5), "visible_blocks_avg": round(n_vis, 1)}
 

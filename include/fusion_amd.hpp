@@ -20,6 +20,7 @@
 #include "libfusion_amd.h"
 
 struct LfTrackState;
+struct LfImageLevels;
 
 namespace fusion {
 
@@ -82,6 +83,7 @@ public:
     int levels() const { return (int)K_.size(); }
 
 private:
+    LfImageLevels levels_view();
     std::vector<IntrinsicMatrix> K_;
     hipStream_t stream_;
     GpuImage depth_raw_, image_;

@@ -85,7 +85,8 @@ typedef struct lf_map_params {
  *   slot_block[num_entries]  pool block index of the slot, -1 = none
  *   slot_flag[num_entries]   1 if the slot's block was touched by the current update
  *   block_pos[4*num_blocks]  {x, y, z, slot} of every pool block in allocation order
- *   sdf[512*num_blocks]      int16 TSDF, value = sdf/32767 truncation units
+ *   sdf[512*num_blocks]      int16 TSDF, value = sdf/32767 truncation units;
+ *                            -32768 marks a voxel whose weight is 0
  *   weight[512*num_blocks]   float32 fusion weight, clamped to 255
  *   rgb[3*512*num_blocks]    uint8 colour, planar per block: R[512] G[512] B[512]
  *   visible[num_blocks]      pool block indices visible in the last update
@@ -206,6 +207,36 @@ int lf_back_project_depth(const float *depth, float *vmap, int cols, int rows,
                           const lf_intrinsics *K, void *stream);
 /* computeNMap (cuda_imgproc.cu:276-305); last row/column written as NaN. */
 int lf_compute_nmap(const float *vmap, float *nmap, int cols, int rows, void *stream);
+
+/* ------------------------------------------- frame pyramid (DeviceImage) */
+
+/* DeviceImage (device_image.h:18-88): one RGB-D frame's device pyramid.  Every
+ * level is exactly half of the one above (even sizes down to the last level). */
+typedef struct lf_image lf_image;
+
+enum {
+    LF_PLANE_DEPTH = 0,      /* float, metres, bilateral-filtered (get_depth)      */
+    LF_PLANE_INTENSITY = 1,  /* float grey (get_intensity)                         */
+    LF_PLANE_DX = 2,         /* float (get_intensity_dx)                           */
+    LF_PLANE_DY = 3,         /* float (get_intensity_dy)                           */
+    LF_PLANE_VMAP = 4,       /* float4 vertices (get_vmap)                         */
+    LF_PLANE_NMAP = 5,       /* float4 normals (get_nmap)                          */
+    LF_PLANE_RAW_DEPTH = 6,  /* float, metres, level 0 only (get_raw_depth)        */
+    LF_PLANE_RGB = 7         /* uint8 RGB copy of the input, level 0 (get_image)   */
+};
+
+/* DeviceImage(K, NUM_PYRS) (device_image.cpp:8-21); work is queued on stream. */
+int lf_image_create(const lf_intrinsics *K, int num_pyr, void *stream, lf_image **out);
+int lf_image_destroy(lf_image *img);
+/* upload (device_image.cpp:146-181) from device-resident depth (uint16) and
+ * RGB (uint8 x3): depth to metres, bilateral filter, intensity, the depth and
+ * intensity pyramids, and every level's derivatives, vertex and normal maps. */
+int lf_image_upload(lf_image *img, const uint16_t *depth, const unsigned char *rgb, float depth_scale);
+/* resize_device_map (device_image.cpp:188-199): vertex pyramid of the level-0
+ * vertex map (e.g. written by lf_raycast) and the normals of every level. */
+int lf_image_resize_device_map(lf_image *img);
+/* Device pointer of a plane (LF_PLANE_*) of a level; rows are packed. */
+int lf_image_plane(lf_image *img, int level, int plane, void **out);
 
 /* ------------------------------------------------------------- tracking */
 

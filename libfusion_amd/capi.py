@@ -17,6 +17,7 @@ LF_OK, LF_ERR_ARG, LF_ERR_HIP, LF_ERR_CAPACITY, LF_ERR_NOT_READY = 0, 1, 2, 3, 4
 CNT_ALLOCATED, CNT_VISIBLE, CNT_OVERFLOW, CNT_TRIANGLES = 0, 1, 2, 3
 CNT_VIS_SUM_LO, CNT_VIS_SUM_HI, CNT_UPDATES, CNT_RENDER_BLOCKS = 4, 5, 6, 7
 OVF_TABLE, OVF_POOL, OVF_TRIANGLES = 1, 2, 4
+SDF_UNSEEN = -32768  # sdf code of a voxel with weight 0
 
 
 class LfIntrinsics(ctypes.Structure):
@@ -74,6 +75,11 @@ SIGNATURES = [
     ("lf_compute_derivative", I, [P, P, P, I, I, P]),
     ("lf_back_project_depth", I, [P, P, I, I, P, P]),
     ("lf_compute_nmap", I, [P, P, I, I, P]),
+    ("lf_image_create", I, [P, I, P, P]),
+    ("lf_image_destroy", I, [P]),
+    ("lf_image_upload", I, [P, P, P, F]),
+    ("lf_image_resize_device_map", I, [P]),
+    ("lf_image_plane", I, [P, I, I, P]),
     ("lf_reduce_scratch_floats", ctypes.c_size_t, []),
     ("lf_icp_reduce", I, [P, P, P, P, I, I, P, P, P, P, P]),
     ("lf_rgb_step", I, [P, P, P, P, P, I, I, F, P, P, P, P, P]),
@@ -172,6 +178,81 @@ def ptr(t):
     return ctypes.c_void_p(0 if t is None else t.data_ptr())
 
 
+PLANE_DEPTH, PLANE_INTENSITY, PLANE_DX, PLANE_DY, PLANE_VMAP, PLANE_NMAP, PLANE_RAW_DEPTH, PLANE_RGB = range(8)
+
+_hip = None
+
+
+def copy_device(dst, src_ptr, nbytes):
+    """Synchronous device-to-device copy from a raw device pointer into a tensor."""
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.restype = ctypes.c_int
+        _hip.hipMemcpy.argtypes = [P, P, ctypes.c_size_t, ctypes.c_int]
+        _hip.hipDeviceSynchronize.restype = ctypes.c_int
+    if dst is None:
+        return
+    assert dst.is_contiguous() and dst.numel() * dst.element_size() >= nbytes
+    if _hip.hipDeviceSynchronize() != 0 or _hip.hipMemcpy(ptr(dst), ctypes.c_void_p(src_ptr), nbytes, 3) != 0:
+        raise LfError("hipMemcpy failed")
+
+
+def copy_device_to(dst_ptr, src):
+    """Synchronous device-to-device copy from a tensor to a raw device pointer."""
+    copy_device(None, 0, 0)  # loads the runtime
+    assert src.is_contiguous()
+    if _hip.hipDeviceSynchronize() != 0 or _hip.hipMemcpy(ctypes.c_void_p(dst_ptr), ptr(src),
+                                                         src.numel() * src.element_size(), 3) != 0:
+        raise LfError("hipMemcpy failed")
+
+
+class Image:
+    """DeviceImage: one RGB-D frame's device pyramid (lf_image_*), on the current stream."""
+
+    def __init__(self, K, levels):
+        self.K = [K]
+        for _ in range(levels - 1):
+            self.K.append(pyr_down(self.K[-1]))
+        self.h = ctypes.c_void_p()
+        check(load().lf_image_create(ctypes.byref(K), int(levels), stream_ptr(), ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            load().lf_image_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, depth, rgb, depth_scale):
+        check(load().lf_image_upload(self.h, ptr(depth), ptr(rgb), float(np.float32(depth_scale))))
+
+    def resize_device_map(self):
+        check(load().lf_image_resize_device_map(self.h))
+
+    def plane_ptr(self, level, plane):
+        out = ctypes.c_void_p()
+        check(load().lf_image_plane(self.h, int(level), int(plane), ctypes.byref(out)))
+        return out.value
+
+    def plane(self, level, plane):
+        """A copy of a plane as a tensor ([h, w], [h, w, 4] for maps, [h, w, 3] uint8 for colour)."""
+        torch = _torch()
+        w, h = self.K[level].width, self.K[level].height
+        if plane in (PLANE_VMAP, PLANE_NMAP):
+            t = torch.empty((h, w, 4), device="cuda")
+        elif plane == PLANE_RGB:
+            t = torch.empty((h, w, 3), dtype=torch.uint8, device="cuda")
+        else:
+            t = torch.empty((h, w), device="cuda")
+        copy_device(t, self.plane_ptr(level, plane), t.numel() * t.element_size())
+        return t
+
+
 class Map:
     """A device voxel-hash map whose arrays are torch tensors (layout of lf_map_storage)."""
 
@@ -249,11 +330,14 @@ class Map:
         return self.t["block_pos"].view(-1, 4)[idx, :3].cpu().numpy()
 
     def voxels(self):
-        """(blocks[n,3], sdf int16[n,512], weight[n,512], rgb uint8[n,3,512]) in pool order."""
+        """(blocks[n,3], sdf int16[n,512], weight[n,512], rgb uint8[n,3,512]) in pool order.
+        The sdf of a voxel with weight 0 is undefined on the device (it holds the
+        unseen code SDF_UNSEEN); it is returned as 0."""
         c = self.counters()
         n = min(int(c[CNT_ALLOCATED]), self.params.num_blocks)
-        return (self.blocks(), self.t["sdf"][: 512 * n].view(n, 512).cpu().numpy(),
-                self.t["weight"][: 512 * n].view(n, 512).cpu().numpy(),
+        sdf = self.t["sdf"][: 512 * n].view(n, 512).cpu().numpy()
+        sdf = np.where(sdf == SDF_UNSEEN, np.int16(0), sdf)
+        return (self.blocks(), sdf, self.t["weight"][: 512 * n].view(n, 512).cpu().numpy(),
                 self.t["rgb"][: 1536 * n].view(n, 3, 512).cpu().numpy())
 
 

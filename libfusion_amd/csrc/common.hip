@@ -113,9 +113,9 @@ __global__ void k_zero_u32(unsigned int *p, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
 }
 
-__global__ void k_zero_u128(uint4 *p, size_t n) {
+__global__ void k_fill_u128(uint4 *p, size_t n, unsigned int v) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        p[i] = make_uint4(0, 0, 0, 0);
+        p[i] = make_uint4(v, v, v, v);
 }
 
 // MapStruct::reset (map_struct.cu:74-93).  Zero-fill kernels rather than
@@ -130,12 +130,15 @@ extern "C" int lf_map_reset(const lf_map_params *p, const lf_map_storage *s, voi
     k_zero_u32<<<1, 64, 0, st>>>(s->counters, LF_NUM_COUNTERS);
     LF_LAUNCH_CHECK();
     const size_t B = (size_t)p->num_blocks;
-    // sdf (2 B), weight (4 B), rgb (3 B) per voxel; every array is a multiple of 512 bytes.
-    struct { void *ptr; size_t bytes; } arrs[] = {{s->sdf, B * 1024}, {s->weight, B * 2048}, {s->rgb, B * 1536}};
+    // sdf (2 B, every voxel starts unseen), weight (4 B, 0), rgb (3 B, 0) per voxel;
+    // every array is a multiple of 512 bytes.
+    const unsigned int unseen2 = 0x80008000u;  // two int16 kSdfUnseen codes
+    struct { void *ptr; size_t bytes; unsigned int v; } arrs[] = {
+        {s->sdf, B * 1024, unseen2}, {s->weight, B * 2048, 0u}, {s->rgb, B * 1536, 0u}};
     for (auto &a : arrs) {
         size_t n16 = a.bytes / 16;
         int grid = (int)std::min<size_t>(8192, (n16 + 255) / 256);
-        k_zero_u128<<<grid, 256, 0, st>>>(reinterpret_cast<uint4 *>(a.ptr), n16);
+        k_fill_u128<<<grid, 256, 0, st>>>(reinterpret_cast<uint4 *>(a.ptr), n16, a.v);
         LF_LAUNCH_CHECK();
     }
     return LF_OK;

@@ -10,6 +10,7 @@
 // All kernels are one lane per output pixel on 16x16 workgroups (4 wavefronts of
 // 16x4 pixels): every row segment a wavefront reads or writes is contiguous.
 #include "lf_common.h"
+#include "lf_internal.h"
 
 using namespace lf;
 
@@ -21,54 +22,46 @@ LF_DEV int reflect101(int i, int n) {
     return i;
 }
 
-__global__ void k_depth_to_float(const uint16_t *__restrict__ in, float *__restrict__ out, int n, float scale) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        out[i] = (float)in[i] * scale;
-}
+LF_DEV float depth_value(uint16_t d, float scale) { return (float)d * scale; }
 
 // cv::cuda::bilateralFilter(src, dst, 5, 1, 1): 5x5 window, reflect-101 border,
 // w = exp(-(dx^2+dy^2)/(2*1^2) - (v-c)^2/(2*1^2)); zero depths are not masked.
-__global__ void k_bilateral(const float *__restrict__ in, float *__restrict__ out, int w, int h) {
-    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (x >= w || y >= h) return;
-    const float c = in[(size_t)y * w + x];
+// at(x, y) returns the source value of an in-range pixel.
+template <class At>
+LF_DEV float bilateral_at(const At &at, int x, int y, int w, int h) {
+    const float c = at(x, y);
     float s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
     for (int dy = -2; dy <= 2; ++dy) {
-        const float *row = in + (size_t)reflect101(y + dy, h) * w;
+        const int yy = reflect101(y + dy, h);
 #pragma unroll
         for (int dx = -2; dx <= 2; ++dx) {
-            const float v = row[reflect101(x + dx, w)];
+            const float v = at(reflect101(x + dx, w), yy);
             const float d = v - c;
             const float wt = expf((float)(dx * dx + dy * dy) * -0.5f + d * d * -0.5f);
             s1 += v * wt;
             s2 += wt;
         }
     }
-    out[(size_t)y * w + x] = s1 / s2;
+    return s1 / s2;
 }
 
 // cv::cuda::resize(x0.5, INTER_LINEAR) with the pixel-centre convention: the
-// mean of the 2x2 source block, NaN propagating.  ch = 1 (depth) or 4 (vmap).
-template <int kCh>
-__global__ void k_resize_half(const float *__restrict__ in, float *__restrict__ out, int w, int h) {
-    const int ow = w / 2, oh = h / 2;
-    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (x >= ow || y >= oh) return;
-#pragma unroll
-    for (int k = 0; k < kCh; ++k) {
-        const float a = in[((size_t)(2 * y) * w + 2 * x) * kCh + k], b = in[((size_t)(2 * y) * w + 2 * x + 1) * kCh + k];
-        const float c = in[((size_t)(2 * y + 1) * w + 2 * x) * kCh + k],
-                    d = in[((size_t)(2 * y + 1) * w + 2 * x + 1) * kCh + k];
-        out[((size_t)y * ow + x) * kCh + k] = ((a + b) + (c + d)) * 0.25f;
-    }
+// mean of the 2x2 source block, NaN propagating.
+LF_DEV float mean4(float a, float b, float c, float d) { return ((a + b) + (c + d)) * 0.25f; }
+LF_DEV float resize_at(const float *in, int w, int x, int y) {
+    const float *r0 = in + (size_t)(2 * y) * w + 2 * x, *r1 = r0 + w;
+    return mean4(r0[0], r0[1], r1[0], r1[1]);
+}
+LF_DEV float4 resize4_at(const float4 *in, int w, int x, int y) {
+    const float4 *r0 = in + (size_t)(2 * y) * w + 2 * x, *r1 = r0 + w;
+    const float4 a = r0[0], b = r0[1], c = r1[0], d = r1[1];
+    return make_float4(mean4(a.x, b.x, c.x, d.x), mean4(a.y, b.y, c.y, d.y), mean4(a.z, b.z, c.z, d.z),
+                       mean4(a.w, b.w, c.w, d.w));
 }
 
 // cv::cuda::pyrDown: [1 4 6 4 1]^2/256 Gaussian at even pixels, reflect-101.
-__global__ void k_pyr_down(const float *__restrict__ in, float *__restrict__ out, int w, int h) {
-    const int ow = (w + 1) / 2, oh = (h + 1) / 2;
-    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (x >= ow || y >= oh) return;
+LF_DEV float pyr_down_at(const float *in, int w, int h, int x, int y) {
     const int x0 = reflect101(2 * x - 2, w), x1 = reflect101(2 * x - 1, w), x2 = 2 * x, x3 = reflect101(2 * x + 1, w),
               x4 = reflect101(2 * x + 2, w);
     float rs[5];
@@ -77,58 +70,194 @@ __global__ void k_pyr_down(const float *__restrict__ in, float *__restrict__ out
         const float *row = in + (size_t)reflect101(2 * y + j - 2, h) * w;
         rs[j] = (row[x0] + row[x4]) + 4.0f * (row[x1] + row[x3]) + 6.0f * row[x2];
     }
-    out[(size_t)y * ow + x] = ((rs[0] + rs[4]) + 4.0f * (rs[1] + rs[3]) + 6.0f * rs[2]) * (1.0f / 256.0f);
+    return ((rs[0] + rs[4]) + 4.0f * (rs[1] + rs[3]) + 6.0f * rs[2]) * (1.0f / 256.0f);
 }
 
 // cvtColor(RGB2GRAY) on float: 0.299 R + 0.587 G + 0.114 B.
-__global__ void k_grey(const unsigned char *__restrict__ rgb, float *__restrict__ out, int n) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        out[i] = (float)rgb[3 * i] * 0.299f + (float)rgb[3 * i + 1] * 0.587f + (float)rgb[3 * i + 2] * 0.114f;
+LF_DEV float grey_of(unsigned char r, unsigned char g, unsigned char b) {
+    return (float)r * 0.299f + (float)g * 0.587f + (float)b * 0.114f;
 }
 
 // computeDerivativeK (cuda_imgproc.cu:217-234); last row / column 0.
+LF_DEV void derivative_at(const float *I, int w, int h, int x, int y, float &dx, float &dy) {
+    if (x >= w - 1 || y >= h - 1) {
+        dx = 0.0f;
+        dy = 0.0f;
+        return;
+    }
+    const int x10 = max(x - 1, 0), x01 = min(x + 1, w), y10 = max(y - 1, 0), y01 = min(y + 1, h);
+    dx = (I[(size_t)y * w + x01] - I[(size_t)y * w + x10]) * 0.5f;
+    dy = (I[(size_t)y01 * w + x] - I[(size_t)y10 * w + x]) * 0.5f;
+}
+
+// backProjectDepthK (cuda_imgproc.cu:249-263).
+LF_DEV float4 back_project_at(float z, int x, int y, const lf_intrinsics &K) {
+    const float qnan = __int_as_float(0x7fc00000);
+    if (z > 0.3f && z < 5.0f) return make_float4(z * ((float)x - K.cx) * K.invfx, z * ((float)y - K.cy) * K.invfy, z, 1.0f);
+    return make_float4(qnan, qnan, qnan, -1.0f);
+}
+
+// computeNMapK (cuda_imgproc.cu:276-294); last row / column NaN.  V(x, y)
+// returns the vertex of an in-range pixel.
+template <class V>
+LF_DEV float4 nmap_at(const V &vert, int w, int h, int x, int y) {
+    const float qnan = __int_as_float(0x7fc00000);
+    if (x >= w - 1 || y >= h - 1) return make_float4(qnan, qnan, qnan, qnan);
+    const int x10 = max(x - 1, 0), x01 = min(x + 1, w), y10 = max(y - 1, 0), y01 = min(y + 1, h);
+    const float4 a = vert(x01, y), b = vert(x10, y), c = vert(x, y01), e = vert(x, y10);
+    const f3 n = normalised(cross(f3{a.x - b.x, a.y - b.y, a.z - b.z}, f3{c.x - e.x, c.y - e.y, c.z - e.z}));
+    return make_float4(n.x, n.y, n.z, 1.0f);
+}
+
+struct FloatAt {
+    const float *p; int w;
+    LF_DEV float operator()(int x, int y) const { return p[(size_t)y * w + x]; }
+};
+struct DepthU16At {
+    const uint16_t *p; int w; float scale;
+    LF_DEV float operator()(int x, int y) const { return depth_value(p[(size_t)y * w + x], scale); }
+};
+struct VertexAt {
+    const float4 *p; int w;
+    LF_DEV float4 operator()(int x, int y) const { return p[(size_t)y * w + x]; }
+};
+// vertices back-projected on the fly from a depth image (same arithmetic as
+// a stored vertex map)
+struct DepthVertexAt {
+    const float *d; int w; lf_intrinsics K;
+    LF_DEV float4 operator()(int x, int y) const { return back_project_at(d[(size_t)y * w + x], x, y, K); }
+};
+// vertex (x, y) of the quarter-size map, through the four half-size vertices
+// resized on the fly from the full-size map of width w (same arithmetic as
+// two stored resizes)
+LF_DEV float4 resize4_quarter_at(const float4 *full, int w, int x, int y) {
+    const float4 a = resize4_at(full, w, 2 * x, 2 * y), b = resize4_at(full, w, 2 * x + 1, 2 * y),
+                 c = resize4_at(full, w, 2 * x, 2 * y + 1), d = resize4_at(full, w, 2 * x + 1, 2 * y + 1);
+    return make_float4(mean4(a.x, b.x, c.x, d.x), mean4(a.y, b.y, c.y, d.y), mean4(a.z, b.z, c.z, d.z),
+                       mean4(a.w, b.w, c.w, d.w));
+}
+
+// ------------------------------------------------------------ stand-alone kernels
+
+__global__ void k_depth_to_float(const uint16_t *__restrict__ in, float *__restrict__ out, int n, float scale) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = depth_value(in[i], scale);
+}
+
+__global__ void k_bilateral(const float *__restrict__ in, float *__restrict__ out, int w, int h) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= w || y >= h) return;
+    out[(size_t)y * w + x] = bilateral_at(FloatAt{in, w}, x, y, w, h);
+}
+
+// ch = 1 (depth) or 4 (vmap).
+template <int kCh>
+__global__ void k_resize_half(const float *__restrict__ in, float *__restrict__ out, int w, int h) {
+    const int ow = w / 2, oh = h / 2;
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= ow || y >= oh) return;
+    if (kCh == 1) out[(size_t)y * ow + x] = resize_at(in, w, x, y);
+    else reinterpret_cast<float4 *>(out)[(size_t)y * ow + x] = resize4_at(reinterpret_cast<const float4 *>(in), w, x, y);
+}
+
+__global__ void k_pyr_down(const float *__restrict__ in, float *__restrict__ out, int w, int h) {
+    const int ow = (w + 1) / 2, oh = (h + 1) / 2;
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= ow || y >= oh) return;
+    out[(size_t)y * ow + x] = pyr_down_at(in, w, h, x, y);
+}
+
+__global__ void k_grey(const unsigned char *__restrict__ rgb, float *__restrict__ out, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = grey_of(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]);
+}
+
 __global__ void k_derivative(const float *__restrict__ I, float *__restrict__ dx, float *__restrict__ dy, int w, int h) {
     const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= w || y >= h) return;
     const size_t i = (size_t)y * w + x;
-    if (x >= w - 1 || y >= h - 1) {
-        dx[i] = 0.0f;
-        dy[i] = 0.0f;
-        return;
-    }
-    const int x10 = max(x - 1, 0), x01 = min(x + 1, w), y10 = max(y - 1, 0), y01 = min(y + 1, h);
-    dx[i] = (I[(size_t)y * w + x01] - I[(size_t)y * w + x10]) * 0.5f;
-    dy[i] = (I[(size_t)y01 * w + x] - I[(size_t)y10 * w + x]) * 0.5f;
+    derivative_at(I, w, h, x, y, dx[i], dy[i]);
 }
 
-// backProjectDepthK (cuda_imgproc.cu:249-263).
 __global__ void k_back_project(const float *__restrict__ d, float *__restrict__ vm, int w, int h, lf_intrinsics K) {
     const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= w || y >= h) return;
     const size_t i = (size_t)y * w + x;
-    const float z = d[i];
-    const float qnan = __int_as_float(0x7fc00000);
-    float4 o = make_float4(qnan, qnan, qnan, -1.0f);
-    if (z > 0.3f && z < 5.0f) o = make_float4(z * ((float)x - K.cx) * K.invfx, z * ((float)y - K.cy) * K.invfy, z, 1.0f);
-    reinterpret_cast<float4 *>(vm)[i] = o;
+    reinterpret_cast<float4 *>(vm)[i] = back_project_at(d[i], x, y, K);
 }
 
-// computeNMapK (cuda_imgproc.cu:276-294); last row / column NaN.
 __global__ void k_nmap(const float *__restrict__ vm, float *__restrict__ nm, int w, int h) {
     const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= w || y >= h) return;
+    reinterpret_cast<float4 *>(nm)[(size_t)y * w + x] = nmap_at(VertexAt{reinterpret_cast<const float4 *>(vm), w}, w, h, x, y);
+}
+
+// ------------------------------------------------------------ fused frame passes
+//
+// A pass is up to kMaxParts independent parts in one launch; each part tiles
+// its own image with 16x16 workgroups, one lane per output pixel, and the
+// workgroups of the parts follow each other in the 1-D grid.
+
+constexpr int kMaxParts = 4;
+enum PartKind { kFrameIn, kPyramid, kLevelMaps, kNmap, kVmapDown, kVmapDown2 };
+
+struct Part {
+    int kind, w, h, wg_begin;  // w, h: the part's output size
+    const void *in0, *in1;
+    void *out0, *out1, *out2, *out3;
+    lf_intrinsics K;
+    float scale;
+};
+struct Pass {
+    Part part[kMaxParts];
+    int nparts;
+};
+
+__global__ __launch_bounds__(256) void k_frame_pass(Pass ps) {
+    int pi = 0;
+#pragma unroll
+    for (int k = 1; k < kMaxParts; ++k)
+        if (k < ps.nparts && (int)blockIdx.x >= ps.part[k].wg_begin) pi = k;
+    const Part &p = ps.part[pi];
+    const int t = blockIdx.x - p.wg_begin, tiles_x = (p.w + 15) >> 4;
+    const int x = (t % tiles_x) * 16 + (threadIdx.x & 15), y = (t / tiles_x) * 16 + (threadIdx.x >> 4);
+    if (x >= p.w || y >= p.h) return;
+    const int w = p.w, h = p.h;
     const size_t i = (size_t)y * w + x;
-    const float qnan = __int_as_float(0x7fc00000);
-    if (x >= w - 1 || y >= h - 1) {
-        reinterpret_cast<float4 *>(nm)[i] = make_float4(qnan, qnan, qnan, qnan);
-        return;
+    switch (p.kind) {
+    case kFrameIn: {  // raw depth, bilateral depth, intensity and the colour copy of a frame
+        const uint16_t *d16 = static_cast<const uint16_t *>(p.in0);
+        const unsigned char *rgb = static_cast<const unsigned char *>(p.in1);
+        static_cast<float *>(p.out0)[i] = depth_value(d16[i], p.scale);
+        static_cast<float *>(p.out1)[i] = bilateral_at(DepthU16At{d16, w, p.scale}, x, y, w, h);
+        const unsigned char r = rgb[3 * i], g = rgb[3 * i + 1], b = rgb[3 * i + 2];
+        static_cast<float *>(p.out2)[i] = grey_of(r, g, b);
+        unsigned char *c = static_cast<unsigned char *>(p.out3) + 3 * i;
+        c[0] = r; c[1] = g; c[2] = b;
+        break;
     }
-    const float4 *V = reinterpret_cast<const float4 *>(vm);
-    const int x10 = max(x - 1, 0), x01 = min(x + 1, w), y10 = max(y - 1, 0), y01 = min(y + 1, h);
-    const float4 a = V[(size_t)y * w + x01], b = V[(size_t)y * w + x10], c = V[(size_t)y01 * w + x],
-                 e = V[(size_t)y10 * w + x];
-    const f3 n = normalised(cross(f3{a.x - b.x, a.y - b.y, a.z - b.z}, f3{c.x - e.x, c.y - e.y, c.z - e.z}));
-    reinterpret_cast<float4 *>(nm)[i] = make_float4(n.x, n.y, n.z, 1.0f);
+    case kPyramid:  // next level of the depth (resize) and intensity (pyrDown) pyramids
+        static_cast<float *>(p.out0)[i] = resize_at(static_cast<const float *>(p.in0), 2 * w, x, y);
+        static_cast<float *>(p.out1)[i] = pyr_down_at(static_cast<const float *>(p.in1), 2 * w, 2 * h, x, y);
+        break;
+    case kLevelMaps: {  // intensity derivatives, vertex and normal maps of a level
+        const float *d = static_cast<const float *>(p.in0);
+        derivative_at(static_cast<const float *>(p.in1), w, h, x, y, static_cast<float *>(p.out0)[i],
+                      static_cast<float *>(p.out1)[i]);
+        static_cast<float4 *>(p.out2)[i] = back_project_at(d[i], x, y, p.K);
+        static_cast<float4 *>(p.out3)[i] = nmap_at(DepthVertexAt{d, w, p.K}, w, h, x, y);
+        break;
+    }
+    case kNmap:  // normal map of a stored vertex map
+        static_cast<float4 *>(p.out0)[i] = nmap_at(VertexAt{static_cast<const float4 *>(p.in0), w}, w, h, x, y);
+        break;
+    case kVmapDown:  // half-size vertex map
+        static_cast<float4 *>(p.out0)[i] = resize4_at(static_cast<const float4 *>(p.in0), 2 * w, x, y);
+        break;
+    case kVmapDown2:  // quarter-size vertex map, through the half-size one
+        static_cast<float4 *>(p.out0)[i] = resize4_quarter_at(static_cast<const float4 *>(p.in0), 4 * w, x, y);
+        break;
+    }
 }
 
 dim3 grid16(int w, int h) { return dim3((w + 15) / 16, (h + 15) / 16); }
@@ -139,7 +268,96 @@ int check_img(const void *a, const void *b, int w, int h) {
     return LF_OK;
 }
 
+Part make_part(int kind, int w, int h) {
+    Part p = {};
+    p.kind = kind;
+    p.w = w;
+    p.h = h;
+    return p;
+}
+
+int launch_pass(Pass &ps, hipStream_t st) {
+    int wg = 0;
+    for (int k = 0; k < ps.nparts; ++k) {
+        ps.part[k].wg_begin = wg;
+        wg += ((ps.part[k].w + 15) / 16) * ((ps.part[k].h + 15) / 16);
+    }
+    if (wg == 0) return LF_OK;
+    k_frame_pass<<<wg, 256, 0, st>>>(ps);
+    LF_LAUNCH_CHECK();
+    return LF_OK;
+}
+
+int check_pyramid(int levels, const lf_intrinsics *K) {
+    if (levels < 1 || levels > 8 || !K) return lf_set_error(LF_ERR_ARG, "bad pyramid (%d levels)", levels);
+    for (int l = 0; l < levels; ++l) {
+        if (K[l].width <= 0 || K[l].height <= 0) return lf_set_error(LF_ERR_ARG, "bad level %d size", l);
+        if (l > 0 && (K[l].width * 2 != K[l - 1].width || K[l].height * 2 != K[l - 1].height))
+            return lf_set_error(LF_ERR_ARG, "level %d is not half of level %d (%dx%d, %dx%d)", l, l - 1, K[l].width,
+                                K[l].height, K[l - 1].width, K[l - 1].height);
+    }
+    return LF_OK;
+}
+
 }  // namespace
+
+// DeviceImage::upload as levels + 1 launches: the frame pass (raw depth,
+// bilateral depth, intensity, colour copy), then per level its maps
+// (derivatives, vertices, normals) together with the next level's pyramids.
+int lf_pyramid_upload(const uint16_t *depth_u16, const unsigned char *rgb, float depth_scale, int levels,
+                    const lf_intrinsics *K, const LfImageLevels &m, hipStream_t st) {
+    if (int rc = check_pyramid(levels, K)) return rc;
+    if (!depth_u16 || !rgb) return lf_set_error(LF_ERR_ARG, "null frame input");
+    Pass ps = {};
+    Part f = make_part(kFrameIn, K[0].width, K[0].height);
+    f.in0 = depth_u16; f.in1 = rgb; f.scale = depth_scale;
+    f.out0 = m.depth_raw; f.out1 = m.depth[0]; f.out2 = m.intensity[0]; f.out3 = m.rgb;
+    ps.part[ps.nparts++] = f;
+    if (int rc = launch_pass(ps, st)) return rc;
+    for (int l = 0; l < levels; ++l) {
+        ps.nparts = 0;
+        Part p = make_part(kLevelMaps, K[l].width, K[l].height);
+        p.in0 = m.depth[l]; p.in1 = m.intensity[l]; p.K = K[l];
+        p.out0 = m.dx[l]; p.out1 = m.dy[l]; p.out2 = m.vmap[l]; p.out3 = m.nmap[l];
+        ps.part[ps.nparts++] = p;
+        if (l + 1 < levels) {
+            Part q = make_part(kPyramid, K[l + 1].width, K[l + 1].height);
+            q.in0 = m.depth[l]; q.in1 = m.intensity[l]; q.out0 = m.depth[l + 1]; q.out1 = m.intensity[l + 1];
+            ps.part[ps.nparts++] = q;
+        }
+        if (int rc = launch_pass(ps, st)) return rc;
+    }
+    return LF_OK;
+}
+
+// DeviceImage::resize_device_map: the vertex pyramid of a raycast level-0 map
+// and every level's normals, two levels per launch (a launch with base level
+// b writes the normals still missing below b + 1 and the vertices of levels
+// b + 1 and b + 2, the latter resized twice on the fly).
+int lf_pyramid_refresh(int levels, const lf_intrinsics *K, const LfImageLevels &m, hipStream_t st) {
+    if (int rc = check_pyramid(levels, K)) return rc;
+    int nmap_done = 0;
+    for (int b = 0; nmap_done < levels; b += 2) {
+        Pass ps = {};
+        for (; nmap_done <= b && nmap_done < levels; ++nmap_done) {
+            Part p = make_part(kNmap, K[nmap_done].width, K[nmap_done].height);
+            p.in0 = m.vmap[nmap_done]; p.out0 = m.nmap[nmap_done];
+            ps.part[ps.nparts++] = p;
+        }
+        if (b + 1 < levels) {
+            Part p = make_part(kVmapDown, K[b + 1].width, K[b + 1].height);
+            p.in0 = m.vmap[b]; p.out0 = m.vmap[b + 1];
+            ps.part[ps.nparts++] = p;
+        }
+        if (b + 2 < levels) {
+            Part p = make_part(kVmapDown2, K[b + 2].width, K[b + 2].height);
+            p.in0 = m.vmap[b]; p.out0 = m.vmap[b + 2];
+            ps.part[ps.nparts++] = p;
+        }
+        if (int rc = launch_pass(ps, st)) return rc;
+    }
+    return LF_OK;
+}
 
 extern "C" int lf_depth_to_float(const uint16_t *in, float *out, int cols, int rows, float scale, void *stream) {
     if (int rc = check_img(in, out, cols, rows)) return rc;

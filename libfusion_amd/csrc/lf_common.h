@@ -74,6 +74,11 @@ LF_HD unsigned long long pack_key(int x, int y, int z) {
            (unsigned long long)(z & 0x1FFFFF);
 }
 
+// sdf code of a voxel whose weight is 0 (never observed, or only with weight 0).
+// packFloat never produces it (|v| <= 1 maps into [-32767, 32767]), so the
+// raycast decides validity from the 2-byte sdf alone, without the weight.
+constexpr short kSdfUnseen = -32768;
+
 LF_HD float unpack_sdf(short v) { return (float)v / 32767.0f; }
 LF_HD short pack_sdf(float v) { return (short)(v * 32767.0f); }
 

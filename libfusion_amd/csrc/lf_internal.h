@@ -16,3 +16,19 @@ int lf_update_ev(const lf_map_params *p, const lf_map_storage *s, const float *d
 int lf_raycast_ev(const lf_map_params *p, const lf_map_storage *s, float *vmap, unsigned char *rgb_out, int cols,
                   int rows, const float *pose24, const lf_intrinsics *K, int check_colour, hipStream_t st,
                   hipEvent_t e0, hipEvent_t e1);
+
+// Device buffers of a DeviceImage: per-level pointers (up to 8 levels).
+struct LfImageLevels {
+    float *depth_raw;                    // level 0, metres
+    unsigned char *rgb;                  // level 0 colour copy
+    float *depth[8], *intensity[8], *dx[8], *dy[8];
+    float *vmap[8], *nmap[8];            // float4 per pixel
+};
+
+// DeviceImage::upload (device_image.cpp:146-181) in levels + 1 launches.  Every
+// level must be exactly half of the one above (K[l] of each level).
+int lf_pyramid_upload(const uint16_t *depth_u16, const unsigned char *rgb, float depth_scale, int levels,
+                    const lf_intrinsics *K, const LfImageLevels &m, hipStream_t st);
+// DeviceImage::resize_device_map (device_image.cpp:188-199): vertex pyramid of
+// vmap[0] and the normals of every level, in (levels + 1) / 2 launches.
+int lf_pyramid_refresh(int levels, const lf_intrinsics *K, const LfImageLevels &m, hipStream_t st);

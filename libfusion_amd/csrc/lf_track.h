@@ -33,18 +33,29 @@ struct LfTrackLevel {
     lf_intrinsics K;
 };
 
-// Scratch floats needed by the reductions.
+// Scratch floats needed by the reductions (partial rows of the workgroups).
 size_t lf_reduce_partials(void);
 
+// Schedule position of one Gauss-Newton step within a tracked frame.
+struct LfStepFlags {
+    int first_of_level;  // first iteration of a pyramid level (resets the error history)
+    int first_of_frame;  // the estimate starts at init (replaces lf_track_begin)
+    int last_level;      // a step of the finest level: breaking out of its loop ends the frame
+    int last_of_frame;   // the frame's last scheduled step: finish after it
+    double init[12];     // initial estimate, 3x4 row-major (first_of_frame)
+};
+
 // One Gauss-Newton iteration of DenseTracking::compute_transform
-// (icp_tracker.cpp:50-140): fused ICP + RGB reduction, FP64 solve, SE3 update,
-// divergence checks — entirely on the device.
-int lf_track_iteration(const LfTrackLevel &lv, float *scratch, LfTrackState *st, int first_iter_of_level,
+// (icp_tracker.cpp:50-140) in two launches: fused ICP + RGB reduction, FP64
+// solve, SE3 update, divergence checks and, when the frame ends at this step,
+// the finish of lf_track_finish — entirely on the device.
+int lf_track_iteration(const LfTrackLevel &lv, float *scratch, LfTrackState *st, const LfStepFlags &f,
                        float icp_weight, hipStream_t stream);
 // Start a frame's estimate at T (3x4 doubles, host): est = prev = init = T.
+// Only needed when a frame schedules no step.
 int lf_track_begin(LfTrackState *st, const double *T12, hipStream_t stream);
 // End of a tracked frame: success test, update = est^-1 (or identity), chain the
-// odometry pose and refresh pose24.
+// odometry pose and refresh pose24.  Only needed when a frame schedules no step.
 int lf_track_finish(LfTrackState *st, hipStream_t stream);
 // Set the odometry pose directly (first frame / restart): T given as 3x4 doubles (host).
 int lf_track_set_pose(LfTrackState *st, const double *T12, hipStream_t stream);

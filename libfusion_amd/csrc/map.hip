@@ -270,7 +270,8 @@ __global__ __launch_bounds__(256) void k_integrate(MapDev m, const float *__rest
             const float wt = kWeighted ? fabsf(sinf(nz)) / (dist * dist) : 1.0f / (dist * dist);
             unsigned char *c = m.rgb + (size_t)b * (3 * LF_BLOCK3) + (size_t)(t + 256 * h);
             if (wp == 0.0f) {
-                m.sdf[vi] = pack_sdf(sdf);
+                // weight stays 0 only when wt is 0: keep the voxel's sdf code unseen
+                m.sdf[vi] = wt == 0.0f ? kSdfUnseen : pack_sdf(sdf);
                 m.weight[vi] = wt > 255.0f ? 255.0f : wt;
                 if (kColour) {
                     c[0] = rgb[3 * pix]; c[LF_BLOCK3] = rgb[3 * pix + 1]; c[2 * LF_BLOCK3] = rgb[3 * pix + 2];

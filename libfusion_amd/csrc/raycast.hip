@@ -147,25 +147,25 @@ LF_DEV int resolve(const RayDev &r, BlockCache &bc, const i3 &v) {
     return bc.b;
 }
 
-// findVoxel + read_sdf (ray_tracing.cu:291-305).  The weight and the sdf are
-// loaded together (one memory round trip), then the weight decides validity.
+// findVoxel + read_sdf (ray_tracing.cu:291-305).  A voxel is valid when its
+// weight is non-zero, which the sdf plane encodes (kSdfUnseen), so one 2-byte
+// load per sample decides both validity and value.
 LF_DEV float read_sdf(const RayDev &r, BlockCache &bc, f3 p, bool &valid, int *voxel_out = nullptr) {
     const i3 v = to_i3(p);
     const int b = resolve(r, bc, v);
     valid = false;
     if (b < 0) return __int_as_float(0x7fc00000);
     const int vi = b * LF_BLOCK3 + lidx(v.x, v.y, v.z);
-    const float w = r.weight[vi];
     const short sd = r.sdf[vi];
     if (voxel_out) *voxel_out = vi;
-    if (w == 0.0f) return __int_as_float(0x7fc00000);
+    if (sd == kSdfUnseen) return __int_as_float(0x7fc00000);
     valid = true;
     return unpack_sdf(sd);
 }
 
 // read_sdf_interped (ray_tracing.cu:307-339).  The 8 corner voxels are
-// resolved first, then all 16 weight / sdf loads are issued in one round trip;
-// the arithmetic is the reference's order.
+// resolved first, then all 8 sdf loads are issued in one round trip; the
+// arithmetic is the reference's order.
 LF_DEV float read_interp(const RayDev &r, BlockCache &bc, f3 p, bool &valid) {
     const f3 xyz = {p.x - floorf(p.x), p.y - floorf(p.y), p.z - floorf(p.z)};
     // corner order of the reference: (0,0,0) (1,0,0) (0,1,0) (1,1,0) (0,0,1) (1,0,1) (0,1,1) (1,1,1)
@@ -177,19 +177,16 @@ LF_DEV float read_interp(const RayDev &r, BlockCache &bc, f3 p, bool &valid) {
         const int b = resolve(r, bc, v);
         vi[k] = b < 0 ? -1 : b * LF_BLOCK3 + lidx(v.x, v.y, v.z);
     }
-    float w[8], s[8];
+    short c[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int i = vi[k] < 0 ? 0 : vi[k];
-        w[k] = r.weight[i];
-        s[k] = unpack_sdf(r.sdf[i]);
-    }
+    for (int k = 0; k < 8; ++k) c[k] = r.sdf[vi[k] < 0 ? 0 : vi[k]];
+    float s[8];
     bool vp = true;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        const bool ok = vi[k] >= 0 && w[k] != 0.0f;
+        const bool ok = vi[k] >= 0 && c[k] != kSdfUnseen;
         vp = vp && ok;
-        if (!ok) s[k] = __int_as_float(0x7fc00000);
+        s[k] = ok ? unpack_sdf(c[k]) : __int_as_float(0x7fc00000);
     }
     float r0 = (1.0f - xyz.x) * s[0] + xyz.x * s[1];
     float r1 = (1.0f - xyz.x) * s[2] + xyz.x * s[3];

@@ -15,16 +15,17 @@
 //  * One lane per reference pixel, so a whole pyramid level is in flight at
 //    once (the pass is bound by memory latency, not bandwidth); each pixel's
 //    loads are issued in two rounds.
-//  * A second one-workgroup kernel sums the partial rows in a fixed order
-//    (deterministic), then one lane builds A = 1e6 H_icp + H_rgb, solves it
-//    with an FP64 LDL^T, applies T <- exp(dx) T in FP64 and runs the
-//    reference's divergence / revert logic.  (A last-workgroup ticket in one
-//    kernel needs an agent-scope release per workgroup, i.e. an L2 write-back
-//    per workgroup on a multi-XCD part; the kernel boundary is cheaper.)
-//    The next iteration reads the updated pose from device memory: a whole
-//    frame of tracking is a fixed sequence of launches with no host round trip
-//    (the reference downloads the sums and solves on the host twice per
-//    iteration).
+//  * A tracking iteration is two launches: the partial rows, then one
+//    workgroup that sums them in a fixed order (deterministic), after which
+//    one lane builds A = 1e6 H_icp + H_rgb, solves it with an FP64 LDL^T,
+//    applies T <- exp(dx) T in FP64 and runs the reference's divergence /
+//    revert logic; the first step of a frame starts the estimate and the step
+//    that ends the frame runs the finish.  The next iteration reads the
+//    updated pose from device memory: a whole frame of tracking is a fixed
+//    sequence of launches with no host round trip (the reference downloads
+//    the sums and solves on the host twice per iteration).
+//  * The stand-alone reductions of the C-ABI (lf_icp_reduce, lf_rgb_step) are
+//    a partial-sum launch and a one-workgroup final-sum launch.
 #include <cfloat>
 
 #include "lf_common.h"
@@ -41,14 +42,18 @@ constexpr int kVals = 58;      // 29 ICP + 29 RGB sums
 
 struct ReduceArgs {
     LfTrackLevel lv;
-    const float *pose12;   // float 3x4 (modes 1/2); mode 3 reads the tracker state
-    float stddev;          // modes 1/2
+    const float *pose12;   // float 3x4 (stand-alone reductions)
+    float stddev;          // stand-alone RGB reduction
     float angle, dthr;
     float *partials;       // kGrid * kVals floats
     float *out29;          // modes 1/2
-    LfTrackState *st;      // mode 3
-    int first_iter;
+    LfTrackState *st;      // tracking step
+    int first_iter;        // first iteration of a level
     float icp_weight;
+    int first_of_frame;    // tracking step: the estimate starts at init
+    int last_level;        // tracking step: the finest level
+    int last_of_frame;     // tracking step: the frame's last scheduled step
+    double init[12];       // tracking step: initial estimate (first_of_frame)
 };
 
 LF_DEV void accumulate(float *s, const float *row);
@@ -59,14 +64,30 @@ LF_DEV void accumulate(float *s, const float *row);
 // two rounds: the pixel's own vertex / normal / intensity first, then every
 // gather (the ICP correspondence and the 12 bilinear taps of the photometric
 // term) together.  The arithmetic follows the reference's operation order.
+// A reference pixel's own inputs (its vertex, normal and intensity): they do
+// not depend on the pose, so they are loaded before the pose is known.
+struct Own {
+    float4 lv, ln;
+    float il;
+};
+
 template <bool kIcp, bool kRgb>
-LF_DEV void pixel_rows(const ReduceArgs &a, const float *P, float stddev, float th, int i, float *si, float *sr) {
+LF_DEV Own load_own(const LfTrackLevel &L, int i) {
+    Own o;
+    o.lv = reinterpret_cast<const float4 *>(L.last_vmap)[i];
+    o.ln = make_float4(0.f, 0.f, 0.f, 0.f);
+    o.il = 0.0f;
+    if (kIcp) o.ln = reinterpret_cast<const float4 *>(L.last_nmap)[i];
+    if (kRgb) o.il = L.last_I[i];
+    return o;
+}
+
+template <bool kIcp, bool kRgb>
+LF_DEV void pixel_rows(const ReduceArgs &a, const Own &o, const float *P, float stddev, float th, float *si,
+                       float *sr) {
     const LfTrackLevel &L = a.lv;
-    const float4 lv = reinterpret_cast<const float4 *>(L.last_vmap)[i];
-    float4 ln = make_float4(0.f, 0.f, 0.f, 0.f);
-    float il = 0.0f;
-    if (kIcp) ln = reinterpret_cast<const float4 *>(L.last_nmap)[i];
-    if (kRgb) il = L.last_I[i];
+    const float4 lv = o.lv, ln = o.ln;
+    const float il = o.il;
     const f3 vg = xform(P, f3{lv.x, lv.y, lv.z});
     // ICP correspondence (pose_estimator.cu:168-193)
     bool icp_ok = false;
@@ -337,42 +358,12 @@ LF_DEV void gn_step(LfTrackState &st, const float *si, const float *sr, int firs
     for (int k = 0; k < 12; ++k) st.pose_f[k] = (float)st.est[k];
 }
 
-// Whether a mode-3 launch has work: the frame has not failed and the level has
-// not broken out of its loop (the first iteration of a level always runs).
-LF_DEV bool track_active(const ReduceArgs &a) {
-    const LfTrackState &st = *a.st;
-    if (st.invalid) return false;
-    return a.first_iter || st.active;
-}
-
-// Partial sums.  kMode 1: ICP rows, 2: RGB rows, 3: both.  Each workgroup
-// writes its kVals partials value-major (value v of workgroup g at v * G + g).
-template <int kMode>
-__global__ __launch_bounds__(kThreads) void k_reduce(ReduceArgs a) {
-    __shared__ float s_wave[kWaves][kVals];
+// Partial sums of one workgroup (kMode 1: ICP rows, 2: RGB rows, 3: both) into
+// out[0..kVals): 64-lane shuffles, then the wavefronts meet in LDS.  Every lane
+// of wavefront 0 with tid < kVals returns its value (the others return 0).
+template <bool kIcp, bool kRgb>
+LF_DEV float block_sums(float *si, float *sr, float (*s_wave)[kVals]) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    float P[12];
-    float stddev = a.stddev;
-    if (kMode == 3) {
-        if (!track_active(a)) return;
-        const LfTrackState &st = *a.st;
-#pragma unroll
-        for (int k = 0; k < 12; ++k) P[k] = st.pose_f[k];
-        stddev = a.first_iter ? 0.0f : st.stddev;
-    } else {
-#pragma unroll
-        for (int k = 0; k < 12; ++k) P[k] = a.pose12[k];
-    }
-    const float th = (float)(1.345 * (double)stddev);
-    constexpr bool kIcp = kMode == 1 || kMode == 3;
-    constexpr bool kRgb = kMode == 2 || kMode == 3;
-    float si[29], sr[29];
-#pragma unroll
-    for (int k = 0; k < 29; ++k) { si[k] = 0.0f; sr[k] = 0.0f; }
-    const int n = a.lv.cols * a.lv.rows;
-    for (int i = blockIdx.x * kThreads + tid; i < n; i += gridDim.x * kThreads)
-        pixel_rows<kIcp, kRgb>(a, P, stddev, th, i, si, sr);
-    // wavefront reduction, then the workgroup's wavefronts in LDS
 #pragma unroll
     for (int k = 0; k < 29; ++k) {
         float vi = si[k], vr = sr[k];
@@ -387,22 +378,19 @@ __global__ __launch_bounds__(kThreads) void k_reduce(ReduceArgs a) {
         }
     }
     __syncthreads();
+    float t = 0.0f;
     if (tid < kVals) {
-        float t = 0.0f;
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) t += s_wave[w][tid];
-        a.partials[(size_t)tid * gridDim.x + blockIdx.x] = t;
     }
+    return t;
 }
 
 // Final sum of the G partial rows in a fixed order (deterministic): wavefront w
 // reduces values w, w+kWaves, ...; lane l adds partials l, l+64, ... of a value
-// (independent loads, in flight together), then a 64-lane tree.  Mode 3 then
-// runs the Gauss-Newton step on one lane.
-template <int kMode>
-__global__ __launch_bounds__(kThreads) void k_reduce_final(ReduceArgs a, int G) {
-    __shared__ float s_sum[kVals];
-    if (kMode == 3 && !track_active(a)) return;
+// (independent loads, in flight together), then a 64-lane tree.  The kVals
+// totals land in s_sum; the caller synchronises before reading them.
+LF_DEV void final_sums(const float *partials, int G, float *s_sum) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr int kPer = (kVals + kWaves - 1) / kWaves;
     constexpr int kChunks = kGrid / 64;
@@ -410,7 +398,7 @@ __global__ __launch_bounds__(kThreads) void k_reduce_final(ReduceArgs a, int G) 
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const int v = wave + kWaves * j;
-        const float *src = a.partials + (size_t)(v < kVals ? v : 0) * G;
+        const float *src = partials + (size_t)(v < kVals ? v : 0) * G;
         // every load of the value first (compile-time bound, predicated), then
         // the fixed-order sum: the loads overlap instead of waiting in turn
         float x[kChunks];
@@ -432,35 +420,137 @@ __global__ __launch_bounds__(kThreads) void k_reduce_final(ReduceArgs a, int G) 
         const int v = wave + kWaves * j;
         if (lane == 0 && v < kVals) s_sum[v] = s;
     }
+}
+
+// Stand-alone reductions (lf_icp_reduce, lf_rgb_step).  kMode 1: ICP rows,
+// 2: RGB rows.  Each workgroup writes its kVals partials value-major (value v
+// of workgroup g at v * G + g).
+template <int kMode>
+__global__ __launch_bounds__(kThreads) void k_reduce(ReduceArgs a) {
+    __shared__ float s_wave[kWaves][kVals];
+    constexpr bool kIcp = kMode == 1, kRgb = kMode == 2;
+    const int tid = threadIdx.x;
+    float P[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) P[k] = a.pose12[k];
+    const float stddev = a.stddev;
+    const float th = (float)(1.345 * (double)stddev);
+    float si[29], sr[29];
+#pragma unroll
+    for (int k = 0; k < 29; ++k) { si[k] = 0.0f; sr[k] = 0.0f; }
+    const int n = a.lv.cols * a.lv.rows;
+    for (int i = blockIdx.x * kThreads + tid; i < n; i += gridDim.x * kThreads)
+        pixel_rows<kIcp, kRgb>(a, load_own<kIcp, kRgb>(a.lv, i), P, stddev, th, si, sr);
+    const float t = block_sums<kIcp, kRgb>(si, sr, s_wave);
+    if (tid < kVals) a.partials[(size_t)tid * gridDim.x + blockIdx.x] = t;
+}
+
+template <int kMode>
+__global__ __launch_bounds__(kThreads) void k_reduce_final(ReduceArgs a, int G) {
+    __shared__ float s_sum[kVals];
+    final_sums(a.partials, G, s_sum);
     __syncthreads();
-    if (tid == 0) {
-        if (kMode == 1) for (int k = 0; k < 29; ++k) a.out29[k] = s_sum[k];
-        if (kMode == 2) for (int k = 0; k < 29; ++k) a.out29[k] = s_sum[29 + k];
-        if (kMode == 3) gn_step(*a.st, s_sum, s_sum + 29, a.first_iter, a.icp_weight);
-    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 29; ++k) a.out29[k] = s_sum[(kMode == 2 ? 29 : 0) + k];
 }
 
 // End of DenseTracking::compute_transform (icp_tracker.cpp:143-155) and the
-// odometry chain of DenseOdometry::trackFrame (rgbd_odometry.cpp:39).
-__global__ void k_track_finish(LfTrackState *st) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// odometry chain of DenseOdometry::trackFrame (rgbd_odometry.cpp:39): success
+// test, update = est^-1 (or identity), pose = pose * update, pose24 refreshed.
+LF_DEV void track_finish(LfTrackState &st) {
     double inv_est[12], D[12];
-    m34_inv(st->est, inv_est);
-    m34_mul(inv_est, st->init, D);
-    const bool ok = !st->invalid && !(se3_log_norm(D) > 0.1);
-    st->success = ok ? 1 : 0;
+    m34_inv(st.est, inv_est);
+    m34_mul(inv_est, st.init, D);
+    const bool ok = !st.invalid && !(se3_log_norm(D) > 0.1);
+    st.success = ok ? 1 : 0;
     double upd[12];
     if (ok) for (int k = 0; k < 12; ++k) upd[k] = inv_est[k];
     else m34_identity(upd);
     double P[12];
-    m34_mul(st->pose, upd, P);
+    m34_mul(st.pose, upd, P);
     double Pi[12];
     m34_inv(P, Pi);
     for (int k = 0; k < 12; ++k) {
-        st->pose[k] = P[k];
-        st->pose24[k] = (float)P[k];
-        st->pose24[12 + k] = (float)Pi[k];
+        st.pose[k] = P[k];
+        st.pose24[k] = (float)P[k];
+        st.pose24[12 + k] = (float)Pi[k];
     }
+}
+
+// One Gauss-Newton iteration of compute_transform, first launch: every
+// workgroup sums the ICP and RGB rows of its pixels into one partial row.  The
+// pose-independent loads of the first pixel are issued together with the
+// tracker-state loads.
+__global__ __launch_bounds__(kThreads) void k_track_step(ReduceArgs a) {
+    __shared__ float s_wave[kWaves][kVals];
+    const int tid = threadIdx.x;
+    const int n = a.lv.cols * a.lv.rows;
+    const int stride = gridDim.x * kThreads;
+    int i = blockIdx.x * kThreads + tid;
+    // the pixel's own loads and the state loads are in flight together
+    Own o = {};
+    if (i < n) o = load_own<true, true>(a.lv, i);
+    float P[12], stddev;
+    if (a.first_of_frame) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) P[k] = (float)a.init[k];
+        stddev = 0.0f;
+    } else {
+        const LfTrackState &st = *a.st;
+        const bool run = !st.invalid && (a.first_iter || st.active);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) P[k] = st.pose_f[k];
+        stddev = a.first_iter ? 0.0f : st.stddev;
+        // a failed frame, or a level that broke out of its loop: nothing to do
+        if (!run) return;
+    }
+    const float th = (float)(1.345 * (double)stddev);
+    float si[29], sr[29];
+#pragma unroll
+    for (int k = 0; k < 29; ++k) { si[k] = 0.0f; sr[k] = 0.0f; }
+    while (i < n) {
+        const Own cur = o;
+        const int next = i + stride;
+        if (next < n) o = load_own<true, true>(a.lv, next);
+        pixel_rows<true, true>(a, cur, P, stddev, th, si, sr);
+        i = next;
+    }
+    const float t = block_sums<true, true>(si, sr, s_wave);
+    if (tid < kVals) a.partials[(size_t)tid * gridDim.x + blockIdx.x] = t;
+}
+
+// Second launch (one workgroup): the partial rows summed in a fixed order,
+// then the FP64 step; the first launch of a frame starts the estimate at init
+// and the launch that ends the frame runs the finish.  (A single launch with a
+// last-workgroup ticket needs an agent-scope release per workgroup, an L2
+// write-back on each of the 8 XCDs per workgroup; measured slower than the
+// kernel boundary.)
+__global__ __launch_bounds__(kThreads) void k_track_final(ReduceArgs a, int G) {
+    __shared__ float s_sum[kVals];
+    const int tid = threadIdx.x;
+    if (!a.first_of_frame) {
+        const LfTrackState &st = *a.st;
+        if (!(!st.invalid && (a.first_iter || st.active))) return;
+    }
+    final_sums(a.partials, G, s_sum);
+    __syncthreads();
+    if (tid == 0) {
+        LfTrackState &st = *a.st;
+        if (a.first_of_frame) {
+            for (int k = 0; k < 12; ++k) st.est[k] = st.prev[k] = st.init[k] = a.init[k];
+            st.invalid = 0;
+            st.iterations = 0;
+        }
+        gn_step(st, s_sum, s_sum + 29, a.first_iter, a.icp_weight);
+        // the frame ends here when the estimate became invalid, when the
+        // finest level broke out of its loop, or after the frame's last step
+        if (st.invalid || (a.last_level && !st.active) || a.last_of_frame) track_finish(st);
+    }
+}
+
+__global__ void k_track_finish(LfTrackState *st) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    track_finish(*st);
 }
 
 __global__ void k_track_begin(LfTrackState *st, double t0, double t1, double t2, double t3, double t4, double t5,
@@ -516,7 +606,7 @@ int lf_track_begin(LfTrackState *st, const double *T, hipStream_t stream) {
     return LF_OK;
 }
 
-int lf_track_iteration(const LfTrackLevel &lv, float *scratch, LfTrackState *st, int first_iter_of_level,
+int lf_track_iteration(const LfTrackLevel &lv, float *scratch, LfTrackState *st, const LfStepFlags &f,
                        float icp_weight, hipStream_t stream) {
     if (int rc = check_level(lv)) return rc;
     ReduceArgs a = {};
@@ -525,12 +615,16 @@ int lf_track_iteration(const LfTrackLevel &lv, float *scratch, LfTrackState *st,
     a.dthr = 0.01f;
     a.partials = scratch;
     a.st = st;
-    a.first_iter = first_iter_of_level;
+    a.first_iter = f.first_of_level;
     a.icp_weight = icp_weight;
+    a.first_of_frame = f.first_of_frame;
+    a.last_level = f.last_level;
+    a.last_of_frame = f.last_of_frame;
+    for (int k = 0; k < 12; ++k) a.init[k] = f.init[k];
     const int G = grid_for(lv);
-    k_reduce<3><<<G, kThreads, 0, stream>>>(a);
+    k_track_step<<<G, kThreads, 0, stream>>>(a);
     LF_LAUNCH_CHECK();
-    k_reduce_final<3><<<1, kThreads, 0, stream>>>(a, G);
+    k_track_final<<<1, kThreads, 0, stream>>>(a, G);
     LF_LAUNCH_CHECK();
     return LF_OK;
 }

@@ -8,6 +8,8 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <stdexcept>
+#include <vector>
 
 #include "fusion_amd.hpp"
 #include "lf_common.h"
@@ -154,6 +156,7 @@ void GpuImage::create(int c, int r, int e) {
 // ------------------------------------------------------------ DeviceImage
 
 DeviceImage::DeviceImage(const IntrinsicMatrix &K, int num_pyr, hipStream_t stream) : stream_(stream) {
+    if (num_pyr < 1 || num_pyr > 8) throw std::invalid_argument("DeviceImage: 1 to 8 pyramid levels");
     BuildIntrinsicPyramid(K, K_, num_pyr);
     depth_raw_.create(K.width, K.height, 4);
     image_.create(K.width, K.height, 3);
@@ -166,34 +169,34 @@ DeviceImage::DeviceImage(const IntrinsicMatrix &K, int num_pyr, hipStream_t stre
     }
 }
 
-int DeviceImage::upload(const uint16_t *depth_u16, const unsigned char *rgb, float depth_scale) {
-    const int W = K_[0].width, H = K_[0].height;
-    void *st = stream_;
-    LF_HIP_CHECK(hipMemcpyAsync(image_.ptr<void>(), rgb, (size_t)W * H * 3, hipMemcpyDeviceToDevice, stream_));
-    LF_RC(lf_depth_to_float(depth_u16, depth_raw_.ptr<float>(), W, H, depth_scale, st));
-    LF_RC(lf_rgb_to_intensity(rgb, intensity_[0].ptr<float>(), W, H, st));
-    LF_RC(lf_bilateral_filter(depth_raw_.ptr<float>(), depth_[0].ptr<float>(), W, H, st));
-    for (int l = 1; l < levels(); ++l) {
-        LF_RC(lf_pyr_down_depth(depth_[l - 1].ptr<float>(), depth_[l].ptr<float>(), K_[l - 1].width, K_[l - 1].height, st));
-        LF_RC(lf_pyr_down_image(intensity_[l - 1].ptr<float>(), intensity_[l].ptr<float>(), K_[l - 1].width,
-                                K_[l - 1].height, st));
-    }
+LfImageLevels DeviceImage::levels_view() {
+    LfImageLevels m = {};
+    m.depth_raw = depth_raw_.ptr<float>();
+    m.rgb = image_.ptr<unsigned char>();
     for (int l = 0; l < levels(); ++l) {
-        const int w = K_[l].width, h = K_[l].height;
-        LF_RC(lf_compute_derivative(intensity_[l].ptr<float>(), dx_[l].ptr<float>(), dy_[l].ptr<float>(), w, h, st));
-        LF_RC(lf_back_project_depth(depth_[l].ptr<float>(), vmap_[l].ptr<float>(), w, h, &K_[l], st));
-        LF_RC(lf_compute_nmap(vmap_[l].ptr<float>(), nmap_[l].ptr<float>(), w, h, st));
+        m.depth[l] = depth_[l].ptr<float>();
+        m.intensity[l] = intensity_[l].ptr<float>();
+        m.dx[l] = dx_[l].ptr<float>();
+        m.dy[l] = dy_[l].ptr<float>();
+        m.vmap[l] = vmap_[l].ptr<float>();
+        m.nmap[l] = nmap_[l].ptr<float>();
     }
-    return LF_OK;
+    return m;
 }
 
+// upload (device_image.cpp:146-181): depth to metres, colour copy, intensity,
+// bilateral filter, depth / intensity pyramids, then per level derivatives,
+// vertices and normals — fused into levels + 1 launches.
+int DeviceImage::upload(const uint16_t *depth_u16, const unsigned char *rgb, float depth_scale) {
+    std::vector<lf_intrinsics> K(K_.begin(), K_.end());
+    return lf_pyramid_upload(depth_u16, rgb, depth_scale, levels(), K.data(), levels_view(), stream_);
+}
+
+// resize_device_map (device_image.cpp:188-199): vertex pyramid of the level-0
+// map and the normals of every level.
 int DeviceImage::resize_device_map() {
-    void *st = stream_;
-    for (int l = 1; l < levels(); ++l)
-        LF_RC(lf_pyr_down_vmap(vmap_[l - 1].ptr<float>(), vmap_[l].ptr<float>(), K_[l - 1].width, K_[l - 1].height, st));
-    for (int l = 0; l < levels(); ++l)
-        LF_RC(lf_compute_nmap(vmap_[l].ptr<float>(), nmap_[l].ptr<float>(), K_[l].width, K_[l].height, st));
-    return LF_OK;
+    std::vector<lf_intrinsics> K(K_.begin(), K_.end());
+    return lf_pyramid_refresh(levels(), K.data(), levels_view(), stream_);
 }
 
 // ------------------------------------------------------------ DenseMapping
@@ -255,13 +258,26 @@ DenseTracking::~DenseTracking() {
 }
 
 // compute_transform(ref, cur, ctx) (icp_tracker.cpp:23-156): levels coarse to
-// fine, max_iterations_[level] Gauss-Newton steps each, all on the device.
+// fine, max_iterations_[level] Gauss-Newton steps each, all on the device.  The
+// first step starts the estimate and the step that ends the frame also runs
+// the finish (success test and odometry chain), so a frame is two launches
+// per step.
 int DenseTracking::compute_transform(DeviceImage &ref, DeviceImage &cur, const TrackingContext &c) {
-    double init[12];
-    (c.use_initial_guess_ ? c.initial_estimate_ : SE3d()).matrix3x4(init);
-    LF_RC(lf_track_begin(state_, init, stream_));
+    LfStepFlags f = {};
+    (c.use_initial_guess_ ? c.initial_estimate_ : SE3d()).matrix3x4(f.init);
+    struct Step { int level, it; };
+    std::vector<Step> plan;
     for (int level = (int)c.max_iterations_.size() - 1; level >= 0; --level) {
         if (level >= ref.levels()) continue;
+        for (int it = 0; it < c.max_iterations_[level]; ++it) plan.push_back({level, it});
+    }
+    if (plan.empty()) {
+        LF_RC(lf_track_begin(state_, f.init, stream_));
+        return lf_track_finish(state_, stream_);
+    }
+    const int finest = plan.back().level;
+    for (size_t s = 0; s < plan.size(); ++s) {
+        const int level = plan[s].level;
         LfTrackLevel lv;
         lv.curr_vmap = cur.get_vmap(level).ptr<float>();
         lv.curr_nmap = cur.get_nmap(level).ptr<float>();
@@ -275,8 +291,11 @@ int DenseTracking::compute_transform(DeviceImage &ref, DeviceImage &cur, const T
         lv.cols = K.width;
         lv.rows = K.height;
         lv.K = K;
-        for (int it = 0; it < c.max_iterations_[level]; ++it)
-            LF_RC(lf_track_iteration(lv, scratch_, state_, it == 0 ? 1 : 0, icp_weight, stream_));
+        f.first_of_level = plan[s].it == 0;
+        f.first_of_frame = s == 0;
+        f.last_level = level == finest;
+        f.last_of_frame = s + 1 == plan.size();
+        LF_RC(lf_track_iteration(lv, scratch_, state_, f, icp_weight, stream_));
     }
     return LF_OK;
 }
@@ -320,8 +339,8 @@ int DenseOdometry::trackFrame(const uint16_t *depth, const unsigned char *rgb, f
     c.initial_estimate_ = SE3d();
     c.intrinsics_pyr_ = K_;
     c.max_iterations_ = iterations_;
+    // compute_transform ends with the finish, which also chains the odometry pose
     LF_RC(tracker_.compute_transform(*ref_, *curr_, c));
-    LF_RC(lf_track_finish(tracker_.state(), stream_));
     std::swap(curr_, ref_);
     trackingLost = false;
     return LF_OK;
@@ -445,20 +464,81 @@ int System::get_camera_pose(double *out16) {
 
 // ------------------------------------------------------------ C-ABI
 
+// ------------------------------------------------------------ lf_image_*
+
+struct lf_image {
+    fusion::DeviceImage *img;
+};
+
+static int check_halving(const lf_intrinsics *K, int num_pyr) {
+    if (num_pyr < 1 || num_pyr > 8) return lf_set_error(LF_ERR_ARG, "num_pyr must be 1..8");
+    if (K->width <= 0 || K->height <= 0) return lf_set_error(LF_ERR_ARG, "bad image size %dx%d", K->width, K->height);
+    for (int l = 0; l + 1 < num_pyr; ++l) {
+        const int sh = 2 << l;
+        if (K->width % sh != 0 || K->height % sh != 0)
+            return lf_set_error(LF_ERR_ARG, "%dx%d does not halve evenly over %d levels", K->width, K->height, num_pyr);
+    }
+    return LF_OK;
+}
+
+extern "C" int lf_image_create(const lf_intrinsics *K, int num_pyr, void *stream, lf_image **out) {
+    if (!K || !out) return lf_set_error(LF_ERR_ARG, "null argument to lf_image_create");
+    if (int rc = check_halving(K, num_pyr)) return rc;
+    fusion::IntrinsicMatrix k(K->width, K->height, K->fx, K->fy, K->cx, K->cy);
+    try {
+        *out = new lf_image{new fusion::DeviceImage(k, num_pyr, lf_stream(stream))};
+    } catch (const std::bad_alloc &) {
+        return lf_set_error(LF_ERR_HIP, "device allocation failed in lf_image_create");
+    }
+    return LF_OK;
+}
+
+extern "C" int lf_image_destroy(lf_image *im) {
+    if (!im) return LF_OK;
+    delete im->img;
+    delete im;
+    return LF_OK;
+}
+
+extern "C" int lf_image_upload(lf_image *im, const uint16_t *depth, const unsigned char *rgb, float depth_scale) {
+    if (!im || !depth || !rgb) return lf_set_error(LF_ERR_ARG, "null argument to lf_image_upload");
+    return im->img->upload(depth, rgb, depth_scale);
+}
+
+extern "C" int lf_image_resize_device_map(lf_image *im) {
+    if (!im) return lf_set_error(LF_ERR_ARG, "null image");
+    return im->img->resize_device_map();
+}
+
+extern "C" int lf_image_plane(lf_image *im, int level, int plane, void **out) {
+    if (!im || !out) return lf_set_error(LF_ERR_ARG, "null argument to lf_image_plane");
+    fusion::DeviceImage &d = *im->img;
+    if (level < 0 || level >= d.levels()) return lf_set_error(LF_ERR_ARG, "bad level %d", level);
+    if ((plane == LF_PLANE_RAW_DEPTH || plane == LF_PLANE_RGB) && level != 0)
+        return lf_set_error(LF_ERR_ARG, "plane %d exists at level 0 only", plane);
+    switch (plane) {
+    case LF_PLANE_DEPTH: *out = d.get_depth(level).ptr<void>(); break;
+    case LF_PLANE_INTENSITY: *out = d.get_intensity(level).ptr<void>(); break;
+    case LF_PLANE_DX: *out = d.get_intensity_dx(level).ptr<void>(); break;
+    case LF_PLANE_DY: *out = d.get_intensity_dy(level).ptr<void>(); break;
+    case LF_PLANE_VMAP: *out = d.get_vmap(level).ptr<void>(); break;
+    case LF_PLANE_NMAP: *out = d.get_nmap(level).ptr<void>(); break;
+    case LF_PLANE_RAW_DEPTH: *out = d.get_raw_depth().ptr<void>(); break;
+    case LF_PLANE_RGB: *out = d.get_image().ptr<void>(); break;
+    default: return lf_set_error(LF_ERR_ARG, "bad plane %d", plane);
+    }
+    return LF_OK;
+}
+
+// ------------------------------------------------------------ lf_system_*
+
 struct lf_system {
     fusion::System *sys;
 };
 
 extern "C" int lf_system_create(const lf_system_config *cfg, lf_system **out) {
     if (!cfg || !out) return lf_set_error(LF_ERR_ARG, "null argument to lf_system_create");
-    if (cfg->num_pyr < 1 || cfg->num_pyr > 8) return lf_set_error(LF_ERR_ARG, "num_pyr must be 1..8");
-    for (int l = 0; l < cfg->num_pyr; ++l) {
-        const int sh = 1 << l;
-        if (cfg->K.width % (2 * sh) != 0 && l + 1 < cfg->num_pyr)
-            return lf_set_error(LF_ERR_ARG, "width %d does not halve evenly over %d levels", cfg->K.width, cfg->num_pyr);
-        if (cfg->K.height % (2 * sh) != 0 && l + 1 < cfg->num_pyr)
-            return lf_set_error(LF_ERR_ARG, "height %d does not halve evenly over %d levels", cfg->K.height, cfg->num_pyr);
-    }
+    if (int rc = check_halving(&cfg->K, cfg->num_pyr)) return rc;
     try {
         *out = new lf_system{new fusion::System(*cfg)};
     } catch (const std::bad_alloc &) {

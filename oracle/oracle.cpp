@@ -145,6 +145,8 @@ template <class R> struct Map {
     // raycast instrumentation: samples, samples in unallocated blocks, trilinear
     // reads, block changes along rays, rays, hits
     long long ray_stats[6] = {0, 0, 0, 0, 0, 0};
+    // per pixel: samples, samples in allocated blocks, trilinear reads
+    std::vector<int> ray_px;
 
     explicit Map(const Params &pp) : p(pp) { reset(); }
     static uint64_t key(I3 b) {
@@ -457,6 +459,7 @@ template <class R> void raycast(Map<R> &m, R *vmap, uint8_t *rgb_out, int cols, 
     if (instrument) {
         m.ray_touched.assign(m.pos.size(), 0);
         for (auto &v : m.ray_stats) v = 0;
+        m.ray_px.assign((size_t)3 * cols * rows, 0);
     }
     Ray<R> ray{m, instrument};
     const R inv_vs = (R)1 / (R)p.voxel_size;
@@ -494,13 +497,19 @@ template <class R> void raycast(Map<R> &m, R *vmap, uint8_t *rgb_out, int cols, 
                 if (instrument) {
                     I3 bb = voxel_to_block(to_i3(res));
                     m.ray_stats[0]++;
+                    int *px = &m.ray_px[3 * ((size_t)y * cols + x)];
+                    px[0]++;
                     if (m.find(bb) < 0) m.ray_stats[1]++;
+                    else px[1]++;
                     if (bb.x != lastb.x || bb.y != lastb.y || bb.z != lastb.z) m.ray_stats[3]++;
                     lastb = bb;
                 }
                 if (sdf <= (R)0.5 && sdf >= (R)-0.5) {
                     sdf = ray.read_interp(res, valid);
-                    if (instrument) m.ray_stats[2]++;
+                    if (instrument) {
+                        m.ray_stats[2]++;
+                        m.ray_px[3 * ((size_t)y * cols + x) + 2]++;
+                    }
                 }
                 if (sdf <= (R)0) break;
                 if (sdf >= (R)0 && last_sdf < (R)0) { back_face = true; break; }
@@ -1183,6 +1192,10 @@ using namespace orc;
     }                                                                                                          \
     extern "C" void orc_ray_stats_##S(void *m, long long *out6) {                                              \
         for (int k = 0; k < 6; ++k) out6[k] = static_cast<Map<R> *>(m)->ray_stats[k];                          \
+    }                                                                                                          \
+    extern "C" void orc_ray_px_##S(void *m, int *out) {                                                        \
+        auto &M = *static_cast<Map<R> *>(m);                                                                   \
+        std::memcpy(out, M.ray_px.data(), M.ray_px.size() * sizeof(int));                                      \
     }                                                                                                          \
     extern "C" int orc_ray_blocks_touched_##S(void *m) {                                                       \
         auto &M = *static_cast<Map<R> *>(m);                                                                   \

@@ -70,6 +70,7 @@ def lib():
             sig("orc_raycast_%s", None, [P, P, P, I, I, P, P, I, I])
             sig("orc_ray_blocks_touched_%s", I, [P])
             sig("orc_ray_stats_%s", None, [P, P])
+            sig("orc_ray_px_%s", None, [P, P])
             sig("orc_zrange_%s", None, [P, P, P, I, I, P, P])
             sig("orc_mesh_%s", ctypes.c_long, [P, I, P, P, P, ctypes.c_long])
             sig("orc_bilateral_%s", None, [P, P, I, I])
@@ -229,6 +230,13 @@ class Map:
         self._f("orc_ray_stats")(self.h, _p(out))
         return dict(zip(("samples", "samples_unallocated", "trilinear", "block_changes", "rays", "hits"),
                         out.tolist()))
+
+    def ray_pixels(self, cols, rows):
+        """Per-pixel counters of the last instrumented raycast: [rows, cols, 3] =
+        samples, samples in allocated blocks, trilinear reads."""
+        out = np.zeros((rows, cols, 3), np.int32)
+        self._f("orc_ray_px")(self.h, _p(out))
+        return out
 
     def ray_blocks_touched(self):
         return int(self._f("orc_ray_blocks_touched")(self.h))

@@ -188,7 +188,8 @@ public:
     DenseOdometry &odometry() { return *odometry_; }
     // Event timing of stages: 0 integrate kernel, 1 raycast kernel, 2 tracking,
     // 3 image pipeline, 4 whole frame.
-    void set_timing(bool on);
+    // stage mask of LF_TIMING_* bits (0: off)
+    void set_timing(int stages);
     int read_timing(int stage, int *launches, double *total_ms);
     bool mapping_enabled = true;
 
@@ -198,7 +199,7 @@ private:
     std::unique_ptr<DenseMapping> mapping_;
     std::unique_ptr<DenseOdometry> odometry_;
     SE3d initial_pose_;
-    bool timing_ = false;
+    int timing_ = 0;
     struct Ev { int stage; hipEvent_t a, b; };
     std::vector<Ev> events_;
     std::vector<hipEvent_t> pool_;

@@ -91,7 +91,14 @@ typedef struct lf_map_params {
  *   rgb[3*512*num_blocks]    uint8 colour, planar per block: R[512] G[512] B[512]
  *   visible[num_blocks]      pool block indices visible in the last update
  *   counters[LF_NUM_COUNTERS]
+ *   occupancy[LF_OCC_WORDS]  hashed occupancy bits of 4x4x4-block regions: bit
+ *                            lf_occupancy_bit(bx, by, bz) is set when a block of
+ *                            the region is inserted (a clear bit proves every
+ *                            block of the region absent; the raycast skips the
+ *                            hash probe there)
  * Voxel i of block b lives at b*512 + z*64 + y*8 + x (map_struct.cu:657-660). */
+#define LF_OCC_WORDS 4096      /* 2^17 occupancy bits */
+
 typedef struct lf_map_storage {
     unsigned long long *keys;
     int *slot_block;
@@ -103,10 +110,15 @@ typedef struct lf_map_storage {
     int *visible;
     unsigned int *counters;
     float *zrange;            /* raycast z-range tiles, 2 * tiles_x * tiles_y floats */
+    unsigned int *occupancy;  /* LF_OCC_WORDS words */
     int tiles_x, tiles_y;
 } lf_map_storage;
 
 /* ---------------------------------------------------------------- general */
+
+/* Occupancy bit of block (bx, by, bz): the computeHash mix of its 4x4x4-block
+ * region (floor division), modulo 2^17. */
+int lf_occupancy_bit(int bx, int by, int bz);
 
 /* Library version string and the gfx target the kernels were built for. */
 const char *lf_version(void);
@@ -299,9 +311,16 @@ int lf_system_map(lf_system *sys, lf_map_params *p, lf_map_storage *s);
 int lf_system_maps(lf_system *sys, int level, const float **vmap, const float **nmap);
 /* The system's HIP stream (as void*). */
 void *lf_system_stream(lf_system *sys);
-/* Kernel timing of the integrate kernel: enable (1) / disable (0) event
- * bracketing, and read {launches, total_ms}. */
-int lf_system_set_timing(lf_system *sys, int enable);
+/* Event timing of the pipeline stages: stages is a bit mask, bit s brackets
+ * stage s with HIP events on the system's stream (0 integrate kernel,
+ * 1 raycast kernel, 2 tracking, 4 whole frame); 0 turns timing off.  Each
+ * event record costs a few microseconds of stream time, so time only the
+ * stages you read.  read_timing returns {launches, total_ms} of a stage. */
+#define LF_TIMING_INTEGRATE 0x1
+#define LF_TIMING_RAYCAST 0x2
+#define LF_TIMING_TRACKING 0x4
+#define LF_TIMING_FRAME 0x10
+int lf_system_set_timing(lf_system *sys, int stages);
 int lf_system_read_timing(lf_system *sys, int stage, int *launches, double *total_ms);
 
 #ifdef __cplusplus

@@ -18,6 +18,9 @@ CNT_ALLOCATED, CNT_VISIBLE, CNT_OVERFLOW, CNT_TRIANGLES = 0, 1, 2, 3
 CNT_VIS_SUM_LO, CNT_VIS_SUM_HI, CNT_UPDATES, CNT_RENDER_BLOCKS = 4, 5, 6, 7
 OVF_TABLE, OVF_POOL, OVF_TRIANGLES = 1, 2, 4
 SDF_UNSEEN = -32768  # sdf code of a voxel with weight 0
+OCC_WORDS = 4096  # occupancy words of a map (2^17 bits)
+TIMING_INTEGRATE, TIMING_RAYCAST, TIMING_TRACKING, TIMING_FRAME = 0x1, 0x2, 0x4, 0x10
+TIMING_ALL = 0x17
 
 
 class LfIntrinsics(ctypes.Structure):
@@ -38,7 +41,8 @@ class LfMapStorage(ctypes.Structure):
     _fields_ = [("keys", ctypes.c_void_p), ("slot_block", ctypes.c_void_p), ("slot_flag", ctypes.c_void_p),
                 ("block_pos", ctypes.c_void_p), ("sdf", ctypes.c_void_p), ("weight", ctypes.c_void_p),
                 ("rgb", ctypes.c_void_p), ("visible", ctypes.c_void_p), ("counters", ctypes.c_void_p),
-                ("zrange", ctypes.c_void_p), ("tiles_x", ctypes.c_int), ("tiles_y", ctypes.c_int)]
+                ("zrange", ctypes.c_void_p), ("occupancy", ctypes.c_void_p), ("tiles_x", ctypes.c_int),
+                ("tiles_y", ctypes.c_int)]
 
 
 class LfSystemConfig(ctypes.Structure):
@@ -55,6 +59,7 @@ SIGNATURES = [
     ("lf_intrinsics_make", None, [I, I, F, F, F, F, P]),
     ("lf_intrinsics_pyr_down", None, [P, P]),
     ("lf_compute_hash", I, [I, I, I, I]),
+    ("lf_occupancy_bit", I, [I, I, I]),
     ("lf_mc_tables", None, [P, P]),
     ("lf_device_count", I, []),
     ("lf_last_error", ctypes.c_char_p, []),
@@ -272,6 +277,7 @@ class Map:
             "visible": torch.empty(B, dtype=torch.int32, device=device),
             "counters": torch.zeros(16, dtype=torch.int32, device=device),
             "zrange": torch.empty(2 * tx * ty, dtype=torch.float32, device=device),
+            "occupancy": torch.empty(OCC_WORDS, dtype=torch.int32, device=device),
         }
         s = LfMapStorage()
         for k, v in self.t.items():
@@ -412,8 +418,11 @@ class System:
             check(rc)
         return n.value
 
-    def set_timing(self, on):
-        check(load().lf_system_set_timing(self.h, 1 if on else 0))
+    def set_timing(self, stages):
+        """Stage mask (TIMING_* bits); True means every stage, False none."""
+        if stages is True:
+            stages = TIMING_ALL
+        check(load().lf_system_set_timing(self.h, int(stages or 0)))
 
     def read_timing(self, stage):
         n = ctypes.c_int(0)

@@ -42,6 +42,8 @@ extern "C" int lf_compute_hash(int x, int y, int z, int num_buckets) {
     return lf::compute_hash(x, y, z, num_buckets);
 }
 
+extern "C" int lf_occupancy_bit(int bx, int by, int bz) { return (int)lf::occupancy_bit(bx, by, bz); }
+
 extern "C" void lf_mc_tables(int *edge_out, int *tri_out) {
     for (int i = 0; i < 256; ++i) {
         edge_out[i] = lf_mc_edge_table[i];
@@ -88,13 +90,14 @@ extern "C" int lf_map_create(const lf_map_params *p, int width, int height, lf_m
     LF_HIP_CHECK(hipMalloc(&s->visible, B * sizeof(int)));
     LF_HIP_CHECK(hipMalloc(&s->counters, LF_NUM_COUNTERS * sizeof(unsigned int)));
     LF_HIP_CHECK(hipMalloc(&s->zrange, 2 * (size_t)s->tiles_x * s->tiles_y * sizeof(float)));
+    LF_HIP_CHECK(hipMalloc(&s->occupancy, LF_OCC_WORDS * sizeof(unsigned int)));
     return LF_OK;
 }
 
 extern "C" int lf_map_release(lf_map_storage *s) {
     if (!s) return LF_OK;
     void *ptrs[] = {s->keys, s->slot_block, s->slot_flag, s->block_pos, s->sdf, s->weight,
-                    s->rgb, s->visible, s->counters, s->zrange};
+                    s->rgb, s->visible, s->counters, s->zrange, s->occupancy};
     for (void *q : ptrs)
         if (q) LF_HIP_CHECK(hipFree(q));
     memset(s, 0, sizeof(*s));
@@ -128,6 +131,8 @@ extern "C" int lf_map_reset(const lf_map_params *p, const lf_map_storage *s, voi
     k_reset_table<<<std::min(4096, (E + 255) / 256), 256, 0, st>>>(s->keys, s->slot_block, s->slot_flag, E);
     LF_LAUNCH_CHECK();
     k_zero_u32<<<1, 64, 0, st>>>(s->counters, LF_NUM_COUNTERS);
+    LF_LAUNCH_CHECK();
+    k_zero_u32<<<LF_OCC_WORDS / 256, 256, 0, st>>>(s->occupancy, LF_OCC_WORDS);
     LF_LAUNCH_CHECK();
     const size_t B = (size_t)p->num_blocks;
     // sdf (2 B, every voxel starts unseen), weight (4 B, 0), rgb (3 B, 0) per voxel;

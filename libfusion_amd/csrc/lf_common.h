@@ -82,6 +82,13 @@ constexpr short kSdfUnseen = -32768;
 LF_HD float unpack_sdf(short v) { return (float)v / 32767.0f; }
 LF_HD short pack_sdf(float v) { return (short)(v * 32767.0f); }
 
+// Occupancy bit of a block: its 4x4x4-block region hashed like computeHash.
+LF_HD unsigned int occupancy_bit(int bx, int by, int bz) {
+    const uint32_t a = (uint32_t)(bx >> 2) * 73856093u, b = (uint32_t)(by >> 2) * 19349669u,
+                   c = (uint32_t)(bz >> 2) * 83492791u;
+    return (a ^ b ^ c) & (uint32_t)(LF_OCC_WORDS * 32 - 1);
+}
+
 // Linear-probing lookup from the computeHash home bucket; returns the pool
 // block of (bx, by, bz) or -1.  The probe is bounded by the table size.
 LF_DEV int find_block(const unsigned long long *__restrict__ keys, const int *__restrict__ slot_block,

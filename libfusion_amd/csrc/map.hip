@@ -41,6 +41,7 @@ struct MapDev {
     unsigned char *rgb;
     int *visible;
     unsigned int *counters;
+    unsigned int *occupancy;
     int num_buckets, num_entries, num_blocks;
     float voxel_size, truncation, zmin_update, zmax_update;
 };
@@ -49,6 +50,7 @@ MapDev make_dev(const lf_map_params *p, const lf_map_storage *s) {
     MapDev d;
     d.keys = s->keys; d.slot_block = s->slot_block; d.slot_flag = s->slot_flag; d.block_pos = s->block_pos;
     d.sdf = s->sdf; d.weight = s->weight; d.rgb = s->rgb; d.visible = s->visible; d.counters = s->counters;
+    d.occupancy = s->occupancy;
     d.num_buckets = p->num_buckets; d.num_entries = p->num_entries; d.num_blocks = p->num_blocks;
     d.voxel_size = p->voxel_size; d.truncation = p->truncation;
     d.zmin_update = p->zmin_update; d.zmax_update = p->zmax_update;
@@ -70,6 +72,8 @@ LF_DEV void insert_block(const MapDev &m, int bx, int by, int bz) {
             claimed = (k == kEmptyKey);
         }
         if (claimed) {
+            const unsigned int ob = occupancy_bit(bx, by, bz);
+            atomicOr(&m.occupancy[ob >> 5], 1u << (ob & 31));
             unsigned int b = atomicAdd(&m.counters[LF_CNT_ALLOCATED], 1u);
             if (b < (unsigned int)m.num_blocks) {
                 m.slot_block[slot] = (int)b;

@@ -35,6 +35,7 @@ struct RayDev {
     const int *visible;
     const int *block_pos;
     unsigned int *counters;
+    const unsigned int *occupancy;  // LF_OCC_WORDS words
     float *zx, *zy;
     int tiles_x, tiles_y;
     int num_buckets, num_entries;
@@ -88,10 +89,13 @@ __global__ __launch_bounds__(256) void k_zrange_fill(RayDev r, const float *__re
         if (uly < 0) uly = 0;
         if (lrx >= r.tiles_x) lrx = r.tiles_x - 1;
         if (lry >= r.tiles_y) lry = r.tiles_y - 1;
-        if (ulx > lrx || uly > lry) continue;
         if (zr0 < r.zmin_raycast) zr0 = r.zmin_raycast;
-        if (zr1 < r.zmin_raycast) continue;
-        atomicAdd(&r.counters[LF_CNT_RENDER_BLOCKS], 1u);
+        const bool renders = !(ulx > lrx || uly > lry) && !(zr1 < r.zmin_raycast);
+        // one counter atomic per wavefront (the active lanes of this iteration)
+        const unsigned long long active = __ballot(1), rend = __ballot(renders);
+        if (lane_id() == __ffsll((long long)active) - 1 && rend)
+            atomicAdd(&r.counters[LF_CNT_RENDER_BLOCKS], (unsigned int)__popcll(rend));
+        if (!renders) continue;
         const int zi0 = __float_as_int(zr0), zi1 = __float_as_int(zr1);
         for (int y = uly; y <= lry; ++y)
             for (int x = ulx; x <= lrx; ++x) {
@@ -114,6 +118,7 @@ constexpr unsigned long long kNoBlock = 0xFFFFFFull;
 struct BlockCache {
     int bx, by, bz, b;
     unsigned long long *lds;  // null: no shared cache (coordinates / pool out of range)
+    const unsigned int *occ;  // the map's occupancy bits, copied to LDS
 };
 
 LF_DEV bool fits13(int v) { return v >= -4096 && v < 4096; }
@@ -124,11 +129,18 @@ LF_DEV unsigned long long lds_tag(int bx, int by, int bz) {
 }
 
 // Resolve the pool block of voxel coordinate v: the lane's last block, then the
-// workgroup's LDS cache, then the hash table.
+// occupancy bits (a clear bit: the whole 4x4x4-block region is unallocated,
+// which is most of the free space a ray crosses), then the workgroup's LDS
+// cache, then the hash table.
 LF_DEV int resolve(const RayDev &r, BlockCache &bc, const i3 &v) {
     const int bx = blk(v.x), by = blk(v.y), bz = blk(v.z);
     if (bx == bc.bx && by == bc.by && bz == bc.bz) return bc.b;
     bc.bx = bx; bc.by = by; bc.bz = bz;
+    const unsigned int ob = occupancy_bit(bx, by, bz);
+    if (!((bc.occ[ob >> 5] >> (ob & 31)) & 1u)) {
+        bc.b = -1;
+        return -1;
+    }
     if (bc.lds && fits13(bx) && fits13(by) && fits13(bz)) {
         const unsigned long long tag = lds_tag(bx, by, bz);
         const unsigned int h = ((unsigned int)bx * 73856093u ^ (unsigned int)by * 19349669u ^
@@ -208,7 +220,10 @@ __global__ __launch_bounds__(256) void k_raycast(RayDev r, float *__restrict__ v
     // (voxel_hashing.cpp:92-93) or no rendering block was made (:105).
     if (r.counters[LF_CNT_VISIBLE] == 0u || r.counters[LF_CNT_RENDER_BLOCKS] == 0u) return;
     __shared__ unsigned long long s_cache[kLdsCache];
+    __shared__ uint4 s_occ[LF_OCC_WORDS / 4];
     for (int i = threadIdx.x; i < kLdsCache; i += blockDim.x) s_cache[i] = 0ull;
+    const uint4 *occ4 = reinterpret_cast<const uint4 *>(r.occupancy);
+    for (int i = threadIdx.x; i < LF_OCC_WORDS / 4; i += blockDim.x) s_occ[i] = occ4[i];
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tile_x = blockIdx.x * 2 + (wave & 1), tile_y = blockIdx.y * 2 + (wave >> 1);
@@ -234,7 +249,8 @@ __global__ __launch_bounds__(256) void k_raycast(RayDev r, float *__restrict__ v
         const f3 be = mul(xform(Twc, pt), inv_vs);
         const f3 dir = normalised(sub(be, bs));
         f3 res = bs;
-        BlockCache bc = {kBad, kBad, kBad, -1, r.pool_fits_lds ? s_cache : nullptr};
+        BlockCache bc = {kBad, kBad, kBad, -1, r.pool_fits_lds ? s_cache : nullptr,
+                         reinterpret_cast<const unsigned int *>(s_occ)};
         float sdf = 1.0f, last_sdf, step;
         bool valid = false, back_face = false;
         while (dist_s < dist_e) {
@@ -286,6 +302,7 @@ int lf_raycast_ev(const lf_map_params *p, const lf_map_storage *s, float *vmap, 
     RayDev r;
     r.keys = s->keys; r.slot_block = s->slot_block; r.sdf = s->sdf; r.weight = s->weight; r.rgb = s->rgb;
     r.visible = s->visible; r.block_pos = s->block_pos; r.counters = s->counters;
+    r.occupancy = s->occupancy;
     r.zx = s->zrange; r.zy = s->zrange + (size_t)s->tiles_x * s->tiles_y;
     r.tiles_x = s->tiles_x; r.tiles_y = s->tiles_y;
     r.num_buckets = p->num_buckets; r.num_entries = p->num_entries;

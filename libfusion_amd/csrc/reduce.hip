@@ -224,14 +224,21 @@ LF_DEV double se3_log_norm(const double *T) {
     if (th < 1e-8) {
         for (int i = 0; i < 3; ++i) w[i] = 0.5 * v[i];
     } else if (M_PI - th < 1e-6) {
-        int k = 0;
-        for (int i = 1; i < 3; ++i)
-            if (T[i * 5] > T[k * 5]) k = i;
-        double ax[3];
-        ax[k] = sqrt(fmax(0.0, (T[k * 5] + 1) * 0.5));
-        for (int i = 0; i < 3; ++i)
-            if (i != k) ax[i] = T[k * 4 + i] / (2 * ax[k]);
-        for (int i = 0; i < 3; ++i) w[i] = ax[i] * th;
+        // k = index of the largest diagonal entry (first one on ties); the
+        // row of R is selected without a runtime array index
+        const int k1 = T[5] > T[0] ? 1 : 0;
+        const int k = T[10] > (k1 ? T[5] : T[0]) ? 2 : k1;
+        const double r0 = k == 0 ? T[0] : k == 1 ? T[4] : T[8];
+        const double r1 = k == 0 ? T[1] : k == 1 ? T[5] : T[9];
+        const double r2 = k == 0 ? T[2] : k == 1 ? T[6] : T[10];
+        const double rkk = k == 0 ? r0 : k == 1 ? r1 : r2;
+        const double axk = sqrt(fmax(0.0, (rkk + 1) * 0.5));
+        const double ax0 = k == 0 ? axk : r0 / (2 * axk);
+        const double ax1 = k == 1 ? axk : r1 / (2 * axk);
+        const double ax2 = k == 2 ? axk : r2 / (2 * axk);
+        w[0] = ax0 * th;
+        w[1] = ax1 * th;
+        w[2] = ax2 * th;
     } else {
         const double s = th / (2 * sin(th));
         for (int i = 0; i < 3; ++i) w[i] = s * v[i];
@@ -358,41 +365,71 @@ LF_DEV void gn_step(LfTrackState &st, const float *si, const float *sr, int firs
     for (int k = 0; k < 12; ++k) st.pose_f[k] = (float)st.est[k];
 }
 
-// Partial sums of one workgroup (kMode 1: ICP rows, 2: RGB rows, 3: both) into
-// out[0..kVals): 64-lane shuffles, then the wavefronts meet in LDS.  Every lane
-// of wavefront 0 with tid < kVals returns its value (the others return 0).
+// Wavefront sum of N values per lane (N a power of two, <= 64) as a
+// transpose butterfly: at each exchange a lane keeps half of its values and
+// sends the other half to its partner, so N values cost N - 1 + 6 - log2(N)
+// shuffles instead of 6 N.  Afterwards lane l holds the wavefront total of
+// value l >> (6 - log2 N).  Fixed pairing: deterministic.
+// (Compile-time recursion keeps every index constant, so v stays in VGPRs.)
+template <int N, int kOff>
+LF_DEV void transpose_halve(float *v, int lane) {
+    if constexpr (N > 1) {
+        constexpr int h = N / 2;
+        const bool hi = (lane & kOff) != 0;
+#pragma unroll
+        for (int i = 0; i < h; ++i) {
+            const float keep = hi ? v[h + i] : v[i];
+            const float send = hi ? v[i] : v[h + i];
+            v[i] = keep + __shfl_xor(send, kOff);
+        }
+        transpose_halve<h, kOff / 2>(v, lane);
+    } else {
+#pragma unroll
+        for (int o = kOff; o >= 1; o >>= 1) v[0] += __shfl_xor(v[0], o);
+    }
+}
+
+template <int N>
+LF_DEV float transpose_reduce(float (&v)[N]) {
+    static_assert(N >= 1 && N <= 64 && (N & (N - 1)) == 0, "N must be a power of two <= 64");
+    transpose_halve<N, 32>(v, __lane_id());
+    return v[0];
+}
+
+// Partial sums of one workgroup (ICP rows into si, RGB rows into sr) into
+// value order: transpose butterfly in each wavefront, then the wavefronts meet
+// in LDS.  Lanes with tid < kVals return value tid (the others return 0).
 template <bool kIcp, bool kRgb>
-LF_DEV float block_sums(float *si, float *sr, float (*s_wave)[kVals]) {
+LF_DEV float block_sums(const float *si, const float *sr, float (*s_wave)[kVals]) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    float v[64];
 #pragma unroll
     for (int k = 0; k < 29; ++k) {
-        float vi = si[k], vr = sr[k];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            if (kIcp) vi += __shfl_xor(vi, off);
-            if (kRgb) vr += __shfl_xor(vr, off);
-        }
-        if (lane == 0) {
-            s_wave[wave][k] = vi;
-            s_wave[wave][29 + k] = vr;
-        }
+        v[k] = kIcp ? si[k] : 0.0f;
+        v[29 + k] = kRgb ? sr[k] : 0.0f;
     }
+#pragma unroll
+    for (int k = kVals; k < 64; ++k) v[k] = 0.0f;
+    const float w = transpose_reduce<64>(v);
+    if (lane < kVals) s_wave[wave][lane] = w;
     __syncthreads();
     float t = 0.0f;
     if (tid < kVals) {
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) t += s_wave[w][tid];
+        for (int k = 0; k < kWaves; ++k) t += s_wave[k][tid];
     }
     return t;
 }
 
 // Final sum of the G partial rows in a fixed order (deterministic): wavefront w
 // reduces values w, w+kWaves, ...; lane l adds partials l, l+64, ... of a value
-// (independent loads, in flight together), then a 64-lane tree.  The kVals
-// totals land in s_sum; the caller synchronises before reading them.
+// (independent loads, in flight together), then a transpose butterfly over the
+// wavefront's values.  The kVals totals land in s_sum; the caller
+// synchronises before reading them.
 LF_DEV void final_sums(const float *partials, int G, float *s_sum) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    constexpr int kPer = (kVals + kWaves - 1) / kWaves;
+    constexpr int kPer = 8;
+    static_assert(kPer * kWaves >= kVals, "values per wavefront");
     constexpr int kChunks = kGrid / 64;
     float acc[kPer];
 #pragma unroll
@@ -412,14 +449,9 @@ LF_DEV void final_sums(const float *partials, int G, float *s_sum) {
         for (int c = 0; c < kChunks; ++c) s += x[c];
         acc[j] = s;
     }
-#pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-        float s = acc[j];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-        const int v = wave + kWaves * j;
-        if (lane == 0 && v < kVals) s_sum[v] = s;
-    }
+    const float s = transpose_reduce<kPer>(acc);
+    const int v = wave + kWaves * (lane >> 3);
+    if ((lane & 7) == 0 && v < kVals) s_sum[v] = s;
 }
 
 // Stand-alone reductions (lf_icp_reduce, lf_rgb_step).  kMode 1: ICP rows,

@@ -380,8 +380,8 @@ hipEvent_t System::take_event() {
     return e;
 }
 
-void System::set_timing(bool on) {
-    timing_ = on;
+void System::set_timing(int stages) {
+    timing_ = stages;
     for (auto &e : events_) { pool_.push_back(e.a); pool_.push_back(e.b); }
     events_.clear();
 }
@@ -404,30 +404,28 @@ int System::read_timing(int stage, int *launches, double *total_ms) {
 
 // process_images (system.cpp:40-114): track, fuse, raycast, re-pyramid.
 int System::process_images(const uint16_t *depth, const unsigned char *rgb) {
+    // events of the stages selected by set_timing (stage s: bit s)
     Ev fr{4, nullptr, nullptr}, ig{0, nullptr, nullptr}, rc{1, nullptr, nullptr}, tr{2, nullptr, nullptr};
-    if (timing_) {
-        fr.a = take_event(); fr.b = take_event(); ig.a = take_event(); ig.b = take_event();
-        rc.a = take_event(); rc.b = take_event(); tr.a = take_event(); tr.b = take_event();
-        LF_HIP_CHECK(hipEventRecord(fr.a, stream_));
+    const bool t_fr = timing_ & LF_TIMING_FRAME, t_ig = (timing_ & LF_TIMING_INTEGRATE) && mapping_enabled,
+               t_rc = timing_ & LF_TIMING_RAYCAST, t_tr = (timing_ & LF_TIMING_TRACKING) && !odometry_->trackingLost;
+    for (Ev *e : {&fr, &ig, &rc, &tr}) {
+        const bool on = e == &fr ? t_fr : e == &ig ? t_ig : e == &rc ? t_rc : t_tr;
+        if (on) { e->a = take_event(); e->b = take_event(); }
     }
+    if (t_fr) LF_HIP_CHECK(hipEventRecord(fr.a, stream_));
     DenseOdometry &od = *odometry_;
     if (!od.trackingLost) {
-        if (timing_) LF_HIP_CHECK(hipEventRecord(tr.a, stream_));
+        if (t_tr) LF_HIP_CHECK(hipEventRecord(tr.a, stream_));
         LF_RC(od.trackFrame(depth, rgb, cfg_.depth_scale, initial_pose_));
-        if (timing_) LF_HIP_CHECK(hipEventRecord(tr.b, stream_));
+        if (t_tr) LF_HIP_CHECK(hipEventRecord(tr.b, stream_));
     }
     DeviceImage &ref = od.get_reference_image();
     if (mapping_enabled) LF_RC(mapping_->update(ref, od.pose24(), cfg_.integrate_colour != 0, ig.a, ig.b));
     LF_RC(mapping_->raycast(ref.get_vmap(0).ptr<float>(), od.pose24(), rc.a, rc.b));
     LF_RC(ref.resize_device_map());
-    if (timing_) {
-        LF_HIP_CHECK(hipEventRecord(fr.b, stream_));
-        events_.push_back(fr);
-        events_.push_back(tr);
-        if (mapping_enabled) events_.push_back(ig);
-        else { pool_.push_back(ig.a); pool_.push_back(ig.b); }
-        events_.push_back(rc);
-    }
+    if (t_fr) LF_HIP_CHECK(hipEventRecord(fr.b, stream_));
+    for (Ev *e : {&fr, &tr, &ig, &rc})
+        if (e->a) events_.push_back(*e);
     return LF_OK;
 }
 
@@ -598,8 +596,8 @@ extern "C" int lf_system_maps(lf_system *s, int level, const float **vmap, const
 
 extern "C" void *lf_system_stream(lf_system *s) { return s->sys->stream(); }
 
-extern "C" int lf_system_set_timing(lf_system *s, int enable) {
-    s->sys->set_timing(enable != 0);
+extern "C" int lf_system_set_timing(lf_system *s, int stages) {
+    s->sys->set_timing(stages);
     return LF_OK;
 }
 

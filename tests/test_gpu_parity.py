@@ -97,6 +97,23 @@ def test_update_c0_block_set_and_tsdf(c0):
     assert np.array_equal(gv, ov), "visible sets differ"
 
 
+def test_occupancy_bits_cover_every_allocated_block(c0):
+    """A clear occupancy bit must prove its region empty: every allocated
+    block's bit is set (the raycast skips hash probes on clear bits)."""
+    sc, K, p, T, depth, nmap, rgb = c0
+    gmap = capi.Map(p, K.width, K.height)
+    gmap.update_weighted(dev(depth), dev(nmap), dev(rgb), dev(capi.pose24(T)), K)
+    torch.cuda.synchronize()
+    blocks = gmap.blocks()
+    assert len(blocks) > 100
+    occ = gmap.t["occupancy"].cpu().numpy().view(np.uint32)
+    lib = capi.load()
+    bits = np.array([lib.lf_occupancy_bit(int(x), int(y), int(z)) for x, y, z in blocks])
+    assert np.all((occ[bits >> 5] >> (bits & 31)) & 1)
+    # the filter stays selective: far fewer bits set than blocks
+    assert np.unpackbits(occ.view(np.uint8)).sum() < len(blocks)
+
+
 def test_update_sequence_fusion_half_res():
     """Five views fused on both sides (exercises the running weighted average)."""
     sc = Scene(1, 1)

@@ -40,6 +40,30 @@ def test_compute_hash_known_answers(pos, n, expect):
     assert capi.load().lf_compute_hash(*pos, n) == expect
 
 
+def test_occupancy_bit_regions():
+    """The occupancy bit depends only on the 4x4x4-block region (floor
+    division, negative coordinates included) and hashes the region like
+    computeHash (known answers from the 32-bit wrapped products)."""
+    lib = capi.load()
+    mask = 32 * capi.OCC_WORDS - 1
+
+    def expect(bx, by, bz):
+        a = ((bx // 4) * 73856093) & 0xFFFFFFFF
+        b = ((by // 4) * 19349669) & 0xFFFFFFFF
+        c = ((bz // 4) * 83492791) & 0xFFFFFFFF
+        return (a ^ b ^ c) & mask
+
+    assert lib.lf_occupancy_bit(0, 0, 0) == 0
+    assert lib.lf_occupancy_bit(4, 0, 0) == 73856093 & mask
+    assert lib.lf_occupancy_bit(-1, 0, 0) == (-73856093 & 0xFFFFFFFF) & mask
+    rng = np.random.default_rng(5)
+    for bx, by, bz in rng.integers(-300, 300, size=(200, 3)).tolist():
+        b = lib.lf_occupancy_bit(bx, by, bz)
+        assert b == expect(bx, by, bz)
+        r = (bx - bx % 4, by - by % 4, bz - bz % 4)
+        assert lib.lf_occupancy_bit(r[0] + 3, r[1] + 3, r[2] + 3) == b
+
+
 def test_mc_tables_digest_and_consistency():
     edge, tri = capi.mc_tables()
     assert hashlib.sha256(struct.pack("<256i", *edge.tolist())).hexdigest() == EDGE_SHA

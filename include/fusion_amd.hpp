@@ -73,6 +73,8 @@ public:
     int resize_device_map();
     const GpuImage &get_raw_depth() const { return depth_raw_; }
     const GpuImage &get_image() const { return image_; }
+    // level-0 integration samples: float4 {depth or NaN, weight, colour bits, 0} per pixel
+    const GpuImage &get_integration_samples() const { return samples_; }
     GpuImage &get_depth(int l = 0) { return depth_[l]; }
     GpuImage &get_vmap(int l = 0) { return vmap_[l]; }
     GpuImage &get_nmap(int l = 0) { return nmap_[l]; }
@@ -86,7 +88,7 @@ private:
     LfImageLevels levels_view();
     std::vector<IntrinsicMatrix> K_;
     hipStream_t stream_;
-    GpuImage depth_raw_, image_;
+    GpuImage depth_raw_, image_, samples_;
     std::vector<GpuImage> depth_, intensity_, dx_, dy_, vmap_, nmap_;
 };
 

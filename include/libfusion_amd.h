@@ -234,7 +234,10 @@ enum {
     LF_PLANE_VMAP = 4,       /* float4 vertices (get_vmap)                         */
     LF_PLANE_NMAP = 5,       /* float4 normals (get_nmap)                          */
     LF_PLANE_RAW_DEPTH = 6,  /* float, metres, level 0 only (get_raw_depth)        */
-    LF_PLANE_RGB = 7         /* uint8 RGB copy of the input, level 0 (get_image)   */
+    LF_PLANE_RGB = 7,        /* uint8 RGB copy of the input, level 0 (get_image)   */
+    LF_PLANE_SAMPLES = 8     /* float4 integration samples, level 0: {raw depth, or
+                                NaN where the normal is invalid; |sin(n_z)|/d^2;
+                                colour r | g<<8 | b<<16 as bits; 0}              */
 };
 
 /* DeviceImage(K, NUM_PYRS) (device_image.cpp:8-21); work is queued on stream. */
@@ -249,6 +252,12 @@ int lf_image_upload(lf_image *img, const uint16_t *depth, const unsigned char *r
 int lf_image_resize_device_map(lf_image *img);
 /* Device pointer of a plane (LF_PLANE_*) of a level; rows are packed. */
 int lf_image_plane(lf_image *img, int level, int plane, void **out);
+/* update(RgbdImagePtr frame) (voxel_hashing.cpp:44-65) into a caller's map:
+ * lf_update_weighted on the frame's raw depth, level-0 normals and (colour != 0)
+ * colour, read through the frame's packed integration samples.  Same result
+ * as lf_update_weighted on those planes. */
+int lf_update_image(const lf_map_params *p, const lf_map_storage *s, lf_image *img, const float *pose24,
+                    int colour, void *stream);
 
 /* ------------------------------------------------------------- tracking */
 

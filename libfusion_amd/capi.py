@@ -85,6 +85,7 @@ SIGNATURES = [
     ("lf_image_upload", I, [P, P, P, F]),
     ("lf_image_resize_device_map", I, [P]),
     ("lf_image_plane", I, [P, I, I, P]),
+    ("lf_update_image", I, [P, P, P, P, I, P]),
     ("lf_reduce_scratch_floats", ctypes.c_size_t, []),
     ("lf_icp_reduce", I, [P, P, P, P, I, I, P, P, P, P, P]),
     ("lf_rgb_step", I, [P, P, P, P, P, I, I, F, P, P, P, P, P]),
@@ -183,7 +184,7 @@ def ptr(t):
     return ctypes.c_void_p(0 if t is None else t.data_ptr())
 
 
-PLANE_DEPTH, PLANE_INTENSITY, PLANE_DX, PLANE_DY, PLANE_VMAP, PLANE_NMAP, PLANE_RAW_DEPTH, PLANE_RGB = range(8)
+PLANE_DEPTH, PLANE_INTENSITY, PLANE_DX, PLANE_DY, PLANE_VMAP, PLANE_NMAP, PLANE_RAW_DEPTH, PLANE_RGB, PLANE_SAMPLES = range(9)
 
 _hip = None
 
@@ -248,7 +249,7 @@ class Image:
         """A copy of a plane as a tensor ([h, w], [h, w, 4] for maps, [h, w, 3] uint8 for colour)."""
         torch = _torch()
         w, h = self.K[level].width, self.K[level].height
-        if plane in (PLANE_VMAP, PLANE_NMAP):
+        if plane in (PLANE_VMAP, PLANE_NMAP, PLANE_SAMPLES):
             t = torch.empty((h, w, 4), device="cuda")
         elif plane == PLANE_RGB:
             t = torch.empty((h, w, 3), dtype=torch.uint8, device="cuda")
@@ -296,6 +297,11 @@ class Map:
         h, w = depth.shape[:2]
         check(load().lf_update_weighted(ctypes.byref(self.params), ctypes.byref(self.storage), ptr(depth), ptr(nmap),
                                         ptr(rgb), w, h, ptr(pose24_dev), ctypes.byref(K), stream_ptr()))
+
+    def update_image(self, image, pose24_dev, colour=True):
+        """update(RgbdImagePtr): weighted update from an Image's planes (packed samples)."""
+        check(load().lf_update_image(ctypes.byref(self.params), ctypes.byref(self.storage), image.h,
+                                     ptr(pose24_dev), 1 if colour else 0, stream_ptr()))
 
     def update(self, depth, rgb, pose24_dev, K):
         h, w = depth.shape[:2]

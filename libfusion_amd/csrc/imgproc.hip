@@ -109,6 +109,18 @@ LF_DEV float4 nmap_at(const V &vert, int w, int h, int x, int y) {
     return make_float4(n.x, n.y, n.z, 1.0f);
 }
 
+// The integration sample of a pixel, read by the weighted TSDF update as one
+// 16-byte gather: x = raw depth, or NaN when the normal is invalid (both skip
+// the pixel); y = the weight |sin(n_z)| / d^2 of update_map_weighted_kernel
+// (depth_fusion.cu:356-428) in its operation order; z = colour r | g << 8 |
+// b << 16 as bits.
+LF_DEV float4 integration_sample(float dist, float4 n, const unsigned char *c) {
+    const float qnan = __int_as_float(0x7fc00000);
+    const float wt = fabsf(sinf(n.z)) / (dist * dist);
+    const unsigned int rgba = (unsigned int)c[0] | ((unsigned int)c[1] << 8) | ((unsigned int)c[2] << 16);
+    return make_float4(isnan(n.x) ? qnan : dist, wt, __uint_as_float(rgba), 0.0f);
+}
+
 struct FloatAt {
     const float *p; int w;
     LF_DEV float operator()(int x, int y) const { return p[(size_t)y * w + x]; }
@@ -203,8 +215,8 @@ enum PartKind { kFrameIn, kPyramid, kLevelMaps, kNmap, kVmapDown, kVmapDown2 };
 
 struct Part {
     int kind, w, h, wg_begin;  // w, h: the part's output size
-    const void *in0, *in1;
-    void *out0, *out1, *out2, *out3;
+    const void *in0, *in1, *in2, *in3;
+    void *out0, *out1, *out2, *out3, *out4;
     lf_intrinsics K;
     float scale;
 };
@@ -245,7 +257,13 @@ __global__ __launch_bounds__(256) void k_frame_pass(Pass ps) {
         derivative_at(static_cast<const float *>(p.in1), w, h, x, y, static_cast<float *>(p.out0)[i],
                       static_cast<float *>(p.out1)[i]);
         static_cast<float4 *>(p.out2)[i] = back_project_at(d[i], x, y, p.K);
-        static_cast<float4 *>(p.out3)[i] = nmap_at(DepthVertexAt{d, w, p.K}, w, h, x, y);
+        const float4 n = nmap_at(DepthVertexAt{d, w, p.K}, w, h, x, y);
+        static_cast<float4 *>(p.out3)[i] = n;
+        if (p.out4) {  // level 0: the pixel's integration sample
+            const float dist = static_cast<const float *>(p.in2)[i];
+            const unsigned char *c = static_cast<const unsigned char *>(p.in3) + 3 * i;
+            static_cast<float4 *>(p.out4)[i] = integration_sample(dist, n, c);
+        }
         break;
     }
     case kNmap:  // normal map of a stored vertex map
@@ -319,6 +337,9 @@ int lf_pyramid_upload(const uint16_t *depth_u16, const unsigned char *rgb, float
         Part p = make_part(kLevelMaps, K[l].width, K[l].height);
         p.in0 = m.depth[l]; p.in1 = m.intensity[l]; p.K = K[l];
         p.out0 = m.dx[l]; p.out1 = m.dy[l]; p.out2 = m.vmap[l]; p.out3 = m.nmap[l];
+        if (l == 0 && m.samples) {
+            p.in2 = m.depth_raw; p.in3 = m.rgb; p.out4 = m.samples;
+        }
         ps.part[ps.nparts++] = p;
         if (l + 1 < levels) {
             Part q = make_part(kPyramid, K[l + 1].width, K[l + 1].height);

@@ -21,6 +21,7 @@ namespace lf {
 constexpr unsigned long long kEmptyKey = ~0ull;
 constexpr int kBad = INT_MIN;
 constexpr int kWave = 64;
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, each with its own L2; workgroups are dealt to them in turn
 
 struct f3 { float x, y, z; };
 LF_HD f3 add(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
@@ -90,15 +91,18 @@ LF_HD unsigned int occupancy_bit(int bx, int by, int bz) {
 }
 
 // Linear-probing lookup from the computeHash home bucket; returns the pool
-// block of (bx, by, bz) or -1.  The probe is bounded by the table size.
+// block of (bx, by, bz) or -1.  The probe is bounded by the table size.  A
+// slot's key and block index are loaded together (one memory round trip per
+// probed slot; most lookups end at the home slot).
 LF_DEV int find_block(const unsigned long long *__restrict__ keys, const int *__restrict__ slot_block,
                       int bx, int by, int bz, int num_buckets, int num_entries) {
     if (bx == kBad || by == kBad || bz == kBad) return -1;
     const unsigned long long key = pack_key(bx, by, bz);
     int slot = compute_hash(bx, by, bz, num_buckets);
     for (int i = 0; i < num_entries; ++i) {
-        unsigned long long k = keys[slot];
-        if (k == key) return slot_block[slot];
+        const unsigned long long k = keys[slot];
+        const int b = slot_block[slot];
+        if (k == key) return b;
         if (k == kEmptyKey) return -1;
         if (++slot == num_entries) slot = 0;
     }

@@ -8,10 +8,13 @@
 #include "libfusion_amd.h"
 
 // lf_update_weighted / lf_update with events recorded immediately before and
-// after the integration kernel (either may be null).
+// after the integration kernel (either may be null).  With samples (the
+// weighted update of a DeviceImage frame) the integration kernel reads the
+// frame's packed per-pixel samples instead of nmap and rgb; depth is still
+// the raw depth of block allocation.
 int lf_update_ev(const lf_map_params *p, const lf_map_storage *s, const float *depth, const float *nmap,
                  const unsigned char *rgb, int cols, int rows, const float *pose24, const lf_intrinsics *K,
-                 hipStream_t st, bool weighted, hipEvent_t e0, hipEvent_t e1);
+                 hipStream_t st, bool weighted, hipEvent_t e0, hipEvent_t e1, const float *samples = nullptr);
 // lf_raycast with events around the ray-marching kernel.
 int lf_raycast_ev(const lf_map_params *p, const lf_map_storage *s, float *vmap, unsigned char *rgb_out, int cols,
                   int rows, const float *pose24, const lf_intrinsics *K, int check_colour, hipStream_t st,
@@ -23,6 +26,7 @@ struct LfImageLevels {
     unsigned char *rgb;                  // level 0 colour copy
     float *depth[8], *intensity[8], *dx[8], *dy[8];
     float *vmap[8], *nmap[8];            // float4 per pixel
+    float *samples;                      // level 0 integration samples, float4 per pixel (may be null)
 };
 
 // DeviceImage::upload (device_image.cpp:146-181) in levels + 1 launches.  Every

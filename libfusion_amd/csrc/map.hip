@@ -227,12 +227,18 @@ __global__ __launch_bounds__(256) void k_visible(MapDev m, const float *__restri
 }
 
 // update_map_weighted_kernel (kWeighted, depth_fusion.cu:356-428) and
-// update_map_with_colour_kernel (!kWeighted, :288-354).
-template <bool kWeighted, bool kColour>
+// update_map_with_colour_kernel (!kWeighted, :288-354).  kPacked (weighted
+// only): the pixel's depth, weight and colour come from one 16-byte
+// integration sample (DeviceImage frames), whose weight is the same
+// expression evaluated when the sample was made; otherwise depth, normal and
+// colour are gathered separately.
+template <bool kWeighted, bool kColour, bool kPacked = false>
 __global__ __launch_bounds__(256) void k_integrate(MapDev m, const float *__restrict__ depth,
                                                    const float *__restrict__ nmap,
                                                    const unsigned char *__restrict__ rgb, int cols, int rows,
-                                                   const float *__restrict__ pose24, lf_intrinsics K) {
+                                                   const float *__restrict__ pose24, lf_intrinsics K,
+                                                   const float4 *__restrict__ samples = nullptr) {
+    static_assert(!kPacked || kWeighted, "packed samples carry the weighted update's weight");
     float Tcw[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) Tcw[i] = pose24[12 + i];
@@ -257,13 +263,21 @@ __global__ __launch_bounds__(256) void k_integrate(MapDev m, const float *__rest
             const int v = proj_floor(K.fy * pt.y / pt.z + K.cy + 0.5f);
             if (u < 0 || v < 0 || u > cols - 1 || v > rows - 1) continue;
             const size_t pix = (size_t)v * cols + u;
-            const float dist = depth[pix];
-            float nz = 0.0f;
-            if (kWeighted) {
+            float dist, nz = 0.0f, wpk = 0.0f;
+            unsigned int cpk = 0;
+            if (kPacked) {
+                const float4 smp = samples[pix];
+                dist = smp.x;  // NaN also when the normal is invalid
+                wpk = smp.y;
+                cpk = __float_as_uint(smp.z);
+                if (isnan(dist) || dist > zmax || dist < zmin) continue;
+            } else if (kWeighted) {
+                dist = depth[pix];
                 const float nx = nmap[4 * pix];
                 nz = nmap[4 * pix + 2];
                 if (isnan(dist) || isnan(nx) || dist > zmax || dist < zmin) continue;
             } else {
+                dist = depth[pix];
                 if (isnan(dist) || (double)dist < 1e-2 || dist > zmax || dist < zmin) continue;
             }
             float sdf = dist - pt.z;
@@ -271,14 +285,20 @@ __global__ __launch_bounds__(256) void k_integrate(MapDev m, const float *__rest
             sdf = fminf(1.0f, sdf * inv_trunc);
             const size_t vi = (size_t)b * LF_BLOCK3 + (size_t)(t + 256 * h);
             const float wp = m.weight[vi];
-            const float wt = kWeighted ? fabsf(sinf(nz)) / (dist * dist) : 1.0f / (dist * dist);
+            const float wt = kPacked ? wpk : kWeighted ? fabsf(sinf(nz)) / (dist * dist) : 1.0f / (dist * dist);
             unsigned char *c = m.rgb + (size_t)b * (3 * LF_BLOCK3) + (size_t)(t + 256 * h);
+            unsigned char cin[3];
+            if (kColour) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    cin[k] = kPacked ? (unsigned char)(cpk >> (8 * k)) : rgb[3 * pix + k];
+            }
             if (wp == 0.0f) {
                 // weight stays 0 only when wt is 0: keep the voxel's sdf code unseen
                 m.sdf[vi] = wt == 0.0f ? kSdfUnseen : pack_sdf(sdf);
                 m.weight[vi] = wt > 255.0f ? 255.0f : wt;
                 if (kColour) {
-                    c[0] = rgb[3 * pix]; c[LF_BLOCK3] = rgb[3 * pix + 1]; c[2 * LF_BLOCK3] = rgb[3 * pix + 2];
+                    c[0] = cin[0]; c[LF_BLOCK3] = cin[1]; c[2 * LF_BLOCK3] = cin[2];
                 }
                 continue;
             }
@@ -290,7 +310,7 @@ __global__ __launch_bounds__(256) void k_integrate(MapDev m, const float *__rest
             if (kColour) {
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    const float mixed = ((float)c[k * LF_BLOCK3] * wp + (float)rgb[3 * pix + k] * wt) / (wp + wt);
+                    const float mixed = ((float)c[k * LF_BLOCK3] * wp + (float)cin[k] * wt) / (wp + wt);
                     c[k * LF_BLOCK3] = (unsigned char)(int)mixed;
                 }
             }
@@ -298,7 +318,7 @@ __global__ __launch_bounds__(256) void k_integrate(MapDev m, const float *__rest
     }
 }
 
-int integrate_grid() { return 4096; }
+int integrate_grid() { return 8192; }
 
 int check_update_args(const lf_map_params *p, const lf_map_storage *s, const float *depth, int cols, int rows,
                       const float *pose24, const lf_intrinsics *K) {
@@ -311,7 +331,7 @@ int check_update_args(const lf_map_params *p, const lf_map_storage *s, const flo
 
 int lf_update_ev(const lf_map_params *p, const lf_map_storage *s, const float *depth, const float *nmap,
                  const unsigned char *rgb, int cols, int rows, const float *pose24, const lf_intrinsics *K,
-                 hipStream_t st, bool weighted, hipEvent_t e0, hipEvent_t e1) {
+                 hipStream_t st, bool weighted, hipEvent_t e0, hipEvent_t e1, const float *samples) {
     const MapDev m = make_dev(p, s);
     dim3 g((cols + 15) / 16, (rows + 15) / 16);
     k_alloc<<<g, 256, 0, st>>>(m, depth, cols, rows, pose24, *K);
@@ -321,7 +341,11 @@ int lf_update_ev(const lf_map_params *p, const lf_map_storage *s, const float *d
     LF_LAUNCH_CHECK();
     const int G = integrate_grid();
     if (e0) LF_HIP_CHECK(hipEventRecord(e0, st));
-    if (weighted) {
+    const float4 *smp = reinterpret_cast<const float4 *>(samples);
+    if (weighted && smp) {
+        if (rgb) k_integrate<true, true, true><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K, smp);
+        else k_integrate<true, false, true><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K, smp);
+    } else if (weighted) {
         if (rgb) k_integrate<true, true><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K);
         else k_integrate<true, false><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K);
     } else {

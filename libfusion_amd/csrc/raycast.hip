@@ -215,7 +215,7 @@ LF_DEV float read_interp(const RayDev &r, BlockCache &bc, f3 p, bool &valid) {
 template <bool kCheckColour>
 __global__ __launch_bounds__(256) void k_raycast(RayDev r, float *__restrict__ vmap, unsigned char *__restrict__ rgb_out,
                                                  int cols, int rows, const float *__restrict__ pose24,
-                                                 lf_intrinsics K) {
+                                                 lf_intrinsics K, int wg_x, int wg_y) {
     // DenseMapping::raycast leaves the map untouched when nothing is visible
     // (voxel_hashing.cpp:92-93) or no rendering block was made (:105).
     if (r.counters[LF_CNT_VISIBLE] == 0u || r.counters[LF_CNT_RENDER_BLOCKS] == 0u) return;
@@ -226,7 +226,14 @@ __global__ __launch_bounds__(256) void k_raycast(RayDev r, float *__restrict__ v
     for (int i = threadIdx.x; i < LF_OCC_WORDS / 4; i += blockDim.x) s_occ[i] = occ4[i];
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tile_x = blockIdx.x * 2 + (wave & 1), tile_y = blockIdx.y * 2 + (wave >> 1);
+    // XCD-aware placement: workgroups are dealt to the 8 XCDs in turn, so
+    // workgroup g (XCD g % 8) takes the (g / 8)-th workgroup tile of band g % 8:
+    // each XCD marches a contiguous band of the image, and the blocks its rays
+    // read stay in that XCD's L2.
+    const int nwg = wg_x * wg_y, per_xcd = (nwg + kXcds - 1) / kXcds;
+    const int L = (int)(blockIdx.x % kXcds) * per_xcd + (int)(blockIdx.x / kXcds);
+    if (L >= nwg) return;
+    const int tile_x = (L % wg_x) * 2 + (wave & 1), tile_y = (L / wg_x) * 2 + (wave >> 1);
     const int x = tile_x * 8 + (lane & 7), y = tile_y * 8 + (lane >> 3);
     if (x >= cols || y >= rows) return;
     float4 *out = reinterpret_cast<float4 *>(vmap) + (size_t)y * cols + x;
@@ -315,10 +322,11 @@ int lf_raycast_ev(const lf_map_params *p, const lf_map_storage *s, float *vmap, 
     k_zrange_fill<<<std::max(1, std::min(2048, (p->num_blocks + 255) / 256)), 256, 0, st>>>(r, pose24, *K);
     LF_LAUNCH_CHECK();
     const int tx = (cols + 7) / 8, ty = (rows + 7) / 8;
-    dim3 g((tx + 1) / 2, (ty + 1) / 2);
+    const int wg_x = (tx + 1) / 2, wg_y = (ty + 1) / 2;  // 2x2 tiles per workgroup
+    const int g = kXcds * ((wg_x * wg_y + kXcds - 1) / kXcds);
     if (e0) LF_HIP_CHECK(hipEventRecord(e0, st));
-    if (check_colour) k_raycast<true><<<g, 256, 0, st>>>(r, vmap, rgb_out, cols, rows, pose24, *K);
-    else k_raycast<false><<<g, 256, 0, st>>>(r, vmap, rgb_out, cols, rows, pose24, *K);
+    if (check_colour) k_raycast<true><<<g, 256, 0, st>>>(r, vmap, rgb_out, cols, rows, pose24, *K, wg_x, wg_y);
+    else k_raycast<false><<<g, 256, 0, st>>>(r, vmap, rgb_out, cols, rows, pose24, *K, wg_x, wg_y);
     LF_LAUNCH_CHECK();
     if (e1) LF_HIP_CHECK(hipEventRecord(e1, st));
     return LF_OK;

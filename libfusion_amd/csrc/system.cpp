@@ -160,6 +160,7 @@ DeviceImage::DeviceImage(const IntrinsicMatrix &K, int num_pyr, hipStream_t stre
     BuildIntrinsicPyramid(K, K_, num_pyr);
     depth_raw_.create(K.width, K.height, 4);
     image_.create(K.width, K.height, 3);
+    samples_.create(K.width, K.height, 16);
     depth_.resize(num_pyr); intensity_.resize(num_pyr); dx_.resize(num_pyr); dy_.resize(num_pyr);
     vmap_.resize(num_pyr); nmap_.resize(num_pyr);
     for (int l = 0; l < num_pyr; ++l) {
@@ -173,6 +174,7 @@ LfImageLevels DeviceImage::levels_view() {
     LfImageLevels m = {};
     m.depth_raw = depth_raw_.ptr<float>();
     m.rgb = image_.ptr<unsigned char>();
+    m.samples = samples_.ptr<float>();
     for (int l = 0; l < levels(); ++l) {
         m.depth[l] = depth_[l].ptr<float>();
         m.intensity[l] = intensity_[l].ptr<float>();
@@ -213,7 +215,7 @@ DenseMapping::~DenseMapping() { lf_map_release(&storage_); }
 int DenseMapping::update(DeviceImage &frame, const float *pose24, bool colour, hipEvent_t e0, hipEvent_t e1) {
     return lf_update_ev(&params_, &storage_, frame.get_raw_depth().ptr<float>(), frame.get_nmap(0).ptr<float>(),
                         colour ? frame.get_image().ptr<unsigned char>() : nullptr, K_.width, K_.height, pose24, &K_,
-                        stream_, true, e0, e1);
+                        stream_, true, e0, e1, frame.get_integration_samples().ptr<float>());
 }
 
 int DenseMapping::update(const float *depth, const unsigned char *rgb, const float *pose24) {
@@ -512,7 +514,7 @@ extern "C" int lf_image_plane(lf_image *im, int level, int plane, void **out) {
     if (!im || !out) return lf_set_error(LF_ERR_ARG, "null argument to lf_image_plane");
     fusion::DeviceImage &d = *im->img;
     if (level < 0 || level >= d.levels()) return lf_set_error(LF_ERR_ARG, "bad level %d", level);
-    if ((plane == LF_PLANE_RAW_DEPTH || plane == LF_PLANE_RGB) && level != 0)
+    if ((plane == LF_PLANE_RAW_DEPTH || plane == LF_PLANE_RGB || plane == LF_PLANE_SAMPLES) && level != 0)
         return lf_set_error(LF_ERR_ARG, "plane %d exists at level 0 only", plane);
     switch (plane) {
     case LF_PLANE_DEPTH: *out = d.get_depth(level).ptr<void>(); break;
@@ -523,9 +525,23 @@ extern "C" int lf_image_plane(lf_image *im, int level, int plane, void **out) {
     case LF_PLANE_NMAP: *out = d.get_nmap(level).ptr<void>(); break;
     case LF_PLANE_RAW_DEPTH: *out = d.get_raw_depth().ptr<void>(); break;
     case LF_PLANE_RGB: *out = d.get_image().ptr<void>(); break;
+    case LF_PLANE_SAMPLES: *out = d.get_integration_samples().ptr<void>(); break;
     default: return lf_set_error(LF_ERR_ARG, "bad plane %d", plane);
     }
     return LF_OK;
+}
+
+// update(RgbdImagePtr frame) (voxel_hashing.cpp:44-65) on a caller's map: the
+// frame's raw depth for allocation, its packed integration samples (depth,
+// weight from the level-0 normals, colour) for the weighted update.
+extern "C" int lf_update_image(const lf_map_params *p, const lf_map_storage *s, lf_image *im, const float *pose24,
+                               int colour, void *stream) {
+    if (!p || !s || !im || !pose24) return lf_set_error(LF_ERR_ARG, "null argument to lf_update_image");
+    fusion::DeviceImage &d = *im->img;
+    const lf_intrinsics &K = d.intrinsics()[0];
+    return lf_update_ev(p, s, d.get_raw_depth().ptr<float>(), d.get_nmap(0).ptr<float>(),
+                        colour ? d.get_image().ptr<unsigned char>() : nullptr, K.width, K.height, pose24, &K,
+                        lf_stream(stream), true, nullptr, nullptr, d.get_integration_samples().ptr<float>());
 }
 
 // ------------------------------------------------------------ lf_system_*

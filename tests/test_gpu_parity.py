@@ -331,6 +331,37 @@ def test_device_image_passes_match_per_op_kernels():
     img.close()
 
 
+def test_update_from_image_samples_equals_plane_update():
+    """update(RgbdImagePtr) through the frame's packed integration samples
+    gives the same map, bit for bit, as lf_update_weighted on the frame's raw
+    depth, level-0 normals and colour (four fused frames, half resolution)."""
+    sc = Scene(1, 1)
+    K = half_res(sc)
+    p = capi.map_params(2 ** 18, 1 << 16, 0.01, 0.03)
+    img = capi.Image(K, 3)
+    ga, gb = capi.Map(p, K.width, K.height), capi.Map(p, K.width, K.height)
+    for f in (0, 3, 6, 9):
+        T = sc.pose(f)
+        d16, rgb = sc.render(T, K.width, K.height, K.fx, K.fy, K.cx, K.cy, noise=True, frame_id=f)
+        t_d, t_rgb, t_pose = dev(d16), dev(rgb), dev(capi.pose24(T))
+        img.upload(t_d, t_rgb, sc.depth_scale)
+        ga.update_image(img, t_pose, colour=True)
+        raw = img.plane(0, capi.PLANE_RAW_DEPTH)
+        nm = img.plane(0, capi.PLANE_NMAP)
+        col = img.plane(0, capi.PLANE_RGB)
+        gb.update_weighted(raw, nm, col, t_pose, K)
+        torch.cuda.synchronize()
+    ca, cb = ga.counters(), gb.counters()
+    assert ca[capi.CNT_ALLOCATED] == cb[capi.CNT_ALLOCATED] > 1000
+    va, vb = ga.voxels(), gb.voxels()  # pool order follows the allocation race: compare in block order
+    ia, ib = np.lexsort(va[0].T[::-1]), np.lexsort(vb[0].T[::-1])
+    for k in range(4):
+        assert np.array_equal(va[k][ia], vb[k][ib]), k
+    smp = img.plane(0, capi.PLANE_SAMPLES).cpu().numpy()
+    assert np.isfinite(smp[..., 0]).mean() > 0.5
+    img.close()
+
+
 def level_maps(sc, frame, K):
     d16, rgb = sc.render(sc.pose(frame), K.width, K.height, K.fx, K.fy, K.cx, K.cy, noise=True, frame_id=frame)
     depth = d16.astype(np.float32) * np.float32(sc.depth_scale)

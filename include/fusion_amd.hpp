@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "libfusion_amd.h"
@@ -107,6 +108,9 @@ public:
     // raycast (voxel_hashing.cpp:87-119) into a float4 vertex map.
     int raycast(float *vmap, const float *pose24_dev, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
     int reset_mapping();
+    // writeMapToDisk / readMapFromDisk (voxel_hashing.cpp:189-205): the reference's file layout.
+    int writeMapToDisk(const std::string &file_name);
+    int readMapFromDisk(const std::string &file_name);
     // fetch_mesh_* (voxel_hashing.cpp:142-187); synchronising; return triangle counts.
     size_t fetch_mesh_vertex_only(float *vertex, int max_triangles);
     size_t fetch_mesh_with_normal(float *vertex, float *normal, int max_triangles);
@@ -181,6 +185,9 @@ public:
     ~System();
     int process_images(const uint16_t *depth, const unsigned char *rgb);
     int restart();
+    // writeMapToDisk / readMapFromDisk (system.cpp:202-210)
+    int writeMapToDisk(const std::string &file_name);
+    int readMapFromDisk(const std::string &file_name);
     size_t fetch_mesh_vertex_only(float *vertex, int max_triangles);
     size_t fetch_mesh_with_normal(float *vertex, float *normal, int max_triangles);
     size_t fetch_mesh_with_colour(float *vertex, unsigned char *colour, int max_triangles);

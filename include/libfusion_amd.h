@@ -192,6 +192,23 @@ int lf_create_mesh(const lf_map_params *p, const lf_map_storage *s,
                    float *vertex_out, float *normal_out, unsigned char *colour_out,
                    int max_triangles, unsigned int *triangles_out, void *stream);
 
+/* ------------------------------------------------- checkpoint / transfer */
+
+/* MapStruct::writeToDisk via DenseMapping::writeMapToDisk (map_struct.cu:529-566;
+ * voxel_hashing.cpp:189-196): the reference's binary layout (12-byte voxels of
+ * the whole pool, 20-byte chained hash entries, heap, bucket mutexes, heap and
+ * excess counters) sized by the map's capacities, plus "<path>.txt" holding
+ * {num_blocks, num_hash_entries, num_buckets}.  LF_ERR_CAPACITY when the
+ * reference table's excess area cannot hold the map's bucket collisions. */
+int lf_map_write(const lf_map_params *p, const lf_map_storage *s, const char *path, void *stream);
+/* MapStruct::readFromDisk via DenseMapping::readMapFromDisk (map_struct.cu:568-613;
+ * voxel_hashing.cpp:198-205): resets the map and loads every live entry of a
+ * reference-layout file (sizes from "<path>.txt", else the map's capacities),
+ * compacted in pool order.  LF_ERR_CAPACITY when the pool or table is too small. */
+int lf_map_read(const lf_map_params *p, const lf_map_storage *s, const char *path, void *stream);
+/* MapStruct::copyTo (map_struct.cu:451-469): src into dst (same parameters). */
+int lf_map_copy(const lf_map_params *p, const lf_map_storage *src, const lf_map_storage *dst, void *stream);
+
 /* --------------------------------------------------------- image pipeline */
 
 /* Depth conversion of System::process_images (system.cpp:43) with the scale as a
@@ -318,6 +335,9 @@ int lf_system_fetch_mesh(lf_system *sys, float *vertex_out, float *normal_out,
 int lf_system_map(lf_system *sys, lf_map_params *p, lf_map_storage *s);
 /* Device pointers of the reference frame's level-l vmap / nmap after the frame. */
 int lf_system_maps(lf_system *sys, int level, const float **vmap, const float **nmap);
+/* System::writeMapToDisk / readMapFromDisk (system.cpp:202-210). */
+int lf_system_write_map(lf_system *sys, const char *path);
+int lf_system_read_map(lf_system *sys, const char *path);
 /* The system's HIP stream (as void*). */
 void *lf_system_stream(lf_system *sys);
 /* Event timing of the pipeline stages: stages is a bit mask, bit s brackets

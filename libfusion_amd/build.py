@@ -27,6 +27,7 @@ HIP_SOURCES = [
     "imgproc.hip",
     "reduce.hip",
     "system.cpp",
+    "map_io.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -71,6 +71,9 @@ SIGNATURES = [
     ("lf_update", I, [P, P, P, P, I, I, P, P, P]),
     ("lf_raycast", I, [P, P, P, P, I, I, P, P, I, P]),
     ("lf_create_mesh", I, [P, P, P, P, P, I, P, P]),
+    ("lf_map_write", I, [P, P, ctypes.c_char_p, P]),
+    ("lf_map_read", I, [P, P, ctypes.c_char_p, P]),
+    ("lf_map_copy", I, [P, P, P, P]),
     ("lf_depth_to_float", I, [P, P, I, I, F, P]),
     ("lf_bilateral_filter", I, [P, P, I, I, P]),
     ("lf_pyr_down_depth", I, [P, P, I, I, P]),
@@ -100,6 +103,8 @@ SIGNATURES = [
     ("lf_system_fetch_mesh", I, [P, P, P, P, I, P]),
     ("lf_system_map", I, [P, P, P]),
     ("lf_system_maps", I, [P, I, P, P]),
+    ("lf_system_write_map", I, [P, ctypes.c_char_p]),
+    ("lf_system_read_map", I, [P, ctypes.c_char_p]),
     ("lf_system_stream", P, [P]),
     ("lf_system_set_timing", I, [P, I]),
     ("lf_system_read_timing", I, [P, I, P, P]),
@@ -297,6 +302,20 @@ class Map:
         h, w = depth.shape[:2]
         check(load().lf_update_weighted(ctypes.byref(self.params), ctypes.byref(self.storage), ptr(depth), ptr(nmap),
                                         ptr(rgb), w, h, ptr(pose24_dev), ctypes.byref(K), stream_ptr()))
+
+    def write(self, path):
+        """writeMapToDisk: the reference's binary layout plus <path>.txt."""
+        check(load().lf_map_write(ctypes.byref(self.params), ctypes.byref(self.storage), str(path).encode(),
+                                  stream_ptr()))
+
+    def read(self, path):
+        """readMapFromDisk: reset, then load a reference-layout file."""
+        check(load().lf_map_read(ctypes.byref(self.params), ctypes.byref(self.storage), str(path).encode(),
+                                 stream_ptr()))
+
+    def copy_to(self, other):
+        check(load().lf_map_copy(ctypes.byref(self.params), ctypes.byref(self.storage), ctypes.byref(other.storage),
+                                 stream_ptr()))
 
     def update_image(self, image, pose24_dev, colour=True):
         """update(RgbdImagePtr): weighted update from an Image's planes (packed samples)."""

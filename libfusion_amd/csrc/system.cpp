@@ -229,6 +229,14 @@ int DenseMapping::raycast(float *vmap, const float *pose24, hipEvent_t e0, hipEv
 
 int DenseMapping::reset_mapping() { return lf_map_reset(&params_, &storage_, stream_); }
 
+int DenseMapping::writeMapToDisk(const std::string &file_name) {
+    return lf_map_write(&params_, &storage_, file_name.c_str(), stream_);
+}
+
+int DenseMapping::readMapFromDisk(const std::string &file_name) {
+    return lf_map_read(&params_, &storage_, file_name.c_str(), stream_);
+}
+
 size_t DenseMapping::fetch_mesh_vertex_only(float *vertex, int max_tri) {
     unsigned int n = 0;
     lf_create_mesh(&params_, &storage_, vertex, nullptr, nullptr, max_tri, &n, stream_);
@@ -431,6 +439,10 @@ int System::process_images(const uint16_t *depth, const unsigned char *rgb) {
     return LF_OK;
 }
 
+int System::writeMapToDisk(const std::string &file_name) { return mapping_->writeMapToDisk(file_name); }
+
+int System::readMapFromDisk(const std::string &file_name) { return mapping_->readMapFromDisk(file_name); }
+
 int System::restart() {  // system.cpp:148-157
     double T[16];
     LF_RC(get_camera_pose(T));
@@ -611,6 +623,16 @@ extern "C" int lf_system_maps(lf_system *s, int level, const float **vmap, const
 }
 
 extern "C" void *lf_system_stream(lf_system *s) { return s->sys->stream(); }
+
+extern "C" int lf_system_write_map(lf_system *s, const char *path) {
+    if (!s || !path) return lf_set_error(LF_ERR_ARG, "null argument to lf_system_write_map");
+    return s->sys->writeMapToDisk(path);
+}
+
+extern "C" int lf_system_read_map(lf_system *s, const char *path) {
+    if (!s || !path) return lf_set_error(LF_ERR_ARG, "null argument to lf_system_read_map");
+    return s->sys->readMapFromDisk(path);
+}
 
 extern "C" int lf_system_set_timing(lf_system *s, int stages) {
     s->sys->set_timing(stages);

@@ -362,6 +362,172 @@ def test_update_from_image_samples_equals_plane_update():
     img.close()
 
 
+# ---------------------------------------------------------------- checkpoints
+
+REF_VOXEL = np.dtype([("sdf", "<i2"), ("pad0", "<i2"), ("weight", "<f4"), ("rgb", "u1", 3), ("pad1", "u1")])
+REF_ENTRY = np.dtype([("ptr", "<i4"), ("offset", "<i4"), ("pos", "<i4", 3)])
+
+
+def ref_hash(x, y, z, n):
+    """computeHash (map_struct.cu:143-148) with int32 wraparound and C remainder."""
+    v = ((x * 73856093) & 0xFFFFFFFF) ^ ((y * 19349669) & 0xFFFFFFFF) ^ ((z * 83492791) & 0xFFFFFFFF)
+    v = v - (1 << 32) if v >= (1 << 31) else v
+    r = abs(v) % n * (1 if v >= 0 else -1)
+    return r + n if r < 0 else r
+
+
+def parse_ref_file(path, B, E, NB):
+    """The reference's writeToDisk layout (map_struct.cu:529-566), restated with numpy."""
+    raw = open(path, "rb").read()
+    assert len(raw) == B * 512 * 12 + E * 20 + 4 * (B + NB + 2)
+    o = 0
+    vox = np.frombuffer(raw, REF_VOXEL, B * 512, o); o += B * 512 * 12
+    tab = np.frombuffer(raw, REF_ENTRY, E, o); o += E * 20
+    heap = np.frombuffer(raw, "<i4", B, o); o += 4 * B
+    mutex = np.frombuffer(raw, "<i4", NB, o); o += 4 * NB
+    counters = np.frombuffer(raw, "<i4", 2, o)
+    size = np.frombuffer(open(str(path) + ".txt", "rb").read(), "<i4")
+    return vox, tab, heap, mutex, counters, size
+
+
+def ref_find(tab, NB, pos):
+    """findEntry (map_struct.cu:365-381): bucket head, then the excess chain."""
+    e = tab[ref_hash(*pos, NB)]
+    if e["ptr"] != -1 and tuple(e["pos"]) == pos:
+        return e
+    while e["offset"] > 0:
+        e = tab[NB + e["offset"] - 1]
+        if e["ptr"] != -1 and tuple(e["pos"]) == pos:
+            return e
+    return None
+
+
+def small_map_c0(buckets=4096, blocks=4096, voxel=0.03):
+    sc, K, _ = configs.c0()
+    p = capi.map_params(buckets, blocks, voxel, 3 * voxel)
+    T = sc.pose(0)
+    depth, nmap, rgb = frame_inputs(sc, T, K, 0)
+    g = capi.Map(p, K.width, K.height)
+    g.update_weighted(dev(depth), dev(nmap), dev(rgb), dev(capi.pose24(T)), K)
+    torch.cuda.synchronize()
+    return sc, K, p, T, g
+
+
+def same_volume(a, b):
+    va, vb = a.voxels(), b.voxels()
+    ia, ib = np.lexsort(va[0].T[::-1]), np.lexsort(vb[0].T[::-1])
+    return len(va[0]) == len(vb[0]) and all(np.array_equal(va[k][ia], vb[k][ib]) for k in range(4))
+
+
+def raycast_equal(a, b, K, T):
+    """Raycast both maps at T after an update with an all-NaN depth frame (it
+    allocates and fuses nothing but selects the visible blocks, which a map
+    read from disk does not carry)."""
+    pose = dev(capi.pose24(T))
+    nan = torch.full((K.height, K.width), float("nan"), device="cuda")
+    nrm = torch.full((K.height, K.width, 4), float("nan"), device="cuda")
+    outs = []
+    for m in (a, b):
+        m.update_weighted(nan, nrm, None, pose, K)
+        vm = torch.full((K.height, K.width, 4), float("nan"), device="cuda")
+        m.raycast(vm, pose, K, check_colour=True)
+        torch.cuda.synchronize()
+        outs.append(vm.cpu().numpy())
+    return np.array_equal(outs[0], outs[1], equal_nan=True), np.isfinite(outs[0][..., 0]).mean()
+
+
+def test_checkpoint_written_in_the_reference_layout(tmp_path):
+    """writeMapToDisk: the file parses as the reference's arrays; every block is
+    found by the reference's findEntry at its pool offset with its voxels;
+    heap, counters and MapSize are what the reference keeps after n allocations."""
+    sc, K, p, T, g = small_map_c0()
+    path = tmp_path / "map.bin"
+    g.write(path)
+    B, E, NB = p.num_blocks, p.num_entries, p.num_buckets
+    vox, tab, heap, mutex, counters, size = parse_ref_file(path, B, E, NB)
+    blocks, sdf, w, rgb = g.voxels()
+    n = len(blocks)
+    assert n > 300
+    chained = 0
+    for b, pos in enumerate(map(tuple, blocks.tolist())):
+        e = ref_find(tab, NB, pos)
+        assert e is not None and e["ptr"] == 512 * b, pos
+        chained += int(ref_hash(*pos, NB) != int(np.nonzero(tab["ptr"] == 512 * b)[0][0]))
+    assert chained > 20  # the excess chains are exercised
+    assert (tab["ptr"] != -1).sum() == n
+    v = vox[: 512 * n].reshape(n, 512)
+    assert np.array_equal(v["weight"], w) and np.array_equal(v["sdf"], sdf)
+    assert np.array_equal(v["rgb"].transpose(0, 2, 1), rgb)
+    assert not vox[512 * n:]["weight"].any()
+    assert np.array_equal(heap, B - 1 - np.arange(B)) and not mutex.any()
+    assert counters[0] == B - 1 - n and counters[1] == chained + 1
+    assert size.tolist() == [B, E, NB]
+
+
+def test_checkpoint_round_trip_and_copy(tmp_path):
+    """readMapFromDisk of a written map and copyTo give the same volume and the
+    same raycast, bit for bit."""
+    sc, K, p, T, g = small_map_c0()
+    path = tmp_path / "map.bin"
+    g.write(path)
+    h = capi.Map(p, K.width, K.height)
+    h.read(path)
+    torch.cuda.synchronize()
+    assert h.counters()[capi.CNT_ALLOCATED] == g.counters()[capi.CNT_ALLOCATED]
+    assert same_volume(g, h)
+    eq, hit = raycast_equal(g, h, K, T)
+    assert eq and hit > 0.5
+    c = capi.Map(p, K.width, K.height)
+    g.copy_to(c)
+    torch.cuda.synchronize()
+    assert same_volume(g, c)
+
+
+def test_checkpoint_reads_a_reference_built_table(tmp_path):
+    """A file built by a restatement of the reference's createBlock (blocks
+    inserted in shuffled order, shuffled heap slots, chained excess entries)
+    loads into the same volume."""
+    sc, K, p, T, g = small_map_c0()
+    blocks, sdf, w, rgb = g.voxels()
+    n = len(blocks)
+    B, E, NB = p.num_blocks, p.num_entries, p.num_buckets
+    rng = np.random.default_rng(3)
+    order = rng.permutation(n)
+    slots = rng.permutation(B)[:n]  # pool block of each inserted block
+    tab = np.zeros(E, REF_ENTRY)
+    tab["ptr"] = -1
+    tab["offset"] = -1
+    vox = np.zeros(B * 512, REF_VOXEL)
+    excess = 1  # createBlock's counter after its first excess entry
+    for k in order:
+        pos = tuple(int(c) for c in blocks[k])
+        i = ref_hash(*pos, NB)
+        if tab[i]["ptr"] == -1:
+            tab[i] = (512 * slots[k], tab[i]["offset"], pos)
+        else:
+            while tab[i]["offset"] > 0:
+                i = NB + tab[i]["offset"] - 1
+            tab[NB + excess - 1] = (512 * slots[k], 0, pos)
+            tab[i]["offset"] = excess
+            excess += 1
+        v = vox[512 * slots[k]: 512 * slots[k] + 512]
+        v["sdf"] = sdf[k]
+        v["weight"] = w[k]
+        v["rgb"] = rgb[k].T
+    path = tmp_path / "ref.bin"
+    with open(path, "wb") as f:
+        f.write(vox.tobytes() + tab.tobytes() + np.arange(B, dtype="<i4").tobytes() +
+                np.zeros(NB, "<i4").tobytes() + np.array([B - 1 - n, excess], "<i4").tobytes())
+    with open(str(path) + ".txt", "wb") as f:
+        f.write(np.array([B, E, NB], "<i4").tobytes())
+    h = capi.Map(p, K.width, K.height)
+    h.read(path)
+    torch.cuda.synchronize()
+    assert same_volume(g, h)
+    eq, hit = raycast_equal(g, h, K, T)
+    assert eq and hit > 0.5
+
+
 def level_maps(sc, frame, K):
     d16, rgb = sc.render(sc.pose(frame), K.width, K.height, K.fx, K.fy, K.cx, K.cy, noise=True, frame_id=frame)
     depth = d16.astype(np.float32) * np.float32(sc.depth_scale)

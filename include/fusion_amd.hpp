@@ -16,6 +16,7 @@
 
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "libfusion_amd.h"
@@ -72,6 +73,14 @@ public:
     int upload(const uint16_t *depth_u16, const unsigned char *rgb, float depth_scale);
     // DeviceImage::resize_device_map (device_image.cpp:188-199).
     int resize_device_map();
+    // Pyramids from given level-0 device planes (build_pyramid.cpp:7-76), as
+    // DenseTracking::set_source_* / set_reference_* build them:
+    int set_vmap(const float *vmap);                 // vertices, then every level's vertices and normals
+    int set_intensity(const float *intensity);       // intensity pyramid and derivatives
+    int set_image(const unsigned char *rgb);         // grey conversion, then set_intensity
+    int set_depth(const float *depth);               // bilateral filter and depth pyramid
+    // exchange the intensity pyramids of two frames (gradients stay)
+    void swap_intensity(DeviceImage &o) { std::swap(intensity_, o.intensity_); }
     const GpuImage &get_raw_depth() const { return depth_raw_; }
     const GpuImage &get_image() const { return image_; }
     // level-0 integration samples: float4 {depth or NaN, weight, colour bits, 0} per pixel
@@ -89,6 +98,7 @@ private:
     LfImageLevels levels_view();
     std::vector<IntrinsicMatrix> K_;
     hipStream_t stream_;
+    int intensity_pyramid();
     GpuImage depth_raw_, image_, samples_;
     std::vector<GpuImage> depth_, intensity_, dx_, dy_, vmap_, nmap_;
 };
@@ -108,6 +118,8 @@ public:
     // raycast (voxel_hashing.cpp:87-119) into a float4 vertex map.
     int raycast(float *vmap, const float *pose24_dev, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
     int reset_mapping();
+    // Remove a frame fused with update(frame, pose, colour) (inverse weighted update).
+    int deintegrate(DeviceImage &frame, const float *pose24_dev, bool colour);
     // writeMapToDisk / readMapFromDisk (voxel_hashing.cpp:189-205): the reference's file layout.
     int writeMapToDisk(const std::string &file_name);
     int readMapFromDisk(const std::string &file_name);
@@ -143,17 +155,42 @@ struct TrackingContext {  // icp_tracker.h:20-26
 class DenseTracking {
 public:
     explicit DenseTracking(hipStream_t stream);
+    // DenseTracking(K, NUM_PYR) (icp_tracker.cpp:8-21): owns source and reference
+    // pyramids for the set_* / compute_transform(ctx) interface.
+    DenseTracking(const IntrinsicMatrix &K, int num_pyr, hipStream_t stream);
     ~DenseTracking();
+    DenseTracking(const DenseTracking &) = delete;
+    DenseTracking &operator=(const DenseTracking &) = delete;
+    // compute_transform(reference, current, ctx) (icp_tracker.cpp:23-156)
     int compute_transform(DeviceImage &reference, DeviceImage &current, const TrackingContext &c);
+    // compute_transform(ctx) (icp_tracker.cpp:210-314) on the set_* pyramids
+    // (source = current frame, reference = last); success is unconditional.
+    int compute_transform(const TrackingContext &c);
+    // set_source_* / set_reference_* (icp_tracker.cpp:164-208) from device planes
+    // at level-0 size: vmap float4, image RGB u8, depth and intensity float.
+    int set_source_vmap(const float *vmap) { return src_->set_vmap(vmap); }
+    int set_source_image(const unsigned char *rgb) { return src_->set_image(rgb); }
+    int set_source_depth(const float *depth) { return src_->set_depth(depth); }
+    int set_source_intensity(const float *intensity) { return src_->set_intensity(intensity); }
+    int set_reference_image(const unsigned char *rgb) { return ref_->set_image(rgb); }
+    int set_reference_intensity(const float *intensity) { return ref_->set_intensity(intensity); }
+    int set_reference_vmap(const float *vmap) { return ref_->set_vmap(vmap); }
+    // swap_intensity_pyr (icp_tracker.cpp:158-162)
+    void swap_intensity_pyr();
+    DeviceImage *source() { return src_.get(); }
+    DeviceImage *reference() { return ref_.get(); }
     LfTrackState *state() const { return state_; }
     // Synchronising read of the last result (update = estimate^-1).
     TrackingResult result();
     float icp_weight = 1e6f;  // icp_tracker.cpp:89-90
 
 private:
+    int enqueue(DeviceImage &reference, DeviceImage &current, const TrackingContext &c, bool context_mode);
     hipStream_t stream_;
     LfTrackState *state_ = nullptr;
     float *scratch_ = nullptr;
+    bool context_mode_ = false;
+    std::unique_ptr<DeviceImage> src_, ref_;
 };
 
 // DenseOdometry (rgbd_odometry.h:13-41).

@@ -269,6 +269,18 @@ int lf_image_upload(lf_image *img, const uint16_t *depth, const unsigned char *r
 int lf_image_resize_device_map(lf_image *img);
 /* Device pointer of a plane (LF_PLANE_*) of a level; rows are packed. */
 int lf_image_plane(lf_image *img, int level, int plane, void **out);
+/* De-integration (not in the reference; deleteBlock, map_struct.cu:327-363, has
+ * no callers): the inverse of lf_update_weighted for the same frame and pose,
+ * over the same voxels (no block is allocated).  weight' = weight - w_frame,
+ * sdf and colour un-averaged; a weight that reaches 0 returns the voxel to
+ * unseen.  Exact up to int16 sdf / uint8 colour rounding except where the
+ * weight saturated at 255. */
+int lf_deintegrate_weighted(const lf_map_params *p, const lf_map_storage *s, const float *depth,
+                            const float *nmap, const unsigned char *rgb, int cols, int rows,
+                            const float *pose24, const lf_intrinsics *K, void *stream);
+/* De-integration of a DeviceImage frame (the inverse of lf_update_image). */
+int lf_deintegrate_image(const lf_map_params *p, const lf_map_storage *s, lf_image *img, const float *pose24,
+                         int colour, void *stream);
 /* update(RgbdImagePtr frame) (voxel_hashing.cpp:44-65) into a caller's map:
  * lf_update_weighted on the frame's raw depth, level-0 normals and (colour != 0)
  * colour, read through the frame's packed integration samples.  Same result
@@ -277,6 +289,48 @@ int lf_update_image(const lf_map_params *p, const lf_map_storage *s, lf_image *i
                     int colour, void *stream);
 
 /* ------------------------------------------------------------- tracking */
+
+/* DenseTracking(K, NUM_PYR) (icp_tracker.h:28-88): a tracker that owns source
+ * (current) and reference pyramids, for re-tracking keyframes against
+ * raycast or stored maps (keyframe_graph.cpp:162-229). */
+typedef struct lf_tracker lf_tracker;
+
+/* TrackingResult (icp_tracker.h:13-18): update = estimate^-1, 3x4 row-major. */
+typedef struct lf_tracking_result {
+    int success;
+    float icp_error;
+    double update[12];
+} lf_tracking_result;
+
+int lf_tracker_create(const lf_intrinsics *K, int num_pyr, void *stream, lf_tracker **out);
+int lf_tracker_destroy(lf_tracker *t);
+/* set_source_* / set_reference_* (icp_tracker.cpp:164-208): level-0 device
+ * planes; vmap float4 (pyrDownVMap pyramid and normals, build_pyramid.cpp:55-76),
+ * image RGB u8 (grey, pyrDown pyramid, derivatives), depth float (bilateral
+ * and depth pyramid), intensity float. */
+int lf_tracker_set_source_vmap(lf_tracker *t, const float *vmap);
+int lf_tracker_set_source_image(lf_tracker *t, const unsigned char *rgb);
+int lf_tracker_set_source_depth(lf_tracker *t, const float *depth);
+int lf_tracker_set_source_intensity(lf_tracker *t, const float *intensity);
+int lf_tracker_set_reference_image(lf_tracker *t, const unsigned char *rgb);
+int lf_tracker_set_reference_intensity(lf_tracker *t, const float *intensity);
+int lf_tracker_set_reference_vmap(lf_tracker *t, const float *vmap);
+/* swap_intensity_pyr (icp_tracker.cpp:158-162). */
+int lf_tracker_swap_intensity(lf_tracker *t);
+/* compute_transform(ctx) (icp_tracker.cpp:210-314) on the set_* pyramids:
+ * iterations[level] Gauss-Newton steps per level (num_levels <= NUM_PYR),
+ * initial estimate 3x4 (null: identity).  Synchronising.  As in the
+ * reference, success is always 1 and icp_error is last_icp_error. */
+int lf_tracker_compute_transform(lf_tracker *t, const double *initial_estimate12, const int *iterations,
+                                 int num_levels, lf_tracking_result *out);
+/* compute_transform(reference, current, ctx) (icp_tracker.cpp:23-156) on two
+ * frames: success requires finite errors and |log(T^-1 T0)| <= 0.1.
+ * Synchronising. */
+int lf_tracker_compute_transform_images(lf_tracker *t, lf_image *reference, lf_image *current,
+                                        const double *initial_estimate12, const int *iterations, int num_levels,
+                                        lf_tracking_result *out);
+/* A plane (LF_PLANE_*) of the tracker's source (source != 0) or reference pyramid. */
+int lf_tracker_plane(lf_tracker *t, int source, int level, int plane, void **out);
 
 /* Scratch for the reductions: lf_reduce_scratch_floats() floats of device memory. */
 size_t lf_reduce_scratch_floats(void);

@@ -45,6 +45,10 @@ class LfMapStorage(ctypes.Structure):
                 ("tiles_y", ctypes.c_int)]
 
 
+class LfTrackingResult(ctypes.Structure):
+    _fields_ = [("success", ctypes.c_int), ("icp_error", ctypes.c_float), ("update", ctypes.c_double * 12)]
+
+
 class LfSystemConfig(ctypes.Structure):
     _fields_ = [("K", LfIntrinsics), ("map", LfMapParams), ("num_pyr", ctypes.c_int),
                 ("iterations", ctypes.c_int * 8), ("depth_scale", ctypes.c_float),
@@ -89,6 +93,21 @@ SIGNATURES = [
     ("lf_image_resize_device_map", I, [P]),
     ("lf_image_plane", I, [P, I, I, P]),
     ("lf_update_image", I, [P, P, P, P, I, P]),
+    ("lf_deintegrate_weighted", I, [P, P, P, P, P, I, I, P, P, P]),
+    ("lf_deintegrate_image", I, [P, P, P, P, I, P]),
+    ("lf_tracker_create", I, [P, I, P, P]),
+    ("lf_tracker_destroy", I, [P]),
+    ("lf_tracker_set_source_vmap", I, [P, P]),
+    ("lf_tracker_set_source_image", I, [P, P]),
+    ("lf_tracker_set_source_depth", I, [P, P]),
+    ("lf_tracker_set_source_intensity", I, [P, P]),
+    ("lf_tracker_set_reference_image", I, [P, P]),
+    ("lf_tracker_set_reference_intensity", I, [P, P]),
+    ("lf_tracker_set_reference_vmap", I, [P, P]),
+    ("lf_tracker_swap_intensity", I, [P]),
+    ("lf_tracker_compute_transform", I, [P, P, P, I, P]),
+    ("lf_tracker_compute_transform_images", I, [P, P, P, P, P, I, P]),
+    ("lf_tracker_plane", I, [P, I, I, I, P]),
     ("lf_reduce_scratch_floats", ctypes.c_size_t, []),
     ("lf_icp_reduce", I, [P, P, P, P, I, I, P, P, P, P, P]),
     ("lf_rgb_step", I, [P, P, P, P, P, I, I, F, P, P, P, P, P]),
@@ -264,6 +283,57 @@ class Image:
         return t
 
 
+class Tracker:
+    """DenseTracking(K, NUM_PYR) with source / reference pyramids (lf_tracker_*), on the current stream."""
+
+    def __init__(self, K, levels):
+        self.K = [K]
+        for _ in range(levels - 1):
+            self.K.append(pyr_down(self.K[-1]))
+        self.h = ctypes.c_void_p()
+        check(load().lf_tracker_create(ctypes.byref(K), int(levels), stream_ptr(), ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            load().lf_tracker_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, which, plane):
+        """which: set_source_vmap, set_source_image, ... (a device tensor at level-0 size)."""
+        check(getattr(load(), "lf_tracker_" + which)(self.h, ptr(plane)))
+
+    def swap_intensity(self):
+        check(load().lf_tracker_swap_intensity(self.h))
+
+    @staticmethod
+    def _result(r):
+        T = np.eye(4)
+        T[:3, :] = np.array(r.update).reshape(3, 4)
+        return bool(r.success), float(r.icp_error), T
+
+    def compute_transform(self, iterations, init=None):
+        """compute_transform(ctx): (success, icp_error, update 4x4)."""
+        r = LfTrackingResult()
+        its = (ctypes.c_int * len(iterations))(*iterations)
+        ini = None if init is None else (ctypes.c_double * 12)(*np.asarray(init, np.float64)[:3, :].ravel())
+        check(load().lf_tracker_compute_transform(self.h, ini, its, len(iterations), ctypes.byref(r)))
+        return self._result(r)
+
+    def compute_transform_images(self, reference, current, iterations, init=None):
+        r = LfTrackingResult()
+        its = (ctypes.c_int * len(iterations))(*iterations)
+        ini = None if init is None else (ctypes.c_double * 12)(*np.asarray(init, np.float64)[:3, :].ravel())
+        check(load().lf_tracker_compute_transform_images(self.h, reference.h, current.h, ini, its, len(iterations),
+                                                         ctypes.byref(r)))
+        return self._result(r)
+
+
 class Map:
     """A device voxel-hash map whose arrays are torch tensors (layout of lf_map_storage)."""
 
@@ -321,6 +391,16 @@ class Map:
         """update(RgbdImagePtr): weighted update from an Image's planes (packed samples)."""
         check(load().lf_update_image(ctypes.byref(self.params), ctypes.byref(self.storage), image.h,
                                      ptr(pose24_dev), 1 if colour else 0, stream_ptr()))
+
+    def deintegrate_image(self, image, pose24_dev, colour=True):
+        """Remove a frame fused with update_image (inverse weighted update)."""
+        check(load().lf_deintegrate_image(ctypes.byref(self.params), ctypes.byref(self.storage), image.h,
+                                          ptr(pose24_dev), 1 if colour else 0, stream_ptr()))
+
+    def deintegrate_weighted(self, depth, nmap, rgb, pose24_dev, K):
+        h, w = depth.shape[:2]
+        check(load().lf_deintegrate_weighted(ctypes.byref(self.params), ctypes.byref(self.storage), ptr(depth),
+                                             ptr(nmap), ptr(rgb), w, h, ptr(pose24_dev), ctypes.byref(K), stream_ptr()))
 
     def update(self, depth, rgb, pose24_dev, K):
         h, w = depth.shape[:2]

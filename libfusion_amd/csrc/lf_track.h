@@ -13,6 +13,7 @@ struct LfTrackState {
     double init[12];     // initial estimate (identity in DenseOdometry)
     float pose_f[12];    // float 3x4 of est, read by the next reduction
     float icp_error, rgb_error, stddev;
+    float last_icp_error;  // icp error before the last step (last_icp_error of compute_transform(ctx))
     int icp_count, rgb_count;
     int active;          // 0 once the current level broke out of its loop
     int invalid;         // an error became NaN this frame
@@ -22,7 +23,7 @@ struct LfTrackState {
     // odometry chain (DenseOdometry::trackFrame, rgbd_odometry.cpp:18-48)
     double pose[12];     // T_world_camera of the current frame
     float pose24[24];    // float T_world_camera and T_camera_world for the mapping kernels
-    int pad[4];
+    int pad[3];
 };
 
 // Per-level inputs of one tracking iteration.
@@ -42,6 +43,8 @@ struct LfStepFlags {
     int first_of_frame;  // the estimate starts at init (replaces lf_track_begin)
     int last_level;      // a step of the finest level: breaking out of its loop ends the frame
     int last_of_frame;   // the frame's last scheduled step: finish after it
+    int context_mode;    // compute_transform(ctx) (icp_tracker.cpp:210-314): NaN errors are not
+                         // flagged and the finish always succeeds with update = estimate^-1
     double init[12];     // initial estimate, 3x4 row-major (first_of_frame)
 };
 

@@ -152,6 +152,16 @@ LF_DEV unsigned int key_mix(unsigned long long k) {
     return (unsigned int)k;
 }
 
+// Mark an existing block as touched by the current frame (no insertion).
+LF_DEV void touch_block(const MapDev &m, int bx, int by, int bz) {
+    const int b = find_block(m.keys, m.slot_block, bx, by, bz, m.num_buckets, m.num_entries);
+    if (b >= 0) m.slot_flag[reinterpret_cast<const int4 *>(m.block_pos)[b].w] = 1;
+}
+
+// kInsert: create_blocks_kernel.  !kInsert (de-integration): the same band
+// walk only marks the blocks that exist, so the visible list of a frame can be
+// rebuilt without allocating.
+template <bool kInsert>
 __global__ __launch_bounds__(256) void k_alloc(MapDev m, const float *__restrict__ depth, int cols, int rows,
                                                const float *__restrict__ pose24, lf_intrinsics K) {
     __shared__ unsigned long long s_set[kDedup];
@@ -169,11 +179,17 @@ __global__ __launch_bounds__(256) void k_alloc(MapDev m, const float *__restrict
         unsigned int h = key_mix(key) & (kDedup - 1);
         for (int probe = 0; probe < kDedup; ++probe) {
             unsigned long long old = atomicCAS(&s_set[h], kEmptyKey, key);
-            if (old == kEmptyKey) { insert_block(m, bx, by, bz); return; }
+            if (old == kEmptyKey) {
+                if (kInsert) insert_block(m, bx, by, bz);
+                else touch_block(m, bx, by, bz);
+                return;
+            }
             if (old == key) return;  // another lane of this tile owns the request
             h = (h + 1) & (kDedup - 1);
         }
-        insert_block(m, bx, by, bz);  // LDS set full: go straight to the table
+        // LDS set full: go straight to the table
+        if (kInsert) insert_block(m, bx, by, bz);
+        else touch_block(m, bx, by, bz);
     });
 }
 
@@ -318,6 +334,82 @@ __global__ __launch_bounds__(256) void k_integrate(MapDev m, const float *__rest
     }
 }
 
+// De-integration: the inverse of the weighted update for one frame, over the
+// same voxels (the selection depends only on depth and pose).  With w the
+// voxel's weight and w_f the frame's: w' = w - w_f, sdf' = (sdf w - sdf_f w_f) / w',
+// colour likewise; w' <= 0 returns the voxel to unseen.  Exact up to the int16
+// sdf and uint8 colour rounding, except where the weight saturated at 255.
+// Not in the reference (deleteBlock, map_struct.cu:327-363, has no callers).
+template <bool kColour, bool kPacked>
+__global__ __launch_bounds__(256) void k_deintegrate(MapDev m, const float *__restrict__ depth,
+                                                     const float *__restrict__ nmap,
+                                                     const unsigned char *__restrict__ rgb, int cols, int rows,
+                                                     const float *__restrict__ pose24, lf_intrinsics K,
+                                                     const float4 *__restrict__ samples) {
+    float Tcw[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) Tcw[i] = pose24[12 + i];
+    const int nvis = (int)m.counters[LF_CNT_VISIBLE];
+    const float vs = m.voxel_size, trunc = m.truncation, inv_trunc = 1.0f / trunc;
+    const float zmin = m.zmin_update, zmax = m.zmax_update;
+    const int t = threadIdx.x;
+    const int lx = t & 7, ly = (t >> 3) & 7, lz0 = t >> 6;
+    for (int i = blockIdx.x; i < nvis; i += gridDim.x) {
+        const int b = m.visible[i];
+        const int4 p = reinterpret_cast<const int4 *>(m.block_pos)[b];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int lz = lz0 + 4 * h;
+            const float wx = (float)(p.x * 8 + lx) * vs, wy = (float)(p.y * 8 + ly) * vs, wz = (float)(p.z * 8 + lz) * vs;
+            const f3 pt = xform(Tcw, f3{wx, wy, wz});
+            const int u = proj_floor(K.fx * pt.x / pt.z + K.cx + 0.5f);
+            const int v = proj_floor(K.fy * pt.y / pt.z + K.cy + 0.5f);
+            if (u < 0 || v < 0 || u > cols - 1 || v > rows - 1) continue;
+            const size_t pix = (size_t)v * cols + u;
+            float dist, wt;
+            unsigned char cin[3] = {0, 0, 0};
+            if (kPacked) {
+                const float4 smp = samples[pix];
+                dist = smp.x;
+                wt = smp.y;
+                const unsigned int c = __float_as_uint(smp.z);
+                cin[0] = (unsigned char)c; cin[1] = (unsigned char)(c >> 8); cin[2] = (unsigned char)(c >> 16);
+                if (isnan(dist) || dist > zmax || dist < zmin) continue;
+            } else {
+                dist = depth[pix];
+                const float nx = nmap[4 * pix], nz = nmap[4 * pix + 2];
+                if (isnan(dist) || isnan(nx) || dist > zmax || dist < zmin) continue;
+                wt = fabsf(sinf(nz)) / (dist * dist);
+                if (kColour) { cin[0] = rgb[3 * pix]; cin[1] = rgb[3 * pix + 1]; cin[2] = rgb[3 * pix + 2]; }
+            }
+            float sdf = dist - pt.z;
+            if (sdf < -trunc) continue;
+            sdf = fminf(1.0f, sdf * inv_trunc);
+            const size_t vi = (size_t)b * LF_BLOCK3 + (size_t)(t + 256 * h);
+            const float w0 = m.weight[vi];
+            if (w0 == 0.0f) continue;  // nothing fused here
+            unsigned char *c = m.rgb + (size_t)b * (3 * LF_BLOCK3) + (size_t)(t + 256 * h);
+            const float wr = w0 - wt;
+            if (!(wr > 0.0f)) {
+                m.sdf[vi] = kSdfUnseen;
+                m.weight[vi] = 0.0f;
+                if (kColour) { c[0] = 0; c[LF_BLOCK3] = 0; c[2 * LF_BLOCK3] = 0; }
+                continue;
+            }
+            const float sr = (unpack_sdf(m.sdf[vi]) * w0 - sdf * wt) / wr;
+            m.sdf[vi] = pack_sdf(fmaxf(-1.0f, fminf(1.0f, sr)));
+            m.weight[vi] = wr;
+            if (kColour) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const float cr = ((float)c[k * LF_BLOCK3] * w0 - (float)cin[k] * wt) / wr;
+                    c[k * LF_BLOCK3] = (unsigned char)(int)fmaxf(0.0f, fminf(255.0f, cr));
+                }
+            }
+        }
+    }
+}
+
 int integrate_grid() { return 8192; }
 
 int check_update_args(const lf_map_params *p, const lf_map_storage *s, const float *depth, int cols, int rows,
@@ -334,7 +426,7 @@ int lf_update_ev(const lf_map_params *p, const lf_map_storage *s, const float *d
                  hipStream_t st, bool weighted, hipEvent_t e0, hipEvent_t e1, const float *samples) {
     const MapDev m = make_dev(p, s);
     dim3 g((cols + 15) / 16, (rows + 15) / 16);
-    k_alloc<<<g, 256, 0, st>>>(m, depth, cols, rows, pose24, *K);
+    k_alloc<true><<<g, 256, 0, st>>>(m, depth, cols, rows, pose24, *K);
     LF_LAUNCH_CHECK();
     int nb = std::min(2048, std::max(1, (p->num_blocks + 255) / 256));
     k_visible<<<nb, 256, 0, st>>>(m, pose24, *K, cols, rows);
@@ -355,6 +447,38 @@ int lf_update_ev(const lf_map_params *p, const lf_map_storage *s, const float *d
     LF_LAUNCH_CHECK();
     if (e1) LF_HIP_CHECK(hipEventRecord(e1, st));
     return LF_OK;
+}
+
+int lf_deintegrate_ev(const lf_map_params *p, const lf_map_storage *s, const float *depth, const float *nmap,
+                      const unsigned char *rgb, int cols, int rows, const float *pose24, const lf_intrinsics *K,
+                      hipStream_t st, const float *samples) {
+    const MapDev m = make_dev(p, s);
+    dim3 g((cols + 15) / 16, (rows + 15) / 16);
+    k_alloc<false><<<g, 256, 0, st>>>(m, depth, cols, rows, pose24, *K);
+    LF_LAUNCH_CHECK();
+    int nb = std::min(2048, std::max(1, (p->num_blocks + 255) / 256));
+    k_visible<<<nb, 256, 0, st>>>(m, pose24, *K, cols, rows);
+    LF_LAUNCH_CHECK();
+    const int G = integrate_grid();
+    const float4 *smp = reinterpret_cast<const float4 *>(samples);
+    if (smp) {
+        if (rgb) k_deintegrate<true, true><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K, smp);
+        else k_deintegrate<false, true><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K, smp);
+    } else {
+        if (rgb) k_deintegrate<true, false><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K, smp);
+        else k_deintegrate<false, false><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K, smp);
+    }
+    LF_LAUNCH_CHECK();
+    return LF_OK;
+}
+
+extern "C" int lf_deintegrate_weighted(const lf_map_params *p, const lf_map_storage *s, const float *depth,
+                                       const float *nmap, const unsigned char *rgb, int cols, int rows,
+                                       const float *pose24, const lf_intrinsics *K, void *stream) {
+    int rc = check_update_args(p, s, depth, cols, rows, pose24, K);
+    if (rc) return rc;
+    if (!nmap) return lf_set_error(LF_ERR_ARG, "deintegrate_weighted needs a normal map");
+    return lf_deintegrate_ev(p, s, depth, nmap, rgb, cols, rows, pose24, K, lf_stream(stream), nullptr);
 }
 
 extern "C" int lf_update_weighted(const lf_map_params *p, const lf_map_storage *s, const float *depth,

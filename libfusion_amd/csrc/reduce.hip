@@ -53,6 +53,7 @@ struct ReduceArgs {
     int first_of_frame;    // tracking step: the estimate starts at init
     int last_level;        // tracking step: the finest level
     int last_of_frame;     // tracking step: the frame's last scheduled step
+    int context_mode;      // tracking step: compute_transform(ctx) semantics
     double init[12];       // tracking step: initial estimate (first_of_frame)
 };
 
@@ -302,7 +303,8 @@ LF_DEV void unpack29(const float *s, float *A, float *b) {
 
 // One Gauss-Newton step of compute_transform (icp_tracker.cpp:50-139), run by
 // a single lane after the sums are final.
-LF_DEV void gn_step(LfTrackState &st, const float *si, const float *sr, int first_iter, float icp_weight) {
+LF_DEV void gn_step(LfTrackState &st, const float *si, const float *sr, int first_iter, float icp_weight,
+                    bool context_mode) {
     if (first_iter) {
         st.icp_error = FLT_MAX;
         st.rgb_error = FLT_MAX;
@@ -313,6 +315,7 @@ LF_DEV void gn_step(LfTrackState &st, const float *si, const float *sr, int firs
     double last[12];
     for (int k = 0; k < 12; ++k) last[k] = st.est[k];
     const float last_icp = st.icp_error, last_rgb = st.rgb_error;
+    st.last_icp_error = last_icp;
     st.stddev = sqrtf(sr[27] / (sr[28] - 6.0f));
     float Ai[36], bi[6], Ar[36], br[6];
     unpack29(si, Ai, bi);
@@ -328,7 +331,7 @@ LF_DEV void gn_step(LfTrackState &st, const float *si, const float *sr, int firs
     st.iterations++;
     bool stop = false;
     float icp_error = sqrtf(si[27]) / si[28];
-    if (isnan(icp_error)) {
+    if (isnan(icp_error) && !context_mode) {
         st.invalid = 1;
         stop = true;
     } else if (icp_error > last_icp) {
@@ -345,7 +348,7 @@ LF_DEV void gn_step(LfTrackState &st, const float *si, const float *sr, int firs
     st.icp_error = icp_error;
     if (!stop) {
         float rgb_error = sqrtf(sr[27]) / sr[28];
-        if (isnan(rgb_error)) {
+        if (isnan(rgb_error) && !context_mode) {
             st.invalid = 1;
             stop = true;
         } else if (rgb_error > last_rgb) {
@@ -489,11 +492,12 @@ __global__ __launch_bounds__(kThreads) void k_reduce_final(ReduceArgs a, int G) 
 // End of DenseTracking::compute_transform (icp_tracker.cpp:143-155) and the
 // odometry chain of DenseOdometry::trackFrame (rgbd_odometry.cpp:39): success
 // test, update = est^-1 (or identity), pose = pose * update, pose24 refreshed.
-LF_DEV void track_finish(LfTrackState &st) {
+LF_DEV void track_finish(LfTrackState &st, bool context_mode = false) {
     double inv_est[12], D[12];
     m34_inv(st.est, inv_est);
     m34_mul(inv_est, st.init, D);
-    const bool ok = !st.invalid && !(se3_log_norm(D) > 0.1);
+    // compute_transform(ctx) reports success unconditionally (icp_tracker.cpp:308-313)
+    const bool ok = context_mode || (!st.invalid && !(se3_log_norm(D) > 0.1));
     st.success = ok ? 1 : 0;
     double upd[12];
     if (ok) for (int k = 0; k < 12; ++k) upd[k] = inv_est[k];
@@ -573,10 +577,10 @@ __global__ __launch_bounds__(kThreads) void k_track_final(ReduceArgs a, int G) {
             st.invalid = 0;
             st.iterations = 0;
         }
-        gn_step(st, s_sum, s_sum + 29, a.first_iter, a.icp_weight);
+        gn_step(st, s_sum, s_sum + 29, a.first_iter, a.icp_weight, a.context_mode != 0);
         // the frame ends here when the estimate became invalid, when the
         // finest level broke out of its loop, or after the frame's last step
-        if (st.invalid || (a.last_level && !st.active) || a.last_of_frame) track_finish(st);
+        if (st.invalid || (a.last_level && !st.active) || a.last_of_frame) track_finish(st, a.context_mode != 0);
     }
 }
 
@@ -652,6 +656,7 @@ int lf_track_iteration(const LfTrackLevel &lv, float *scratch, LfTrackState *st,
     a.first_of_frame = f.first_of_frame;
     a.last_level = f.last_level;
     a.last_of_frame = f.last_of_frame;
+    a.context_mode = f.context_mode;
     for (int k = 0; k < 12; ++k) a.init[k] = f.init[k];
     const int G = grid_for(lv);
     k_track_step<<<G, kThreads, 0, stream>>>(a);

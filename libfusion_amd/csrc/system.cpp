@@ -5,6 +5,7 @@
 // Each method below enqueues kernels on its object's stream; nothing waits for
 // the GPU inside a frame (the reference synchronises twice per Gauss-Newton
 // iteration and once per integrate / raycast).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -201,6 +202,47 @@ int DeviceImage::resize_device_map() {
     return lf_pyramid_refresh(levels(), K.data(), levels_view(), stream_);
 }
 
+int DeviceImage::set_vmap(const float *vmap) {
+    if (!vmap) return lf_set_error(LF_ERR_ARG, "null vertex map");
+    const size_t n = (size_t)K_[0].width * K_[0].height;
+    LF_HIP_CHECK(hipMemcpyAsync(vmap_[0].ptr<float>(), vmap, n * 16, hipMemcpyDeviceToDevice, stream_));
+    return resize_device_map();
+}
+
+int DeviceImage::intensity_pyramid() {
+    void *st = stream_;
+    for (int l = 1; l < levels(); ++l)
+        LF_RC(lf_pyr_down_image(intensity_[l - 1].ptr<float>(), intensity_[l].ptr<float>(), K_[l - 1].width,
+                                K_[l - 1].height, st));
+    for (int l = 0; l < levels(); ++l)
+        LF_RC(lf_compute_derivative(intensity_[l].ptr<float>(), dx_[l].ptr<float>(), dy_[l].ptr<float>(),
+                                    K_[l].width, K_[l].height, st));
+    return LF_OK;
+}
+
+int DeviceImage::set_intensity(const float *intensity) {
+    if (!intensity) return lf_set_error(LF_ERR_ARG, "null intensity");
+    const size_t n = (size_t)K_[0].width * K_[0].height;
+    LF_HIP_CHECK(hipMemcpyAsync(intensity_[0].ptr<float>(), intensity, n * 4, hipMemcpyDeviceToDevice, stream_));
+    return intensity_pyramid();
+}
+
+int DeviceImage::set_image(const unsigned char *rgb) {
+    if (!rgb) return lf_set_error(LF_ERR_ARG, "null image");
+    LF_RC(lf_rgb_to_intensity(rgb, intensity_[0].ptr<float>(), K_[0].width, K_[0].height, stream_));
+    return intensity_pyramid();
+}
+
+int DeviceImage::set_depth(const float *depth) {
+    if (!depth) return lf_set_error(LF_ERR_ARG, "null depth");
+    void *st = stream_;
+    LF_RC(lf_bilateral_filter(depth, depth_[0].ptr<float>(), K_[0].width, K_[0].height, st));
+    for (int l = 1; l < levels(); ++l)
+        LF_RC(lf_pyr_down_depth(depth_[l - 1].ptr<float>(), depth_[l].ptr<float>(), K_[l - 1].width,
+                                K_[l - 1].height, st));
+    return LF_OK;
+}
+
 // ------------------------------------------------------------ DenseMapping
 
 DenseMapping::DenseMapping(const IntrinsicMatrix &K, const lf_map_params &params, hipStream_t stream)
@@ -228,6 +270,12 @@ int DenseMapping::raycast(float *vmap, const float *pose24, hipEvent_t e0, hipEv
 }
 
 int DenseMapping::reset_mapping() { return lf_map_reset(&params_, &storage_, stream_); }
+
+int DenseMapping::deintegrate(DeviceImage &frame, const float *pose24, bool colour) {
+    return lf_deintegrate_ev(&params_, &storage_, frame.get_raw_depth().ptr<float>(), frame.get_nmap(0).ptr<float>(),
+                             colour ? frame.get_image().ptr<unsigned char>() : nullptr, K_.width, K_.height, pose24,
+                             &K_, stream_, frame.get_integration_samples().ptr<float>());
+}
 
 int DenseMapping::writeMapToDisk(const std::string &file_name) {
     return lf_map_write(&params_, &storage_, file_name.c_str(), stream_);
@@ -262,6 +310,22 @@ DenseTracking::DenseTracking(hipStream_t stream) : stream_(stream) {
     lf_track_set_pose(state_, I, stream_);
 }
 
+DenseTracking::DenseTracking(const IntrinsicMatrix &K, int num_pyr, hipStream_t stream) : DenseTracking(stream) {
+    src_.reset(new DeviceImage(K, num_pyr, stream));
+    ref_.reset(new DeviceImage(K, num_pyr, stream));
+}
+
+void DenseTracking::swap_intensity_pyr() {
+    if (src_ && ref_) src_->swap_intensity(*ref_);
+}
+
+int DenseTracking::compute_transform(const TrackingContext &c) {
+    if (!src_ || !ref_) return lf_set_error(LF_ERR_ARG, "tracker made without pyramids (use DenseTracking(K, NUM_PYR))");
+    TrackingContext k = c;
+    if (k.intrinsics_pyr_.empty()) k.intrinsics_pyr_ = ref_->intrinsics();
+    return enqueue(*ref_, *src_, k, true);
+}
+
 DenseTracking::~DenseTracking() {
     (void)hipFree(state_);
     (void)hipFree(scratch_);
@@ -273,7 +337,13 @@ DenseTracking::~DenseTracking() {
 // the finish (success test and odometry chain), so a frame is two launches
 // per step.
 int DenseTracking::compute_transform(DeviceImage &ref, DeviceImage &cur, const TrackingContext &c) {
+    return enqueue(ref, cur, c, false);
+}
+
+int DenseTracking::enqueue(DeviceImage &ref, DeviceImage &cur, const TrackingContext &c, bool context_mode) {
+    context_mode_ = context_mode;
     LfStepFlags f = {};
+    f.context_mode = context_mode ? 1 : 0;
     (c.use_initial_guess_ ? c.initial_estimate_ : SE3d()).matrix3x4(f.init);
     struct Step { int level, it; };
     std::vector<Step> plan;
@@ -316,7 +386,8 @@ TrackingResult DenseTracking::result() {
     if (hipMemcpyAsync(&h, state_, sizeof(h), hipMemcpyDeviceToHost, stream_) != hipSuccess) return r;
     if (hipStreamSynchronize(stream_) != hipSuccess) return r;
     r.sucess = h.success != 0;
-    r.icp_error = h.icp_error;
+    // compute_transform(ctx) reports last_icp_error (icp_tracker.cpp:310)
+    r.icp_error = context_mode_ ? h.last_icp_error : h.icp_error;
     r.update = r.sucess ? SE3d::from3x4(h.est).inverse() : SE3d();
     return r;
 }
@@ -522,9 +593,7 @@ extern "C" int lf_image_resize_device_map(lf_image *im) {
     return im->img->resize_device_map();
 }
 
-extern "C" int lf_image_plane(lf_image *im, int level, int plane, void **out) {
-    if (!im || !out) return lf_set_error(LF_ERR_ARG, "null argument to lf_image_plane");
-    fusion::DeviceImage &d = *im->img;
+static int image_plane(fusion::DeviceImage &d, int level, int plane, void **out) {
     if (level < 0 || level >= d.levels()) return lf_set_error(LF_ERR_ARG, "bad level %d", level);
     if ((plane == LF_PLANE_RAW_DEPTH || plane == LF_PLANE_RGB || plane == LF_PLANE_SAMPLES) && level != 0)
         return lf_set_error(LF_ERR_ARG, "plane %d exists at level 0 only", plane);
@@ -543,6 +612,101 @@ extern "C" int lf_image_plane(lf_image *im, int level, int plane, void **out) {
     return LF_OK;
 }
 
+extern "C" int lf_image_plane(lf_image *im, int level, int plane, void **out) {
+    if (!im || !out) return lf_set_error(LF_ERR_ARG, "null argument to lf_image_plane");
+    return image_plane(*im->img, level, plane, out);
+}
+
+// ------------------------------------------------------------ lf_tracker_*
+
+struct lf_tracker {
+    fusion::DenseTracking *t;
+};
+
+extern "C" int lf_tracker_create(const lf_intrinsics *K, int num_pyr, void *stream, lf_tracker **out) {
+    if (!K || !out) return lf_set_error(LF_ERR_ARG, "null argument to lf_tracker_create");
+    if (int rc = check_halving(K, num_pyr)) return rc;
+    fusion::IntrinsicMatrix k(K->width, K->height, K->fx, K->fy, K->cx, K->cy);
+    try {
+        *out = new lf_tracker{new fusion::DenseTracking(k, num_pyr, lf_stream(stream))};
+    } catch (const std::bad_alloc &) {
+        return lf_set_error(LF_ERR_HIP, "device allocation failed in lf_tracker_create");
+    }
+    return LF_OK;
+}
+
+extern "C" int lf_tracker_destroy(lf_tracker *t) {
+    if (!t) return LF_OK;
+    delete t->t;
+    delete t;
+    return LF_OK;
+}
+
+#define LF_TRACKER_SET(name, type)                                                        \
+    extern "C" int lf_tracker_##name(lf_tracker *t, const type *plane) {                 \
+        if (!t || !plane) return lf_set_error(LF_ERR_ARG, "null argument to lf_tracker_" #name); \
+        return t->t->name(plane);                                                         \
+    }
+LF_TRACKER_SET(set_source_vmap, float)
+LF_TRACKER_SET(set_source_image, unsigned char)
+LF_TRACKER_SET(set_source_depth, float)
+LF_TRACKER_SET(set_source_intensity, float)
+LF_TRACKER_SET(set_reference_image, unsigned char)
+LF_TRACKER_SET(set_reference_intensity, float)
+LF_TRACKER_SET(set_reference_vmap, float)
+#undef LF_TRACKER_SET
+
+extern "C" int lf_tracker_swap_intensity(lf_tracker *t) {
+    if (!t) return lf_set_error(LF_ERR_ARG, "null tracker");
+    t->t->swap_intensity_pyr();
+    return LF_OK;
+}
+
+static int tracking_context(const double *init12, const int *iterations, int num_levels, int max_levels,
+                            fusion::TrackingContext &c) {
+    if (!iterations || num_levels < 1 || num_levels > max_levels)
+        return lf_set_error(LF_ERR_ARG, "need 1..%d levels of iterations", max_levels);
+    c.use_initial_guess_ = init12 != nullptr;
+    if (init12) c.initial_estimate_ = fusion::SE3d::from3x4(init12);
+    c.max_iterations_.assign(iterations, iterations + num_levels);
+    return LF_OK;
+}
+
+static int read_result(fusion::DenseTracking &t, lf_tracking_result *out) {
+    const fusion::TrackingResult r = t.result();
+    out->success = r.sucess ? 1 : 0;
+    out->icp_error = r.icp_error;
+    r.update.matrix3x4(out->update);
+    return LF_OK;
+}
+
+extern "C" int lf_tracker_compute_transform(lf_tracker *t, const double *init12, const int *iterations,
+                                            int num_levels, lf_tracking_result *out) {
+    if (!t || !out) return lf_set_error(LF_ERR_ARG, "null argument to lf_tracker_compute_transform");
+    fusion::TrackingContext c;
+    if (int rc = tracking_context(init12, iterations, num_levels, t->t->source()->levels(), c)) return rc;
+    LF_RC(t->t->compute_transform(c));
+    return read_result(*t->t, out);
+}
+
+extern "C" int lf_tracker_compute_transform_images(lf_tracker *t, lf_image *reference, lf_image *current,
+                                                   const double *init12, const int *iterations, int num_levels,
+                                                   lf_tracking_result *out) {
+    if (!t || !reference || !current || !out)
+        return lf_set_error(LF_ERR_ARG, "null argument to lf_tracker_compute_transform_images");
+    const int levels = std::min(reference->img->levels(), current->img->levels());
+    fusion::TrackingContext c;
+    if (int rc = tracking_context(init12, iterations, num_levels, levels, c)) return rc;
+    c.intrinsics_pyr_ = reference->img->intrinsics();
+    LF_RC(t->t->compute_transform(*reference->img, *current->img, c));
+    return read_result(*t->t, out);
+}
+
+extern "C" int lf_tracker_plane(lf_tracker *t, int source, int level, int plane, void **out) {
+    if (!t || !out) return lf_set_error(LF_ERR_ARG, "null argument to lf_tracker_plane");
+    return image_plane(source ? *t->t->source() : *t->t->reference(), level, plane, out);
+}
+
 // update(RgbdImagePtr frame) (voxel_hashing.cpp:44-65) on a caller's map: the
 // frame's raw depth for allocation, its packed integration samples (depth,
 // weight from the level-0 normals, colour) for the weighted update.
@@ -554,6 +718,16 @@ extern "C" int lf_update_image(const lf_map_params *p, const lf_map_storage *s, 
     return lf_update_ev(p, s, d.get_raw_depth().ptr<float>(), d.get_nmap(0).ptr<float>(),
                         colour ? d.get_image().ptr<unsigned char>() : nullptr, K.width, K.height, pose24, &K,
                         lf_stream(stream), true, nullptr, nullptr, d.get_integration_samples().ptr<float>());
+}
+
+extern "C" int lf_deintegrate_image(const lf_map_params *p, const lf_map_storage *s, lf_image *im,
+                                    const float *pose24, int colour, void *stream) {
+    if (!p || !s || !im || !pose24) return lf_set_error(LF_ERR_ARG, "null argument to lf_deintegrate_image");
+    fusion::DeviceImage &d = *im->img;
+    const lf_intrinsics &K = d.intrinsics()[0];
+    return lf_deintegrate_ev(p, s, d.get_raw_depth().ptr<float>(), d.get_nmap(0).ptr<float>(),
+                             colour ? d.get_image().ptr<unsigned char>() : nullptr, K.width, K.height, pose24, &K,
+                             lf_stream(stream), d.get_integration_samples().ptr<float>());
 }
 
 // ------------------------------------------------------------ lf_system_*

@@ -1024,15 +1024,42 @@ template <class R> struct Frame {
     }
 };
 
-// DenseTracking::compute_transform(ref, cur, ctx) (icp_tracker.cpp:23-156).
+inline SE3 se3_from12(const double *m) {
+    SE3 T;
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) T.R[i * 3 + j] = m[i * 4 + j];
+        T.t[i] = m[i * 4 + 3];
+    }
+    return T;
+}
+inline void se3_to12(const SE3 &T, double *m) {
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) m[i * 4 + j] = T.R[i * 3 + j];
+        m[i * 4 + 3] = T.t[i];
+    }
+}
+
+// set_source_vmap / set_source_image (icp_tracker.cpp:164-176; build_pyramid.cpp:
+// 17-76): a frame's pyramids from a level-0 vertex map and an RGB image.
+template <class R> void frame_from_maps(Frame<R> &f, const R *vmap0, const uint8_t *rgb) {
+    const int W = f.Ks[0].width, H = f.Ks[0].height;
+    std::copy(vmap0, vmap0 + 4 * (size_t)W * H, f.vmap[0].begin());
+    f.resize_device_map();
+    grey(rgb, f.inten[0].data(), W, H);
+    for (int l = 1; l < f.levels; ++l) pyr_down(f.inten[l - 1].data(), f.inten[l].data(), f.Ks[l - 1].width, f.Ks[l - 1].height);
+    for (int l = 0; l < f.levels; ++l) derivative(f.inten[l].data(), f.dx[l].data(), f.dy[l].data(), f.Ks[l].width, f.Ks[l].height);
+}
+
+// DenseTracking::compute_transform(ref, cur, ctx) (icp_tracker.cpp:23-156);
+// with context_mode, compute_transform(ctx) (icp_tracker.cpp:210-314): NaN
+// errors are not flagged and success is unconditional.  Starts at init.
 template <class R> struct Tracker {
     int levels;
     std::vector<int> iters;
     bool success = true;
     float last_icp_error = 0;
-    SE3 compute(const Frame<R> &ref, const Frame<R> &cur) {
-        SE3 est, prev;  // Revertable<SE3d> (revertable.h:7-45)
-        SE3 init = est;
+    SE3 compute(const Frame<R> &ref, const Frame<R> &cur, const SE3 &init = SE3(), bool context_mode = false) {
+        SE3 est = init, prev = init;  // Revertable<SE3d> (revertable.h:7-45)
         bool invalid = false;
         for (int level = levels - 1; level >= 0; --level) {
             const Intr &Ki = ref.Ks[level];
@@ -1061,14 +1088,15 @@ template <class R> struct Tracker {
                 prev = est;
                 est = se3_mul(se3_exp(x), last);
                 icp_error = std::sqrt(si[27]) / si[28];
-                if (isnan_(icp_error)) { invalid = true; break; }
+                last_icp_error = (float)last_icp;
+                if (isnan_(icp_error) && !context_mode) { invalid = true; break; }
                 if (icp_error > last_icp) {
                     if (icp_count >= 2) { est = prev; break; }
                     icp_count++;
                     icp_error = last_icp;
                 } else icp_count = 0;
                 rgb_error = std::sqrt(sr[27]) / sr[28];
-                if (isnan_(rgb_error)) { invalid = true; break; }
+                if (isnan_(rgb_error) && !context_mode) { invalid = true; break; }
                 if (rgb_error > last_rgb) {
                     if (rgb_count >= 2) { est = prev; break; }
                     rgb_count++;
@@ -1076,6 +1104,7 @@ template <class R> struct Tracker {
                 } else rgb_count = 0;
             }
         }
+        if (context_mode) { success = true; return se3_inv(est); }
         double xi[6];
         se3_log(se3_mul(se3_inv(est), init), xi);
         double nrm = 0;
@@ -1228,6 +1257,18 @@ using namespace orc;
     extern "C" void orc_rgb_step_##S(const R *ci, const R *li, const R *lv, const R *dx, const R *dy, int w,   \
                                      int h, R stddev, const R *pose12, const Intr *K, R *out) {                \
         rgb_step(ci, li, lv, dx, dy, w, h, stddev, m34_from(pose12), K_<R>(*K), out);                          \
+    }                                                                                                          \
+    extern "C" int orc_track_maps_##S(const R *ref_vmap, const uint8_t *ref_rgb, const R *cur_vmap,            \
+                                      const uint8_t *cur_rgb, const Intr *K, int L, const int *iters,          \
+                                      const double *init12, int context_mode, double *update12) {              \
+        Frame<R> ref(*K, L), cur(*K, L);                                                                       \
+        frame_from_maps(ref, ref_vmap, ref_rgb);                                                               \
+        frame_from_maps(cur, cur_vmap, cur_rgb);                                                               \
+        Tracker<R> t;                                                                                          \
+        t.levels = L;                                                                                          \
+        t.iters.assign(iters, iters + L);                                                                      \
+        se3_to12(t.compute(ref, cur, se3_from12(init12), context_mode != 0), update12);                        \
+        return t.success ? 1 : 0;                                                                              \
     }                                                                                                          \
     extern "C" void *orc_system_create_##S(const Params *p, const Intr *K, int L, const int *iters,            \
                                            double scale) {                                                     \

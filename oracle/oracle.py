@@ -71,6 +71,7 @@ def lib():
             sig("orc_ray_blocks_touched_%s", I, [P])
             sig("orc_ray_stats_%s", None, [P, P])
             sig("orc_ray_px_%s", None, [P, P])
+            sig("orc_track_maps_%s", I, [P, P, P, P, P, I, P, P, I, P])
             sig("orc_zrange_%s", None, [P, P, P, I, I, P, P])
             sig("orc_mesh_%s", ctypes.c_long, [P, I, P, P, P, ctypes.c_long])
             sig("orc_bilateral_%s", None, [P, P, I, I])
@@ -346,6 +347,21 @@ def unpack29(s):
             else:
                 A[i, j] = A[j, i] = v
     return A, b, np.array([s[27], s[28]], np.float64)
+
+
+def track_maps(ref_vmap, ref_rgb, cur_vmap, cur_rgb, K, iters, init=None, context_mode=True, dtype=np.float64):
+    """compute_transform on pyramids built from level-0 vertex maps and images
+    (set_source_vmap / set_source_image).  Returns (success, update 4x4)."""
+    it = np.ascontiguousarray(iters, np.int32)
+    T0 = np.eye(4) if init is None else np.asarray(init, np.float64)
+    ini = np.ascontiguousarray(T0[:3, :], np.float64)
+    out = np.zeros(12, np.float64)
+    ok = lib()["orc_track_maps_" + _sfx(dtype)](
+        _p(_c(ref_vmap, dtype)), _p(_c(ref_rgb, np.uint8)), _p(_c(cur_vmap, dtype)), _p(_c(cur_rgb, np.uint8)),
+        ctypes.byref(K), len(it), _p(it), _p(ini), 1 if context_mode else 0, _p(out))
+    U = np.eye(4)
+    U[:3, :] = out.reshape(3, 4)
+    return bool(ok), U
 
 
 class System:

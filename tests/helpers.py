@@ -38,3 +38,31 @@ def vmap_compare(a, b):
     mism = float(np.mean(va != vb))
     d = float(np.max(np.abs(a[both][:, :3] - b[both][:, :3]))) if both.any() else 0.0
     return mism, d, int(both.sum())
+
+
+def keyframe_pair(sc, a, b, perturb=(0.01, 0.5), seed=0):
+    """Half-resolution inputs of re-tracking frame b (source) against frame a
+    (reference): bilateral back-projected vertex maps and images, the true
+    update (T_ref^-1 T_cur) and an initial estimate of T_cur^-1 T_ref perturbed
+    by perturb = (metres, degrees)."""
+    from oracle import oracle as O
+    from libfusion_amd import capi
+    K = capi.intrinsics(sc.width // 2, sc.height // 2, sc.fx / 2, sc.fy / 2, (sc.cx + 0.5) / 2 - 0.5,
+                        (sc.cy + 0.5) / 2 - 0.5)
+    oK = O.make_intr(K.width, K.height, K.fx, K.fy, K.cx, K.cy)
+    out = []
+    for f in (a, b):
+        d16, rgb = sc.render(sc.pose(f), K.width, K.height, K.fx, K.fy, K.cx, K.cy, noise=True, frame_id=f)
+        depth = d16.astype(np.float32) * np.float32(sc.depth_scale)
+        out.append((O.back_project(O.bilateral(depth), oK), rgb))
+    truth = np.linalg.inv(sc.pose(a)) @ sc.pose(b)
+    rng = np.random.default_rng(seed)
+    w = rng.normal(size=3)
+    w *= np.deg2rad(perturb[1]) / np.linalg.norm(w)
+    t = rng.normal(size=3)
+    t *= perturb[0] / np.linalg.norm(t)
+    P = np.eye(4)
+    P[:3, :3] = O.se3_exp(np.concatenate([np.zeros(3), w]))[:3, :3]
+    P[:3, 3] = t
+    init = P @ np.linalg.inv(truth)
+    return K, oK, out[0], out[1], truth, init

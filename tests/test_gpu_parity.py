@@ -528,6 +528,112 @@ def test_checkpoint_reads_a_reference_built_table(tmp_path):
     assert eq and hit > 0.5
 
 
+def test_keyframe_retracking_matches_fp64_oracle():
+    """DenseTracking(K, NUM_PYR) with set_source_* / set_reference_* and
+    compute_transform(ctx): pyramids bit-exact with the oracle's image ops, pose
+    within 0.5 mm / 0.05 deg of the FP64 oracle; failure flags as the reference."""
+    from tests.helpers import keyframe_pair
+    sc = Scene(1, 1)
+    K, oK, (rv, rc), (cv, cc), truth, init = keyframe_pair(sc, 0, 4)
+    tr = capi.Tracker(K, 3)
+    t_rv, t_rc, t_cv, t_cc = dev(rv), dev(rc), dev(cv), dev(cc)
+    tr.set("set_source_vmap", t_cv)
+    tr.set("set_source_image", t_cc)
+    tr.set("set_reference_vmap", t_rv)
+    tr.set("set_reference_image", t_rc)
+    ok, err, U = tr.compute_transform((10, 5, 3), init)
+    ok64, U64 = O.track_maps(rv, rc, cv, cc, oK, (10, 5, 3), init, True, np.float64)
+    dt, dr = pose_err(U, U64)
+    assert ok and np.isfinite(err) and dt <= 5e-4 and dr <= 0.05, (dt, dr)
+    dt, dr = pose_err(U, truth)
+    assert dt < 5e-3 and dr < 0.3, (dt, dr)
+    # the pyramids the iterations read
+    P = capi.ptr
+    lib = capi.load()
+    v1 = O.resize_half(cv)
+    I1 = O.pyr_down(O.grey(cc))
+
+    def plane(src, level, which, shape, dtype=torch.float32):
+        out = ctypes.c_void_p()
+        capi.check(lib.lf_tracker_plane(tr.h, src, level, which, ctypes.byref(out)))
+        t = torch.empty(shape, dtype=dtype, device="cuda")
+        capi.copy_device(t, out.value, t.numel() * t.element_size())
+        return t.cpu().numpy()
+
+    h1, w1 = tr.K[1].height, tr.K[1].width
+    assert np.array_equal(plane(1, 1, capi.PLANE_VMAP, (h1, w1, 4)), v1, equal_nan=True)
+    assert np.array_equal(plane(1, 1, capi.PLANE_NMAP, (h1, w1, 4)), O.nmap(v1), equal_nan=True)
+    assert np.array_equal(plane(1, 1, capi.PLANE_INTENSITY, (h1, w1)), I1)
+    assert np.array_equal(plane(1, 1, capi.PLANE_DX, (h1, w1)), O.derivative(I1)[0])
+    # no correspondences: compute_transform(ctx) still reports success
+    empty = torch.full_like(t_rv, float("nan"))
+    tr.set("set_reference_vmap", empty)
+    ok, _, _ = tr.compute_transform((2, 1, 1), init)
+    assert ok
+    # the frame-pair form on uploaded frames: success, near the true motion;
+    # and the failure flag with the identity update when the errors are NaN
+    ia, ib = capi.Image(K, 3), capi.Image(K, 3)
+    for img, f in ((ia, 0), (ib, 4)):
+        d16, rgb = sc.render(sc.pose(f), K.width, K.height, K.fx, K.fy, K.cx, K.cy, noise=True, frame_id=f)
+        img.upload(dev(d16), dev(rgb), sc.depth_scale)
+    ok, _, U = tr.compute_transform_images(ia, ib, (10, 5, 3), init)
+    dt, dr = pose_err(U, truth)
+    assert ok and dt < 5e-3 and dr < 0.3, (dt, dr)
+    torch.cuda.synchronize()
+    capi.copy_device_to(ia.plane_ptr(0, capi.PLANE_VMAP), empty)
+    ia.resize_device_map()
+    ok, _, U = tr.compute_transform_images(ia, ib, (2, 1, 1), init)
+    assert not ok and np.array_equal(U, np.eye(4))
+    tr.close()
+
+
+def test_deintegration_undoes_a_frame():
+    """Fuse frames A and B, de-integrate B: the volume matches fusing A alone
+    (weights to float rounding, sdf to a few int16 codes, colour to a few
+    levels); blocks only B allocated are left unseen."""
+    sc = Scene(1, 1)
+    K = half_res(sc)
+    p = capi.map_params(2 ** 18, 1 << 16, 0.01, 0.03)
+    imgs, poses = [], []
+    for f in (0, 3):
+        T = sc.pose(f)
+        d16, rgb = sc.render(T, K.width, K.height, K.fx, K.fy, K.cx, K.cy, noise=True, frame_id=f)
+        img = capi.Image(K, 3)
+        img.upload(dev(d16), dev(rgb), sc.depth_scale)
+        imgs.append(img)
+        poses.append(dev(capi.pose24(T)))
+    g1, g2 = capi.Map(p, K.width, K.height), capi.Map(p, K.width, K.height)
+    g1.update_image(imgs[0], poses[0])
+    g1.update_image(imgs[1], poses[1])
+    w_ab = g1.voxels()[2].copy()
+    g1.deintegrate_image(imgs[1], poses[1])
+    g2.update_image(imgs[0], poses[0])
+    torch.cuda.synchronize()
+    b1, s1, w1, c1 = g1.voxels()
+    b2, s2, w2, c2 = g2.voxels()
+    assert (w_ab != w1).sum() > 10000  # frame B changed many voxels and was removed
+    k1 = {tuple(b): i for i, b in enumerate(b1.tolist())}
+    idx = np.array([k1[tuple(b)] for b in b2.tolist()])
+    assert len(b1) >= len(b2) > 1000
+    W1, S1, C1 = w1[idx], s1[idx].astype(np.int32), c1[idx].astype(np.int32)
+    assert np.allclose(W1, w2, rtol=1e-5, atol=1e-7)
+    seen = w2 > 0
+    assert np.array_equal(W1 > 0, seen)
+    # the fused sdf was stored truncated (< 1 code); un-averaging divides that
+    # error by w_A / w_AB, and both results are truncated again (< 1 code each)
+    ds = np.abs(S1 - s2)[seen]
+    bound = 3.0 + 1.01 * (w_ab[idx][seen] / w2[seen])
+    assert np.all(ds <= bound), (ds - bound).max()
+    assert np.percentile(ds, 99) <= 4
+    # colour: the same argument with the uint8 truncation of the fused mean
+    dc = np.abs(C1 - c2.astype(np.int32)).transpose(0, 2, 1)[seen]
+    assert np.all(dc <= 2.0 + 1.01 * (w_ab[idx][seen] / w2[seen])[:, None])
+    only_b = np.setdiff1d(np.arange(len(b1)), idx)
+    assert not w1[only_b].any()
+    for img in imgs:
+        img.close()
+
+
 def level_maps(sc, frame, K):
     d16, rgb = sc.render(sc.pose(frame), K.width, K.height, K.fx, K.fy, K.cx, K.cy, noise=True, frame_id=frame)
     depth = d16.astype(np.float32) * np.float32(sc.depth_scale)

@@ -79,6 +79,10 @@ public:
     int set_intensity(const float *intensity);       // intensity pyramid and derivatives
     int set_image(const unsigned char *rgb);         // grey conversion, then set_intensity
     int set_depth(const float *depth);               // bilateral filter and depth pyramid
+    // get_rendered_image / get_rendered_scene_textured (device_image.cpp:119-129):
+    // RGBA8 shading of the level-0 vertex / normal maps (enqueued)
+    const GpuImage &get_rendered_image();
+    const GpuImage &get_rendered_scene_textured();
     // exchange the intensity pyramids of two frames (gradients stay)
     void swap_intensity(DeviceImage &o) { std::swap(intensity_, o.intensity_); }
     const GpuImage &get_raw_depth() const { return depth_raw_; }
@@ -99,7 +103,7 @@ private:
     std::vector<IntrinsicMatrix> K_;
     hipStream_t stream_;
     int intensity_pyramid();
-    GpuImage depth_raw_, image_, samples_;
+    GpuImage depth_raw_, image_, samples_, rendered_;
     std::vector<GpuImage> depth_, intensity_, dx_, dy_, vmap_, nmap_;
 };
 
@@ -117,6 +121,8 @@ public:
     int update(const float *depth, const unsigned char *rgb, const float *pose24_dev);
     // raycast (voxel_hashing.cpp:87-119) into a float4 vertex map.
     int raycast(float *vmap, const float *pose24_dev, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+    // raycast_check_visibility (voxel_hashing.cpp:121-133): visible blocks from the pose, then raycast.
+    int raycast_check_visibility(float *vmap, const float *pose24_dev);
     int reset_mapping();
     // Remove a frame fused with update(frame, pose, colour) (inverse weighted update).
     int deintegrate(DeviceImage &frame, const float *pose24_dev, bool colour);

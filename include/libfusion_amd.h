@@ -181,6 +181,21 @@ int lf_raycast(const lf_map_params *p, const lf_map_storage *s,
                float *vmap, unsigned char *rgb_out, int cols, int rows,
                const float *pose24, const lf_intrinsics *K, int check_colour, void *stream);
 
+/* count_visible_entry (ray_tracing.cu:647-807): the map's visible list becomes
+ * every allocated block with a corner inside a cols x rows image at the pose
+ * (T_world_camera, then its inverse) with z in [zmin_update, zmax_update]; the
+ * count is returned when count_out is non-null (synchronising).  The
+ * reference's functor never initialises its entry count (ray_tracing.cu:733);
+ * this is the intended semantic over allocated blocks. */
+int lf_select_visible(const lf_map_params *p, const lf_map_storage *s, const float *pose24,
+                      const lf_intrinsics *K, int cols, int rows, unsigned int *count_out, void *stream);
+/* raycast_check_visibility (voxel_hashing.cpp:121-133): lf_select_visible at the
+ * pose, then lf_raycast from it — renders the map from any pose, not only the
+ * last update's (relocalisation). */
+int lf_raycast_check_visibility(const lf_map_params *p, const lf_map_storage *s, float *vmap,
+                                unsigned char *rgb_out, int cols, int rows, const float *pose24,
+                                const lf_intrinsics *K, int check_colour, void *stream);
+
 /* fusion::cuda::create_mesh_vertex_only / _with_normal / _with_colour
  * (scene_mesh.cu:260-301,308-351,600-643): marching cubes over every allocated
  * block.  vertex_out: 9 floats per triangle (metres); normal_out: 9 floats per
@@ -237,6 +252,22 @@ int lf_back_project_depth(const float *depth, float *vmap, int cols, int rows,
 /* computeNMap (cuda_imgproc.cu:276-305); last row/column written as NaN. */
 int lf_compute_nmap(const float *vmap, float *nmap, int cols, int rows, void *stream);
 
+/* renderScene / renderSceneTextured (cuda_imgproc.cu:17-108): RGBA8 Phong
+ * shading of a camera-frame vertex / normal map (float4 each) with the light
+ * at (5, 5, 5); rgb (uint8 x3) textures it, null renders grey.  NaN vertices
+ * get (4, 2, 2, 255). */
+int lf_render_scene(const float *vmap, const float *nmap, const unsigned char *rgb, int cols, int rows,
+                    unsigned char *rgba_out, void *stream);
+/* ToSemiDenseImageK (cuda_imgproc.cu:110-128): image where dx > th_dx or
+ * dy > th_dy, else 255. */
+int lf_semi_dense(const float *image, const float *dx, const float *dy, int cols, int rows, float th_dx,
+                  float th_dy, float *semi_out, void *stream);
+/* warp_image (cuda_imgproc.cu:155-195): dst(x) = src bilinearly sampled where
+ * vmap_dst(x) lands under pose12 (float 3x4, host) and K, inside a one-pixel
+ * margin; 0 elsewhere.  src / dst uint8 x3. */
+int lf_warp_image(const unsigned char *src, const float *vmap_dst, int cols, int rows, const float *pose12,
+                  const lf_intrinsics *K, unsigned char *dst, void *stream);
+
 /* ------------------------------------------- frame pyramid (DeviceImage) */
 
 /* DeviceImage (device_image.h:18-88): one RGB-D frame's device pyramid.  Every
@@ -269,6 +300,10 @@ int lf_image_upload(lf_image *img, const uint16_t *depth, const unsigned char *r
 int lf_image_resize_device_map(lf_image *img);
 /* Device pointer of a plane (LF_PLANE_*) of a level; rows are packed. */
 int lf_image_plane(lf_image *img, int level, int plane, void **out);
+/* get_rendered_image / get_rendered_scene_textured (device_image.cpp:119-129):
+ * render the level-0 vertex / normal maps (textured with the frame's colour
+ * when textured != 0) into the image's RGBA8 plane; *rgba_out receives it. */
+int lf_image_render(lf_image *img, int textured, void **rgba_out);
 /* De-integration (not in the reference; deleteBlock, map_struct.cu:327-363, has
  * no callers): the inverse of lf_update_weighted for the same frame and pose,
  * over the same voxels (no block is allocated).  weight' = weight - w_frame,
@@ -389,6 +424,11 @@ int lf_system_fetch_mesh(lf_system *sys, float *vertex_out, float *normal_out,
 int lf_system_map(lf_system *sys, lf_map_params *p, lf_map_storage *s);
 /* Device pointers of the reference frame's level-l vmap / nmap after the frame. */
 int lf_system_maps(lf_system *sys, int level, const float **vmap, const float **nmap);
+/* get_shaded_depth (which 0: the odometry's current frame), get_rendered_scene
+ * (1: the reference frame, i.e. the last raycast) and get_rendered_scene_textured
+ * (2) of System (system.cpp:132-146): *rgba_out receives the RGBA8 image
+ * (device); synchronising. */
+int lf_system_render(lf_system *sys, int which, void **rgba_out);
 /* System::writeMapToDisk / readMapFromDisk (system.cpp:202-210). */
 int lf_system_write_map(lf_system *sys, const char *path);
 int lf_system_read_map(lf_system *sys, const char *path);

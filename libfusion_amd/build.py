@@ -26,6 +26,7 @@ HIP_SOURCES = [
     "mesh.hip",
     "imgproc.hip",
     "reduce.hip",
+    "render.hip",
     "system.cpp",
     "map_io.cpp",
 ]

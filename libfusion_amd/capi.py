@@ -74,6 +74,8 @@ SIGNATURES = [
     ("lf_update_weighted", I, [P, P, P, P, P, I, I, P, P, P]),
     ("lf_update", I, [P, P, P, P, I, I, P, P, P]),
     ("lf_raycast", I, [P, P, P, P, I, I, P, P, I, P]),
+    ("lf_select_visible", I, [P, P, P, P, I, I, P, P]),
+    ("lf_raycast_check_visibility", I, [P, P, P, P, I, I, P, P, I, P]),
     ("lf_create_mesh", I, [P, P, P, P, P, I, P, P]),
     ("lf_map_write", I, [P, P, ctypes.c_char_p, P]),
     ("lf_map_read", I, [P, P, ctypes.c_char_p, P]),
@@ -87,11 +89,15 @@ SIGNATURES = [
     ("lf_compute_derivative", I, [P, P, P, I, I, P]),
     ("lf_back_project_depth", I, [P, P, I, I, P, P]),
     ("lf_compute_nmap", I, [P, P, I, I, P]),
+    ("lf_render_scene", I, [P, P, P, I, I, P, P]),
+    ("lf_semi_dense", I, [P, P, P, I, I, F, F, P, P]),
+    ("lf_warp_image", I, [P, P, I, I, P, P, P, P]),
     ("lf_image_create", I, [P, I, P, P]),
     ("lf_image_destroy", I, [P]),
     ("lf_image_upload", I, [P, P, P, F]),
     ("lf_image_resize_device_map", I, [P]),
     ("lf_image_plane", I, [P, I, I, P]),
+    ("lf_image_render", I, [P, I, P]),
     ("lf_update_image", I, [P, P, P, P, I, P]),
     ("lf_deintegrate_weighted", I, [P, P, P, P, P, I, I, P, P, P]),
     ("lf_deintegrate_image", I, [P, P, P, P, I, P]),
@@ -122,6 +128,7 @@ SIGNATURES = [
     ("lf_system_fetch_mesh", I, [P, P, P, P, I, P]),
     ("lf_system_map", I, [P, P, P]),
     ("lf_system_maps", I, [P, I, P, P]),
+    ("lf_system_render", I, [P, I, P]),
     ("lf_system_write_map", I, [P, ctypes.c_char_p]),
     ("lf_system_read_map", I, [P, ctypes.c_char_p]),
     ("lf_system_stream", P, [P]),
@@ -269,6 +276,15 @@ class Image:
         check(load().lf_image_plane(self.h, int(level), int(plane), ctypes.byref(out)))
         return out.value
 
+    def render(self, textured=False):
+        """get_rendered_image / get_rendered_scene_textured: a copy of the RGBA8 render."""
+        torch = _torch()
+        out = ctypes.c_void_p()
+        check(load().lf_image_render(self.h, 1 if textured else 0, ctypes.byref(out)))
+        t = torch.empty((self.K[0].height, self.K[0].width, 4), dtype=torch.uint8, device="cuda")
+        copy_device(t, out.value, t.numel())
+        return t
+
     def plane(self, level, plane):
         """A copy of a plane as a tensor ([h, w], [h, w, 4] for maps, [h, w, 3] uint8 for colour)."""
         torch = _torch()
@@ -411,6 +427,18 @@ class Map:
         h, w = vmap.shape[:2]
         check(load().lf_raycast(ctypes.byref(self.params), ctypes.byref(self.storage), ptr(vmap), ptr(rgb_out),
                                 w, h, ptr(pose24_dev), ctypes.byref(K), 1 if check_colour else 0, stream_ptr()))
+
+    def select_visible(self, pose24_dev, K, cols, rows):
+        n = ctypes.c_uint(0)
+        check(load().lf_select_visible(ctypes.byref(self.params), ctypes.byref(self.storage), ptr(pose24_dev),
+                                       ctypes.byref(K), int(cols), int(rows), ctypes.byref(n), stream_ptr()))
+        return n.value
+
+    def raycast_check_visibility(self, vmap, pose24_dev, K, check_colour=True, rgb_out=None):
+        h, w = vmap.shape[:2]
+        check(load().lf_raycast_check_visibility(ctypes.byref(self.params), ctypes.byref(self.storage), ptr(vmap),
+                                                 ptr(rgb_out), w, h, ptr(pose24_dev), ctypes.byref(K),
+                                                 1 if check_colour else 0, stream_ptr()))
 
     def mesh(self, mode=1, max_triangles=None):
         """mode 0 vertices, 1 + normals, 2 + colours -> (tri[n,3,3], extra, count)."""

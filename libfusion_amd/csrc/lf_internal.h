@@ -19,6 +19,9 @@ int lf_update_ev(const lf_map_params *p, const lf_map_storage *s, const float *d
 int lf_deintegrate_ev(const lf_map_params *p, const lf_map_storage *s, const float *depth, const float *nmap,
                       const unsigned char *rgb, int cols, int rows, const float *pose24, const lf_intrinsics *K,
                       hipStream_t st, const float *samples);
+// lf_select_visible without the count read-back.
+int lf_select_visible_ev(const lf_map_params *p, const lf_map_storage *s, const float *pose24, const lf_intrinsics *K,
+                         int cols, int rows, hipStream_t st);
 // lf_raycast with events around the ray-marching kernel.
 int lf_raycast_ev(const lf_map_params *p, const lf_map_storage *s, float *vmap, unsigned char *rgb_out, int cols,
                   int rows, const float *pose24, const lf_intrinsics *K, int check_colour, hipStream_t st,

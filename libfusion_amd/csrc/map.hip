@@ -211,6 +211,9 @@ LF_DEV bool block_visible(int bx, int by, int bz, const float *Tcw, const lf_int
 
 // check_visibility_flag_kernel + exclusive scan + copy (depth_fusion.cu:49-93,
 // 549-557) over the allocated blocks: touched-this-update, or frustum-visible.
+// kGeometric: count_visible_entry (ray_tracing.cu:647-807), frustum-visible
+// only, touched flags left alone.
+template <bool kGeometric = false>
 __global__ __launch_bounds__(256) void k_visible(MapDev m, const float *__restrict__ pose24, lf_intrinsics K,
                                                  int cols, int rows) {
     float Tcw[12];
@@ -225,7 +228,7 @@ __global__ __launch_bounds__(256) void k_visible(MapDev m, const float *__restri
         bool vis = false;
         if (b < n) {
             const int4 p = reinterpret_cast<const int4 *>(m.block_pos)[b];
-            if (m.slot_flag[p.w]) {
+            if (!kGeometric && m.slot_flag[p.w]) {
                 vis = true;
                 m.slot_flag[p.w] = 0;
             } else {
@@ -479,6 +482,35 @@ extern "C" int lf_deintegrate_weighted(const lf_map_params *p, const lf_map_stor
     if (rc) return rc;
     if (!nmap) return lf_set_error(LF_ERR_ARG, "deintegrate_weighted needs a normal map");
     return lf_deintegrate_ev(p, s, depth, nmap, rgb, cols, rows, pose24, K, lf_stream(stream), nullptr);
+}
+
+__global__ void k_reset_visible(MapDev m) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) m.counters[LF_CNT_VISIBLE] = 0u;
+}
+
+int lf_select_visible_ev(const lf_map_params *p, const lf_map_storage *s, const float *pose24, const lf_intrinsics *K,
+                         int cols, int rows, hipStream_t st) {
+    const MapDev m = make_dev(p, s);
+    k_reset_visible<<<1, 64, 0, st>>>(m);
+    LF_LAUNCH_CHECK();
+    int nb = std::min(2048, std::max(1, (p->num_blocks + 255) / 256));
+    k_visible<true><<<nb, 256, 0, st>>>(m, pose24, *K, cols, rows);
+    LF_LAUNCH_CHECK();
+    return LF_OK;
+}
+
+extern "C" int lf_select_visible(const lf_map_params *p, const lf_map_storage *s, const float *pose24,
+                                 const lf_intrinsics *K, int cols, int rows, unsigned int *count_out, void *stream) {
+    if (!p || !s || !pose24 || !K || cols <= 0 || rows <= 0)
+        return lf_set_error(LF_ERR_ARG, "bad arguments to lf_select_visible");
+    hipStream_t st = lf_stream(stream);
+    if (int rc = lf_select_visible_ev(p, s, pose24, K, cols, rows, st)) return rc;
+    if (count_out) {
+        LF_HIP_CHECK(hipMemcpyAsync(count_out, s->counters + LF_CNT_VISIBLE, sizeof(unsigned int),
+                                    hipMemcpyDeviceToHost, st));
+        LF_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    return LF_OK;
 }
 
 extern "C" int lf_update_weighted(const lf_map_params *p, const lf_map_storage *s, const float *depth,

@@ -21,6 +21,7 @@
 //    probe entirely.
 #include "lf_common.h"
 #include "lf_internal.h"
+#include "lf_internal.h"
 
 using namespace lf;
 
@@ -330,6 +331,17 @@ int lf_raycast_ev(const lf_map_params *p, const lf_map_storage *s, float *vmap, 
     LF_LAUNCH_CHECK();
     if (e1) LF_HIP_CHECK(hipEventRecord(e1, st));
     return LF_OK;
+}
+
+// raycast_check_visibility (voxel_hashing.cpp:121-133): the blocks visible from
+// the pose (count_visible_entry, ray_tracing.cu:647-807), then the raycast.
+extern "C" int lf_raycast_check_visibility(const lf_map_params *p, const lf_map_storage *s, float *vmap,
+                                           unsigned char *rgb_out, int cols, int rows, const float *pose24,
+                                           const lf_intrinsics *K, int check_colour, void *stream) {
+    if (!p || !s || !vmap || !pose24 || !K) return lf_set_error(LF_ERR_ARG, "null argument to raycast_check_visibility");
+    hipStream_t st = lf_stream(stream);
+    if (int rc = lf_select_visible_ev(p, s, pose24, K, cols, rows, st)) return rc;
+    return lf_raycast_ev(p, s, vmap, rgb_out, cols, rows, pose24, K, check_colour, st, nullptr, nullptr);
 }
 
 extern "C" int lf_raycast(const lf_map_params *p, const lf_map_storage *s, float *vmap, unsigned char *rgb_out,

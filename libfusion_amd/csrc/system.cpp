@@ -162,6 +162,7 @@ DeviceImage::DeviceImage(const IntrinsicMatrix &K, int num_pyr, hipStream_t stre
     depth_raw_.create(K.width, K.height, 4);
     image_.create(K.width, K.height, 3);
     samples_.create(K.width, K.height, 16);
+    rendered_.create(K.width, K.height, 4);
     depth_.resize(num_pyr); intensity_.resize(num_pyr); dx_.resize(num_pyr); dy_.resize(num_pyr);
     vmap_.resize(num_pyr); nmap_.resize(num_pyr);
     for (int l = 0; l < num_pyr; ++l) {
@@ -200,6 +201,18 @@ int DeviceImage::upload(const uint16_t *depth_u16, const unsigned char *rgb, flo
 int DeviceImage::resize_device_map() {
     std::vector<lf_intrinsics> K(K_.begin(), K_.end());
     return lf_pyramid_refresh(levels(), K.data(), levels_view(), stream_);
+}
+
+const GpuImage &DeviceImage::get_rendered_image() {
+    lf_render_scene(vmap_[0].ptr<float>(), nmap_[0].ptr<float>(), nullptr, K_[0].width, K_[0].height,
+                    rendered_.ptr<unsigned char>(), stream_);
+    return rendered_;
+}
+
+const GpuImage &DeviceImage::get_rendered_scene_textured() {
+    lf_render_scene(vmap_[0].ptr<float>(), nmap_[0].ptr<float>(), image_.ptr<unsigned char>(), K_[0].width,
+                    K_[0].height, rendered_.ptr<unsigned char>(), stream_);
+    return rendered_;
 }
 
 int DeviceImage::set_vmap(const float *vmap) {
@@ -270,6 +283,11 @@ int DenseMapping::raycast(float *vmap, const float *pose24, hipEvent_t e0, hipEv
 }
 
 int DenseMapping::reset_mapping() { return lf_map_reset(&params_, &storage_, stream_); }
+
+int DenseMapping::raycast_check_visibility(float *vmap, const float *pose24) {
+    return lf_raycast_check_visibility(&params_, &storage_, vmap, nullptr, K_.width, K_.height, pose24, &K_, 1,
+                                       stream_);
+}
 
 int DenseMapping::deintegrate(DeviceImage &frame, const float *pose24, bool colour) {
     return lf_deintegrate_ev(&params_, &storage_, frame.get_raw_depth().ptr<float>(), frame.get_nmap(0).ptr<float>(),
@@ -612,6 +630,13 @@ static int image_plane(fusion::DeviceImage &d, int level, int plane, void **out)
     return LF_OK;
 }
 
+extern "C" int lf_image_render(lf_image *im, int textured, void **rgba_out) {
+    if (!im || !rgba_out) return lf_set_error(LF_ERR_ARG, "null argument to lf_image_render");
+    const fusion::GpuImage &g = textured ? im->img->get_rendered_scene_textured() : im->img->get_rendered_image();
+    *rgba_out = g.ptr<void>();
+    return LF_OK;
+}
+
 extern "C" int lf_image_plane(lf_image *im, int level, int plane, void **out) {
     if (!im || !out) return lf_set_error(LF_ERR_ARG, "null argument to lf_image_plane");
     return image_plane(*im->img, level, plane, out);
@@ -797,6 +822,16 @@ extern "C" int lf_system_maps(lf_system *s, int level, const float **vmap, const
 }
 
 extern "C" void *lf_system_stream(lf_system *s) { return s->sys->stream(); }
+
+extern "C" int lf_system_render(lf_system *s, int which, void **rgba_out) {
+    if (!s || !rgba_out || which < 0 || which > 2) return lf_set_error(LF_ERR_ARG, "bad arguments to lf_system_render");
+    fusion::DenseOdometry &od = s->sys->odometry();
+    fusion::DeviceImage &img = which == 0 ? od.get_current_image() : od.get_reference_image();
+    const fusion::GpuImage &g = which == 2 ? img.get_rendered_scene_textured() : img.get_rendered_image();
+    LF_HIP_CHECK(hipStreamSynchronize(s->sys->stream()));
+    *rgba_out = g.ptr<void>();
+    return LF_OK;
+}
 
 extern "C" int lf_system_write_map(lf_system *s, const char *path) {
     if (!s || !path) return lf_set_error(LF_ERR_ARG, "null argument to lf_system_write_map");

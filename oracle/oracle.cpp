@@ -280,15 +280,19 @@ template <class R> bool block_visible(I3 bp, const M34<R> &inv, const K_<R> &K, 
 // check_visibility_flag_kernel + scan + copy (depth_fusion.cu:49-93,549-557).
 // DEVIATION: the count includes the last entry (the reference reads the
 // exclusive scan's last element) and the flag buffer starts at zero.
-template <class R> void select_visible(Map<R> &m, const M34<R> &inv, const K_<R> &K, int cols, int rows) {
+// geometric_only: count_visible_entry (ray_tracing.cu:647-807) over the
+// allocated blocks, touched flags left alone.
+template <class R> void select_visible(Map<R> &m, const M34<R> &inv, const K_<R> &K, int cols, int rows,
+                                       bool geometric_only = false) {
     const int n = (int)m.pos.size();
     std::vector<uint8_t> vis(n, 0);
 #pragma omp parallel for schedule(static)
-    for (int i = 0; i < n; ++i) vis[i] = m.touched[i] ? 1 : (block_visible(m.pos[i], inv, K, cols, rows, m.p) ? 1 : 0);
+    for (int i = 0; i < n; ++i)
+        vis[i] = (!geometric_only && m.touched[i]) ? 1 : (block_visible(m.pos[i], inv, K, cols, rows, m.p) ? 1 : 0);
     m.visible.clear();
     for (int i = 0; i < n; ++i) {
         if (vis[i]) m.visible.push_back(i);
-        m.touched[i] = 0;
+        if (!geometric_only) m.touched[i] = 0;
     }
 }
 
@@ -753,6 +757,69 @@ template <class R> void compute_nmap(const R *vm, R *nm, int w, int h) {
         }
 }
 
+// ------------------------------------------------------------ render / warp
+// renderPoint (cuda_imgproc.cu:17-48): Phong shading, light at light_pos in the
+// camera frame; pixel = (ambient, diffuse, specular) colour.  Float arithmetic
+// in the reference's order; saturate-then-truncate to uint8.
+inline void render_point(const float *p, const float *nrm, const float *pixel, uint8_t *out) {
+    float c[3] = {4.f / 255.f, 2.f / 255.f, 2.f / 255.f};
+    if (!(p[0] != p[0])) {
+        const float Ka = 0.3f, Kd = 0.5f, Ks = 0.2f, n = 20.f, Lx = 1.f;
+        const float Ax = pixel[0], Dx = pixel[1], Sx = pixel[2];
+        V3<float> P{p[0], p[1], p[2]}, N{nrm[0], nrm[1], nrm[2]}, light{5.f, 5.f, 5.f}, zero{0.f, 0.f, 0.f};
+        V3<float> L = normalised(sub(light, P));
+        V3<float> V = normalised(sub(zero, P));
+        V3<float> Rv = normalised(sub(mul(mul(N, 2.0f), dot(N, L)), L));
+        const float Ix = Ax * Ka * Dx + Lx * Kd * Dx * std::max(0.f, dot(N, L)) +
+                         Lx * Ks * Sx * std::pow(std::max(0.f, dot(Rv, V)), n);
+        c[0] = c[1] = c[2] = Ix;
+    }
+    for (int k = 0; k < 3; ++k) {
+        float v = c[k];
+        v = v != v ? 0.f : std::min(1.f, std::max(0.f, v));  // __saturatef
+        out[k] = (uint8_t)(v * 255.f);
+    }
+    out[3] = 255;
+}
+
+// renderSceneK / renderSceneTexturedK (cuda_imgproc.cu:50-108); rgb null: plain.
+inline void render_scene(const float *vm, const float *nm, const uint8_t *rgb, int w, int h, uint8_t *out) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < w * h; ++i) {
+        float pixel[3] = {1.f, 1.f, 1.f};
+        if (rgb)
+            for (int k = 0; k < 3; ++k) pixel[k] = (float)rgb[3 * i + k] / 255.f;
+        render_point(vm + 4 * (size_t)i, nm + 4 * (size_t)i, pixel, out + 4 * (size_t)i);
+    }
+}
+
+// interpolate_bilinear + warp_image_kernel (cuda_imgproc.cu:155-185).
+inline void warp_image(const uint8_t *src, const float *vm, int w, int h, const float *P, const K_<float> &K,
+                       uint8_t *dst) {
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            const size_t i = (size_t)y * w + x;
+            uint8_t o[3] = {0, 0, 0};
+            const float *v = vm + 4 * i;
+            const float qx = P[0] * v[0] + P[1] * v[1] + P[2] * v[2] + P[3];
+            const float qy = P[4] * v[0] + P[5] * v[1] + P[6] * v[2] + P[7];
+            const float qz = P[8] * v[0] + P[9] * v[1] + P[10] * v[2] + P[11];
+            const float u = K.fx * qx / qz + K.cx, vv = K.fy * qy / qz + K.cy;
+            if (u >= 1 && vv >= 1 && u < (float)(w - 1) && vv < (float)(h - 1)) {
+                const int u0 = (int)std::floor(u), v0 = (int)std::floor(vv);
+                const float cx = u - (float)u0, cy = vv - (float)v0;
+                const uint8_t *a = src + 3 * ((size_t)v0 * w + u0), *b = a + 3 * (size_t)w;
+                for (int k = 0; k < 3; ++k) {
+                    const float r = ((float)a[k] * (1 - cx) + (float)a[3 + k] * cx) * (1 - cy) +
+                                    ((float)b[k] * (1 - cx) + (float)b[3 + k] * cx) * cy;
+                    o[k] = (uint8_t)(int)r;
+                }
+            }
+            dst[3 * i] = o[0]; dst[3 * i + 1] = o[1]; dst[3 * i + 2] = o[2];
+        }
+}
+
 // ------------------------------------------------------------ reductions
 // Rows are accumulated in 64 fixed chunks (each summed in pixel order), and the
 // chunks in order, so the result does not depend on the thread count.
@@ -1169,6 +1236,14 @@ template <class R> M34<R> m34_from(const R *p) { M34<R> m; for (int i = 0; i < 1
 
 using namespace orc;
 
+extern "C" void orc_render_scene(const float *vm, const float *nm, const uint8_t *rgb, int w, int h, uint8_t *out) {
+    render_scene(vm, nm, rgb, w, h, out);
+}
+extern "C" void orc_warp_image(const uint8_t *src, const float *vm, int w, int h, const float *pose12, const Intr *K,
+                               uint8_t *dst) {
+    warp_image(src, vm, w, h, pose12, K_<float>(*K), dst);
+}
+
 #define ORC_INSTANTIATE(R, S)                                                                                     \
     extern "C" void *orc_map_create_##S(const Params *p) { return new Map<R>(*p); }                            \
     extern "C" void orc_map_free_##S(void *m) { delete static_cast<Map<R> *>(m); }                             \
@@ -1213,6 +1288,11 @@ using namespace orc;
         allocate(M, depth, cols, rows, m34_from(pose24), k);                                                   \
         select_visible(M, m34_from(pose24 + 12), k, cols, rows);                                                           \
         integrate(M, depth, nmap, rgb, cols, rows, m34_from(pose24 + 12), k);                                  \
+    }                                                                                                          \
+    extern "C" int orc_select_visible_##S(void *m, const R *pose24, const Intr *K, int cols, int rows) {         \
+        auto &M = *static_cast<Map<R> *>(m);                                                                   \
+        select_visible(M, m34_from(pose24 + 12), K_<R>(*K), cols, rows, true);                                 \
+        return (int)M.visible.size();                                                                          \
     }                                                                                                          \
     extern "C" void orc_raycast_##S(void *m, R *vmap, uint8_t *rgb, int cols, int rows, const R *pose24,       \
                                     const Intr *K, int check_colour, int instrument) {                        \

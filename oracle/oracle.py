@@ -71,6 +71,7 @@ def lib():
             sig("orc_ray_blocks_touched_%s", I, [P])
             sig("orc_ray_stats_%s", None, [P, P])
             sig("orc_ray_px_%s", None, [P, P])
+            sig("orc_select_visible_%s", I, [P, P, P, I, I])
             sig("orc_track_maps_%s", I, [P, P, P, P, P, I, P, P, I, P])
             sig("orc_zrange_%s", None, [P, P, P, I, I, P, P])
             sig("orc_mesh_%s", ctypes.c_long, [P, I, P, P, P, ctypes.c_long])
@@ -97,6 +98,8 @@ def lib():
         L.orc_se3_exp.argtypes = [P, P]
         L.orc_se3_log.argtypes = [P, P]
         L.orc_ldlt6.argtypes = [P, P, P]
+        L.orc_render_scene.argtypes = [P, P, P, I, I, P]
+        L.orc_warp_image.argtypes = [P, P, I, I, P, P, P]
         _lib = L
     return _lib
 
@@ -224,6 +227,10 @@ class Map:
                                ctypes.byref(K), 1 if check_colour else 0, 1 if instrument else 0)
         return (vmap, rgb) if want_rgb else vmap
 
+    def select_visible(self, T, K, cols, rows):
+        """count_visible_entry: the visible list becomes the frustum-visible blocks at T."""
+        return int(self._f("orc_select_visible")(self.h, _p(pose24(T, self.dtype)), ctypes.byref(K), cols, rows))
+
     def ray_stats(self):
         """Counters of the last instrumented raycast: samples, samples in unallocated
         blocks, trilinear reads, block changes along rays, rays started, hits."""
@@ -316,6 +323,24 @@ def nmap(vmap, dtype=np.float32):
     return out
 
 
+def render_scene(vmap, nmap, rgb=None):
+    """renderScene / renderSceneTextured: RGBA8 Phong shading, light at (5, 5, 5)."""
+    v, n = _c(vmap, np.float32), _c(nmap, np.float32)
+    out = np.zeros(v.shape[:2] + (4,), np.uint8)
+    c = None if rgb is None else _c(rgb, np.uint8)
+    lib().orc_render_scene(_p(v), _p(n), _p(c) if c is not None else None, v.shape[1], v.shape[0], _p(out))
+    return out
+
+
+def warp_image(src, vmap_dst, pose12, K):
+    """warp_image: src (RGB8) sampled where vmap_dst projects under pose12 (float 3x4)."""
+    s, v = _c(src, np.uint8), _c(vmap_dst, np.float32)
+    P = _c(pose12, np.float32)
+    out = np.zeros_like(s)
+    lib().orc_warp_image(_p(s), _p(v), v.shape[1], v.shape[0], _p(P), ctypes.byref(K), _p(out))
+    return out
+
+
 def icp_reduce(curr_v, curr_n, last_v, last_n, pose12, K, dtype=np.float32):
     out = np.zeros(29, dtype)
     cv = _c(curr_v, dtype)
@@ -356,7 +381,7 @@ def track_maps(ref_vmap, ref_rgb, cur_vmap, cur_rgb, K, iters, init=None, contex
     T0 = np.eye(4) if init is None else np.asarray(init, np.float64)
     ini = np.ascontiguousarray(T0[:3, :], np.float64)
     out = np.zeros(12, np.float64)
-    ok = lib()["orc_track_maps_" + _sfx(dtype)](
+    ok = getattr(lib(), "orc_track_maps_" + _sfx(dtype))(
         _p(_c(ref_vmap, dtype)), _p(_c(ref_rgb, np.uint8)), _p(_c(cur_vmap, dtype)), _p(_c(cur_rgb, np.uint8)),
         ctypes.byref(K), len(it), _p(it), _p(ini), 1 if context_mode else 0, _p(out))
     U = np.eye(4)

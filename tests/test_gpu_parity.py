@@ -178,6 +178,38 @@ def test_raycast_bit_exact_on_identical_volume(c0):
             assert np.array_equal(rgb_out.cpu().numpy(), orgb)
 
 
+def test_raycast_check_visibility_from_a_new_pose():
+    """raycast_check_visibility renders the map from a pose no update used:
+    the frustum-visible block set and the vertex map are bit-exact with the
+    oracle on the same volume."""
+    sc = Scene(1, 1)
+    K = half_res(sc)
+    p = capi.map_params(2 ** 18, 1 << 17, 0.01, 0.03)
+    gmap = capi.Map(p, K.width, K.height)
+    for f in (0, 5, 10):
+        T = sc.pose(f)
+        depth, nmap, rgb = frame_inputs(sc, T, K, f)
+        gmap.update_weighted(dev(depth), dev(nmap), dev(rgb), dev(capi.pose24(T)), K)
+    torch.cuda.synchronize()
+    omap = O.Map(o_params(p), np.float32)
+    load_gpu_volume_into_oracle(gmap, omap)
+    T = sc.pose(7) @ np.array([[1, 0, 0, 0.05], [0, 1, 0, 0.0], [0, 0, 1, 0.1], [0, 0, 0, 1.0]])
+    pose = dev(capi.pose24(T))
+    n = gmap.select_visible(pose, K, K.width, K.height)
+    on = omap.select_visible(T, o_K(K), K.width, K.height)
+    assert n == on > 1000
+    _, gv = sort_blocks(gmap.visible_blocks())
+    _, ov = sort_blocks(omap.visible())
+    assert np.array_equal(gv, ov)
+    vm = torch.full((K.height, K.width, 4), float("nan"), device="cuda")
+    gmap.raycast_check_visibility(vm, pose, K)
+    torch.cuda.synchronize()
+    ovm = omap.raycast(T, o_K(K), K.width, K.height)
+    mism, dmax, nboth = vmap_compare(vm.cpu().numpy(), ovm)
+    assert nboth > 0.5 * K.width * K.height
+    assert mism == 0.0 and dmax == 0.0, (mism, dmax)
+
+
 def test_raycast_close_to_surface(c0):
     """Raycast hits of a noise-free view lie on the analytic surface (independent
     of the oracle): within half a voxel for almost every ray."""
@@ -632,6 +664,61 @@ def test_deintegration_undoes_a_frame():
     assert not w1[only_b].any()
     for img in imgs:
         img.close()
+
+
+def test_render_warp_and_semi_dense_match_oracle():
+    """renderScene / renderSceneTextured (RGBA8 Phong; at most one level apart
+    where powf rounds differently), warp_image and the semi-dense image
+    (bit-exact) against the oracle on identical inputs; DeviceImage renders
+    equal the stand-alone render of its planes."""
+    sc = Scene(1, 1)
+    K = half_res(sc)
+    lib, st, P = capi.load(), capi.stream_ptr(), capi.ptr
+    frames = []
+    for f in (0, 2):
+        d16, rgb = sc.render(sc.pose(f), K.width, K.height, K.fx, K.fy, K.cx, K.cy, noise=True, frame_id=f)
+        depth = d16.astype(np.float32) * np.float32(sc.depth_scale)
+        vm = O.back_project(O.bilateral(depth), o_K(K))
+        frames.append((d16, rgb, vm, O.nmap(vm)))
+    _, rgb, vm, nm = frames[1]
+    t_vm, t_nm, t_rgb = dev(vm), dev(nm), dev(rgb)
+    for tex in (None, t_rgb):
+        out = torch.zeros((K.height, K.width, 4), dtype=torch.uint8, device="cuda")
+        capi.check(lib.lf_render_scene(P(t_vm), P(t_nm), P(tex), K.width, K.height, P(out), st))
+        torch.cuda.synchronize()
+        ref = O.render_scene(vm, nm, None if tex is None else rgb)
+        d = np.abs(out.cpu().numpy().astype(int) - ref.astype(int))
+        assert d.max() <= 1 and (d > 0).mean() < 0.01, (d.max(), (d > 0).mean())
+        assert len(np.unique(ref[..., 0])) > 50
+    # warp frame 0's colour into frame 2's view
+    Trel = (np.linalg.inv(sc.pose(0)) @ sc.pose(2))[:3, :].astype(np.float32)
+    src = dev(frames[0][1])
+    dst = torch.zeros_like(src)
+    pose12 = np.ascontiguousarray(Trel.ravel())
+    capi.check(lib.lf_warp_image(P(src), P(t_vm), K.width, K.height, pose12.ctypes.data_as(ctypes.c_void_p),
+                                 ctypes.byref(K), P(dst), st))
+    torch.cuda.synchronize()
+    ref = O.warp_image(frames[0][1], vm, pose12, o_K(K))
+    assert np.array_equal(dst.cpu().numpy(), ref) and (ref.sum(axis=2) > 0).mean() > 0.5
+    # semi-dense image
+    I = O.grey(rgb)
+    dx, dy = O.derivative(I)
+    semi = torch.zeros((K.height, K.width), device="cuda")
+    t_I, t_dx, t_dy = dev(I), dev(dx), dev(dy)
+    capi.check(lib.lf_semi_dense(P(t_I), P(t_dx), P(t_dy), K.width, K.height, 5.0, 5.0, P(semi), st))
+    torch.cuda.synchronize()
+    assert np.array_equal(semi.cpu().numpy(), np.where((dx > 5) | (dy > 5), I, np.float32(255)))
+    # DeviceImage renders: its own level-0 maps and colour
+    img = capi.Image(K, 3)
+    img.upload(dev(frames[1][0]), t_rgb, sc.depth_scale)
+    for tex in (False, True):
+        got = img.render(textured=tex)
+        out = torch.zeros_like(got)
+        v0, n0 = img.plane(0, capi.PLANE_VMAP), img.plane(0, capi.PLANE_NMAP)
+        capi.check(lib.lf_render_scene(P(v0), P(n0), P(t_rgb if tex else None), K.width, K.height, P(out), st))
+        torch.cuda.synchronize()
+        assert torch.equal(got, out)
+    img.close()
 
 
 def level_maps(sc, frame, K):

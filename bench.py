@@ -152,8 +152,22 @@ def main():
         traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_integrate<true,true>", "algorithmic_bytes_per_launch": int(bytes_per_launch),
+                "kernel": "k_integrate<true,true,true> (packed samples)", "algorithmic_bytes_per_launch": int(bytes_per_launch),
                 "avg_launch_ms": round(integrate_avg_ms, 5), "visible_blocks_avg": round(n_vis, 1)}
+
+    # raycast: compulsory bytes (vertex map out, sdf + weight of the distinct
+    # blocks the march reads, their hash entries, z-range tiles), with the
+    # distinct-block count measured by the oracle's instrumented march on C1
+    stage_rooflines = {}
+    rf = os.path.join(ROOT, "profiles", "raycast_c1_touched.json")
+    if "raycast_kernel" in stages and os.path.exists(rf):
+        rb = json.load(open(rf))["algorithmic_bytes_per_raycast"]
+        rs = stages["raycast_kernel"]["avg_ms"] * 1e-3
+        ra = rb / rs / 1e9
+        stage_rooflines["raycast"] = {"bound": "latency (dependent sample chain)", "achieved": round(ra, 1),
+                                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ra / HBM_PEAK_GBS, 4),
+                                      "algorithmic_bytes_per_launch": rb,
+                                      "avg_launch_ms": round(stages["raycast_kernel"]["avg_ms"], 5)}
 
     cpu = None
     if not args.no_cpu:
@@ -161,7 +175,7 @@ def main():
         cpu = cpu_baseline(sc, params, K, fd, fc, args.cpu_frames)
         log("cpu baseline %.2f frames/s in %.1fs" % (cpu["value"], time.perf_counter() - t))
     out = {
-        "metric": "frames/s at 640x480, 5 mm voxels: plain FP32 frame-to-model pipeline",
+        "metric": "frames/s at 640x480, 5 mm voxels: mixed-prec+batched vs plain FP32",
         "value": round(fps, 2),
         "unit": "frames/s",
         "n_gpus": 1,
@@ -173,12 +187,14 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic (seeded analytic room, Perlin albedo, depth noise + 2% dropouts)",
-        "config": {"workload": "C1: 300-frame 640x480 orbit, room + 8 primitives, 5 mm voxels, 15 mm truncation, "
-                               "2^20 buckets, 3-level ICP+RGB {10,5,3}; frames %d-%d timed" % (args.warmup, n_total - 1),
+        "config": {"workload": "C1 plain FP32 GPU path: 300-frame 640x480 orbit, room + 8 primitives, 5 mm voxels, "
+                               "15 mm truncation, 2^20 buckets, 3-level ICP+RGB {10,5,3}; frames %d-%d timed"
+                               % (args.warmup, n_total - 1),
                    "width": K.width, "height": K.height, "voxel_mm": 5, "buckets": params.num_buckets,
                    "pool_blocks": params.num_blocks, "precision": "plain FP32"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "stage_rooflines": stage_rooflines,
         "stages_ms": {k: round(v["avg_ms"], 4) for k, v in stages.items()},
         "allocated_blocks": int(min(c1[capi.CNT_ALLOCATED], params.num_blocks)),
         "overflow": int(c1[capi.CNT_OVERFLOW]),

@@ -61,14 +61,37 @@ LF_DEV int sat16(float v) {
 }
 
 // RenderingBlockDelegate::create_rendering_block (ray_tracing.cu:63-126) and
-// the tile fill (:187-210).
+// the tile fill (:187-210).  kLds: a workgroup first takes the min / max of
+// its blocks' tiles in an LDS copy of the tile grid (its blocks are
+// consecutive visible-list entries, so their rectangles overlap heavily), then
+// flushes the tiles it changed, inside the bounding box of its rectangles, to
+// the global grid with one atomic each.  Min and max do not depend on order,
+// so the tiles are the same as with per-block global atomics.
+template <bool kLds>
 __global__ __launch_bounds__(256) void k_zrange_fill(RayDev r, const float *__restrict__ pose24, lf_intrinsics K) {
+    extern __shared__ int s_tile[];  // kLds: [0, T) near z bits, [T, 2T) far z bits
+    __shared__ int s_box[4];         // ulx, uly, lrx, lry of the workgroup's rectangles
+    const int tid = threadIdx.x;
+    const int nvis = (int)r.counters[LF_CNT_VISIBLE];
+    if ((int)(blockIdx.x * blockDim.x) >= nvis) return;  // whole workgroup
+    const int T = r.tiles_x * r.tiles_y;
+    const int near0 = __float_as_int(100.0f), far0 = __float_as_int(0.0f);  // k_zrange_clear's values
+    if (kLds) {
+        for (int i = tid; i < T; i += blockDim.x) {
+            s_tile[i] = near0;
+            s_tile[T + i] = far0;
+        }
+        if (tid == 0) {
+            s_box[0] = s_box[1] = INT_MAX;
+            s_box[2] = s_box[3] = -1;
+        }
+        __syncthreads();
+    }
     float Tcw[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) Tcw[i] = pose24[12 + i];
-    const int nvis = (int)r.counters[LF_CNT_VISIBLE];
     const float scale = 8.0f * r.voxel_size;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nvis; i += gridDim.x * blockDim.x) {
+    for (int i = blockIdx.x * blockDim.x + tid; i < nvis; i += gridDim.x * blockDim.x) {
         const int4 bp = reinterpret_cast<const int4 *>(r.block_pos)[r.visible[i]];
         int ulx = r.tiles_x, uly = r.tiles_y, lrx = -1, lry = -1;
         float zr0 = r.zmax_raycast, zr1 = r.zmin_raycast;
@@ -97,13 +120,34 @@ __global__ __launch_bounds__(256) void k_zrange_fill(RayDev r, const float *__re
         if (lane_id() == __ffsll((long long)active) - 1 && rend)
             atomicAdd(&r.counters[LF_CNT_RENDER_BLOCKS], (unsigned int)__popcll(rend));
         if (!renders) continue;
+        if (kLds) {
+            atomicMin(&s_box[0], ulx);
+            atomicMin(&s_box[1], uly);
+            atomicMax(&s_box[2], lrx);
+            atomicMax(&s_box[3], lry);
+        }
         const int zi0 = __float_as_int(zr0), zi1 = __float_as_int(zr1);
         for (int y = uly; y <= lry; ++y)
             for (int x = ulx; x <= lrx; ++x) {
                 const int t = y * r.tiles_x + x;
-                atomicMin(reinterpret_cast<int *>(r.zx) + t, zi0);
-                atomicMax(reinterpret_cast<int *>(r.zy) + t, zi1);
+                if (kLds) {
+                    atomicMin(&s_tile[t], zi0);
+                    atomicMax(&s_tile[T + t], zi1);
+                } else {
+                    atomicMin(reinterpret_cast<int *>(r.zx) + t, zi0);
+                    atomicMax(reinterpret_cast<int *>(r.zy) + t, zi1);
+                }
             }
+    }
+    if (kLds) {
+        __syncthreads();
+        const int bx0 = s_box[0], by0 = s_box[1], bw = s_box[2] - bx0 + 1, bh = s_box[3] - by0 + 1;
+        if (bw <= 0 || bh <= 0) return;
+        for (int j = tid; j < bw * bh; j += blockDim.x) {
+            const int t = (by0 + j / bw) * r.tiles_x + bx0 + j % bw;
+            if (s_tile[t] != near0) atomicMin(reinterpret_cast<int *>(r.zx) + t, s_tile[t]);
+            if (s_tile[T + t] != far0) atomicMax(reinterpret_cast<int *>(r.zy) + t, s_tile[T + t]);
+        }
     }
 }
 
@@ -298,6 +342,19 @@ __global__ __launch_bounds__(256) void k_raycast(RayDev r, float *__restrict__ v
     }
 }
 
+// Dynamic LDS available to one workgroup on this device (less the static use).
+size_t zrange_lds_limit() {
+    static size_t limit = 0;
+    if (!limit) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            v = 32768;
+        limit = v > 1024 ? (size_t)v - 1024 : 0;
+    }
+    return limit;
+}
+
 }  // namespace
 
 int lf_raycast_ev(const lf_map_params *p, const lf_map_storage *s, float *vmap, unsigned char *rgb_out, int cols,
@@ -320,7 +377,11 @@ int lf_raycast_ev(const lf_map_params *p, const lf_map_storage *s, float *vmap, 
     const int T = s->tiles_x * s->tiles_y;
     k_zrange_clear<<<std::max(1, std::min(1024, (T + 255) / 256)), 256, 0, st>>>(r);
     LF_LAUNCH_CHECK();
-    k_zrange_fill<<<std::max(1, std::min(2048, (p->num_blocks + 255) / 256)), 256, 0, st>>>(r, pose24, *K);
+    // the per-workgroup tile grid in LDS when it fits (C1: 4800 tiles, 38 KB)
+    const size_t lds = 2 * (size_t)T * sizeof(int);
+    const int zgrid = std::max(1, std::min(2048, (p->num_blocks + 255) / 256));
+    if (lds <= zrange_lds_limit()) k_zrange_fill<true><<<zgrid, 256, lds, st>>>(r, pose24, *K);
+    else k_zrange_fill<false><<<zgrid, 256, 0, st>>>(r, pose24, *K);
     LF_LAUNCH_CHECK();
     const int tx = (cols + 7) / 8, ty = (rows + 7) / 8;
     const int wg_x = (tx + 1) / 2, wg_y = (ty + 1) / 2;  // 2x2 tiles per workgroup

@@ -210,9 +210,15 @@ public:
     DeviceImage &get_reference_image() { return *ref_; }
     DenseTracking &tracker() { return tracker_; }
     const float *pose24() const;
+    bool initialized() const { return initialized_; }
+    // which of the two frame buffers is current (flips with every tracked frame)
+    int buffer_parity() const { return parity_; }
+    // the host side of a tracked frame whose device work was replayed from a graph
+    void replayed_frame() { std::swap(curr_, ref_); parity_ ^= 1; trackingLost = false; }
     bool trackingLost = false;
 
 private:
+    int parity_ = 0;
     std::vector<IntrinsicMatrix> K_;
     std::vector<int> iterations_;
     hipStream_t stream_;
@@ -242,6 +248,11 @@ public:
     // 3 image pipeline, 4 whole frame.
     // stage mask of LF_TIMING_* bits (0: off)
     void set_timing(int stages);
+    // Replay steady-state frames from HIP graphs: each of the two frame-buffer
+    // parities is captured once (the frame's whole device work, inputs copied
+    // into fixed buffers first) and replayed afterwards.  Frames with timing,
+    // before initialisation or with mapping off run eagerly.
+    void set_graphs(bool on);
     int read_timing(int stage, int *launches, double *total_ms);
     bool mapping_enabled = true;
 
@@ -255,6 +266,11 @@ private:
     struct Ev { int stage; hipEvent_t a, b; };
     std::vector<Ev> events_;
     std::vector<hipEvent_t> pool_;
+    int enqueue_frame(const uint16_t *depth, const unsigned char *rgb);
+    void drop_graphs();
+    bool graphs_ = false;
+    hipGraphExec_t exec_[2] = {nullptr, nullptr};
+    GpuImage depth_in_, rgb_in_;
     hipEvent_t take_event();
 };
 

@@ -444,6 +444,12 @@ void *lf_system_stream(lf_system *sys);
 #define LF_TIMING_TRACKING 0x4
 #define LF_TIMING_FRAME 0x10
 int lf_system_set_timing(lf_system *sys, int stages);
+/* Replay steady-state frames from HIP graphs (enable != 0): the first tracked
+ * frame of each of the two frame-buffer parities is captured (its whole device
+ * work; the inputs are first copied into fixed buffers) and later frames
+ * replay it.  Results are those of the eager path; frames with timing on, the
+ * first frame and frames with mapping off run eagerly. */
+int lf_system_set_graphs(lf_system *sys, int enable);
 int lf_system_read_timing(lf_system *sys, int stage, int *launches, double *total_ms);
 
 #ifdef __cplusplus

@@ -133,6 +133,7 @@ SIGNATURES = [
     ("lf_system_read_map", I, [P, ctypes.c_char_p]),
     ("lf_system_stream", P, [P]),
     ("lf_system_set_timing", I, [P, I]),
+    ("lf_system_set_graphs", I, [P, I]),
     ("lf_system_read_timing", I, [P, I, P, P]),
 ]
 
@@ -550,6 +551,9 @@ class System:
         if rc not in (LF_OK, LF_ERR_CAPACITY):
             check(rc)
         return n.value
+
+    def set_graphs(self, on=True):
+        check(load().lf_system_set_graphs(self.h, 1 if on else 0))
 
     def set_timing(self, stages):
         """Stage mask (TIMING_* bits); True means every stage, False none."""

@@ -324,6 +324,9 @@ size_t DenseMapping::fetch_mesh_with_colour(float *vertex, unsigned char *colour
 DenseTracking::DenseTracking(hipStream_t stream) : stream_(stream) {
     if (hipMalloc(&state_, sizeof(LfTrackState)) != hipSuccess) throw std::bad_alloc();
     if (hipMalloc(&scratch_, lf_reduce_partials() * sizeof(float)) != hipSuccess) throw std::bad_alloc();
+    // every field defined before the first frame (errors 0, success 0, counts 0)
+    if (hipMemsetAsync(state_, 0, sizeof(LfTrackState), stream_) != hipSuccess)
+        throw std::runtime_error("hipMemsetAsync failed");
     const double I[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
     lf_track_set_pose(state_, I, stream_);
 }
@@ -430,6 +433,7 @@ int DenseOdometry::trackFrame(const uint16_t *depth, const unsigned char *rgb, f
         initial_pose.matrix3x4(T);
         LF_RC(lf_track_set_pose(tracker_.state(), T, stream_));
         std::swap(curr_, ref_);
+        parity_ ^= 1;
         initialized_ = true;
         return LF_OK;
     }
@@ -441,6 +445,7 @@ int DenseOdometry::trackFrame(const uint16_t *depth, const unsigned char *rgb, f
     // compute_transform ends with the finish, which also chains the odometry pose
     LF_RC(tracker_.compute_transform(*ref_, *curr_, c));
     std::swap(curr_, ref_);
+    parity_ ^= 1;
     trackingLost = false;
     return LF_OK;
 }
@@ -461,6 +466,7 @@ System::System(const lf_system_config &cfg) : cfg_(cfg) {
 }
 
 System::~System() {
+    drop_graphs();
     mapping_.reset();
     odometry_.reset();
     for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
@@ -502,7 +508,66 @@ int System::read_timing(int stage, int *launches, double *total_ms) {
 }
 
 // process_images (system.cpp:40-114): track, fuse, raycast, re-pyramid.
+void System::drop_graphs() {
+    for (hipGraphExec_t &e : exec_) {
+        if (e) (void)hipGraphExecDestroy(e);
+        e = nullptr;
+    }
+}
+
+void System::set_graphs(bool on) {
+    graphs_ = on;
+    if (!on) drop_graphs();
+}
+
+// The frame's device work (trackFrame, update, raycast, re-pyramid) without events.
+int System::enqueue_frame(const uint16_t *depth, const unsigned char *rgb) {
+    DenseOdometry &od = *odometry_;
+    if (!od.trackingLost) LF_RC(od.trackFrame(depth, rgb, cfg_.depth_scale, initial_pose_));
+    DeviceImage &ref = od.get_reference_image();
+    if (mapping_enabled) LF_RC(mapping_->update(ref, od.pose24(), cfg_.integrate_colour != 0));
+    LF_RC(mapping_->raycast(ref.get_vmap(0).ptr<float>(), od.pose24()));
+    return ref.resize_device_map();
+}
+
 int System::process_images(const uint16_t *depth, const unsigned char *rgb) {
+    DenseOdometry &od = *odometry_;
+    if (graphs_ && timing_ == 0 && mapping_enabled && od.initialized() && !od.trackingLost) {
+        // steady state: the inputs go to fixed buffers, the frame is a graph of
+        // this buffer parity (captured on its first use, replayed after)
+        const int W = cfg_.K.width, H = cfg_.K.height;
+        if (depth_in_.empty()) {
+            depth_in_.create(W, H, 2);
+            rgb_in_.create(W, H, 3);
+        }
+        LF_HIP_CHECK(hipMemcpyAsync(depth_in_.ptr<void>(), depth, (size_t)W * H * 2, hipMemcpyDeviceToDevice, stream_));
+        LF_HIP_CHECK(hipMemcpyAsync(rgb_in_.ptr<void>(), rgb, (size_t)W * H * 3, hipMemcpyDeviceToDevice, stream_));
+        const int k = od.buffer_parity();
+        if (exec_[k]) {
+            LF_HIP_CHECK(hipGraphLaunch(exec_[k], stream_));
+            od.replayed_frame();
+            return LF_OK;
+        }
+        hipGraph_t g = nullptr;
+        LF_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_frame(depth_in_.ptr<uint16_t>(), rgb_in_.ptr<unsigned char>());
+        const hipError_t ec = hipStreamEndCapture(stream_, &g);
+        if (rc != LF_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess) return lf_set_error(LF_ERR_HIP, "frame capture failed: %s", hipGetErrorString(ec));
+        const hipError_t ei = hipGraphInstantiate(&exec_[k], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            exec_[k] = nullptr;
+            return lf_set_error(LF_ERR_HIP, "frame graph instantiation failed: %s", hipGetErrorString(ei));
+        }
+        // the capture swapped the frame buffers on the host; the graph of the
+        // parity the frame started in runs now
+        LF_HIP_CHECK(hipGraphLaunch(exec_[k], stream_));
+        return LF_OK;
+    }
     // events of the stages selected by set_timing (stage s: bit s)
     Ev fr{4, nullptr, nullptr}, ig{0, nullptr, nullptr}, rc{1, nullptr, nullptr}, tr{2, nullptr, nullptr};
     const bool t_fr = timing_ & LF_TIMING_FRAME, t_ig = (timing_ & LF_TIMING_INTEGRATE) && mapping_enabled,
@@ -512,7 +577,6 @@ int System::process_images(const uint16_t *depth, const unsigned char *rgb) {
         if (on) { e->a = take_event(); e->b = take_event(); }
     }
     if (t_fr) LF_HIP_CHECK(hipEventRecord(fr.a, stream_));
-    DenseOdometry &od = *odometry_;
     if (!od.trackingLost) {
         if (t_tr) LF_HIP_CHECK(hipEventRecord(tr.a, stream_));
         LF_RC(od.trackFrame(depth, rgb, cfg_.depth_scale, initial_pose_));
@@ -841,6 +905,12 @@ extern "C" int lf_system_write_map(lf_system *s, const char *path) {
 extern "C" int lf_system_read_map(lf_system *s, const char *path) {
     if (!s || !path) return lf_set_error(LF_ERR_ARG, "null argument to lf_system_read_map");
     return s->sys->readMapFromDisk(path);
+}
+
+extern "C" int lf_system_set_graphs(lf_system *s, int enable) {
+    if (!s) return lf_set_error(LF_ERR_ARG, "null system");
+    s->sys->set_graphs(enable != 0);
+    return LF_OK;
 }
 
 extern "C" int lf_system_set_timing(lf_system *s, int stages) {

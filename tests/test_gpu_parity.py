@@ -800,6 +800,45 @@ def test_tracked_sequence_vs_fp64_oracle():
     assert dt < 0.05 and dr < 2.0, (dt, dr)
 
 
+def test_graph_replayed_frames_equal_eager_frames():
+    """Frames replayed from the captured HIP graphs (both buffer parities, many
+    replays, with eager work and synchronisations in between) give the eager
+    path's pose at every frame and the same volume."""
+    sc = Scene(1, 1)
+    K = half_res(sc)
+    p = capi.map_params(2 ** 18, 1 << 17, 0.01, 0.03)
+    eager = capi.System(K, p, 3, (10, 5, 3), sc.depth_scale)
+    graph = capi.System(K, p, 3, (10, 5, 3), sc.depth_scale)
+    graph.set_graphs(True)
+    for f in range(10):
+        d16, rgb = sc.render(sc.pose(f), K.width, K.height, K.fx, K.fy, K.cx, K.cy, noise=True, frame_id=f)
+        t_d, t_rgb = dev(d16), dev(rgb)
+        torch.cuda.synchronize()
+        eager.process(t_d, t_rgb)
+        eager.synchronize()
+        graph.process(t_d, t_rgb)
+        graph.synchronize()
+        assert np.array_equal(eager.pose(), graph.pose()), f
+        ta, tb = eager.tracking(), graph.tracking()
+        assert ta[0] == tb[0] and np.array_equal(np.float32(ta[1]), np.float32(tb[1]), equal_nan=True), f
+    a, b = eager.counters(), graph.counters()
+    assert a[capi.CNT_ALLOCATED] == b[capi.CNT_ALLOCATED] > 1000
+    assert a[capi.CNT_UPDATES] == b[capi.CNT_UPDATES] == 10
+    pa, sa = eager.map_storage()
+    pb, sb = graph.map_storage()
+    n = int(a[capi.CNT_ALLOCATED])
+    blocks = []
+    for s in (sa, sb):
+        pos = torch.empty(4 * n, dtype=torch.int32, device="cuda")
+        capi.copy_device(pos, s.block_pos, pos.numel() * 4)
+        w = torch.empty(512 * n, dtype=torch.float32, device="cuda")
+        capi.copy_device(w, s.weight, w.numel() * 4)
+        blocks.append((pos.view(n, 4)[:, :3].cpu().numpy(), w.view(n, 512).cpu().numpy()))
+    ia, ib = np.lexsort(blocks[0][0].T[::-1]), np.lexsort(blocks[1][0].T[::-1])
+    assert np.array_equal(blocks[0][0][ia], blocks[1][0][ib])
+    assert np.array_equal(blocks[0][1][ia], blocks[1][1][ib])
+
+
 def test_gpu_end_to_end_matches_golden_c0():
     """C0 through the C-ABI only (depth conversion, bilateral, normals, update,
     raycast, mesh) against the committed fixture tests/golden/c0.npz."""

@@ -19,7 +19,6 @@ struct LfTrackState {
     int invalid;         // an error became NaN this frame
     int success;         // result of the last finished frame
     int iterations;      // Gauss-Newton steps taken this frame
-    float last_icp_sums[29], last_rgb_sums[29];
     // odometry chain (DenseOdometry::trackFrame, rgbd_odometry.cpp:18-48)
     double pose[12];     // T_world_camera of the current frame
     float pose24[24];    // float T_world_camera and T_camera_world for the mapping kernels

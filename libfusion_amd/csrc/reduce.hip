@@ -177,10 +177,14 @@ LF_DEV void accumulate(float *s, const float *row) {
 }
 
 // ---------------------------------------------------------- FP64 SE(3) helpers
+// The FP64 helpers let the compiler fuse multiply-adds (one rounding instead
+// of two, half the instructions of the serial step); the FP32 row arithmetic
+// above keeps the reference's separate roundings.
 LF_DEV void m34_identity(double *T) {
     for (int i = 0; i < 12; ++i) T[i] = (i == 0 || i == 5 || i == 10) ? 1.0 : 0.0;
 }
 LF_DEV void m34_mul(const double *A, const double *B, double *C) {
+#pragma clang fp contract(fast)
     double t[12];
     for (int i = 0; i < 3; ++i) {
         for (int j = 0; j < 3; ++j) t[i * 4 + j] = A[i * 4] * B[j] + A[i * 4 + 1] * B[4 + j] + A[i * 4 + 2] * B[8 + j];
@@ -189,6 +193,7 @@ LF_DEV void m34_mul(const double *A, const double *B, double *C) {
     for (int i = 0; i < 12; ++i) C[i] = t[i];
 }
 LF_DEV void m34_inv(const double *A, double *C) {
+#pragma clang fp contract(fast)
     double t[12];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) t[i * 4 + j] = A[j * 4 + i];
@@ -196,16 +201,32 @@ LF_DEV void m34_inv(const double *A, double *C) {
     for (int i = 0; i < 12; ++i) C[i] = t[i];
 }
 // SE(3) exponential, tangent [upsilon; omega] (Sophus order); Barfoot sec. 7.1.
+// The coefficients A = sin t / t, B = (1 - cos t) / t^2, C = (t - sin t) / t^3:
+// below t = 0.02 (every step of a converging track) their Maclaurin series to
+// the t^8 term, whose remainder is below 1e-20 relative there and which avoids
+// the cancellation of 1 - cos t; above it one sincos and one division.
 LF_DEV void se3_exp(const double *xi, double *T) {
+#pragma clang fp contract(fast)
     const double *u = xi, *w = xi + 3;
-    const double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2], th = sqrt(th2);
+    const double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
     const double W[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
     double W2[9];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) W2[i * 3 + j] = W[i * 3] * W[j] + W[i * 3 + 1] * W[3 + j] + W[i * 3 + 2] * W[6 + j];
     double A, B, C;
-    if (th < 1e-8) { A = 1 - th2 / 6; B = 0.5 - th2 / 24; C = 1.0 / 6 - th2 / 120; }
-    else { A = sin(th) / th; B = (1 - cos(th)) / th2; C = (th - sin(th)) / (th2 * th); }
+    if (th2 < 4e-4) {
+        const double t = th2;
+        A = 1.0 - t * (1.0 / 6) * (1.0 - t * (1.0 / 20) * (1.0 - t * (1.0 / 42) * (1.0 - t * (1.0 / 72))));
+        B = 0.5 * (1.0 - t * (1.0 / 12) * (1.0 - t * (1.0 / 30) * (1.0 - t * (1.0 / 56) * (1.0 - t * (1.0 / 90)))));
+        C = (1.0 / 6) * (1.0 - t * (1.0 / 20) * (1.0 - t * (1.0 / 42) * (1.0 - t * (1.0 / 72) * (1.0 - t * (1.0 / 110)))));
+    } else {
+        const double th = sqrt(th2), r = 1.0 / th;
+        double s, c;
+        sincos(th, &s, &c);
+        A = s * r;
+        B = (1.0 - c) * r * r;
+        C = (th - s) * r * r * r;
+    }
     double V[9];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) {
@@ -217,6 +238,7 @@ LF_DEV void se3_exp(const double *xi, double *T) {
 }
 // Norm of the SE(3) logarithm (Barfoot sec. 7.1.3).
 LF_DEV double se3_log_norm(const double *T) {
+#pragma clang fp contract(fast)
     double c = (T[0] + T[5] + T[10] - 1) * 0.5;
     c = fmax(-1.0, fmin(1.0, c));
     double th = acos(c);
@@ -262,30 +284,48 @@ LF_DEV double se3_log_norm(const double *T) {
     }
     return sqrt(n2);
 }
-// LDL^T solve of a symmetric positive definite 6x6 (no pivoting).
+// LDL^T solve of a symmetric positive definite 6x6 (no pivoting); one
+// division per pivot (its reciprocal scales the column and the solution).
 LF_DEV void ldlt6(const double *A, const double *b, double *x) {
-    double L[36], D[6];
+#pragma clang fp contract(fast)
+    double L[36], D[6], Dinv[6];
+#pragma unroll
     for (int i = 0; i < 36; ++i) L[i] = 0.0;
+#pragma unroll
     for (int j = 0; j < 6; ++j) {
+        // row j of L D, shared by the updates of the rows below it
+        double LD[6];
         double d = A[j * 6 + j];
-        for (int k = 0; k < j; ++k) d -= L[j * 6 + k] * L[j * 6 + k] * D[k];
+#pragma unroll
+        for (int k = 0; k < j; ++k) {
+            LD[k] = L[j * 6 + k] * D[k];
+            d -= L[j * 6 + k] * LD[k];
+        }
         D[j] = d;
+        Dinv[j] = d != 0.0 ? 1.0 / d : 0.0;
         L[j * 6 + j] = 1.0;
+#pragma unroll
         for (int i = j + 1; i < 6; ++i) {
             double s = A[i * 6 + j];
-            for (int k = 0; k < j; ++k) s -= L[i * 6 + k] * L[j * 6 + k] * D[k];
-            L[i * 6 + j] = d != 0.0 ? s / d : 0.0;
+#pragma unroll
+            for (int k = 0; k < j; ++k) s -= L[i * 6 + k] * LD[k];
+            L[i * 6 + j] = s * Dinv[j];
         }
     }
     double y[6];
+#pragma unroll
     for (int i = 0; i < 6; ++i) {
         double s = b[i];
+#pragma unroll
         for (int k = 0; k < i; ++k) s -= L[i * 6 + k] * y[k];
         y[i] = s;
     }
-    for (int i = 0; i < 6; ++i) y[i] = D[i] != 0.0 ? y[i] / D[i] : 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) y[i] *= Dinv[i];
+#pragma unroll
     for (int i = 5; i >= 0; --i) {
         double s = y[i];
+#pragma unroll
         for (int k = i + 1; k < 6; ++k) s -= L[k * 6 + i] * x[k];
         x[i] = s;
     }
@@ -301,20 +341,43 @@ LF_DEV void unpack29(const float *s, float *A, float *b) {
         }
 }
 
+// What a Gauss-Newton step reads of the tracker state.  k_track_final loads
+// it at entry, together with the flags that decide whether the step runs, so
+// the serial step starts from registers rather than from dependent loads.
+struct GnState {
+    double est[12];
+    float icp_error, rgb_error;
+    int icp_count, rgb_count, active, invalid, iterations;
+};
+
+LF_DEV GnState load_gn(const LfTrackState &st) {
+    GnState g;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) g.est[k] = st.est[k];
+    g.icp_error = st.icp_error;
+    g.rgb_error = st.rgb_error;
+    g.icp_count = st.icp_count;
+    g.rgb_count = st.rgb_count;
+    g.active = st.active;
+    g.invalid = st.invalid;
+    g.iterations = st.iterations;
+    return g;
+}
+
 // One Gauss-Newton step of compute_transform (icp_tracker.cpp:50-139), run by
-// a single lane after the sums are final.
-LF_DEV void gn_step(LfTrackState &st, const float *si, const float *sr, int first_iter, float icp_weight,
-                    bool context_mode) {
+// a single lane after the sums are final: reads g, updates g and writes the
+// step's results to st.
+LF_DEV void gn_step(LfTrackState &st, GnState &g, const float *si, const float *sr, int first_iter,
+                    float icp_weight, bool context_mode) {
     if (first_iter) {
-        st.icp_error = FLT_MAX;
-        st.rgb_error = FLT_MAX;
-        st.icp_count = st.rgb_count = 0;
-        st.active = 1;
+        g.icp_error = FLT_MAX;
+        g.rgb_error = FLT_MAX;
+        g.icp_count = g.rgb_count = 0;
+        g.active = 1;
     }
-    for (int k = 0; k < 29; ++k) { st.last_icp_sums[k] = si[k]; st.last_rgb_sums[k] = sr[k]; }
     double last[12];
-    for (int k = 0; k < 12; ++k) last[k] = st.est[k];
-    const float last_icp = st.icp_error, last_rgb = st.rgb_error;
+    for (int k = 0; k < 12; ++k) last[k] = g.est[k];
+    const float last_icp = g.icp_error, last_rgb = g.rgb_error;
     st.last_icp_error = last_icp;
     st.stddev = sqrtf(sr[27] / (sr[28] - 6.0f));
     float Ai[36], bi[6], Ar[36], br[6];
@@ -326,46 +389,57 @@ LF_DEV void gn_step(LfTrackState &st, const float *si, const float *sr, int firs
     ldlt6(A, b, x);
     double E[12];
     se3_exp(x, E);
-    for (int k = 0; k < 12; ++k) st.prev[k] = st.est[k];
-    m34_mul(E, last, st.est);
-    st.iterations++;
-    bool stop = false;
+    m34_mul(E, last, g.est);
+    g.iterations++;
+    bool stop = false, revert = false;
     float icp_error = sqrtf(si[27]) / si[28];
     if (isnan(icp_error) && !context_mode) {
-        st.invalid = 1;
+        g.invalid = 1;
         stop = true;
     } else if (icp_error > last_icp) {
-        if (st.icp_count >= 2) {
-            for (int k = 0; k < 12; ++k) st.est[k] = st.prev[k];
-            stop = true;
+        if (g.icp_count >= 2) {
+            revert = stop = true;
         } else {
-            st.icp_count++;
+            g.icp_count++;
             icp_error = last_icp;
         }
     } else {
-        st.icp_count = 0;
+        g.icp_count = 0;
     }
-    st.icp_error = icp_error;
+    g.icp_error = icp_error;
     if (!stop) {
         float rgb_error = sqrtf(sr[27]) / sr[28];
         if (isnan(rgb_error) && !context_mode) {
-            st.invalid = 1;
+            g.invalid = 1;
             stop = true;
         } else if (rgb_error > last_rgb) {
-            if (st.rgb_count >= 2) {
-                for (int k = 0; k < 12; ++k) st.est[k] = st.prev[k];
-                stop = true;
+            if (g.rgb_count >= 2) {
+                revert = stop = true;
             } else {
-                st.rgb_count++;
+                g.rgb_count++;
                 rgb_error = last_rgb;
             }
         } else {
-            st.rgb_count = 0;
+            g.rgb_count = 0;
         }
-        st.rgb_error = rgb_error;
+        g.rgb_error = rgb_error;
     }
-    st.active = stop ? 0 : 1;
-    for (int k = 0; k < 12; ++k) st.pose_f[k] = (float)st.est[k];
+    // Revertable<SE3d>::revert: the estimate goes back to the one before the step
+    if (revert)
+        for (int k = 0; k < 12; ++k) g.est[k] = last[k];
+    g.active = stop ? 0 : 1;
+    for (int k = 0; k < 12; ++k) {
+        st.prev[k] = last[k];
+        st.est[k] = g.est[k];
+        st.pose_f[k] = (float)g.est[k];
+    }
+    st.icp_error = g.icp_error;
+    st.rgb_error = g.rgb_error;
+    st.icp_count = g.icp_count;
+    st.rgb_count = g.rgb_count;
+    st.active = g.active;
+    st.invalid = g.invalid;
+    st.iterations = g.iterations;
 }
 
 // Wavefront sum of N values per lane (N a power of two, <= 64) as a
@@ -439,13 +513,17 @@ LF_DEV void final_sums(const float *partials, int G, float *s_sum) {
     for (int j = 0; j < kPer; ++j) {
         const int v = wave + kWaves * j;
         const float *src = partials + (size_t)(v < kVals ? v : 0) * G;
-        // every load of the value first (compile-time bound, predicated), then
-        // the fixed-order sum: the loads overlap instead of waiting in turn
+        // every load of the value first, then the fixed-order sum.  Indices
+        // past G are clamped and their values masked to +0 with a bitwise AND
+        // (a select of a loaded value becomes a branch around the load with a
+        // wait inside it), so all the loads are in flight together.
         float x[kChunks];
 #pragma unroll
         for (int c = 0; c < kChunks; ++c) {
             const int g = c * 64 + lane;
-            x[c] = (v < kVals && g < G) ? src[g] : 0.0f;
+            const float t = src[g < G ? g : G - 1];
+            const unsigned keep = (v < kVals && g < G) ? 0xffffffffu : 0u;
+            x[c] = __uint_as_float(__float_as_uint(t) & keep);
         }
         float s = 0.0f;
 #pragma unroll
@@ -492,12 +570,13 @@ __global__ __launch_bounds__(kThreads) void k_reduce_final(ReduceArgs a, int G) 
 // End of DenseTracking::compute_transform (icp_tracker.cpp:143-155) and the
 // odometry chain of DenseOdometry::trackFrame (rgbd_odometry.cpp:39): success
 // test, update = est^-1 (or identity), pose = pose * update, pose24 refreshed.
-LF_DEV void track_finish(LfTrackState &st, bool context_mode = false) {
+// est and invalid are the frame's final estimate and flag (already in st).
+LF_DEV void track_finish(LfTrackState &st, const double *est, int invalid, bool context_mode) {
     double inv_est[12], D[12];
-    m34_inv(st.est, inv_est);
+    m34_inv(est, inv_est);
     m34_mul(inv_est, st.init, D);
     // compute_transform(ctx) reports success unconditionally (icp_tracker.cpp:308-313)
-    const bool ok = context_mode || (!st.invalid && !(se3_log_norm(D) > 0.1));
+    const bool ok = context_mode || (!invalid && !(se3_log_norm(D) > 0.1));
     st.success = ok ? 1 : 0;
     double upd[12];
     if (ok) for (int k = 0; k < 12; ++k) upd[k] = inv_est[k];
@@ -564,29 +643,35 @@ __global__ __launch_bounds__(kThreads) void k_track_step(ReduceArgs a) {
 __global__ __launch_bounds__(kThreads) void k_track_final(ReduceArgs a, int G) {
     __shared__ float s_sum[kVals];
     const int tid = threadIdx.x;
-    if (!a.first_of_frame) {
-        const LfTrackState &st = *a.st;
-        if (!(!st.invalid && (a.first_iter || st.active))) return;
+    GnState g = load_gn(*a.st);
+    if (a.first_of_frame) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) g.est[k] = a.init[k];
+        g.invalid = 0;
+        g.iterations = 0;
+    } else if (!(!g.invalid && (a.first_iter || g.active))) {
+        return;
     }
     final_sums(a.partials, G, s_sum);
     __syncthreads();
     if (tid == 0) {
         LfTrackState &st = *a.st;
-        if (a.first_of_frame) {
-            for (int k = 0; k < 12; ++k) st.est[k] = st.prev[k] = st.init[k] = a.init[k];
-            st.invalid = 0;
-            st.iterations = 0;
-        }
-        gn_step(st, s_sum, s_sum + 29, a.first_iter, a.icp_weight, a.context_mode != 0);
+        if (a.first_of_frame)
+            for (int k = 0; k < 12; ++k) st.init[k] = a.init[k];
+        gn_step(st, g, s_sum, s_sum + 29, a.first_iter, a.icp_weight, a.context_mode != 0);
         // the frame ends here when the estimate became invalid, when the
         // finest level broke out of its loop, or after the frame's last step
-        if (st.invalid || (a.last_level && !st.active) || a.last_of_frame) track_finish(st, a.context_mode != 0);
+        if (g.invalid || (a.last_level && !g.active) || a.last_of_frame)
+            track_finish(st, g.est, g.invalid, a.context_mode != 0);
     }
 }
 
 __global__ void k_track_finish(LfTrackState *st) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    track_finish(*st);
+    LfTrackState &s = *st;
+    double est[12];
+    for (int k = 0; k < 12; ++k) est[k] = s.est[k];
+    track_finish(s, est, s.invalid, false);
 }
 
 __global__ void k_track_begin(LfTrackState *st, double t0, double t1, double t2, double t3, double t4, double t5,

@@ -45,11 +45,13 @@ LF_HD f3 rotate(const float *m, f3 v) {
 
 struct i3 { int x, y, z; };
 
-// ToVector3i (vector_type.h:218-225) with NaN / out-of-range sent to kBad.
+// ToVector3i (vector_type.h:218-225) with NaN / out-of-range sent to kBad:
+// floor, one range test on |floor(v)| (false for NaN), one conversion.  Equal
+// to the truncate-and-adjust form (b = (int)v; b - ((float)b > v)) for every
+// float inside the range, checked over all 2^32 bit patterns.
 LF_HD int floor_i(float v) {
-    if (!(v > -2147483520.0f && v < 2147483520.0f)) return kBad;
-    int b = (int)v;
-    return (float)b > v ? b - 1 : b;
+    const float f = floorf(v);
+    return fabsf(f) < 2147483520.0f ? (int)f : kBad;
 }
 LF_HD i3 to_i3(f3 v) { return {floor_i(v.x), floor_i(v.y), floor_i(v.z)}; }
 // __float2int_rd of a projected coordinate; non-finite / huge values leave every image.
@@ -80,7 +82,15 @@ LF_HD unsigned long long pack_key(int x, int y, int z) {
 // raycast decides validity from the 2-byte sdf alone, without the weight.
 constexpr short kSdfUnseen = -32768;
 
-LF_HD float unpack_sdf(short v) { return (float)v / 32767.0f; }
+// v / 32767 correctly rounded, as a product with the rounded reciprocal and one
+// fused correction from the exact residual: bit-identical to the IEEE
+// division for all 65536 inputs (checked exhaustively), in 3 instructions
+// instead of the division's 11.
+LF_HD float unpack_sdf(short v) {
+    const float x = (float)v, r = 1.0f / 32767.0f;
+    const float q = x * r;
+    return fmaf(fmaf(-q, 32767.0f, x), r, q);
+}
 LF_HD short pack_sdf(float v) { return (short)(v * 32767.0f); }
 
 // Occupancy bit of a block: its 4x4x4-block region hashed like computeHash.

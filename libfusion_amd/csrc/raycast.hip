@@ -21,7 +21,6 @@
 //    probe entirely.
 #include "lf_common.h"
 #include "lf_internal.h"
-#include "lf_internal.h"
 
 using namespace lf;
 

@@ -35,7 +35,11 @@ using namespace lf;
 
 namespace {
 
-constexpr int kGrid = 1024;    // maximum workgroups of a reduction
+// Maximum workgroups of a reduction: the workgroups resident at once on
+// MI355X (256 CUs x 2 workgroups of 512 lanes at the step kernel's register
+// count).  A larger level loops over pixels with the next pixel's loads in
+// flight, rather than waiting for a second round of workgroups.
+constexpr int kGrid = 512;
 constexpr int kThreads = 512;  // lanes per workgroup (8 wavefronts)
 constexpr int kWaves = kThreads / 64;
 constexpr int kVals = 58;      // 29 ICP + 29 RGB sums
@@ -708,8 +712,7 @@ int check_level(const LfTrackLevel &lv) {
 
 float icp_angle() { return (float)cos(30 * 3.14 / 180); }
 
-// Workgroups of a reduction: one pixel per lane (every pixel of a level is in
-// flight at once), at most kGrid workgroups.
+// Workgroups of a reduction: one pixel per lane, at most kGrid workgroups.
 int grid_for(const LfTrackLevel &lv) {
     const long n = (long)lv.cols * lv.rows;
     return (int)std::max(1L, std::min((long)kGrid, (n + kThreads - 1) / kThreads));

@@ -209,22 +209,31 @@ LF_DEV bool block_visible(int bx, int by, int bz, const float *Tcw, const lf_int
     return false;
 }
 
+constexpr int kVisThreads = 1024;  // lanes of a k_visible workgroup
+constexpr int kVisGrid = 512;      // at most this many workgroups (grid-stride beyond)
+
 // check_visibility_flag_kernel + exclusive scan + copy (depth_fusion.cu:49-93,
 // 549-557) over the allocated blocks: touched-this-update, or frustum-visible.
 // kGeometric: count_visible_entry (ray_tracing.cu:647-807), frustum-visible
-// only, touched flags left alone.
+// only, touched flags left alone.  The list is appended with one global atomic
+// per workgroup (a wavefront ballot, then the wavefront counts prefixed in
+// LDS): same-address atomics serialise, so one per wavefront cost most of the
+// kernel.  The list order is unspecified, as with the per-wavefront append.
 template <bool kGeometric = false>
-__global__ __launch_bounds__(256) void k_visible(MapDev m, const float *__restrict__ pose24, lf_intrinsics K,
-                                                 int cols, int rows) {
+__global__ __launch_bounds__(kVisThreads) void k_visible(MapDev m, const float *__restrict__ pose24, lf_intrinsics K,
+                                                         int cols, int rows) {
+    __shared__ unsigned int s_cnt[kVisThreads / kWave];
+    __shared__ unsigned int s_base;
     float Tcw[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) Tcw[i] = pose24[12 + i];
     const unsigned int n = min(m.counters[LF_CNT_ALLOCATED], (unsigned int)m.num_blocks);
     const float scale = 8.0f * m.voxel_size;
     const unsigned int stride = gridDim.x * blockDim.x;
-    const unsigned int n_round = (n + kWave - 1) / kWave * kWave;
-    const int lane = lane_id();
-    for (unsigned int b = blockIdx.x * blockDim.x + threadIdx.x; b < n_round; b += stride) {
+    const int lane = lane_id(), wave = threadIdx.x / kWave;
+    // the trip count is uniform over the workgroup (its barriers need every lane)
+    for (unsigned int b0 = blockIdx.x * blockDim.x; b0 < n; b0 += stride) {
+        const unsigned int b = b0 + threadIdx.x;
         bool vis = false;
         if (b < n) {
             const int4 p = reinterpret_cast<const int4 *>(m.block_pos)[b];
@@ -236,13 +245,25 @@ __global__ __launch_bounds__(256) void k_visible(MapDev m, const float *__restri
             }
         }
         const unsigned long long ballot = __ballot(vis);
-        if (ballot == 0ull) continue;
-        const int leader = __ffsll((long long)ballot) - 1;
-        unsigned int base = 0;
-        if (lane == leader) base = atomicAdd(&m.counters[LF_CNT_VISIBLE], (unsigned int)__popcll(ballot));
-        base = __shfl(base, leader);
-        if (vis) m.visible[base + __popcll(ballot & ((1ull << lane) - 1ull))] = (int)b;
+        if (lane == 0) s_cnt[wave] = (unsigned int)__popcll(ballot);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned int total = 0;
+            for (int w = 0; w < kVisThreads / kWave; ++w) {
+                const unsigned int c = s_cnt[w];
+                s_cnt[w] = total;
+                total += c;
+            }
+            s_base = total ? atomicAdd(&m.counters[LF_CNT_VISIBLE], total) : 0u;
+        }
+        __syncthreads();
+        if (vis) m.visible[s_base + s_cnt[wave] + __popcll(ballot & ((1ull << lane) - 1ull))] = (int)b;
+        __syncthreads();  // s_cnt and s_base are rewritten by the next iteration
     }
+}
+
+int visible_grid(const lf_map_params *p) {
+    return std::min(kVisGrid, std::max(1, (p->num_blocks + kVisThreads - 1) / kVisThreads));
 }
 
 // update_map_weighted_kernel (kWeighted, depth_fusion.cu:356-428) and
@@ -431,8 +452,7 @@ int lf_update_ev(const lf_map_params *p, const lf_map_storage *s, const float *d
     dim3 g((cols + 15) / 16, (rows + 15) / 16);
     k_alloc<true><<<g, 256, 0, st>>>(m, depth, cols, rows, pose24, *K);
     LF_LAUNCH_CHECK();
-    int nb = std::min(2048, std::max(1, (p->num_blocks + 255) / 256));
-    k_visible<<<nb, 256, 0, st>>>(m, pose24, *K, cols, rows);
+    k_visible<<<visible_grid(p), kVisThreads, 0, st>>>(m, pose24, *K, cols, rows);
     LF_LAUNCH_CHECK();
     const int G = integrate_grid();
     if (e0) LF_HIP_CHECK(hipEventRecord(e0, st));
@@ -459,8 +479,7 @@ int lf_deintegrate_ev(const lf_map_params *p, const lf_map_storage *s, const flo
     dim3 g((cols + 15) / 16, (rows + 15) / 16);
     k_alloc<false><<<g, 256, 0, st>>>(m, depth, cols, rows, pose24, *K);
     LF_LAUNCH_CHECK();
-    int nb = std::min(2048, std::max(1, (p->num_blocks + 255) / 256));
-    k_visible<<<nb, 256, 0, st>>>(m, pose24, *K, cols, rows);
+    k_visible<<<visible_grid(p), kVisThreads, 0, st>>>(m, pose24, *K, cols, rows);
     LF_LAUNCH_CHECK();
     const int G = integrate_grid();
     const float4 *smp = reinterpret_cast<const float4 *>(samples);
@@ -493,8 +512,7 @@ int lf_select_visible_ev(const lf_map_params *p, const lf_map_storage *s, const 
     const MapDev m = make_dev(p, s);
     k_reset_visible<<<1, 64, 0, st>>>(m);
     LF_LAUNCH_CHECK();
-    int nb = std::min(2048, std::max(1, (p->num_blocks + 255) / 256));
-    k_visible<true><<<nb, 256, 0, st>>>(m, pose24, *K, cols, rows);
+    k_visible<true><<<visible_grid(p), kVisThreads, 0, st>>>(m, pose24, *K, cols, rows);
     LF_LAUNCH_CHECK();
     return LF_OK;
 }

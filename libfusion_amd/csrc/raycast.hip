@@ -70,9 +70,11 @@ template <bool kLds>
 __global__ __launch_bounds__(256) void k_zrange_fill(RayDev r, const float *__restrict__ pose24, lf_intrinsics K) {
     extern __shared__ int s_tile[];  // kLds: [0, T) near z bits, [T, 2T) far z bits
     __shared__ int s_box[4];         // ulx, uly, lrx, lry of the workgroup's rectangles
+    __shared__ unsigned int s_rend;  // rendering blocks of this workgroup
     const int tid = threadIdx.x;
     const int nvis = (int)r.counters[LF_CNT_VISIBLE];
     if ((int)(blockIdx.x * blockDim.x) >= nvis) return;  // whole workgroup
+    if (tid == 0) s_rend = 0u;
     const int T = r.tiles_x * r.tiles_y;
     const int near0 = __float_as_int(100.0f), far0 = __float_as_int(0.0f);  // k_zrange_clear's values
     if (kLds) {
@@ -84,8 +86,8 @@ __global__ __launch_bounds__(256) void k_zrange_fill(RayDev r, const float *__re
             s_box[0] = s_box[1] = INT_MAX;
             s_box[2] = s_box[3] = -1;
         }
-        __syncthreads();
     }
+    __syncthreads();
     float Tcw[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) Tcw[i] = pose24[12 + i];
@@ -114,10 +116,9 @@ __global__ __launch_bounds__(256) void k_zrange_fill(RayDev r, const float *__re
         if (lry >= r.tiles_y) lry = r.tiles_y - 1;
         if (zr0 < r.zmin_raycast) zr0 = r.zmin_raycast;
         const bool renders = !(ulx > lrx || uly > lry) && !(zr1 < r.zmin_raycast);
-        // one counter atomic per wavefront (the active lanes of this iteration)
+        // counted per wavefront in LDS; one global atomic per workgroup below
         const unsigned long long active = __ballot(1), rend = __ballot(renders);
-        if (lane_id() == __ffsll((long long)active) - 1 && rend)
-            atomicAdd(&r.counters[LF_CNT_RENDER_BLOCKS], (unsigned int)__popcll(rend));
+        if (lane_id() == __ffsll((long long)active) - 1 && rend) atomicAdd(&s_rend, (unsigned int)__popcll(rend));
         if (!renders) continue;
         if (kLds) {
             atomicMin(&s_box[0], ulx);
@@ -138,8 +139,9 @@ __global__ __launch_bounds__(256) void k_zrange_fill(RayDev r, const float *__re
                 }
             }
     }
+    __syncthreads();
+    if (tid == 0 && s_rend) atomicAdd(&r.counters[LF_CNT_RENDER_BLOCKS], s_rend);
     if (kLds) {
-        __syncthreads();
         const int bx0 = s_box[0], by0 = s_box[1], bw = s_box[2] - bx0 + 1, bh = s_box[3] - by0 + 1;
         if (bw <= 0 || bh <= 0) return;
         for (int j = tid; j < bw * bh; j += blockDim.x) {

@@ -59,24 +59,34 @@ LF_DEV int sat16(float v) {
     return (int)v;
 }
 
+constexpr int kZThreads = 512;  // lanes of a k_zrange_fill workgroup: 8 per visible block
+
 // RenderingBlockDelegate::create_rendering_block (ray_tracing.cu:63-126) and
-// the tile fill (:187-210).  kLds: a workgroup first takes the min / max of
-// its blocks' tiles in an LDS copy of the tile grid (its blocks are
-// consecutive visible-list entries, so their rectangles overlap heavily), then
-// flushes the tiles it changed, inside the bounding box of its rectangles, to
-// the global grid with one atomic each.  Min and max do not depend on order,
-// so the tiles are the same as with per-block global atomics.
+// the tile fill (:187-210).  Eight lanes take one visible block, one corner
+// each, and combine their projections with three butterfly steps: the
+// reference's compare-and-set loop over the corners, starting from the image
+// bounds, is min / max with NaN ignored followed by the monotone 16-bit
+// saturation, which is what fminf / fmaxf and sat16 compute.  The 8 lanes then
+// share the rectangle's tiles (lane c takes columns c, c + 8, ... of each
+// row).  kLds: the min / max go to an LDS copy of the tile grid first (a
+// workgroup's blocks are consecutive visible-list entries, so their
+// rectangles overlap heavily), then the tiles it changed, inside the bounding
+// box of its rectangles, are flushed to the global grid with one atomic each.
+// Min and max do not depend on order, so the tiles are the same as with
+// per-block global atomics.
 template <bool kLds>
-__global__ __launch_bounds__(256) void k_zrange_fill(RayDev r, const float *__restrict__ pose24, lf_intrinsics K) {
+__global__ __launch_bounds__(kZThreads) void k_zrange_fill(RayDev r, const float *__restrict__ pose24,
+                                                           lf_intrinsics K) {
     extern __shared__ int s_tile[];  // kLds: [0, T) near z bits, [T, 2T) far z bits
     __shared__ int s_box[4];         // ulx, uly, lrx, lry of the workgroup's rectangles
     __shared__ unsigned int s_rend;  // rendering blocks of this workgroup
-    const int tid = threadIdx.x;
+    constexpr int kPer = kZThreads / 8;  // blocks per workgroup pass
+    const int tid = threadIdx.x, c = tid & 7;
     const int nvis = (int)r.counters[LF_CNT_VISIBLE];
-    if ((int)(blockIdx.x * blockDim.x) >= nvis) return;  // whole workgroup
-    if (tid == 0) s_rend = 0u;
+    if ((int)(blockIdx.x * kPer) >= nvis) return;  // whole workgroup
     const int T = r.tiles_x * r.tiles_y;
     const int near0 = __float_as_int(100.0f), far0 = __float_as_int(0.0f);  // k_zrange_clear's values
+    if (tid == 0) s_rend = 0u;
     if (kLds) {
         for (int i = tid; i < T; i += blockDim.x) {
             s_tile[i] = near0;
@@ -91,36 +101,45 @@ __global__ __launch_bounds__(256) void k_zrange_fill(RayDev r, const float *__re
     float Tcw[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) Tcw[i] = pose24[12 + i];
-    const float scale = 8.0f * r.voxel_size;
-    for (int i = blockIdx.x * blockDim.x + tid; i < nvis; i += gridDim.x * blockDim.x) {
-        const int4 bp = reinterpret_cast<const int4 *>(r.block_pos)[r.visible[i]];
-        int ulx = r.tiles_x, uly = r.tiles_y, lrx = -1, lry = -1;
-        float zr0 = r.zmax_raycast, zr1 = r.zmin_raycast;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
+    const float scale = 8.0f * r.voxel_size, qnan = __int_as_float(0x7fc00000);
+    // the trip count is uniform over the workgroup, so every lane reaches the shuffles
+    for (int i0 = blockIdx.x * kPer; i0 < nvis; i0 += gridDim.x * kPer) {
+        const int i = i0 + tid / 8;
+        const bool have = i < nvis;
+        float fu = qnan, cu = qnan, fv = qnan, cv = qnan, zlo = qnan, zhi = qnan;  // NaN: ignored
+        if (have) {
+            const int4 bp = reinterpret_cast<const int4 *>(r.block_pos)[r.visible[i]];
             const float wx = (float)(bp.x + (c & 1)) * scale, wy = (float)(bp.y + ((c >> 1) & 1)) * scale,
                         wz = (float)(bp.z + ((c >> 2) & 1)) * scale;
             const f3 pt = xform(Tcw, f3{wx, wy, wz});
             const float u = (K.fx * pt.x / pt.z + K.cx) / 8.0f, v = (K.fy * pt.y / pt.z + K.cy) / 8.0f;
-            const float fu = floorf(u), cu = ceilf(u), fv = floorf(v), cv = ceilf(v);
-            if ((float)ulx > fu) ulx = sat16(fu);
-            if ((float)lrx < cu) lrx = sat16(cu);
-            if ((float)uly > fv) uly = sat16(fv);
-            if ((float)lry < cv) lry = sat16(cv);
-            if (zr0 > pt.z) zr0 = pt.z;
-            if (zr1 < pt.z) zr1 = pt.z;
+            fu = floorf(u); cu = ceilf(u); fv = floorf(v); cv = ceilf(v);
+            zlo = zhi = pt.z;
         }
+#pragma unroll
+        for (int o = 1; o < 8; o <<= 1) {
+            fu = fminf(fu, __shfl_xor(fu, o));
+            cu = fmaxf(cu, __shfl_xor(cu, o));
+            fv = fminf(fv, __shfl_xor(fv, o));
+            cv = fmaxf(cv, __shfl_xor(cv, o));
+            zlo = fminf(zlo, __shfl_xor(zlo, o));
+            zhi = fmaxf(zhi, __shfl_xor(zhi, o));
+        }
+        int ulx = sat16(fminf((float)r.tiles_x, fu)), lrx = sat16(fmaxf(-1.0f, cu));
+        int uly = sat16(fminf((float)r.tiles_y, fv)), lry = sat16(fmaxf(-1.0f, cv));
+        float zr0 = fminf(r.zmax_raycast, zlo);
+        const float zr1 = fmaxf(r.zmin_raycast, zhi);
         if (ulx < 0) ulx = 0;
         if (uly < 0) uly = 0;
         if (lrx >= r.tiles_x) lrx = r.tiles_x - 1;
         if (lry >= r.tiles_y) lry = r.tiles_y - 1;
         if (zr0 < r.zmin_raycast) zr0 = r.zmin_raycast;
-        const bool renders = !(ulx > lrx || uly > lry) && !(zr1 < r.zmin_raycast);
-        // counted per wavefront in LDS; one global atomic per workgroup below
-        const unsigned long long active = __ballot(1), rend = __ballot(renders);
-        if (lane_id() == __ffsll((long long)active) - 1 && rend) atomicAdd(&s_rend, (unsigned int)__popcll(rend));
+        const bool renders = have && !(ulx > lrx || uly > lry) && !(zr1 < r.zmin_raycast);
+        // counted per wavefront in LDS (one lane per block); one global atomic per workgroup below
+        const unsigned long long rend = __ballot(renders && c == 0);
+        if (lane_id() == 0 && rend) atomicAdd(&s_rend, (unsigned int)__popcll(rend));
         if (!renders) continue;
-        if (kLds) {
+        if (kLds && c == 0) {
             atomicMin(&s_box[0], ulx);
             atomicMin(&s_box[1], uly);
             atomicMax(&s_box[2], lrx);
@@ -128,7 +147,7 @@ __global__ __launch_bounds__(256) void k_zrange_fill(RayDev r, const float *__re
         }
         const int zi0 = __float_as_int(zr0), zi1 = __float_as_int(zr1);
         for (int y = uly; y <= lry; ++y)
-            for (int x = ulx; x <= lrx; ++x) {
+            for (int x = ulx + c; x <= lrx; x += 8) {
                 const int t = y * r.tiles_x + x;
                 if (kLds) {
                     atomicMin(&s_tile[t], zi0);
@@ -380,9 +399,9 @@ int lf_raycast_ev(const lf_map_params *p, const lf_map_storage *s, float *vmap, 
     LF_LAUNCH_CHECK();
     // the per-workgroup tile grid in LDS when it fits (C1: 4800 tiles, 38 KB)
     const size_t lds = 2 * (size_t)T * sizeof(int);
-    const int zgrid = std::max(1, std::min(2048, (p->num_blocks + 255) / 256));
-    if (lds <= zrange_lds_limit()) k_zrange_fill<true><<<zgrid, 256, lds, st>>>(r, pose24, *K);
-    else k_zrange_fill<false><<<zgrid, 256, 0, st>>>(r, pose24, *K);
+    const int zgrid = std::max(1, std::min(1024, (p->num_blocks + kZThreads / 8 - 1) / (kZThreads / 8)));
+    if (lds <= zrange_lds_limit()) k_zrange_fill<true><<<zgrid, kZThreads, lds, st>>>(r, pose24, *K);
+    else k_zrange_fill<false><<<zgrid, kZThreads, 0, st>>>(r, pose24, *K);
     LF_LAUNCH_CHECK();
     const int tx = (cols + 7) / 8, ty = (rows + 7) / 8;
     const int wg_x = (tx + 1) / 2, wg_y = (ty + 1) / 2;  // 2x2 tiles per workgroup

@@ -96,6 +96,9 @@ typedef struct lf_map_params {
  *                            the region is inserted (a clear bit proves every
  *                            block of the region absent; the raycast skips the
  *                            hash probe there)
+ *   visible_pos[4*num_blocks] {x, y, z, pool block} of visible[i]: the list's
+ *                            block coordinates beside their indices, so a
+ *                            kernel walking the list needs one load per block
  * Voxel i of block b lives at b*512 + z*64 + y*8 + x (map_struct.cu:657-660). */
 #define LF_OCC_WORDS 4096      /* 2^17 occupancy bits */
 
@@ -111,6 +114,7 @@ typedef struct lf_map_storage {
     unsigned int *counters;
     float *zrange;            /* raycast z-range tiles, 2 * tiles_x * tiles_y floats */
     unsigned int *occupancy;  /* LF_OCC_WORDS words */
+    int *visible_pos;         /* 4 * num_blocks ints */
     int tiles_x, tiles_y;
 } lf_map_storage;
 

@@ -41,7 +41,8 @@ class LfMapStorage(ctypes.Structure):
     _fields_ = [("keys", ctypes.c_void_p), ("slot_block", ctypes.c_void_p), ("slot_flag", ctypes.c_void_p),
                 ("block_pos", ctypes.c_void_p), ("sdf", ctypes.c_void_p), ("weight", ctypes.c_void_p),
                 ("rgb", ctypes.c_void_p), ("visible", ctypes.c_void_p), ("counters", ctypes.c_void_p),
-                ("zrange", ctypes.c_void_p), ("occupancy", ctypes.c_void_p), ("tiles_x", ctypes.c_int),
+                ("zrange", ctypes.c_void_p), ("occupancy", ctypes.c_void_p),
+                ("visible_pos", ctypes.c_void_p), ("tiles_x", ctypes.c_int),
                 ("tiles_y", ctypes.c_int)]
 
 
@@ -371,6 +372,7 @@ class Map:
             "counters": torch.zeros(16, dtype=torch.int32, device=device),
             "zrange": torch.empty(2 * tx * ty, dtype=torch.float32, device=device),
             "occupancy": torch.empty(OCC_WORDS, dtype=torch.int32, device=device),
+            "visible_pos": torch.empty(B * 4, dtype=torch.int32, device=device),
         }
         s = LfMapStorage()
         for k, v in self.t.items():

@@ -71,6 +71,23 @@ static int check_params(const lf_map_params *p) {
     return LF_OK;
 }
 
+// Every array of a map storage present and the tile grid non-empty; entry
+// points call this before any launch, so storage filled in by hand with an
+// array missing is an error rather than a kernel writing through null.
+int lf_check_storage(const lf_map_storage *s) {
+    if (!s) return lf_set_error(LF_ERR_ARG, "null map storage");
+    const struct { const void *ptr; const char *name; } arrays[] = {
+        {s->keys, "keys"}, {s->slot_block, "slot_block"}, {s->slot_flag, "slot_flag"},
+        {s->block_pos, "block_pos"}, {s->sdf, "sdf"}, {s->weight, "weight"}, {s->rgb, "rgb"},
+        {s->visible, "visible"}, {s->counters, "counters"}, {s->zrange, "zrange"},
+        {s->occupancy, "occupancy"}, {s->visible_pos, "visible_pos"}};
+    for (const auto &a : arrays)
+        if (!a.ptr) return lf_set_error(LF_ERR_ARG, "map storage array %s is null", a.name);
+    if (s->tiles_x <= 0 || s->tiles_y <= 0)
+        return lf_set_error(LF_ERR_ARG, "map storage tile grid %dx%d", s->tiles_x, s->tiles_y);
+    return LF_OK;
+}
+
 extern "C" int lf_map_create(const lf_map_params *p, int width, int height, lf_map_storage *s) {
     int rc = check_params(p);
     if (rc) return rc;
@@ -91,13 +108,14 @@ extern "C" int lf_map_create(const lf_map_params *p, int width, int height, lf_m
     LF_HIP_CHECK(hipMalloc(&s->counters, LF_NUM_COUNTERS * sizeof(unsigned int)));
     LF_HIP_CHECK(hipMalloc(&s->zrange, 2 * (size_t)s->tiles_x * s->tiles_y * sizeof(float)));
     LF_HIP_CHECK(hipMalloc(&s->occupancy, LF_OCC_WORDS * sizeof(unsigned int)));
+    LF_HIP_CHECK(hipMalloc(&s->visible_pos, B * 4 * sizeof(int)));
     return LF_OK;
 }
 
 extern "C" int lf_map_release(lf_map_storage *s) {
     if (!s) return LF_OK;
     void *ptrs[] = {s->keys, s->slot_block, s->slot_flag, s->block_pos, s->sdf, s->weight,
-                    s->rgb, s->visible, s->counters, s->zrange, s->occupancy};
+                    s->rgb, s->visible, s->counters, s->zrange, s->occupancy, s->visible_pos};
     for (void *q : ptrs)
         if (q) LF_HIP_CHECK(hipFree(q));
     memset(s, 0, sizeof(*s));
@@ -126,6 +144,7 @@ __global__ void k_fill_u128(uint4 *p, size_t n, unsigned int v) {
 extern "C" int lf_map_reset(const lf_map_params *p, const lf_map_storage *s, void *stream) {
     int rc = check_params(p);
     if (rc) return rc;
+    if ((rc = lf_check_storage(s))) return rc;
     hipStream_t st = lf_stream(stream);
     const int E = p->num_entries;
     k_reset_table<<<std::min(4096, (E + 255) / 256), 256, 0, st>>>(s->keys, s->slot_block, s->slot_flag, E);
@@ -150,6 +169,8 @@ extern "C" int lf_map_reset(const lf_map_params *p, const lf_map_storage *s, voi
 }
 
 extern "C" int lf_map_read_counters(const lf_map_storage *s, unsigned int *out16, void *stream) {
+    if (int rc = lf_check_storage(s)) return rc;
+    if (!out16) return lf_set_error(LF_ERR_ARG, "null counter output");
     hipStream_t st = lf_stream(stream);
     LF_HIP_CHECK(hipMemcpyAsync(out16, s->counters, LF_NUM_COUNTERS * sizeof(unsigned int),
                                 hipMemcpyDeviceToHost, st));

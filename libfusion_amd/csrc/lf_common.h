@@ -125,6 +125,8 @@ LF_DEV int lane_id() { return __lane_id(); }
 
 // Error reporting (common.hip).
 int lf_set_error(int code, const char *fmt, ...);
+// LF_OK when every array of the storage is present (common.hip).
+int lf_check_storage(const lf_map_storage *s);
 #define LF_HIP_CHECK(call)                                                               \
     do {                                                                                 \
         hipError_t e_ = (call);                                                          \

@@ -40,6 +40,7 @@ struct MapDev {
     float *weight;
     unsigned char *rgb;
     int *visible;
+    int *visible_pos;  // {x, y, z, pool block} of visible[i]
     unsigned int *counters;
     unsigned int *occupancy;
     int num_buckets, num_entries, num_blocks;
@@ -51,6 +52,7 @@ MapDev make_dev(const lf_map_params *p, const lf_map_storage *s) {
     d.keys = s->keys; d.slot_block = s->slot_block; d.slot_flag = s->slot_flag; d.block_pos = s->block_pos;
     d.sdf = s->sdf; d.weight = s->weight; d.rgb = s->rgb; d.visible = s->visible; d.counters = s->counters;
     d.occupancy = s->occupancy;
+    d.visible_pos = s->visible_pos;
     d.num_buckets = p->num_buckets; d.num_entries = p->num_entries; d.num_blocks = p->num_blocks;
     d.voxel_size = p->voxel_size; d.truncation = p->truncation;
     d.zmin_update = p->zmin_update; d.zmax_update = p->zmax_update;
@@ -235,8 +237,9 @@ __global__ __launch_bounds__(kVisThreads) void k_visible(MapDev m, const float *
     for (unsigned int b0 = blockIdx.x * blockDim.x; b0 < n; b0 += stride) {
         const unsigned int b = b0 + threadIdx.x;
         bool vis = false;
+        int4 p = make_int4(0, 0, 0, 0);
         if (b < n) {
-            const int4 p = reinterpret_cast<const int4 *>(m.block_pos)[b];
+            p = reinterpret_cast<const int4 *>(m.block_pos)[b];
             if (!kGeometric && m.slot_flag[p.w]) {
                 vis = true;
                 m.slot_flag[p.w] = 0;
@@ -257,7 +260,11 @@ __global__ __launch_bounds__(kVisThreads) void k_visible(MapDev m, const float *
             s_base = total ? atomicAdd(&m.counters[LF_CNT_VISIBLE], total) : 0u;
         }
         __syncthreads();
-        if (vis) m.visible[s_base + s_cnt[wave] + __popcll(ballot & ((1ull << lane) - 1ull))] = (int)b;
+        if (vis) {
+            const unsigned int slot = s_base + s_cnt[wave] + __popcll(ballot & ((1ull << lane) - 1ull));
+            m.visible[slot] = (int)b;
+            reinterpret_cast<int4 *>(m.visible_pos)[slot] = make_int4(p.x, p.y, p.z, (int)b);
+        }
         __syncthreads();  // s_cnt and s_base are rewritten by the next iteration
     }
 }
@@ -292,8 +299,8 @@ __global__ __launch_bounds__(256) void k_integrate(MapDev m, const float *__rest
     const int t = threadIdx.x;
     const int lx = t & 7, ly = (t >> 3) & 7, lz0 = t >> 6;
     for (int i = blockIdx.x; i < nvis; i += gridDim.x) {
-        const int b = m.visible[i];
-        const int4 p = reinterpret_cast<const int4 *>(m.block_pos)[b];
+        const int4 p = reinterpret_cast<const int4 *>(m.visible_pos)[i];  // x, y, z, pool block
+        const int b = p.w;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int lz = lz0 + 4 * h;
@@ -379,8 +386,8 @@ __global__ __launch_bounds__(256) void k_deintegrate(MapDev m, const float *__re
     const int t = threadIdx.x;
     const int lx = t & 7, ly = (t >> 3) & 7, lz0 = t >> 6;
     for (int i = blockIdx.x; i < nvis; i += gridDim.x) {
-        const int b = m.visible[i];
-        const int4 p = reinterpret_cast<const int4 *>(m.block_pos)[b];
+        const int4 p = reinterpret_cast<const int4 *>(m.visible_pos)[i];  // x, y, z, pool block
+        const int b = p.w;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int lz = lz0 + 4 * h;
@@ -448,6 +455,7 @@ int check_update_args(const lf_map_params *p, const lf_map_storage *s, const flo
 int lf_update_ev(const lf_map_params *p, const lf_map_storage *s, const float *depth, const float *nmap,
                  const unsigned char *rgb, int cols, int rows, const float *pose24, const lf_intrinsics *K,
                  hipStream_t st, bool weighted, hipEvent_t e0, hipEvent_t e1, const float *samples) {
+    if (int rc = lf_check_storage(s)) return rc;
     const MapDev m = make_dev(p, s);
     dim3 g((cols + 15) / 16, (rows + 15) / 16);
     k_alloc<true><<<g, 256, 0, st>>>(m, depth, cols, rows, pose24, *K);
@@ -475,6 +483,7 @@ int lf_update_ev(const lf_map_params *p, const lf_map_storage *s, const float *d
 int lf_deintegrate_ev(const lf_map_params *p, const lf_map_storage *s, const float *depth, const float *nmap,
                       const unsigned char *rgb, int cols, int rows, const float *pose24, const lf_intrinsics *K,
                       hipStream_t st, const float *samples) {
+    if (int rc = lf_check_storage(s)) return rc;
     const MapDev m = make_dev(p, s);
     dim3 g((cols + 15) / 16, (rows + 15) / 16);
     k_alloc<false><<<g, 256, 0, st>>>(m, depth, cols, rows, pose24, *K);
@@ -509,6 +518,7 @@ __global__ void k_reset_visible(MapDev m) {
 
 int lf_select_visible_ev(const lf_map_params *p, const lf_map_storage *s, const float *pose24, const lf_intrinsics *K,
                          int cols, int rows, hipStream_t st) {
+    if (int rc = lf_check_storage(s)) return rc;
     const MapDev m = make_dev(p, s);
     k_reset_visible<<<1, 64, 0, st>>>(m);
     LF_LAUNCH_CHECK();

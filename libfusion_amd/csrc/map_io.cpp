@@ -89,6 +89,7 @@ struct File {
 // capacities (num_blocks, num_entries, num_buckets), plus "<path>.txt".
 extern "C" int lf_map_write(const lf_map_params *p, const lf_map_storage *s, const char *path, void *stream) {
     if (!p || !s || !path) return lf_set_error(LF_ERR_ARG, "null argument to lf_map_write");
+    if (int rc = lf_check_storage(s)) return rc;
     hipStream_t st = lf_stream(stream);
     HostMap h;
     if (int rc = download(p, s, st, h)) return rc;
@@ -154,6 +155,7 @@ extern "C" int lf_map_write(const lf_map_params *p, const lf_map_storage *s, con
 // state); the map is reset and refilled.
 extern "C" int lf_map_read(const lf_map_params *p, const lf_map_storage *s, const char *path, void *stream) {
     if (!p || !s || !path) return lf_set_error(LF_ERR_ARG, "null argument to lf_map_read");
+    if (int rc = lf_check_storage(s)) return rc;
     hipStream_t st = lf_stream(stream);
     RefSize sz{p->num_blocks, p->num_entries, p->num_buckets};
     {
@@ -250,6 +252,8 @@ extern "C" int lf_map_read(const lf_map_params *p, const lf_map_storage *s, cons
 extern "C" int lf_map_copy(const lf_map_params *p, const lf_map_storage *src, const lf_map_storage *dst,
                            void *stream) {
     if (!p || !src || !dst) return lf_set_error(LF_ERR_ARG, "null argument to lf_map_copy");
+    if (int rc = lf_check_storage(src)) return rc;
+    if (int rc = lf_check_storage(dst)) return rc;
     if (src->tiles_x != dst->tiles_x || src->tiles_y != dst->tiles_y)
         return lf_set_error(LF_ERR_ARG, "maps made for different raycast sizes");
     hipStream_t st = lf_stream(stream);
@@ -263,6 +267,7 @@ extern "C" int lf_map_copy(const lf_map_params *p, const lf_map_storage *src, co
         {dst->weight, src->weight, B * LF_BLOCK3 * sizeof(float)},
         {dst->rgb, src->rgb, 3 * B * LF_BLOCK3},
         {dst->visible, src->visible, B * sizeof(int)},
+        {dst->visible_pos, src->visible_pos, B * 4 * sizeof(int)},
         {dst->counters, src->counters, LF_NUM_COUNTERS * sizeof(unsigned int)},
         {dst->occupancy, src->occupancy, LF_OCC_WORDS * sizeof(unsigned int)},
     };

@@ -198,6 +198,7 @@ extern "C" int lf_create_mesh(const lf_map_params *p, const lf_map_storage *s, f
     if (!p || !s || !triangles_out) return lf_set_error(LF_ERR_ARG, "null argument to create_mesh");
     if (max_triangles < 0) return lf_set_error(LF_ERR_ARG, "negative triangle capacity");
     if (normal_out && colour_out) return lf_set_error(LF_ERR_ARG, "ask for normals or colours, not both");
+    if (int rc = lf_check_storage(s)) return rc;
     hipStream_t st = lf_stream(stream);
     MeshDev m;
     m.keys = s->keys; m.slot_block = s->slot_block; m.block_pos = s->block_pos; m.sdf = s->sdf;

@@ -33,6 +33,7 @@ struct RayDev {
     const float *weight;
     const unsigned char *rgb;
     const int *visible;
+    const int *visible_pos;  // {x, y, z, pool block} of visible[i]
     const int *block_pos;
     unsigned int *counters;
     const unsigned int *occupancy;  // LF_OCC_WORDS words
@@ -108,7 +109,7 @@ __global__ __launch_bounds__(kZThreads) void k_zrange_fill(RayDev r, const float
         const bool have = i < nvis;
         float fu = qnan, cu = qnan, fv = qnan, cv = qnan, zlo = qnan, zhi = qnan;  // NaN: ignored
         if (have) {
-            const int4 bp = reinterpret_cast<const int4 *>(r.block_pos)[r.visible[i]];
+            const int4 bp = reinterpret_cast<const int4 *>(r.visible_pos)[i];
             const float wx = (float)(bp.x + (c & 1)) * scale, wy = (float)(bp.y + ((c >> 1) & 1)) * scale,
                         wz = (float)(bp.z + ((c >> 2) & 1)) * scale;
             const f3 pt = xform(Tcw, f3{wx, wy, wz});
@@ -381,12 +382,13 @@ int lf_raycast_ev(const lf_map_params *p, const lf_map_storage *s, float *vmap, 
                   int rows, const float *pose24, const lf_intrinsics *K, int check_colour, hipStream_t st,
                   hipEvent_t e0, hipEvent_t e1) {
     if (!p || !s || !vmap || !pose24 || !K) return lf_set_error(LF_ERR_ARG, "null argument to raycast");
+    if (int rc = lf_check_storage(s)) return rc;
     if (cols / 8 != s->tiles_x || rows / 8 != s->tiles_y)
         return lf_set_error(LF_ERR_ARG, "raycast size %dx%d does not match the map's %dx%d tiles", cols, rows,
                             s->tiles_x, s->tiles_y);
     RayDev r;
     r.keys = s->keys; r.slot_block = s->slot_block; r.sdf = s->sdf; r.weight = s->weight; r.rgb = s->rgb;
-    r.visible = s->visible; r.block_pos = s->block_pos; r.counters = s->counters;
+    r.visible = s->visible; r.visible_pos = s->visible_pos; r.block_pos = s->block_pos; r.counters = s->counters;
     r.occupancy = s->occupancy;
     r.zx = s->zrange; r.zy = s->zrange + (size_t)s->tiles_x * s->tiles_y;
     r.tiles_x = s->tiles_x; r.tiles_y = s->tiles_y;

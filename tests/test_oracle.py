@@ -111,6 +111,35 @@ def test_no_gpu_calls_fail_cleanly_without_device():
     assert b"null" in lib.lf_last_error()
 
 
+def test_storage_with_a_missing_array_is_rejected_before_device_work():
+    """Storage filled in by hand must name every array: each map entry point
+    checks them before any launch (a null array would otherwise be written
+    through by a kernel).  The other pointers are never dereferenced."""
+    from libfusion_amd import configs
+
+    lib = capi.load()
+    _, K, p = configs.c0()
+    fake = ctypes.c_void_p(0x10000)
+    arrays = [f for f, t in capi.LfMapStorage._fields_ if t is ctypes.c_void_p]
+    assert "visible_pos" in arrays and len(arrays) == 12
+    for missing in arrays:
+        s = capi.LfMapStorage()
+        for i, f in enumerate(arrays):
+            setattr(s, f, 0 if f == missing else 0x100000 * (i + 1))
+        s.tiles_x, s.tiles_y = K.width // 8, K.height // 8
+        calls = [
+            lambda: lib.lf_map_reset(ctypes.byref(p), ctypes.byref(s), None),
+            lambda: lib.lf_update_weighted(ctypes.byref(p), ctypes.byref(s), fake, fake, fake, K.width, K.height,
+                                           fake, ctypes.byref(K), None),
+            lambda: lib.lf_raycast(ctypes.byref(p), ctypes.byref(s), fake, None, K.width, K.height, fake,
+                                   ctypes.byref(K), 1, None),
+            lambda: lib.lf_map_copy(ctypes.byref(p), ctypes.byref(s), ctypes.byref(s), None),
+        ]
+        for call in calls:
+            assert call() == capi.LF_ERR_ARG, missing
+            assert missing.encode() in lib.lf_last_error(), (missing, lib.lf_last_error())
+
+
 # ------------------------------------------------------------- SE(3), LDLT
 
 def twist_matrix(xi):

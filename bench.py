@@ -141,9 +141,9 @@ def main():
     P = K.width * K.height
     # TSDF integration kernel, algorithmic bytes per launch: every visible block's
     # 512 voxels read and written (sdf int16 + weight f32 + rgb 3 x u8 = 9 B),
-    # its list entry and coordinate (4 + 16 B), plus one depth (4 B), normal x/z
-    # (8 B) and colour (3 B) gather per pixel.
-    bytes_per_launch = n_vis * (2 * 512 * 9 + 20) + P * 15
+    # its list entry (coordinate and pool index, 16 B), plus one depth (4 B),
+    # normal x/z (8 B) and colour (3 B) gather per pixel.
+    bytes_per_launch = n_vis * (2 * 512 * 9 + 16) + P * 15
     avg_s = integrate_avg_ms * 1e-3
     achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
     traffic = None

@@ -198,7 +198,7 @@ def main():
         "stages_ms": {k: round(v["avg_ms"], 4) for k, v in stages.items()},
         "allocated_blocks": int(min(c1[capi.CNT_ALLOCATED], params.num_blocks)),
         "overflow": int(c1[capi.CNT_OVERFLOW]),
-        "final_pose_t": [round(float(x), 4) for x in pose[:3, 3]],
+        "final_pose_t": [round(float(x), 7) for x in pose[:3, 3]],
     }
     print(json.dumps(out), flush=True)
 

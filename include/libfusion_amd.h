@@ -99,8 +99,16 @@ typedef struct lf_map_params {
  *   visible_pos[4*num_blocks] {x, y, z, pool block} of visible[i]: the list's
  *                            block coordinates beside their indices, so a
  *                            kernel walking the list needs one load per block
+ *   block_dir[LF_DIR_CELLS]  pool block of every block coordinate with each axis in
+ *                            [-LF_DIR_DIM/2, LF_DIR_DIM/2), -1 = none: a direct
+ *                            directory beside the hash table (4x4x4-block bricks of
+ *                            64 cells, cell lf_dir_cell(bx, by, bz)); the raycast
+ *                            resolves blocks with one load, blocks outside it
+ *                            through the hash table
  * Voxel i of block b lives at b*512 + z*64 + y*8 + x (map_struct.cu:657-660). */
 #define LF_OCC_WORDS 4096      /* 2^17 occupancy bits */
+#define LF_DIR_DIM 256         /* block directory: 256^3 cells (64 MB), 10.24 m per axis at 5 mm voxels */
+#define LF_DIR_CELLS (LF_DIR_DIM * LF_DIR_DIM * LF_DIR_DIM)
 
 typedef struct lf_map_storage {
     unsigned long long *keys;
@@ -115,6 +123,7 @@ typedef struct lf_map_storage {
     float *zrange;            /* raycast z-range tiles, 2 * tiles_x * tiles_y floats */
     unsigned int *occupancy;  /* LF_OCC_WORDS words */
     int *visible_pos;         /* 4 * num_blocks ints */
+    int *block_dir;           /* LF_DIR_CELLS ints */
     int tiles_x, tiles_y;
 } lf_map_storage;
 
@@ -123,6 +132,8 @@ typedef struct lf_map_storage {
 /* Occupancy bit of block (bx, by, bz): the computeHash mix of its 4x4x4-block
  * region (floor division), modulo 2^17. */
 int lf_occupancy_bit(int bx, int by, int bz);
+/* Directory cell of block (bx, by, bz), or -1 outside the directory. */
+int lf_dir_cell(int bx, int by, int bz);
 
 /* Library version string and the gfx target the kernels were built for. */
 const char *lf_version(void);

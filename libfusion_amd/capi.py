@@ -19,6 +19,8 @@ CNT_VIS_SUM_LO, CNT_VIS_SUM_HI, CNT_UPDATES, CNT_RENDER_BLOCKS = 4, 5, 6, 7
 OVF_TABLE, OVF_POOL, OVF_TRIANGLES = 1, 2, 4
 SDF_UNSEEN = -32768  # sdf code of a voxel with weight 0
 OCC_WORDS = 4096  # occupancy words of a map (2^17 bits)
+DIR_DIM = 256  # block directory: DIR_DIM^3 cells
+DIR_CELLS = DIR_DIM ** 3
 TIMING_INTEGRATE, TIMING_RAYCAST, TIMING_TRACKING, TIMING_FRAME = 0x1, 0x2, 0x4, 0x10
 TIMING_ALL = 0x17
 
@@ -42,7 +44,7 @@ class LfMapStorage(ctypes.Structure):
                 ("block_pos", ctypes.c_void_p), ("sdf", ctypes.c_void_p), ("weight", ctypes.c_void_p),
                 ("rgb", ctypes.c_void_p), ("visible", ctypes.c_void_p), ("counters", ctypes.c_void_p),
                 ("zrange", ctypes.c_void_p), ("occupancy", ctypes.c_void_p),
-                ("visible_pos", ctypes.c_void_p), ("tiles_x", ctypes.c_int),
+                ("visible_pos", ctypes.c_void_p), ("block_dir", ctypes.c_void_p), ("tiles_x", ctypes.c_int),
                 ("tiles_y", ctypes.c_int)]
 
 
@@ -65,6 +67,7 @@ SIGNATURES = [
     ("lf_intrinsics_pyr_down", None, [P, P]),
     ("lf_compute_hash", I, [I, I, I, I]),
     ("lf_occupancy_bit", I, [I, I, I]),
+    ("lf_dir_cell", I, [I, I, I]),
     ("lf_mc_tables", None, [P, P]),
     ("lf_device_count", I, []),
     ("lf_last_error", ctypes.c_char_p, []),
@@ -373,6 +376,7 @@ class Map:
             "zrange": torch.empty(2 * tx * ty, dtype=torch.float32, device=device),
             "occupancy": torch.empty(OCC_WORDS, dtype=torch.int32, device=device),
             "visible_pos": torch.empty(B * 4, dtype=torch.int32, device=device),
+            "block_dir": torch.empty(DIR_CELLS, dtype=torch.int32, device=device),
         }
         s = LfMapStorage()
         for k, v in self.t.items():

@@ -44,6 +44,8 @@ extern "C" int lf_compute_hash(int x, int y, int z, int num_buckets) {
 
 extern "C" int lf_occupancy_bit(int bx, int by, int bz) { return (int)lf::occupancy_bit(bx, by, bz); }
 
+extern "C" int lf_dir_cell(int bx, int by, int bz) { return lf::dir_cell(bx, by, bz); }
+
 extern "C" void lf_mc_tables(int *edge_out, int *tri_out) {
     for (int i = 0; i < 256; ++i) {
         edge_out[i] = lf_mc_edge_table[i];
@@ -80,7 +82,7 @@ int lf_check_storage(const lf_map_storage *s) {
         {s->keys, "keys"}, {s->slot_block, "slot_block"}, {s->slot_flag, "slot_flag"},
         {s->block_pos, "block_pos"}, {s->sdf, "sdf"}, {s->weight, "weight"}, {s->rgb, "rgb"},
         {s->visible, "visible"}, {s->counters, "counters"}, {s->zrange, "zrange"},
-        {s->occupancy, "occupancy"}, {s->visible_pos, "visible_pos"}};
+        {s->occupancy, "occupancy"}, {s->visible_pos, "visible_pos"}, {s->block_dir, "block_dir"}};
     for (const auto &a : arrays)
         if (!a.ptr) return lf_set_error(LF_ERR_ARG, "map storage array %s is null", a.name);
     if (s->tiles_x <= 0 || s->tiles_y <= 0)
@@ -109,13 +111,14 @@ extern "C" int lf_map_create(const lf_map_params *p, int width, int height, lf_m
     LF_HIP_CHECK(hipMalloc(&s->zrange, 2 * (size_t)s->tiles_x * s->tiles_y * sizeof(float)));
     LF_HIP_CHECK(hipMalloc(&s->occupancy, LF_OCC_WORDS * sizeof(unsigned int)));
     LF_HIP_CHECK(hipMalloc(&s->visible_pos, B * 4 * sizeof(int)));
+    LF_HIP_CHECK(hipMalloc(&s->block_dir, (size_t)LF_DIR_CELLS * sizeof(int)));
     return LF_OK;
 }
 
 extern "C" int lf_map_release(lf_map_storage *s) {
     if (!s) return LF_OK;
     void *ptrs[] = {s->keys, s->slot_block, s->slot_flag, s->block_pos, s->sdf, s->weight,
-                    s->rgb, s->visible, s->counters, s->zrange, s->occupancy, s->visible_pos};
+                    s->rgb, s->visible, s->counters, s->zrange, s->occupancy, s->visible_pos, s->block_dir};
     for (void *q : ptrs)
         if (q) LF_HIP_CHECK(hipFree(q));
     memset(s, 0, sizeof(*s));
@@ -154,17 +157,39 @@ extern "C" int lf_map_reset(const lf_map_params *p, const lf_map_storage *s, voi
     k_zero_u32<<<LF_OCC_WORDS / 256, 256, 0, st>>>(s->occupancy, LF_OCC_WORDS);
     LF_LAUNCH_CHECK();
     const size_t B = (size_t)p->num_blocks;
-    // sdf (2 B, every voxel starts unseen), weight (4 B, 0), rgb (3 B, 0) per voxel;
-    // every array is a multiple of 512 bytes.
+    // sdf (2 B, every voxel starts unseen), weight (4 B, 0), rgb (3 B, 0) per voxel,
+    // and the block directory (every cell -1); every array is a multiple of 512 bytes.
     const unsigned int unseen2 = 0x80008000u;  // two int16 kSdfUnseen codes
     struct { void *ptr; size_t bytes; unsigned int v; } arrs[] = {
-        {s->sdf, B * 1024, unseen2}, {s->weight, B * 2048, 0u}, {s->rgb, B * 1536, 0u}};
+        {s->sdf, B * 1024, unseen2}, {s->weight, B * 2048, 0u}, {s->rgb, B * 1536, 0u},
+        {s->block_dir, (size_t)LF_DIR_CELLS * sizeof(int), 0xFFFFFFFFu}};
     for (auto &a : arrs) {
         size_t n16 = a.bytes / 16;
         int grid = (int)std::min<size_t>(8192, (n16 + 255) / 256);
         k_fill_u128<<<grid, 256, 0, st>>>(reinterpret_cast<uint4 *>(a.ptr), n16, a.v);
         LF_LAUNCH_CHECK();
     }
+    return LF_OK;
+}
+
+__global__ void k_dir_fill(const int4 *block_pos, const unsigned int *counters, int num_blocks, int *dir) {
+    const unsigned int n = min(counters[LF_CNT_ALLOCATED], (unsigned int)num_blocks);
+    for (unsigned int b = blockIdx.x * blockDim.x + threadIdx.x; b < n; b += gridDim.x * blockDim.x) {
+        const int4 p = block_pos[b];
+        const int c = lf::dir_cell(p.x, p.y, p.z);
+        if (c >= 0) dir[c] = (int)b;
+    }
+}
+
+int lf_rebuild_dir(const lf_map_params *p, const lf_map_storage *s, hipStream_t st) {
+    if (int rc = lf_check_storage(s)) return rc;
+    const size_t n16 = (size_t)LF_DIR_CELLS * sizeof(int) / 16;
+    k_fill_u128<<<(int)std::min<size_t>(8192, (n16 + 255) / 256), 256, 0, st>>>(
+        reinterpret_cast<uint4 *>(s->block_dir), n16, 0xFFFFFFFFu);
+    LF_LAUNCH_CHECK();
+    k_dir_fill<<<std::max(1, std::min(2048, (p->num_blocks + 255) / 256)), 256, 0, st>>>(
+        reinterpret_cast<const int4 *>(s->block_pos), s->counters, p->num_blocks, s->block_dir);
+    LF_LAUNCH_CHECK();
     return LF_OK;
 }
 

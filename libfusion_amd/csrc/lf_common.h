@@ -100,6 +100,16 @@ LF_HD unsigned int occupancy_bit(int bx, int by, int bz) {
     return (a ^ b ^ c) & (uint32_t)(LF_OCC_WORDS * 32 - 1);
 }
 
+// Block directory cell (4x4x4-block bricks of 64 consecutive cells, bricks in
+// x-fastest order), or -1 for a block outside the directory.
+LF_HD int dir_cell(int bx, int by, int bz) {
+    constexpr int h = LF_DIR_DIM / 2, nb = LF_DIR_DIM / 4;
+    const unsigned int x = (unsigned int)bx + h, y = (unsigned int)by + h, z = (unsigned int)bz + h;
+    if (x >= (unsigned int)LF_DIR_DIM || y >= (unsigned int)LF_DIR_DIM || z >= (unsigned int)LF_DIR_DIM) return -1;
+    const unsigned int brick = ((z >> 2) * nb + (y >> 2)) * nb + (x >> 2);
+    return (int)((brick << 6) | ((z & 3u) << 4) | ((y & 3u) << 2) | (x & 3u));
+}
+
 // Linear-probing lookup from the computeHash home bucket; returns the pool
 // block of (bx, by, bz) or -1.  The probe is bounded by the table size.  A
 // slot's key and block index are loaded together (one memory round trip per
@@ -127,6 +137,9 @@ LF_DEV int lane_id() { return __lane_id(); }
 int lf_set_error(int code, const char *fmt, ...);
 // LF_OK when every array of the storage is present (common.hip).
 int lf_check_storage(const lf_map_storage *s);
+// Block directory rebuilt from the pool (clear, then every allocated block's
+// cell), for storage whose pool was written directly (common.hip).
+int lf_rebuild_dir(const lf_map_params *p, const lf_map_storage *s, hipStream_t st);
 #define LF_HIP_CHECK(call)                                                               \
     do {                                                                                 \
         hipError_t e_ = (call);                                                          \

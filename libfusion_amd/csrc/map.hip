@@ -41,6 +41,7 @@ struct MapDev {
     unsigned char *rgb;
     int *visible;
     int *visible_pos;  // {x, y, z, pool block} of visible[i]
+    int *block_dir;    // LF_DIR_CELLS pool blocks, -1 = none
     unsigned int *counters;
     unsigned int *occupancy;
     int num_buckets, num_entries, num_blocks;
@@ -53,6 +54,7 @@ MapDev make_dev(const lf_map_params *p, const lf_map_storage *s) {
     d.sdf = s->sdf; d.weight = s->weight; d.rgb = s->rgb; d.visible = s->visible; d.counters = s->counters;
     d.occupancy = s->occupancy;
     d.visible_pos = s->visible_pos;
+    d.block_dir = s->block_dir;
     d.num_buckets = p->num_buckets; d.num_entries = p->num_entries; d.num_blocks = p->num_blocks;
     d.voxel_size = p->voxel_size; d.truncation = p->truncation;
     d.zmin_update = p->zmin_update; d.zmax_update = p->zmax_update;
@@ -80,6 +82,8 @@ LF_DEV void insert_block(const MapDev &m, int bx, int by, int bz) {
             if (b < (unsigned int)m.num_blocks) {
                 m.slot_block[slot] = (int)b;
                 reinterpret_cast<int4 *>(m.block_pos)[b] = make_int4(bx, by, bz, slot);
+                const int cell = dir_cell(bx, by, bz);
+                if (cell >= 0) m.block_dir[cell] = (int)b;
             } else {
                 atomicOr(&m.counters[LF_CNT_OVERFLOW], LF_OVF_POOL);
             }

@@ -244,6 +244,7 @@ extern "C" int lf_map_read(const lf_map_params *p, const lf_map_storage *s, cons
     }
     const unsigned int allocated = (unsigned int)n;
     LF_HIP_CHECK(hipMemcpyAsync(s->counters + LF_CNT_ALLOCATED, &allocated, sizeof(allocated), hipMemcpyHostToDevice, st));
+    if (int rc = lf_rebuild_dir(p, s, st)) return rc;
     LF_HIP_CHECK(hipStreamSynchronize(st));
     return LF_OK;
 }
@@ -268,6 +269,7 @@ extern "C" int lf_map_copy(const lf_map_params *p, const lf_map_storage *src, co
         {dst->rgb, src->rgb, 3 * B * LF_BLOCK3},
         {dst->visible, src->visible, B * sizeof(int)},
         {dst->visible_pos, src->visible_pos, B * 4 * sizeof(int)},
+        {dst->block_dir, src->block_dir, (size_t)LF_DIR_CELLS * sizeof(int)},
         {dst->counters, src->counters, LF_NUM_COUNTERS * sizeof(unsigned int)},
         {dst->occupancy, src->occupancy, LF_OCC_WORDS * sizeof(unsigned int)},
     };

@@ -114,6 +114,35 @@ def test_occupancy_bits_cover_every_allocated_block(c0):
     assert np.unpackbits(occ.view(np.uint8)).sum() < len(blocks)
 
 
+def test_block_directory_and_visible_positions_match_the_pool():
+    """After several fused views the block directory holds exactly the
+    allocated blocks (cell of block_pos[b] = b, every other cell -1; the
+    raycast resolves through it), and each visible-list entry carries its
+    block's coordinates."""
+    sc = Scene(1, 1)
+    K = half_res(sc)
+    p = capi.map_params(2 ** 18, 1 << 17, 0.01, 0.03)
+    gmap = capi.Map(p, K.width, K.height)
+    for f in range(0, 20, 5):
+        T = sc.pose(f)
+        depth, nmap, rgb = frame_inputs(sc, T, K, f)
+        gmap.update_weighted(dev(depth), dev(nmap), dev(rgb), dev(capi.pose24(T)), K)
+    torch.cuda.synchronize()
+    blocks = gmap.blocks()
+    assert len(blocks) > 300
+    lib = capi.load()
+    cells = np.array([lib.lf_dir_cell(int(x), int(y), int(z)) for x, y, z in blocks])
+    assert np.all(cells >= 0)
+    d = gmap.t["block_dir"].cpu().numpy()
+    assert np.array_equal(d[cells], np.arange(len(blocks)))
+    assert int((d >= 0).sum()) == len(blocks)
+    n = int(gmap.counters()[capi.CNT_VISIBLE])
+    vis = gmap.t["visible"][:n].cpu().numpy()
+    vpos = gmap.t["visible_pos"][: 4 * n].view(n, 4).cpu().numpy()
+    assert n > 100 and np.array_equal(vpos[:, 3], vis)
+    assert np.array_equal(vpos[:, :3], blocks[vis])
+
+
 def test_update_sequence_fusion_half_res():
     """Five views fused on both sides (exercises the running weighted average)."""
     sc = Scene(1, 1)

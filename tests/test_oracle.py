@@ -64,6 +64,33 @@ def test_occupancy_bit_regions():
         assert lib.lf_occupancy_bit(r[0] + 3, r[1] + 3, r[2] + 3) == b
 
 
+def test_block_directory_cells_are_a_bijection_in_range():
+    """lf_dir_cell: every block coordinate with each axis in [-128, 128) has its
+    own cell in [0, 256^3), a 4x4x4-block brick takes 64 consecutive cells, and
+    everything outside maps to -1."""
+    lib = capi.load()
+    D, h = capi.DIR_DIM, capi.DIR_DIM // 2
+
+    def expect(bx, by, bz):
+        x, y, z = bx + h, by + h, bz + h
+        if not (0 <= x < D and 0 <= y < D and 0 <= z < D):
+            return -1
+        brick = ((z >> 2) * (D // 4) + (y >> 2)) * (D // 4) + (x >> 2)
+        return (brick << 6) | ((z & 3) << 4) | ((y & 3) << 2) | (x & 3)
+
+    rng = np.random.default_rng(7)
+    pts = rng.integers(-h, h, size=(20000, 3)).tolist()
+    cells = [lib.lf_dir_cell(*p) for p in pts]
+    assert cells == [expect(*p) for p in pts]
+    uniq = {tuple(p) for p in pts}
+    assert len({c for p, c in zip(pts, cells)}) == len(uniq)
+    assert all(0 <= c < capi.DIR_CELLS for c in cells)
+    brick = [lib.lf_dir_cell(8 + x, -4 + y, 100 + z) for z in range(4) for y in range(4) for x in range(4)]
+    assert sorted(brick) == list(range(min(brick), min(brick) + 64))
+    for p in ((h, 0, 0), (0, -h - 1, 0), (0, 0, 10 ** 6), (-(2 ** 31), 0, 0)):
+        assert lib.lf_dir_cell(*p) == -1
+
+
 def test_mc_tables_digest_and_consistency():
     edge, tri = capi.mc_tables()
     assert hashlib.sha256(struct.pack("<256i", *edge.tolist())).hexdigest() == EDGE_SHA
@@ -121,7 +148,7 @@ def test_storage_with_a_missing_array_is_rejected_before_device_work():
     _, K, p = configs.c0()
     fake = ctypes.c_void_p(0x10000)
     arrays = [f for f, t in capi.LfMapStorage._fields_ if t is ctypes.c_void_p]
-    assert "visible_pos" in arrays and len(arrays) == 12
+    assert "visible_pos" in arrays and "block_dir" in arrays and len(arrays) == 13
     for missing in arrays:
         s = capi.LfMapStorage()
         for i, f in enumerate(arrays):

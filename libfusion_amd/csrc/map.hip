@@ -302,9 +302,17 @@ __global__ __launch_bounds__(256) void k_integrate(MapDev m, const float *__rest
     const float zmin = m.zmin_update, zmax = m.zmax_update;
     const int t = threadIdx.x;
     const int lx = t & 7, ly = (t >> 3) & 7, lz0 = t >> 6;
+    // A lane owns voxels t and t + 256 of the block.  Three phases, each
+    // issuing both voxels' loads before anything waits on them: the
+    // projections and input gathers, then (inside the truncation band) the
+    // voxel's weight, sdf and colour together, then the fusion and stores.
     for (int i = blockIdx.x; i < nvis; i += gridDim.x) {
         const int4 p = reinterpret_cast<const int4 *>(m.visible_pos)[i];  // x, y, z, pool block
         const int b = p.w;
+        bool ok[2];
+        size_t pix[2];
+        float ptz[2], dist[2], nz[2], wpk[2];
+        unsigned int cpk[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int lz = lz0 + 4 * h;
@@ -312,56 +320,82 @@ __global__ __launch_bounds__(256) void k_integrate(MapDev m, const float *__rest
             const f3 pt = xform(Tcw, f3{wx, wy, wz});
             const int u = proj_floor(K.fx * pt.x / pt.z + K.cx + 0.5f);
             const int v = proj_floor(K.fy * pt.y / pt.z + K.cy + 0.5f);
-            if (u < 0 || v < 0 || u > cols - 1 || v > rows - 1) continue;
-            const size_t pix = (size_t)v * cols + u;
-            float dist, nz = 0.0f, wpk = 0.0f;
-            unsigned int cpk = 0;
+            ok[h] = !(u < 0 || v < 0 || u > cols - 1 || v > rows - 1);
+            pix[h] = ok[h] ? (size_t)v * cols + u : 0;
+            ptz[h] = pt.z;
+            dist[h] = 0.0f; nz[h] = 0.0f; wpk[h] = 0.0f; cpk[h] = 0u;
+        }
+        float nx[2] = {0.0f, 0.0f};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (!ok[h]) continue;
             if (kPacked) {
-                const float4 smp = samples[pix];
-                dist = smp.x;  // NaN also when the normal is invalid
-                wpk = smp.y;
-                cpk = __float_as_uint(smp.z);
-                if (isnan(dist) || dist > zmax || dist < zmin) continue;
-            } else if (kWeighted) {
-                dist = depth[pix];
-                const float nx = nmap[4 * pix];
-                nz = nmap[4 * pix + 2];
-                if (isnan(dist) || isnan(nx) || dist > zmax || dist < zmin) continue;
+                const float4 smp = samples[pix[h]];
+                dist[h] = smp.x;  // NaN also when the normal is invalid
+                wpk[h] = smp.y;
+                cpk[h] = __float_as_uint(smp.z);
             } else {
-                dist = depth[pix];
-                if (isnan(dist) || (double)dist < 1e-2 || dist > zmax || dist < zmin) continue;
+                dist[h] = depth[pix[h]];
+                if (kWeighted) {
+                    nx[h] = nmap[4 * pix[h]];
+                    nz[h] = nmap[4 * pix[h] + 2];
+                }
             }
-            float sdf = dist - pt.z;
-            if (sdf < -trunc) continue;
-            sdf = fminf(1.0f, sdf * inv_trunc);
+        }
+        float sdf[2], wp[2];
+        short sp[2];
+        unsigned char cold[2][3];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (!ok[h]) continue;
+            const float d = dist[h];
+            if (kPacked) ok[h] = !(isnan(d) || d > zmax || d < zmin);
+            else if (kWeighted) ok[h] = !(isnan(d) || isnan(nx[h]) || d > zmax || d < zmin);
+            else ok[h] = !(isnan(d) || (double)d < 1e-2 || d > zmax || d < zmin);
+            if (!ok[h]) continue;
+            sdf[h] = d - ptz[h];
+            ok[h] = !(sdf[h] < -trunc);
+            if (!ok[h]) continue;
             const size_t vi = (size_t)b * LF_BLOCK3 + (size_t)(t + 256 * h);
-            const float wp = m.weight[vi];
-            const float wt = kPacked ? wpk : kWeighted ? fabsf(sinf(nz)) / (dist * dist) : 1.0f / (dist * dist);
+            wp[h] = m.weight[vi];
+            sp[h] = m.sdf[vi];
+            if (kColour) {
+                const unsigned char *c = m.rgb + (size_t)b * (3 * LF_BLOCK3) + (size_t)(t + 256 * h);
+                cold[h][0] = c[0]; cold[h][1] = c[LF_BLOCK3]; cold[h][2] = c[2 * LF_BLOCK3];
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (!ok[h]) continue;
+            const float d = dist[h];
+            const float sdf_h = fminf(1.0f, sdf[h] * inv_trunc);
+            const size_t vi = (size_t)b * LF_BLOCK3 + (size_t)(t + 256 * h);
+            const float wt = kPacked ? wpk[h] : kWeighted ? fabsf(sinf(nz[h])) / (d * d) : 1.0f / (d * d);
             unsigned char *c = m.rgb + (size_t)b * (3 * LF_BLOCK3) + (size_t)(t + 256 * h);
             unsigned char cin[3];
             if (kColour) {
 #pragma unroll
                 for (int k = 0; k < 3; ++k)
-                    cin[k] = kPacked ? (unsigned char)(cpk >> (8 * k)) : rgb[3 * pix + k];
+                    cin[k] = kPacked ? (unsigned char)(cpk[h] >> (8 * k)) : rgb[3 * pix[h] + k];
             }
-            if (wp == 0.0f) {
+            if (wp[h] == 0.0f) {
                 // weight stays 0 only when wt is 0: keep the voxel's sdf code unseen
-                m.sdf[vi] = wt == 0.0f ? kSdfUnseen : pack_sdf(sdf);
+                m.sdf[vi] = wt == 0.0f ? kSdfUnseen : pack_sdf(sdf_h);
                 m.weight[vi] = wt > 255.0f ? 255.0f : wt;
                 if (kColour) {
                     c[0] = cin[0]; c[LF_BLOCK3] = cin[1]; c[2 * LF_BLOCK3] = cin[2];
                 }
                 continue;
             }
-            const float sp = unpack_sdf(m.sdf[vi]);
-            const float fused = (sp * wp + sdf * wt) / (wp + wt);
+            const float spf = unpack_sdf(sp[h]);
+            const float fused = (spf * wp[h] + sdf_h * wt) / (wp[h] + wt);
             m.sdf[vi] = pack_sdf(fused);
-            const float wn = wp + wt;
+            const float wn = wp[h] + wt;
             m.weight[vi] = wn > 255.0f ? 255.0f : wn;
             if (kColour) {
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    const float mixed = ((float)c[k * LF_BLOCK3] * wp + (float)cin[k] * wt) / (wp + wt);
+                    const float mixed = ((float)cold[h][k] * wp[h] + (float)cin[k] * wt) / (wp[h] + wt);
                     c[k * LF_BLOCK3] = (unsigned char)(int)mixed;
                 }
             }

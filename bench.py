@@ -57,6 +57,46 @@ def cpu_baseline(sc, params, K, frames_d, frames_c, n_frames):
             "sample": "C1 frames 0-%d (640x480, 5 mm), FP32 oracle System, %d OpenMP threads" % (n_frames - 1, threads)}
 
 
+def batch_throughput(n, warmup, steps, single_fps):
+    """Aggregate frames/s of n independent systems stepped together: stream i
+    is its own seeded C1 scene and orbit (seed 100 + i) with its own map and
+    HIP stream; each frame step enqueues one frame of every system, and the
+    device overlaps their kernels."""
+    import torch
+
+    from libfusion_amd import capi, configs
+    from libfusion_amd.synth import Scene
+
+    frames, systems = [], []
+    for i in range(n):
+        sc = Scene(1, 100 + i)
+        d, c = make_frames(sc, warmup + steps)
+        frames.append((torch.from_numpy(d).cuda(), torch.from_numpy(c).cuda()))
+        K = capi.intrinsics(sc.width, sc.height, sc.fx, sc.fy, sc.cx, sc.cy)
+        _, _, params = configs.c1()
+        systems.append(capi.System(K, params, configs.C1_LEVELS, configs.C1_ITERATIONS, sc.depth_scale))
+    torch.cuda.synchronize()
+    for f in range(warmup):
+        for s, (d, c) in zip(systems, frames):
+            s.process(d[f], c[f])
+    for s in systems:
+        s.synchronize()
+    t0 = time.perf_counter()
+    for f in range(warmup, warmup + steps):
+        for s, (d, c) in zip(systems, frames):
+            s.process(d[f], c[f])
+    for s in systems:
+        s.synchronize()
+    dt = time.perf_counter() - t0
+    value = n * steps / dt
+    return {"systems": n, "value": round(value, 1), "unit": "frames/s (all systems)",
+            "ms_per_step": round(1000.0 * dt / steps, 4), "vs_single_system": round(value / single_fps, 3),
+            "steps": steps, "warmup": warmup,
+            "workload": "%d independent plain FP32 C1 systems (seeds 100-%d: own scene, orbit, map and HIP "
+                        "stream), one frame of each per step" % (n, 99 + n),
+            "launch": "separate launches per system on its own stream (no fused cross-system launches)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +105,9 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stages", action="store_true", help="skip the instrumented stage-breakdown pass")
+    ap.add_argument("--batch", type=int, default=16,
+                    help="also time this many independent systems stepped together (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device-to-device (a plain copy kernel of known size, "
                          "to scale rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -169,6 +212,13 @@ def main():
                                       "algorithmic_bytes_per_launch": rb,
                                       "avg_launch_ms": round(stages["raycast_kernel"]["avg_ms"], 5)}
 
+    batched = None
+    if args.batch > 0:
+        del sysm
+        torch.cuda.empty_cache()
+        batched = batch_throughput(args.batch, args.warmup, args.batch_steps, fps)
+        log("batched: %d systems, %.1f frames/s aggregate" % (args.batch, batched["value"]))
+
     cpu = None
     if not args.no_cpu:
         t = time.perf_counter()
@@ -196,6 +246,7 @@ def main():
         "cpu_baseline": cpu,
         "stage_rooflines": stage_rooflines,
         "stages_ms": {k: round(v["avg_ms"], 4) for k, v in stages.items()},
+        "batched": batched,
         "allocated_blocks": int(min(c1[capi.CNT_ALLOCATED], params.num_blocks)),
         "overflow": int(c1[capi.CNT_OVERFLOW]),
         "final_pose_t": [round(float(x), 7) for x in pose[:3, 3]],

@@ -868,6 +868,44 @@ def test_graph_replayed_frames_equal_eager_frames():
     assert np.array_equal(blocks[0][1][ia], blocks[1][1][ib])
 
 
+def test_systems_stepped_together_equal_systems_run_alone():
+    """Independent systems enqueued frame by frame on their own streams (the
+    benchmark's batched mode) do not interact: each ends with exactly the pose,
+    tracking result and volume counters of the same sequence run alone."""
+    seqs = [Scene(1, 1), Scene(1, 7)]
+    K = half_res(seqs[0])
+    p = capi.map_params(2 ** 18, 1 << 17, 0.01, 0.03)
+    frames = []
+    for sc in seqs:
+        fs = []
+        for f in range(8):
+            d16, rgb = sc.render(sc.pose(f), K.width, K.height, K.fx, K.fy, K.cx, K.cy, noise=True, frame_id=f)
+            fs.append((dev(d16), dev(rgb)))
+        frames.append(fs)
+    torch.cuda.synchronize()
+    alone = []
+    for sc, fs in zip(seqs, frames):
+        s = capi.System(K, p, 3, (10, 5, 3), sc.depth_scale)
+        for d, c in fs:
+            s.process(d, c)
+        s.synchronize()
+        alone.append((s.pose(), s.tracking(), s.counters()))
+        del s
+    together = [capi.System(K, p, 3, (10, 5, 3), sc.depth_scale) for sc in seqs]
+    for f in range(8):
+        for s, fs in zip(together, frames):
+            s.process(*fs[f])
+    for s, (pose, trk, cnt) in zip(together, alone):
+        s.synchronize()
+        assert np.array_equal(s.pose(), pose)
+        t = s.tracking()
+        assert t[0] == trk[0] and np.array_equal(np.float32(t[1]), np.float32(trk[1]), equal_nan=True)
+        c = s.counters()
+        for k in (capi.CNT_ALLOCATED, capi.CNT_UPDATES, capi.CNT_VIS_SUM_LO, capi.CNT_RENDER_BLOCKS):
+            assert c[k] == cnt[k], k
+    assert alone[0][2][capi.CNT_ALLOCATED] != alone[1][2][capi.CNT_ALLOCATED]
+
+
 def test_gpu_end_to_end_matches_golden_c0():
     """C0 through the C-ABI only (depth conversion, bilateral, normals, update,
     raycast, mesh) against the committed fixture tests/golden/c0.npz."""

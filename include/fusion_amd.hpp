@@ -134,6 +134,7 @@ public:
     size_t fetch_mesh_with_normal(float *vertex, float *normal, int max_triangles);
     size_t fetch_mesh_with_colour(float *vertex, unsigned char *colour, int max_triangles);
     const lf_map_params &params() const { return params_; }
+    const IntrinsicMatrix &intrinsics() const { return K_; }
     const lf_map_storage &storage() const { return storage_; }
 
 private:
@@ -255,6 +256,13 @@ public:
     void set_graphs(bool on);
     int read_timing(int stage, int *launches, double *total_ms);
     bool mapping_enabled = true;
+    // The frame of process_images in three parts, for a Batch: tracking and
+    // fusion, then (by the caller) the raycast into raycast_vmap() at
+    // raycast_pose24(), then the re-pyramid.  No timing, no graphs.
+    int enqueue_before_raycast(const uint16_t *depth, const unsigned char *rgb);
+    float *raycast_vmap();
+    const float *raycast_pose24();
+    int enqueue_after_raycast();
 
 private:
     lf_system_config cfg_;
@@ -272,6 +280,35 @@ private:
     hipGraphExec_t exec_[2] = {nullptr, nullptr};
     GpuImage depth_in_, rgb_in_;
     hipEvent_t take_event();
+};
+
+// Several independent systems stepped together (no counterpart in the
+// reference, which runs one System per process).  A step enqueues every
+// system's tracking and fusion on its own stream, joins the streams, runs one
+// batched z-range + raycast launch per stage for all maps on the batch's
+// stream, then releases each system's re-pyramid.  Each system's results equal
+// those of the same system run alone; the batched raycast packs the systems'
+// long-ray tails onto one grid.
+class Batch {
+public:
+    Batch(const lf_system_config *cfgs, int n);
+    ~Batch();
+    int process(const uint16_t *const *depth, const unsigned char *const *rgb);
+    int synchronize();
+    int size() const { return (int)sys_.size(); }
+    System &system(int i) { return *sys_[i]; }
+    hipStream_t stream() const { return stream_; }
+
+private:
+    void release();
+    std::vector<std::unique_ptr<System>> sys_;
+    hipStream_t stream_ = nullptr;
+    std::vector<hipEvent_t> ready_;  // per system: its tracking and fusion are enqueued
+    hipEvent_t cast_ = nullptr;      // the batched raycast of the step
+    void *jobs_host_[2] = {nullptr, nullptr}, *jobs_dev_[2] = {nullptr, nullptr};
+    hipEvent_t jobs_read_[2] = {nullptr, nullptr};
+    bool jobs_used_[2] = {false, false};
+    int parity_ = 0;
 };
 
 }  // namespace fusion

@@ -467,6 +467,28 @@ int lf_system_set_timing(lf_system *sys, int stages);
 int lf_system_set_graphs(lf_system *sys, int enable);
 int lf_system_read_timing(lf_system *sys, int stage, int *launches, double *total_ms);
 
+/* ---------------------------------------------------------------- batch */
+
+/* Several independent systems stepped together (no counterpart in the
+ * reference, which runs one System per process; for callers that serve many
+ * RGB-D streams on one GPU).  System i is made from cfgs[i] (sizes and voxel
+ * sizes may differ).  lf_batch_process takes one frame per system (device
+ * pointers, uint16 depth and uint8 RGB of system i's size): each system's
+ * tracking and fusion run on its own stream, then one batched z-range +
+ * raycast launch per stage covers every map, then each system's re-pyramid.
+ * Each system's poses, tracking results and volume equal those of the same
+ * system run alone.  Asynchronous; lf_batch_synchronize waits for all. */
+typedef struct lf_batch lf_batch;
+int lf_batch_create(const lf_system_config *cfgs, int n, lf_batch **out);
+int lf_batch_destroy(lf_batch *b);
+int lf_batch_process(lf_batch *b, const uint16_t *const *depth, const unsigned char *const *rgb);
+int lf_batch_synchronize(lf_batch *b);
+/* System i's camera pose (4x4 row-major doubles; synchronising), last
+ * tracking result and map counters. */
+int lf_batch_get_pose(lf_batch *b, int i, double *out16);
+int lf_batch_get_tracking(lf_batch *b, int i, int *success, float *icp_error);
+int lf_batch_read_counters(lf_batch *b, int i, unsigned int *out16);
+
 #ifdef __cplusplus
 }
 #endif

@@ -139,6 +139,13 @@ SIGNATURES = [
     ("lf_system_set_timing", I, [P, I]),
     ("lf_system_set_graphs", I, [P, I]),
     ("lf_system_read_timing", I, [P, I, P, P]),
+    ("lf_batch_create", I, [P, I, P]),
+    ("lf_batch_destroy", I, [P]),
+    ("lf_batch_process", I, [P, P, P]),
+    ("lf_batch_synchronize", I, [P]),
+    ("lf_batch_get_pose", I, [P, I, P]),
+    ("lf_batch_get_tracking", I, [P, I, P, P]),
+    ("lf_batch_read_counters", I, [P, I, P]),
 ]
 
 _lib = None
@@ -485,6 +492,67 @@ class Map:
         sdf = np.where(sdf == SDF_UNSEEN, np.int16(0), sdf)
         return (self.blocks(), sdf, self.t["weight"][: 512 * n].view(n, 512).cpu().numpy(),
                 self.t["rgb"][: 1536 * n].view(n, 3, 512).cpu().numpy())
+
+
+def system_config(K, params, num_pyr, iterations, depth_scale, integrate_colour=True):
+    cfg = LfSystemConfig()
+    cfg.K = K
+    cfg.map = params
+    cfg.num_pyr = int(num_pyr)
+    for i, v in enumerate(iterations):
+        cfg.iterations[i] = int(v)
+    cfg.depth_scale = float(depth_scale)
+    cfg.integrate_colour = 1 if integrate_colour else 0
+    return cfg
+
+
+class Batch:
+    """lf_batch_*: independent systems stepped together, one frame each per
+    step, with one batched raycast launch per stage for all of them."""
+
+    def __init__(self, configs):
+        n = len(configs)
+        arr = (LfSystemConfig * n)(*configs)
+        h = ctypes.c_void_p()
+        check(load().lf_batch_create(arr, n, ctypes.byref(h)))
+        self.h, self.n = h, n
+
+    def close(self):
+        if self.h:
+            load().lf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def process(self, depths, rgbs):
+        """One frame per system: device tensors (uint16 depth, uint8 RGB)."""
+        assert len(depths) == len(rgbs) == self.n
+        d = (ctypes.c_void_p * self.n)(*[t.data_ptr() for t in depths])
+        c = (ctypes.c_void_p * self.n)(*[t.data_ptr() for t in rgbs])
+        check(load().lf_batch_process(self.h, d, c))
+
+    def synchronize(self):
+        check(load().lf_batch_synchronize(self.h))
+
+    def pose(self, i):
+        T = np.zeros((4, 4))
+        check(load().lf_batch_get_pose(self.h, int(i), T.ctypes.data_as(P)))
+        return T
+
+    def tracking(self, i):
+        s = ctypes.c_int(0)
+        e = ctypes.c_float(0)
+        check(load().lf_batch_get_tracking(self.h, int(i), ctypes.byref(s), ctypes.byref(e)))
+        return bool(s.value), float(e.value)
+
+    def counters(self, i):
+        out = np.zeros(16, np.uint32)
+        check(load().lf_batch_read_counters(self.h, int(i), out.ctypes.data_as(P)))
+        return out
 
 
 class System:

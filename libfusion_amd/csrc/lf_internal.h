@@ -43,3 +43,23 @@ int lf_pyramid_upload(const uint16_t *depth_u16, const unsigned char *rgb, float
 // DeviceImage::resize_device_map (device_image.cpp:188-199): vertex pyramid of
 // vmap[0] and the normals of every level, in (levels + 1) / 2 launches.
 int lf_pyramid_refresh(int levels, const lf_intrinsics *K, const LfImageLevels &m, hipStream_t st);
+
+// One raycast of a batch: a map, its target vertex map (and colour, or null),
+// the float T_wc / T_cw pose and the image size.
+struct LfRayItem {
+    const lf_map_params *p;
+    const lf_map_storage *s;
+    float *vmap;
+    unsigned char *rgb_out;
+    const float *pose24;
+    const lf_intrinsics *K;
+    int cols, rows;
+};
+// Bytes of one item's device descriptor; jobs_host (pinned) and jobs_dev
+// (device) hold n of them.  The host copy is read by an asynchronous copy on
+// st: it must not be rewritten before that copy has run.
+size_t lf_ray_job_bytes(void);
+// z-range and raycast of n maps in one launch per stage on st; each item's
+// result equals lf_raycast_ev's.
+int lf_raycast_batch_ev(const LfRayItem *items, int n, void *jobs_host, void *jobs_dev, int check_colour,
+                        hipStream_t st);

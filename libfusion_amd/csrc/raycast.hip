@@ -49,13 +49,31 @@ struct RayDev {
     int pool_fits_lds;  // pool indices fit the 24-bit field of the LDS cache
 };
 
-__global__ void k_zrange_clear(RayDev r) {
+// One raycast's inputs and outputs: the map, the target image and the pose
+// (single launches take it by value, batched launches read job blockIdx.y).
+struct RayJob {
+    RayDev r;
+    float *vmap;
+    unsigned char *rgb_out;
+    const float *pose24;
+    lf_intrinsics K;
+    int cols, rows, wg_x, wg_y;
+};
+
+// Workgroup wg of nwg: the z-range tiles back to k_zrange_clear's values.
+LF_DEV void zrange_clear_wg(const RayDev &r, int wg, int nwg) {
     const int n = r.tiles_x * r.tiles_y;
-    if (blockIdx.x == 0 && threadIdx.x == 0) r.counters[LF_CNT_RENDER_BLOCKS] = 0u;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (wg == 0 && threadIdx.x == 0) r.counters[LF_CNT_RENDER_BLOCKS] = 0u;
+    for (int i = wg * blockDim.x + threadIdx.x; i < n; i += nwg * blockDim.x) {
         r.zx[i] = 100.0f;
         r.zy[i] = 0.0f;
     }
+}
+
+__global__ void k_zrange_clear(RayDev r) { zrange_clear_wg(r, blockIdx.x, gridDim.x); }
+
+__global__ void k_zrange_clear_batch(const RayJob *__restrict__ jobs) {
+    zrange_clear_wg(jobs[blockIdx.y].r, blockIdx.x, gridDim.x);
 }
 
 LF_DEV int sat16(float v) {
@@ -80,16 +98,16 @@ constexpr int kZThreads = 512;  // lanes of a k_zrange_fill workgroup: 8 per vis
 // box of its rectangles, are flushed to the global grid with one atomic each.
 // Min and max do not depend on order, so the tiles are the same as with
 // per-block global atomics.
+// Workgroup wg of nwg.  s_tile: kLds: [0, T) near z bits, [T, 2T) far z bits;
+// s_box: ulx, uly, lrx, lry of the workgroup's rectangles; s_rend: its
+// rendering blocks.
 template <bool kLds>
-__global__ __launch_bounds__(kZThreads) void k_zrange_fill(RayDev r, const float *__restrict__ pose24,
-                                                           lf_intrinsics K) {
-    extern __shared__ int s_tile[];  // kLds: [0, T) near z bits, [T, 2T) far z bits
-    __shared__ int s_box[4];         // ulx, uly, lrx, lry of the workgroup's rectangles
-    __shared__ unsigned int s_rend;  // rendering blocks of this workgroup
+LF_DEV void zrange_fill_wg(const RayDev &r, const float *__restrict__ pose24, const lf_intrinsics &K, int wg, int nwg,
+                           int *s_tile, int *s_box, unsigned int &s_rend) {
     constexpr int kPer = kZThreads / 8;  // blocks per workgroup pass
     const int tid = threadIdx.x, c = tid & 7;
     const int nvis = (int)r.counters[LF_CNT_VISIBLE];
-    if ((int)(blockIdx.x * kPer) >= nvis) return;  // whole workgroup
+    if (wg * kPer >= nvis) return;  // whole workgroup
     const int T = r.tiles_x * r.tiles_y;
     const int near0 = __float_as_int(100.0f), far0 = __float_as_int(0.0f);  // k_zrange_clear's values
     if (tid == 0) s_rend = 0u;
@@ -109,7 +127,7 @@ __global__ __launch_bounds__(kZThreads) void k_zrange_fill(RayDev r, const float
     for (int i = 0; i < 12; ++i) Tcw[i] = pose24[12 + i];
     const float scale = 8.0f * r.voxel_size, qnan = __int_as_float(0x7fc00000);
     // the trip count is uniform over the workgroup, so every lane reaches the shuffles
-    for (int i0 = blockIdx.x * kPer; i0 < nvis; i0 += gridDim.x * kPer) {
+    for (int i0 = wg * kPer; i0 < nvis; i0 += nwg * kPer) {
         const int i = i0 + tid / 8;
         const bool have = i < nvis;
         float fu = qnan, cu = qnan, fv = qnan, cv = qnan, zlo = qnan, zhi = qnan;  // NaN: ignored
@@ -175,6 +193,24 @@ __global__ __launch_bounds__(kZThreads) void k_zrange_fill(RayDev r, const float
             if (s_tile[T + t] != far0) atomicMax(reinterpret_cast<int *>(r.zy) + t, s_tile[T + t]);
         }
     }
+}
+
+template <bool kLds>
+__global__ __launch_bounds__(kZThreads) void k_zrange_fill(RayDev r, const float *__restrict__ pose24,
+                                                           lf_intrinsics K) {
+    extern __shared__ int s_tile[];
+    __shared__ int s_box[4];
+    __shared__ unsigned int s_rend;
+    zrange_fill_wg<kLds>(r, pose24, K, blockIdx.x, gridDim.x, s_tile, s_box, s_rend);
+}
+
+template <bool kLds>
+__global__ __launch_bounds__(kZThreads) void k_zrange_fill_batch(const RayJob *__restrict__ jobs) {
+    extern __shared__ int s_tile[];
+    __shared__ int s_box[4];
+    __shared__ unsigned int s_rend;
+    const RayJob &j = jobs[blockIdx.y];
+    zrange_fill_wg<kLds>(j.r, j.pose24, j.K, blockIdx.x, gridDim.x, s_tile, s_box, s_rend);
 }
 
 // Workgroup-shared, direct-mapped cache of block lookups in LDS, negative
@@ -290,15 +326,15 @@ LF_DEV float read_interp(const RayDev &r, BlockCache &bc, f3 p, bool &valid) {
 
 // MapRenderingDelegate::operator() (check_colour == false) and
 // ::raycast_with_colour (check_colour == true): ray_tracing.cu:346-425,478-564.
+// Workgroup wg (its index within this raycast's grid; the grid's x extent is
+// a multiple of 8, so wg % 8 is the XCD the workgroup runs on).
 template <bool kCheckColour>
-__global__ __launch_bounds__(256) void k_raycast(RayDev r, float *__restrict__ vmap, unsigned char *__restrict__ rgb_out,
-                                                 int cols, int rows, const float *__restrict__ pose24,
-                                                 lf_intrinsics K, int wg_x, int wg_y) {
+LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char *__restrict__ rgb_out, int cols,
+                       int rows, const float *__restrict__ pose24, const lf_intrinsics &K, int wg_x, int wg_y, int wg,
+                       unsigned long long *s_cache, uint4 *s_occ) {
     // DenseMapping::raycast leaves the map untouched when nothing is visible
     // (voxel_hashing.cpp:92-93) or no rendering block was made (:105).
     if (r.counters[LF_CNT_VISIBLE] == 0u || r.counters[LF_CNT_RENDER_BLOCKS] == 0u) return;
-    __shared__ unsigned long long s_cache[kLdsCache];
-    __shared__ uint4 s_occ[LF_OCC_WORDS / 4];
     for (int i = threadIdx.x; i < kLdsCache; i += blockDim.x) s_cache[i] = 0ull;
     const uint4 *occ4 = reinterpret_cast<const uint4 *>(r.occupancy);
     for (int i = threadIdx.x; i < LF_OCC_WORDS / 4; i += blockDim.x) s_occ[i] = occ4[i];
@@ -309,7 +345,7 @@ __global__ __launch_bounds__(256) void k_raycast(RayDev r, float *__restrict__ v
     // each XCD marches a contiguous band of the image, and the blocks its rays
     // read stay in that XCD's L2.
     const int nwg = wg_x * wg_y, per_xcd = (nwg + kXcds - 1) / kXcds;
-    const int L = (int)(blockIdx.x % kXcds) * per_xcd + (int)(blockIdx.x / kXcds);
+    const int L = (wg % kXcds) * per_xcd + wg / kXcds;
     if (L >= nwg) return;
     const int tile_x = (L % wg_x) * 2 + (wave & 1), tile_y = (L / wg_x) * 2 + (wave >> 1);
     const int x = tile_x * 8 + (lane & 7), y = tile_y * 8 + (lane >> 3);
@@ -375,6 +411,26 @@ __global__ __launch_bounds__(256) void k_raycast(RayDev r, float *__restrict__ v
     }
 }
 
+template <bool kCheckColour>
+__global__ __launch_bounds__(256) void k_raycast(RayDev r, float *__restrict__ vmap, unsigned char *__restrict__ rgb_out,
+                                                 int cols, int rows, const float *__restrict__ pose24,
+                                                 lf_intrinsics K, int wg_x, int wg_y) {
+    __shared__ unsigned long long s_cache[kLdsCache];
+    __shared__ uint4 s_occ[LF_OCC_WORDS / 4];
+    raycast_wg<kCheckColour>(r, vmap, rgb_out, cols, rows, pose24, K, wg_x, wg_y, blockIdx.x, s_cache, s_occ);
+}
+
+template <bool kCheckColour>
+__global__ __launch_bounds__(256) void k_raycast_batch(const RayJob *__restrict__ jobs) {
+    __shared__ unsigned long long s_cache[kLdsCache];
+    __shared__ uint4 s_occ[LF_OCC_WORDS / 4];
+    const RayJob &j = jobs[blockIdx.y];
+    const int g = kXcds * ((j.wg_x * j.wg_y + kXcds - 1) / kXcds);
+    if ((int)blockIdx.x >= g) return;  // this raycast's grid is smaller than the batch's
+    raycast_wg<kCheckColour>(j.r, j.vmap, j.rgb_out, j.cols, j.rows, j.pose24, j.K, j.wg_x, j.wg_y, blockIdx.x,
+                             s_cache, s_occ);
+}
+
 // Dynamic LDS available to one workgroup on this device (less the static use).
 size_t zrange_lds_limit() {
     static size_t limit = 0;
@@ -390,6 +446,25 @@ size_t zrange_lds_limit() {
 
 }  // namespace
 
+static RayDev make_raydev(const lf_map_params *p, const lf_map_storage *s) {
+    RayDev r;
+    r.keys = s->keys; r.slot_block = s->slot_block; r.sdf = s->sdf; r.weight = s->weight; r.rgb = s->rgb;
+    r.visible = s->visible; r.visible_pos = s->visible_pos; r.block_dir = s->block_dir; r.block_pos = s->block_pos;
+    r.counters = s->counters;
+    r.occupancy = s->occupancy;
+    r.zx = s->zrange; r.zy = s->zrange + (size_t)s->tiles_x * s->tiles_y;
+    r.tiles_x = s->tiles_x; r.tiles_y = s->tiles_y;
+    r.num_buckets = p->num_buckets; r.num_entries = p->num_entries;
+    r.voxel_size = p->voxel_size; r.truncation = p->truncation;
+    r.zmin_raycast = p->zmin_raycast; r.zmax_raycast = p->zmax_raycast;
+    r.pool_fits_lds = p->num_blocks < (int)kNoBlock ? 1 : 0;
+    return r;
+}
+
+static int zrange_grid(const lf_map_params *p) {
+    return std::max(1, std::min(1024, (p->num_blocks + kZThreads / 8 - 1) / (kZThreads / 8)));
+}
+
 int lf_raycast_ev(const lf_map_params *p, const lf_map_storage *s, float *vmap, unsigned char *rgb_out, int cols,
                   int rows, const float *pose24, const lf_intrinsics *K, int check_colour, hipStream_t st,
                   hipEvent_t e0, hipEvent_t e1) {
@@ -398,22 +473,13 @@ int lf_raycast_ev(const lf_map_params *p, const lf_map_storage *s, float *vmap, 
     if (cols / 8 != s->tiles_x || rows / 8 != s->tiles_y)
         return lf_set_error(LF_ERR_ARG, "raycast size %dx%d does not match the map's %dx%d tiles", cols, rows,
                             s->tiles_x, s->tiles_y);
-    RayDev r;
-    r.keys = s->keys; r.slot_block = s->slot_block; r.sdf = s->sdf; r.weight = s->weight; r.rgb = s->rgb;
-    r.visible = s->visible; r.visible_pos = s->visible_pos; r.block_dir = s->block_dir; r.block_pos = s->block_pos; r.counters = s->counters;
-    r.occupancy = s->occupancy;
-    r.zx = s->zrange; r.zy = s->zrange + (size_t)s->tiles_x * s->tiles_y;
-    r.tiles_x = s->tiles_x; r.tiles_y = s->tiles_y;
-    r.num_buckets = p->num_buckets; r.num_entries = p->num_entries;
-    r.voxel_size = p->voxel_size; r.truncation = p->truncation;
-    r.zmin_raycast = p->zmin_raycast; r.zmax_raycast = p->zmax_raycast;
-    r.pool_fits_lds = p->num_blocks < (int)kNoBlock ? 1 : 0;
+    const RayDev r = make_raydev(p, s);
     const int T = s->tiles_x * s->tiles_y;
     k_zrange_clear<<<std::max(1, std::min(1024, (T + 255) / 256)), 256, 0, st>>>(r);
     LF_LAUNCH_CHECK();
     // the per-workgroup tile grid in LDS when it fits (C1: 4800 tiles, 38 KB)
     const size_t lds = 2 * (size_t)T * sizeof(int);
-    const int zgrid = std::max(1, std::min(1024, (p->num_blocks + kZThreads / 8 - 1) / (kZThreads / 8)));
+    const int zgrid = zrange_grid(p);
     if (lds <= zrange_lds_limit()) k_zrange_fill<true><<<zgrid, kZThreads, lds, st>>>(r, pose24, *K);
     else k_zrange_fill<false><<<zgrid, kZThreads, 0, st>>>(r, pose24, *K);
     LF_LAUNCH_CHECK();
@@ -425,6 +491,57 @@ int lf_raycast_ev(const lf_map_params *p, const lf_map_storage *s, float *vmap, 
     else k_raycast<false><<<g, 256, 0, st>>>(r, vmap, rgb_out, cols, rows, pose24, *K, wg_x, wg_y);
     LF_LAUNCH_CHECK();
     if (e1) LF_HIP_CHECK(hipEventRecord(e1, st));
+    return LF_OK;
+}
+
+size_t lf_ray_job_bytes(void) { return sizeof(RayJob); }
+
+// Several raycasts (their own maps, images and poses) in one launch per stage:
+// a 2-D grid whose y index is the raycast, x the workgroup within it.  Each
+// raycast's workgroups run exactly the single-launch code, so its tiles and
+// vertex map equal those of lf_raycast_ev; the batch packs the raycasts'
+// long-ray tails onto one grid instead of leaving each tail alone on the chip.
+int lf_raycast_batch_ev(const LfRayItem *items, int n, void *jobs_host, void *jobs_dev, int check_colour,
+                        hipStream_t st) {
+    if (!items || n <= 0 || !jobs_host || !jobs_dev) return lf_set_error(LF_ERR_ARG, "bad raycast batch arguments");
+    if (n > 65535) return lf_set_error(LF_ERR_ARG, "raycast batch of %d exceeds the grid's y extent", n);
+    RayJob *J = static_cast<RayJob *>(jobs_host);
+    int tmax = 0, zmax = 0, gmax = 0;
+    size_t lds_max = 0;
+    for (int i = 0; i < n; ++i) {
+        const LfRayItem &it = items[i];
+        if (!it.p || !it.s || !it.vmap || !it.pose24 || !it.K)
+            return lf_set_error(LF_ERR_ARG, "null argument in raycast batch item %d", i);
+        if (int rc = lf_check_storage(it.s)) return rc;
+        if (it.cols / 8 != it.s->tiles_x || it.rows / 8 != it.s->tiles_y)
+            return lf_set_error(LF_ERR_ARG, "batch item %d: raycast size %dx%d does not match the map's %dx%d tiles",
+                                i, it.cols, it.rows, it.s->tiles_x, it.s->tiles_y);
+        J[i].r = make_raydev(it.p, it.s);
+        J[i].vmap = it.vmap;
+        J[i].rgb_out = it.rgb_out;
+        J[i].pose24 = it.pose24;
+        J[i].K = *it.K;
+        J[i].cols = it.cols;
+        J[i].rows = it.rows;
+        const int tx = (it.cols + 7) / 8, ty = (it.rows + 7) / 8;
+        J[i].wg_x = (tx + 1) / 2;
+        J[i].wg_y = (ty + 1) / 2;
+        const int T = it.s->tiles_x * it.s->tiles_y;
+        tmax = std::max(tmax, T);
+        zmax = std::max(zmax, zrange_grid(it.p));
+        gmax = std::max(gmax, kXcds * ((J[i].wg_x * J[i].wg_y + kXcds - 1) / kXcds));
+        lds_max = std::max(lds_max, 2 * (size_t)T * sizeof(int));
+    }
+    const RayJob *jobs = static_cast<const RayJob *>(jobs_dev);
+    LF_HIP_CHECK(hipMemcpyAsync(jobs_dev, jobs_host, (size_t)n * sizeof(RayJob), hipMemcpyHostToDevice, st));
+    k_zrange_clear_batch<<<dim3(std::max(1, std::min(1024, (tmax + 255) / 256)), n), 256, 0, st>>>(jobs);
+    LF_LAUNCH_CHECK();
+    if (lds_max <= zrange_lds_limit()) k_zrange_fill_batch<true><<<dim3(zmax, n), kZThreads, lds_max, st>>>(jobs);
+    else k_zrange_fill_batch<false><<<dim3(zmax, n), kZThreads, 0, st>>>(jobs);
+    LF_LAUNCH_CHECK();
+    if (check_colour) k_raycast_batch<true><<<dim3(gmax, n), 256, 0, st>>>(jobs);
+    else k_raycast_batch<false><<<dim3(gmax, n), 256, 0, st>>>(jobs);
+    LF_LAUNCH_CHECK();
     return LF_OK;
 }
 

@@ -520,14 +520,106 @@ void System::set_graphs(bool on) {
     if (!on) drop_graphs();
 }
 
-// The frame's device work (trackFrame, update, raycast, re-pyramid) without events.
-int System::enqueue_frame(const uint16_t *depth, const unsigned char *rgb) {
+int System::enqueue_before_raycast(const uint16_t *depth, const unsigned char *rgb) {
     DenseOdometry &od = *odometry_;
     if (!od.trackingLost) LF_RC(od.trackFrame(depth, rgb, cfg_.depth_scale, initial_pose_));
     DeviceImage &ref = od.get_reference_image();
     if (mapping_enabled) LF_RC(mapping_->update(ref, od.pose24(), cfg_.integrate_colour != 0));
-    LF_RC(mapping_->raycast(ref.get_vmap(0).ptr<float>(), od.pose24()));
-    return ref.resize_device_map();
+    return LF_OK;
+}
+
+float *System::raycast_vmap() { return odometry_->get_reference_image().get_vmap(0).ptr<float>(); }
+
+const float *System::raycast_pose24() { return odometry_->pose24(); }
+
+int System::enqueue_after_raycast() { return odometry_->get_reference_image().resize_device_map(); }
+
+// The frame's device work (trackFrame, update, raycast, re-pyramid) without events.
+int System::enqueue_frame(const uint16_t *depth, const unsigned char *rgb) {
+    LF_RC(enqueue_before_raycast(depth, rgb));
+    LF_RC(mapping_->raycast(raycast_vmap(), raycast_pose24()));
+    return enqueue_after_raycast();
+}
+
+// ------------------------------------------------------------------- Batch
+
+Batch::Batch(const lf_system_config *cfgs, int n) {
+    auto fail = [&]() {
+        release();
+        throw std::bad_alloc();
+    };
+    for (int i = 0; i < n; ++i) sys_.emplace_back(new System(cfgs[i]));
+    if (hipStreamCreate(&stream_) != hipSuccess) fail();
+    ready_.assign(n, nullptr);
+    for (hipEvent_t &e : ready_)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) fail();
+    if (hipEventCreateWithFlags(&cast_, hipEventDisableTiming) != hipSuccess) fail();
+    for (int k = 0; k < 2; ++k) {
+        const size_t bytes = (size_t)n * lf_ray_job_bytes();
+        if (hipHostMalloc(&jobs_host_[k], bytes, hipHostMallocDefault) != hipSuccess ||
+            hipMalloc(&jobs_dev_[k], bytes) != hipSuccess ||
+            hipEventCreateWithFlags(&jobs_read_[k], hipEventDisableTiming) != hipSuccess)
+            fail();
+    }
+}
+
+Batch::~Batch() { release(); }
+
+void Batch::release() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (auto &s : sys_)
+        if (s) (void)hipStreamSynchronize(s->stream());
+    for (hipEvent_t e : ready_)
+        if (e) (void)hipEventDestroy(e);
+    ready_.clear();
+    if (cast_) (void)hipEventDestroy(cast_);
+    cast_ = nullptr;
+    for (int k = 0; k < 2; ++k) {
+        if (jobs_host_[k]) (void)hipHostFree(jobs_host_[k]);
+        if (jobs_dev_[k]) (void)hipFree(jobs_dev_[k]);
+        if (jobs_read_[k]) (void)hipEventDestroy(jobs_read_[k]);
+        jobs_host_[k] = jobs_dev_[k] = nullptr;
+        jobs_read_[k] = nullptr;
+    }
+    if (stream_) (void)hipStreamDestroy(stream_);
+    stream_ = nullptr;
+    sys_.clear();
+}
+
+int Batch::process(const uint16_t *const *depth, const unsigned char *const *rgb) {
+    const int n = size();
+    for (int i = 0; i < n; ++i) {
+        LF_RC(sys_[i]->enqueue_before_raycast(depth[i], rgb[i]));
+        LF_HIP_CHECK(hipEventRecord(ready_[i], sys_[i]->stream()));
+        LF_HIP_CHECK(hipStreamWaitEvent(stream_, ready_[i], 0));
+    }
+    // the pinned descriptors of two steps ago must have been copied before
+    // they are rewritten
+    const int k = parity_;
+    if (jobs_used_[k]) LF_HIP_CHECK(hipEventSynchronize(jobs_read_[k]));
+    std::vector<LfRayItem> items(n);
+    for (int i = 0; i < n; ++i) {
+        DenseMapping &m = sys_[i]->mapping();
+        items[i] = LfRayItem{&m.params(), &m.storage(), sys_[i]->raycast_vmap(), nullptr,
+                             sys_[i]->raycast_pose24(), &m.intrinsics(), m.intrinsics().width,
+                             m.intrinsics().height};
+    }
+    LF_RC(lf_raycast_batch_ev(items.data(), n, jobs_host_[k], jobs_dev_[k], 1, stream_));
+    LF_HIP_CHECK(hipEventRecord(jobs_read_[k], stream_));
+    jobs_used_[k] = true;
+    parity_ ^= 1;
+    LF_HIP_CHECK(hipEventRecord(cast_, stream_));
+    for (int i = 0; i < n; ++i) {
+        LF_HIP_CHECK(hipStreamWaitEvent(sys_[i]->stream(), cast_, 0));
+        LF_RC(sys_[i]->enqueue_after_raycast());
+    }
+    return LF_OK;
+}
+
+int Batch::synchronize() {
+    for (auto &s : sys_) LF_HIP_CHECK(hipStreamSynchronize(s->stream()));
+    LF_HIP_CHECK(hipStreamSynchronize(stream_));
+    return LF_OK;
 }
 
 int System::process_images(const uint16_t *depth, const unsigned char *rgb) {
@@ -920,4 +1012,67 @@ extern "C" int lf_system_set_timing(lf_system *s, int stages) {
 
 extern "C" int lf_system_read_timing(lf_system *s, int stage, int *launches, double *total_ms) {
     return s->sys->read_timing(stage, launches, total_ms);
+}
+
+// ------------------------------------------------------------ batch C-ABI
+
+struct lf_batch {
+    fusion::Batch *b;
+};
+
+extern "C" int lf_batch_create(const lf_system_config *cfgs, int n, lf_batch **out) {
+    if (!cfgs || !out || n <= 0) return lf_set_error(LF_ERR_ARG, "bad arguments to lf_batch_create");
+    for (int i = 0; i < n; ++i)
+        if (int rc = check_halving(&cfgs[i].K, cfgs[i].num_pyr)) return rc;
+    try {
+        *out = new lf_batch{new fusion::Batch(cfgs, n)};
+    } catch (const std::exception &e) {
+        return lf_set_error(LF_ERR_HIP, "batch creation failed: %s", e.what());
+    }
+    return LF_OK;
+}
+
+extern "C" int lf_batch_destroy(lf_batch *b) {
+    if (!b) return LF_OK;
+    delete b->b;
+    delete b;
+    return LF_OK;
+}
+
+extern "C" int lf_batch_process(lf_batch *b, const uint16_t *const *depth, const unsigned char *const *rgb) {
+    if (!b || !depth || !rgb) return lf_set_error(LF_ERR_ARG, "null argument to lf_batch_process");
+    for (int i = 0; i < b->b->size(); ++i)
+        if (!depth[i] || !rgb[i]) return lf_set_error(LF_ERR_ARG, "null frame for batch system %d", i);
+    return b->b->process(depth, rgb);
+}
+
+extern "C" int lf_batch_synchronize(lf_batch *b) {
+    if (!b) return lf_set_error(LF_ERR_ARG, "null batch");
+    return b->b->synchronize();
+}
+
+static int batch_index(lf_batch *b, int i) {
+    if (!b) return lf_set_error(LF_ERR_ARG, "null batch");
+    if (i < 0 || i >= b->b->size()) return lf_set_error(LF_ERR_ARG, "batch system %d of %d", i, b->b->size());
+    return LF_OK;
+}
+
+extern "C" int lf_batch_get_pose(lf_batch *b, int i, double *out16) {
+    if (int rc = batch_index(b, i)) return rc;
+    if (!out16) return lf_set_error(LF_ERR_ARG, "null pose output");
+    return b->b->system(i).get_camera_pose(out16);
+}
+
+extern "C" int lf_batch_get_tracking(lf_batch *b, int i, int *success, float *icp_error) {
+    if (int rc = batch_index(b, i)) return rc;
+    fusion::TrackingResult r = b->b->system(i).odometry().tracker().result();
+    if (success) *success = r.sucess ? 1 : 0;
+    if (icp_error) *icp_error = r.icp_error;
+    return LF_OK;
+}
+
+extern "C" int lf_batch_read_counters(lf_batch *b, int i, unsigned int *out16) {
+    if (int rc = batch_index(b, i)) return rc;
+    fusion::System &s = b->b->system(i);
+    return lf_map_read_counters(&s.mapping().storage(), out16, s.stream());
 }

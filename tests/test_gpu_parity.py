@@ -906,6 +906,45 @@ def test_systems_stepped_together_equal_systems_run_alone():
     assert alone[0][2][capi.CNT_ALLOCATED] != alone[1][2][capi.CNT_ALLOCATED]
 
 
+def test_batch_with_one_raycast_launch_equals_systems_run_alone():
+    """lf_batch: three systems of different image sizes and voxel sizes whose
+    raycasts share one batched launch per stage.  Each ends with exactly the
+    pose, tracking result and volume counters of the same system run alone."""
+    specs = [(Scene(1, 1), 2, 0.01), (Scene(1, 7), 2, 0.008), (Scene(1, 3), 4, 0.01)]
+    runs = []
+    for sc, div, vs in specs:
+        K = capi.intrinsics(sc.width // div, sc.height // div, sc.fx / div, sc.fy / div,
+                            (sc.cx + 0.5) / div - 0.5, (sc.cy + 0.5) / div - 0.5)
+        p = capi.map_params(2 ** 18, 1 << 17, vs, 3 * vs)
+        fs = []
+        for f in range(8):
+            d16, rgb = sc.render(sc.pose(f), K.width, K.height, K.fx, K.fy, K.cx, K.cy, noise=True, frame_id=f)
+            fs.append((dev(d16), dev(rgb)))
+        runs.append((K, p, sc.depth_scale, fs))
+    torch.cuda.synchronize()
+    alone = []
+    for K, p, ds, fs in runs:
+        s = capi.System(K, p, 3, (10, 5, 3), ds)
+        for d, c in fs:
+            s.process(d, c)
+        s.synchronize()
+        alone.append((s.pose(), s.tracking(), s.counters()))
+        del s
+    batch = capi.Batch([capi.system_config(K, p, 3, (10, 5, 3), ds) for K, p, ds, _ in runs])
+    for f in range(8):
+        batch.process([r[3][f][0] for r in runs], [r[3][f][1] for r in runs])
+    batch.synchronize()
+    for i, (pose, trk, cnt) in enumerate(alone):
+        assert np.array_equal(batch.pose(i), pose), i
+        t = batch.tracking(i)
+        assert t[0] == trk[0] and np.array_equal(np.float32(t[1]), np.float32(trk[1]), equal_nan=True), i
+        c = batch.counters(i)
+        for k in (capi.CNT_ALLOCATED, capi.CNT_UPDATES, capi.CNT_VIS_SUM_LO, capi.CNT_RENDER_BLOCKS):
+            assert c[k] == cnt[k], (i, k)
+        assert cnt[capi.CNT_ALLOCATED] > 300 and cnt[capi.CNT_RENDER_BLOCKS] > 0, i
+    batch.close()
+
+
 def test_gpu_end_to_end_matches_golden_c0():
     """C0 through the C-ABI only (depth conversion, bilateral, normals, update,
     raycast, mesh) against the committed fixture tests/golden/c0.npz."""

@@ -60,9 +60,10 @@ def cpu_baseline(sc, params, K, frames_d, frames_c, n_frames):
 def batch_throughput(n, warmup, steps, single_fps):
     """Aggregate frames/s of n independent C1 systems stepped together: system i
     is its own seeded scene and orbit (seed 100 + i) with its own map.  Two
-    ways: lf_batch (each system's tracking and fusion on its own stream, one
-    batched z-range + raycast launch per stage for all maps), and n separate
-    lf_system objects on their own streams with no shared launches."""
+    ways: lf_batch (step s of every system's tracking in one launch pair, each
+    map fused on its own stream, one batched z-range + raycast launch per stage
+    for all maps), and n separate lf_system objects on their own streams with
+    no shared launches."""
     import torch
 
     from libfusion_amd import capi, configs
@@ -111,8 +112,8 @@ def batch_throughput(n, warmup, steps, single_fps):
             "steps": steps, "warmup": warmup,
             "workload": "%d independent plain FP32 C1 systems (seeds 100-%d: own scene, orbit and map), one frame "
                         "of each per step" % (n, 99 + n),
-            "launch": "lf_batch: tracking and fusion per system on its own stream, one batched z-range + raycast "
-                      "launch per stage for all maps",
+            "launch": "lf_batch: each Gauss-Newton step of all systems in one launch pair, fusion per system on "
+                      "its own stream, one batched z-range + raycast launch per stage for all maps",
             "separate_systems": {"value": round(v_s, 1), "ms_per_step": round(1000.0 * dt_s / steps, 4),
                                  "launch": "separate lf_system objects on their own streams, no shared launches"}}
 

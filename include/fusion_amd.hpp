@@ -170,6 +170,11 @@ public:
     DenseTracking &operator=(const DenseTracking &) = delete;
     // compute_transform(reference, current, ctx) (icp_tracker.cpp:23-156)
     int compute_transform(DeviceImage &reference, DeviceImage &current, const TrackingContext &c);
+    // The same frame as launch arguments instead of launches (for a Batch):
+    // one job of lf_track_job_bytes() bytes per Gauss-Newton step appended to
+    // jobs, its grid to grids.  A plan without steps is launched at once.
+    int plan(DeviceImage &reference, DeviceImage &current, const TrackingContext &c,
+             std::vector<unsigned char> &jobs, std::vector<int> &grids);
     // compute_transform(ctx) (icp_tracker.cpp:210-314) on the set_* pyramids
     // (source = current frame, reference = last); success is unconditional.
     int compute_transform(const TrackingContext &c);
@@ -192,7 +197,8 @@ public:
     float icp_weight = 1e6f;  // icp_tracker.cpp:89-90
 
 private:
-    int enqueue(DeviceImage &reference, DeviceImage &current, const TrackingContext &c, bool context_mode);
+    int enqueue(DeviceImage &reference, DeviceImage &current, const TrackingContext &c, bool context_mode,
+                std::vector<unsigned char> *jobs = nullptr, std::vector<int> *grids = nullptr);
     hipStream_t stream_;
     LfTrackState *state_ = nullptr;
     float *scratch_ = nullptr;
@@ -206,6 +212,10 @@ public:
     DenseOdometry(const IntrinsicMatrix &base, int num_pyr, std::vector<int> iterations, hipStream_t stream);
     // trackFrame (rgbd_odometry.cpp:18-48) on device-resident input.
     int trackFrame(const uint16_t *depth, const unsigned char *rgb, float depth_scale, const SE3d &initial_pose);
+    // trackFrame with the Gauss-Newton steps planned into jobs / grids instead
+    // of launched (the upload and the host bookkeeping happen as in trackFrame).
+    int planFrame(const uint16_t *depth, const unsigned char *rgb, float depth_scale, const SE3d &initial_pose,
+                  std::vector<unsigned char> &jobs, std::vector<int> &grids);
     void reset();
     DeviceImage &get_current_image() { return *curr_; }
     DeviceImage &get_reference_image() { return *ref_; }
@@ -260,6 +270,11 @@ public:
     // fusion, then (by the caller) the raycast into raycast_vmap() at
     // raycast_pose24(), then the re-pyramid.  No timing, no graphs.
     int enqueue_before_raycast(const uint16_t *depth, const unsigned char *rgb);
+    // enqueue_before_raycast in two parts for a Batch: the upload with the
+    // tracking steps planned (launched by the batch), then the fusion.
+    int plan_tracking(const uint16_t *depth, const unsigned char *rgb, std::vector<unsigned char> &jobs,
+                      std::vector<int> &grids);
+    int enqueue_fusion();
     float *raycast_vmap();
     const float *raycast_pose24();
     int enqueue_after_raycast();
@@ -283,12 +298,14 @@ private:
 };
 
 // Several independent systems stepped together (no counterpart in the
-// reference, which runs one System per process).  A step enqueues every
-// system's tracking and fusion on its own stream, joins the streams, runs one
-// batched z-range + raycast launch per stage for all maps on the batch's
-// stream, then releases each system's re-pyramid.  Each system's results equal
-// those of the same system run alone; the batched raycast packs the systems'
-// long-ray tails onto one grid.
+// reference, which runs one System per process).  A step uploads every
+// system's frame on its own stream; runs the Gauss-Newton steps of all systems
+// on the batch's stream, step s of every system in one launch pair; fuses each
+// map on its own stream; runs one batched z-range + raycast launch per stage
+// for all maps; then each system's re-pyramid.  Each system's results equal
+// those of the same system run alone; the batched launches replace 2 x 18
+// tracking launches per system and pack the raycasts' long-ray tails onto one
+// grid.
 class Batch {
 public:
     Batch(const lf_system_config *cfgs, int n);
@@ -309,6 +326,11 @@ private:
     hipEvent_t jobs_read_[2] = {nullptr, nullptr};
     bool jobs_used_[2] = {false, false};
     int parity_ = 0;
+    // tracking jobs: [step][system], double-buffered like the raycast jobs
+    void *track_host_[2] = {nullptr, nullptr}, *track_dev_[2] = {nullptr, nullptr};
+    size_t track_cap_ = 0;  // jobs per buffer
+    hipEvent_t tracked_ = nullptr;
+    int reserve_track(size_t jobs);
 };
 
 }  // namespace fusion

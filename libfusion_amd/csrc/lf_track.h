@@ -61,3 +61,15 @@ int lf_track_begin(LfTrackState *st, const double *T12, hipStream_t stream);
 int lf_track_finish(LfTrackState *st, hipStream_t stream);
 // Set the odometry pose directly (first frame / restart): T given as 3x4 doubles (host).
 int lf_track_set_pose(LfTrackState *st, const double *T12, hipStream_t stream);
+
+// Batched tracking: one step of several systems in one launch pair.  A job is
+// the launch arguments of one system's step (lf_track_job_bytes each);
+// lf_track_job_make fills one and reports its grid, lf_track_job_skip marks a
+// system without a step at this index.  lf_track_batch_step runs step jobs[0..n)
+// (device memory) with a grid of gmax workgroups per system (the largest job
+// grid); each system's results equal lf_track_iteration's.
+size_t lf_track_job_bytes(void);
+int lf_track_job_make(const LfTrackLevel &lv, float *scratch, LfTrackState *st, const LfStepFlags &f,
+                      float icp_weight, void *job, int *grid);
+void lf_track_job_skip(void *job);
+int lf_track_batch_step(const void *jobs, int n, int gmax, hipStream_t stream);

@@ -600,12 +600,12 @@ LF_DEV void track_finish(LfTrackState &st, const double *est, int invalid, bool 
 // workgroup sums the ICP and RGB rows of its pixels into one partial row.  The
 // pose-independent loads of the first pixel are issued together with the
 // tracker-state loads.
-__global__ __launch_bounds__(kThreads) void k_track_step(ReduceArgs a) {
-    __shared__ float s_wave[kWaves][kVals];
+// Workgroup wg of the nwg computing the step's partial rows.
+LF_DEV void track_step_wg(const ReduceArgs &a, int wg, int nwg, float (*s_wave)[kVals]) {
     const int tid = threadIdx.x;
     const int n = a.lv.cols * a.lv.rows;
-    const int stride = gridDim.x * kThreads;
-    int i = blockIdx.x * kThreads + tid;
+    const int stride = nwg * kThreads;
+    int i = wg * kThreads + tid;
     // the pixel's own loads and the state loads are in flight together
     Own o = {};
     if (i < n) o = load_own<true, true>(a.lv, i);
@@ -635,7 +635,12 @@ __global__ __launch_bounds__(kThreads) void k_track_step(ReduceArgs a) {
         i = next;
     }
     const float t = block_sums<true, true>(si, sr, s_wave);
-    if (tid < kVals) a.partials[(size_t)tid * gridDim.x + blockIdx.x] = t;
+    if (tid < kVals) a.partials[(size_t)tid * nwg + wg] = t;
+}
+
+__global__ __launch_bounds__(kThreads) void k_track_step(ReduceArgs a) {
+    __shared__ float s_wave[kWaves][kVals];
+    track_step_wg(a, blockIdx.x, gridDim.x, s_wave);
 }
 
 // Second launch (one workgroup): the partial rows summed in a fixed order,
@@ -644,8 +649,7 @@ __global__ __launch_bounds__(kThreads) void k_track_step(ReduceArgs a) {
 // last-workgroup ticket needs an agent-scope release per workgroup, an L2
 // write-back on each of the 8 XCDs per workgroup; measured slower than the
 // kernel boundary.)
-__global__ __launch_bounds__(kThreads) void k_track_final(ReduceArgs a, int G) {
-    __shared__ float s_sum[kVals];
+LF_DEV void track_final_wg(const ReduceArgs &a, int G, float *s_sum) {
     const int tid = threadIdx.x;
     GnState g = load_gn(*a.st);
     if (a.first_of_frame) {
@@ -668,6 +672,36 @@ __global__ __launch_bounds__(kThreads) void k_track_final(ReduceArgs a, int G) {
         if (g.invalid || (a.last_level && !g.active) || a.last_of_frame)
             track_finish(st, g.est, g.invalid, a.context_mode != 0);
     }
+}
+
+__global__ __launch_bounds__(kThreads) void k_track_final(ReduceArgs a, int G) {
+    __shared__ float s_sum[kVals];
+    track_final_wg(a, G, s_sum);
+}
+
+// One system's step of a batch: its launch arguments and its grid (0: the
+// system has no step at this index).
+struct TrackJob {
+    ReduceArgs a;
+    int G;
+};
+
+// Step s of several systems (each its own tracker state, scratch and
+// pyramids): a 2-D grid whose y index is the system; each system's workgroups
+// run exactly the single launch's code.
+__global__ __launch_bounds__(kThreads) void k_track_step_batch(const TrackJob *__restrict__ jobs) {
+    __shared__ float s_wave[kWaves][kVals];
+    const TrackJob &j = jobs[blockIdx.y];
+    if ((int)blockIdx.x >= j.G) return;
+    track_step_wg(j.a, blockIdx.x, j.G, s_wave);
+}
+
+// The final sums and Gauss-Newton step of each system: workgroup i, system i.
+__global__ __launch_bounds__(kThreads) void k_track_final_batch(const TrackJob *__restrict__ jobs) {
+    __shared__ float s_sum[kVals];
+    const TrackJob &j = jobs[blockIdx.x];
+    if (j.G <= 0) return;
+    track_final_wg(j.a, j.G, s_sum);
 }
 
 __global__ void k_track_finish(LfTrackState *st) {
@@ -730,9 +764,8 @@ int lf_track_begin(LfTrackState *st, const double *T, hipStream_t stream) {
     return LF_OK;
 }
 
-int lf_track_iteration(const LfTrackLevel &lv, float *scratch, LfTrackState *st, const LfStepFlags &f,
-                       float icp_weight, hipStream_t stream) {
-    if (int rc = check_level(lv)) return rc;
+static ReduceArgs step_args(const LfTrackLevel &lv, float *scratch, LfTrackState *st, const LfStepFlags &f,
+                            float icp_weight) {
     ReduceArgs a = {};
     a.lv = lv;
     a.angle = icp_angle();
@@ -746,10 +779,48 @@ int lf_track_iteration(const LfTrackLevel &lv, float *scratch, LfTrackState *st,
     a.last_of_frame = f.last_of_frame;
     a.context_mode = f.context_mode;
     for (int k = 0; k < 12; ++k) a.init[k] = f.init[k];
+    return a;
+}
+
+int lf_track_iteration(const LfTrackLevel &lv, float *scratch, LfTrackState *st, const LfStepFlags &f,
+                       float icp_weight, hipStream_t stream) {
+    if (int rc = check_level(lv)) return rc;
+    const ReduceArgs a = step_args(lv, scratch, st, f, icp_weight);
     const int G = grid_for(lv);
     k_track_step<<<G, kThreads, 0, stream>>>(a);
     LF_LAUNCH_CHECK();
     k_track_final<<<1, kThreads, 0, stream>>>(a, G);
+    LF_LAUNCH_CHECK();
+    return LF_OK;
+}
+
+size_t lf_track_job_bytes(void) { return sizeof(TrackJob); }
+
+int lf_track_job_make(const LfTrackLevel &lv, float *scratch, LfTrackState *st, const LfStepFlags &f,
+                      float icp_weight, void *job, int *grid) {
+    if (int rc = check_level(lv)) return rc;
+    TrackJob &j = *static_cast<TrackJob *>(job);
+    j = TrackJob{};
+    j.a = step_args(lv, scratch, st, f, icp_weight);
+    j.G = grid_for(lv);
+    if (grid) *grid = j.G;
+    return LF_OK;
+}
+
+void lf_track_job_skip(void *job) {
+    TrackJob &j = *static_cast<TrackJob *>(job);
+    j = TrackJob{};
+    j.G = 0;
+}
+
+int lf_track_batch_step(const void *jobs, int n, int gmax, hipStream_t stream) {
+    if (!jobs || n <= 0 || n > 65535) return lf_set_error(LF_ERR_ARG, "bad tracking batch of %d", n);
+    const TrackJob *J = static_cast<const TrackJob *>(jobs);
+    if (gmax > 0) {
+        k_track_step_batch<<<dim3(gmax, n), kThreads, 0, stream>>>(J);
+        LF_LAUNCH_CHECK();
+    }
+    k_track_final_batch<<<n, kThreads, 0, stream>>>(J);
     LF_LAUNCH_CHECK();
     return LF_OK;
 }

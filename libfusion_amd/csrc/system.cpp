@@ -361,7 +361,13 @@ int DenseTracking::compute_transform(DeviceImage &ref, DeviceImage &cur, const T
     return enqueue(ref, cur, c, false);
 }
 
-int DenseTracking::enqueue(DeviceImage &ref, DeviceImage &cur, const TrackingContext &c, bool context_mode) {
+int DenseTracking::plan(DeviceImage &ref, DeviceImage &cur, const TrackingContext &c,
+                        std::vector<unsigned char> &jobs, std::vector<int> &grids) {
+    return enqueue(ref, cur, c, false, &jobs, &grids);
+}
+
+int DenseTracking::enqueue(DeviceImage &ref, DeviceImage &cur, const TrackingContext &c, bool context_mode,
+                           std::vector<unsigned char> *jobs, std::vector<int> *grids) {
     context_mode_ = context_mode;
     LfStepFlags f = {};
     f.context_mode = context_mode ? 1 : 0;
@@ -396,7 +402,15 @@ int DenseTracking::enqueue(DeviceImage &ref, DeviceImage &cur, const TrackingCon
         f.first_of_frame = s == 0;
         f.last_level = level == finest;
         f.last_of_frame = s + 1 == plan.size();
-        LF_RC(lf_track_iteration(lv, scratch_, state_, f, icp_weight, stream_));
+        if (jobs) {
+            const size_t J = lf_track_job_bytes();
+            jobs->resize(jobs->size() + J);
+            int G = 0;
+            LF_RC(lf_track_job_make(lv, scratch_, state_, f, icp_weight, jobs->data() + jobs->size() - J, &G));
+            grids->push_back(G);
+        } else {
+            LF_RC(lf_track_iteration(lv, scratch_, state_, f, icp_weight, stream_));
+        }
     }
     return LF_OK;
 }
@@ -444,6 +458,30 @@ int DenseOdometry::trackFrame(const uint16_t *depth, const unsigned char *rgb, f
     c.max_iterations_ = iterations_;
     // compute_transform ends with the finish, which also chains the odometry pose
     LF_RC(tracker_.compute_transform(*ref_, *curr_, c));
+    std::swap(curr_, ref_);
+    parity_ ^= 1;
+    trackingLost = false;
+    return LF_OK;
+}
+
+int DenseOdometry::planFrame(const uint16_t *depth, const unsigned char *rgb, float depth_scale,
+                             const SE3d &initial_pose, std::vector<unsigned char> &jobs, std::vector<int> &grids) {
+    LF_RC(curr_->upload(depth, rgb, depth_scale));
+    if (!initialized_) {
+        double T[12];
+        initial_pose.matrix3x4(T);
+        LF_RC(lf_track_set_pose(tracker_.state(), T, stream_));
+        std::swap(curr_, ref_);
+        parity_ ^= 1;
+        initialized_ = true;
+        return LF_OK;
+    }
+    TrackingContext c;
+    c.use_initial_guess_ = true;
+    c.initial_estimate_ = SE3d();
+    c.intrinsics_pyr_ = K_;
+    c.max_iterations_ = iterations_;
+    LF_RC(tracker_.plan(*ref_, *curr_, c, jobs, grids));
     std::swap(curr_, ref_);
     parity_ ^= 1;
     trackingLost = false;
@@ -528,6 +566,19 @@ int System::enqueue_before_raycast(const uint16_t *depth, const unsigned char *r
     return LF_OK;
 }
 
+int System::plan_tracking(const uint16_t *depth, const unsigned char *rgb, std::vector<unsigned char> &jobs,
+                          std::vector<int> &grids) {
+    DenseOdometry &od = *odometry_;
+    if (!od.trackingLost) LF_RC(od.planFrame(depth, rgb, cfg_.depth_scale, initial_pose_, jobs, grids));
+    return LF_OK;
+}
+
+int System::enqueue_fusion() {
+    DenseOdometry &od = *odometry_;
+    if (mapping_enabled) LF_RC(mapping_->update(od.get_reference_image(), od.pose24(), cfg_.integrate_colour != 0));
+    return LF_OK;
+}
+
 float *System::raycast_vmap() { return odometry_->get_reference_image().get_vmap(0).ptr<float>(); }
 
 const float *System::raycast_pose24() { return odometry_->pose24(); }
@@ -553,7 +604,9 @@ Batch::Batch(const lf_system_config *cfgs, int n) {
     ready_.assign(n, nullptr);
     for (hipEvent_t &e : ready_)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) fail();
-    if (hipEventCreateWithFlags(&cast_, hipEventDisableTiming) != hipSuccess) fail();
+    if (hipEventCreateWithFlags(&cast_, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&tracked_, hipEventDisableTiming) != hipSuccess)
+        fail();
     for (int k = 0; k < 2; ++k) {
         const size_t bytes = (size_t)n * lf_ray_job_bytes();
         if (hipHostMalloc(&jobs_host_[k], bytes, hipHostMallocDefault) != hipSuccess ||
@@ -574,7 +627,12 @@ void Batch::release() {
     ready_.clear();
     if (cast_) (void)hipEventDestroy(cast_);
     cast_ = nullptr;
+    if (tracked_) (void)hipEventDestroy(tracked_);
+    tracked_ = nullptr;
     for (int k = 0; k < 2; ++k) {
+        if (track_host_[k]) (void)hipHostFree(track_host_[k]);
+        if (track_dev_[k]) (void)hipFree(track_dev_[k]);
+        track_host_[k] = track_dev_[k] = nullptr;
         if (jobs_host_[k]) (void)hipHostFree(jobs_host_[k]);
         if (jobs_dev_[k]) (void)hipFree(jobs_dev_[k]);
         if (jobs_read_[k]) (void)hipEventDestroy(jobs_read_[k]);
@@ -586,17 +644,66 @@ void Batch::release() {
     sys_.clear();
 }
 
+// Room for `jobs` tracking jobs per buffer.  Growing waits for the batch
+// stream, so no launch still reads the old buffers.
+int Batch::reserve_track(size_t jobs) {
+    if (jobs <= track_cap_) return LF_OK;
+    LF_HIP_CHECK(hipStreamSynchronize(stream_));
+    const size_t cap = std::max(jobs, 2 * track_cap_), bytes = cap * lf_track_job_bytes();
+    for (int k = 0; k < 2; ++k) {
+        if (track_host_[k]) LF_HIP_CHECK(hipHostFree(track_host_[k]));
+        if (track_dev_[k]) LF_HIP_CHECK(hipFree(track_dev_[k]));
+        track_host_[k] = track_dev_[k] = nullptr;
+        LF_HIP_CHECK(hipHostMalloc(&track_host_[k], bytes, hipHostMallocDefault));
+        LF_HIP_CHECK(hipMalloc(&track_dev_[k], bytes));
+    }
+    track_cap_ = cap;
+    return LF_OK;
+}
+
 int Batch::process(const uint16_t *const *depth, const unsigned char *const *rgb) {
     const int n = size();
+    // uploads and pyramids on the systems' streams; their steps planned
+    std::vector<std::vector<unsigned char>> jobs(n);
+    std::vector<std::vector<int>> grids(n);
+    size_t steps = 0;
     for (int i = 0; i < n; ++i) {
-        LF_RC(sys_[i]->enqueue_before_raycast(depth[i], rgb[i]));
+        LF_RC(sys_[i]->plan_tracking(depth[i], rgb[i], jobs[i], grids[i]));
+        steps = std::max(steps, grids[i].size());
         LF_HIP_CHECK(hipEventRecord(ready_[i], sys_[i]->stream()));
         LF_HIP_CHECK(hipStreamWaitEvent(stream_, ready_[i], 0));
     }
-    // the pinned descriptors of two steps ago must have been copied before
-    // they are rewritten
+    // the pinned buffers of two steps ago must have been copied before they
+    // are rewritten
     const int k = parity_;
     if (jobs_used_[k]) LF_HIP_CHECK(hipEventSynchronize(jobs_read_[k]));
+    if (steps) {
+        LF_RC(reserve_track(steps * n));
+        const size_t J = lf_track_job_bytes();
+        unsigned char *host = static_cast<unsigned char *>(track_host_[k]);
+        std::vector<int> gmax(steps, 0);
+        for (size_t s = 0; s < steps; ++s)
+            for (int i = 0; i < n; ++i) {
+                unsigned char *dst = host + (s * n + i) * J;
+                if (s < grids[i].size()) {
+                    std::memcpy(dst, jobs[i].data() + s * J, J);
+                    gmax[s] = std::max(gmax[s], grids[i][s]);
+                } else {
+                    lf_track_job_skip(dst);
+                }
+            }
+        LF_HIP_CHECK(hipMemcpyAsync(track_dev_[k], host, steps * n * J, hipMemcpyHostToDevice, stream_));
+        for (size_t s = 0; s < steps; ++s)
+            LF_RC(lf_track_batch_step(static_cast<unsigned char *>(track_dev_[k]) + s * n * J, n, gmax[s], stream_));
+    }
+    // fusion on the systems' streams once their poses are final
+    LF_HIP_CHECK(hipEventRecord(tracked_, stream_));
+    for (int i = 0; i < n; ++i) {
+        LF_HIP_CHECK(hipStreamWaitEvent(sys_[i]->stream(), tracked_, 0));
+        LF_RC(sys_[i]->enqueue_fusion());
+        LF_HIP_CHECK(hipEventRecord(ready_[i], sys_[i]->stream()));
+        LF_HIP_CHECK(hipStreamWaitEvent(stream_, ready_[i], 0));
+    }
     std::vector<LfRayItem> items(n);
     for (int i = 0; i < n; ++i) {
         DenseMapping &m = sys_[i]->mapping();

@@ -156,7 +156,7 @@ def load():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise RuntimeError("libfusion_amd.so is not built (run __graft_entry__.build()); "
+            raise RuntimeError("libfusion_amd.so is not built (run libfusion_amd.build.build_all()); "
                                "there is no fallback path")
         lib = ctypes.CDLL(LIB_PATH)
         for name, res, args in SIGNATURES:

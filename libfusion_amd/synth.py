@@ -23,7 +23,7 @@ def _load():
     if _lib is None:
         path = os.path.join(_HERE, "liblf_synth.so")
         if not os.path.exists(path):
-            raise RuntimeError("liblf_synth.so is not built; run __graft_entry__.build()")
+            raise RuntimeError("liblf_synth.so is not built; run libfusion_amd.build.build_all()")
         lib = ctypes.CDLL(path)
         P = ctypes.c_void_p
         lib.synth_scene_create.restype = P

@@ -73,6 +73,11 @@ public:
     int upload(const uint16_t *depth_u16, const unsigned char *rgb, float depth_scale);
     // DeviceImage::resize_device_map (device_image.cpp:188-199).
     int resize_device_map();
+    // upload / resize_device_map planned as frame-pass descriptors appended
+    // to out (launched by a Batch together with other images' passes).
+    int plan_upload(const uint16_t *depth_u16, const unsigned char *rgb, float depth_scale,
+                    std::vector<unsigned char> &out);
+    int plan_refresh(std::vector<unsigned char> &out);
     // Pyramids from given level-0 device planes (build_pyramid.cpp:7-76), as
     // DenseTracking::set_source_* / set_reference_* build them:
     int set_vmap(const float *vmap);                 // vertices, then every level's vertices and normals
@@ -213,9 +218,11 @@ public:
     // trackFrame (rgbd_odometry.cpp:18-48) on device-resident input.
     int trackFrame(const uint16_t *depth, const unsigned char *rgb, float depth_scale, const SE3d &initial_pose);
     // trackFrame with the Gauss-Newton steps planned into jobs / grids instead
-    // of launched (the upload and the host bookkeeping happen as in trackFrame).
+    // of launched (the host bookkeeping happens as in trackFrame); the upload
+    // is launched, or planned into *upload when that is given.
     int planFrame(const uint16_t *depth, const unsigned char *rgb, float depth_scale, const SE3d &initial_pose,
-                  std::vector<unsigned char> &jobs, std::vector<int> &grids);
+                  std::vector<unsigned char> &jobs, std::vector<int> &grids,
+                  std::vector<unsigned char> *upload = nullptr);
     void reset();
     DeviceImage &get_current_image() { return *curr_; }
     DeviceImage &get_reference_image() { return *ref_; }
@@ -271,13 +278,16 @@ public:
     // raycast_pose24(), then the re-pyramid.  No timing, no graphs.
     int enqueue_before_raycast(const uint16_t *depth, const unsigned char *rgb);
     // enqueue_before_raycast in two parts for a Batch: the upload with the
-    // tracking steps planned (launched by the batch), then the fusion.
+    // tracking steps planned (launched by the batch; the upload too when
+    // upload is given), then the fusion.
     int plan_tracking(const uint16_t *depth, const unsigned char *rgb, std::vector<unsigned char> &jobs,
-                      std::vector<int> &grids);
+                      std::vector<int> &grids, std::vector<unsigned char> *upload = nullptr);
     int enqueue_fusion();
     float *raycast_vmap();
     const float *raycast_pose24();
     int enqueue_after_raycast();
+    // enqueue_after_raycast planned as frame passes (after plan_tracking).
+    int plan_after_raycast(std::vector<unsigned char> &passes);
 
 private:
     lf_system_config cfg_;
@@ -299,13 +309,15 @@ private:
 
 // Several independent systems stepped together (no counterpart in the
 // reference, which runs one System per process).  A step uploads every
-// system's frame on its own stream; runs the Gauss-Newton steps of all systems
-// on the batch's stream, step s of every system in one launch pair; fuses each
-// map on its own stream; runs one batched z-range + raycast launch per stage
-// for all maps; then each system's re-pyramid.  Each system's results equal
-// those of the same system run alone; the batched launches replace 2 x 18
-// tracking launches per system and pack the raycasts' long-ray tails onto one
-// grid.
+// system's frame, pass p of all systems in one launch, on an upload stream
+// that overlaps the previous step's fusion and raycast; runs the Gauss-Newton
+// steps of all systems on the batch's stream, step s of every system in one
+// launch pair; fuses each map on its own stream; runs one batched z-range +
+// raycast launch per stage for all maps; then the re-pyramids, again one
+// launch per pass for all systems.  Each system's results equal those of the
+// same system run alone; the batched launches replace 2 x 18 tracking
+// launches and 6 pyramid launches per system and pack the raycasts' long-ray
+// tails onto one grid.
 class Batch {
 public:
     Batch(const lf_system_config *cfgs, int n);
@@ -321,7 +333,8 @@ private:
     std::vector<std::unique_ptr<System>> sys_;
     hipStream_t stream_ = nullptr;
     std::vector<hipEvent_t> ready_;  // per system: its tracking and fusion are enqueued
-    hipEvent_t cast_ = nullptr;      // the batched raycast of the step
+    hipStream_t aux_ = nullptr;      // uploads of the next step's frames
+    hipEvent_t uploaded_ = nullptr;  // the step's uploads
     void *jobs_host_[2] = {nullptr, nullptr}, *jobs_dev_[2] = {nullptr, nullptr};
     hipEvent_t jobs_read_[2] = {nullptr, nullptr};
     bool jobs_used_[2] = {false, false};
@@ -330,7 +343,12 @@ private:
     void *track_host_[2] = {nullptr, nullptr}, *track_dev_[2] = {nullptr, nullptr};
     size_t track_cap_ = 0;  // jobs per buffer
     hipEvent_t tracked_ = nullptr;
+    bool tracked_once_ = false;
     int reserve_track(size_t jobs);
+    // frame passes: [upload pass][system] then [re-pyramid pass][system]
+    void *pass_host_[2] = {nullptr, nullptr}, *pass_dev_[2] = {nullptr, nullptr};
+    size_t pass_cap_ = 0;  // bytes per buffer
+    int reserve_pass(size_t bytes);
 };
 
 }  // namespace fusion

@@ -9,6 +9,8 @@
 //
 // All kernels are one lane per output pixel on 16x16 workgroups (4 wavefronts of
 // 16x4 pixels): every row segment a wavefront reads or writes is contiguous.
+#include <vector>
+
 #include "lf_common.h"
 #include "lf_internal.h"
 
@@ -223,15 +225,17 @@ struct Part {
 struct Pass {
     Part part[kMaxParts];
     int nparts;
+    int total_wg;  // workgroups of all parts
 };
 
-__global__ __launch_bounds__(256) void k_frame_pass(Pass ps) {
+// Workgroup wg of a pass.
+LF_DEV void frame_pass_wg(const Pass &ps, int wg) {
     int pi = 0;
 #pragma unroll
     for (int k = 1; k < kMaxParts; ++k)
-        if (k < ps.nparts && (int)blockIdx.x >= ps.part[k].wg_begin) pi = k;
+        if (k < ps.nparts && wg >= ps.part[k].wg_begin) pi = k;
     const Part &p = ps.part[pi];
-    const int t = blockIdx.x - p.wg_begin, tiles_x = (p.w + 15) >> 4;
+    const int t = wg - p.wg_begin, tiles_x = (p.w + 15) >> 4;
     const int x = (t % tiles_x) * 16 + (threadIdx.x & 15), y = (t / tiles_x) * 16 + (threadIdx.x >> 4);
     if (x >= p.w || y >= p.h) return;
     const int w = p.w, h = p.h;
@@ -278,6 +282,15 @@ __global__ __launch_bounds__(256) void k_frame_pass(Pass ps) {
     }
 }
 
+__global__ __launch_bounds__(256) void k_frame_pass(Pass ps) { frame_pass_wg(ps, blockIdx.x); }
+
+// One pass of each of gridDim.y images: row y of the grid runs table[y].
+__global__ __launch_bounds__(256) void k_frame_pass_batch(const Pass *__restrict__ table) {
+    const Pass &ps = table[blockIdx.y];
+    if ((int)blockIdx.x >= ps.total_wg) return;
+    frame_pass_wg(ps, blockIdx.x);
+}
+
 dim3 grid16(int w, int h) { return dim3((w + 15) / 16, (h + 15) / 16); }
 int grid1d(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>(4096, (n + 255) / 256)); }
 
@@ -294,14 +307,19 @@ Part make_part(int kind, int w, int h) {
     return p;
 }
 
-int launch_pass(Pass &ps, hipStream_t st) {
+// Workgroup offsets of the parts and the pass's total.
+void seal_pass(Pass &ps) {
     int wg = 0;
     for (int k = 0; k < ps.nparts; ++k) {
         ps.part[k].wg_begin = wg;
         wg += ((ps.part[k].w + 15) / 16) * ((ps.part[k].h + 15) / 16);
     }
-    if (wg == 0) return LF_OK;
-    k_frame_pass<<<wg, 256, 0, st>>>(ps);
+    ps.total_wg = wg;
+}
+
+int launch_pass(const Pass &ps, hipStream_t st) {
+    if (ps.total_wg == 0) return LF_OK;
+    k_frame_pass<<<ps.total_wg, 256, 0, st>>>(ps);
     LF_LAUNCH_CHECK();
     return LF_OK;
 }
@@ -317,13 +335,11 @@ int check_pyramid(int levels, const lf_intrinsics *K) {
     return LF_OK;
 }
 
-}  // namespace
-
-// DeviceImage::upload as levels + 1 launches: the frame pass (raw depth,
+// DeviceImage::upload as levels + 1 passes: the frame pass (raw depth,
 // bilateral depth, intensity, colour copy), then per level its maps
 // (derivatives, vertices, normals) together with the next level's pyramids.
-int lf_pyramid_upload(const uint16_t *depth_u16, const unsigned char *rgb, float depth_scale, int levels,
-                    const lf_intrinsics *K, const LfImageLevels &m, hipStream_t st) {
+int plan_upload(const uint16_t *depth_u16, const unsigned char *rgb, float depth_scale, int levels,
+                const lf_intrinsics *K, const LfImageLevels &m, std::vector<Pass> &out) {
     if (int rc = check_pyramid(levels, K)) return rc;
     if (!depth_u16 || !rgb) return lf_set_error(LF_ERR_ARG, "null frame input");
     Pass ps = {};
@@ -331,9 +347,10 @@ int lf_pyramid_upload(const uint16_t *depth_u16, const unsigned char *rgb, float
     f.in0 = depth_u16; f.in1 = rgb; f.scale = depth_scale;
     f.out0 = m.depth_raw; f.out1 = m.depth[0]; f.out2 = m.intensity[0]; f.out3 = m.rgb;
     ps.part[ps.nparts++] = f;
-    if (int rc = launch_pass(ps, st)) return rc;
+    seal_pass(ps);
+    out.push_back(ps);
     for (int l = 0; l < levels; ++l) {
-        ps.nparts = 0;
+        ps = Pass{};
         Part p = make_part(kLevelMaps, K[l].width, K[l].height);
         p.in0 = m.depth[l]; p.in1 = m.intensity[l]; p.K = K[l];
         p.out0 = m.dx[l]; p.out1 = m.dy[l]; p.out2 = m.vmap[l]; p.out3 = m.nmap[l];
@@ -346,16 +363,17 @@ int lf_pyramid_upload(const uint16_t *depth_u16, const unsigned char *rgb, float
             q.in0 = m.depth[l]; q.in1 = m.intensity[l]; q.out0 = m.depth[l + 1]; q.out1 = m.intensity[l + 1];
             ps.part[ps.nparts++] = q;
         }
-        if (int rc = launch_pass(ps, st)) return rc;
+        seal_pass(ps);
+        out.push_back(ps);
     }
     return LF_OK;
 }
 
 // DeviceImage::resize_device_map: the vertex pyramid of a raycast level-0 map
-// and every level's normals, two levels per launch (a launch with base level
-// b writes the normals still missing below b + 1 and the vertices of levels
+// and every level's normals, two levels per pass (a pass with base level b
+// writes the normals still missing below b + 1 and the vertices of levels
 // b + 1 and b + 2, the latter resized twice on the fly).
-int lf_pyramid_refresh(int levels, const lf_intrinsics *K, const LfImageLevels &m, hipStream_t st) {
+int plan_refresh(int levels, const lf_intrinsics *K, const LfImageLevels &m, std::vector<Pass> &out) {
     if (int rc = check_pyramid(levels, K)) return rc;
     int nmap_done = 0;
     for (int b = 0; nmap_done < levels; b += 2) {
@@ -375,8 +393,64 @@ int lf_pyramid_refresh(int levels, const lf_intrinsics *K, const LfImageLevels &
             p.in0 = m.vmap[b]; p.out0 = m.vmap[b + 2];
             ps.part[ps.nparts++] = p;
         }
-        if (int rc = launch_pass(ps, st)) return rc;
+        seal_pass(ps);
+        out.push_back(ps);
     }
+    return LF_OK;
+}
+
+int launch_all(const std::vector<Pass> &passes, hipStream_t st) {
+    for (const Pass &ps : passes)
+        if (int rc = launch_pass(ps, st)) return rc;
+    return LF_OK;
+}
+
+void append_bytes(const std::vector<Pass> &passes, std::vector<unsigned char> &out) {
+    const unsigned char *b = reinterpret_cast<const unsigned char *>(passes.data());
+    out.insert(out.end(), b, b + passes.size() * sizeof(Pass));
+}
+
+}  // namespace
+
+int lf_pyramid_upload(const uint16_t *depth_u16, const unsigned char *rgb, float depth_scale, int levels,
+                      const lf_intrinsics *K, const LfImageLevels &m, hipStream_t st) {
+    std::vector<Pass> passes;
+    if (int rc = plan_upload(depth_u16, rgb, depth_scale, levels, K, m, passes)) return rc;
+    return launch_all(passes, st);
+}
+
+int lf_pyramid_refresh(int levels, const lf_intrinsics *K, const LfImageLevels &m, hipStream_t st) {
+    std::vector<Pass> passes;
+    if (int rc = plan_refresh(levels, K, m, passes)) return rc;
+    return launch_all(passes, st);
+}
+
+size_t lf_pass_bytes(void) { return sizeof(Pass); }
+
+int lf_pyramid_upload_plan(const uint16_t *depth_u16, const unsigned char *rgb, float depth_scale, int levels,
+                           const lf_intrinsics *K, const LfImageLevels &m, std::vector<unsigned char> &out) {
+    std::vector<Pass> passes;
+    if (int rc = plan_upload(depth_u16, rgb, depth_scale, levels, K, m, passes)) return rc;
+    append_bytes(passes, out);
+    return LF_OK;
+}
+
+int lf_pyramid_refresh_plan(int levels, const lf_intrinsics *K, const LfImageLevels &m, std::vector<unsigned char> &out) {
+    std::vector<Pass> passes;
+    if (int rc = plan_refresh(levels, K, m, passes)) return rc;
+    append_bytes(passes, out);
+    return LF_OK;
+}
+
+int lf_pass_workgroups(const void *pass) { return static_cast<const Pass *>(pass)->total_wg; }
+
+void lf_pass_skip(void *pass) { *static_cast<Pass *>(pass) = Pass{}; }
+
+int lf_pass_batch(const void *passes, int n, int max_wg, hipStream_t st) {
+    if (!passes || n <= 0 || n > 65535 || max_wg < 0) return lf_set_error(LF_ERR_ARG, "bad pass batch of %d", n);
+    if (max_wg == 0) return LF_OK;
+    k_frame_pass_batch<<<dim3(max_wg, n), 256, 0, st>>>(static_cast<const Pass *>(passes));
+    LF_LAUNCH_CHECK();
     return LF_OK;
 }
 

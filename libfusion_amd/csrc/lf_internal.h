@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "libfusion_amd.h"
 
 // lf_update_weighted / lf_update with events recorded immediately before and
@@ -43,6 +45,20 @@ int lf_pyramid_upload(const uint16_t *depth_u16, const unsigned char *rgb, float
 // DeviceImage::resize_device_map (device_image.cpp:188-199): vertex pyramid of
 // vmap[0] and the normals of every level, in (levels + 1) / 2 launches.
 int lf_pyramid_refresh(int levels, const lf_intrinsics *K, const LfImageLevels &m, hipStream_t st);
+
+// The same passes planned instead of launched, for a batch of images: each
+// pass is an lf_pass_bytes() device descriptor appended to out, and pass p of
+// n images runs as one launch (lf_pass_batch over n consecutive descriptors in
+// device memory; max_wg = the largest lf_pass_workgroups of them).  An image
+// with fewer passes fills its place with lf_pass_skip.
+size_t lf_pass_bytes(void);
+int lf_pyramid_upload_plan(const uint16_t *depth_u16, const unsigned char *rgb, float depth_scale, int levels,
+                           const lf_intrinsics *K, const LfImageLevels &m, std::vector<unsigned char> &out);
+int lf_pyramid_refresh_plan(int levels, const lf_intrinsics *K, const LfImageLevels &m,
+                            std::vector<unsigned char> &out);
+int lf_pass_workgroups(const void *pass);
+void lf_pass_skip(void *pass);
+int lf_pass_batch(const void *passes, int n, int max_wg, hipStream_t st);
 
 // One raycast of a batch: a map, its target vertex map (and colour, or null),
 // the float T_wc / T_cw pose and the image size.

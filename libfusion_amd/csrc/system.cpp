@@ -203,6 +203,17 @@ int DeviceImage::resize_device_map() {
     return lf_pyramid_refresh(levels(), K.data(), levels_view(), stream_);
 }
 
+int DeviceImage::plan_upload(const uint16_t *depth_u16, const unsigned char *rgb, float depth_scale,
+                             std::vector<unsigned char> &out) {
+    std::vector<lf_intrinsics> K(K_.begin(), K_.end());
+    return lf_pyramid_upload_plan(depth_u16, rgb, depth_scale, levels(), K.data(), levels_view(), out);
+}
+
+int DeviceImage::plan_refresh(std::vector<unsigned char> &out) {
+    std::vector<lf_intrinsics> K(K_.begin(), K_.end());
+    return lf_pyramid_refresh_plan(levels(), K.data(), levels_view(), out);
+}
+
 const GpuImage &DeviceImage::get_rendered_image() {
     lf_render_scene(vmap_[0].ptr<float>(), nmap_[0].ptr<float>(), nullptr, K_[0].width, K_[0].height,
                     rendered_.ptr<unsigned char>(), stream_);
@@ -465,8 +476,12 @@ int DenseOdometry::trackFrame(const uint16_t *depth, const unsigned char *rgb, f
 }
 
 int DenseOdometry::planFrame(const uint16_t *depth, const unsigned char *rgb, float depth_scale,
-                             const SE3d &initial_pose, std::vector<unsigned char> &jobs, std::vector<int> &grids) {
-    LF_RC(curr_->upload(depth, rgb, depth_scale));
+                             const SE3d &initial_pose, std::vector<unsigned char> &jobs, std::vector<int> &grids,
+                             std::vector<unsigned char> *upload) {
+    if (upload)
+        LF_RC(curr_->plan_upload(depth, rgb, depth_scale, *upload));
+    else
+        LF_RC(curr_->upload(depth, rgb, depth_scale));
     if (!initialized_) {
         double T[12];
         initial_pose.matrix3x4(T);
@@ -567,9 +582,9 @@ int System::enqueue_before_raycast(const uint16_t *depth, const unsigned char *r
 }
 
 int System::plan_tracking(const uint16_t *depth, const unsigned char *rgb, std::vector<unsigned char> &jobs,
-                          std::vector<int> &grids) {
+                          std::vector<int> &grids, std::vector<unsigned char> *upload) {
     DenseOdometry &od = *odometry_;
-    if (!od.trackingLost) LF_RC(od.planFrame(depth, rgb, cfg_.depth_scale, initial_pose_, jobs, grids));
+    if (!od.trackingLost) LF_RC(od.planFrame(depth, rgb, cfg_.depth_scale, initial_pose_, jobs, grids, upload));
     return LF_OK;
 }
 
@@ -584,6 +599,10 @@ float *System::raycast_vmap() { return odometry_->get_reference_image().get_vmap
 const float *System::raycast_pose24() { return odometry_->pose24(); }
 
 int System::enqueue_after_raycast() { return odometry_->get_reference_image().resize_device_map(); }
+
+int System::plan_after_raycast(std::vector<unsigned char> &passes) {
+    return odometry_->get_reference_image().plan_refresh(passes);
+}
 
 // The frame's device work (trackFrame, update, raycast, re-pyramid) without events.
 int System::enqueue_frame(const uint16_t *depth, const unsigned char *rgb) {
@@ -600,11 +619,11 @@ Batch::Batch(const lf_system_config *cfgs, int n) {
         throw std::bad_alloc();
     };
     for (int i = 0; i < n; ++i) sys_.emplace_back(new System(cfgs[i]));
-    if (hipStreamCreate(&stream_) != hipSuccess) fail();
+    if (hipStreamCreate(&stream_) != hipSuccess || hipStreamCreate(&aux_) != hipSuccess) fail();
     ready_.assign(n, nullptr);
     for (hipEvent_t &e : ready_)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) fail();
-    if (hipEventCreateWithFlags(&cast_, hipEventDisableTiming) != hipSuccess ||
+    if (hipEventCreateWithFlags(&uploaded_, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&tracked_, hipEventDisableTiming) != hipSuccess)
         fail();
     for (int k = 0; k < 2; ++k) {
@@ -619,20 +638,24 @@ Batch::Batch(const lf_system_config *cfgs, int n) {
 Batch::~Batch() { release(); }
 
 void Batch::release() {
+    if (aux_) (void)hipStreamSynchronize(aux_);
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (auto &s : sys_)
         if (s) (void)hipStreamSynchronize(s->stream());
     for (hipEvent_t e : ready_)
         if (e) (void)hipEventDestroy(e);
     ready_.clear();
-    if (cast_) (void)hipEventDestroy(cast_);
-    cast_ = nullptr;
+    if (uploaded_) (void)hipEventDestroy(uploaded_);
+    uploaded_ = nullptr;
     if (tracked_) (void)hipEventDestroy(tracked_);
     tracked_ = nullptr;
     for (int k = 0; k < 2; ++k) {
         if (track_host_[k]) (void)hipHostFree(track_host_[k]);
         if (track_dev_[k]) (void)hipFree(track_dev_[k]);
         track_host_[k] = track_dev_[k] = nullptr;
+        if (pass_host_[k]) (void)hipHostFree(pass_host_[k]);
+        if (pass_dev_[k]) (void)hipFree(pass_dev_[k]);
+        pass_host_[k] = pass_dev_[k] = nullptr;
         if (jobs_host_[k]) (void)hipHostFree(jobs_host_[k]);
         if (jobs_dev_[k]) (void)hipFree(jobs_dev_[k]);
         if (jobs_read_[k]) (void)hipEventDestroy(jobs_read_[k]);
@@ -641,6 +664,8 @@ void Batch::release() {
     }
     if (stream_) (void)hipStreamDestroy(stream_);
     stream_ = nullptr;
+    if (aux_) (void)hipStreamDestroy(aux_);
+    aux_ = nullptr;
     sys_.clear();
 }
 
@@ -661,15 +686,38 @@ int Batch::reserve_track(size_t jobs) {
     return LF_OK;
 }
 
+// Room for `bytes` of frame passes per buffer.  The pass tables are read on
+// the upload stream and on the batch stream: growing waits for both.
+int Batch::reserve_pass(size_t bytes) {
+    if (bytes <= pass_cap_) return LF_OK;
+    LF_HIP_CHECK(hipStreamSynchronize(aux_));
+    LF_HIP_CHECK(hipStreamSynchronize(stream_));
+    const size_t cap = std::max(bytes, 2 * pass_cap_);
+    for (int k = 0; k < 2; ++k) {
+        if (pass_host_[k]) LF_HIP_CHECK(hipHostFree(pass_host_[k]));
+        if (pass_dev_[k]) LF_HIP_CHECK(hipFree(pass_dev_[k]));
+        pass_host_[k] = pass_dev_[k] = nullptr;
+        LF_HIP_CHECK(hipHostMalloc(&pass_host_[k], cap, hipHostMallocDefault));
+        LF_HIP_CHECK(hipMalloc(&pass_dev_[k], cap));
+    }
+    pass_cap_ = cap;
+    return LF_OK;
+}
+
 int Batch::process(const uint16_t *const *depth, const unsigned char *const *rgb) {
     const int n = size();
-    // uploads and pyramids on the systems' streams; their steps planned
-    std::vector<std::vector<unsigned char>> jobs(n);
+    const size_t PB = lf_pass_bytes();
+    // every system's upload and re-pyramid planned as frame passes, its
+    // Gauss-Newton steps as jobs
+    std::vector<std::vector<unsigned char>> jobs(n), up(n), after(n);
     std::vector<std::vector<int>> grids(n);
-    size_t steps = 0;
+    size_t steps = 0, n_up = 0, n_after = 0;
     for (int i = 0; i < n; ++i) {
-        LF_RC(sys_[i]->plan_tracking(depth[i], rgb[i], jobs[i], grids[i]));
+        LF_RC(sys_[i]->plan_tracking(depth[i], rgb[i], jobs[i], grids[i], &up[i]));
+        LF_RC(sys_[i]->plan_after_raycast(after[i]));
         steps = std::max(steps, grids[i].size());
+        n_up = std::max(n_up, up[i].size() / PB);
+        n_after = std::max(n_after, after[i].size() / PB);
         LF_HIP_CHECK(hipEventRecord(ready_[i], sys_[i]->stream()));
         LF_HIP_CHECK(hipStreamWaitEvent(stream_, ready_[i], 0));
     }
@@ -677,6 +725,35 @@ int Batch::process(const uint16_t *const *depth, const unsigned char *const *rgb
     // are rewritten
     const int k = parity_;
     if (jobs_used_[k]) LF_HIP_CHECK(hipEventSynchronize(jobs_read_[k]));
+    // frame passes: pass p of every system in one launch
+    const size_t n_pass = n_up + n_after;
+    std::vector<int> wmax(n_pass, 0);
+    unsigned char *pass_dev = nullptr;
+    if (n_pass) {
+        LF_RC(reserve_pass(n_pass * n * PB));
+        unsigned char *host = static_cast<unsigned char *>(pass_host_[k]);
+        pass_dev = static_cast<unsigned char *>(pass_dev_[k]);
+        for (size_t p = 0; p < n_pass; ++p)
+            for (int i = 0; i < n; ++i) {
+                const std::vector<unsigned char> &src = p < n_up ? up[i] : after[i];
+                const size_t q = p < n_up ? p : p - n_up;
+                unsigned char *dst = host + (p * n + i) * PB;
+                if (q < src.size() / PB) {
+                    std::memcpy(dst, src.data() + q * PB, PB);
+                    wmax[p] = std::max(wmax[p], lf_pass_workgroups(dst));
+                } else {
+                    lf_pass_skip(dst);
+                }
+            }
+        // the uploads rewrite the images that the previous step's tracking
+        // read; the previous step's fusion, raycast and re-pyramid use the
+        // other image of each system, so the uploads run beside them
+        if (tracked_once_) LF_HIP_CHECK(hipStreamWaitEvent(aux_, tracked_, 0));
+        LF_HIP_CHECK(hipMemcpyAsync(pass_dev, host, n_pass * n * PB, hipMemcpyHostToDevice, aux_));
+        for (size_t p = 0; p < n_up; ++p) LF_RC(lf_pass_batch(pass_dev + p * n * PB, n, wmax[p], aux_));
+        LF_HIP_CHECK(hipEventRecord(uploaded_, aux_));
+        LF_HIP_CHECK(hipStreamWaitEvent(stream_, uploaded_, 0));
+    }
     if (steps) {
         LF_RC(reserve_track(steps * n));
         const size_t J = lf_track_job_bytes();
@@ -698,6 +775,7 @@ int Batch::process(const uint16_t *const *depth, const unsigned char *const *rgb
     }
     // fusion on the systems' streams once their poses are final
     LF_HIP_CHECK(hipEventRecord(tracked_, stream_));
+    tracked_once_ = true;
     for (int i = 0; i < n; ++i) {
         LF_HIP_CHECK(hipStreamWaitEvent(sys_[i]->stream(), tracked_, 0));
         LF_RC(sys_[i]->enqueue_fusion());
@@ -712,19 +790,17 @@ int Batch::process(const uint16_t *const *depth, const unsigned char *const *rgb
                              m.intrinsics().height};
     }
     LF_RC(lf_raycast_batch_ev(items.data(), n, jobs_host_[k], jobs_dev_[k], 1, stream_));
+    // re-pyramids of the raycast maps
+    for (size_t p = n_up; p < n_pass; ++p) LF_RC(lf_pass_batch(pass_dev + p * n * PB, n, wmax[p], stream_));
     LF_HIP_CHECK(hipEventRecord(jobs_read_[k], stream_));
     jobs_used_[k] = true;
     parity_ ^= 1;
-    LF_HIP_CHECK(hipEventRecord(cast_, stream_));
-    for (int i = 0; i < n; ++i) {
-        LF_HIP_CHECK(hipStreamWaitEvent(sys_[i]->stream(), cast_, 0));
-        LF_RC(sys_[i]->enqueue_after_raycast());
-    }
     return LF_OK;
 }
 
 int Batch::synchronize() {
     for (auto &s : sys_) LF_HIP_CHECK(hipStreamSynchronize(s->stream()));
+    LF_HIP_CHECK(hipStreamSynchronize(aux_));
     LF_HIP_CHECK(hipStreamSynchronize(stream_));
     return LF_OK;
 }

@@ -907,12 +907,14 @@ def test_systems_stepped_together_equal_systems_run_alone():
 
 
 def test_batch_with_one_raycast_launch_equals_systems_run_alone():
-    """lf_batch: three systems of different image sizes and voxel sizes whose
-    raycasts share one batched launch per stage.  Each ends with exactly the
-    pose, tracking result and volume counters of the same system run alone."""
-    specs = [(Scene(1, 1), 2, 0.01), (Scene(1, 7), 2, 0.008), (Scene(1, 3), 4, 0.01)]
+    """lf_batch: three systems of different image sizes, voxel sizes and
+    pyramid depths whose pyramid passes, tracking steps and raycasts share
+    batched launches.  Each ends with exactly the pose, tracking result and
+    volume counters of the same system run alone."""
+    specs = [(Scene(1, 1), 2, 0.01, 3, (10, 5, 3)), (Scene(1, 7), 2, 0.008, 3, (10, 5, 3)),
+             (Scene(1, 3), 4, 0.01, 2, (6, 4))]
     runs = []
-    for sc, div, vs in specs:
+    for sc, div, vs, levels, its in specs:
         K = capi.intrinsics(sc.width // div, sc.height // div, sc.fx / div, sc.fy / div,
                             (sc.cx + 0.5) / div - 0.5, (sc.cy + 0.5) / div - 0.5)
         p = capi.map_params(2 ** 18, 1 << 17, vs, 3 * vs)
@@ -920,17 +922,17 @@ def test_batch_with_one_raycast_launch_equals_systems_run_alone():
         for f in range(8):
             d16, rgb = sc.render(sc.pose(f), K.width, K.height, K.fx, K.fy, K.cx, K.cy, noise=True, frame_id=f)
             fs.append((dev(d16), dev(rgb)))
-        runs.append((K, p, sc.depth_scale, fs))
+        runs.append((K, p, sc.depth_scale, fs, levels, its))
     torch.cuda.synchronize()
     alone = []
-    for K, p, ds, fs in runs:
-        s = capi.System(K, p, 3, (10, 5, 3), ds)
+    for K, p, ds, fs, levels, its in runs:
+        s = capi.System(K, p, levels, its, ds)
         for d, c in fs:
             s.process(d, c)
         s.synchronize()
         alone.append((s.pose(), s.tracking(), s.counters()))
         del s
-    batch = capi.Batch([capi.system_config(K, p, 3, (10, 5, 3), ds) for K, p, ds, _ in runs])
+    batch = capi.Batch([capi.system_config(K, p, levels, its, ds) for K, p, ds, _, levels, its in runs])
     for f in range(8):
         batch.process([r[3][f][0] for r in runs], [r[3][f][1] for r in runs])
     batch.synchronize()

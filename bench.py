@@ -57,13 +57,13 @@ def cpu_baseline(sc, params, K, frames_d, frames_c, n_frames):
             "sample": "C1 frames 0-%d (640x480, 5 mm), FP32 oracle System, %d OpenMP threads" % (n_frames - 1, threads)}
 
 
-def batch_throughput(n, warmup, steps, single_fps):
+def batch_throughput(n, warmup, steps, single_fps, precision=0, separate=True):
     """Aggregate frames/s of n independent C1 systems stepped together: system i
     is its own seeded scene and orbit (seed 100 + i) with its own map.  Two
-    ways: lf_batch (step s of every system's tracking in one launch pair, each
-    map fused on its own stream, one batched z-range + raycast launch per stage
-    for all maps), and n separate lf_system objects on their own streams with
-    no shared launches."""
+    ways: lf_batch (pyramid passes and Gauss-Newton steps of all systems in one
+    launch each, each map fused on its own stream, one batched z-range +
+    raycast launch per stage for all maps), and (separate) n lf_system objects
+    on their own streams with no shared launches."""
     import torch
 
     from libfusion_amd import capi, configs
@@ -76,6 +76,7 @@ def batch_throughput(n, warmup, steps, single_fps):
         frames.append((torch.from_numpy(d).cuda(), torch.from_numpy(c).cuda()))
         K = capi.intrinsics(sc.width, sc.height, sc.fx, sc.fy, sc.cx, sc.cy)
         _, _, params = configs.c1()
+        params.precision = precision
         cfgs.append((K, params, sc.depth_scale))
     torch.cuda.synchronize()
 
@@ -93,6 +94,17 @@ def batch_throughput(n, warmup, steps, single_fps):
     dt_b = timed(lambda f: batch.process([d[f] for d, _ in frames], [c[f] for _, c in frames]), batch.synchronize)
     batch.close()
     del batch
+    v_b = n * steps / dt_b
+    name = "plain FP32" if precision == 0 else "mixed-precision"
+    out = {"systems": n, "value": round(v_b, 1), "unit": "frames/s (all systems)",
+           "ms_per_step": round(1000.0 * dt_b / steps, 4), "vs_single_system": round(v_b / single_fps, 3),
+           "steps": steps, "warmup": warmup,
+           "workload": "%d independent %s C1 systems (seeds 100-%d: own scene, orbit and map), one frame "
+                       "of each per step" % (n, name, 99 + n),
+           "launch": "lf_batch: pyramid passes and Gauss-Newton steps of all systems in one launch each, fusion "
+                     "per system on its own stream, one batched z-range + raycast launch per stage for all maps"}
+    if not separate:
+        return out
     systems = [capi.System(K, p, configs.C1_LEVELS, configs.C1_ITERATIONS, ds) for K, p, ds in cfgs]
 
     def step_systems(f):
@@ -106,16 +118,31 @@ def batch_throughput(n, warmup, steps, single_fps):
     dt_s = timed(step_systems, sync_systems)
     for s in systems:
         s.close()
-    v_b, v_s = n * steps / dt_b, n * steps / dt_s
-    return {"systems": n, "value": round(v_b, 1), "unit": "frames/s (all systems)",
-            "ms_per_step": round(1000.0 * dt_b / steps, 4), "vs_single_system": round(v_b / single_fps, 3),
-            "steps": steps, "warmup": warmup,
-            "workload": "%d independent plain FP32 C1 systems (seeds 100-%d: own scene, orbit and map), one frame "
-                        "of each per step" % (n, 99 + n),
-            "launch": "lf_batch: each Gauss-Newton step of all systems in one launch pair, fusion per system on "
-                      "its own stream, one batched z-range + raycast launch per stage for all maps",
-            "separate_systems": {"value": round(v_s, 1), "ms_per_step": round(1000.0 * dt_s / steps, 4),
-                                 "launch": "separate lf_system objects on their own streams, no shared launches"}}
+    v_s = n * steps / dt_s
+    out["separate_systems"] = {"value": round(v_s, 1), "ms_per_step": round(1000.0 * dt_s / steps, 4),
+                               "launch": "separate lf_system objects on their own streams, no shared launches"}
+    return out
+
+
+def timed_system(K, params, dev_d, dev_c, warmup, n_total, depth_scale):
+    """frames/s of one System over frames warmup..n_total-1 (after warmup frames)."""
+    import torch
+
+    from libfusion_amd import capi, configs
+
+    s = capi.System(K, params, configs.C1_LEVELS, configs.C1_ITERATIONS, depth_scale)
+    for i in range(warmup):
+        s.process(dev_d[i], dev_c[i])
+    s.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(warmup, n_total):
+        s.process(dev_d[i], dev_c[i])
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    pose = s.pose()
+    s.close()
+    return (n_total - warmup) / dt, dt, pose
 
 
 def main():
@@ -129,6 +156,7 @@ def main():
     ap.add_argument("--batch", type=int, default=16,
                     help="also time this many independent systems stepped together (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=50)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision passes")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device-to-device (a plain copy kernel of known size, "
                          "to scale rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -234,11 +262,33 @@ def main():
                                       "avg_launch_ms": round(stages["raycast_kernel"]["avg_ms"], 5)}
 
     batched = None
+    del sysm
+    torch.cuda.empty_cache()
     if args.batch > 0:
-        del sysm
-        torch.cuda.empty_cache()
         batched = batch_throughput(args.batch, args.warmup, args.batch_steps, fps)
         log("batched: %d systems, %.1f frames/s aggregate" % (args.batch, batched["value"]))
+    # mixed precision (FP16 TSDF + 8-bit weight codes, FP16 maps): the same
+    # frames through one system, then batched; frames/s against this run's
+    # plain FP32 single system
+    mixed = None
+    if not args.no_mixed:
+        pm = configs.c1()[2]
+        pm.precision = capi.PRECISION_MIXED
+        m_fps, m_dt, m_pose = timed_system(K, pm, dev_d, dev_c, args.warmup, n_total, sc.depth_scale)
+        log("mixed precision: %.1f frames/s" % m_fps)
+        mixed = {"value": round(m_fps, 2), "unit": "frames/s", "ms_per_step": round(1000.0 * m_dt / args.steps, 4),
+                 "vs_plain_fp32": round(m_fps / fps, 3),
+                 "final_pose_t": [round(float(x), 7) for x in m_pose[:3, 3]],
+                 "storage": "FP16 TSDF + 8-bit weight codes + 3 x u8 colour (6 B/voxel); FP16 vertex / normal / "
+                            "intensity / gradient maps; FP32 pairwise normal-equation sums; FP64 solve and SE(3)"}
+        if args.batch > 0:
+            torch.cuda.empty_cache()
+            mb = batch_throughput(args.batch, args.warmup, args.batch_steps, fps, capi.PRECISION_MIXED,
+                                  separate=False)
+            mb["vs_plain_fp32_single_system"] = round(mb["value"] / fps, 3)
+            mixed["batched"] = mb
+            log("mixed batched: %d systems, %.1f frames/s aggregate (%.2fx plain FP32)"
+                % (args.batch, mb["value"], mb["value"] / fps))
 
     cpu = None
     if not args.no_cpu:
@@ -268,6 +318,7 @@ def main():
         "stage_rooflines": stage_rooflines,
         "stages_ms": {k: round(v["avg_ms"], 4) for k, v in stages.items()},
         "batched": batched,
+        "mixed_precision": mixed,
         "allocated_blocks": int(min(c1[capi.CNT_ALLOCATED], params.num_blocks)),
         "overflow": int(c1[capi.CNT_OVERFLOW]),
         "final_pose_t": [round(float(x), 7) for x in pose[:3, 3]],

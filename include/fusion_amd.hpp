@@ -68,7 +68,10 @@ private:
 // DeviceImage (device_image.h:18-80): one RGB-D frame and its pyramids.
 class DeviceImage {
 public:
-    DeviceImage(const IntrinsicMatrix &K, int num_pyr, hipStream_t stream);
+    // half_maps: intensity, gradients, vertices and normals stored as FP16
+    // (mixed precision); the set_* builders below take FP32 images only.
+    DeviceImage(const IntrinsicMatrix &K, int num_pyr, hipStream_t stream, bool half_maps = false);
+    bool half_maps() const { return half_maps_; }
     // DeviceImage::upload (device_image.cpp:146-181) from device-resident input.
     int upload(const uint16_t *depth_u16, const unsigned char *rgb, float depth_scale);
     // DeviceImage::resize_device_map (device_image.cpp:188-199).
@@ -110,6 +113,7 @@ private:
     int intensity_pyramid();
     GpuImage depth_raw_, image_, samples_, rendered_;
     std::vector<GpuImage> depth_, intensity_, dx_, dy_, vmap_, nmap_;
+    bool half_maps_ = false;
 };
 
 // DenseMapping (voxel_hashing.h:12-51).
@@ -214,7 +218,8 @@ private:
 // DenseOdometry (rgbd_odometry.h:13-41).
 class DenseOdometry {
 public:
-    DenseOdometry(const IntrinsicMatrix &base, int num_pyr, std::vector<int> iterations, hipStream_t stream);
+    DenseOdometry(const IntrinsicMatrix &base, int num_pyr, std::vector<int> iterations, hipStream_t stream,
+                  bool half_maps = false);
     // trackFrame (rgbd_odometry.cpp:18-48) on device-resident input.
     int trackFrame(const uint16_t *depth, const unsigned char *rgb, float depth_scale, const SE3d &initial_pose);
     // trackFrame with the Gauss-Newton steps planned into jobs / grids instead
@@ -344,6 +349,7 @@ private:
     size_t track_cap_ = 0;  // jobs per buffer
     hipEvent_t tracked_ = nullptr;
     bool tracked_once_ = false;
+    bool half_maps_ = false;  // the systems are mixed precision
     int reserve_track(size_t jobs);
     // frame passes: [upload pass][system] then [re-pyramid pass][system]
     void *pass_host_[2] = {nullptr, nullptr}, *pass_dev_[2] = {nullptr, nullptr};

@@ -77,7 +77,19 @@ typedef struct lf_map_params {
     float zmin_update, zmax_update;
     float voxel_size;         /* metres */
     float truncation;         /* metres */
+    int precision;            /* LF_PRECISION_FP32 or LF_PRECISION_MIXED */
 } lf_map_params;
+
+/* Storage precision of a map and of the images and tracking of a System.
+ *   FP32  the reference's method: int16 TSDF, float32 weights, float32 vertex /
+ *         normal / intensity / gradient maps.
+ *   MIXED FP16 TSDF with 8-bit weight codes (lf_weight_code); FP16 vertex,
+ *         normal, intensity and gradient maps; the same FP32 normal-equation
+ *         sums (pairwise: per-lane, wavefront tree, fixed-order final sum) and
+ *         FP64 solve and pose updates.  Results are stated against the FP64
+ *         method, not bit for bit. */
+#define LF_PRECISION_FP32 0
+#define LF_PRECISION_MIXED 1
 
 /* Device storage of one voxel-hash map — the counterpart of fusion::MapStorage
  * (map_struct.h:69-77).  Layout (all arrays in HBM):
@@ -85,9 +97,13 @@ typedef struct lf_map_params {
  *   slot_block[num_entries]  pool block index of the slot, -1 = none
  *   slot_flag[num_entries]   1 if the slot's block was touched by the current update
  *   block_pos[4*num_blocks]  {x, y, z, slot} of every pool block in allocation order
- *   sdf[512*num_blocks]      int16 TSDF, value = sdf/32767 truncation units;
- *                            -32768 marks a voxel whose weight is 0
- *   weight[512*num_blocks]   float32 fusion weight, clamped to 255
+ *   sdf[512*num_blocks]      FP32 precision: int16 TSDF, value = sdf/32767
+ *                            truncation units; -32768 marks a voxel whose weight is 0.
+ *                            Mixed: FP16 TSDF in truncation units; 0xFFFF (a NaN)
+ *                            marks a voxel whose weight is 0
+ *   weight[512*num_blocks]   FP32 precision: float32 fusion weight, clamped to 255.
+ *                            Mixed: 512*num_blocks bytes of 8-bit weight codes
+ *                            (lf_weight_value), the weight clamped to 255 first
  *   rgb[3*512*num_blocks]    uint8 colour, planar per block: R[512] G[512] B[512]
  *   visible[num_blocks]      pool block indices visible in the last update
  *   counters[LF_NUM_COUNTERS]
@@ -134,6 +150,13 @@ typedef struct lf_map_storage {
 int lf_occupancy_bit(int bx, int by, int bz);
 /* Directory cell of block (bx, by, bz), or -1 outside the directory. */
 int lf_dir_cell(int bx, int by, int bz);
+/* 8-bit weight code of a mixed-precision map: an unsigned float with 4
+ * exponent bits (bias 7) and 4 mantissa bits; code 0 is weight 0, codes
+ * 1..15 are m * 2^-10, code e<<4|m (e >= 1) is (1 + m/16) * 2^(e-7), up to 496.
+ * lf_weight_code rounds a weight >= 0 to the nearest code (ties to the even
+ * code); weights above 496 give 0xFF. */
+int lf_weight_code(float w);
+float lf_weight_value(int code);
 
 /* Library version string and the gfx target the kernels were built for. */
 const char *lf_version(void);
@@ -286,7 +309,9 @@ int lf_warp_image(const unsigned char *src, const float *vmap_dst, int cols, int
 /* ------------------------------------------- frame pyramid (DeviceImage) */
 
 /* DeviceImage (device_image.h:18-88): one RGB-D frame's device pyramid.  Every
- * level is exactly half of the one above (even sizes down to the last level). */
+ * level is exactly half of the one above (even sizes down to the last level).
+ * The images of a mixed-precision System store INTENSITY, DX and DY as FP16
+ * and VMAP and NMAP as 4 x FP16; every other plane is as listed. */
 typedef struct lf_image lf_image;
 
 enum {

@@ -18,6 +18,8 @@ CNT_ALLOCATED, CNT_VISIBLE, CNT_OVERFLOW, CNT_TRIANGLES = 0, 1, 2, 3
 CNT_VIS_SUM_LO, CNT_VIS_SUM_HI, CNT_UPDATES, CNT_RENDER_BLOCKS = 4, 5, 6, 7
 OVF_TABLE, OVF_POOL, OVF_TRIANGLES = 1, 2, 4
 SDF_UNSEEN = -32768  # sdf code of a voxel with weight 0
+PRECISION_FP32 = 0   # int16 TSDF, float32 weights and maps
+PRECISION_MIXED = 1  # FP16 TSDF, 8-bit weight codes, FP16 maps
 OCC_WORDS = 4096  # occupancy words of a map (2^17 bits)
 DIR_DIM = 256  # block directory: DIR_DIM^3 cells
 DIR_CELLS = DIR_DIM ** 3
@@ -36,7 +38,8 @@ class LfMapParams(ctypes.Structure):
                 ("num_blocks", ctypes.c_int), ("max_triangles", ctypes.c_int),
                 ("zmin_raycast", ctypes.c_float), ("zmax_raycast", ctypes.c_float),
                 ("zmin_update", ctypes.c_float), ("zmax_update", ctypes.c_float),
-                ("voxel_size", ctypes.c_float), ("truncation", ctypes.c_float)]
+                ("voxel_size", ctypes.c_float), ("truncation", ctypes.c_float),
+                ("precision", ctypes.c_int)]
 
 
 class LfMapStorage(ctypes.Structure):
@@ -68,6 +71,8 @@ SIGNATURES = [
     ("lf_compute_hash", I, [I, I, I, I]),
     ("lf_occupancy_bit", I, [I, I, I]),
     ("lf_dir_cell", I, [I, I, I]),
+    ("lf_weight_code", I, [F]),
+    ("lf_weight_value", F, [I]),
     ("lf_mc_tables", None, [P, P]),
     ("lf_device_count", I, []),
     ("lf_last_error", ctypes.c_char_p, []),
@@ -191,10 +196,44 @@ def pyr_down(K):
 
 
 def map_params(num_buckets, num_blocks, voxel_size, truncation, excess_ratio=0.25, zmin=0.3, zmax=5.0,
-               max_triangles=20_000_000):
+               max_triangles=20_000_000, precision=PRECISION_FP32):
     return LfMapParams(int(num_buckets), int(num_buckets + int(num_buckets * excess_ratio)), int(num_blocks),
                        int(max_triangles), float(zmin), float(zmax), float(zmin), float(zmax),
-                       float(voxel_size), float(truncation))
+                       float(voxel_size), float(truncation), int(precision))
+
+
+def weight_values():
+    """The 256 weights of the 8-bit codes of a mixed-precision map."""
+    return np.array([load().lf_weight_value(c) for c in range(256)], np.float32)
+
+
+def read_tsdf(params, storage):
+    """(blocks[n,3], tsdf float32[n,512] in truncation units with NaN where the
+    weight is 0, weight float32[n,512]) of a map's allocated blocks in pool
+    order, for either precision."""
+    torch = _torch()
+    c = np.zeros(16, np.uint32)
+    check(load().lf_map_read_counters(ctypes.byref(storage), c.ctypes.data_as(P), None))
+    n = min(int(c[CNT_ALLOCATED]), params.num_blocks)
+    pos = torch.empty(4 * max(n, 1), dtype=torch.int32, device="cuda")
+    sdf = torch.empty(512 * max(n, 1), dtype=torch.int16, device="cuda")
+    mixed = params.precision == PRECISION_MIXED
+    w = torch.empty(512 * max(n, 1), dtype=torch.uint8 if mixed else torch.float32, device="cuda")
+    if n:
+        copy_device(pos, storage.block_pos, 16 * n)
+        copy_device(sdf, storage.sdf, 1024 * n)
+        copy_device(w, storage.weight, 512 * n * (1 if mixed else 4))
+    pos = pos.cpu().numpy()[: 4 * n].reshape(n, 4)[:, :3]
+    sdf = sdf.cpu().numpy()[: 512 * n].reshape(n, 512)
+    w = w.cpu().numpy()[: 512 * n].reshape(n, 512)
+    if mixed:
+        weight = weight_values()[w]
+        tsdf = sdf.view(np.float16).astype(np.float32)
+    else:
+        weight = w.astype(np.float32)
+        tsdf = np.where(sdf == SDF_UNSEEN, np.nan, sdf.astype(np.float32) / np.float32(32767)).astype(np.float32)
+    tsdf[weight == 0] = np.nan
+    return pos, tsdf, weight
 
 
 def mc_tables():
@@ -376,7 +415,8 @@ class Map:
             "slot_flag": torch.empty(E, dtype=torch.uint8, device=device),
             "block_pos": torch.empty(B * 4, dtype=torch.int32, device=device),
             "sdf": torch.empty(B * 512, dtype=torch.int16, device=device),
-            "weight": torch.empty(B * 512, dtype=torch.float32, device=device),
+            "weight": torch.empty(B * 512, dtype=torch.uint8 if params.precision == PRECISION_MIXED
+                                  else torch.float32, device=device),
             "rgb": torch.empty(B * 1536, dtype=torch.uint8, device=device),
             "visible": torch.empty(B, dtype=torch.int32, device=device),
             "counters": torch.zeros(16, dtype=torch.int32, device=device),

@@ -70,8 +70,13 @@ static int check_params(const lf_map_params *p) {
         return lf_set_error(LF_ERR_ARG, "pool of %d blocks exceeds 32-bit voxel indexing", p->num_blocks);
     if (!(p->voxel_size > 0) || !(p->truncation > 0) || !(p->zmin_raycast > 0) || !(p->zmin_update >= 0))
         return lf_set_error(LF_ERR_ARG, "bad voxel size / truncation / z ranges");
+    if (p->precision != LF_PRECISION_FP32 && p->precision != LF_PRECISION_MIXED)
+        return lf_set_error(LF_ERR_ARG, "unknown map precision %d", p->precision);
     return LF_OK;
 }
+
+// Bytes of one voxel's weight: float32, or an 8-bit code in mixed precision.
+size_t lf_weight_bytes(const lf_map_params *p) { return p->precision == LF_PRECISION_MIXED ? 1 : sizeof(float); }
 
 // Every array of a map storage present and the tile grid non-empty; entry
 // points call this before any launch, so storage filled in by hand with an
@@ -104,7 +109,7 @@ extern "C" int lf_map_create(const lf_map_params *p, int width, int height, lf_m
     LF_HIP_CHECK(hipMalloc(&s->slot_flag, E));
     LF_HIP_CHECK(hipMalloc(&s->block_pos, B * 4 * sizeof(int)));
     LF_HIP_CHECK(hipMalloc(&s->sdf, B * LF_BLOCK3 * sizeof(short)));
-    LF_HIP_CHECK(hipMalloc(&s->weight, B * LF_BLOCK3 * sizeof(float)));
+    LF_HIP_CHECK(hipMalloc(&s->weight, B * LF_BLOCK3 * lf_weight_bytes(p)));
     LF_HIP_CHECK(hipMalloc(&s->rgb, B * LF_BLOCK3 * 3));
     LF_HIP_CHECK(hipMalloc(&s->visible, B * sizeof(int)));
     LF_HIP_CHECK(hipMalloc(&s->counters, LF_NUM_COUNTERS * sizeof(unsigned int)));
@@ -157,11 +162,14 @@ extern "C" int lf_map_reset(const lf_map_params *p, const lf_map_storage *s, voi
     k_zero_u32<<<LF_OCC_WORDS / 256, 256, 0, st>>>(s->occupancy, LF_OCC_WORDS);
     LF_LAUNCH_CHECK();
     const size_t B = (size_t)p->num_blocks;
-    // sdf (2 B, every voxel starts unseen), weight (4 B, 0), rgb (3 B, 0) per voxel,
-    // and the block directory (every cell -1); every array is a multiple of 512 bytes.
-    const unsigned int unseen2 = 0x80008000u;  // two int16 kSdfUnseen codes
+    // sdf (2 B, every voxel starts unseen), weight (4 B or 1 B, 0), rgb (3 B, 0)
+    // per voxel, and the block directory (every cell -1); every array is a
+    // multiple of 512 bytes.
+    const bool mixed = p->precision == LF_PRECISION_MIXED;
+    const unsigned int unseen2 = mixed ? 0xFFFFFFFFu   // two FP16 kHalfUnseen codes
+                                       : 0x80008000u;  // two int16 kSdfUnseen codes
     struct { void *ptr; size_t bytes; unsigned int v; } arrs[] = {
-        {s->sdf, B * 1024, unseen2}, {s->weight, B * 2048, 0u}, {s->rgb, B * 1536, 0u},
+        {s->sdf, B * 1024, unseen2}, {s->weight, B * LF_BLOCK3 * lf_weight_bytes(p), 0u}, {s->rgb, B * 1536, 0u},
         {s->block_dir, (size_t)LF_DIR_CELLS * sizeof(int), 0xFFFFFFFFu}};
     for (auto &a : arrs) {
         size_t n16 = a.bytes / 16;
@@ -192,6 +200,10 @@ int lf_rebuild_dir(const lf_map_params *p, const lf_map_storage *s, hipStream_t 
     LF_LAUNCH_CHECK();
     return LF_OK;
 }
+
+extern "C" int lf_weight_code(float w) { return (int)lf::weight_code(w); }
+
+extern "C" float lf_weight_value(int code) { return lf::weight_value((unsigned int)code & 0xFFu); }
 
 extern "C" int lf_map_read_counters(const lf_map_storage *s, unsigned int *out16, void *stream) {
     if (int rc = lf_check_storage(s)) return rc;

@@ -55,22 +55,25 @@ LF_DEV float resize_at(const float *in, int w, int x, int y) {
     const float *r0 = in + (size_t)(2 * y) * w + 2 * x, *r1 = r0 + w;
     return mean4(r0[0], r0[1], r1[0], r1[1]);
 }
-LF_DEV float4 resize4_at(const float4 *in, int w, int x, int y) {
-    const float4 *r0 = in + (size_t)(2 * y) * w + 2 * x, *r1 = r0 + w;
-    const float4 a = r0[0], b = r0[1], c = r1[0], d = r1[1];
+// V(x, y) returns the vertex of an in-range pixel of the full-size map.
+template <class V>
+LF_DEV float4 resize4_at(const V &in, int x, int y) {
+    const float4 a = in(2 * x, 2 * y), b = in(2 * x + 1, 2 * y), c = in(2 * x, 2 * y + 1), d = in(2 * x + 1, 2 * y + 1);
     return make_float4(mean4(a.x, b.x, c.x, d.x), mean4(a.y, b.y, c.y, d.y), mean4(a.z, b.z, c.z, d.z),
                        mean4(a.w, b.w, c.w, d.w));
 }
 
 // cv::cuda::pyrDown: [1 4 6 4 1]^2/256 Gaussian at even pixels, reflect-101.
-LF_DEV float pyr_down_at(const float *in, int w, int h, int x, int y) {
+// I(x, y) returns the source value of an in-range pixel.
+template <class I>
+LF_DEV float pyr_down_at(const I &in, int w, int h, int x, int y) {
     const int x0 = reflect101(2 * x - 2, w), x1 = reflect101(2 * x - 1, w), x2 = 2 * x, x3 = reflect101(2 * x + 1, w),
               x4 = reflect101(2 * x + 2, w);
     float rs[5];
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
-        const float *row = in + (size_t)reflect101(2 * y + j - 2, h) * w;
-        rs[j] = (row[x0] + row[x4]) + 4.0f * (row[x1] + row[x3]) + 6.0f * row[x2];
+        const int r = reflect101(2 * y + j - 2, h);
+        rs[j] = (in(x0, r) + in(x4, r)) + 4.0f * (in(x1, r) + in(x3, r)) + 6.0f * in(x2, r);
     }
     return ((rs[0] + rs[4]) + 4.0f * (rs[1] + rs[3]) + 6.0f * rs[2]) * (1.0f / 256.0f);
 }
@@ -81,15 +84,16 @@ LF_DEV float grey_of(unsigned char r, unsigned char g, unsigned char b) {
 }
 
 // computeDerivativeK (cuda_imgproc.cu:217-234); last row / column 0.
-LF_DEV void derivative_at(const float *I, int w, int h, int x, int y, float &dx, float &dy) {
+template <class I>
+LF_DEV void derivative_at(const I &in, int w, int h, int x, int y, float &dx, float &dy) {
     if (x >= w - 1 || y >= h - 1) {
         dx = 0.0f;
         dy = 0.0f;
         return;
     }
     const int x10 = max(x - 1, 0), x01 = min(x + 1, w), y10 = max(y - 1, 0), y01 = min(y + 1, h);
-    dx = (I[(size_t)y * w + x01] - I[(size_t)y * w + x10]) * 0.5f;
-    dy = (I[(size_t)y01 * w + x] - I[(size_t)y10 * w + x]) * 0.5f;
+    dx = (in(x01, y) - in(x10, y)) * 0.5f;
+    dy = (in(x, y01) - in(x, y10)) * 0.5f;
 }
 
 // backProjectDepthK (cuda_imgproc.cu:249-263).
@@ -135,6 +139,44 @@ struct VertexAt {
     const float4 *p; int w;
     LF_DEV float4 operator()(int x, int y) const { return p[(size_t)y * w + x]; }
 };
+// a plane of FP32 values, or of FP16 values (kHalf)
+template <bool kHalf>
+struct PlaneAt {
+    const void *p; int w;
+    LF_DEV float operator()(int x, int y) const {
+        const size_t i = (size_t)y * w + x;
+        if constexpr (kHalf) return half_bits_to_float(static_cast<const unsigned short *>(p)[i]);
+        else return static_cast<const float *>(p)[i];
+    }
+};
+// a map of float4, or of 4 x FP16 (kHalf)
+template <bool kHalf>
+struct Vec4At {
+    const void *p; int w;
+    LF_DEV float4 operator()(int x, int y) const {
+        const size_t i = (size_t)y * w + x;
+        if constexpr (kHalf) {
+            const uint2 u = static_cast<const uint2 *>(p)[i];
+            return make_float4(half_bits_to_float((unsigned short)u.x), half_bits_to_float((unsigned short)(u.x >> 16)),
+                               half_bits_to_float((unsigned short)u.y), half_bits_to_float((unsigned short)(u.y >> 16)));
+        } else {
+            return static_cast<const float4 *>(p)[i];
+        }
+    }
+};
+template <bool kHalf>
+LF_DEV void store1(void *p, size_t i, float v) {
+    if constexpr (kHalf) static_cast<unsigned short *>(p)[i] = float_to_half_bits(v);
+    else static_cast<float *>(p)[i] = v;
+}
+template <bool kHalf>
+LF_DEV void store4(void *p, size_t i, float4 v) {
+    if constexpr (kHalf)
+        static_cast<uint2 *>(p)[i] = make_uint2(float_to_half_bits(v.x) | ((unsigned int)float_to_half_bits(v.y) << 16),
+                                                float_to_half_bits(v.z) | ((unsigned int)float_to_half_bits(v.w) << 16));
+    else
+        static_cast<float4 *>(p)[i] = v;
+}
 // vertices back-projected on the fly from a depth image (same arithmetic as
 // a stored vertex map)
 struct DepthVertexAt {
@@ -142,11 +184,12 @@ struct DepthVertexAt {
     LF_DEV float4 operator()(int x, int y) const { return back_project_at(d[(size_t)y * w + x], x, y, K); }
 };
 // vertex (x, y) of the quarter-size map, through the four half-size vertices
-// resized on the fly from the full-size map of width w (same arithmetic as
-// two stored resizes)
-LF_DEV float4 resize4_quarter_at(const float4 *full, int w, int x, int y) {
-    const float4 a = resize4_at(full, w, 2 * x, 2 * y), b = resize4_at(full, w, 2 * x + 1, 2 * y),
-                 c = resize4_at(full, w, 2 * x, 2 * y + 1), d = resize4_at(full, w, 2 * x + 1, 2 * y + 1);
+// resized on the fly from the full-size map (same arithmetic as two stored
+// resizes)
+template <class V>
+LF_DEV float4 resize4_quarter_at(const V &full, int x, int y) {
+    const float4 a = resize4_at(full, 2 * x, 2 * y), b = resize4_at(full, 2 * x + 1, 2 * y),
+                 c = resize4_at(full, 2 * x, 2 * y + 1), d = resize4_at(full, 2 * x + 1, 2 * y + 1);
     return make_float4(mean4(a.x, b.x, c.x, d.x), mean4(a.y, b.y, c.y, d.y), mean4(a.z, b.z, c.z, d.z),
                        mean4(a.w, b.w, c.w, d.w));
 }
@@ -171,14 +214,14 @@ __global__ void k_resize_half(const float *__restrict__ in, float *__restrict__ 
     const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= ow || y >= oh) return;
     if (kCh == 1) out[(size_t)y * ow + x] = resize_at(in, w, x, y);
-    else reinterpret_cast<float4 *>(out)[(size_t)y * ow + x] = resize4_at(reinterpret_cast<const float4 *>(in), w, x, y);
+    else reinterpret_cast<float4 *>(out)[(size_t)y * ow + x] = resize4_at(VertexAt{reinterpret_cast<const float4 *>(in), w}, x, y);
 }
 
 __global__ void k_pyr_down(const float *__restrict__ in, float *__restrict__ out, int w, int h) {
     const int ow = (w + 1) / 2, oh = (h + 1) / 2;
     const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= ow || y >= oh) return;
-    out[(size_t)y * ow + x] = pyr_down_at(in, w, h, x, y);
+    out[(size_t)y * ow + x] = pyr_down_at(FloatAt{in, w}, w, h, x, y);
 }
 
 __global__ void k_grey(const unsigned char *__restrict__ rgb, float *__restrict__ out, int n) {
@@ -190,7 +233,7 @@ __global__ void k_derivative(const float *__restrict__ I, float *__restrict__ dx
     const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= w || y >= h) return;
     const size_t i = (size_t)y * w + x;
-    derivative_at(I, w, h, x, y, dx[i], dy[i]);
+    derivative_at(FloatAt{I, w}, w, h, x, y, dx[i], dy[i]);
 }
 
 __global__ void k_back_project(const float *__restrict__ d, float *__restrict__ vm, int w, int h, lf_intrinsics K) {
@@ -217,6 +260,7 @@ enum PartKind { kFrameIn, kPyramid, kLevelMaps, kNmap, kVmapDown, kVmapDown2 };
 
 struct Part {
     int kind, w, h, wg_begin;  // w, h: the part's output size
+    int half;                  // FP16 intensity / gradient / vertex / normal planes
     const void *in0, *in1, *in2, *in3;
     void *out0, *out1, *out2, *out3, *out4;
     lf_intrinsics K;
@@ -228,6 +272,56 @@ struct Pass {
     int total_wg;  // workgroups of all parts
 };
 
+// Pixel (x, y) of a part.  kHalf: the intensity, gradient, vertex and normal
+// planes are FP16 (read and written); depth, colour and samples are not.
+template <bool kHalf>
+LF_DEV void frame_part(const Part &p, int x, int y) {
+    const int w = p.w, h = p.h;
+    const size_t i = (size_t)y * w + x;
+    switch (p.kind) {
+    case kFrameIn: {  // raw depth, bilateral depth, intensity and the colour copy of a frame
+        const uint16_t *d16 = static_cast<const uint16_t *>(p.in0);
+        const unsigned char *rgb = static_cast<const unsigned char *>(p.in1);
+        static_cast<float *>(p.out0)[i] = depth_value(d16[i], p.scale);
+        static_cast<float *>(p.out1)[i] = bilateral_at(DepthU16At{d16, w, p.scale}, x, y, w, h);
+        const unsigned char r = rgb[3 * i], g = rgb[3 * i + 1], b = rgb[3 * i + 2];
+        store1<kHalf>(p.out2, i, grey_of(r, g, b));
+        unsigned char *c = static_cast<unsigned char *>(p.out3) + 3 * i;
+        c[0] = r; c[1] = g; c[2] = b;
+        break;
+    }
+    case kPyramid:  // next level of the depth (resize) and intensity (pyrDown) pyramids
+        static_cast<float *>(p.out0)[i] = resize_at(static_cast<const float *>(p.in0), 2 * w, x, y);
+        store1<kHalf>(p.out1, i, pyr_down_at(PlaneAt<kHalf>{p.in1, 2 * w}, 2 * w, 2 * h, x, y));
+        break;
+    case kLevelMaps: {  // intensity derivatives, vertex and normal maps of a level
+        const float *d = static_cast<const float *>(p.in0);
+        float gx, gy;
+        derivative_at(PlaneAt<kHalf>{p.in1, w}, w, h, x, y, gx, gy);
+        store1<kHalf>(p.out0, i, gx);
+        store1<kHalf>(p.out1, i, gy);
+        store4<kHalf>(p.out2, i, back_project_at(d[i], x, y, p.K));
+        const float4 n = nmap_at(DepthVertexAt{d, w, p.K}, w, h, x, y);
+        store4<kHalf>(p.out3, i, n);
+        if (p.out4) {  // level 0: the pixel's integration sample
+            const float dist = static_cast<const float *>(p.in2)[i];
+            const unsigned char *c = static_cast<const unsigned char *>(p.in3) + 3 * i;
+            static_cast<float4 *>(p.out4)[i] = integration_sample(dist, n, c);
+        }
+        break;
+    }
+    case kNmap:  // normal map of a stored vertex map
+        store4<kHalf>(p.out0, i, nmap_at(Vec4At<kHalf>{p.in0, w}, w, h, x, y));
+        break;
+    case kVmapDown:  // half-size vertex map
+        store4<kHalf>(p.out0, i, resize4_at(Vec4At<kHalf>{p.in0, 2 * w}, x, y));
+        break;
+    case kVmapDown2:  // quarter-size vertex map, through the half-size one
+        store4<kHalf>(p.out0, i, resize4_quarter_at(Vec4At<kHalf>{p.in0, 4 * w}, x, y));
+        break;
+    }
+}
+
 // Workgroup wg of a pass.
 LF_DEV void frame_pass_wg(const Pass &ps, int wg) {
     int pi = 0;
@@ -238,48 +332,8 @@ LF_DEV void frame_pass_wg(const Pass &ps, int wg) {
     const int t = wg - p.wg_begin, tiles_x = (p.w + 15) >> 4;
     const int x = (t % tiles_x) * 16 + (threadIdx.x & 15), y = (t / tiles_x) * 16 + (threadIdx.x >> 4);
     if (x >= p.w || y >= p.h) return;
-    const int w = p.w, h = p.h;
-    const size_t i = (size_t)y * w + x;
-    switch (p.kind) {
-    case kFrameIn: {  // raw depth, bilateral depth, intensity and the colour copy of a frame
-        const uint16_t *d16 = static_cast<const uint16_t *>(p.in0);
-        const unsigned char *rgb = static_cast<const unsigned char *>(p.in1);
-        static_cast<float *>(p.out0)[i] = depth_value(d16[i], p.scale);
-        static_cast<float *>(p.out1)[i] = bilateral_at(DepthU16At{d16, w, p.scale}, x, y, w, h);
-        const unsigned char r = rgb[3 * i], g = rgb[3 * i + 1], b = rgb[3 * i + 2];
-        static_cast<float *>(p.out2)[i] = grey_of(r, g, b);
-        unsigned char *c = static_cast<unsigned char *>(p.out3) + 3 * i;
-        c[0] = r; c[1] = g; c[2] = b;
-        break;
-    }
-    case kPyramid:  // next level of the depth (resize) and intensity (pyrDown) pyramids
-        static_cast<float *>(p.out0)[i] = resize_at(static_cast<const float *>(p.in0), 2 * w, x, y);
-        static_cast<float *>(p.out1)[i] = pyr_down_at(static_cast<const float *>(p.in1), 2 * w, 2 * h, x, y);
-        break;
-    case kLevelMaps: {  // intensity derivatives, vertex and normal maps of a level
-        const float *d = static_cast<const float *>(p.in0);
-        derivative_at(static_cast<const float *>(p.in1), w, h, x, y, static_cast<float *>(p.out0)[i],
-                      static_cast<float *>(p.out1)[i]);
-        static_cast<float4 *>(p.out2)[i] = back_project_at(d[i], x, y, p.K);
-        const float4 n = nmap_at(DepthVertexAt{d, w, p.K}, w, h, x, y);
-        static_cast<float4 *>(p.out3)[i] = n;
-        if (p.out4) {  // level 0: the pixel's integration sample
-            const float dist = static_cast<const float *>(p.in2)[i];
-            const unsigned char *c = static_cast<const unsigned char *>(p.in3) + 3 * i;
-            static_cast<float4 *>(p.out4)[i] = integration_sample(dist, n, c);
-        }
-        break;
-    }
-    case kNmap:  // normal map of a stored vertex map
-        static_cast<float4 *>(p.out0)[i] = nmap_at(VertexAt{static_cast<const float4 *>(p.in0), w}, w, h, x, y);
-        break;
-    case kVmapDown:  // half-size vertex map
-        static_cast<float4 *>(p.out0)[i] = resize4_at(static_cast<const float4 *>(p.in0), 2 * w, x, y);
-        break;
-    case kVmapDown2:  // quarter-size vertex map, through the half-size one
-        static_cast<float4 *>(p.out0)[i] = resize4_quarter_at(static_cast<const float4 *>(p.in0), 4 * w, x, y);
-        break;
-    }
+    if (p.half) frame_part<true>(p, x, y);
+    else frame_part<false>(p, x, y);
 }
 
 __global__ __launch_bounds__(256) void k_frame_pass(Pass ps) { frame_pass_wg(ps, blockIdx.x); }
@@ -308,9 +362,10 @@ Part make_part(int kind, int w, int h) {
 }
 
 // Workgroup offsets of the parts and the pass's total.
-void seal_pass(Pass &ps) {
+void seal_pass(Pass &ps, int half) {
     int wg = 0;
     for (int k = 0; k < ps.nparts; ++k) {
+        ps.part[k].half = half;
         ps.part[k].wg_begin = wg;
         wg += ((ps.part[k].w + 15) / 16) * ((ps.part[k].h + 15) / 16);
     }
@@ -347,7 +402,7 @@ int plan_upload(const uint16_t *depth_u16, const unsigned char *rgb, float depth
     f.in0 = depth_u16; f.in1 = rgb; f.scale = depth_scale;
     f.out0 = m.depth_raw; f.out1 = m.depth[0]; f.out2 = m.intensity[0]; f.out3 = m.rgb;
     ps.part[ps.nparts++] = f;
-    seal_pass(ps);
+    seal_pass(ps, m.half_maps);
     out.push_back(ps);
     for (int l = 0; l < levels; ++l) {
         ps = Pass{};
@@ -363,7 +418,7 @@ int plan_upload(const uint16_t *depth_u16, const unsigned char *rgb, float depth
             q.in0 = m.depth[l]; q.in1 = m.intensity[l]; q.out0 = m.depth[l + 1]; q.out1 = m.intensity[l + 1];
             ps.part[ps.nparts++] = q;
         }
-        seal_pass(ps);
+        seal_pass(ps, m.half_maps);
         out.push_back(ps);
     }
     return LF_OK;
@@ -393,7 +448,7 @@ int plan_refresh(int levels, const lf_intrinsics *K, const LfImageLevels &m, std
             p.in0 = m.vmap[b]; p.out0 = m.vmap[b + 2];
             ps.part[ps.nparts++] = p;
         }
-        seal_pass(ps);
+        seal_pass(ps, m.half_maps);
         out.push_back(ps);
     }
     return LF_OK;

@@ -93,6 +93,32 @@ LF_HD float unpack_sdf(short v) {
 }
 LF_HD short pack_sdf(float v) { return (short)(v * 32767.0f); }
 
+// Mixed-precision voxels: FP16 sdf in truncation units, kHalfUnseen (a NaN)
+// where the weight is 0, so one 2-byte load decides validity and value.
+constexpr unsigned short kHalfUnseen = 0xFFFF;
+LF_HD float half_bits_to_float(unsigned short h) { return (float)__builtin_bit_cast(_Float16, h); }
+LF_HD unsigned short float_to_half_bits(float v) { return __builtin_bit_cast(unsigned short, (_Float16)v); }
+
+// 8-bit weight code: an unsigned float with 4 exponent bits (bias 7) and 4
+// mantissa bits.  Code 0 is 0; codes 1..15 are m * 2^-10; code e<<4 | m with
+// e >= 1 is (1 + m/16) * 2^(e-7), up to 496.
+LF_HD float weight_value(unsigned int c) {
+    const unsigned int e = c >> 4, m = c & 15u;
+    return e == 0 ? (float)m * (1.0f / 1024.0f) : __builtin_bit_cast(float, ((e + 120u) << 23) | (m << 19));
+}
+// Nearest code of w >= 0, ties to the even code; above 496, 0xFF.  Below 2^-6
+// the code is round(w * 1024) (exact scaling; 16 rounds up into code 0x10 =
+// 2^-6).  Above, the float's exponent is re-biased (127 -> 7) and its 23
+// mantissa bits rounded to 4 by adding just under half a unit plus the kept
+// part's last bit; a carry moves into the exponent as it should.
+LF_HD unsigned int weight_code(float w) {
+    if (!(w > 0.0f)) return 0u;
+    if (w < 0.015625f) return (unsigned int)__builtin_rintf(w * 1024.0f);
+    const unsigned int r = __builtin_bit_cast(unsigned int, w) - (120u << 23);
+    const unsigned int c = (r + 0x3FFFFu + ((r >> 19) & 1u)) >> 19;
+    return c > 0xFFu ? 0xFFu : c;
+}
+
 // Occupancy bit of a block: its 4x4x4-block region hashed like computeHash.
 LF_HD unsigned int occupancy_bit(int bx, int by, int bz) {
     const uint32_t a = (uint32_t)(bx >> 2) * 73856093u, b = (uint32_t)(by >> 2) * 19349669u,
@@ -137,6 +163,8 @@ LF_DEV int lane_id() { return __lane_id(); }
 int lf_set_error(int code, const char *fmt, ...);
 // LF_OK when every array of the storage is present (common.hip).
 int lf_check_storage(const lf_map_storage *s);
+// Bytes of one voxel's weight for the map's precision (common.hip).
+size_t lf_weight_bytes(const lf_map_params *p);
 // Block directory rebuilt from the pool (clear, then every allocated block's
 // cell), for storage whose pool was written directly (common.hip).
 int lf_rebuild_dir(const lf_map_params *p, const lf_map_storage *s, hipStream_t st);

@@ -36,6 +36,7 @@ struct LfImageLevels {
     float *depth[8], *intensity[8], *dx[8], *dy[8];
     float *vmap[8], *nmap[8];            // float4 per pixel
     float *samples;                      // level 0 integration samples, float4 per pixel (may be null)
+    int half_maps;                       // intensity / dx / dy are FP16, vmap / nmap 4 x FP16
 };
 
 // DeviceImage::upload (device_image.cpp:146-181) in levels + 1 launches.  Every
@@ -59,6 +60,10 @@ int lf_pyramid_refresh_plan(int levels, const lf_intrinsics *K, const LfImageLev
 int lf_pass_workgroups(const void *pass);
 void lf_pass_skip(void *pass);
 int lf_pass_batch(const void *passes, int n, int max_wg, hipStream_t st);
+
+// lf_render_scene over float4 maps, or 4 x FP16 maps (half_maps).
+int lf_render_scene_maps(const void *vmap, const void *nmap, int half_maps, const unsigned char *rgb, int cols,
+                         int rows, unsigned char *rgba_out, hipStream_t st);
 
 // One raycast of a batch: a map, its target vertex map (and colour, or null),
 // the float T_wc / T_cw pose and the image size.

@@ -31,6 +31,7 @@ struct LfTrackLevel {
     const float *curr_I, *last_I, *dIdx, *dIdy;
     int cols, rows;
     lf_intrinsics K;
+    int half_maps;  // the maps hold FP16 (vmap / nmap 4 x FP16, intensity and gradients FP16)
 };
 
 // Scratch floats needed by the reductions (partial rows of the workgroups).
@@ -67,9 +68,10 @@ int lf_track_set_pose(LfTrackState *st, const double *T12, hipStream_t stream);
 // lf_track_job_make fills one and reports its grid, lf_track_job_skip marks a
 // system without a step at this index.  lf_track_batch_step runs step jobs[0..n)
 // (device memory) with a grid of gmax workgroups per system (the largest job
-// grid); each system's results equal lf_track_iteration's.
+// grid); each system's results equal lf_track_iteration's.  The jobs of one
+// batch share one map format (half_maps as in their LfTrackLevel).
 size_t lf_track_job_bytes(void);
 int lf_track_job_make(const LfTrackLevel &lv, float *scratch, LfTrackState *st, const LfStepFlags &f,
                       float icp_weight, void *job, int *grid);
 void lf_track_job_skip(void *job);
-int lf_track_batch_step(const void *jobs, int n, int gmax, hipStream_t stream);
+int lf_track_batch_step(const void *jobs, int n, int gmax, int half_maps, hipStream_t stream);

@@ -282,8 +282,10 @@ int visible_grid(const lf_map_params *p) {
 // only): the pixel's depth, weight and colour come from one 16-byte
 // integration sample (DeviceImage frames), whose weight is the same
 // expression evaluated when the sample was made; otherwise depth, normal and
-// colour are gathered separately.
-template <bool kWeighted, bool kColour, bool kPacked = false>
+// colour are gathered separately.  kMixed: FP16 sdf and 8-bit weight codes,
+// and the projection and the weighted means use one reciprocal each instead
+// of correctly rounded divisions.
+template <bool kWeighted, bool kColour, bool kPacked = false, bool kMixed = false>
 __global__ __launch_bounds__(256) void k_integrate(MapDev m, const float *__restrict__ depth,
                                                    const float *__restrict__ nmap,
                                                    const unsigned char *__restrict__ rgb, int cols, int rows,
@@ -318,8 +320,15 @@ __global__ __launch_bounds__(256) void k_integrate(MapDev m, const float *__rest
             const int lz = lz0 + 4 * h;
             const float wx = (float)(p.x * 8 + lx) * vs, wy = (float)(p.y * 8 + ly) * vs, wz = (float)(p.z * 8 + lz) * vs;
             const f3 pt = xform(Tcw, f3{wx, wy, wz});
-            const int u = proj_floor(K.fx * pt.x / pt.z + K.cx + 0.5f);
-            const int v = proj_floor(K.fy * pt.y / pt.z + K.cy + 0.5f);
+            int u, v;
+            if constexpr (kMixed) {
+                const float iz = __builtin_amdgcn_rcpf(pt.z);
+                u = proj_floor(K.fx * pt.x * iz + K.cx + 0.5f);
+                v = proj_floor(K.fy * pt.y * iz + K.cy + 0.5f);
+            } else {
+                u = proj_floor(K.fx * pt.x / pt.z + K.cx + 0.5f);
+                v = proj_floor(K.fy * pt.y / pt.z + K.cy + 0.5f);
+            }
             ok[h] = !(u < 0 || v < 0 || u > cols - 1 || v > rows - 1);
             pix[h] = ok[h] ? (size_t)v * cols + u : 0;
             ptz[h] = pt.z;
@@ -357,7 +366,11 @@ __global__ __launch_bounds__(256) void k_integrate(MapDev m, const float *__rest
             ok[h] = !(sdf[h] < -trunc);
             if (!ok[h]) continue;
             const size_t vi = (size_t)b * LF_BLOCK3 + (size_t)(t + 256 * h);
-            wp[h] = m.weight[vi];
+            if constexpr (kMixed) {
+                wp[h] = weight_value(reinterpret_cast<const unsigned char *>(m.weight)[vi]);
+            } else {
+                wp[h] = m.weight[vi];
+            }
             sp[h] = m.sdf[vi];
             if (kColour) {
                 const unsigned char *c = m.rgb + (size_t)b * (3 * LF_BLOCK3) + (size_t)(t + 256 * h);
@@ -377,6 +390,32 @@ __global__ __launch_bounds__(256) void k_integrate(MapDev m, const float *__rest
 #pragma unroll
                 for (int k = 0; k < 3; ++k)
                     cin[k] = kPacked ? (unsigned char)(cpk[h] >> (8 * k)) : rgb[3 * pix[h] + k];
+            }
+            if constexpr (kMixed) {
+                unsigned short *sdf16 = reinterpret_cast<unsigned short *>(m.sdf);
+                unsigned char *w8 = reinterpret_cast<unsigned char *>(m.weight);
+                if (wp[h] == 0.0f) {
+                    // a weight whose code is 0 leaves the voxel unseen
+                    const unsigned int code = weight_code(fminf(wt, 255.0f));
+                    sdf16[vi] = code == 0u ? kHalfUnseen : float_to_half_bits(sdf_h);
+                    w8[vi] = (unsigned char)code;
+                    if (kColour) {
+                        c[0] = cin[0]; c[LF_BLOCK3] = cin[1]; c[2 * LF_BLOCK3] = cin[2];
+                    }
+                    continue;
+                }
+                const float inv = __builtin_amdgcn_rcpf(wp[h] + wt);
+                const float spf = half_bits_to_float((unsigned short)sp[h]);
+                sdf16[vi] = float_to_half_bits((spf * wp[h] + sdf_h * wt) * inv);
+                w8[vi] = (unsigned char)weight_code(fminf(wp[h] + wt, 255.0f));
+                if (kColour) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const float mc = ((float)cold[h][k] * wp[h] + (float)cin[k] * wt) * inv;
+                        c[k * LF_BLOCK3] = (unsigned char)(int)fminf(mc, 255.0f);
+                    }
+                }
+                continue;
             }
             if (wp[h] == 0.0f) {
                 // weight stays 0 only when wt is 0: keep the voxel's sdf code unseen
@@ -409,7 +448,8 @@ __global__ __launch_bounds__(256) void k_integrate(MapDev m, const float *__rest
 // colour likewise; w' <= 0 returns the voxel to unseen.  Exact up to the int16
 // sdf and uint8 colour rounding, except where the weight saturated at 255.
 // Not in the reference (deleteBlock, map_struct.cu:327-363, has no callers).
-template <bool kColour, bool kPacked>
+// kMixed: FP16 sdf and 8-bit weight codes (exact up to their rounding).
+template <bool kColour, bool kPacked, bool kMixed = false>
 __global__ __launch_bounds__(256) void k_deintegrate(MapDev m, const float *__restrict__ depth,
                                                      const float *__restrict__ nmap,
                                                      const unsigned char *__restrict__ rgb, int cols, int rows,
@@ -455,19 +495,33 @@ __global__ __launch_bounds__(256) void k_deintegrate(MapDev m, const float *__re
             if (sdf < -trunc) continue;
             sdf = fminf(1.0f, sdf * inv_trunc);
             const size_t vi = (size_t)b * LF_BLOCK3 + (size_t)(t + 256 * h);
-            const float w0 = m.weight[vi];
+            unsigned short *sdf16 = reinterpret_cast<unsigned short *>(m.sdf);
+            unsigned char *w8 = reinterpret_cast<unsigned char *>(m.weight);
+            const float w0 = kMixed ? weight_value(w8[vi]) : m.weight[vi];
             if (w0 == 0.0f) continue;  // nothing fused here
             unsigned char *c = m.rgb + (size_t)b * (3 * LF_BLOCK3) + (size_t)(t + 256 * h);
             const float wr = w0 - wt;
-            if (!(wr > 0.0f)) {
-                m.sdf[vi] = kSdfUnseen;
-                m.weight[vi] = 0.0f;
+            const unsigned int wr_code = kMixed && wr > 0.0f ? weight_code(wr) : 0u;
+            if (!(wr > 0.0f) || (kMixed && wr_code == 0u)) {
+                if (kMixed) {
+                    sdf16[vi] = kHalfUnseen;
+                    w8[vi] = 0;
+                } else {
+                    m.sdf[vi] = kSdfUnseen;
+                    m.weight[vi] = 0.0f;
+                }
                 if (kColour) { c[0] = 0; c[LF_BLOCK3] = 0; c[2 * LF_BLOCK3] = 0; }
                 continue;
             }
-            const float sr = (unpack_sdf(m.sdf[vi]) * w0 - sdf * wt) / wr;
-            m.sdf[vi] = pack_sdf(fmaxf(-1.0f, fminf(1.0f, sr)));
-            m.weight[vi] = wr;
+            if (kMixed) {
+                const float sr = (half_bits_to_float(sdf16[vi]) * w0 - sdf * wt) / wr;
+                sdf16[vi] = float_to_half_bits(fmaxf(-1.0f, fminf(1.0f, sr)));
+                w8[vi] = (unsigned char)wr_code;
+            } else {
+                const float sr = (unpack_sdf(m.sdf[vi]) * w0 - sdf * wt) / wr;
+                m.sdf[vi] = pack_sdf(fmaxf(-1.0f, fminf(1.0f, sr)));
+                m.weight[vi] = wr;
+            }
             if (kColour) {
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
@@ -503,7 +557,19 @@ int lf_update_ev(const lf_map_params *p, const lf_map_storage *s, const float *d
     const int G = integrate_grid();
     if (e0) LF_HIP_CHECK(hipEventRecord(e0, st));
     const float4 *smp = reinterpret_cast<const float4 *>(samples);
-    if (weighted && smp) {
+    if (p->precision == LF_PRECISION_MIXED) {
+        // the frame's packed samples (DeviceImage) or separate depth / normal / colour
+        if (weighted && smp) {
+            if (rgb) k_integrate<true, true, true, true><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K, smp);
+            else k_integrate<true, false, true, true><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K, smp);
+        } else if (weighted) {
+            if (rgb) k_integrate<true, true, false, true><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K);
+            else k_integrate<true, false, false, true><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K);
+        } else {
+            if (rgb) k_integrate<false, true, false, true><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K);
+            else k_integrate<false, false, false, true><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K);
+        }
+    } else if (weighted && smp) {
         if (rgb) k_integrate<true, true, true><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K, smp);
         else k_integrate<true, false, true><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K, smp);
     } else if (weighted) {
@@ -530,7 +596,15 @@ int lf_deintegrate_ev(const lf_map_params *p, const lf_map_storage *s, const flo
     LF_LAUNCH_CHECK();
     const int G = integrate_grid();
     const float4 *smp = reinterpret_cast<const float4 *>(samples);
-    if (smp) {
+    if (p->precision == LF_PRECISION_MIXED) {
+        if (smp) {
+            if (rgb) k_deintegrate<true, true, true><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K, smp);
+            else k_deintegrate<false, true, true><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K, smp);
+        } else {
+            if (rgb) k_deintegrate<true, false, true><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K, smp);
+            else k_deintegrate<false, false, true><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K, smp);
+        }
+    } else if (smp) {
         if (rgb) k_deintegrate<true, true><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K, smp);
         else k_deintegrate<false, true><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K, smp);
     } else {

@@ -20,7 +20,10 @@
 // uncontended table.  Reading accepts any file whose live entries (ptr != -1)
 // name distinct blocks: it compacts them in pool order and rebuilds this
 // library's table.  The sdf of a weight-0 voxel is not observable in the
-// reference; it is written as 0 and read as the unseen code.
+// reference; it is written as 0 and read as the unseen code.  A mixed-
+// precision map converts on the way: FP16 sdf to the reference's int16
+// (packFloat's truncation) and 8-bit weight codes to float32, and back to the
+// nearest FP16 and nearest code on reading.
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -67,13 +70,24 @@ int download(const lf_map_params *p, const lf_map_storage *s, hipStream_t st, Ho
     h.sdf.resize(n * LF_BLOCK3);
     h.weight.resize(n * LF_BLOCK3);
     h.rgb.resize(3 * n * LF_BLOCK3);
+    const bool mixed = p->precision == LF_PRECISION_MIXED;
+    std::vector<unsigned char> codes(mixed ? n * LF_BLOCK3 : 0);
     if (n) {
         LF_HIP_CHECK(hipMemcpyAsync(h.pos.data(), s->block_pos, 4 * n * sizeof(int), hipMemcpyDeviceToHost, st));
         LF_HIP_CHECK(hipMemcpyAsync(h.sdf.data(), s->sdf, n * LF_BLOCK3 * sizeof(short), hipMemcpyDeviceToHost, st));
-        LF_HIP_CHECK(hipMemcpyAsync(h.weight.data(), s->weight, n * LF_BLOCK3 * sizeof(float), hipMemcpyDeviceToHost, st));
+        if (mixed)
+            LF_HIP_CHECK(hipMemcpyAsync(codes.data(), s->weight, n * LF_BLOCK3, hipMemcpyDeviceToHost, st));
+        else
+            LF_HIP_CHECK(hipMemcpyAsync(h.weight.data(), s->weight, n * LF_BLOCK3 * sizeof(float), hipMemcpyDeviceToHost, st));
         LF_HIP_CHECK(hipMemcpyAsync(h.rgb.data(), s->rgb, 3 * n * LF_BLOCK3, hipMemcpyDeviceToHost, st));
     }
     LF_HIP_CHECK(hipStreamSynchronize(st));
+    if (mixed)  // to the FP32 layout: int16 sdf (unseen kept), float weights
+        for (size_t v = 0; v < n * LF_BLOCK3; ++v) {
+            h.weight[v] = lf::weight_value(codes[v]);
+            const unsigned short hb = (unsigned short)h.sdf[v];
+            h.sdf[v] = hb == lf::kHalfUnseen ? lf::kSdfUnseen : lf::pack_sdf(lf::half_bits_to_float(hb));
+        }
     return LF_OK;
 }
 
@@ -236,10 +250,21 @@ extern "C" int lf_map_read(const lf_map_params *p, const lf_map_storage *s, cons
     LF_HIP_CHECK(hipMemcpyAsync(s->keys, keys.data(), keys.size() * sizeof(unsigned long long), hipMemcpyHostToDevice, st));
     LF_HIP_CHECK(hipMemcpyAsync(s->slot_block, slot_block.data(), slot_block.size() * sizeof(int), hipMemcpyHostToDevice, st));
     LF_HIP_CHECK(hipMemcpyAsync(s->occupancy, occ.data(), occ.size() * sizeof(unsigned int), hipMemcpyHostToDevice, st));
+    std::vector<unsigned char> codes;
+    if (p->precision == LF_PRECISION_MIXED) {  // nearest FP16 sdf and weight code
+        codes.resize(weight.size());
+        for (size_t v = 0; v < weight.size(); ++v) {
+            codes[v] = (unsigned char)lf::weight_code(std::min(weight[v], 255.0f));
+            sdf[v] = (short)(codes[v] == 0 ? lf::kHalfUnseen : lf::float_to_half_bits(lf::unpack_sdf(sdf[v])));
+        }
+    }
     if (n) {
         LF_HIP_CHECK(hipMemcpyAsync(s->block_pos, pos.data(), pos.size() * sizeof(int), hipMemcpyHostToDevice, st));
         LF_HIP_CHECK(hipMemcpyAsync(s->sdf, sdf.data(), sdf.size() * sizeof(short), hipMemcpyHostToDevice, st));
-        LF_HIP_CHECK(hipMemcpyAsync(s->weight, weight.data(), weight.size() * sizeof(float), hipMemcpyHostToDevice, st));
+        if (codes.empty())
+            LF_HIP_CHECK(hipMemcpyAsync(s->weight, weight.data(), weight.size() * sizeof(float), hipMemcpyHostToDevice, st));
+        else
+            LF_HIP_CHECK(hipMemcpyAsync(s->weight, codes.data(), codes.size(), hipMemcpyHostToDevice, st));
         LF_HIP_CHECK(hipMemcpyAsync(s->rgb, rgb.data(), rgb.size(), hipMemcpyHostToDevice, st));
     }
     const unsigned int allocated = (unsigned int)n;
@@ -265,7 +290,7 @@ extern "C" int lf_map_copy(const lf_map_params *p, const lf_map_storage *src, co
         {dst->slot_flag, src->slot_flag, E},
         {dst->block_pos, src->block_pos, 4 * B * sizeof(int)},
         {dst->sdf, src->sdf, B * LF_BLOCK3 * sizeof(short)},
-        {dst->weight, src->weight, B * LF_BLOCK3 * sizeof(float)},
+        {dst->weight, src->weight, B * LF_BLOCK3 * lf_weight_bytes(p)},
         {dst->rgb, src->rgb, 3 * B * LF_BLOCK3},
         {dst->visible, src->visible, B * sizeof(int)},
         {dst->visible_pos, src->visible_pos, B * 4 * sizeof(int)},

@@ -49,8 +49,9 @@ LF_DEV int tri_count(int ci) {
     return n / 3;
 }
 
-// mode 0: vertices, 1: + flat normals, 2: + corner colours.
-template <int kMode>
+// mode 0: vertices, 1: + flat normals, 2: + corner colours.  kMixed: FP16
+// sdf and 8-bit weight codes.
+template <int kMode, bool kMixed>
 __global__ __launch_bounds__(256) void k_mesh(MeshDev m, float *__restrict__ vtx, float *__restrict__ nrm,
                                               unsigned char *__restrict__ col, unsigned int max_tri) {
     __shared__ int s_nb[8];
@@ -77,9 +78,11 @@ __global__ __launch_bounds__(256) void k_mesh(MeshDev m, float *__restrict__ vtx
             if (nb >= 0) {
                 const int l = lidx(x, y, z);
                 const size_t vi = (size_t)nb * LF_BLOCK3 + l;
-                const float w = m.weight[vi];
+                const float w = kMixed ? weight_value(reinterpret_cast<const unsigned char *>(m.weight)[vi])
+                                       : m.weight[vi];
                 ok = kMode == 2 ? ((double)w > 1e-3) : (w != 0.0f);
-                s = unpack_sdf(m.sdf[vi]);
+                s = kMixed ? half_bits_to_float(reinterpret_cast<const unsigned short *>(m.sdf)[vi])
+                           : unpack_sdf(m.sdf[vi]);
                 if (kMode == 2) {
                     const unsigned char *c = m.rgb + (size_t)nb * (3 * LF_BLOCK3) + l;
                     s_col[3 * i] = c[0]; s_col[3 * i + 1] = c[LF_BLOCK3]; s_col[3 * i + 2] = c[2 * LF_BLOCK3];
@@ -209,9 +212,15 @@ extern "C" int lf_create_mesh(const lf_map_params *p, const lf_map_storage *s, f
     LF_LAUNCH_CHECK();
     const int G = std::max(1, std::min(p->num_blocks, 8192));
     const unsigned int cap = (unsigned int)max_triangles;
-    if (colour_out) k_mesh<2><<<G, 256, 0, st>>>(m, vertex_out, nullptr, colour_out, cap);
-    else if (normal_out) k_mesh<1><<<G, 256, 0, st>>>(m, vertex_out, normal_out, nullptr, cap);
-    else k_mesh<0><<<G, 256, 0, st>>>(m, vertex_out, nullptr, nullptr, cap);
+    if (p->precision == LF_PRECISION_MIXED) {
+        if (colour_out) k_mesh<2, true><<<G, 256, 0, st>>>(m, vertex_out, nullptr, colour_out, cap);
+        else if (normal_out) k_mesh<1, true><<<G, 256, 0, st>>>(m, vertex_out, normal_out, nullptr, cap);
+        else k_mesh<0, true><<<G, 256, 0, st>>>(m, vertex_out, nullptr, nullptr, cap);
+    } else {
+        if (colour_out) k_mesh<2, false><<<G, 256, 0, st>>>(m, vertex_out, nullptr, colour_out, cap);
+        else if (normal_out) k_mesh<1, false><<<G, 256, 0, st>>>(m, vertex_out, normal_out, nullptr, cap);
+        else k_mesh<0, false><<<G, 256, 0, st>>>(m, vertex_out, nullptr, nullptr, cap);
+    }
     LF_LAUNCH_CHECK();
     unsigned int cnt = 0;
     LF_HIP_CHECK(hipMemcpyAsync(&cnt, s->counters + LF_CNT_TRIANGLES, sizeof(cnt), hipMemcpyDeviceToHost, st));

@@ -274,16 +274,23 @@ LF_DEV int resolve(const RayDev &r, BlockCache &bc, const i3 &v) {
 }
 
 // findVoxel + read_sdf (ray_tracing.cu:291-305).  A voxel is valid when its
-// weight is non-zero, which the sdf plane encodes (kSdfUnseen), so one 2-byte
-// load per sample decides both validity and value.
+// weight is non-zero, which the sdf plane encodes (kSdfUnseen; kHalfUnseen, a
+// NaN, in mixed precision), so one 2-byte load per sample decides both
+// validity and value.
+template <bool kMixed>
 LF_DEV float read_sdf(const RayDev &r, BlockCache &bc, f3 p, bool &valid, int *voxel_out = nullptr) {
     const i3 v = to_i3(p);
     const int b = resolve(r, bc, v);
     valid = false;
     if (b < 0) return __int_as_float(0x7fc00000);
     const int vi = b * LF_BLOCK3 + lidx(v.x, v.y, v.z);
-    const short sd = r.sdf[vi];
     if (voxel_out) *voxel_out = vi;
+    if constexpr (kMixed) {
+        const float sv = half_bits_to_float(reinterpret_cast<const unsigned short *>(r.sdf)[vi]);
+        valid = !isnan(sv);
+        return sv;
+    }
+    const short sd = r.sdf[vi];
     if (sd == kSdfUnseen) return __int_as_float(0x7fc00000);
     valid = true;
     return unpack_sdf(sd);
@@ -292,6 +299,7 @@ LF_DEV float read_sdf(const RayDev &r, BlockCache &bc, f3 p, bool &valid, int *v
 // read_sdf_interped (ray_tracing.cu:307-339).  The 8 corner voxels are
 // resolved first, then all 8 sdf loads are issued in one round trip; the
 // arithmetic is the reference's order.
+template <bool kMixed>
 LF_DEV float read_interp(const RayDev &r, BlockCache &bc, f3 p, bool &valid) {
     const f3 xyz = {p.x - floorf(p.x), p.y - floorf(p.y), p.z - floorf(p.z)};
     // corner order of the reference: (0,0,0) (1,0,0) (0,1,0) (1,1,0) (0,0,1) (1,0,1) (0,1,1) (1,1,1)
@@ -310,9 +318,16 @@ LF_DEV float read_interp(const RayDev &r, BlockCache &bc, f3 p, bool &valid) {
     bool vp = true;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        const bool ok = vi[k] >= 0 && c[k] != kSdfUnseen;
-        vp = vp && ok;
-        s[k] = ok ? unpack_sdf(c[k]) : __int_as_float(0x7fc00000);
+        if constexpr (kMixed) {
+            const float sv = half_bits_to_float((unsigned short)c[k]);
+            const bool ok = vi[k] >= 0 && !isnan(sv);
+            vp = vp && ok;
+            s[k] = ok ? sv : __int_as_float(0x7fc00000);
+        } else {
+            const bool ok = vi[k] >= 0 && c[k] != kSdfUnseen;
+            vp = vp && ok;
+            s[k] = ok ? unpack_sdf(c[k]) : __int_as_float(0x7fc00000);
+        }
     }
     float r0 = (1.0f - xyz.x) * s[0] + xyz.x * s[1];
     float r1 = (1.0f - xyz.x) * s[2] + xyz.x * s[3];
@@ -327,8 +342,9 @@ LF_DEV float read_interp(const RayDev &r, BlockCache &bc, f3 p, bool &valid) {
 // MapRenderingDelegate::operator() (check_colour == false) and
 // ::raycast_with_colour (check_colour == true): ray_tracing.cu:346-425,478-564.
 // Workgroup wg (its index within this raycast's grid; the grid's x extent is
-// a multiple of 8, so wg % 8 is the XCD the workgroup runs on).
-template <bool kCheckColour>
+// a multiple of 8, so wg % 8 is the XCD the workgroup runs on).  kMixed: FP16
+// voxels in, 4 x FP16 vertices out.
+template <bool kCheckColour, bool kMixed>
 LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char *__restrict__ rgb_out, int cols,
                        int rows, const float *__restrict__ pose24, const lf_intrinsics &K, int wg_x, int wg_y, int wg,
                        unsigned long long *s_cache, uint4 *s_occ) {
@@ -350,7 +366,7 @@ LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char 
     const int tile_x = (L % wg_x) * 2 + (wave & 1), tile_y = (L / wg_x) * 2 + (wave >> 1);
     const int x = tile_x * 8 + (lane & 7), y = tile_y * 8 + (lane >> 3);
     if (x >= cols || y >= rows) return;
-    float4 *out = reinterpret_cast<float4 *>(vmap) + (size_t)y * cols + x;
+    const size_t px = (size_t)y * cols + x;
     const float qnan = __int_as_float(0x7fc00000);
     float4 res4 = make_float4(qnan, qnan, qnan, qnan);
     unsigned char col[3] = {255, 255, 255};
@@ -376,8 +392,8 @@ LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char 
         bool valid = false, back_face = false;
         while (dist_s < dist_e) {
             last_sdf = sdf;
-            sdf = read_sdf(r, bc, res, valid);
-            if (sdf <= 0.5f && sdf >= -0.5f) sdf = read_interp(r, bc, res, valid);
+            sdf = read_sdf<kMixed>(r, bc, res, valid);
+            if (sdf <= 0.5f && sdf >= -0.5f) sdf = read_interp<kMixed>(r, bc, res, valid);
             if (sdf <= 0.0f) break;
             if (sdf >= 0.0f && last_sdf < 0.0f) { back_face = true; break; }
             step = valid ? fmaxf(sdf * step_scale, 1.0f) : 2.0f;
@@ -387,14 +403,14 @@ LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char 
         if (back_face || !(sdf <= 0.0f)) break;
         step = sdf * step_scale;
         res = add(res, mul(dir, step));
-        sdf = read_interp(r, bc, res, valid);
+        sdf = read_interp<kMixed>(r, bc, res, valid);
         step = sdf * step_scale;
         res = add(res, mul(dir, step));
         if (!valid) break;
         if (kCheckColour) {  // read_colour (ray_tracing.cu:427-441)
             bool cv;
             int vi = -1;
-            read_sdf(r, bc, res, cv, &vi);
+            read_sdf<kMixed>(r, bc, res, cv, &vi);
             if (!cv) break;
             const int b = vi / LF_BLOCK3, l = vi % LF_BLOCK3;
             const unsigned char *c = r.rgb + (size_t)b * (3 * LF_BLOCK3) + l;
@@ -404,30 +420,36 @@ LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char 
         const f3 q = xform(Tcw, mul(res, r.voxel_size));
         res4 = make_float4(q.x, q.y, q.z, 1.0f);
     } while (false);
-    *out = res4;
+    if constexpr (kMixed) {
+        const unsigned int lo = float_to_half_bits(res4.x) | ((unsigned int)float_to_half_bits(res4.y) << 16);
+        const unsigned int hi = float_to_half_bits(res4.z) | ((unsigned int)float_to_half_bits(res4.w) << 16);
+        reinterpret_cast<uint2 *>(vmap)[px] = make_uint2(lo, hi);
+    } else {
+        reinterpret_cast<float4 *>(vmap)[px] = res4;
+    }
     if (rgb_out) {
         unsigned char *o = rgb_out + 3 * ((size_t)y * cols + x);
         o[0] = col[0]; o[1] = col[1]; o[2] = col[2];
     }
 }
 
-template <bool kCheckColour>
+template <bool kCheckColour, bool kMixed>
 __global__ __launch_bounds__(256) void k_raycast(RayDev r, float *__restrict__ vmap, unsigned char *__restrict__ rgb_out,
                                                  int cols, int rows, const float *__restrict__ pose24,
                                                  lf_intrinsics K, int wg_x, int wg_y) {
     __shared__ unsigned long long s_cache[kLdsCache];
     __shared__ uint4 s_occ[LF_OCC_WORDS / 4];
-    raycast_wg<kCheckColour>(r, vmap, rgb_out, cols, rows, pose24, K, wg_x, wg_y, blockIdx.x, s_cache, s_occ);
+    raycast_wg<kCheckColour, kMixed>(r, vmap, rgb_out, cols, rows, pose24, K, wg_x, wg_y, blockIdx.x, s_cache, s_occ);
 }
 
-template <bool kCheckColour>
+template <bool kCheckColour, bool kMixed>
 __global__ __launch_bounds__(256) void k_raycast_batch(const RayJob *__restrict__ jobs) {
     __shared__ unsigned long long s_cache[kLdsCache];
     __shared__ uint4 s_occ[LF_OCC_WORDS / 4];
     const RayJob &j = jobs[blockIdx.y];
     const int g = kXcds * ((j.wg_x * j.wg_y + kXcds - 1) / kXcds);
     if ((int)blockIdx.x >= g) return;  // this raycast's grid is smaller than the batch's
-    raycast_wg<kCheckColour>(j.r, j.vmap, j.rgb_out, j.cols, j.rows, j.pose24, j.K, j.wg_x, j.wg_y, blockIdx.x,
+    raycast_wg<kCheckColour, kMixed>(j.r, j.vmap, j.rgb_out, j.cols, j.rows, j.pose24, j.K, j.wg_x, j.wg_y, blockIdx.x,
                              s_cache, s_occ);
 }
 
@@ -487,8 +509,13 @@ int lf_raycast_ev(const lf_map_params *p, const lf_map_storage *s, float *vmap, 
     const int wg_x = (tx + 1) / 2, wg_y = (ty + 1) / 2;  // 2x2 tiles per workgroup
     const int g = kXcds * ((wg_x * wg_y + kXcds - 1) / kXcds);
     if (e0) LF_HIP_CHECK(hipEventRecord(e0, st));
-    if (check_colour) k_raycast<true><<<g, 256, 0, st>>>(r, vmap, rgb_out, cols, rows, pose24, *K, wg_x, wg_y);
-    else k_raycast<false><<<g, 256, 0, st>>>(r, vmap, rgb_out, cols, rows, pose24, *K, wg_x, wg_y);
+    if (p->precision == LF_PRECISION_MIXED) {  // FP16 voxels, 4 x FP16 vertex map
+        if (check_colour) k_raycast<true, true><<<g, 256, 0, st>>>(r, vmap, rgb_out, cols, rows, pose24, *K, wg_x, wg_y);
+        else k_raycast<false, true><<<g, 256, 0, st>>>(r, vmap, rgb_out, cols, rows, pose24, *K, wg_x, wg_y);
+    } else {
+        if (check_colour) k_raycast<true, false><<<g, 256, 0, st>>>(r, vmap, rgb_out, cols, rows, pose24, *K, wg_x, wg_y);
+        else k_raycast<false, false><<<g, 256, 0, st>>>(r, vmap, rgb_out, cols, rows, pose24, *K, wg_x, wg_y);
+    }
     LF_LAUNCH_CHECK();
     if (e1) LF_HIP_CHECK(hipEventRecord(e1, st));
     return LF_OK;
@@ -512,6 +539,8 @@ int lf_raycast_batch_ev(const LfRayItem *items, int n, void *jobs_host, void *jo
         const LfRayItem &it = items[i];
         if (!it.p || !it.s || !it.vmap || !it.pose24 || !it.K)
             return lf_set_error(LF_ERR_ARG, "null argument in raycast batch item %d", i);
+        if (it.p->precision != items[0].p->precision)
+            return lf_set_error(LF_ERR_ARG, "raycast batch item %d: maps of one batch share one precision", i);
         if (int rc = lf_check_storage(it.s)) return rc;
         if (it.cols / 8 != it.s->tiles_x || it.rows / 8 != it.s->tiles_y)
             return lf_set_error(LF_ERR_ARG, "batch item %d: raycast size %dx%d does not match the map's %dx%d tiles",
@@ -539,8 +568,14 @@ int lf_raycast_batch_ev(const LfRayItem *items, int n, void *jobs_host, void *jo
     if (lds_max <= zrange_lds_limit()) k_zrange_fill_batch<true><<<dim3(zmax, n), kZThreads, lds_max, st>>>(jobs);
     else k_zrange_fill_batch<false><<<dim3(zmax, n), kZThreads, 0, st>>>(jobs);
     LF_LAUNCH_CHECK();
-    if (check_colour) k_raycast_batch<true><<<dim3(gmax, n), 256, 0, st>>>(jobs);
-    else k_raycast_batch<false><<<dim3(gmax, n), 256, 0, st>>>(jobs);
+    const dim3 grid(gmax, n);
+    if (items[0].p->precision == LF_PRECISION_MIXED) {
+        if (check_colour) k_raycast_batch<true, true><<<grid, 256, 0, st>>>(jobs);
+        else k_raycast_batch<false, true><<<grid, 256, 0, st>>>(jobs);
+    } else {
+        if (check_colour) k_raycast_batch<true, false><<<grid, 256, 0, st>>>(jobs);
+        else k_raycast_batch<false, false><<<grid, 256, 0, st>>>(jobs);
+    }
     LF_LAUNCH_CHECK();
     return LF_OK;
 }

@@ -76,18 +76,36 @@ struct Own {
     float il;
 };
 
-template <bool kIcp, bool kRgb>
+// Element i of a map of float4 or of 4 x FP16 (kHalf), and of a plane of
+// float or FP16: the mixed-precision maps widen to FP32 as they are loaded.
+template <bool kHalf>
+LF_DEV float4 map4(const float *p, size_t i) {
+    if constexpr (kHalf) {
+        const uint2 u = reinterpret_cast<const uint2 *>(p)[i];
+        return make_float4(half_bits_to_float((unsigned short)u.x), half_bits_to_float((unsigned short)(u.x >> 16)),
+                           half_bits_to_float((unsigned short)u.y), half_bits_to_float((unsigned short)(u.y >> 16)));
+    } else {
+        return reinterpret_cast<const float4 *>(p)[i];
+    }
+}
+template <bool kHalf>
+LF_DEV float plane1(const float *p, size_t i) {
+    if constexpr (kHalf) return half_bits_to_float(reinterpret_cast<const unsigned short *>(p)[i]);
+    else return p[i];
+}
+
+template <bool kIcp, bool kRgb, bool kHalf = false>
 LF_DEV Own load_own(const LfTrackLevel &L, int i) {
     Own o;
-    o.lv = reinterpret_cast<const float4 *>(L.last_vmap)[i];
+    o.lv = map4<kHalf>(L.last_vmap, i);
     o.ln = make_float4(0.f, 0.f, 0.f, 0.f);
     o.il = 0.0f;
-    if (kIcp) o.ln = reinterpret_cast<const float4 *>(L.last_nmap)[i];
-    if (kRgb) o.il = L.last_I[i];
+    if (kIcp) o.ln = map4<kHalf>(L.last_nmap, i);
+    if (kRgb) o.il = plane1<kHalf>(L.last_I, i);
     return o;
 }
 
-template <bool kIcp, bool kRgb>
+template <bool kIcp, bool kRgb, bool kHalf = false>
 LF_DEV void pixel_rows(const ReduceArgs &a, const Own &o, const float *P, float stddev, float th, float *si,
                        float *sr) {
     const LfTrackLevel &L = a.lv;
@@ -118,17 +136,17 @@ LF_DEV void pixel_rows(const ReduceArgs &a, const Own &o, const float *P, float 
     // gathers
     float4 cv = make_float4(0.f, 0.f, 0.f, 0.f), cn = cv;
     if (kIcp) {
-        cv = reinterpret_cast<const float4 *>(L.curr_vmap)[j];
-        cn = reinterpret_cast<const float4 *>(L.curr_nmap)[j];
+        cv = map4<kHalf>(L.curr_vmap, j);
+        cn = map4<kHalf>(L.curr_nmap, j);
     }
     float tI[4], tx[4], ty[4];
     if (kRgb) {
         const size_t o[4] = {t0, t0 + 1, t0 + L.cols, t0 + L.cols + 1};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            tI[k] = L.curr_I[o[k]];
-            tx[k] = L.dIdx[o[k]];
-            ty[k] = L.dIdy[o[k]];
+            tI[k] = plane1<kHalf>(L.curr_I, o[k]);
+            tx[k] = plane1<kHalf>(L.dIdx, o[k]);
+            ty[k] = plane1<kHalf>(L.dIdy, o[k]);
         }
     }
     float row[7];
@@ -600,7 +618,8 @@ LF_DEV void track_finish(LfTrackState &st, const double *est, int invalid, bool 
 // workgroup sums the ICP and RGB rows of its pixels into one partial row.  The
 // pose-independent loads of the first pixel are issued together with the
 // tracker-state loads.
-// Workgroup wg of the nwg computing the step's partial rows.
+// Workgroup wg of the nwg computing the step's partial rows.  kHalf: FP16 maps.
+template <bool kHalf>
 LF_DEV void track_step_wg(const ReduceArgs &a, int wg, int nwg, float (*s_wave)[kVals]) {
     const int tid = threadIdx.x;
     const int n = a.lv.cols * a.lv.rows;
@@ -608,7 +627,7 @@ LF_DEV void track_step_wg(const ReduceArgs &a, int wg, int nwg, float (*s_wave)[
     int i = wg * kThreads + tid;
     // the pixel's own loads and the state loads are in flight together
     Own o = {};
-    if (i < n) o = load_own<true, true>(a.lv, i);
+    if (i < n) o = load_own<true, true, kHalf>(a.lv, i);
     float P[12], stddev;
     if (a.first_of_frame) {
 #pragma unroll
@@ -630,17 +649,18 @@ LF_DEV void track_step_wg(const ReduceArgs &a, int wg, int nwg, float (*s_wave)[
     while (i < n) {
         const Own cur = o;
         const int next = i + stride;
-        if (next < n) o = load_own<true, true>(a.lv, next);
-        pixel_rows<true, true>(a, cur, P, stddev, th, si, sr);
+        if (next < n) o = load_own<true, true, kHalf>(a.lv, next);
+        pixel_rows<true, true, kHalf>(a, cur, P, stddev, th, si, sr);
         i = next;
     }
     const float t = block_sums<true, true>(si, sr, s_wave);
     if (tid < kVals) a.partials[(size_t)tid * nwg + wg] = t;
 }
 
+template <bool kHalf>
 __global__ __launch_bounds__(kThreads) void k_track_step(ReduceArgs a) {
     __shared__ float s_wave[kWaves][kVals];
-    track_step_wg(a, blockIdx.x, gridDim.x, s_wave);
+    track_step_wg<kHalf>(a, blockIdx.x, gridDim.x, s_wave);
 }
 
 // Second launch (one workgroup): the partial rows summed in a fixed order,
@@ -689,11 +709,12 @@ struct TrackJob {
 // Step s of several systems (each its own tracker state, scratch and
 // pyramids): a 2-D grid whose y index is the system; each system's workgroups
 // run exactly the single launch's code.
+template <bool kHalf>
 __global__ __launch_bounds__(kThreads) void k_track_step_batch(const TrackJob *__restrict__ jobs) {
     __shared__ float s_wave[kWaves][kVals];
     const TrackJob &j = jobs[blockIdx.y];
     if ((int)blockIdx.x >= j.G) return;
-    track_step_wg(j.a, blockIdx.x, j.G, s_wave);
+    track_step_wg<kHalf>(j.a, blockIdx.x, j.G, s_wave);
 }
 
 // The final sums and Gauss-Newton step of each system: workgroup i, system i.
@@ -787,7 +808,8 @@ int lf_track_iteration(const LfTrackLevel &lv, float *scratch, LfTrackState *st,
     if (int rc = check_level(lv)) return rc;
     const ReduceArgs a = step_args(lv, scratch, st, f, icp_weight);
     const int G = grid_for(lv);
-    k_track_step<<<G, kThreads, 0, stream>>>(a);
+    if (lv.half_maps) k_track_step<true><<<G, kThreads, 0, stream>>>(a);
+    else k_track_step<false><<<G, kThreads, 0, stream>>>(a);
     LF_LAUNCH_CHECK();
     k_track_final<<<1, kThreads, 0, stream>>>(a, G);
     LF_LAUNCH_CHECK();
@@ -813,11 +835,12 @@ void lf_track_job_skip(void *job) {
     j.G = 0;
 }
 
-int lf_track_batch_step(const void *jobs, int n, int gmax, hipStream_t stream) {
+int lf_track_batch_step(const void *jobs, int n, int gmax, int half_maps, hipStream_t stream) {
     if (!jobs || n <= 0 || n > 65535) return lf_set_error(LF_ERR_ARG, "bad tracking batch of %d", n);
     const TrackJob *J = static_cast<const TrackJob *>(jobs);
     if (gmax > 0) {
-        k_track_step_batch<<<dim3(gmax, n), kThreads, 0, stream>>>(J);
+        if (half_maps) k_track_step_batch<true><<<dim3(gmax, n), kThreads, 0, stream>>>(J);
+        else k_track_step_batch<false><<<dim3(gmax, n), kThreads, 0, stream>>>(J);
         LF_LAUNCH_CHECK();
     }
     k_track_final_batch<<<n, kThreads, 0, stream>>>(J);

@@ -8,6 +8,7 @@
 // get_rendered_scene_textured (system.cpp:132-146).  Arithmetic in the
 // reference's operation order (the HIP files are built with -ffp-contract=off).
 #include "lf_common.h"
+#include "lf_internal.h"
 
 using namespace lf;
 
@@ -31,13 +32,22 @@ LF_DEV uchar4 render_point(f3 point, f3 normal, f3 pixel, f3 light_pos) {
                        (unsigned char)(__saturatef(colour.z) * 255.f), 255);
 }
 
+LF_DEV float4 half4_at(const void *p, size_t i) {
+    const uint2 u = static_cast<const uint2 *>(p)[i];
+    return make_float4(half_bits_to_float((unsigned short)u.x), half_bits_to_float((unsigned short)(u.x >> 16)),
+                       half_bits_to_float((unsigned short)u.y), half_bits_to_float((unsigned short)(u.y >> 16)));
+}
+
 // renderSceneK / renderSceneTexturedK (cuda_imgproc.cu:50-108); rgb null: plain.
-__global__ void k_render(const float4 *__restrict__ vmap, const float4 *__restrict__ nmap,
+// kHalf: the maps are 4 x FP16.
+template <bool kHalf>
+__global__ void k_render(const void *__restrict__ vmap, const void *__restrict__ nmap,
                          const unsigned char *__restrict__ rgb, int cols, int rows, uchar4 *__restrict__ out) {
     const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= cols || y >= rows) return;
     const size_t i = (size_t)y * cols + x;
-    const float4 v = vmap[i], n = nmap[i];
+    const float4 v = kHalf ? half4_at(vmap, i) : static_cast<const float4 *>(vmap)[i];
+    const float4 n = kHalf ? half4_at(nmap, i) : static_cast<const float4 *>(nmap)[i];
     f3 pixel = {1.f, 1.f, 1.f};
     if (rgb) pixel = f3{(float)rgb[3 * i] / 255.f, (float)rgb[3 * i + 1] / 255.f, (float)rgb[3 * i + 2] / 255.f};
     out[i] = render_point(f3{v.x, v.y, v.z}, f3{n.x, n.y, n.z}, pixel, f3{5.f, 5.f, 5.f});
@@ -84,14 +94,19 @@ dim3 grid16(int w, int h) { return dim3((w + 15) / 16, (h + 15) / 16); }
 
 }  // namespace
 
-extern "C" int lf_render_scene(const float *vmap, const float *nmap, const unsigned char *rgb, int cols, int rows,
-                               unsigned char *rgba_out, void *stream) {
+int lf_render_scene_maps(const void *vmap, const void *nmap, int half_maps, const unsigned char *rgb, int cols,
+                         int rows, unsigned char *rgba_out, hipStream_t st) {
     if (!vmap || !nmap || !rgba_out || cols <= 0 || rows <= 0) return lf_set_error(LF_ERR_ARG, "bad render arguments");
-    k_render<<<grid16(cols, rows), 256, 0, lf_stream(stream)>>>(reinterpret_cast<const float4 *>(vmap),
-                                                               reinterpret_cast<const float4 *>(nmap), rgb, cols,
-                                                               rows, reinterpret_cast<uchar4 *>(rgba_out));
+    uchar4 *out = reinterpret_cast<uchar4 *>(rgba_out);
+    if (half_maps) k_render<true><<<grid16(cols, rows), 256, 0, st>>>(vmap, nmap, rgb, cols, rows, out);
+    else k_render<false><<<grid16(cols, rows), 256, 0, st>>>(vmap, nmap, rgb, cols, rows, out);
     LF_LAUNCH_CHECK();
     return LF_OK;
+}
+
+extern "C" int lf_render_scene(const float *vmap, const float *nmap, const unsigned char *rgb, int cols, int rows,
+                               unsigned char *rgba_out, void *stream) {
+    return lf_render_scene_maps(vmap, nmap, 0, rgb, cols, rows, rgba_out, lf_stream(stream));
 }
 
 extern "C" int lf_semi_dense(const float *image, const float *dx, const float *dy, int cols, int rows, float th_dx,

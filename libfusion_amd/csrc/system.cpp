@@ -156,7 +156,8 @@ void GpuImage::create(int c, int r, int e) {
 
 // ------------------------------------------------------------ DeviceImage
 
-DeviceImage::DeviceImage(const IntrinsicMatrix &K, int num_pyr, hipStream_t stream) : stream_(stream) {
+DeviceImage::DeviceImage(const IntrinsicMatrix &K, int num_pyr, hipStream_t stream, bool half_maps)
+    : stream_(stream), half_maps_(half_maps) {
     if (num_pyr < 1 || num_pyr > 8) throw std::invalid_argument("DeviceImage: 1 to 8 pyramid levels");
     BuildIntrinsicPyramid(K, K_, num_pyr);
     depth_raw_.create(K.width, K.height, 4);
@@ -167,8 +168,9 @@ DeviceImage::DeviceImage(const IntrinsicMatrix &K, int num_pyr, hipStream_t stre
     vmap_.resize(num_pyr); nmap_.resize(num_pyr);
     for (int l = 0; l < num_pyr; ++l) {
         const int w = K_[l].width, h = K_[l].height;
-        depth_[l].create(w, h, 4); intensity_[l].create(w, h, 4); dx_[l].create(w, h, 4); dy_[l].create(w, h, 4);
-        vmap_[l].create(w, h, 16); nmap_[l].create(w, h, 16);
+        const int e1 = half_maps ? 2 : 4, e4 = 4 * e1;  // FP16 or FP32 components
+        depth_[l].create(w, h, 4); intensity_[l].create(w, h, e1); dx_[l].create(w, h, e1); dy_[l].create(w, h, e1);
+        vmap_[l].create(w, h, e4); nmap_[l].create(w, h, e4);
     }
 }
 
@@ -177,6 +179,7 @@ LfImageLevels DeviceImage::levels_view() {
     m.depth_raw = depth_raw_.ptr<float>();
     m.rgb = image_.ptr<unsigned char>();
     m.samples = samples_.ptr<float>();
+    m.half_maps = half_maps_ ? 1 : 0;
     for (int l = 0; l < levels(); ++l) {
         m.depth[l] = depth_[l].ptr<float>();
         m.intensity[l] = intensity_[l].ptr<float>();
@@ -215,19 +218,20 @@ int DeviceImage::plan_refresh(std::vector<unsigned char> &out) {
 }
 
 const GpuImage &DeviceImage::get_rendered_image() {
-    lf_render_scene(vmap_[0].ptr<float>(), nmap_[0].ptr<float>(), nullptr, K_[0].width, K_[0].height,
-                    rendered_.ptr<unsigned char>(), stream_);
+    lf_render_scene_maps(vmap_[0].ptr<void>(), nmap_[0].ptr<void>(), half_maps_, nullptr, K_[0].width, K_[0].height,
+                         rendered_.ptr<unsigned char>(), stream_);
     return rendered_;
 }
 
 const GpuImage &DeviceImage::get_rendered_scene_textured() {
-    lf_render_scene(vmap_[0].ptr<float>(), nmap_[0].ptr<float>(), image_.ptr<unsigned char>(), K_[0].width,
-                    K_[0].height, rendered_.ptr<unsigned char>(), stream_);
+    lf_render_scene_maps(vmap_[0].ptr<void>(), nmap_[0].ptr<void>(), half_maps_, image_.ptr<unsigned char>(),
+                         K_[0].width, K_[0].height, rendered_.ptr<unsigned char>(), stream_);
     return rendered_;
 }
 
 int DeviceImage::set_vmap(const float *vmap) {
     if (!vmap) return lf_set_error(LF_ERR_ARG, "null vertex map");
+    if (half_maps_) return lf_set_error(LF_ERR_ARG, "set_vmap takes FP32 images only");
     const size_t n = (size_t)K_[0].width * K_[0].height;
     LF_HIP_CHECK(hipMemcpyAsync(vmap_[0].ptr<float>(), vmap, n * 16, hipMemcpyDeviceToDevice, stream_));
     return resize_device_map();
@@ -246,6 +250,7 @@ int DeviceImage::intensity_pyramid() {
 
 int DeviceImage::set_intensity(const float *intensity) {
     if (!intensity) return lf_set_error(LF_ERR_ARG, "null intensity");
+    if (half_maps_) return lf_set_error(LF_ERR_ARG, "set_intensity takes FP32 images only");
     const size_t n = (size_t)K_[0].width * K_[0].height;
     LF_HIP_CHECK(hipMemcpyAsync(intensity_[0].ptr<float>(), intensity, n * 4, hipMemcpyDeviceToDevice, stream_));
     return intensity_pyramid();
@@ -253,12 +258,14 @@ int DeviceImage::set_intensity(const float *intensity) {
 
 int DeviceImage::set_image(const unsigned char *rgb) {
     if (!rgb) return lf_set_error(LF_ERR_ARG, "null image");
+    if (half_maps_) return lf_set_error(LF_ERR_ARG, "set_image takes FP32 images only");
     LF_RC(lf_rgb_to_intensity(rgb, intensity_[0].ptr<float>(), K_[0].width, K_[0].height, stream_));
     return intensity_pyramid();
 }
 
 int DeviceImage::set_depth(const float *depth) {
     if (!depth) return lf_set_error(LF_ERR_ARG, "null depth");
+    if (half_maps_) return lf_set_error(LF_ERR_ARG, "set_depth takes FP32 images only");
     void *st = stream_;
     LF_RC(lf_bilateral_filter(depth, depth_[0].ptr<float>(), K_[0].width, K_[0].height, st));
     for (int l = 1; l < levels(); ++l)
@@ -409,6 +416,7 @@ int DenseTracking::enqueue(DeviceImage &ref, DeviceImage &cur, const TrackingCon
         lv.cols = K.width;
         lv.rows = K.height;
         lv.K = K;
+        lv.half_maps = cur.half_maps() ? 1 : 0;
         f.first_of_level = plan[s].it == 0;
         f.first_of_frame = s == 0;
         f.last_level = level == finest;
@@ -441,11 +449,11 @@ TrackingResult DenseTracking::result() {
 // ------------------------------------------------------------ DenseOdometry
 
 DenseOdometry::DenseOdometry(const IntrinsicMatrix &base, int num_pyr, std::vector<int> iterations,
-                             hipStream_t stream)
+                             hipStream_t stream, bool half_maps)
     : iterations_(std::move(iterations)), stream_(stream), tracker_(stream) {
     BuildIntrinsicPyramid(base, K_, num_pyr);
-    curr_.reset(new DeviceImage(base, num_pyr, stream));
-    ref_.reset(new DeviceImage(base, num_pyr, stream));
+    curr_.reset(new DeviceImage(base, num_pyr, stream, half_maps));
+    ref_.reset(new DeviceImage(base, num_pyr, stream, half_maps));
 }
 
 const float *DenseOdometry::pose24() const { return tracker_.state()->pose24; }
@@ -515,7 +523,7 @@ System::System(const lf_system_config &cfg) : cfg_(cfg) {
     IntrinsicMatrix K(cfg.K.width, cfg.K.height, cfg.K.fx, cfg.K.fy, cfg.K.cx, cfg.K.cy);
     mapping_.reset(new DenseMapping(K, cfg.map, stream_));
     std::vector<int> its(cfg.iterations, cfg.iterations + std::max(1, std::min(8, cfg.num_pyr)));
-    odometry_.reset(new DenseOdometry(K, cfg.num_pyr, its, stream_));
+    odometry_.reset(new DenseOdometry(K, cfg.num_pyr, its, stream_, cfg.map.precision == LF_PRECISION_MIXED));
 }
 
 System::~System() {
@@ -618,7 +626,12 @@ Batch::Batch(const lf_system_config *cfgs, int n) {
         release();
         throw std::bad_alloc();
     };
-    for (int i = 0; i < n; ++i) sys_.emplace_back(new System(cfgs[i]));
+    for (int i = 0; i < n; ++i) {
+        // batched launches run one map format
+        if (cfgs[i].map.precision != cfgs[0].map.precision) throw std::invalid_argument("Batch: mixed precisions");
+        sys_.emplace_back(new System(cfgs[i]));
+    }
+    half_maps_ = n > 0 && cfgs[0].map.precision == LF_PRECISION_MIXED;
     if (hipStreamCreate(&stream_) != hipSuccess || hipStreamCreate(&aux_) != hipSuccess) fail();
     ready_.assign(n, nullptr);
     for (hipEvent_t &e : ready_)
@@ -771,7 +784,8 @@ int Batch::process(const uint16_t *const *depth, const unsigned char *const *rgb
             }
         LF_HIP_CHECK(hipMemcpyAsync(track_dev_[k], host, steps * n * J, hipMemcpyHostToDevice, stream_));
         for (size_t s = 0; s < steps; ++s)
-            LF_RC(lf_track_batch_step(static_cast<unsigned char *>(track_dev_[k]) + s * n * J, n, gmax[s], stream_));
+            LF_RC(lf_track_batch_step(static_cast<unsigned char *>(track_dev_[k]) + s * n * J, n, gmax[s],
+                                      half_maps_, stream_));
     }
     // fusion on the systems' streams once their poses are final
     LF_HIP_CHECK(hipEventRecord(tracked_, stream_));
@@ -1205,8 +1219,11 @@ struct lf_batch {
 
 extern "C" int lf_batch_create(const lf_system_config *cfgs, int n, lf_batch **out) {
     if (!cfgs || !out || n <= 0) return lf_set_error(LF_ERR_ARG, "bad arguments to lf_batch_create");
-    for (int i = 0; i < n; ++i)
+    for (int i = 0; i < n; ++i) {
         if (int rc = check_halving(&cfgs[i].K, cfgs[i].num_pyr)) return rc;
+        if (cfgs[i].map.precision != cfgs[0].map.precision)
+            return lf_set_error(LF_ERR_ARG, "the systems of a batch share one precision (system %d differs)", i);
+    }
     try {
         *out = new lf_batch{new fusion::Batch(cfgs, n)};
     } catch (const std::exception &e) {

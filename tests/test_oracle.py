@@ -91,6 +91,36 @@ def test_block_directory_cells_are_a_bijection_in_range():
         assert lib.lf_dir_cell(*p) == -1
 
 
+def test_weight_codes_are_the_nearest_code_ties_to_even():
+    """8-bit weights of a mixed-precision map: the 256 code values follow the
+    closed form (m * 2^-10 below 2^-6, (1 + m/16) * 2^(e-7) above), and
+    lf_weight_code is the nearest code by brute force over all 256 values,
+    ties to the even code, at random weights, at every midpoint between
+    adjacent codes and one float step either side of it, and at the ends."""
+    lib = capi.load()
+    vals = capi.weight_values().astype(np.float64)
+    e, m = np.arange(256) >> 4, np.arange(256) & 15
+    closed = np.where(e == 0, m * 2.0 ** -10, (1 + m / 16) * 2.0 ** (e - 7.0))
+    assert np.array_equal(vals, closed)
+    assert np.all(np.diff(vals) > 0) and vals[0] == 0 and vals[-1] == 496
+
+    def nearest(x):
+        d = np.abs(vals - x)
+        best = np.flatnonzero(d == d.min())
+        return int(best[0]) if len(best) == 1 else int(best[best % 2 == 0][0])
+
+    rng = np.random.default_rng(11)
+    xs = list(np.float32(rng.uniform(0, 496, 3000))) + list(np.float32(np.exp(rng.uniform(-12, 6, 3000))))
+    mids = (vals[1:] + vals[:-1]) / 2
+    for md in mids.astype(np.float32):
+        xs += [md, np.nextafter(md, np.float32(0)), np.nextafter(md, np.float32(1000))]
+    xs += [np.float32(0), np.float32(255), np.float32(496), np.float32(1e-30)]
+    for x in xs:
+        x = np.float32(x)
+        assert lib.lf_weight_code(float(x)) == nearest(np.float64(x)), x
+    assert lib.lf_weight_code(1e9) == 255 and lib.lf_weight_code(-1.0) == 0
+
+
 def test_mc_tables_digest_and_consistency():
     edge, tri = capi.mc_tables()
     assert hashlib.sha256(struct.pack("<256i", *edge.tolist())).hexdigest() == EDGE_SHA

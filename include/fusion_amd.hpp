@@ -184,6 +184,9 @@ public:
     // jobs, its grid to grids.  A plan without steps is launched at once.
     int plan(DeviceImage &reference, DeviceImage &current, const TrackingContext &c,
              std::vector<unsigned char> &jobs, std::vector<int> &grids);
+    // Planned steps of a tracker that shares its launches with share - 1
+    // others take 1/share of the reduction grid (a Batch sets it).
+    void set_batch_share(int share) { batch_share_ = share < 1 ? 1 : share; }
     // compute_transform(ctx) (icp_tracker.cpp:210-314) on the set_* pyramids
     // (source = current frame, reference = last); success is unconditional.
     int compute_transform(const TrackingContext &c);
@@ -212,6 +215,7 @@ private:
     LfTrackState *state_ = nullptr;
     float *scratch_ = nullptr;
     bool context_mode_ = false;
+    int batch_share_ = 1;
     std::unique_ptr<DeviceImage> src_, ref_;
 };
 
@@ -319,8 +323,7 @@ private:
 // steps of all systems on the batch's stream, step s of every system in one
 // launch pair; fuses each map on its own stream; runs one batched z-range +
 // raycast launch per stage for all maps; then the re-pyramids, again one
-// launch per pass for all systems.  Each system's results equal those of the
-// same system run alone; the batched launches replace 2 x 18 tracking
+// launch per pass for all systems.  The batched launches replace 2 x 18 tracking
 // launches and 6 pyramid launches per system and pack the raycasts' long-ray
 // tails onto one grid.
 class Batch {

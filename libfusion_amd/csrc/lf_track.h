@@ -68,10 +68,13 @@ int lf_track_set_pose(LfTrackState *st, const double *T12, hipStream_t stream);
 // lf_track_job_make fills one and reports its grid, lf_track_job_skip marks a
 // system without a step at this index.  lf_track_batch_step runs step jobs[0..n)
 // (device memory) with a grid of gmax workgroups per system (the largest job
-// grid); each system's results equal lf_track_iteration's.  The jobs of one
-// batch share one map format (half_maps as in their LfTrackLevel).
+// grid).  A job made with share n takes 1/n of lf_track_iteration's
+// workgroups (at least one), so n systems together fill the resident grid
+// once; each lane then sums several pixels, and the system's sums equal
+// lf_track_iteration's up to FP32 summation order (exactly, for share 1).
+// The jobs of one batch share one map format (half_maps as in their LfTrackLevel).
 size_t lf_track_job_bytes(void);
 int lf_track_job_make(const LfTrackLevel &lv, float *scratch, LfTrackState *st, const LfStepFlags &f,
-                      float icp_weight, void *job, int *grid);
+                      float icp_weight, int share, void *job, int *grid);
 void lf_track_job_skip(void *job);
 int lf_track_batch_step(const void *jobs, int n, int gmax, int half_maps, hipStream_t stream);

@@ -425,7 +425,8 @@ int DenseTracking::enqueue(DeviceImage &ref, DeviceImage &cur, const TrackingCon
             const size_t J = lf_track_job_bytes();
             jobs->resize(jobs->size() + J);
             int G = 0;
-            LF_RC(lf_track_job_make(lv, scratch_, state_, f, icp_weight, jobs->data() + jobs->size() - J, &G));
+            LF_RC(lf_track_job_make(lv, scratch_, state_, f, icp_weight, batch_share_, jobs->data() + jobs->size() - J,
+                                    &G));
             grids->push_back(G);
         } else {
             LF_RC(lf_track_iteration(lv, scratch_, state_, f, icp_weight, stream_));
@@ -630,6 +631,7 @@ Batch::Batch(const lf_system_config *cfgs, int n) {
         // batched launches run one map format
         if (cfgs[i].map.precision != cfgs[0].map.precision) throw std::invalid_argument("Batch: mixed precisions");
         sys_.emplace_back(new System(cfgs[i]));
+        sys_.back()->odometry().tracker().set_batch_share(n);
     }
     half_maps_ = n > 0 && cfgs[0].map.precision == LF_PRECISION_MIXED;
     if (hipStreamCreate(&stream_) != hipSuccess || hipStreamCreate(&aux_) != hipSuccess) fail();

@@ -102,8 +102,9 @@ def test_mixed_sequence_vs_fp64_oracle():
 
 def test_mixed_batch_equals_mixed_systems_alone():
     """Two mixed-precision systems in one lf_batch (batched FP16 pyramids,
-    tracking and raycast launches) end with exactly the poses and counters of
-    the same systems run alone."""
+    tracking and raycast launches; each system's reductions on half a lone
+    system's grid) end within FP32 summation-order noise of the same systems
+    run alone: poses within 0.02 mm / 0.002 deg, counters within 0.5%."""
     scenes = [Scene(1, 1), Scene(1, 7)]
     K = half_res(scenes[0])
     p = capi.map_params(2 ** 18, 1 << 17, 0.01, 0.03, precision=MIXED)
@@ -121,11 +122,13 @@ def test_mixed_batch_equals_mixed_systems_alone():
         batch.process([fs[f][2] for fs in frames], [fs[f][3] for fs in frames])
     batch.synchronize()
     for i, (pose, trk, cnt) in enumerate(alone):
-        assert np.array_equal(batch.pose(i), pose), i
+        dt, dr = pose_err(batch.pose(i), pose)
+        assert dt <= 2e-5 and dr <= 2e-3, (i, dt, dr)
         assert batch.tracking(i)[0] == trk[0], i
         c = batch.counters(i)
-        for k in (capi.CNT_ALLOCATED, capi.CNT_UPDATES, capi.CNT_VIS_SUM_LO, capi.CNT_RENDER_BLOCKS):
-            assert c[k] == cnt[k], (i, k)
+        assert c[capi.CNT_UPDATES] == cnt[capi.CNT_UPDATES], i
+        for k in (capi.CNT_ALLOCATED, capi.CNT_RENDER_BLOCKS):
+            assert abs(int(c[k]) - int(cnt[k])) <= 0.005 * cnt[k], (i, k, c[k], cnt[k])
     batch.close()
 
 

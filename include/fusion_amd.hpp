@@ -341,23 +341,19 @@ private:
     std::vector<std::unique_ptr<System>> sys_;
     hipStream_t stream_ = nullptr;
     std::vector<hipEvent_t> ready_;  // per system: its tracking and fusion are enqueued
-    hipStream_t aux_ = nullptr;      // uploads of the next step's frames
+    hipStream_t aux_ = nullptr;      // the step's descriptor table and uploads
     hipEvent_t uploaded_ = nullptr;  // the step's uploads
-    void *jobs_host_[2] = {nullptr, nullptr}, *jobs_dev_[2] = {nullptr, nullptr};
-    hipEvent_t jobs_read_[2] = {nullptr, nullptr};
-    bool jobs_used_[2] = {false, false};
-    int parity_ = 0;
-    // tracking jobs: [step][system], double-buffered like the raycast jobs
-    void *track_host_[2] = {nullptr, nullptr}, *track_dev_[2] = {nullptr, nullptr};
-    size_t track_cap_ = 0;  // jobs per buffer
     hipEvent_t tracked_ = nullptr;
     bool tracked_once_ = false;
     bool half_maps_ = false;  // the systems are mixed precision
-    int reserve_track(size_t jobs);
-    // frame passes: [upload pass][system] then [re-pyramid pass][system]
-    void *pass_host_[2] = {nullptr, nullptr}, *pass_dev_[2] = {nullptr, nullptr};
-    size_t pass_cap_ = 0;  // bytes per buffer
-    int reserve_pass(size_t bytes);
+    int parity_ = 0;
+    // the step's descriptors (frame passes, tracking jobs, raycast jobs),
+    // double-buffered in pinned host and device memory
+    void *table_host_[2] = {nullptr, nullptr}, *table_dev_[2] = {nullptr, nullptr};
+    hipEvent_t table_read_[2] = {nullptr, nullptr};
+    bool table_used_[2] = {false, false};
+    size_t table_cap_ = 0;  // bytes per buffer
+    int reserve_table(size_t bytes);
 };
 
 }  // namespace fusion

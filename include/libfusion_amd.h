@@ -498,11 +498,15 @@ int lf_system_read_timing(lf_system *sys, int stage, int *launches, double *tota
  * reference, which runs one System per process; for callers that serve many
  * RGB-D streams on one GPU).  System i is made from cfgs[i] (sizes and voxel
  * sizes may differ).  lf_batch_process takes one frame per system (device
- * pointers, uint16 depth and uint8 RGB of system i's size): each system's
- * tracking and fusion run on its own stream, then one batched z-range +
- * raycast launch per stage covers every map, then each system's re-pyramid.
- * Each system's poses, tracking results and volume equal those of the same
- * system run alone.  Asynchronous; lf_batch_synchronize waits for all. */
+ * pointers, uint16 depth and uint8 RGB of system i's size): pass p of every
+ * system's image pyramid runs in one launch, Gauss-Newton step s of every
+ * system in one launch pair, each map is fused on its own stream, then one
+ * batched z-range + raycast launch per stage covers every map, then the
+ * re-pyramids.  All systems share one precision (LF_ERR_ARG otherwise).  The
+ * tracking reductions of n systems take 1/n of a lone system's grid each, so
+ * each system's poses, tracking results and volume equal those of the same
+ * system run alone up to FP32 summation order (exactly when n is 1).
+ * Asynchronous; lf_batch_synchronize waits for all. */
 typedef struct lf_batch lf_batch;
 int lf_batch_create(const lf_system_config *cfgs, int n, lf_batch **out);
 int lf_batch_destroy(lf_batch *b);

@@ -84,3 +84,11 @@ size_t lf_ray_job_bytes(void);
 // result equals lf_raycast_ev's.
 int lf_raycast_batch_ev(const LfRayItem *items, int n, void *jobs_host, void *jobs_dev, int check_colour,
                         hipStream_t st);
+// The same in two parts: the n descriptors written to jobs_host with the
+// launch dimensions, then (once a copy of them is in device memory) the launches.
+struct LfRayBatchDims {
+    int n, mixed, tiles_max, zgrid_max, grid_max;
+    size_t zrange_lds;
+};
+int lf_raycast_batch_make(const LfRayItem *items, int n, void *jobs_host, LfRayBatchDims *dims);
+int lf_raycast_batch_launch(const void *jobs_dev, const LfRayBatchDims &d, int check_colour, hipStream_t st);

@@ -528,12 +528,13 @@ size_t lf_ray_job_bytes(void) { return sizeof(RayJob); }
 // raycast's workgroups run exactly the single-launch code, so its tiles and
 // vertex map equal those of lf_raycast_ev; the batch packs the raycasts'
 // long-ray tails onto one grid instead of leaving each tail alone on the chip.
-int lf_raycast_batch_ev(const LfRayItem *items, int n, void *jobs_host, void *jobs_dev, int check_colour,
-                        hipStream_t st) {
-    if (!items || n <= 0 || !jobs_host || !jobs_dev) return lf_set_error(LF_ERR_ARG, "bad raycast batch arguments");
+int lf_raycast_batch_make(const LfRayItem *items, int n, void *jobs_host, LfRayBatchDims *dims) {
+    if (!items || n <= 0 || !jobs_host || !dims) return lf_set_error(LF_ERR_ARG, "bad raycast batch arguments");
     if (n > 65535) return lf_set_error(LF_ERR_ARG, "raycast batch of %d exceeds the grid's y extent", n);
     RayJob *J = static_cast<RayJob *>(jobs_host);
-    int tmax = 0, zmax = 0, gmax = 0;
+    LfRayBatchDims d = {};
+    d.n = n;
+    d.mixed = items[0].p && items[0].p->precision == LF_PRECISION_MIXED;
     size_t lds_max = 0;
     for (int i = 0; i < n; ++i) {
         const LfRayItem &it = items[i];
@@ -556,20 +557,29 @@ int lf_raycast_batch_ev(const LfRayItem *items, int n, void *jobs_host, void *jo
         J[i].wg_x = (tx + 1) / 2;
         J[i].wg_y = (ty + 1) / 2;
         const int T = it.s->tiles_x * it.s->tiles_y;
-        tmax = std::max(tmax, T);
-        zmax = std::max(zmax, zrange_grid(it.p));
-        gmax = std::max(gmax, kXcds * ((J[i].wg_x * J[i].wg_y + kXcds - 1) / kXcds));
+        d.tiles_max = std::max(d.tiles_max, T);
+        d.zgrid_max = std::max(d.zgrid_max, zrange_grid(it.p));
+        d.grid_max = std::max(d.grid_max, kXcds * ((J[i].wg_x * J[i].wg_y + kXcds - 1) / kXcds));
         lds_max = std::max(lds_max, 2 * (size_t)T * sizeof(int));
     }
+    d.zrange_lds = lds_max;
+    *dims = d;
+    return LF_OK;
+}
+
+int lf_raycast_batch_launch(const void *jobs_dev, const LfRayBatchDims &d, int check_colour, hipStream_t st) {
+    if (!jobs_dev || d.n <= 0) return lf_set_error(LF_ERR_ARG, "bad raycast batch launch");
     const RayJob *jobs = static_cast<const RayJob *>(jobs_dev);
-    LF_HIP_CHECK(hipMemcpyAsync(jobs_dev, jobs_host, (size_t)n * sizeof(RayJob), hipMemcpyHostToDevice, st));
-    k_zrange_clear_batch<<<dim3(std::max(1, std::min(1024, (tmax + 255) / 256)), n), 256, 0, st>>>(jobs);
+    const int n = d.n;
+    k_zrange_clear_batch<<<dim3(std::max(1, std::min(1024, (d.tiles_max + 255) / 256)), n), 256, 0, st>>>(jobs);
     LF_LAUNCH_CHECK();
-    if (lds_max <= zrange_lds_limit()) k_zrange_fill_batch<true><<<dim3(zmax, n), kZThreads, lds_max, st>>>(jobs);
-    else k_zrange_fill_batch<false><<<dim3(zmax, n), kZThreads, 0, st>>>(jobs);
+    if (d.zrange_lds <= zrange_lds_limit())
+        k_zrange_fill_batch<true><<<dim3(d.zgrid_max, n), kZThreads, d.zrange_lds, st>>>(jobs);
+    else
+        k_zrange_fill_batch<false><<<dim3(d.zgrid_max, n), kZThreads, 0, st>>>(jobs);
     LF_LAUNCH_CHECK();
-    const dim3 grid(gmax, n);
-    if (items[0].p->precision == LF_PRECISION_MIXED) {
+    const dim3 grid(d.grid_max, n);
+    if (d.mixed) {
         if (check_colour) k_raycast_batch<true, true><<<grid, 256, 0, st>>>(jobs);
         else k_raycast_batch<false, true><<<grid, 256, 0, st>>>(jobs);
     } else {
@@ -578,6 +588,15 @@ int lf_raycast_batch_ev(const LfRayItem *items, int n, void *jobs_host, void *jo
     }
     LF_LAUNCH_CHECK();
     return LF_OK;
+}
+
+int lf_raycast_batch_ev(const LfRayItem *items, int n, void *jobs_host, void *jobs_dev, int check_colour,
+                        hipStream_t st) {
+    LfRayBatchDims d;
+    if (int rc = lf_raycast_batch_make(items, n, jobs_host, &d)) return rc;
+    if (!jobs_dev) return lf_set_error(LF_ERR_ARG, "bad raycast batch arguments");
+    LF_HIP_CHECK(hipMemcpyAsync(jobs_dev, jobs_host, (size_t)n * sizeof(RayJob), hipMemcpyHostToDevice, st));
+    return lf_raycast_batch_launch(jobs_dev, d, check_colour, st);
 }
 
 // raycast_check_visibility (voxel_hashing.cpp:121-133): the blocks visible from

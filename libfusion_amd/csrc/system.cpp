@@ -641,13 +641,8 @@ Batch::Batch(const lf_system_config *cfgs, int n) {
     if (hipEventCreateWithFlags(&uploaded_, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&tracked_, hipEventDisableTiming) != hipSuccess)
         fail();
-    for (int k = 0; k < 2; ++k) {
-        const size_t bytes = (size_t)n * lf_ray_job_bytes();
-        if (hipHostMalloc(&jobs_host_[k], bytes, hipHostMallocDefault) != hipSuccess ||
-            hipMalloc(&jobs_dev_[k], bytes) != hipSuccess ||
-            hipEventCreateWithFlags(&jobs_read_[k], hipEventDisableTiming) != hipSuccess)
-            fail();
-    }
+    for (int k = 0; k < 2; ++k)
+        if (hipEventCreateWithFlags(&table_read_[k], hipEventDisableTiming) != hipSuccess) fail();
 }
 
 Batch::~Batch() { release(); }
@@ -665,17 +660,11 @@ void Batch::release() {
     if (tracked_) (void)hipEventDestroy(tracked_);
     tracked_ = nullptr;
     for (int k = 0; k < 2; ++k) {
-        if (track_host_[k]) (void)hipHostFree(track_host_[k]);
-        if (track_dev_[k]) (void)hipFree(track_dev_[k]);
-        track_host_[k] = track_dev_[k] = nullptr;
-        if (pass_host_[k]) (void)hipHostFree(pass_host_[k]);
-        if (pass_dev_[k]) (void)hipFree(pass_dev_[k]);
-        pass_host_[k] = pass_dev_[k] = nullptr;
-        if (jobs_host_[k]) (void)hipHostFree(jobs_host_[k]);
-        if (jobs_dev_[k]) (void)hipFree(jobs_dev_[k]);
-        if (jobs_read_[k]) (void)hipEventDestroy(jobs_read_[k]);
-        jobs_host_[k] = jobs_dev_[k] = nullptr;
-        jobs_read_[k] = nullptr;
+        if (table_host_[k]) (void)hipHostFree(table_host_[k]);
+        if (table_dev_[k]) (void)hipFree(table_dev_[k]);
+        if (table_read_[k]) (void)hipEventDestroy(table_read_[k]);
+        table_host_[k] = table_dev_[k] = nullptr;
+        table_read_[k] = nullptr;
     }
     if (stream_) (void)hipStreamDestroy(stream_);
     stream_ = nullptr;
@@ -684,111 +673,102 @@ void Batch::release() {
     sys_.clear();
 }
 
-// Room for `jobs` tracking jobs per buffer.  Growing waits for the batch
-// stream, so no launch still reads the old buffers.
-int Batch::reserve_track(size_t jobs) {
-    if (jobs <= track_cap_) return LF_OK;
-    LF_HIP_CHECK(hipStreamSynchronize(stream_));
-    const size_t cap = std::max(jobs, 2 * track_cap_), bytes = cap * lf_track_job_bytes();
-    for (int k = 0; k < 2; ++k) {
-        if (track_host_[k]) LF_HIP_CHECK(hipHostFree(track_host_[k]));
-        if (track_dev_[k]) LF_HIP_CHECK(hipFree(track_dev_[k]));
-        track_host_[k] = track_dev_[k] = nullptr;
-        LF_HIP_CHECK(hipHostMalloc(&track_host_[k], bytes, hipHostMallocDefault));
-        LF_HIP_CHECK(hipMalloc(&track_dev_[k], bytes));
-    }
-    track_cap_ = cap;
-    return LF_OK;
-}
-
-// Room for `bytes` of frame passes per buffer.  The pass tables are read on
+// Room for `bytes` of step descriptors per buffer.  The tables are read on
 // the upload stream and on the batch stream: growing waits for both.
-int Batch::reserve_pass(size_t bytes) {
-    if (bytes <= pass_cap_) return LF_OK;
+int Batch::reserve_table(size_t bytes) {
+    if (bytes <= table_cap_) return LF_OK;
     LF_HIP_CHECK(hipStreamSynchronize(aux_));
     LF_HIP_CHECK(hipStreamSynchronize(stream_));
-    const size_t cap = std::max(bytes, 2 * pass_cap_);
+    const size_t cap = std::max(bytes, 2 * table_cap_);
     for (int k = 0; k < 2; ++k) {
-        if (pass_host_[k]) LF_HIP_CHECK(hipHostFree(pass_host_[k]));
-        if (pass_dev_[k]) LF_HIP_CHECK(hipFree(pass_dev_[k]));
-        pass_host_[k] = pass_dev_[k] = nullptr;
-        LF_HIP_CHECK(hipHostMalloc(&pass_host_[k], cap, hipHostMallocDefault));
-        LF_HIP_CHECK(hipMalloc(&pass_dev_[k], cap));
+        if (table_host_[k]) LF_HIP_CHECK(hipHostFree(table_host_[k]));
+        if (table_dev_[k]) LF_HIP_CHECK(hipFree(table_dev_[k]));
+        table_host_[k] = table_dev_[k] = nullptr;
+        LF_HIP_CHECK(hipHostMalloc(&table_host_[k], cap, hipHostMallocDefault));
+        LF_HIP_CHECK(hipMalloc(&table_dev_[k], cap));
     }
-    pass_cap_ = cap;
+    table_cap_ = cap;
     return LF_OK;
 }
 
 int Batch::process(const uint16_t *const *depth, const unsigned char *const *rgb) {
     const int n = size();
-    const size_t PB = lf_pass_bytes();
+    const size_t PB = lf_pass_bytes(), J = lf_track_job_bytes(), RJ = lf_ray_job_bytes();
     // every system's upload and re-pyramid planned as frame passes, its
     // Gauss-Newton steps as jobs
     std::vector<std::vector<unsigned char>> jobs(n), up(n), after(n);
     std::vector<std::vector<int>> grids(n);
     size_t steps = 0, n_up = 0, n_after = 0;
     for (int i = 0; i < n; ++i) {
+        DenseOdometry &od = sys_[i]->odometry();
+        const bool tracks = od.initialized() && !od.trackingLost;
         LF_RC(sys_[i]->plan_tracking(depth[i], rgb[i], jobs[i], grids[i], &up[i]));
         LF_RC(sys_[i]->plan_after_raycast(after[i]));
         steps = std::max(steps, grids[i].size());
         n_up = std::max(n_up, up[i].size() / PB);
         n_after = std::max(n_after, after[i].size() / PB);
-        LF_HIP_CHECK(hipEventRecord(ready_[i], sys_[i]->stream()));
-        LF_HIP_CHECK(hipStreamWaitEvent(stream_, ready_[i], 0));
+        // planning launches on the system's stream only to start a frame
+        // without steps (first frame: the initial pose); the batch's tracking
+        // waits for that, and needs no event otherwise
+        if (!tracks || grids[i].empty()) {
+            LF_HIP_CHECK(hipEventRecord(ready_[i], sys_[i]->stream()));
+            LF_HIP_CHECK(hipStreamWaitEvent(stream_, ready_[i], 0));
+        }
     }
-    // the pinned buffers of two steps ago must have been copied before they
-    // are rewritten
-    const int k = parity_;
-    if (jobs_used_[k]) LF_HIP_CHECK(hipEventSynchronize(jobs_read_[k]));
-    // frame passes: pass p of every system in one launch
+    // One table per step, copied in one transfer: [pass p][system] (uploads,
+    // then re-pyramids), [step s][system] tracking jobs, [system] raycasts.
     const size_t n_pass = n_up + n_after;
-    std::vector<int> wmax(n_pass, 0);
-    unsigned char *pass_dev = nullptr;
-    if (n_pass) {
-        LF_RC(reserve_pass(n_pass * n * PB));
-        unsigned char *host = static_cast<unsigned char *>(pass_host_[k]);
-        pass_dev = static_cast<unsigned char *>(pass_dev_[k]);
-        for (size_t p = 0; p < n_pass; ++p)
-            for (int i = 0; i < n; ++i) {
-                const std::vector<unsigned char> &src = p < n_up ? up[i] : after[i];
-                const size_t q = p < n_up ? p : p - n_up;
-                unsigned char *dst = host + (p * n + i) * PB;
-                if (q < src.size() / PB) {
-                    std::memcpy(dst, src.data() + q * PB, PB);
-                    wmax[p] = std::max(wmax[p], lf_pass_workgroups(dst));
-                } else {
-                    lf_pass_skip(dst);
-                }
+    const size_t off_track = n_pass * n * PB, off_ray = off_track + steps * n * J, bytes = off_ray + n * RJ;
+    // the buffers of two steps ago must have been copied before they are rewritten
+    const int k = parity_;
+    if (table_used_[k]) LF_HIP_CHECK(hipEventSynchronize(table_read_[k]));
+    LF_RC(reserve_table(bytes));
+    unsigned char *host = static_cast<unsigned char *>(table_host_[k]);
+    unsigned char *dev = static_cast<unsigned char *>(table_dev_[k]);
+    std::vector<int> wmax(n_pass, 0), gmax(steps, 0);
+    for (size_t p = 0; p < n_pass; ++p)
+        for (int i = 0; i < n; ++i) {
+            const std::vector<unsigned char> &src = p < n_up ? up[i] : after[i];
+            const size_t q = p < n_up ? p : p - n_up;
+            unsigned char *dst = host + (p * n + i) * PB;
+            if (q < src.size() / PB) {
+                std::memcpy(dst, src.data() + q * PB, PB);
+                wmax[p] = std::max(wmax[p], lf_pass_workgroups(dst));
+            } else {
+                lf_pass_skip(dst);
             }
-        // the uploads rewrite the images that the previous step's tracking
-        // read; the previous step's fusion, raycast and re-pyramid use the
-        // other image of each system, so the uploads run beside them
-        if (tracked_once_) LF_HIP_CHECK(hipStreamWaitEvent(aux_, tracked_, 0));
-        LF_HIP_CHECK(hipMemcpyAsync(pass_dev, host, n_pass * n * PB, hipMemcpyHostToDevice, aux_));
-        for (size_t p = 0; p < n_up; ++p) LF_RC(lf_pass_batch(pass_dev + p * n * PB, n, wmax[p], aux_));
-        LF_HIP_CHECK(hipEventRecord(uploaded_, aux_));
-        LF_HIP_CHECK(hipStreamWaitEvent(stream_, uploaded_, 0));
-    }
-    if (steps) {
-        LF_RC(reserve_track(steps * n));
-        const size_t J = lf_track_job_bytes();
-        unsigned char *host = static_cast<unsigned char *>(track_host_[k]);
-        std::vector<int> gmax(steps, 0);
-        for (size_t s = 0; s < steps; ++s)
-            for (int i = 0; i < n; ++i) {
-                unsigned char *dst = host + (s * n + i) * J;
-                if (s < grids[i].size()) {
-                    std::memcpy(dst, jobs[i].data() + s * J, J);
-                    gmax[s] = std::max(gmax[s], grids[i][s]);
-                } else {
-                    lf_track_job_skip(dst);
-                }
+        }
+    for (size_t s = 0; s < steps; ++s)
+        for (int i = 0; i < n; ++i) {
+            unsigned char *dst = host + off_track + (s * n + i) * J;
+            if (s < grids[i].size()) {
+                std::memcpy(dst, jobs[i].data() + s * J, J);
+                gmax[s] = std::max(gmax[s], grids[i][s]);
+            } else {
+                lf_track_job_skip(dst);
             }
-        LF_HIP_CHECK(hipMemcpyAsync(track_dev_[k], host, steps * n * J, hipMemcpyHostToDevice, stream_));
-        for (size_t s = 0; s < steps; ++s)
-            LF_RC(lf_track_batch_step(static_cast<unsigned char *>(track_dev_[k]) + s * n * J, n, gmax[s],
-                                      half_maps_, stream_));
+        }
+    std::vector<LfRayItem> items(n);
+    for (int i = 0; i < n; ++i) {
+        DenseMapping &m = sys_[i]->mapping();
+        items[i] = LfRayItem{&m.params(), &m.storage(), sys_[i]->raycast_vmap(), nullptr,
+                             sys_[i]->raycast_pose24(), &m.intrinsics(), m.intrinsics().width,
+                             m.intrinsics().height};
     }
+    LfRayBatchDims ray;
+    LF_RC(lf_raycast_batch_make(items.data(), n, host + off_ray, &ray));
+    // Upload stream: the table, then the uploads.  The uploads rewrite the
+    // images that the previous step's tracking read (the previous step's
+    // fusion, raycast and re-pyramid use the other image of each system), and
+    // the table buffer was last read by the step before that: both wait only
+    // for the previous step's tracking, so they run beside its fusion and raycast.
+    if (tracked_once_) LF_HIP_CHECK(hipStreamWaitEvent(aux_, tracked_, 0));
+    LF_HIP_CHECK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, aux_));
+    for (size_t p = 0; p < n_up; ++p) LF_RC(lf_pass_batch(dev + p * n * PB, n, wmax[p], aux_));
+    LF_HIP_CHECK(hipEventRecord(uploaded_, aux_));
+    LF_HIP_CHECK(hipStreamWaitEvent(stream_, uploaded_, 0));
+    for (size_t s = 0; s < steps; ++s)
+        LF_RC(lf_track_batch_step(dev + off_track + s * n * J, n, gmax[s], half_maps_, stream_));
     // fusion on the systems' streams once their poses are final
     LF_HIP_CHECK(hipEventRecord(tracked_, stream_));
     tracked_once_ = true;
@@ -798,18 +778,11 @@ int Batch::process(const uint16_t *const *depth, const unsigned char *const *rgb
         LF_HIP_CHECK(hipEventRecord(ready_[i], sys_[i]->stream()));
         LF_HIP_CHECK(hipStreamWaitEvent(stream_, ready_[i], 0));
     }
-    std::vector<LfRayItem> items(n);
-    for (int i = 0; i < n; ++i) {
-        DenseMapping &m = sys_[i]->mapping();
-        items[i] = LfRayItem{&m.params(), &m.storage(), sys_[i]->raycast_vmap(), nullptr,
-                             sys_[i]->raycast_pose24(), &m.intrinsics(), m.intrinsics().width,
-                             m.intrinsics().height};
-    }
-    LF_RC(lf_raycast_batch_ev(items.data(), n, jobs_host_[k], jobs_dev_[k], 1, stream_));
+    LF_RC(lf_raycast_batch_launch(dev + off_ray, ray, 1, stream_));
     // re-pyramids of the raycast maps
-    for (size_t p = n_up; p < n_pass; ++p) LF_RC(lf_pass_batch(pass_dev + p * n * PB, n, wmax[p], stream_));
-    LF_HIP_CHECK(hipEventRecord(jobs_read_[k], stream_));
-    jobs_used_[k] = true;
+    for (size_t p = n_up; p < n_pass; ++p) LF_RC(lf_pass_batch(dev + p * n * PB, n, wmax[p], stream_));
+    LF_HIP_CHECK(hipEventRecord(table_read_[k], stream_));
+    table_used_[k] = true;
     parity_ ^= 1;
     return LF_OK;
 }

@@ -23,6 +23,7 @@
 
 struct LfTrackState;
 struct LfImageLevels;
+struct LfFuseItem;
 
 namespace fusion {
 
@@ -295,8 +296,10 @@ public:
     float *raycast_vmap();
     const float *raycast_pose24();
     int enqueue_after_raycast();
-    // enqueue_after_raycast planned as frame passes (after plan_tracking).
+    // enqueue_after_raycast planned as frame passes, and enqueue_fusion as one
+    // map's item of a batched update (both after plan_tracking).
     int plan_after_raycast(std::vector<unsigned char> &passes);
+    int plan_fusion(LfFuseItem &item);
 
 private:
     lf_system_config cfg_;

@@ -65,6 +65,30 @@ int lf_pass_batch(const void *passes, int n, int max_wg, hipStream_t st);
 int lf_render_scene_maps(const void *vmap, const void *nmap, int half_maps, const unsigned char *rgb, int cols,
                          int rows, unsigned char *rgba_out, hipStream_t st);
 
+// One map's weighted update in a batch (the System frame's form: raw depth for
+// block allocation, packed samples for integration, colour or null); a map
+// with enabled = 0 is left alone.  lf_update_batch_make writes n descriptors
+// of lf_fuse_job_bytes() each to jobs_host with the launch dimensions;
+// lf_update_batch_launch runs block allocation, the visible lists and the
+// integration of every map as three launches (y index = map) from a device
+// copy of the descriptors.  Each map's volume equals lf_update_ev's.
+struct LfFuseItem {
+    const lf_map_params *p;
+    const lf_map_storage *s;
+    const float *depth;
+    const float *samples;
+    const unsigned char *rgb;
+    const float *pose24;
+    const lf_intrinsics *K;
+    int cols, rows, enabled;
+};
+struct LfFuseBatchDims {
+    int n, any, mixed, tiles_max, vis_grid_max, integrate_grid;
+};
+size_t lf_fuse_job_bytes(void);
+int lf_update_batch_make(const LfFuseItem *items, int n, void *jobs_host, LfFuseBatchDims *dims);
+int lf_update_batch_launch(const void *jobs_dev, const LfFuseBatchDims &d, hipStream_t st);
+
 // One raycast of a batch: a map, its target vertex map (and colour, or null),
 // the float T_wc / T_cw pose and the image size.
 struct LfRayItem {

@@ -167,18 +167,19 @@ LF_DEV void touch_block(const MapDev &m, int bx, int by, int bz) {
 // kInsert: create_blocks_kernel.  !kInsert (de-integration): the same band
 // walk only marks the blocks that exist, so the visible list of a frame can be
 // rebuilt without allocating.
+// 16x16 pixel tile (tx, ty) of the frame.
 template <bool kInsert>
-__global__ __launch_bounds__(256) void k_alloc(MapDev m, const float *__restrict__ depth, int cols, int rows,
-                                               const float *__restrict__ pose24, lf_intrinsics K) {
-    __shared__ unsigned long long s_set[kDedup];
+LF_DEV void alloc_wg(const MapDev &m, const float *__restrict__ depth, int cols, int rows,
+                     const float *__restrict__ pose24, const lf_intrinsics &K, int tx, int ty,
+                     unsigned long long *s_set) {
     const int tid = threadIdx.x;
     for (int i = tid; i < kDedup; i += 256) s_set[i] = kEmptyKey;
-    if (blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) m.counters[LF_CNT_VISIBLE] = 0u;
+    if (tx == 0 && ty == 0 && tid == 0) m.counters[LF_CNT_VISIBLE] = 0u;
     float Twc[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) Twc[i] = pose24[i];
     __syncthreads();
-    const int x = blockIdx.x * 16 + (tid & 15), y = blockIdx.y * 16 + (tid >> 4);
+    const int x = tx * 16 + (tid & 15), y = ty * 16 + (tid >> 4);
     if (x >= cols || y >= rows) return;
     band_blocks(x, y, depth[(size_t)y * cols + x], Twc, K, m, [&](int bx, int by, int bz) {
         const unsigned long long key = pack_key(bx, by, bz);
@@ -197,6 +198,13 @@ __global__ __launch_bounds__(256) void k_alloc(MapDev m, const float *__restrict
         if (kInsert) insert_block(m, bx, by, bz);
         else touch_block(m, bx, by, bz);
     });
+}
+
+template <bool kInsert>
+__global__ __launch_bounds__(256) void k_alloc(MapDev m, const float *__restrict__ depth, int cols, int rows,
+                                               const float *__restrict__ pose24, lf_intrinsics K) {
+    __shared__ unsigned long long s_set[kDedup];
+    alloc_wg<kInsert>(m, depth, cols, rows, pose24, K, blockIdx.x, blockIdx.y, s_set);
 }
 
 // is_block_visible (depth_fusion.cu:27-47): any of the 8 block corners inside
@@ -225,20 +233,19 @@ constexpr int kVisGrid = 512;      // at most this many workgroups (grid-stride 
 // per workgroup (a wavefront ballot, then the wavefront counts prefixed in
 // LDS): same-address atomics serialise, so one per wavefront cost most of the
 // kernel.  The list order is unspecified, as with the per-wavefront append.
-template <bool kGeometric = false>
-__global__ __launch_bounds__(kVisThreads) void k_visible(MapDev m, const float *__restrict__ pose24, lf_intrinsics K,
-                                                         int cols, int rows) {
-    __shared__ unsigned int s_cnt[kVisThreads / kWave];
-    __shared__ unsigned int s_base;
+// Workgroup wg of nwg.
+template <bool kGeometric>
+LF_DEV void visible_wg(const MapDev &m, const float *__restrict__ pose24, const lf_intrinsics &K, int cols, int rows,
+                       int wg, int nwg, unsigned int *s_cnt, unsigned int &s_base) {
     float Tcw[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) Tcw[i] = pose24[12 + i];
     const unsigned int n = min(m.counters[LF_CNT_ALLOCATED], (unsigned int)m.num_blocks);
     const float scale = 8.0f * m.voxel_size;
-    const unsigned int stride = gridDim.x * blockDim.x;
+    const unsigned int stride = nwg * blockDim.x;
     const int lane = lane_id(), wave = threadIdx.x / kWave;
     // the trip count is uniform over the workgroup (its barriers need every lane)
-    for (unsigned int b0 = blockIdx.x * blockDim.x; b0 < n; b0 += stride) {
+    for (unsigned int b0 = wg * blockDim.x; b0 < n; b0 += stride) {
         const unsigned int b = b0 + threadIdx.x;
         bool vis = false;
         int4 p = make_int4(0, 0, 0, 0);
@@ -273,6 +280,14 @@ __global__ __launch_bounds__(kVisThreads) void k_visible(MapDev m, const float *
     }
 }
 
+template <bool kGeometric = false>
+__global__ __launch_bounds__(kVisThreads) void k_visible(MapDev m, const float *__restrict__ pose24, lf_intrinsics K,
+                                                         int cols, int rows) {
+    __shared__ unsigned int s_cnt[kVisThreads / kWave];
+    __shared__ unsigned int s_base;
+    visible_wg<kGeometric>(m, pose24, K, cols, rows, blockIdx.x, gridDim.x, s_cnt, s_base);
+}
+
 int visible_grid(const lf_map_params *p) {
     return std::min(kVisGrid, std::max(1, (p->num_blocks + kVisThreads - 1) / kVisThreads));
 }
@@ -285,18 +300,17 @@ int visible_grid(const lf_map_params *p) {
 // colour are gathered separately.  kMixed: FP16 sdf and 8-bit weight codes,
 // and the projection and the weighted means use one reciprocal each instead
 // of correctly rounded divisions.
-template <bool kWeighted, bool kColour, bool kPacked = false, bool kMixed = false>
-__global__ __launch_bounds__(256) void k_integrate(MapDev m, const float *__restrict__ depth,
-                                                   const float *__restrict__ nmap,
-                                                   const unsigned char *__restrict__ rgb, int cols, int rows,
-                                                   const float *__restrict__ pose24, lf_intrinsics K,
-                                                   const float4 *__restrict__ samples = nullptr) {
+// Workgroup wg of nwg (grid-stride over the visible list).
+template <bool kWeighted, bool kColour, bool kPacked, bool kMixed>
+LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const float *__restrict__ nmap,
+                         const unsigned char *__restrict__ rgb, int cols, int rows, const float *__restrict__ pose24,
+                         const lf_intrinsics &K, const float4 *__restrict__ samples, int wg, int nwg) {
     static_assert(!kPacked || kWeighted, "packed samples carry the weighted update's weight");
     float Tcw[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) Tcw[i] = pose24[12 + i];
     const int nvis = (int)m.counters[LF_CNT_VISIBLE];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (wg == 0 && threadIdx.x == 0) {
         atomicAdd(&m.counters[LF_CNT_UPDATES], 1u);
         atomicAdd(reinterpret_cast<unsigned long long *>(&m.counters[LF_CNT_VIS_SUM_LO]), (unsigned long long)nvis);
     }
@@ -308,7 +322,7 @@ __global__ __launch_bounds__(256) void k_integrate(MapDev m, const float *__rest
     // issuing both voxels' loads before anything waits on them: the
     // projections and input gathers, then (inside the truncation band) the
     // voxel's weight, sdf and colour together, then the fusion and stores.
-    for (int i = blockIdx.x; i < nvis; i += gridDim.x) {
+    for (int i = wg; i < nvis; i += nwg) {
         const int4 p = reinterpret_cast<const int4 *>(m.visible_pos)[i];  // x, y, z, pool block
         const int b = p.w;
         bool ok[2];
@@ -440,6 +454,57 @@ __global__ __launch_bounds__(256) void k_integrate(MapDev m, const float *__rest
             }
         }
     }
+}
+
+template <bool kWeighted, bool kColour, bool kPacked = false, bool kMixed = false>
+__global__ __launch_bounds__(256) void k_integrate(MapDev m, const float *__restrict__ depth,
+                                                   const float *__restrict__ nmap,
+                                                   const unsigned char *__restrict__ rgb, int cols, int rows,
+                                                   const float *__restrict__ pose24, lf_intrinsics K,
+                                                   const float4 *__restrict__ samples = nullptr) {
+    integrate_wg<kWeighted, kColour, kPacked, kMixed>(m, depth, nmap, rgb, cols, rows, pose24, K, samples, blockIdx.x,
+                                                      gridDim.x);
+}
+
+// One map's weighted update from a frame's packed samples within a batch
+// (each a 2-D launch whose y index is the map): the arguments of the three
+// kernels, and each map's grids.
+struct FuseJob {
+    MapDev m;
+    const float *depth;
+    const float4 *samples;
+    const unsigned char *rgb;  // null: no colour
+    const float *pose24;
+    lf_intrinsics K;
+    int cols, rows, tiles_x, tiles_y, vis_grid;
+};
+
+__global__ __launch_bounds__(256) void k_alloc_batch(const FuseJob *__restrict__ jobs) {
+    __shared__ unsigned long long s_set[kDedup];
+    const FuseJob &j = jobs[blockIdx.y];
+    const int t = blockIdx.x;
+    if (t >= j.tiles_x * j.tiles_y) return;
+    alloc_wg<true>(j.m, j.depth, j.cols, j.rows, j.pose24, j.K, t % j.tiles_x, t / j.tiles_x, s_set);
+}
+
+__global__ __launch_bounds__(kVisThreads) void k_visible_batch(const FuseJob *__restrict__ jobs) {
+    __shared__ unsigned int s_cnt[kVisThreads / kWave];
+    __shared__ unsigned int s_base;
+    const FuseJob &j = jobs[blockIdx.y];
+    if ((int)blockIdx.x >= j.vis_grid) return;
+    visible_wg<false>(j.m, j.pose24, j.K, j.cols, j.rows, blockIdx.x, j.vis_grid, s_cnt, s_base);
+}
+
+template <bool kMixed>
+__global__ __launch_bounds__(256) void k_integrate_batch(const FuseJob *__restrict__ jobs) {
+    const FuseJob &j = jobs[blockIdx.y];
+    if (j.tiles_x == 0) return;  // a map without an update this step
+    if (j.rgb)
+        integrate_wg<true, true, true, kMixed>(j.m, j.depth, nullptr, j.rgb, j.cols, j.rows, j.pose24, j.K, j.samples,
+                                               blockIdx.x, gridDim.x);
+    else
+        integrate_wg<true, false, true, kMixed>(j.m, j.depth, nullptr, j.rgb, j.cols, j.rows, j.pose24, j.K,
+                                                j.samples, blockIdx.x, gridDim.x);
 }
 
 // De-integration: the inverse of the weighted update for one frame, over the
@@ -611,6 +676,60 @@ int lf_deintegrate_ev(const lf_map_params *p, const lf_map_storage *s, const flo
         if (rgb) k_deintegrate<true, false><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K, smp);
         else k_deintegrate<false, false><<<G, 256, 0, st>>>(m, depth, nmap, rgb, cols, rows, pose24, *K, smp);
     }
+    LF_LAUNCH_CHECK();
+    return LF_OK;
+}
+
+size_t lf_fuse_job_bytes(void) { return sizeof(FuseJob); }
+
+int lf_update_batch_make(const LfFuseItem *items, int n, void *jobs_host, LfFuseBatchDims *dims) {
+    if (!items || n <= 0 || n > 65535 || !jobs_host || !dims) return lf_set_error(LF_ERR_ARG, "bad update batch");
+    FuseJob *J = static_cast<FuseJob *>(jobs_host);
+    LfFuseBatchDims d = {};
+    d.n = n;
+    for (int i = 0; i < n; ++i) {
+        const LfFuseItem &it = items[i];
+        FuseJob &j = J[i];
+        j = FuseJob{};
+        if (!it.enabled) continue;  // tiles 0: every kernel skips this map
+        if (!it.p || !it.s || !it.depth || !it.samples || !it.pose24 || !it.K || it.cols <= 0 || it.rows <= 0)
+            return lf_set_error(LF_ERR_ARG, "bad update batch item %d", i);
+        if (int rc = lf_check_storage(it.s)) return rc;
+        const int mixed = it.p->precision == LF_PRECISION_MIXED;
+        if (d.any && mixed != d.mixed)
+            return lf_set_error(LF_ERR_ARG, "update batch item %d: maps of one batch share one precision", i);
+        d.any = 1;
+        d.mixed = mixed;
+        j.m = make_dev(it.p, it.s);
+        j.depth = it.depth;
+        j.samples = reinterpret_cast<const float4 *>(it.samples);
+        j.rgb = it.rgb;
+        j.pose24 = it.pose24;
+        j.K = *it.K;
+        j.cols = it.cols;
+        j.rows = it.rows;
+        j.tiles_x = (it.cols + 15) / 16;
+        j.tiles_y = (it.rows + 15) / 16;
+        j.vis_grid = visible_grid(it.p);
+        d.tiles_max = std::max(d.tiles_max, j.tiles_x * j.tiles_y);
+        d.vis_grid_max = std::max(d.vis_grid_max, j.vis_grid);
+    }
+    // the single launch's 8192 workgroups shared out, at least 256 per map
+    d.integrate_grid = std::max(256, integrate_grid() / n);
+    *dims = d;
+    return LF_OK;
+}
+
+int lf_update_batch_launch(const void *jobs_dev, const LfFuseBatchDims &d, hipStream_t st) {
+    if (!jobs_dev || d.n <= 0) return lf_set_error(LF_ERR_ARG, "bad update batch launch");
+    if (!d.any) return LF_OK;
+    const FuseJob *J = static_cast<const FuseJob *>(jobs_dev);
+    k_alloc_batch<<<dim3(d.tiles_max, d.n), 256, 0, st>>>(J);
+    LF_LAUNCH_CHECK();
+    k_visible_batch<<<dim3(d.vis_grid_max, d.n), kVisThreads, 0, st>>>(J);
+    LF_LAUNCH_CHECK();
+    if (d.mixed) k_integrate_batch<true><<<dim3(d.integrate_grid, d.n), 256, 0, st>>>(J);
+    else k_integrate_batch<false><<<dim3(d.integrate_grid, d.n), 256, 0, st>>>(J);
     LF_LAUNCH_CHECK();
     return LF_OK;
 }

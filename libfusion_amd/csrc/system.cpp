@@ -609,6 +609,16 @@ const float *System::raycast_pose24() { return odometry_->pose24(); }
 
 int System::enqueue_after_raycast() { return odometry_->get_reference_image().resize_device_map(); }
 
+int System::plan_fusion(LfFuseItem &it) {
+    DeviceImage &ref = odometry_->get_reference_image();
+    DenseMapping &m = *mapping_;
+    it = LfFuseItem{&m.params(), &m.storage(), ref.get_raw_depth().ptr<float>(),
+                    ref.get_integration_samples().ptr<float>(),
+                    cfg_.integrate_colour ? ref.get_image().ptr<unsigned char>() : nullptr, odometry_->pose24(),
+                    &m.intrinsics(), m.intrinsics().width, m.intrinsics().height, mapping_enabled ? 1 : 0};
+    return LF_OK;
+}
+
 int System::plan_after_raycast(std::vector<unsigned char> &passes) {
     return odometry_->get_reference_image().plan_refresh(passes);
 }
@@ -693,7 +703,7 @@ int Batch::reserve_table(size_t bytes) {
 
 int Batch::process(const uint16_t *const *depth, const unsigned char *const *rgb) {
     const int n = size();
-    const size_t PB = lf_pass_bytes(), J = lf_track_job_bytes(), RJ = lf_ray_job_bytes();
+    const size_t PB = lf_pass_bytes(), J = lf_track_job_bytes(), RJ = lf_ray_job_bytes(), FJ = lf_fuse_job_bytes();
     // every system's upload and re-pyramid planned as frame passes, its
     // Gauss-Newton steps as jobs
     std::vector<std::vector<unsigned char>> jobs(n), up(n), after(n);
@@ -716,9 +726,11 @@ int Batch::process(const uint16_t *const *depth, const unsigned char *const *rgb
         }
     }
     // One table per step, copied in one transfer: [pass p][system] (uploads,
-    // then re-pyramids), [step s][system] tracking jobs, [system] raycasts.
+    // then re-pyramids), [step s][system] tracking jobs, [system] updates,
+    // [system] raycasts.
     const size_t n_pass = n_up + n_after;
-    const size_t off_track = n_pass * n * PB, off_ray = off_track + steps * n * J, bytes = off_ray + n * RJ;
+    const size_t off_track = n_pass * n * PB, off_fuse = off_track + steps * n * J, off_ray = off_fuse + n * FJ,
+                 bytes = off_ray + n * RJ;
     // the buffers of two steps ago must have been copied before they are rewritten
     const int k = parity_;
     if (table_used_[k]) LF_HIP_CHECK(hipEventSynchronize(table_read_[k]));
@@ -748,13 +760,17 @@ int Batch::process(const uint16_t *const *depth, const unsigned char *const *rgb
                 lf_track_job_skip(dst);
             }
         }
+    std::vector<LfFuseItem> fuse(n);
     std::vector<LfRayItem> items(n);
     for (int i = 0; i < n; ++i) {
+        LF_RC(sys_[i]->plan_fusion(fuse[i]));
         DenseMapping &m = sys_[i]->mapping();
         items[i] = LfRayItem{&m.params(), &m.storage(), sys_[i]->raycast_vmap(), nullptr,
                              sys_[i]->raycast_pose24(), &m.intrinsics(), m.intrinsics().width,
                              m.intrinsics().height};
     }
+    LfFuseBatchDims fd;
+    LF_RC(lf_update_batch_make(fuse.data(), n, host + off_fuse, &fd));
     LfRayBatchDims ray;
     LF_RC(lf_raycast_batch_make(items.data(), n, host + off_ray, &ray));
     // Upload stream: the table, then the uploads.  The uploads rewrite the
@@ -769,15 +785,10 @@ int Batch::process(const uint16_t *const *depth, const unsigned char *const *rgb
     LF_HIP_CHECK(hipStreamWaitEvent(stream_, uploaded_, 0));
     for (size_t s = 0; s < steps; ++s)
         LF_RC(lf_track_batch_step(dev + off_track + s * n * J, n, gmax[s], half_maps_, stream_));
-    // fusion on the systems' streams once their poses are final
     LF_HIP_CHECK(hipEventRecord(tracked_, stream_));
     tracked_once_ = true;
-    for (int i = 0; i < n; ++i) {
-        LF_HIP_CHECK(hipStreamWaitEvent(sys_[i]->stream(), tracked_, 0));
-        LF_RC(sys_[i]->enqueue_fusion());
-        LF_HIP_CHECK(hipEventRecord(ready_[i], sys_[i]->stream()));
-        LF_HIP_CHECK(hipStreamWaitEvent(stream_, ready_[i], 0));
-    }
+    // block allocation, visible lists and integration of every map
+    LF_RC(lf_update_batch_launch(dev + off_fuse, fd, stream_));
     LF_RC(lf_raycast_batch_launch(dev + off_ray, ray, 1, stream_));
     // re-pyramids of the raycast maps
     for (size_t p = n_up; p < n_pass; ++p) LF_RC(lf_pass_batch(dev + p * n * PB, n, wmax[p], stream_));

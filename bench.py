@@ -101,8 +101,8 @@ def batch_throughput(n, warmup, steps, single_fps, precision=0, separate=True):
            "steps": steps, "warmup": warmup,
            "workload": "%d independent %s C1 systems (seeds 100-%d: own scene, orbit and map), one frame "
                        "of each per step" % (n, name, 99 + n),
-           "launch": "lf_batch: pyramid passes and Gauss-Newton steps of all systems in one launch each, fusion "
-                     "per system on its own stream, one batched z-range + raycast launch per stage for all maps"}
+           "launch": "lf_batch: one descriptor table per step; pyramid passes, Gauss-Newton steps, block allocation, "
+                     "visible lists, integration, z-range and raycast of all systems in one launch each"}
     if not separate:
         return out
     systems = [capi.System(K, p, configs.C1_LEVELS, configs.C1_ITERATIONS, ds) for K, p, ds in cfgs]

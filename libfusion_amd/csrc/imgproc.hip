@@ -338,11 +338,25 @@ LF_DEV void frame_pass_wg(const Pass &ps, int wg) {
 
 __global__ __launch_bounds__(256) void k_frame_pass(Pass ps) { frame_pass_wg(ps, blockIdx.x); }
 
-// One pass of each of gridDim.y images: row y of the grid runs table[y].
+// One pass of each of gridDim.y images: row y of the grid runs table[y] (its
+// part's pointers asserted global: they are read from memory).
 __global__ __launch_bounds__(256) void k_frame_pass_batch(const Pass *__restrict__ table) {
     const Pass &ps = table[blockIdx.y];
-    if ((int)blockIdx.x >= ps.total_wg) return;
-    frame_pass_wg(ps, blockIdx.x);
+    const int wg = blockIdx.x;
+    if (wg >= ps.total_wg) return;
+    int pi = 0;
+#pragma unroll
+    for (int k = 1; k < kMaxParts; ++k)
+        if (k < ps.nparts && wg >= ps.part[k].wg_begin) pi = k;
+    Part p = ps.part[pi];
+    p.in0 = global_ptr(p.in0); p.in1 = global_ptr(p.in1); p.in2 = global_ptr(p.in2); p.in3 = global_ptr(p.in3);
+    p.out0 = global_ptr(p.out0); p.out1 = global_ptr(p.out1); p.out2 = global_ptr(p.out2);
+    p.out3 = global_ptr(p.out3); p.out4 = global_ptr(p.out4);
+    const int t = wg - p.wg_begin, tiles_x = (p.w + 15) >> 4;
+    const int x = (t % tiles_x) * 16 + (threadIdx.x & 15), y = (t / tiles_x) * 16 + (threadIdx.x >> 4);
+    if (x >= p.w || y >= p.h) return;
+    if (p.half) frame_part<true>(p, x, y);
+    else frame_part<false>(p, x, y);
 }
 
 dim3 grid16(int w, int h) { return dim3((w + 15) / 16, (h + 15) / 16); }

@@ -157,6 +157,16 @@ LF_DEV int find_block(const unsigned long long *__restrict__ keys, const int *__
 
 LF_DEV int lane_id() { return __lane_id(); }
 
+// A device pointer asserted to address global memory.  Pointers read from a
+// descriptor table in memory are generic, and every access through them
+// compiles to a flat_ instruction (which also waits on the LDS / scalar
+// counter); through the result they compile to global_ instructions, as
+// kernel arguments do.
+template <class T>
+LF_DEV T *global_ptr(T *p) {
+    return (T *)((__attribute__((address_space(1))) T *)reinterpret_cast<uintptr_t>(p));
+}
+
 }  // namespace lf
 
 // Error reporting (common.hip).

@@ -48,6 +48,16 @@ struct MapDev {
     float voxel_size, truncation, zmin_update, zmax_update;
 };
 
+// A map read from a descriptor table, its pointers asserted global.
+LF_DEV MapDev global_dev(const MapDev &d) {
+    MapDev m = d;
+    m.keys = global_ptr(d.keys); m.slot_block = global_ptr(d.slot_block); m.slot_flag = global_ptr(d.slot_flag);
+    m.block_pos = global_ptr(d.block_pos); m.sdf = global_ptr(d.sdf); m.weight = global_ptr(d.weight);
+    m.rgb = global_ptr(d.rgb); m.visible = global_ptr(d.visible); m.visible_pos = global_ptr(d.visible_pos);
+    m.block_dir = global_ptr(d.block_dir); m.counters = global_ptr(d.counters); m.occupancy = global_ptr(d.occupancy);
+    return m;
+}
+
 MapDev make_dev(const lf_map_params *p, const lf_map_storage *s) {
     MapDev d;
     d.keys = s->keys; d.slot_block = s->slot_block; d.slot_flag = s->slot_flag; d.block_pos = s->block_pos;
@@ -484,7 +494,8 @@ __global__ __launch_bounds__(256) void k_alloc_batch(const FuseJob *__restrict__
     const FuseJob &j = jobs[blockIdx.y];
     const int t = blockIdx.x;
     if (t >= j.tiles_x * j.tiles_y) return;
-    alloc_wg<true>(j.m, j.depth, j.cols, j.rows, j.pose24, j.K, t % j.tiles_x, t / j.tiles_x, s_set);
+    alloc_wg<true>(global_dev(j.m), global_ptr(j.depth), j.cols, j.rows, global_ptr(j.pose24), j.K, t % j.tiles_x,
+                   t / j.tiles_x, s_set);
 }
 
 __global__ __launch_bounds__(kVisThreads) void k_visible_batch(const FuseJob *__restrict__ jobs) {
@@ -492,19 +503,24 @@ __global__ __launch_bounds__(kVisThreads) void k_visible_batch(const FuseJob *__
     __shared__ unsigned int s_base;
     const FuseJob &j = jobs[blockIdx.y];
     if ((int)blockIdx.x >= j.vis_grid) return;
-    visible_wg<false>(j.m, j.pose24, j.K, j.cols, j.rows, blockIdx.x, j.vis_grid, s_cnt, s_base);
+    visible_wg<false>(global_dev(j.m), global_ptr(j.pose24), j.K, j.cols, j.rows, blockIdx.x, j.vis_grid, s_cnt,
+                      s_base);
 }
 
 template <bool kMixed>
 __global__ __launch_bounds__(256) void k_integrate_batch(const FuseJob *__restrict__ jobs) {
     const FuseJob &j = jobs[blockIdx.y];
     if (j.tiles_x == 0) return;  // a map without an update this step
-    if (j.rgb)
-        integrate_wg<true, true, true, kMixed>(j.m, j.depth, nullptr, j.rgb, j.cols, j.rows, j.pose24, j.K, j.samples,
+    const MapDev m = global_dev(j.m);
+    const float *depth = global_ptr(j.depth), *pose24 = global_ptr(j.pose24);
+    const float4 *samples = global_ptr(j.samples);
+    const unsigned char *rgb = global_ptr(j.rgb);
+    if (rgb)
+        integrate_wg<true, true, true, kMixed>(m, depth, nullptr, rgb, j.cols, j.rows, pose24, j.K, samples,
                                                blockIdx.x, gridDim.x);
     else
-        integrate_wg<true, false, true, kMixed>(j.m, j.depth, nullptr, j.rgb, j.cols, j.rows, j.pose24, j.K,
-                                                j.samples, blockIdx.x, gridDim.x);
+        integrate_wg<true, false, true, kMixed>(m, depth, nullptr, rgb, j.cols, j.rows, pose24, j.K, samples,
+                                                blockIdx.x, gridDim.x);
 }
 
 // De-integration: the inverse of the weighted update for one frame, over the

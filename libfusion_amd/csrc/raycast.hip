@@ -49,6 +49,17 @@ struct RayDev {
     int pool_fits_lds;  // pool indices fit the 24-bit field of the LDS cache
 };
 
+// A map read from a descriptor table, its pointers asserted global.
+LF_DEV RayDev global_ray(const RayDev &d) {
+    RayDev r = d;
+    r.keys = global_ptr(d.keys); r.slot_block = global_ptr(d.slot_block); r.sdf = global_ptr(d.sdf);
+    r.weight = global_ptr(d.weight); r.rgb = global_ptr(d.rgb); r.visible = global_ptr(d.visible);
+    r.visible_pos = global_ptr(d.visible_pos); r.block_dir = global_ptr(d.block_dir);
+    r.block_pos = global_ptr(d.block_pos); r.counters = global_ptr(d.counters); r.occupancy = global_ptr(d.occupancy);
+    r.zx = global_ptr(d.zx); r.zy = global_ptr(d.zy);
+    return r;
+}
+
 // One raycast's inputs and outputs: the map, the target image and the pose
 // (single launches take it by value, batched launches read job blockIdx.y).
 struct RayJob {
@@ -73,7 +84,7 @@ LF_DEV void zrange_clear_wg(const RayDev &r, int wg, int nwg) {
 __global__ void k_zrange_clear(RayDev r) { zrange_clear_wg(r, blockIdx.x, gridDim.x); }
 
 __global__ void k_zrange_clear_batch(const RayJob *__restrict__ jobs) {
-    zrange_clear_wg(jobs[blockIdx.y].r, blockIdx.x, gridDim.x);
+    zrange_clear_wg(global_ray(jobs[blockIdx.y].r), blockIdx.x, gridDim.x);
 }
 
 LF_DEV int sat16(float v) {
@@ -210,7 +221,7 @@ __global__ __launch_bounds__(kZThreads) void k_zrange_fill_batch(const RayJob *_
     __shared__ int s_box[4];
     __shared__ unsigned int s_rend;
     const RayJob &j = jobs[blockIdx.y];
-    zrange_fill_wg<kLds>(j.r, j.pose24, j.K, blockIdx.x, gridDim.x, s_tile, s_box, s_rend);
+    zrange_fill_wg<kLds>(global_ray(j.r), global_ptr(j.pose24), j.K, blockIdx.x, gridDim.x, s_tile, s_box, s_rend);
 }
 
 // Workgroup-shared, direct-mapped cache of block lookups in LDS, negative
@@ -449,8 +460,8 @@ __global__ __launch_bounds__(256) void k_raycast_batch(const RayJob *__restrict_
     const RayJob &j = jobs[blockIdx.y];
     const int g = kXcds * ((j.wg_x * j.wg_y + kXcds - 1) / kXcds);
     if ((int)blockIdx.x >= g) return;  // this raycast's grid is smaller than the batch's
-    raycast_wg<kCheckColour, kMixed>(j.r, j.vmap, j.rgb_out, j.cols, j.rows, j.pose24, j.K, j.wg_x, j.wg_y, blockIdx.x,
-                             s_cache, s_occ);
+    raycast_wg<kCheckColour, kMixed>(global_ray(j.r), global_ptr(j.vmap), global_ptr(j.rgb_out), j.cols, j.rows,
+                                     global_ptr(j.pose24), j.K, j.wg_x, j.wg_y, blockIdx.x, s_cache, s_occ);
 }
 
 // Dynamic LDS available to one workgroup on this device (less the static use).

@@ -699,6 +699,19 @@ __global__ __launch_bounds__(kThreads) void k_track_final(ReduceArgs a, int G) {
     track_final_wg(a, G, s_sum);
 }
 
+// Launch arguments read from a descriptor table, their pointers asserted global.
+LF_DEV ReduceArgs global_args(const ReduceArgs &d) {
+    ReduceArgs a = d;
+    LfTrackLevel &L = a.lv;
+    L.curr_vmap = global_ptr(d.lv.curr_vmap); L.curr_nmap = global_ptr(d.lv.curr_nmap);
+    L.last_vmap = global_ptr(d.lv.last_vmap); L.last_nmap = global_ptr(d.lv.last_nmap);
+    L.curr_I = global_ptr(d.lv.curr_I); L.last_I = global_ptr(d.lv.last_I);
+    L.dIdx = global_ptr(d.lv.dIdx); L.dIdy = global_ptr(d.lv.dIdy);
+    a.pose12 = global_ptr(d.pose12); a.partials = global_ptr(d.partials); a.out29 = global_ptr(d.out29);
+    a.st = global_ptr(d.st);
+    return a;
+}
+
 // One system's step of a batch: its launch arguments and its grid (0: the
 // system has no step at this index).
 struct TrackJob {
@@ -714,7 +727,7 @@ __global__ __launch_bounds__(kThreads) void k_track_step_batch(const TrackJob *_
     __shared__ float s_wave[kWaves][kVals];
     const TrackJob &j = jobs[blockIdx.y];
     if ((int)blockIdx.x >= j.G) return;
-    track_step_wg<kHalf>(j.a, blockIdx.x, j.G, s_wave);
+    track_step_wg<kHalf>(global_args(j.a), blockIdx.x, j.G, s_wave);
 }
 
 // The final sums and Gauss-Newton step of each system: workgroup i, system i.
@@ -722,7 +735,7 @@ __global__ __launch_bounds__(kThreads) void k_track_final_batch(const TrackJob *
     __shared__ float s_sum[kVals];
     const TrackJob &j = jobs[blockIdx.x];
     if (j.G <= 0) return;
-    track_final_wg(j.a, j.G, s_sum);
+    track_final_wg(global_args(j.a), j.G, s_sum);
 }
 
 __global__ void k_track_finish(LfTrackState *st) {

@@ -29,7 +29,9 @@ LF_DEV float depth_value(uint16_t d, float scale) { return (float)d * scale; }
 // cv::cuda::bilateralFilter(src, dst, 5, 1, 1): 5x5 window, reflect-101 border,
 // w = exp(-(dx^2+dy^2)/(2*1^2) - (v-c)^2/(2*1^2)); zero depths are not masked.
 // at(x, y) returns the source value of an in-range pixel.
-template <class At>
+// kFast (mixed precision): the hardware exponential (v_exp_f32 on a base-2
+// argument) instead of the correctly rounded expf.
+template <class At, bool kFast = false>
 LF_DEV float bilateral_at(const At &at, int x, int y, int w, int h) {
     const float c = at(x, y);
     float s1 = 0.0f, s2 = 0.0f;
@@ -40,7 +42,8 @@ LF_DEV float bilateral_at(const At &at, int x, int y, int w, int h) {
         for (int dx = -2; dx <= 2; ++dx) {
             const float v = at(reflect101(x + dx, w), yy);
             const float d = v - c;
-            const float wt = expf((float)(dx * dx + dy * dy) * -0.5f + d * d * -0.5f);
+            const float arg = (float)(dx * dx + dy * dy) * -0.5f + d * d * -0.5f;
+            const float wt = kFast ? __expf(arg) : expf(arg);
             s1 += v * wt;
             s2 += wt;
         }
@@ -105,13 +108,15 @@ LF_DEV float4 back_project_at(float z, int x, int y, const lf_intrinsics &K) {
 
 // computeNMapK (cuda_imgproc.cu:276-294); last row / column NaN.  V(x, y)
 // returns the vertex of an in-range pixel.
-template <class V>
+// kFast (mixed precision): normalised with the hardware reciprocal square root.
+template <class V, bool kFast = false>
 LF_DEV float4 nmap_at(const V &vert, int w, int h, int x, int y) {
     const float qnan = __int_as_float(0x7fc00000);
     if (x >= w - 1 || y >= h - 1) return make_float4(qnan, qnan, qnan, qnan);
     const int x10 = max(x - 1, 0), x01 = min(x + 1, w), y10 = max(y - 1, 0), y01 = min(y + 1, h);
     const float4 a = vert(x01, y), b = vert(x10, y), c = vert(x, y01), e = vert(x, y10);
-    const f3 n = normalised(cross(f3{a.x - b.x, a.y - b.y, a.z - b.z}, f3{c.x - e.x, c.y - e.y, c.z - e.z}));
+    const f3 cr = cross(f3{a.x - b.x, a.y - b.y, a.z - b.z}, f3{c.x - e.x, c.y - e.y, c.z - e.z});
+    const f3 n = kFast ? mul(cr, __builtin_amdgcn_rsqf(dot(cr, cr))) : normalised(cr);
     return make_float4(n.x, n.y, n.z, 1.0f);
 }
 
@@ -120,9 +125,11 @@ LF_DEV float4 nmap_at(const V &vert, int w, int h, int x, int y) {
 // the pixel); y = the weight |sin(n_z)| / d^2 of update_map_weighted_kernel
 // (depth_fusion.cu:356-428) in its operation order; z = colour r | g << 8 |
 // b << 16 as bits.
+template <bool kFast = false>
 LF_DEV float4 integration_sample(float dist, float4 n, const unsigned char *c) {
     const float qnan = __int_as_float(0x7fc00000);
-    const float wt = fabsf(sinf(n.z)) / (dist * dist);
+    // kFast (mixed precision): hardware sine and reciprocal
+    const float wt = kFast ? fabsf(__sinf(n.z)) * __builtin_amdgcn_rcpf(dist * dist) : fabsf(sinf(n.z)) / (dist * dist);
     const unsigned int rgba = (unsigned int)c[0] | ((unsigned int)c[1] << 8) | ((unsigned int)c[2] << 16);
     return make_float4(isnan(n.x) ? qnan : dist, wt, __uint_as_float(rgba), 0.0f);
 }
@@ -272,8 +279,10 @@ struct Pass {
     int total_wg;  // workgroups of all parts
 };
 
-// Pixel (x, y) of a part.  kHalf: the intensity, gradient, vertex and normal
-// planes are FP16 (read and written); depth, colour and samples are not.
+// Pixel (x, y) of a part.  kHalf (mixed precision): the intensity, gradient,
+// vertex and normal planes are FP16 (read and written; depth, colour and
+// samples are not), and the bilateral weights, normals and sample weights use
+// the hardware exponential, reciprocal square root, sine and reciprocal.
 template <bool kHalf>
 LF_DEV void frame_part(const Part &p, int x, int y) {
     const int w = p.w, h = p.h;
@@ -283,7 +292,7 @@ LF_DEV void frame_part(const Part &p, int x, int y) {
         const uint16_t *d16 = static_cast<const uint16_t *>(p.in0);
         const unsigned char *rgb = static_cast<const unsigned char *>(p.in1);
         static_cast<float *>(p.out0)[i] = depth_value(d16[i], p.scale);
-        static_cast<float *>(p.out1)[i] = bilateral_at(DepthU16At{d16, w, p.scale}, x, y, w, h);
+        static_cast<float *>(p.out1)[i] = bilateral_at<DepthU16At, kHalf>(DepthU16At{d16, w, p.scale}, x, y, w, h);
         const unsigned char r = rgb[3 * i], g = rgb[3 * i + 1], b = rgb[3 * i + 2];
         store1<kHalf>(p.out2, i, grey_of(r, g, b));
         unsigned char *c = static_cast<unsigned char *>(p.out3) + 3 * i;
@@ -301,17 +310,17 @@ LF_DEV void frame_part(const Part &p, int x, int y) {
         store1<kHalf>(p.out0, i, gx);
         store1<kHalf>(p.out1, i, gy);
         store4<kHalf>(p.out2, i, back_project_at(d[i], x, y, p.K));
-        const float4 n = nmap_at(DepthVertexAt{d, w, p.K}, w, h, x, y);
+        const float4 n = nmap_at<DepthVertexAt, kHalf>(DepthVertexAt{d, w, p.K}, w, h, x, y);
         store4<kHalf>(p.out3, i, n);
         if (p.out4) {  // level 0: the pixel's integration sample
             const float dist = static_cast<const float *>(p.in2)[i];
             const unsigned char *c = static_cast<const unsigned char *>(p.in3) + 3 * i;
-            static_cast<float4 *>(p.out4)[i] = integration_sample(dist, n, c);
+            static_cast<float4 *>(p.out4)[i] = integration_sample<kHalf>(dist, n, c);
         }
         break;
     }
     case kNmap:  // normal map of a stored vertex map
-        store4<kHalf>(p.out0, i, nmap_at(Vec4At<kHalf>{p.in0, w}, w, h, x, y));
+        store4<kHalf>(p.out0, i, nmap_at<Vec4At<kHalf>, kHalf>(Vec4At<kHalf>{p.in0, w}, w, h, x, y));
         break;
     case kVmapDown:  // half-size vertex map
         store4<kHalf>(p.out0, i, resize4_at(Vec4At<kHalf>{p.in0, 2 * w}, x, y));

@@ -111,8 +111,9 @@ LF_DEV void insert_block(const MapDev &m, int bx, int by, int bz) {
 
 // One pixel of create_blocks_kernel (depth_fusion.cu:121-228): the 3-D
 // Bresenham walk over the blocks between the near and far ends of the
-// truncation band around the measured depth.
-template <class F>
+// truncation band around the measured depth.  kFast (mixed precision): the
+// voxel coordinates by a product with the reciprocal voxel size.
+template <bool kFast, class F>
 LF_DEV void band_blocks(int x, int y, float z, const float *Twc, const lf_intrinsics &K, const MapDev &m, F emit) {
     const float zmin = m.zmin_update, zmax = m.zmax_update;
     if (isnan(z) || z < zmin || z > zmax) return;
@@ -124,7 +125,15 @@ LF_DEV void band_blocks(int x, int y, float z, const float *Twc, const lf_intrin
     const float ux = K.invfx * ((float)x - K.cx), uy = K.invfy * ((float)y - K.cy);
     f3 pn = xform(Twc, f3{ux * zn, uy * zn, zn});
     f3 pf = xform(Twc, f3{ux * zf, uy * zf, zf});
-    i3 vn = to_i3(dvd(pn, vs)), vf = to_i3(dvd(pf, vs));
+    i3 vn, vf;
+    if (kFast) {
+        const float iv = 1.0f / vs;
+        vn = to_i3(mul(pn, iv));
+        vf = to_i3(mul(pf, iv));
+    } else {
+        vn = to_i3(dvd(pn, vs));
+        vf = to_i3(dvd(pf, vs));
+    }
     if (vn.x == kBad || vn.y == kBad || vn.z == kBad || vf.x == kBad || vf.y == kBad || vf.z == kBad) return;
     i3 a = {blk(vn.x), blk(vn.y), blk(vn.z)}, b = {blk(vf.x), blk(vf.y), blk(vf.z)};
     const int dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
@@ -178,7 +187,7 @@ LF_DEV void touch_block(const MapDev &m, int bx, int by, int bz) {
 // walk only marks the blocks that exist, so the visible list of a frame can be
 // rebuilt without allocating.
 // 16x16 pixel tile (tx, ty) of the frame.
-template <bool kInsert>
+template <bool kInsert, bool kFast = false>
 LF_DEV void alloc_wg(const MapDev &m, const float *__restrict__ depth, int cols, int rows,
                      const float *__restrict__ pose24, const lf_intrinsics &K, int tx, int ty,
                      unsigned long long *s_set) {
@@ -191,7 +200,7 @@ LF_DEV void alloc_wg(const MapDev &m, const float *__restrict__ depth, int cols,
     __syncthreads();
     const int x = tx * 16 + (tid & 15), y = ty * 16 + (tid >> 4);
     if (x >= cols || y >= rows) return;
-    band_blocks(x, y, depth[(size_t)y * cols + x], Twc, K, m, [&](int bx, int by, int bz) {
+    band_blocks<kFast>(x, y, depth[(size_t)y * cols + x], Twc, K, m, [&](int bx, int by, int bz) {
         const unsigned long long key = pack_key(bx, by, bz);
         unsigned int h = key_mix(key) & (kDedup - 1);
         for (int probe = 0; probe < kDedup; ++probe) {
@@ -210,15 +219,17 @@ LF_DEV void alloc_wg(const MapDev &m, const float *__restrict__ depth, int cols,
     });
 }
 
-template <bool kInsert>
+template <bool kInsert, bool kFast = false>
 __global__ __launch_bounds__(256) void k_alloc(MapDev m, const float *__restrict__ depth, int cols, int rows,
                                                const float *__restrict__ pose24, lf_intrinsics K) {
     __shared__ unsigned long long s_set[kDedup];
-    alloc_wg<kInsert>(m, depth, cols, rows, pose24, K, blockIdx.x, blockIdx.y, s_set);
+    alloc_wg<kInsert, kFast>(m, depth, cols, rows, pose24, K, blockIdx.x, blockIdx.y, s_set);
 }
 
 // is_block_visible (depth_fusion.cu:27-47): any of the 8 block corners inside
-// the image with z in [zmin, zmax] of the update.
+// the image with z in [zmin, zmax] of the update.  kFast (mixed precision):
+// the projection with the hardware reciprocal.
+template <bool kFast = false>
 LF_DEV bool block_visible(int bx, int by, int bz, const float *Tcw, const lf_intrinsics &K, int cols, int rows,
                           float zmin, float zmax, float scale) {
 #pragma unroll
@@ -226,7 +237,15 @@ LF_DEV bool block_visible(int bx, int by, int bz, const float *Tcw, const lf_int
         const float wx = (float)(bx + (c & 1)) * scale, wy = (float)(by + ((c >> 1) & 1)) * scale,
                     wz = (float)(bz + ((c >> 2) & 1)) * scale;
         f3 pt = xform(Tcw, f3{wx, wy, wz});
-        float u = K.fx * pt.x / pt.z + K.cx, v = K.fy * pt.y / pt.z + K.cy;
+        float u, v;
+        if (kFast) {
+            const float iz = __builtin_amdgcn_rcpf(pt.z);
+            u = K.fx * pt.x * iz + K.cx;
+            v = K.fy * pt.y * iz + K.cy;
+        } else {
+            u = K.fx * pt.x / pt.z + K.cx;
+            v = K.fy * pt.y / pt.z + K.cy;
+        }
         if (!(u < 0.0f || v < 0.0f || u > (float)(cols - 1) || v > (float)(rows - 1) || pt.z < zmin || pt.z > zmax))
             return true;
     }
@@ -244,7 +263,7 @@ constexpr int kVisGrid = 512;      // at most this many workgroups (grid-stride 
 // LDS): same-address atomics serialise, so one per wavefront cost most of the
 // kernel.  The list order is unspecified, as with the per-wavefront append.
 // Workgroup wg of nwg.
-template <bool kGeometric>
+template <bool kGeometric, bool kFast = false>
 LF_DEV void visible_wg(const MapDev &m, const float *__restrict__ pose24, const lf_intrinsics &K, int cols, int rows,
                        int wg, int nwg, unsigned int *s_cnt, unsigned int &s_base) {
     float Tcw[12];
@@ -265,7 +284,7 @@ LF_DEV void visible_wg(const MapDev &m, const float *__restrict__ pose24, const 
                 vis = true;
                 m.slot_flag[p.w] = 0;
             } else {
-                vis = block_visible(p.x, p.y, p.z, Tcw, K, cols, rows, m.zmin_update, m.zmax_update, scale);
+                vis = block_visible<kFast>(p.x, p.y, p.z, Tcw, K, cols, rows, m.zmin_update, m.zmax_update, scale);
             }
         }
         const unsigned long long ballot = __ballot(vis);
@@ -290,12 +309,12 @@ LF_DEV void visible_wg(const MapDev &m, const float *__restrict__ pose24, const 
     }
 }
 
-template <bool kGeometric = false>
+template <bool kGeometric = false, bool kFast = false>
 __global__ __launch_bounds__(kVisThreads) void k_visible(MapDev m, const float *__restrict__ pose24, lf_intrinsics K,
                                                          int cols, int rows) {
     __shared__ unsigned int s_cnt[kVisThreads / kWave];
     __shared__ unsigned int s_base;
-    visible_wg<kGeometric>(m, pose24, K, cols, rows, blockIdx.x, gridDim.x, s_cnt, s_base);
+    visible_wg<kGeometric, kFast>(m, pose24, K, cols, rows, blockIdx.x, gridDim.x, s_cnt, s_base);
 }
 
 int visible_grid(const lf_map_params *p) {
@@ -489,21 +508,23 @@ struct FuseJob {
     int cols, rows, tiles_x, tiles_y, vis_grid;
 };
 
+template <bool kFast>
 __global__ __launch_bounds__(256) void k_alloc_batch(const FuseJob *__restrict__ jobs) {
     __shared__ unsigned long long s_set[kDedup];
     const FuseJob &j = jobs[blockIdx.y];
     const int t = blockIdx.x;
     if (t >= j.tiles_x * j.tiles_y) return;
-    alloc_wg<true>(global_dev(j.m), global_ptr(j.depth), j.cols, j.rows, global_ptr(j.pose24), j.K, t % j.tiles_x,
+    alloc_wg<true, kFast>(global_dev(j.m), global_ptr(j.depth), j.cols, j.rows, global_ptr(j.pose24), j.K, t % j.tiles_x,
                    t / j.tiles_x, s_set);
 }
 
+template <bool kFast>
 __global__ __launch_bounds__(kVisThreads) void k_visible_batch(const FuseJob *__restrict__ jobs) {
     __shared__ unsigned int s_cnt[kVisThreads / kWave];
     __shared__ unsigned int s_base;
     const FuseJob &j = jobs[blockIdx.y];
     if ((int)blockIdx.x >= j.vis_grid) return;
-    visible_wg<false>(global_dev(j.m), global_ptr(j.pose24), j.K, j.cols, j.rows, blockIdx.x, j.vis_grid, s_cnt,
+    visible_wg<false, kFast>(global_dev(j.m), global_ptr(j.pose24), j.K, j.cols, j.rows, blockIdx.x, j.vis_grid, s_cnt,
                       s_base);
 }
 
@@ -631,9 +652,12 @@ int lf_update_ev(const lf_map_params *p, const lf_map_storage *s, const float *d
     if (int rc = lf_check_storage(s)) return rc;
     const MapDev m = make_dev(p, s);
     dim3 g((cols + 15) / 16, (rows + 15) / 16);
-    k_alloc<true><<<g, 256, 0, st>>>(m, depth, cols, rows, pose24, *K);
+    const bool mixed = p->precision == LF_PRECISION_MIXED;
+    if (mixed) k_alloc<true, true><<<g, 256, 0, st>>>(m, depth, cols, rows, pose24, *K);
+    else k_alloc<true, false><<<g, 256, 0, st>>>(m, depth, cols, rows, pose24, *K);
     LF_LAUNCH_CHECK();
-    k_visible<<<visible_grid(p), kVisThreads, 0, st>>>(m, pose24, *K, cols, rows);
+    if (mixed) k_visible<false, true><<<visible_grid(p), kVisThreads, 0, st>>>(m, pose24, *K, cols, rows);
+    else k_visible<false, false><<<visible_grid(p), kVisThreads, 0, st>>>(m, pose24, *K, cols, rows);
     LF_LAUNCH_CHECK();
     const int G = integrate_grid();
     if (e0) LF_HIP_CHECK(hipEventRecord(e0, st));
@@ -671,9 +695,12 @@ int lf_deintegrate_ev(const lf_map_params *p, const lf_map_storage *s, const flo
     if (int rc = lf_check_storage(s)) return rc;
     const MapDev m = make_dev(p, s);
     dim3 g((cols + 15) / 16, (rows + 15) / 16);
-    k_alloc<false><<<g, 256, 0, st>>>(m, depth, cols, rows, pose24, *K);
+    const bool mixed = p->precision == LF_PRECISION_MIXED;
+    if (mixed) k_alloc<false, true><<<g, 256, 0, st>>>(m, depth, cols, rows, pose24, *K);
+    else k_alloc<false, false><<<g, 256, 0, st>>>(m, depth, cols, rows, pose24, *K);
     LF_LAUNCH_CHECK();
-    k_visible<<<visible_grid(p), kVisThreads, 0, st>>>(m, pose24, *K, cols, rows);
+    if (mixed) k_visible<false, true><<<visible_grid(p), kVisThreads, 0, st>>>(m, pose24, *K, cols, rows);
+    else k_visible<false, false><<<visible_grid(p), kVisThreads, 0, st>>>(m, pose24, *K, cols, rows);
     LF_LAUNCH_CHECK();
     const int G = integrate_grid();
     const float4 *smp = reinterpret_cast<const float4 *>(samples);
@@ -740,9 +767,15 @@ int lf_update_batch_launch(const void *jobs_dev, const LfFuseBatchDims &d, hipSt
     if (!jobs_dev || d.n <= 0) return lf_set_error(LF_ERR_ARG, "bad update batch launch");
     if (!d.any) return LF_OK;
     const FuseJob *J = static_cast<const FuseJob *>(jobs_dev);
-    k_alloc_batch<<<dim3(d.tiles_max, d.n), 256, 0, st>>>(J);
-    LF_LAUNCH_CHECK();
-    k_visible_batch<<<dim3(d.vis_grid_max, d.n), kVisThreads, 0, st>>>(J);
+    if (d.mixed) {
+        k_alloc_batch<true><<<dim3(d.tiles_max, d.n), 256, 0, st>>>(J);
+        LF_LAUNCH_CHECK();
+        k_visible_batch<true><<<dim3(d.vis_grid_max, d.n), kVisThreads, 0, st>>>(J);
+    } else {
+        k_alloc_batch<false><<<dim3(d.tiles_max, d.n), 256, 0, st>>>(J);
+        LF_LAUNCH_CHECK();
+        k_visible_batch<false><<<dim3(d.vis_grid_max, d.n), kVisThreads, 0, st>>>(J);
+    }
     LF_LAUNCH_CHECK();
     if (d.mixed) k_integrate_batch<true><<<dim3(d.integrate_grid, d.n), 256, 0, st>>>(J);
     else k_integrate_batch<false><<<dim3(d.integrate_grid, d.n), 256, 0, st>>>(J);
@@ -769,7 +802,10 @@ int lf_select_visible_ev(const lf_map_params *p, const lf_map_storage *s, const 
     const MapDev m = make_dev(p, s);
     k_reset_visible<<<1, 64, 0, st>>>(m);
     LF_LAUNCH_CHECK();
-    k_visible<true><<<visible_grid(p), kVisThreads, 0, st>>>(m, pose24, *K, cols, rows);
+    if (p->precision == LF_PRECISION_MIXED)
+        k_visible<true, true><<<visible_grid(p), kVisThreads, 0, st>>>(m, pose24, *K, cols, rows);
+    else
+        k_visible<true, false><<<visible_grid(p), kVisThreads, 0, st>>>(m, pose24, *K, cols, rows);
     LF_LAUNCH_CHECK();
     return LF_OK;
 }

@@ -111,8 +111,9 @@ constexpr int kZThreads = 512;  // lanes of a k_zrange_fill workgroup: 8 per vis
 // per-block global atomics.
 // Workgroup wg of nwg.  s_tile: kLds: [0, T) near z bits, [T, 2T) far z bits;
 // s_box: ulx, uly, lrx, lry of the workgroup's rectangles; s_rend: its
-// rendering blocks.
-template <bool kLds>
+// rendering blocks.  kFast (mixed precision): the projection with the
+// hardware reciprocal.
+template <bool kLds, bool kFast = false>
 LF_DEV void zrange_fill_wg(const RayDev &r, const float *__restrict__ pose24, const lf_intrinsics &K, int wg, int nwg,
                            int *s_tile, int *s_box, unsigned int &s_rend) {
     constexpr int kPer = kZThreads / 8;  // blocks per workgroup pass
@@ -147,7 +148,15 @@ LF_DEV void zrange_fill_wg(const RayDev &r, const float *__restrict__ pose24, co
             const float wx = (float)(bp.x + (c & 1)) * scale, wy = (float)(bp.y + ((c >> 1) & 1)) * scale,
                         wz = (float)(bp.z + ((c >> 2) & 1)) * scale;
             const f3 pt = xform(Tcw, f3{wx, wy, wz});
-            const float u = (K.fx * pt.x / pt.z + K.cx) / 8.0f, v = (K.fy * pt.y / pt.z + K.cy) / 8.0f;
+            float u, v;
+            if (kFast) {
+                const float iz = __builtin_amdgcn_rcpf(pt.z);
+                u = (K.fx * pt.x * iz + K.cx) * 0.125f;
+                v = (K.fy * pt.y * iz + K.cy) * 0.125f;
+            } else {
+                u = (K.fx * pt.x / pt.z + K.cx) / 8.0f;
+                v = (K.fy * pt.y / pt.z + K.cy) / 8.0f;
+            }
             fu = floorf(u); cu = ceilf(u); fv = floorf(v); cv = ceilf(v);
             zlo = zhi = pt.z;
         }
@@ -206,22 +215,22 @@ LF_DEV void zrange_fill_wg(const RayDev &r, const float *__restrict__ pose24, co
     }
 }
 
-template <bool kLds>
+template <bool kLds, bool kFast>
 __global__ __launch_bounds__(kZThreads) void k_zrange_fill(RayDev r, const float *__restrict__ pose24,
                                                            lf_intrinsics K) {
     extern __shared__ int s_tile[];
     __shared__ int s_box[4];
     __shared__ unsigned int s_rend;
-    zrange_fill_wg<kLds>(r, pose24, K, blockIdx.x, gridDim.x, s_tile, s_box, s_rend);
+    zrange_fill_wg<kLds, kFast>(r, pose24, K, blockIdx.x, gridDim.x, s_tile, s_box, s_rend);
 }
 
-template <bool kLds>
+template <bool kLds, bool kFast>
 __global__ __launch_bounds__(kZThreads) void k_zrange_fill_batch(const RayJob *__restrict__ jobs) {
     extern __shared__ int s_tile[];
     __shared__ int s_box[4];
     __shared__ unsigned int s_rend;
     const RayJob &j = jobs[blockIdx.y];
-    zrange_fill_wg<kLds>(global_ray(j.r), global_ptr(j.pose24), j.K, blockIdx.x, gridDim.x, s_tile, s_box, s_rend);
+    zrange_fill_wg<kLds, kFast>(global_ray(j.r), global_ptr(j.pose24), j.K, blockIdx.x, gridDim.x, s_tile, s_box, s_rend);
 }
 
 // Workgroup-shared, direct-mapped cache of block lookups in LDS, negative
@@ -513,8 +522,14 @@ int lf_raycast_ev(const lf_map_params *p, const lf_map_storage *s, float *vmap, 
     // the per-workgroup tile grid in LDS when it fits (C1: 4800 tiles, 38 KB)
     const size_t lds = 2 * (size_t)T * sizeof(int);
     const int zgrid = zrange_grid(p);
-    if (lds <= zrange_lds_limit()) k_zrange_fill<true><<<zgrid, kZThreads, lds, st>>>(r, pose24, *K);
-    else k_zrange_fill<false><<<zgrid, kZThreads, 0, st>>>(r, pose24, *K);
+    const bool fast = p->precision == LF_PRECISION_MIXED;
+    if (lds <= zrange_lds_limit()) {
+        if (fast) k_zrange_fill<true, true><<<zgrid, kZThreads, lds, st>>>(r, pose24, *K);
+        else k_zrange_fill<true, false><<<zgrid, kZThreads, lds, st>>>(r, pose24, *K);
+    } else {
+        if (fast) k_zrange_fill<false, true><<<zgrid, kZThreads, 0, st>>>(r, pose24, *K);
+        else k_zrange_fill<false, false><<<zgrid, kZThreads, 0, st>>>(r, pose24, *K);
+    }
     LF_LAUNCH_CHECK();
     const int tx = (cols + 7) / 8, ty = (rows + 7) / 8;
     const int wg_x = (tx + 1) / 2, wg_y = (ty + 1) / 2;  // 2x2 tiles per workgroup
@@ -584,10 +599,14 @@ int lf_raycast_batch_launch(const void *jobs_dev, const LfRayBatchDims &d, int c
     const int n = d.n;
     k_zrange_clear_batch<<<dim3(std::max(1, std::min(1024, (d.tiles_max + 255) / 256)), n), 256, 0, st>>>(jobs);
     LF_LAUNCH_CHECK();
-    if (d.zrange_lds <= zrange_lds_limit())
-        k_zrange_fill_batch<true><<<dim3(d.zgrid_max, n), kZThreads, d.zrange_lds, st>>>(jobs);
-    else
-        k_zrange_fill_batch<false><<<dim3(d.zgrid_max, n), kZThreads, 0, st>>>(jobs);
+    const dim3 zg(d.zgrid_max, n);
+    if (d.zrange_lds <= zrange_lds_limit()) {
+        if (d.mixed) k_zrange_fill_batch<true, true><<<zg, kZThreads, d.zrange_lds, st>>>(jobs);
+        else k_zrange_fill_batch<true, false><<<zg, kZThreads, d.zrange_lds, st>>>(jobs);
+    } else {
+        if (d.mixed) k_zrange_fill_batch<false, true><<<zg, kZThreads, 0, st>>>(jobs);
+        else k_zrange_fill_batch<false, false><<<zg, kZThreads, 0, st>>>(jobs);
+    }
     LF_LAUNCH_CHECK();
     const dim3 grid(d.grid_max, n);
     if (d.mixed) {

@@ -105,6 +105,11 @@ LF_DEV Own load_own(const LfTrackLevel &L, int i) {
     return o;
 }
 
+// kHalf (mixed precision) also takes one hardware reciprocal of the depth for
+// every projection and the hardware square root, where the FP32 path keeps
+// the reference's correctly rounded divisions and roots.
+LF_DEV float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 template <bool kIcp, bool kRgb, bool kHalf = false>
 LF_DEV void pixel_rows(const ReduceArgs &a, const Own &o, const float *P, float stddev, float th, float *si,
                        float *sr) {
@@ -112,11 +117,12 @@ LF_DEV void pixel_rows(const ReduceArgs &a, const Own &o, const float *P, float 
     const float4 lv = o.lv, ln = o.ln;
     const float il = o.il;
     const f3 vg = xform(P, f3{lv.x, lv.y, lv.z});
+    const float iz = kHalf ? __builtin_amdgcn_rcpf(vg.z) : 0.0f;
     // ICP correspondence (pose_estimator.cu:168-193)
     bool icp_ok = false;
     size_t j = 0;
     if (kIcp) {
-        const float invz = 1.0f / vg.z;
+        const float invz = kHalf ? iz : 1.0f / vg.z;
         const int u = proj_floor(vg.x * invz * L.K.fx + L.K.cx + 0.5f);
         const int v = proj_floor(vg.y * invz * L.K.fy + L.K.cy + 0.5f);
         icp_ok = !isnan(lv.x) && u >= 0 && v >= 0 && u < L.cols && v < L.rows;
@@ -127,8 +133,8 @@ LF_DEV void pixel_rows(const ReduceArgs &a, const Own &o, const float *P, float 
     float u0 = 0.0f, v0 = 0.0f;
     size_t t0 = 0;
     if (kRgb) {
-        u0 = vg.x / vg.z * L.K.fx + L.K.cx;
-        v0 = vg.y / vg.z * L.K.fy + L.K.cy;
+        u0 = (kHalf ? vg.x * iz : vg.x / vg.z) * L.K.fx + L.K.cx;
+        v0 = (kHalf ? vg.y * iz : vg.y / vg.z) * L.K.fy + L.K.cy;
         rgb_ok = !(lv.w < 0.0f || isnan(lv.x)) && isfinite(il) && u0 >= 2.0f && u0 < (float)(L.cols - 2) &&
                  v0 >= 2.0f && v0 < (float)(L.rows - 2);
         if (rgb_ok) t0 = (size_t)(int)floorf(v0) * L.cols + (int)floorf(u0);
@@ -154,8 +160,9 @@ LF_DEV void pixel_rows(const ReduceArgs &a, const Own &o, const float *P, float 
         const f3 vc = {cv.x, cv.y, cv.z};
         const f3 ng = rotate(P, f3{ln.x, ln.y, ln.z});
         const f3 nc = {cn.x, cn.y, cn.z};
-        const float dist = norm(sub(vg, vc));
-        const float sine = norm(cross(nc, ng));
+        const f3 dv = sub(vg, vc), cn3 = cross(nc, ng);
+        const float dist = kHalf ? fast_sqrt(dot(dv, dv)) : norm(dv);
+        const float sine = kHalf ? fast_sqrt(dot(cn3, cn3)) : norm(cn3);
         if (sine < a.angle && dist <= a.dthr && !isnan(nc.x) && !isnan(ng.x)) {
             const f3 c = cross(vg, ng);
             row[0] = ng.x; row[1] = ng.y; row[2] = ng.z;
@@ -170,16 +177,17 @@ LF_DEV void pixel_rows(const ReduceArgs &a, const Own &o, const float *P, float 
         const float dx = (tx[0] * (1.0f - cx) + tx[1] * cx) * (1.0f - cy) + (tx[2] * (1.0f - cx) + tx[3] * cx) * cy;
         const float dy = (ty[0] * (1.0f - cx) + ty[1] * cx) * (1.0f - cy) + (ty[2] * (1.0f - cx) + ty[3] * cx) * cy;
         if ((dx > 2.0f || dy > 2.0f) && isfinite(ic) && isfinite(dx) && isfinite(dy)) {
-            const float zi = 1.0f / vg.z;
+            const float zi = kHalf ? iz : 1.0f / vg.z;
             f3 l;
             l.x = dx * L.K.fx * zi;
             l.y = dy * L.K.fy * zi;
             l.z = -(l.x * vg.x + l.y * vg.y) * zi;
             float r = ic - il;
             const double sd = (double)stddev;
-            if (sd > 10e-5) r /= stddev;
+            if (sd > 10e-5) r = kHalf ? r * __builtin_amdgcn_rcpf(stddev) : r / stddev;
             float w = 1.0f;
-            if (fabsf(r) > th && sd > 10e-6) w = sqrtf(th / fabsf(r));
+            if (fabsf(r) > th && sd > 10e-6)
+                w = kHalf ? fast_sqrt(th * __builtin_amdgcn_rcpf(fabsf(r))) : sqrtf(th / fabsf(r));
             const f3 c = cross(vg, l);
             row[6] = w * (-r);
             row[0] = l.x * w; row[1] = l.y * w; row[2] = l.z * w;

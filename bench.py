@@ -53,11 +53,28 @@ def cpu_baseline(sc, params, K, frames_d, frames_c, n_frames):
     for i in range(n_frames):
         s.process(frames_d[i], frames_c[i])
     dt = time.perf_counter() - t0
+    # the same oracle on one core, over the first frames only
+    n1 = max(1, min(n_frames, 6))
+    O.set_threads(1)
+    s1 = O.System(op, oK, configs.C1_LEVELS, configs.C1_ITERATIONS, sc.depth_scale, np.float32)
+    t1 = time.perf_counter()
+    for i in range(n1):
+        s1.process(frames_d[i], frames_c[i])
+    dt1 = time.perf_counter() - t1
+    O.set_threads(threads)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": n_frames / dt, "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": "C1 frames 0-%d (640x480, 5 mm), FP32 oracle System, %d OpenMP threads" % (n_frames - 1, threads)}
+            "sample": "C1 frames 0-%d (640x480, 5 mm), FP32 oracle System, %d OpenMP threads" % (n_frames - 1, threads),
+            "one_core": {"value": round(n1 / dt1, 3), "sample": "C1 frames 0-%d, 1 thread" % (n1 - 1)},
+            "nproc": os.cpu_count(), "cpu_model": model}
 
 
-def batch_throughput(n, warmup, steps, single_fps, precision=0, separate=True):
+def batch_throughput(n, warmup, steps, single_fps, precision=0, separate=True, repeats=3):
     """Aggregate frames/s of n independent C1 systems stepped together: system i
     is its own seeded scene and orbit (seed 100 + i) with its own map.  Two
     ways: lf_batch (pyramid passes and Gauss-Newton steps of all systems in one
@@ -90,15 +107,23 @@ def batch_throughput(n, warmup, steps, single_fps, precision=0, separate=True):
         sync()
         return time.perf_counter() - t0
 
-    batch = capi.Batch([capi.system_config(K, p, configs.C1_LEVELS, configs.C1_ITERATIONS, ds) for K, p, ds in cfgs])
-    dt_b = timed(lambda f: batch.process([d[f] for d, _ in frames], [c[f] for _, c in frames]), batch.synchronize)
-    batch.close()
-    del batch
+    # the median of `repeats` runs, each a fresh batch over the same frames
+    runs = []
+    for _ in range(repeats):
+        batch = capi.Batch([capi.system_config(K, p, configs.C1_LEVELS, configs.C1_ITERATIONS, ds)
+                            for K, p, ds in cfgs])
+        runs.append(timed(lambda f: batch.process([d[f] for d, _ in frames], [c[f] for _, c in frames]),
+                          batch.synchronize))
+        batch.close()
+        del batch
+        torch.cuda.empty_cache()
+    dt_b = sorted(runs)[len(runs) // 2]
     v_b = n * steps / dt_b
     name = "plain FP32" if precision == 0 else "mixed-precision"
     out = {"systems": n, "value": round(v_b, 1), "unit": "frames/s (all systems)",
            "ms_per_step": round(1000.0 * dt_b / steps, 4), "vs_single_system": round(v_b / single_fps, 3),
-           "steps": steps, "warmup": warmup,
+           "steps": steps, "warmup": warmup, "runs": [round(n * steps / t, 1) for t in runs],
+           "statistic": "median of %d runs" % repeats,
            "workload": "%d independent %s C1 systems (seeds 100-%d: own scene, orbit and map), one frame "
                        "of each per step" % (n, name, 99 + n),
            "launch": "lf_batch: one descriptor table per step; pyramid passes, Gauss-Newton steps, block allocation, "
@@ -206,8 +231,8 @@ def main():
     elapsed = time.perf_counter() - t0
     c1 = sysm.counters()
     pose = sysm.pose()
-    fps = args.steps / elapsed
-    log("timed %d frames in %.3fs: %.1f frames/s" % (args.steps, elapsed, fps))
+    fps_evented = args.steps / elapsed
+    log("timed %d frames in %.3fs: %.1f frames/s" % (args.steps, elapsed, fps_evented))
 
     n_ig, ms_ig = sysm.read_timing(0)
     integrate_avg_ms = ms_ig / max(n_ig, 1)
@@ -226,6 +251,18 @@ def main():
         for sid, name in stage_names.items():
             n, ms = sysm.read_timing(sid)
             stages[name] = {"launches": n, "avg_ms": ms / max(n, 1)}
+    # two more runs of the same frames without the integrate events; the
+    # reported frame rate is the median of the three
+    run_fps = [fps_evented]
+    for _ in range(2):
+        del sysm
+        torch.cuda.empty_cache()
+        f_r, _, _ = timed_system(K, params, dev_d, dev_c, args.warmup, n_total, sc.depth_scale)
+        run_fps.append(f_r)
+        sysm = None
+    fps = sorted(run_fps)[1]
+    elapsed = args.steps / fps
+    log("runs %s frames/s, median %.1f" % (", ".join("%.1f" % f for f in run_fps), fps))
     vis_sum = (int(c1[capi.CNT_VIS_SUM_LO]) | (int(c1[capi.CNT_VIS_SUM_HI]) << 32)) - \
               (int(c0[capi.CNT_VIS_SUM_LO]) | (int(c0[capi.CNT_VIS_SUM_HI]) << 32))
     updates = int(c1[capi.CNT_UPDATES]) - int(c0[capi.CNT_UPDATES])
@@ -317,8 +354,12 @@ def main():
         "cpu_baseline": cpu,
         "stage_rooflines": stage_rooflines,
         "stages_ms": {k: round(v["avg_ms"], 4) for k, v in stages.items()},
+        "runs": [round(f, 2) for f in run_fps],
+        "statistic": "median of 3 runs over the same frames (the first with HIP events around the integration kernel)",
         "batched": batched,
         "mixed_precision": mixed,
+        "mixed_batched_vs_plain_fp32": (mixed["batched"]["vs_plain_fp32_single_system"]
+                                        if mixed and mixed.get("batched") else None),
         "allocated_blocks": int(min(c1[capi.CNT_ALLOCATED], params.num_blocks)),
         "overflow": int(c1[capi.CNT_OVERFLOW]),
         "final_pose_t": [round(float(x), 7) for x in pose[:3, 3]],

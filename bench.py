@@ -281,7 +281,7 @@ def main():
         traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_integrate<true,true,true> (packed samples)", "algorithmic_bytes_per_launch": int(bytes_per_launch),
+                "kernel": "k_integrate<true,true,true,false> (FP32, packed samples)", "algorithmic_bytes_per_launch": int(bytes_per_launch),
                 "avg_launch_ms": round(integrate_avg_ms, 5), "visible_blocks_avg": round(n_vis, 1)}
 
     # raycast: compulsory bytes (vertex map out, sdf + weight of the distinct

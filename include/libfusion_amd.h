@@ -83,8 +83,9 @@ typedef struct lf_map_params {
 /* Storage precision of a map and of the images and tracking of a System.
  *   FP32  the reference's method: int16 TSDF, float32 weights, float32 vertex /
  *         normal / intensity / gradient maps.
- *   MIXED FP16 TSDF with 8-bit weight codes (lf_weight_code); FP16 vertex,
- *         normal, intensity and gradient maps; the same FP32 normal-equation
+ *   MIXED FP16 TSDF with 8-bit weight codes (lf_weight_code); FP16 normal,
+ *         intensity and gradient maps (vertex maps stay float4: FP16's 2 mm
+ *         step at 2-4 m biased the ICP fit); the same FP32 normal-equation
  *         sums (pairwise: per-lane, wavefront tree, fixed-order final sum) and
  *         FP64 solve and pose updates.  Results are stated against the FP64
  *         method, not bit for bit. */
@@ -311,7 +312,7 @@ int lf_warp_image(const unsigned char *src, const float *vmap_dst, int cols, int
 /* DeviceImage (device_image.h:18-88): one RGB-D frame's device pyramid.  Every
  * level is exactly half of the one above (even sizes down to the last level).
  * The images of a mixed-precision System store INTENSITY, DX and DY as FP16
- * and VMAP and NMAP as 4 x FP16; every other plane is as listed. */
+ * and NMAP as 4 x FP16; every other plane is as listed. */
 typedef struct lf_image lf_image;
 
 enum {

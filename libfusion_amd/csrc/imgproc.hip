@@ -279,8 +279,8 @@ struct Pass {
     int total_wg;  // workgroups of all parts
 };
 
-// Pixel (x, y) of a part.  kHalf (mixed precision): the intensity, gradient,
-// vertex and normal planes are FP16 (read and written; depth, colour and
+// Pixel (x, y) of a part.  kHalf (mixed precision): the intensity, gradient
+// and normal planes are FP16 (read and written; depth, vertices, colour and
 // samples are not), and the bilateral weights, normals and sample weights use
 // the hardware exponential, reciprocal square root, sine and reciprocal.
 template <bool kHalf>
@@ -309,7 +309,7 @@ LF_DEV void frame_part(const Part &p, int x, int y) {
         derivative_at(PlaneAt<kHalf>{p.in1, w}, w, h, x, y, gx, gy);
         store1<kHalf>(p.out0, i, gx);
         store1<kHalf>(p.out1, i, gy);
-        store4<kHalf>(p.out2, i, back_project_at(d[i], x, y, p.K));
+        store4<false>(p.out2, i, back_project_at(d[i], x, y, p.K));  // vertices stay FP32
         const float4 n = nmap_at<DepthVertexAt, kHalf>(DepthVertexAt{d, w, p.K}, w, h, x, y);
         store4<kHalf>(p.out3, i, n);
         if (p.out4) {  // level 0: the pixel's integration sample
@@ -320,13 +320,13 @@ LF_DEV void frame_part(const Part &p, int x, int y) {
         break;
     }
     case kNmap:  // normal map of a stored vertex map
-        store4<kHalf>(p.out0, i, nmap_at<Vec4At<kHalf>, kHalf>(Vec4At<kHalf>{p.in0, w}, w, h, x, y));
+        store4<kHalf>(p.out0, i, nmap_at<Vec4At<false>, kHalf>(Vec4At<false>{p.in0, w}, w, h, x, y));
         break;
     case kVmapDown:  // half-size vertex map
-        store4<kHalf>(p.out0, i, resize4_at(Vec4At<kHalf>{p.in0, 2 * w}, x, y));
+        store4<false>(p.out0, i, resize4_at(Vec4At<false>{p.in0, 2 * w}, x, y));
         break;
     case kVmapDown2:  // quarter-size vertex map, through the half-size one
-        store4<kHalf>(p.out0, i, resize4_quarter_at(Vec4At<kHalf>{p.in0, 4 * w}, x, y));
+        store4<false>(p.out0, i, resize4_quarter_at(Vec4At<false>{p.in0, 4 * w}, x, y));
         break;
     }
 }

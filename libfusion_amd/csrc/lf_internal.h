@@ -36,7 +36,7 @@ struct LfImageLevels {
     float *depth[8], *intensity[8], *dx[8], *dy[8];
     float *vmap[8], *nmap[8];            // float4 per pixel
     float *samples;                      // level 0 integration samples, float4 per pixel (may be null)
-    int half_maps;                       // intensity / dx / dy are FP16, vmap / nmap 4 x FP16
+    int half_maps;                       // intensity / dx / dy are FP16, nmap 4 x FP16 (vmap stays float4)
 };
 
 // DeviceImage::upload (device_image.cpp:146-181) in levels + 1 launches.  Every
@@ -61,7 +61,7 @@ int lf_pass_workgroups(const void *pass);
 void lf_pass_skip(void *pass);
 int lf_pass_batch(const void *passes, int n, int max_wg, hipStream_t st);
 
-// lf_render_scene over float4 maps, or 4 x FP16 maps (half_maps).
+// lf_render_scene over float4 maps, or a 4 x FP16 normal map (half_maps).
 int lf_render_scene_maps(const void *vmap, const void *nmap, int half_maps, const unsigned char *rgb, int cols,
                          int rows, unsigned char *rgba_out, hipStream_t st);
 

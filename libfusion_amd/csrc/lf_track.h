@@ -31,7 +31,7 @@ struct LfTrackLevel {
     const float *curr_I, *last_I, *dIdx, *dIdy;
     int cols, rows;
     lf_intrinsics K;
-    int half_maps;  // the maps hold FP16 (vmap / nmap 4 x FP16, intensity and gradients FP16)
+    int half_maps;  // the maps hold FP16 (nmap 4 x FP16, intensity and gradients FP16; vmaps stay float4)
 };
 
 // Scratch floats needed by the reductions (partial rows of the workgroups).

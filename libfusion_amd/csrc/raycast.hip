@@ -363,7 +363,7 @@ LF_DEV float read_interp(const RayDev &r, BlockCache &bc, f3 p, bool &valid) {
 // ::raycast_with_colour (check_colour == true): ray_tracing.cu:346-425,478-564.
 // Workgroup wg (its index within this raycast's grid; the grid's x extent is
 // a multiple of 8, so wg % 8 is the XCD the workgroup runs on).  kMixed: FP16
-// voxels in, 4 x FP16 vertices out.
+// voxels in (the vertices out are float4 in both precisions).
 template <bool kCheckColour, bool kMixed>
 LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char *__restrict__ rgb_out, int cols,
                        int rows, const float *__restrict__ pose24, const lf_intrinsics &K, int wg_x, int wg_y, int wg,
@@ -440,13 +440,7 @@ LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char 
         const f3 q = xform(Tcw, mul(res, r.voxel_size));
         res4 = make_float4(q.x, q.y, q.z, 1.0f);
     } while (false);
-    if constexpr (kMixed) {
-        const unsigned int lo = float_to_half_bits(res4.x) | ((unsigned int)float_to_half_bits(res4.y) << 16);
-        const unsigned int hi = float_to_half_bits(res4.z) | ((unsigned int)float_to_half_bits(res4.w) << 16);
-        reinterpret_cast<uint2 *>(vmap)[px] = make_uint2(lo, hi);
-    } else {
-        reinterpret_cast<float4 *>(vmap)[px] = res4;
-    }
+    reinterpret_cast<float4 *>(vmap)[px] = res4;
     if (rgb_out) {
         unsigned char *o = rgb_out + 3 * ((size_t)y * cols + x);
         o[0] = col[0]; o[1] = col[1]; o[2] = col[2];
@@ -535,7 +529,7 @@ int lf_raycast_ev(const lf_map_params *p, const lf_map_storage *s, float *vmap, 
     const int wg_x = (tx + 1) / 2, wg_y = (ty + 1) / 2;  // 2x2 tiles per workgroup
     const int g = kXcds * ((wg_x * wg_y + kXcds - 1) / kXcds);
     if (e0) LF_HIP_CHECK(hipEventRecord(e0, st));
-    if (p->precision == LF_PRECISION_MIXED) {  // FP16 voxels, 4 x FP16 vertex map
+    if (p->precision == LF_PRECISION_MIXED) {  // FP16 voxels
         if (check_colour) k_raycast<true, true><<<g, 256, 0, st>>>(r, vmap, rgb_out, cols, rows, pose24, *K, wg_x, wg_y);
         else k_raycast<false, true><<<g, 256, 0, st>>>(r, vmap, rgb_out, cols, rows, pose24, *K, wg_x, wg_y);
     } else {

@@ -97,7 +97,7 @@ LF_DEV float plane1(const float *p, size_t i) {
 template <bool kIcp, bool kRgb, bool kHalf = false>
 LF_DEV Own load_own(const LfTrackLevel &L, int i) {
     Own o;
-    o.lv = map4<kHalf>(L.last_vmap, i);
+    o.lv = map4<false>(L.last_vmap, i);  // vertices are FP32 in both precisions
     o.ln = make_float4(0.f, 0.f, 0.f, 0.f);
     o.il = 0.0f;
     if (kIcp) o.ln = map4<kHalf>(L.last_nmap, i);
@@ -142,7 +142,7 @@ LF_DEV void pixel_rows(const ReduceArgs &a, const Own &o, const float *P, float 
     // gathers
     float4 cv = make_float4(0.f, 0.f, 0.f, 0.f), cn = cv;
     if (kIcp) {
-        cv = map4<kHalf>(L.curr_vmap, j);
+        cv = map4<false>(L.curr_vmap, j);
         cn = map4<kHalf>(L.curr_nmap, j);
     }
     float tI[4], tx[4], ty[4];

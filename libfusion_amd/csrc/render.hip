@@ -39,14 +39,14 @@ LF_DEV float4 half4_at(const void *p, size_t i) {
 }
 
 // renderSceneK / renderSceneTexturedK (cuda_imgproc.cu:50-108); rgb null: plain.
-// kHalf: the maps are 4 x FP16.
+// kHalf: the normal map is 4 x FP16.
 template <bool kHalf>
 __global__ void k_render(const void *__restrict__ vmap, const void *__restrict__ nmap,
                          const unsigned char *__restrict__ rgb, int cols, int rows, uchar4 *__restrict__ out) {
     const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= cols || y >= rows) return;
     const size_t i = (size_t)y * cols + x;
-    const float4 v = kHalf ? half4_at(vmap, i) : static_cast<const float4 *>(vmap)[i];
+    const float4 v = static_cast<const float4 *>(vmap)[i];  // vertices are FP32 in both precisions
     const float4 n = kHalf ? half4_at(nmap, i) : static_cast<const float4 *>(nmap)[i];
     f3 pixel = {1.f, 1.f, 1.f};
     if (rgb) pixel = f3{(float)rgb[3 * i] / 255.f, (float)rgb[3 * i + 1] / 255.f, (float)rgb[3 * i + 2] / 255.f};

@@ -168,9 +168,9 @@ DeviceImage::DeviceImage(const IntrinsicMatrix &K, int num_pyr, hipStream_t stre
     vmap_.resize(num_pyr); nmap_.resize(num_pyr);
     for (int l = 0; l < num_pyr; ++l) {
         const int w = K_[l].width, h = K_[l].height;
-        const int e1 = half_maps ? 2 : 4, e4 = 4 * e1;  // FP16 or FP32 components
+        const int e1 = half_maps ? 2 : 4, e4 = 4 * e1;  // FP16 or FP32 components; vertices always FP32
         depth_[l].create(w, h, 4); intensity_[l].create(w, h, e1); dx_[l].create(w, h, e1); dy_[l].create(w, h, e1);
-        vmap_[l].create(w, h, e4); nmap_[l].create(w, h, e4);
+        vmap_[l].create(w, h, 16); nmap_[l].create(w, h, e4);
     }
 }
 

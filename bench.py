@@ -316,8 +316,8 @@ def main():
         mixed = {"value": round(m_fps, 2), "unit": "frames/s", "ms_per_step": round(1000.0 * m_dt / args.steps, 4),
                  "vs_plain_fp32": round(m_fps / fps, 3),
                  "final_pose_t": [round(float(x), 7) for x in m_pose[:3, 3]],
-                 "storage": "FP16 TSDF + 8-bit weight codes + 3 x u8 colour (6 B/voxel); FP16 vertex / normal / "
-                            "intensity / gradient maps; FP32 pairwise normal-equation sums; FP64 solve and SE(3)"}
+                 "storage": "FP16 TSDF + 8-bit weight codes + 3 x u8 colour (6 B/voxel); FP16 normal / intensity / "
+                            "gradient maps (vertex maps FP32); FP32 pairwise normal-equation sums; FP64 solve and SE(3)"}
         if args.batch > 0:
             torch.cuda.empty_cache()
             mb = batch_throughput(args.batch, args.warmup, args.batch_steps, fps, capi.PRECISION_MIXED,

@@ -358,11 +358,24 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
         size_t pix[2];
         float ptz[2], dist[2], nz[2], wpk[2];
         unsigned int cpk[2];
+        f3 pt0 = {0.f, 0.f, 0.f};
+        if constexpr (kMixed) {
+            const float wx = (float)(p.x * 8 + lx) * vs, wy = (float)(p.y * 8 + ly) * vs, wz = (float)(p.z * 8 + lz0) * vs;
+            pt0 = xform(Tcw, f3{wx, wy, wz});
+        }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int lz = lz0 + 4 * h;
-            const float wx = (float)(p.x * 8 + lx) * vs, wy = (float)(p.y * 8 + ly) * vs, wz = (float)(p.z * 8 + lz) * vs;
-            const f3 pt = xform(Tcw, f3{wx, wy, wz});
+            f3 pt;
+            if constexpr (kMixed) {
+                // mixed: the second voxel, 4 voxels along z, is one step of the transform's z column
+                pt = h == 0 ? pt0
+                            : f3{pt0.x + Tcw[2] * (4.0f * vs), pt0.y + Tcw[6] * (4.0f * vs), pt0.z + Tcw[10] * (4.0f * vs)};
+            } else {
+                const float wx = (float)(p.x * 8 + lx) * vs, wy = (float)(p.y * 8 + ly) * vs,
+                            wz = (float)(p.z * 8 + lz) * vs;
+                pt = xform(Tcw, f3{wx, wy, wz});
+            }
             int u, v;
             if constexpr (kMixed) {
                 const float iz = __builtin_amdgcn_rcpf(pt.z);

@@ -66,3 +66,18 @@ def keyframe_pair(sc, a, b, perturb=(0.01, 0.5), seed=0):
     P[:3, 3] = t
     init = P @ np.linalg.inv(truth)
     return K, oK, out[0], out[1], truth, init
+
+
+def triangles_in_hash_order(tri):
+    """Triangles (n,3,3) sorted by a 64-bit mix of their 9 float32 bit patterns
+    (-0.0 taken as 0.0): two meshes hold bit-identical triangle multisets iff
+    the results are equal (a hash tie between different triangles can only
+    make equal multisets compare unequal). Linear-time stand-in for
+    canonical_triangles on meshes of millions of triangles."""
+    t = np.ascontiguousarray(np.asarray(tri, np.float32) + np.float32(0.0)).reshape(len(tri), 9)
+    u = t.view(np.uint32).astype(np.uint64)
+    h = np.full(len(t), 0x9E3779B97F4A7C15, np.uint64)
+    for k in range(9):
+        h = (h ^ u[:, k]) * np.uint64(0xBF58476D1CE4E5B9)
+        h ^= h >> np.uint64(31)
+    return t[np.argsort(h, kind="stable")]

@@ -16,7 +16,7 @@ import torch
 from libfusion_amd import capi, configs
 from libfusion_amd.synth import Scene
 from oracle import oracle as O
-from tests.helpers import canonical_triangles, pose_err, sort_blocks, vmap_compare
+from tests.helpers import canonical_triangles, pose_err, sort_blocks, triangles_in_hash_order, vmap_compare
 
 pytestmark = pytest.mark.gpu
 
@@ -283,6 +283,59 @@ def test_mesh_capacity_overflow_reported(c0):
     gmap.update_weighted(dev(depth), dev(nmap), dev(rgb), dev(capi.pose24(T)), K)
     tri, _, count = gmap.mesh(mode=0, max_triangles=100)
     assert len(tri) == 100 and count > 100
+
+
+def test_c4_large_volume_fusion_raycast_and_mesh():
+    """Large volume: the 8x8x3 m multi-room (seed 4) at 4 mm voxels, 12 mm
+    truncation and 2^22 buckets, fused from 20 keyframes spread over its
+    1000-frame random walk (> 100k blocks, 2^17 block pool headroom). The world
+    frame is shifted 3 m along x so that a tenth of the blocks (over a quarter
+    of those seen from the last keyframe) lie beyond the block directory and
+    resolve through the hash table. The block
+    set is exact and the TSDF within the bar; on the identical volume the
+    1280x960 raycasts from two poses (visible sets, vertex maps, colours) are
+    bit-exact and the full mesh (~12M triangles) is the same multiset of
+    triangles, bit for bit."""
+    sc = Scene(4, 4)
+    K = capi.intrinsics(sc.width, sc.height, sc.fx, sc.fy, sc.cx, sc.cy)
+    p = capi.map_params(2 ** 22, 1 << 18, 0.004, 0.012)
+    shift = np.eye(4)
+    shift[0, 3] = 3.0
+    gmap = capi.Map(p, 2 * K.width, 2 * K.height)
+    omap = O.Map(o_params(p), np.float32)
+    for f in range(0, 1000, 50):
+        T = shift @ sc.pose(f)
+        depth, nmap, rgb = frame_inputs(sc, sc.pose(f), K, f)
+        gmap.update_weighted(dev(depth), dev(nmap), dev(rgb), dev(capi.pose24(T)), K)
+        omap.update_weighted(depth, nmap, rgb, T, o_K(K))
+    torch.cuda.synchronize()
+    _, n = compare_maps(gmap, omap)
+    assert n > 100_000
+    lib = capi.load()
+    outside = sum(lib.lf_dir_cell(int(x), int(y), int(z)) < 0 for x, y, z in gmap.blocks())
+    assert 0.05 * n < outside < 0.5 * n, (outside, n)
+    load_gpu_volume_into_oracle(gmap, omap)
+
+    W, H = 2 * K.width, 2 * K.height
+    K2 = capi.intrinsics(W, H, 2 * K.fx, 2 * K.fy, 2 * K.cx + 0.5, 2 * K.cy + 0.5)
+    for f in (975, 510):
+        T = shift @ sc.pose(f)
+        pose = dev(capi.pose24(T))
+        assert gmap.select_visible(pose, K2, W, H) == omap.select_visible(T, o_K(K2), W, H) > 5000
+        vm = torch.full((H, W, 4), float("nan"), device="cuda")
+        rgb_out = torch.zeros((H, W, 3), dtype=torch.uint8, device="cuda")
+        gmap.raycast_check_visibility(vm, pose, K2, rgb_out=rgb_out)
+        torch.cuda.synchronize()
+        ovm, orgb = omap.raycast(T, o_K(K2), W, H, want_rgb=True)
+        mism, dmax, nboth = vmap_compare(vm.cpu().numpy(), ovm)
+        assert nboth > 0.5 * W * H
+        assert mism == 0.0 and dmax == 0.0, (f, mism, dmax)
+        assert np.array_equal(rgb_out.cpu().numpy(), orgb)
+
+    gtri, _, gcount = gmap.mesh(mode=0, max_triangles=16_000_000)
+    otri, _, ocount = omap.mesh(mode=0)
+    assert gcount == ocount > 5_000_000 and len(gtri) == gcount
+    assert np.array_equal(triangles_in_hash_order(gtri.cpu().numpy()), triangles_in_hash_order(otri))
 
 
 def test_image_pipeline_matches_oracle():

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <climits>
+#include <type_traits>
 
 #include "libfusion_amd.h"
 
@@ -104,7 +105,9 @@ LF_HD unsigned short float_to_half_bits(float v) { return __builtin_bit_cast(uns
 // e >= 1 is (1 + m/16) * 2^(e-7), up to 496.
 LF_HD float weight_value(unsigned int c) {
     const unsigned int e = c >> 4, m = c & 15u;
-    return e == 0 ? (float)m * (1.0f / 1024.0f) : __builtin_bit_cast(float, ((e + 120u) << 23) | (m << 19));
+    const float sub = (float)m * (1.0f / 1024.0f);
+    const float nrm = __builtin_bit_cast(float, ((e + 120u) << 23) | (m << 19));
+    return e == 0 ? sub : nrm;  // both computed: a select, not a branch
 }
 // Nearest code of w >= 0, ties to the even code; above 496, 0xFF.  Below 2^-6
 // the code is round(w * 1024) (exact scaling; 16 rounds up into code 0x10 =
@@ -112,11 +115,20 @@ LF_HD float weight_value(unsigned int c) {
 // mantissa bits rounded to 4 by adding just under half a unit plus the kept
 // part's last bit; a carry moves into the exponent as it should.
 LF_HD unsigned int weight_code(float w) {
-    if (!(w > 0.0f)) return 0u;
-    if (w < 0.015625f) return (unsigned int)__builtin_rintf(w * 1024.0f);
+    // every case computed and selected (no branch); r wraps for w < 2^-6,
+    // whose code is the scaled one
+    const unsigned int small = (unsigned int)__builtin_rintf(fminf(fmaxf(w, 0.0f), 0.015625f) * 1024.0f);
     const unsigned int r = __builtin_bit_cast(unsigned int, w) - (120u << 23);
     const unsigned int c = (r + 0x3FFFFu + ((r >> 19) & 1u)) >> 19;
-    return c > 0xFFu ? 0xFFu : c;
+    const unsigned int big = c > 0xFFu ? 0xFFu : c;
+    return !(w > 0.0f) ? 0u : w < 0.015625f ? small : big;
+}
+
+// *(T *)((char *)base + off): a 32-bit byte offset from a (scalar) base, so
+// the access takes the offset in one VGPR instead of 64-bit address math.
+template <class T> LF_DEV T &at_byte(T *base, unsigned int off) {
+    using C = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return *reinterpret_cast<T *>(reinterpret_cast<C *>(base) + off);
 }
 
 // Occupancy bit of a block: its 4x4x4-block region hashed like computeHash.

@@ -351,11 +351,22 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
     // issuing both voxels' loads before anything waits on them: the
     // projections and input gathers, then (inside the truncation band) the
     // voxel's weight, sdf and colour together, then the fusion and stores.
+    // The block's list entry is the same for every lane (scalar registers,
+    // loaded one iteration ahead); voxel and pixel addresses are 32-bit byte
+    // offsets from a scalar base.
+    const int4 *vpos = reinterpret_cast<const int4 *>(m.visible_pos);  // x, y, z, pool block
+    int4 p_next = wg < nvis ? vpos[wg] : make_int4(0, 0, 0, 0);
     for (int i = wg; i < nvis; i += nwg) {
-        const int4 p = reinterpret_cast<const int4 *>(m.visible_pos)[i];  // x, y, z, pool block
-        const int b = p.w;
+        const int4 p = make_int4(__builtin_amdgcn_readfirstlane(p_next.x), __builtin_amdgcn_readfirstlane(p_next.y),
+                                 __builtin_amdgcn_readfirstlane(p_next.z), __builtin_amdgcn_readfirstlane(p_next.w));
+        p_next = vpos[min(i + nwg, nvis - 1)];  // unconditional: no wait at a join
+        const size_t b = (size_t)p.w;
+        short *sdf_b = m.sdf + b * LF_BLOCK3;
+        float *w_b = m.weight + b * LF_BLOCK3;
+        unsigned char *w8_b = reinterpret_cast<unsigned char *>(m.weight) + b * LF_BLOCK3;
+        unsigned char *c_b = m.rgb + b * (3 * LF_BLOCK3);
         bool ok[2];
-        size_t pix[2];
+        unsigned int pix[2];
         float ptz[2], dist[2], nz[2], wpk[2];
         unsigned int cpk[2];
         f3 pt0 = {0.f, 0.f, 0.f};
@@ -386,61 +397,68 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
                 v = proj_floor(K.fy * pt.y / pt.z + K.cy + 0.5f);
             }
             ok[h] = !(u < 0 || v < 0 || u > cols - 1 || v > rows - 1);
-            pix[h] = ok[h] ? (size_t)v * cols + u : 0;
+            pix[h] = ok[h] ? (unsigned int)(v * cols + u) : 0u;
             ptz[h] = pt.z;
             dist[h] = 0.0f; nz[h] = 0.0f; wpk[h] = 0.0f; cpk[h] = 0u;
         }
+        // The gathers and voxel loads below are unconditional, so that both
+        // voxels' loads issue before the first wait: a lane with nothing to
+        // load reads pixel 0, or voxel 0 of its block, and its values are
+        // never used.
         float nx[2] = {0.0f, 0.0f};
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            if (!ok[h]) continue;
             if (kPacked) {
-                const float4 smp = samples[pix[h]];
+                const float4 smp = at_byte(samples, 16u * pix[h]);
                 dist[h] = smp.x;  // NaN also when the normal is invalid
                 wpk[h] = smp.y;
                 cpk[h] = __float_as_uint(smp.z);
             } else {
-                dist[h] = depth[pix[h]];
+                dist[h] = at_byte(depth, 4u * pix[h]);
                 if (kWeighted) {
-                    nx[h] = nmap[4 * pix[h]];
-                    nz[h] = nmap[4 * pix[h] + 2];
+                    nx[h] = at_byte(nmap, 16u * pix[h]);
+                    nz[h] = at_byte(nmap, 16u * pix[h] + 8u);
                 }
             }
         }
-        float sdf[2], wp[2];
+        float sdf[2], wraw[2];
+        unsigned int wcode[2], vo[2];
         short sp[2];
         unsigned char cold[2][3];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            if (!ok[h]) continue;
             const float d = dist[h];
-            if (kPacked) ok[h] = !(isnan(d) || d > zmax || d < zmin);
-            else if (kWeighted) ok[h] = !(isnan(d) || isnan(nx[h]) || d > zmax || d < zmin);
-            else ok[h] = !(isnan(d) || (double)d < 1e-2 || d > zmax || d < zmin);
-            if (!ok[h]) continue;
+            bool in = ok[h];
+            if (kPacked) in = in && !(isnan(d) || d > zmax || d < zmin);
+            else if (kWeighted) in = in && !(isnan(d) || isnan(nx[h]) || d > zmax || d < zmin);
+            else in = in && !(isnan(d) || (double)d < 1e-2 || d > zmax || d < zmin);
             sdf[h] = d - ptz[h];
-            ok[h] = !(sdf[h] < -trunc);
-            if (!ok[h]) continue;
-            const size_t vi = (size_t)b * LF_BLOCK3 + (size_t)(t + 256 * h);
+            in = in && !(sdf[h] < -trunc);
+            ok[h] = in;
+            vo[h] = (unsigned int)(t + 256 * h);
+            const unsigned int o = in ? vo[h] : 0u;
             if constexpr (kMixed) {
-                wp[h] = weight_value(reinterpret_cast<const unsigned char *>(m.weight)[vi]);
+                wcode[h] = at_byte(w8_b, o);
             } else {
-                wp[h] = m.weight[vi];
+                wraw[h] = at_byte(w_b, 4u * o);
             }
-            sp[h] = m.sdf[vi];
+            sp[h] = at_byte(sdf_b, 2u * o);
             if (kColour) {
-                const unsigned char *c = m.rgb + (size_t)b * (3 * LF_BLOCK3) + (size_t)(t + 256 * h);
-                cold[h][0] = c[0]; cold[h][1] = c[LF_BLOCK3]; cold[h][2] = c[2 * LF_BLOCK3];
+                cold[h][0] = at_byte(c_b, o);
+                cold[h][1] = at_byte(c_b, o + LF_BLOCK3);
+                cold[h][2] = at_byte(c_b, o + 2 * LF_BLOCK3);
             }
         }
+        float wp[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) wp[h] = kMixed ? weight_value(wcode[h]) : wraw[h];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             if (!ok[h]) continue;
             const float d = dist[h];
             const float sdf_h = fminf(1.0f, sdf[h] * inv_trunc);
-            const size_t vi = (size_t)b * LF_BLOCK3 + (size_t)(t + 256 * h);
+            const unsigned int o = vo[h];
             const float wt = kPacked ? wpk[h] : kWeighted ? fabsf(sinf(nz[h])) / (d * d) : 1.0f / (d * d);
-            unsigned char *c = m.rgb + (size_t)b * (3 * LF_BLOCK3) + (size_t)(t + 256 * h);
             unsigned char cin[3];
             if (kColour) {
 #pragma unroll
@@ -448,50 +466,51 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
                     cin[k] = kPacked ? (unsigned char)(cpk[h] >> (8 * k)) : rgb[3 * pix[h] + k];
             }
             if constexpr (kMixed) {
-                unsigned short *sdf16 = reinterpret_cast<unsigned short *>(m.sdf);
-                unsigned char *w8 = reinterpret_cast<unsigned char *>(m.weight);
+                unsigned short *sdf16 = reinterpret_cast<unsigned short *>(sdf_b);
                 if (wp[h] == 0.0f) {
                     // a weight whose code is 0 leaves the voxel unseen
                     const unsigned int code = weight_code(fminf(wt, 255.0f));
-                    sdf16[vi] = code == 0u ? kHalfUnseen : float_to_half_bits(sdf_h);
-                    w8[vi] = (unsigned char)code;
+                    at_byte(sdf16, 2u * o) = code == 0u ? kHalfUnseen : float_to_half_bits(sdf_h);
+                    at_byte(w8_b, o) = (unsigned char)code;
                     if (kColour) {
-                        c[0] = cin[0]; c[LF_BLOCK3] = cin[1]; c[2 * LF_BLOCK3] = cin[2];
+                        at_byte(c_b, o) = cin[0]; at_byte(c_b, o + LF_BLOCK3) = cin[1];
+                        at_byte(c_b, o + 2 * LF_BLOCK3) = cin[2];
                     }
                     continue;
                 }
                 const float inv = __builtin_amdgcn_rcpf(wp[h] + wt);
                 const float spf = half_bits_to_float((unsigned short)sp[h]);
-                sdf16[vi] = float_to_half_bits((spf * wp[h] + sdf_h * wt) * inv);
-                w8[vi] = (unsigned char)weight_code(fminf(wp[h] + wt, 255.0f));
+                at_byte(sdf16, 2u * o) = float_to_half_bits((spf * wp[h] + sdf_h * wt) * inv);
+                at_byte(w8_b, o) = (unsigned char)weight_code(fminf(wp[h] + wt, 255.0f));
                 if (kColour) {
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
                         const float mc = ((float)cold[h][k] * wp[h] + (float)cin[k] * wt) * inv;
-                        c[k * LF_BLOCK3] = (unsigned char)(int)fminf(mc, 255.0f);
+                        at_byte(c_b, o + k * LF_BLOCK3) = (unsigned char)(int)fminf(mc, 255.0f);
                     }
                 }
                 continue;
             }
             if (wp[h] == 0.0f) {
                 // weight stays 0 only when wt is 0: keep the voxel's sdf code unseen
-                m.sdf[vi] = wt == 0.0f ? kSdfUnseen : pack_sdf(sdf_h);
-                m.weight[vi] = wt > 255.0f ? 255.0f : wt;
+                at_byte(sdf_b, 2u * o) = wt == 0.0f ? kSdfUnseen : pack_sdf(sdf_h);
+                at_byte(w_b, 4u * o) = wt > 255.0f ? 255.0f : wt;
                 if (kColour) {
-                    c[0] = cin[0]; c[LF_BLOCK3] = cin[1]; c[2 * LF_BLOCK3] = cin[2];
+                    at_byte(c_b, o) = cin[0]; at_byte(c_b, o + LF_BLOCK3) = cin[1];
+                    at_byte(c_b, o + 2 * LF_BLOCK3) = cin[2];
                 }
                 continue;
             }
             const float spf = unpack_sdf(sp[h]);
             const float fused = (spf * wp[h] + sdf_h * wt) / (wp[h] + wt);
-            m.sdf[vi] = pack_sdf(fused);
+            at_byte(sdf_b, 2u * o) = pack_sdf(fused);
             const float wn = wp[h] + wt;
-            m.weight[vi] = wn > 255.0f ? 255.0f : wn;
+            at_byte(w_b, 4u * o) = wn > 255.0f ? 255.0f : wn;
             if (kColour) {
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     const float mixed = ((float)cold[h][k] * wp[h] + (float)cin[k] * wt) / (wp[h] + wt);
-                    c[k * LF_BLOCK3] = (unsigned char)(int)mixed;
+                    at_byte(c_b, o + k * LF_BLOCK3) = (unsigned char)(int)mixed;
                 }
             }
         }

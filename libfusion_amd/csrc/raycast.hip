@@ -482,8 +482,11 @@ __global__ __launch_bounds__(256) void k_raycast(RayDev r, float *__restrict__ v
     raycast_wg<kCheckColour, kMixed>(r, vmap, rgb_out, cols, rows, pose24, K, wg_x, wg_y, blockIdx.x, s_cache, s_occ);
 }
 
+// Many maps' rays fill the chip many times over, so this kernel is held to 64
+// VGPRs for 8 waves per SIMD; the single-map kernel (about 5 waves per SIMD
+// in all) keeps the compiler's own allocation, whose sample chain is shorter.
 template <bool kCheckColour, bool kMixed>
-__global__ __launch_bounds__(256) void k_raycast_batch(const RayJob *__restrict__ jobs) {
+__global__ __launch_bounds__(256, 8) void k_raycast_batch(const RayJob *__restrict__ jobs) {
     __shared__ unsigned long long s_cache[kLdsCache];
     __shared__ uint4 s_occ[LF_OCC_WORDS / 4];
     const RayJob &j = jobs[blockIdx.y];

@@ -667,6 +667,8 @@ __global__ __launch_bounds__(256) void k_deintegrate(MapDev m, const float *__re
     }
 }
 
+// one map at 640x480 / 5 mm (~40k visible blocks): 4096 workgroups measured
+// 70 us, 8192 and 16384 both 66 us
 int integrate_grid() { return 8192; }
 
 int check_update_args(const lf_map_params *p, const lf_map_storage *s, const float *depth, int cols, int rows,

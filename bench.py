@@ -1,17 +1,28 @@
-"""Benchmark: frames/s of the dense-reconstruction pipeline on one MI355X.
+"""Benchmark: frames/s at 640x480, 5 mm voxels, mixed precision + batched
+against the plain FP32 path, on one MI355X.
 
-Workload (C1): 300 synthetic 640x480 RGB-D frames of a 4x4x3 m room with 8
-seeded primitives along a 1.5 m orbit, 5 mm voxels, 15 mm truncation, 2^20
-buckets, 3-level ICP+RGB tracking with {10, 5, 3} Gauss-Newton iterations.
-A step is one frame of System::process_images: image pyramid -> tracking ->
-block allocation + TSDF integration -> raycast -> vertex/normal re-pyramid.
-All frames are resident in HBM before the timed region starts.
+Workload: 16 independent synthetic RGB-D streams at 640x480 (seeds 100-115:
+each its own 4x4x3 m room with 8 seeded primitives, Perlin albedo, depth noise
+and 2% dropouts, and its own 1.5 m orbit), 5 mm voxels, 15 mm truncation,
+2^20 buckets, 3-level ICP + RGB tracking with {10, 5, 3} Gauss-Newton
+iterations.  A step is one frame of every stream through System::process_images
+(upload -> image pyramid -> tracking -> block allocation + TSDF integration ->
+raycast -> vertex/normal re-pyramid).  All frames are resident in HBM before
+the timed region starts.
 
-Prints one JSON line.  `roofline` describes the TSDF integration kernel
-(achieved algorithmic bytes / its average HIP-event duration, measured inside
-the timed region on the system's stream); `cpu_baseline` is the CPU oracle's
-FP32 instantiation (the same method, OpenMP over the host cores) on a bounded
-sample of the same workload.
+value       mixed precision (FP16 TSDF + 8-bit weights, FP16 normal / intensity /
+            gradient maps, FP32 sums, FP64 solve) with the 16 streams stepped
+            together by lf_batch: aggregate frames/s over exactly --steps timed
+            steps after --warmup steps, median of 3 runs.
+vs_plain_fp32  value / the plain FP32 GPU path (the reference's method, one
+            System per stream) over the same streams and the same frames.
+
+Also printed: the integration kernel's roofline (algorithmic bytes / its
+average HIP-event duration on the batch stream, inside the first timed run),
+the raycast's, a stage breakdown, the same step with every frame copied from
+pinned host memory (PCIe-inclusive), the uneven 16-stream batched
+configuration, and the CPU oracle (the same method in FP32, OpenMP over the
+host cores) on a bounded sample of the same streams.
 """
 import argparse
 import json
@@ -25,6 +36,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "frames/s at 640x480, 5 mm voxels: mixed-prec+batched vs plain FP32"
+SEED0 = 100  # streams are seeds SEED0 .. SEED0 + n - 1
 
 
 def log(*a):
@@ -39,28 +52,37 @@ def make_frames(sc, n):
     return d, c
 
 
-def cpu_baseline(sc, params, K, frames_d, frames_c, n_frames):
+def cpu_baseline(params, n_frames, seconds, seconds_one):
+    """The FP32 oracle System (the same method, OpenMP over the host cores) on
+    the first frames of stream SEED0: frames until n_frames or `seconds` of
+    work, then the same on one core until `seconds_one`.  It generates its own
+    frames, so the sample never depends on the GPU legs' frame counts."""
+    from libfusion_amd import configs
+    from libfusion_amd.synth import Scene
     from oracle import oracle as O
 
-    threads = O.max_threads()
+    sc = Scene(1, SEED0)
+    K = _intrinsics(sc)
     op = O.Params(params.num_buckets, params.num_entries, params.num_blocks, params.max_triangles,
                   params.zmin_raycast, params.zmax_raycast, params.zmin_update, params.zmax_update,
                   params.voxel_size, params.truncation)
     oK = O.Intr(K.width, K.height, K.fx, K.fy, K.cx, K.cy, K.invfx, K.invfy)
-    from libfusion_amd import configs
-    s = O.System(op, oK, configs.C1_LEVELS, configs.C1_ITERATIONS, sc.depth_scale, np.float32)
-    t0 = time.perf_counter()
-    for i in range(n_frames):
-        s.process(frames_d[i], frames_c[i])
-    dt = time.perf_counter() - t0
-    # the same oracle on one core, over the first frames only
-    n1 = max(1, min(n_frames, 6))
+
+    def run(limit_frames, limit_s):
+        s = O.System(op, oK, configs.C1_LEVELS, configs.C1_ITERATIONS, sc.depth_scale, np.float32)
+        done, busy = 0, 0.0
+        while done < limit_frames and busy < limit_s:
+            d, c = sc.frame(done)  # generation is outside the timing
+            t0 = time.perf_counter()
+            s.process(d, c)
+            busy += time.perf_counter() - t0
+            done += 1
+        return done, busy
+
+    threads = O.max_threads()
+    n, dt = run(n_frames, seconds)
     O.set_threads(1)
-    s1 = O.System(op, oK, configs.C1_LEVELS, configs.C1_ITERATIONS, sc.depth_scale, np.float32)
-    t1 = time.perf_counter()
-    for i in range(n1):
-        s1.process(frames_d[i], frames_c[i])
-    dt1 = time.perf_counter() - t1
+    n1, dt1 = run(n_frames, seconds_one)
     O.set_threads(threads)
     model = ""
     try:
@@ -68,132 +90,220 @@ def cpu_baseline(sc, params, K, frames_d, frames_c, n_frames):
             model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
     except OSError:
         pass
-    return {"value": n_frames / dt, "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": "C1 frames 0-%d (640x480, 5 mm), FP32 oracle System, %d OpenMP threads" % (n_frames - 1, threads),
-            "one_core": {"value": round(n1 / dt1, 3), "sample": "C1 frames 0-%d, 1 thread" % (n1 - 1)},
+    return {"value": round(n / dt, 3), "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": "stream %d frames 0-%d (640x480, 5 mm, C1 tracking schedule), FP32 oracle System, "
+                      "%d OpenMP threads, %.1f s of work" % (SEED0, n - 1, threads, dt),
+            "one_core": {"value": round(n1 / dt1, 3), "sample": "stream %d frames 0-%d, 1 thread, %.1f s"
+                                                                 % (SEED0, n1 - 1, dt1)},
             "nproc": os.cpu_count(), "cpu_model": model}
 
 
-def batch_throughput(n, warmup, steps, single_fps, precision=0, separate=True, repeats=3):
-    """Aggregate frames/s of n independent C1 systems stepped together: system i
-    is its own seeded scene and orbit (seed 100 + i) with its own map.  Two
-    ways: lf_batch (pyramid passes and Gauss-Newton steps of all systems in one
-    launch each, each map fused on its own stream, one batched z-range +
-    raycast launch per stage for all maps), and (separate) n lf_system objects
-    on their own streams with no shared launches."""
+def _intrinsics(sc):
+    from libfusion_amd import capi
+    return capi.intrinsics(sc.width, sc.height, sc.fx, sc.fy, sc.cx, sc.cy)
+
+
+def c1_params(precision):
+    from libfusion_amd import configs
+    p = configs.c1()[2]
+    p.precision = precision
+    return p
+
+
+def median(xs):
+    return sorted(xs)[len(xs) // 2]
+
+
+def vis_sum(c):
+    from libfusion_amd import capi
+    return int(c[capi.CNT_VIS_SUM_LO]) | (int(c[capi.CNT_VIS_SUM_HI]) << 32)
+
+
+class Streams:
+    """The frames of n streams, device-resident (and pinned host copies on demand)."""
+
+    def __init__(self, n, frames, config=1):
+        import torch
+
+        from libfusion_amd.synth import Scene
+        t = time.perf_counter()
+        self.scenes = [Scene(config, SEED0 + i) for i in range(n)]
+        self.host = [make_frames(sc, frames) for sc in self.scenes]
+        self.dev = [(torch.from_numpy(d).cuda(), torch.from_numpy(c).cuda()) for d, c in self.host]
+        torch.cuda.synchronize()
+        self.n, self.frames = n, frames
+        self.pinned = None
+        log("generated %d x %d frames in %.1fs" % (n, frames, time.perf_counter() - t))
+
+    def pin(self):
+        import torch
+        if self.pinned is None:
+            self.pinned = [(torch.from_numpy(d).pin_memory(), torch.from_numpy(c).pin_memory())
+                           for d, c in self.host]
+        return self.pinned
+
+
+def batch_runs(streams, cfgs, warmup, steps, runs, timing_first=0, host=False):
+    """Aggregate frames/s of the streams stepped together by lf_batch: `runs`
+    fresh batches over the same frames, each W untimed steps then exactly K
+    timed steps bracketed by synchronisation.  The first run records the
+    stage events of timing_first (HIP events on the batch stream)."""
     import torch
 
-    from libfusion_amd import capi, configs
-    from libfusion_amd.synth import Scene
-
-    frames, cfgs = [], []
-    for i in range(n):
-        sc = Scene(1, 100 + i)
-        d, c = make_frames(sc, warmup + steps)
-        frames.append((torch.from_numpy(d).cuda(), torch.from_numpy(c).cuda()))
-        K = capi.intrinsics(sc.width, sc.height, sc.fx, sc.fy, sc.cx, sc.cy)
-        _, _, params = configs.c1()
-        params.precision = precision
-        cfgs.append((K, params, sc.depth_scale))
-    torch.cuda.synchronize()
-
-    def timed(step, sync):
+    from libfusion_amd import capi
+    n = streams.n
+    src = streams.pin() if host else streams.dev
+    out = []
+    for r in range(runs):
+        b = capi.Batch(cfgs)
+        step = (lambda f: b.process_host([d[f] for d, _ in src], [c[f] for _, c in src])) if host else \
+            (lambda f: b.process([d[f] for d, _ in src], [c[f] for _, c in src]))
         for f in range(warmup):
             step(f)
-        sync()
+        b.synchronize()
+        c0 = [b.counters(i) for i in range(n)]
+        if r == 0 and timing_first:
+            b.set_timing(timing_first)
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         for f in range(warmup, warmup + steps):
             step(f)
-        sync()
-        return time.perf_counter() - t0
-
-    # the median of `repeats` runs, each a fresh batch over the same frames
-    runs = []
-    for _ in range(repeats):
-        batch = capi.Batch([capi.system_config(K, p, configs.C1_LEVELS, configs.C1_ITERATIONS, ds)
-                            for K, p, ds in cfgs])
-        runs.append(timed(lambda f: batch.process([d[f] for d, _ in frames], [c[f] for _, c in frames]),
-                          batch.synchronize))
-        batch.close()
-        del batch
+        b.synchronize()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        c1 = [b.counters(i) for i in range(n)]
+        rec = {"s": dt, "fps": n * steps / dt,
+               "vis_blocks_per_step": sum(vis_sum(y) - vis_sum(x) for x, y in zip(c0, c1)) / steps,
+               "overflow": int(max(int(c[capi.CNT_OVERFLOW]) for c in c1)),
+               "tracked": [b.tracking(i)[0] for i in range(n)]}
+        if r == 0 and timing_first:
+            rec["timing"] = {s: b.read_timing(s) for s in range(5) if timing_first & (1 << s)}
+        out.append(rec)
+        b.close()
+        del b
         torch.cuda.empty_cache()
-    dt_b = sorted(runs)[len(runs) // 2]
-    v_b = n * steps / dt_b
-    name = "plain FP32" if precision == 0 else "mixed-precision"
-    out = {"systems": n, "value": round(v_b, 1), "unit": "frames/s (all systems)",
-           "ms_per_step": round(1000.0 * dt_b / steps, 4), "vs_single_system": round(v_b / single_fps, 3),
-           "steps": steps, "warmup": warmup, "runs": [round(n * steps / t, 1) for t in runs],
-           "statistic": "median of %d runs" % repeats,
-           "workload": "%d independent %s C1 systems (seeds 100-%d: own scene, orbit and map), one frame "
-                       "of each per step" % (n, name, 99 + n),
-           "launch": "lf_batch: one descriptor table per step; pyramid passes, Gauss-Newton steps, block allocation, "
-                     "visible lists, integration, z-range and raycast of all systems in one launch each"}
-    if not separate:
-        return out
-    systems = [capi.System(K, p, configs.C1_LEVELS, configs.C1_ITERATIONS, ds) for K, p, ds in cfgs]
-
-    def step_systems(f):
-        for s, (d, c) in zip(systems, frames):
-            s.process(d[f], c[f])
-
-    def sync_systems():
-        for s in systems:
-            s.synchronize()
-
-    dt_s = timed(step_systems, sync_systems)
-    for s in systems:
-        s.close()
-    v_s = n * steps / dt_s
-    out["separate_systems"] = {"value": round(v_s, 1), "ms_per_step": round(1000.0 * dt_s / steps, 4),
-                               "launch": "separate lf_system objects on their own streams, no shared launches"}
     return out
 
 
-def timed_system(K, params, dev_d, dev_c, warmup, n_total, depth_scale):
-    """frames/s of one System over frames warmup..n_total-1 (after warmup frames)."""
+def single_runs(streams, params, warmup, steps, runs, timing=0, host=False):
+    """The same streams and frames, each stream alone through one System (the
+    plain path: one stream at a time): frames / summed wall time of the timed
+    frames.  timing: System stage mask recorded in the first run."""
     import torch
 
     from libfusion_amd import capi, configs
+    src = streams.pin() if host else streams.dev
+    out = []
+    for r in range(runs):
+        total, ev, vsum = 0.0, {}, 0
+        for i, sc in enumerate(streams.scenes):
+            d, c = src[i]
+            s = capi.System(_intrinsics(sc), params, configs.C1_LEVELS, configs.C1_ITERATIONS, sc.depth_scale)
+            proc = s.process_host if host else s.process
+            for f in range(warmup):
+                proc(d[f], c[f])
+            s.synchronize()
+            c0 = s.counters()
+            if r == 0 and timing:
+                s.set_timing(timing)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for f in range(warmup, warmup + steps):
+                proc(d[f], c[f])
+            s.synchronize()
+            total += time.perf_counter() - t0
+            vsum += vis_sum(s.counters()) - vis_sum(c0)
+            if r == 0 and timing:
+                for sid in range(5):
+                    if timing & (1 << sid):
+                        k, ms = s.read_timing(sid)
+                        a = ev.setdefault(sid, [0, 0.0])
+                        a[0] += k
+                        a[1] += ms
+            s.close()
+        rec = {"s": total, "fps": streams.n * steps / total, "vis_blocks_per_frame": vsum / (streams.n * steps)}
+        if ev:
+            rec["timing"] = {k: tuple(v) for k, v in ev.items()}
+        out.append(rec)
+        torch.cuda.empty_cache()
+    return out
 
-    s = capi.System(K, params, configs.C1_LEVELS, configs.C1_ITERATIONS, depth_scale)
-    for i in range(warmup):
-        s.process(dev_d[i], dev_c[i])
-    s.synchronize()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(warmup, n_total):
-        s.process(dev_d[i], dev_c[i])
-    s.synchronize()
-    dt = time.perf_counter() - t0
-    pose = s.pose()
-    s.close()
-    return (n_total - warmup) / dt, dt, pose
+
+def integrate_bytes(vis_blocks, pixels, voxel_bytes):
+    """Algorithmic bytes of TSDF integration: every visible block's 512 voxels
+    read and written, its list entry (coordinate + pool index, 16 B), and one
+    depth (4 B) + normal x / z (8 B) + colour (3 B) gather per pixel."""
+    return vis_blocks * (2 * 512 * voxel_bytes + 16) + pixels * 15
+
+
+def roofline(bytes_per_launch, launches, total_ms, kernel, extra=None):
+    avg_s = total_ms * 1e-3 / max(launches, 1)
+    achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
+           "algorithmic_bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(total_ms / max(launches, 1), 5),
+           "launches": launches}
+    if extra:
+        out.update(extra)
+    return out
+
+
+def profile_value(name, key):
+    path = os.path.join(ROOT, "profiles", name)
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get(key)
+    return None
+
+
+def c3_leg(warmup, steps, runs):
+    """The uneven batched configuration: 16 mixed-precision streams at 160x120,
+    320x240, 640x480 and 848x480 (seeds 100-115, resolution by seed), voxels of
+    4 / 5 / 8 / 10 mm (truncation 3 voxels), 2^18-2^20 buckets, one lf_batch."""
+    from libfusion_amd import capi
+    streams = Streams(16, warmup + steps, config=3)
+    cfgs, shapes = [], []
+    for i, sc in enumerate(streams.scenes):
+        vs = (0.004, 0.005, 0.008, 0.010)[i // 4]
+        p = capi.map_params(2 ** (18 + i % 3), 1 << 18, vs, 3 * vs, precision=capi.PRECISION_MIXED)
+        cfgs.append(capi.system_config(_intrinsics(sc), p, 3, (10, 5, 3), sc.depth_scale))
+        shapes.append("%dx%d/%gmm" % (sc.width, sc.height, vs * 1e3))
+    rs = batch_runs(streams, cfgs, warmup, steps, runs)
+    fps = median([r["fps"] for r in rs])
+    pixels = sum(sc.width * sc.height for sc in streams.scenes)
+    return {"value": round(fps, 1), "unit": "frames/s (all streams)", "runs": [round(r["fps"], 1) for r in rs],
+            "ms_per_step": round(1000.0 * 16 / fps, 4), "steps": steps, "warmup": warmup,
+            "megapixels_per_s": round(fps * pixels / 16 / 1e6, 1), "streams": shapes,
+            "workload": "16 mixed-precision streams of uneven resolution and voxel size in one lf_batch"}
 
 
 def main():
-    ap = argparse.ArgumentParser()
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=290)
+    ap.add_argument("--steps", type=int, default=50, help="timed steps (one frame of every stream each)")
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--cpu-frames", type=int, default=60)
+    ap.add_argument("--streams", type=int, default=16)
+    ap.add_argument("--runs", type=int, default=3, help="runs of the timed steps (median reported)")
+    ap.add_argument("--cpu-frames", type=int, default=60, help="at most this many frames in the CPU sample")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="at most this much CPU work, all cores")
+    ap.add_argument("--cpu-seconds-one", type=float, default=6.0, help="at most this much CPU work, one core")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-stages", action="store_true", help="skip the instrumented stage-breakdown pass")
-    ap.add_argument("--batch", type=int, default=16,
-                    help="also time this many independent systems stepped together (0: skip)")
-    ap.add_argument("--batch-steps", type=int, default=50)
-    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision passes")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the headline and its plain FP32 comparison (no breakdown, PCIe or C3 legs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device-to-device (a plain copy kernel of known size, "
                          "to scale rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: the path runs one process per GPU with no "
-              "cross-GPU exchange in this build; run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: this build's path runs on one GPU (independent streams "
+              "would shard one process per GPU with no exchange); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.steps < 1 or args.warmup < 0 or args.streams < 1:
+        raise SystemExit("need --steps >= 1, --warmup >= 0, --streams >= 1")
 
     import torch
 
-    from libfusion_amd import capi, configs
+    from libfusion_amd import capi
 
     capi.load()
     if args.copy_calibration:
@@ -202,168 +312,124 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    sc, K, params = configs.c1()
-    n_total = args.warmup + args.steps
-    if n_total > sc.num_frames:
-        raise SystemExit("warmup + steps must be <= %d frames" % sc.num_frames)
-    t = time.perf_counter()
-    fd, fc = make_frames(sc, n_total)
-    log("generated %d frames in %.1fs" % (n_total, time.perf_counter() - t))
-    dev_d = torch.from_numpy(fd).cuda()
-    dev_c = torch.from_numpy(fc).cuda()
-    torch.cuda.synchronize()
+    W, K, n = args.warmup, args.steps, args.streams
+    if W + K > 300:
+        raise SystemExit("warmup + steps must be <= 300 frames (the orbit's length)")
+    streams = Streams(n, W + K)
+    P = streams.scenes[0].width * streams.scenes[0].height
+    mixed, plain = capi.PRECISION_MIXED, capi.PRECISION_FP32
 
-    sysm = capi.System(K, params, configs.C1_LEVELS, configs.C1_ITERATIONS, sc.depth_scale)
-    for i in range(args.warmup):
-        sysm.process(dev_d[i], dev_c[i])
-    sysm.synchronize()
-    c0 = sysm.counters()
-    # only the integrate kernel is bracketed in the timed region (its average
-    # launch duration is the roofline's denominator); each event record costs
-    # stream time, so the stage breakdown comes from a separate pass below
-    sysm.set_timing(capi.TIMING_INTEGRATE)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.warmup, n_total):
-        sysm.process(dev_d[i], dev_c[i])
-    sysm.synchronize()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    c1 = sysm.counters()
-    pose = sysm.pose()
-    fps_evented = args.steps / elapsed
-    log("timed %d frames in %.3fs: %.1f frames/s" % (args.steps, elapsed, fps_evented))
+    def cfgs(precision):
+        from libfusion_amd import configs
+        return [capi.system_config(_intrinsics(sc), c1_params(precision), configs.C1_LEVELS, configs.C1_ITERATIONS,
+                                   sc.depth_scale) for sc in streams.scenes]
 
-    n_ig, ms_ig = sysm.read_timing(0)
-    integrate_avg_ms = ms_ig / max(n_ig, 1)
-    # stage breakdown: a second system over the same frames with every stage bracketed
-    stage_names = {0: "integrate_kernel", 1: "raycast_kernel", 2: "tracking", 4: "frame"}
-    stages = {}
-    if not args.no_stages:
-        del sysm
-        sysm = capi.System(K, params, configs.C1_LEVELS, configs.C1_ITERATIONS, sc.depth_scale)
-        for i in range(args.warmup):
-            sysm.process(dev_d[i], dev_c[i])
-        sysm.set_timing(capi.TIMING_ALL)
-        for i in range(args.warmup, n_total):
-            sysm.process(dev_d[i], dev_c[i])
-        sysm.synchronize()
-        for sid, name in stage_names.items():
-            n, ms = sysm.read_timing(sid)
-            stages[name] = {"launches": n, "avg_ms": ms / max(n, 1)}
-    # two more runs of the same frames without the integrate events; the
-    # reported frame rate is the median of the three
-    run_fps = [fps_evented]
-    for _ in range(2):
-        del sysm
-        torch.cuda.empty_cache()
-        f_r, _, _ = timed_system(K, params, dev_d, dev_c, args.warmup, n_total, sc.depth_scale)
-        run_fps.append(f_r)
-        sysm = None
-    fps = sorted(run_fps)[1]
-    elapsed = args.steps / fps
-    log("runs %s frames/s, median %.1f" % (", ".join("%.1f" % f for f in run_fps), fps))
-    vis_sum = (int(c1[capi.CNT_VIS_SUM_LO]) | (int(c1[capi.CNT_VIS_SUM_HI]) << 32)) - \
-              (int(c0[capi.CNT_VIS_SUM_LO]) | (int(c0[capi.CNT_VIS_SUM_HI]) << 32))
-    updates = int(c1[capi.CNT_UPDATES]) - int(c0[capi.CNT_UPDATES])
-    n_vis = vis_sum / max(updates, 1)
-    P = K.width * K.height
-    # TSDF integration kernel, algorithmic bytes per launch: every visible block's
-    # 512 voxels read and written (sdf int16 + weight f32 + rgb 3 x u8 = 9 B),
-    # its list entry (coordinate and pool index, 16 B), plus one depth (4 B),
-    # normal x/z (8 B) and colour (3 B) gather per pixel.
-    bytes_per_launch = n_vis * (2 * 512 * 9 + 16) + P * 15
-    avg_s = integrate_avg_ms * 1e-3
-    achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "integrate_c1_traffic.json")
-    if os.path.exists(tf):  # HBM bytes per launch measured with rocprofv3 --pmc (see that file)
-        traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_integrate<true,true,true,false> (FP32, packed samples)", "algorithmic_bytes_per_launch": int(bytes_per_launch),
-                "avg_launch_ms": round(integrate_avg_ms, 5), "visible_blocks_avg": round(n_vis, 1)}
+    # headline: mixed precision, batched
+    kmask = (1 << capi.BATCH_INTEGRATE) | (1 << capi.BATCH_RAYCAST)
+    head = batch_runs(streams, cfgs(mixed), W, K, args.runs, timing_first=kmask)
+    fps = median([r["fps"] for r in head])
+    log("mixed batched: runs %s frames/s, median %.1f" % (", ".join("%.1f" % r["fps"] for r in head), fps))
+    # the comparison: plain FP32, one System per stream, same frames
+    base = single_runs(streams, c1_params(plain), W, K, args.runs, timing=capi.TIMING_INTEGRATE)
+    fps_plain = median([r["fps"] for r in base])
+    ratio = fps / fps_plain
+    log("plain FP32 single systems: runs %s frames/s, median %.1f; ratio %.3f"
+        % (", ".join("%.1f" % r["fps"] for r in base), fps_plain, ratio))
 
-    # raycast: compulsory bytes (vertex map out, sdf + weight of the distinct
-    # blocks the march reads, their hash entries, z-range tiles), with the
-    # distinct-block count measured by the oracle's instrumented march on C1
+    # roofline of the batched integration kernel (HIP events in the first timed run)
+    t = head[0]["timing"]
+    ig_n, ig_ms = t[capi.BATCH_INTEGRATE]
+    ig_bytes = integrate_bytes(head[0]["vis_blocks_per_step"], n * P, 6)
+    rl = roofline(ig_bytes, ig_n, ig_ms, "k_integrate_batch<true> (mixed: FP16 sdf + 8-bit weight + 3 x u8 colour, "
+                  "%d maps per launch)" % n,
+                  {"visible_blocks_per_launch": round(head[0]["vis_blocks_per_step"], 1),
+                   "basis": "per visible block 2*512*6 + 16 B, per pixel 15 B (SURVEY 8d, mixed voxel = 6 B)"})
+    rl["traffic"] = profile_value("integrate_batch_mixed_traffic.json", "hbm_bytes_per_launch")
     stage_rooflines = {}
-    rf = os.path.join(ROOT, "profiles", "raycast_c1_touched.json")
-    if "raycast_kernel" in stages and os.path.exists(rf):
-        rb = json.load(open(rf))["algorithmic_bytes_per_raycast"]
-        rs = stages["raycast_kernel"]["avg_ms"] * 1e-3
-        ra = rb / rs / 1e9
-        stage_rooflines["raycast"] = {"bound": "latency (dependent sample chain)", "achieved": round(ra, 1),
-                                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ra / HBM_PEAK_GBS, 4),
-                                      "algorithmic_bytes_per_launch": rb,
-                                      "avg_launch_ms": round(stages["raycast_kernel"]["avg_ms"], 5)}
+    rc_n, rc_ms = t[capi.BATCH_RAYCAST]
+    touched = profile_value("raycast_batch_touched.json", "blocks_touched_mean_per_map")
+    if touched:
+        # compulsory: vertex map out (16 B / pixel), sdf + weight (3 B mixed) of
+        # the distinct blocks the march reads, their 20 B entries, z-range tiles
+        rb = n * (P * 16 + touched * 512 * 3 + touched * 20 + 2 * (P // 64) * 4)
+        stage_rooflines["raycast"] = roofline(rb, rc_n, rc_ms, "k_raycast_batch<true, true> (%d maps per launch)" % n,
+                                              {"bound": "latency (dependent sample chain)",
+                                               "blocks_touched_per_map": touched})
+    else:
+        stage_rooflines["raycast"] = {"avg_launch_ms": round(rc_ms / max(rc_n, 1), 5), "launches": rc_n}
+    pt = base[0]["timing"]
+    plain_rl = roofline(integrate_bytes(base[0]["vis_blocks_per_frame"], P, 9), pt[0][0], pt[0][1],
+                        "k_integrate<true, true, true, false> (FP32, one map)")
 
-    batched = None
-    del sysm
-    torch.cuda.empty_cache()
-    if args.batch > 0:
-        batched = batch_throughput(args.batch, args.warmup, args.batch_steps, fps)
-        log("batched: %d systems, %.1f frames/s aggregate" % (args.batch, batched["value"]))
-    # mixed precision (FP16 TSDF + 8-bit weight codes, FP16 maps): the same
-    # frames through one system, then batched; frames/s against this run's
-    # plain FP32 single system
-    mixed = None
-    if not args.no_mixed:
-        pm = configs.c1()[2]
-        pm.precision = capi.PRECISION_MIXED
-        m_fps, m_dt, m_pose = timed_system(K, pm, dev_d, dev_c, args.warmup, n_total, sc.depth_scale)
-        log("mixed precision: %.1f frames/s" % m_fps)
-        mixed = {"value": round(m_fps, 2), "unit": "frames/s", "ms_per_step": round(1000.0 * m_dt / args.steps, 4),
-                 "vs_plain_fp32": round(m_fps / fps, 3),
-                 "final_pose_t": [round(float(x), 7) for x in m_pose[:3, 3]],
-                 "storage": "FP16 TSDF + 8-bit weight codes + 3 x u8 colour (6 B/voxel); FP16 normal / intensity / "
-                            "gradient maps (vertex maps FP32); FP32 pairwise normal-equation sums; FP64 solve and SE(3)"}
-        if args.batch > 0:
-            torch.cuda.empty_cache()
-            mb = batch_throughput(args.batch, args.warmup, args.batch_steps, fps, capi.PRECISION_MIXED,
-                                  separate=False)
-            mb["vs_plain_fp32_single_system"] = round(mb["value"] / fps, 3)
-            mixed["batched"] = mb
-            log("mixed batched: %d systems, %.1f frames/s aggregate (%.2fx plain FP32)"
-                % (args.batch, mb["value"], mb["value"] / fps))
+    extras = {}
+    if not args.no_extras:
+        # stage breakdown of the batched step (every stage bracketed: slower)
+        st = batch_runs(streams, cfgs(mixed), W, K, 1, timing_first=capi.BATCH_TIMING_ALL)[0]["timing"]
+        names = {capi.BATCH_INTEGRATE: "integrate_kernel", capi.BATCH_RAYCAST: "raycast_kernel",
+                 capi.BATCH_TRACKING: "tracking", capi.BATCH_FUSION: "fusion", capi.BATCH_STEP: "batch_stream_step"}
+        extras["stages_ms"] = {names[s]: round(ms / max(k, 1), 4) for s, (k, ms) in st.items()}
+        pb = batch_runs(streams, cfgs(plain), W, K, 1)[0]
+        ms = single_runs(streams, c1_params(mixed), W, K, 1)[0]
+        extras["breakdown"] = {
+            "plain_fp32_batched": {"value": round(pb["fps"], 1), "vs_plain_fp32": round(pb["fps"] / fps_plain, 3)},
+            "mixed_single_systems": {"value": round(ms["fps"], 1), "vs_plain_fp32": round(ms["fps"] / fps_plain, 3)}}
+        log("breakdown: plain batched %.1f, mixed single %.1f" % (pb["fps"], ms["fps"]))
+        # the same headline step with every frame copied from pinned host memory
+        ph = batch_runs(streams, cfgs(mixed), W, K, 1, host=True)[0]
+        extras["pcie_inclusive"] = {
+            "value": round(ph["fps"], 1), "unit": "frames/s (all streams)",
+            "h2d_bytes_per_step": int(n * P * 5),
+            "note": "lf_batch_process_host from pinned host buffers: each step's 16 depth + RGB frames are copied "
+                    "on the upload stream, overlapping the previous step's fusion and raycast"}
+        log("pcie-inclusive mixed batched: %.1f" % ph["fps"])
+        del streams
+        torch.cuda.empty_cache()
+        extras["c3_uneven_streams"] = c3_leg(W, K, args.runs)
+        log("C3 uneven batched: %.1f frames/s" % extras["c3_uneven_streams"]["value"])
 
     cpu = None
     if not args.no_cpu:
-        t = time.perf_counter()
-        cpu = cpu_baseline(sc, params, K, fd, fc, args.cpu_frames)
-        log("cpu baseline %.2f frames/s in %.1fs" % (cpu["value"], time.perf_counter() - t))
+        t0 = time.perf_counter()
+        cpu = cpu_baseline(c1_params(plain), args.cpu_frames, args.cpu_seconds, args.cpu_seconds_one)
+        log("cpu baseline %.2f frames/s (%d threads), %.2f on one core, in %.1fs"
+            % (cpu["value"], cpu["cores"], cpu["one_core"]["value"], time.perf_counter() - t0))
+
     out = {
-        "metric": "frames/s at 640x480, 5 mm voxels: mixed-prec+batched vs plain FP32",
-        "value": round(fps, 2),
+        "metric": METRIC,
+        "value": round(fps, 1),
         "unit": "frames/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(1000.0 * elapsed / args.steps, 4),
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(1000.0 * n / fps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
-        "data": "synthetic (seeded analytic room, Perlin albedo, depth noise + 2% dropouts)",
-        "config": {"workload": "C1 plain FP32 GPU path: 300-frame 640x480 orbit, room + 8 primitives, 5 mm voxels, "
-                               "15 mm truncation, 2^20 buckets, 3-level ICP+RGB {10,5,3}; frames %d-%d timed"
-                               % (args.warmup, n_total - 1),
-                   "width": K.width, "height": K.height, "voxel_mm": 5, "buckets": params.num_buckets,
-                   "pool_blocks": params.num_blocks, "precision": "plain FP32"},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
+        "vs_baseline_note": "no published number exists for this metric; its comparison, mixed+batched over the "
+                            "plain FP32 GPU path on the same streams and frames, is vs_plain_fp32",
+        "vs_plain_fp32": round(ratio, 3),
+        "dtype": "mixed: fp16 storage (TSDF, normal / intensity / gradient maps), u8 weights, fp32 arithmetic, "
+                 "fp64 solve",
+        "data": "synthetic (seeded analytic rooms with 8 primitives, Perlin albedo, depth noise + 2% dropouts)",
+        "config": {"workload": "%d independent 640x480 RGB-D streams (seeds %d-%d, own orbit and map each), 5 mm "
+                               "voxels, 15 mm truncation, 2^20 buckets, 3-level ICP+RGB {10,5,3}; one step = one "
+                               "frame of every stream; frames %d-%d timed" % (n, SEED0, SEED0 + n - 1, W, W + K - 1),
+                   "streams": n, "width": 640, "height": 480, "voxel_mm": 5, "buckets": 2 ** 20,
+                   "pool_blocks": c1_params(mixed).num_blocks, "precision": "mixed", "launch": "lf_batch"},
+        "roofline": rl,
         "stage_rooflines": stage_rooflines,
-        "stages_ms": {k: round(v["avg_ms"], 4) for k, v in stages.items()},
-        "runs": [round(f, 2) for f in run_fps],
-        "statistic": "median of 3 runs over the same frames (the first with HIP events around the integration kernel)",
-        "batched": batched,
-        "mixed_precision": mixed,
-        "mixed_batched_vs_plain_fp32": (mixed["batched"]["vs_plain_fp32_single_system"]
-                                        if mixed and mixed.get("batched") else None),
-        "allocated_blocks": int(min(c1[capi.CNT_ALLOCATED], params.num_blocks)),
-        "overflow": int(c1[capi.CNT_OVERFLOW]),
-        "final_pose_t": [round(float(x), 7) for x in pose[:3, 3]],
+        "cpu_baseline": cpu,
+        "runs": [round(r["fps"], 1) for r in head],
+        "statistic": "median of %d runs over the same frames (the first with HIP events around the integration "
+                     "and ray-marching kernels)" % args.runs,
+        "plain_fp32": {"value": round(fps_plain, 1), "unit": "frames/s", "runs": [round(r["fps"], 1) for r in base],
+                       "ms_per_frame": round(1000.0 / fps_plain, 4), "roofline": plain_rl,
+                       "workload": "the same %d streams and frames, each through one plain FP32 System (int16 TSDF, "
+                                   "float32 weights and maps, correctly rounded arithmetic)" % n},
+        "tracked_streams": int(sum(head[0]["tracked"])),
+        "overflow": head[0]["overflow"],
     }
+    out.update(extras)
     print(json.dumps(out), flush=True)
 
 

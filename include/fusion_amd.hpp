@@ -59,6 +59,7 @@ public:
     GpuImage &operator=(GpuImage &&o) noexcept;
     void create(int cols, int rows, int elem_bytes);
     bool empty() const { return data_ == nullptr; }
+    size_t bytes() const { return (size_t)cols * rows * elem; }
     template <class T> T *ptr() const { return static_cast<T *>(data_); }
     int cols = 0, rows = 0, elem = 0;
 
@@ -261,6 +262,10 @@ public:
     System(const lf_system_config &cfg);
     ~System();
     int process_images(const uint16_t *depth, const unsigned char *rgb);
+    // process_images (system.cpp:40-114) from host memory, as the reference
+    // takes its cv::Mat frames: the frame is copied into device staging on the
+    // system's stream first (asynchronous when the host buffers are pinned).
+    int process_images_host(const uint16_t *depth, const unsigned char *rgb);
     int restart();
     // writeMapToDisk / readMapFromDisk (system.cpp:202-210)
     int writeMapToDisk(const std::string &file_name);
@@ -316,6 +321,7 @@ private:
     bool graphs_ = false;
     hipGraphExec_t exec_[2] = {nullptr, nullptr};
     GpuImage depth_in_, rgb_in_;
+    GpuImage host_depth_, host_rgb_;  // device staging of process_images_host
     hipEvent_t take_event();
 };
 
@@ -334,12 +340,22 @@ public:
     Batch(const lf_system_config *cfgs, int n);
     ~Batch();
     int process(const uint16_t *const *depth, const unsigned char *const *rgb);
+    // process() from host memory (the reference's process_images takes host
+    // images): each frame is copied into the batch's device staging buffers on
+    // the upload stream, which overlaps the previous step's fusion and
+    // raycast; pinned host buffers make the copies asynchronous.
+    int process_host(const uint16_t *const *depth, const unsigned char *const *rgb);
     int synchronize();
     int size() const { return (int)sys_.size(); }
     System &system(int i) { return *sys_[i]; }
     hipStream_t stream() const { return stream_; }
+    // Event timing on the batch stream (LF_BATCH_TIMING_* bits; 0: off) and
+    // {launches, total ms} of a stage (synchronising).
+    void set_timing(int stages);
+    int read_timing(int stage, int *launches, double *total_ms);
 
 private:
+    int step(const uint16_t *const *depth, const unsigned char *const *rgb, bool from_host);
     void release();
     std::vector<std::unique_ptr<System>> sys_;
     hipStream_t stream_ = nullptr;
@@ -357,6 +373,14 @@ private:
     bool table_used_[2] = {false, false};
     size_t table_cap_ = 0;  // bytes per buffer
     int reserve_table(size_t bytes);
+    // device staging of host frames (process_host), per system
+    std::vector<GpuImage> depth_in_, rgb_in_;
+    // stage timing: an event pair per timed stage and step
+    int timing_ = 0;
+    struct Ev { int stage; hipEvent_t a, b; };
+    std::vector<Ev> events_;
+    std::vector<hipEvent_t> pool_;
+    hipEvent_t take_event();
 };
 
 }  // namespace fusion

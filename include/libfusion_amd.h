@@ -449,6 +449,13 @@ int lf_system_destroy(lf_system *sys);
  * depth uint16 [rows*cols] and rgb uint8x3.  Enqueues the whole frame on the
  * system's stream without host synchronisation. */
 int lf_system_process_images(lf_system *sys, const uint16_t *depth, const unsigned char *rgb);
+/* System::process_images(const cv::Mat depth, const cv::Mat image)
+ * (system.cpp:40-114) on host-resident input, as the reference takes it:
+ * depth uint16 and rgb uint8x3 in host memory are copied to device staging on
+ * the system's stream, then the frame runs as lf_system_process_images.  With
+ * pinned host buffers (hipHostMalloc) the copies are asynchronous and the host
+ * buffers must stay unchanged until the frame has been synchronised. */
+int lf_system_process_images_host(lf_system *sys, const uint16_t *depth, const unsigned char *rgb);
 /* Wait for all enqueued frames. */
 int lf_system_synchronize(lf_system *sys);
 /* Camera pose of the last processed frame, row-major 4x4 doubles (synchronises). */
@@ -512,7 +519,28 @@ typedef struct lf_batch lf_batch;
 int lf_batch_create(const lf_system_config *cfgs, int n, lf_batch **out);
 int lf_batch_destroy(lf_batch *b);
 int lf_batch_process(lf_batch *b, const uint16_t *const *depth, const unsigned char *const *rgb);
+/* lf_batch_process from host memory (System::process_images, system.cpp:40-114,
+ * takes host images): each system's frame is copied into the batch's device
+ * staging on the upload stream, which overlaps the previous step's fusion and
+ * raycast.  With pinned host buffers (hipHostMalloc) the copies are
+ * asynchronous; the host buffers must stay unchanged until the step has been
+ * synchronised. */
+int lf_batch_process_host(lf_batch *b, const uint16_t *const *depth, const unsigned char *const *rgb);
 int lf_batch_synchronize(lf_batch *b);
+/* Event timing of a batch's steps on its batch stream: stages is a bit mask,
+ * bit s brackets stage s (0 the batched integration kernel, 1 the batched
+ * ray-marching kernel, 2 the Gauss-Newton launches of a step, 3 block
+ * allocation + visible lists + integration, 4 the batch stream's whole step;
+ * the uploads overlap on another stream and are not in stage 4); 0 turns
+ * timing off.  read_timing returns {launches, total_ms} of a stage
+ * (synchronising). */
+#define LF_BATCH_TIMING_INTEGRATE 0x1
+#define LF_BATCH_TIMING_RAYCAST 0x2
+#define LF_BATCH_TIMING_TRACKING 0x4
+#define LF_BATCH_TIMING_FUSION 0x8
+#define LF_BATCH_TIMING_STEP 0x10
+int lf_batch_set_timing(lf_batch *b, int stages);
+int lf_batch_read_timing(lf_batch *b, int stage, int *launches, double *total_ms);
 /* System i's camera pose (4x4 row-major doubles; synchronising), last
  * tracking result and map counters. */
 int lf_batch_get_pose(lf_batch *b, int i, double *out16);

@@ -25,6 +25,9 @@ DIR_DIM = 256  # block directory: DIR_DIM^3 cells
 DIR_CELLS = DIR_DIM ** 3
 TIMING_INTEGRATE, TIMING_RAYCAST, TIMING_TRACKING, TIMING_FRAME = 0x1, 0x2, 0x4, 0x10
 TIMING_ALL = 0x17
+# batch stage ids (bit s of the mask brackets stage s)
+BATCH_INTEGRATE, BATCH_RAYCAST, BATCH_TRACKING, BATCH_FUSION, BATCH_STEP = range(5)
+BATCH_TIMING_ALL = 0x1F
 
 
 class LfIntrinsics(ctypes.Structure):
@@ -130,6 +133,7 @@ SIGNATURES = [
     ("lf_system_create", I, [P, P]),
     ("lf_system_destroy", I, [P]),
     ("lf_system_process_images", I, [P, P, P]),
+    ("lf_system_process_images_host", I, [P, P, P]),
     ("lf_system_synchronize", I, [P]),
     ("lf_system_get_pose", I, [P, P]),
     ("lf_system_get_tracking", I, [P, P, P]),
@@ -147,7 +151,10 @@ SIGNATURES = [
     ("lf_batch_create", I, [P, I, P]),
     ("lf_batch_destroy", I, [P]),
     ("lf_batch_process", I, [P, P, P]),
+    ("lf_batch_process_host", I, [P, P, P]),
     ("lf_batch_synchronize", I, [P]),
+    ("lf_batch_set_timing", I, [P, I]),
+    ("lf_batch_read_timing", I, [P, I, P, P]),
     ("lf_batch_get_pose", I, [P, I, P]),
     ("lf_batch_get_tracking", I, [P, I, P, P]),
     ("lf_batch_read_counters", I, [P, I, P]),
@@ -264,6 +271,15 @@ def stream_ptr():
 
 def ptr(t):
     return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def host_ptr(a):
+    """Address of a contiguous host buffer (a CPU tensor or a numpy array)."""
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"]
+        return a.ctypes.data
+    assert not a.is_cuda and a.is_contiguous()
+    return a.data_ptr()
 
 
 PLANE_DEPTH, PLANE_INTENSITY, PLANE_DX, PLANE_DY, PLANE_VMAP, PLANE_NMAP, PLANE_RAW_DEPTH, PLANE_RGB, PLANE_SAMPLES = range(9)
@@ -575,8 +591,27 @@ class Batch:
         c = (ctypes.c_void_p * self.n)(*[t.data_ptr() for t in rgbs])
         check(load().lf_batch_process(self.h, d, c))
 
+    def process_host(self, depths, rgbs):
+        """One frame per system from host memory: CPU tensors (pinned for
+        asynchronous copies) or contiguous numpy arrays.  They must stay
+        unchanged until the step has been synchronised."""
+        assert len(depths) == len(rgbs) == self.n
+        d = (ctypes.c_void_p * self.n)(*[host_ptr(t) for t in depths])
+        c = (ctypes.c_void_p * self.n)(*[host_ptr(t) for t in rgbs])
+        check(load().lf_batch_process_host(self.h, d, c))
+
     def synchronize(self):
         check(load().lf_batch_synchronize(self.h))
+
+    def set_timing(self, stages):
+        """Stage mask of BATCH_TIMING_* bits (0: off)."""
+        check(load().lf_batch_set_timing(self.h, int(stages)))
+
+    def read_timing(self, stage):
+        n = ctypes.c_int(0)
+        t = ctypes.c_double(0)
+        check(load().lf_batch_read_timing(self.h, int(stage), ctypes.byref(n), ctypes.byref(t)))
+        return n.value, t.value
 
     def pose(self, i):
         T = np.zeros((4, 4))
@@ -625,6 +660,10 @@ class System:
 
     def process(self, depth_dev, rgb_dev):
         check(load().lf_system_process_images(self.h, ptr(depth_dev), ptr(rgb_dev)))
+
+    def process_host(self, depth, rgb):
+        """process_images from host memory (CPU tensors or contiguous numpy arrays)."""
+        check(load().lf_system_process_images_host(self.h, host_ptr(depth), host_ptr(rgb)))
 
     def synchronize(self):
         check(load().lf_system_synchronize(self.h))

@@ -87,7 +87,9 @@ struct LfFuseBatchDims {
 };
 size_t lf_fuse_job_bytes(void);
 int lf_update_batch_make(const LfFuseItem *items, int n, void *jobs_host, LfFuseBatchDims *dims);
-int lf_update_batch_launch(const void *jobs_dev, const LfFuseBatchDims &d, hipStream_t st);
+// e0 / e1 (either may be null) are recorded immediately around the integration launch.
+int lf_update_batch_launch(const void *jobs_dev, const LfFuseBatchDims &d, hipStream_t st, hipEvent_t e0 = nullptr,
+                           hipEvent_t e1 = nullptr);
 
 // One raycast of a batch: a map, its target vertex map (and colour, or null),
 // the float T_wc / T_cw pose and the image size.
@@ -115,4 +117,6 @@ struct LfRayBatchDims {
     size_t zrange_lds;
 };
 int lf_raycast_batch_make(const LfRayItem *items, int n, void *jobs_host, LfRayBatchDims *dims);
-int lf_raycast_batch_launch(const void *jobs_dev, const LfRayBatchDims &d, int check_colour, hipStream_t st);
+// e0 / e1 (either may be null) are recorded immediately around the ray-marching launch.
+int lf_raycast_batch_launch(const void *jobs_dev, const LfRayBatchDims &d, int check_colour, hipStream_t st,
+                            hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);

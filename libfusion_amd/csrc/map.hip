@@ -799,7 +799,8 @@ int lf_update_batch_make(const LfFuseItem *items, int n, void *jobs_host, LfFuse
     return LF_OK;
 }
 
-int lf_update_batch_launch(const void *jobs_dev, const LfFuseBatchDims &d, hipStream_t st) {
+int lf_update_batch_launch(const void *jobs_dev, const LfFuseBatchDims &d, hipStream_t st, hipEvent_t e0,
+                           hipEvent_t e1) {
     if (!jobs_dev || d.n <= 0) return lf_set_error(LF_ERR_ARG, "bad update batch launch");
     if (!d.any) return LF_OK;
     const FuseJob *J = static_cast<const FuseJob *>(jobs_dev);
@@ -813,9 +814,11 @@ int lf_update_batch_launch(const void *jobs_dev, const LfFuseBatchDims &d, hipSt
         k_visible_batch<false><<<dim3(d.vis_grid_max, d.n), kVisThreads, 0, st>>>(J);
     }
     LF_LAUNCH_CHECK();
+    if (e0) LF_HIP_CHECK(hipEventRecord(e0, st));
     if (d.mixed) k_integrate_batch<true><<<dim3(d.integrate_grid, d.n), 256, 0, st>>>(J);
     else k_integrate_batch<false><<<dim3(d.integrate_grid, d.n), 256, 0, st>>>(J);
     LF_LAUNCH_CHECK();
+    if (e1) LF_HIP_CHECK(hipEventRecord(e1, st));
     return LF_OK;
 }
 

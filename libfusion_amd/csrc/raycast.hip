@@ -616,7 +616,8 @@ int lf_raycast_batch_make(const LfRayItem *items, int n, void *jobs_host, LfRayB
     return LF_OK;
 }
 
-int lf_raycast_batch_launch(const void *jobs_dev, const LfRayBatchDims &d, int check_colour, hipStream_t st) {
+int lf_raycast_batch_launch(const void *jobs_dev, const LfRayBatchDims &d, int check_colour, hipStream_t st,
+                            hipEvent_t e0, hipEvent_t e1) {
     if (!jobs_dev || d.n <= 0) return lf_set_error(LF_ERR_ARG, "bad raycast batch launch");
     const RayJob *jobs = static_cast<const RayJob *>(jobs_dev);
     const int n = d.n;
@@ -632,6 +633,7 @@ int lf_raycast_batch_launch(const void *jobs_dev, const LfRayBatchDims &d, int c
     }
     LF_LAUNCH_CHECK();
     const dim3 grid(d.grid_max, n);
+    if (e0) LF_HIP_CHECK(hipEventRecord(e0, st));
     if (d.mixed) {
         if (check_colour) k_raycast_batch<true, true><<<grid, 256, 0, st>>>(jobs);
         else k_raycast_batch<false, true><<<grid, 256, 0, st>>>(jobs);
@@ -640,6 +642,7 @@ int lf_raycast_batch_launch(const void *jobs_dev, const LfRayBatchDims &d, int c
         else k_raycast_batch<false, false><<<grid, 256, 0, st>>>(jobs);
     }
     LF_LAUNCH_CHECK();
+    if (e1) LF_HIP_CHECK(hipEventRecord(e1, st));
     return LF_OK;
 }
 

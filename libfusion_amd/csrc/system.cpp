@@ -680,6 +680,12 @@ void Batch::release() {
     stream_ = nullptr;
     if (aux_) (void)hipStreamDestroy(aux_);
     aux_ = nullptr;
+    for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+    pool_.clear();
+    for (auto &e : events_) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+    events_.clear();
+    depth_in_.clear();
+    rgb_in_.clear();
     sys_.clear();
 }
 
@@ -701,9 +707,71 @@ int Batch::reserve_table(size_t bytes) {
     return LF_OK;
 }
 
-int Batch::process(const uint16_t *const *depth, const unsigned char *const *rgb) {
+hipEvent_t Batch::take_event() {
+    hipEvent_t e = nullptr;
+    if (!pool_.empty()) {
+        e = pool_.back();
+        pool_.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        e = nullptr;
+    }
+    return e;
+}
+
+void Batch::set_timing(int stages) {
+    timing_ = stages;
+    for (auto &e : events_) { pool_.push_back(e.a); pool_.push_back(e.b); }
+    events_.clear();
+}
+
+int Batch::read_timing(int stage, int *launches, double *total_ms) {
+    LF_HIP_CHECK(hipStreamSynchronize(stream_));
+    int n = 0;
+    double t = 0;
+    for (auto &e : events_) {
+        if (e.stage != stage) continue;
+        float ms = 0;
+        LF_HIP_CHECK(hipEventElapsedTime(&ms, e.a, e.b));
+        t += ms;
+        ++n;
+    }
+    *launches = n;
+    *total_ms = t;
+    return LF_OK;
+}
+
+int Batch::process(const uint16_t *const *depth, const unsigned char *const *rgb) { return step(depth, rgb, false); }
+
+int Batch::process_host(const uint16_t *const *depth, const unsigned char *const *rgb) {
+    return step(depth, rgb, true);
+}
+
+int Batch::step(const uint16_t *const *depth_in, const unsigned char *const *rgb_in, bool from_host) {
     const int n = size();
     const size_t PB = lf_pass_bytes(), J = lf_track_job_bytes(), RJ = lf_ray_job_bytes(), FJ = lf_fuse_job_bytes();
+    // host frames are read by the upload passes from the batch's staging
+    // buffers, written by copies on the upload stream ahead of those passes
+    std::vector<const uint16_t *> dstage(n);
+    std::vector<const unsigned char *> cstage(n);
+    if (from_host) {
+        if (depth_in_.size() != (size_t)n) {
+            depth_in_.resize(n);
+            rgb_in_.resize(n);
+        }
+        for (int i = 0; i < n; ++i) {
+            const lf_intrinsics &K = sys_[i]->mapping().intrinsics();
+            try {
+                depth_in_[i].create(K.width, K.height, 2);
+                rgb_in_[i].create(K.width, K.height, 3);
+            } catch (const std::bad_alloc &) {
+                return lf_set_error(LF_ERR_HIP, "device allocation of the batch's input staging failed");
+            }
+            dstage[i] = depth_in_[i].ptr<uint16_t>();
+            cstage[i] = rgb_in_[i].ptr<unsigned char>();
+        }
+    }
+    const uint16_t *const *depth = from_host ? dstage.data() : depth_in;
+    const unsigned char *const *rgb = from_host ? cstage.data() : rgb_in;
     // every system's upload and re-pyramid planned as frame passes, its
     // Gauss-Newton steps as jobs
     std::vector<std::vector<unsigned char>> jobs(n), up(n), after(n);
@@ -780,18 +848,50 @@ int Batch::process(const uint16_t *const *depth, const unsigned char *const *rgb
     // for the previous step's tracking, so they run beside its fusion and raycast.
     if (tracked_once_) LF_HIP_CHECK(hipStreamWaitEvent(aux_, tracked_, 0));
     LF_HIP_CHECK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, aux_));
+    if (from_host) {
+        // the staging buffers were last read by the previous step's upload
+        // passes, earlier on this same stream
+        for (int i = 0; i < n; ++i) {
+            LF_HIP_CHECK(hipMemcpyAsync(depth_in_[i].ptr<void>(), depth_in[i], depth_in_[i].bytes(),
+                                        hipMemcpyHostToDevice, aux_));
+            LF_HIP_CHECK(hipMemcpyAsync(rgb_in_[i].ptr<void>(), rgb_in[i], rgb_in_[i].bytes(), hipMemcpyHostToDevice,
+                                        aux_));
+        }
+    }
     for (size_t p = 0; p < n_up; ++p) LF_RC(lf_pass_batch(dev + p * n * PB, n, wmax[p], aux_));
     LF_HIP_CHECK(hipEventRecord(uploaded_, aux_));
     LF_HIP_CHECK(hipStreamWaitEvent(stream_, uploaded_, 0));
+    // stage events (set_timing), recorded on the batch stream
+    Ev ev[5];
+    for (int sid = 0; sid < 5; ++sid) {
+        ev[sid] = Ev{sid, nullptr, nullptr};
+        if (timing_ & (1 << sid)) {
+            ev[sid].a = take_event();
+            ev[sid].b = take_event();
+            if (!ev[sid].a || !ev[sid].b) return lf_set_error(LF_ERR_HIP, "hipEventCreate failed");
+        }
+    }
+    if (ev[4].a) LF_HIP_CHECK(hipEventRecord(ev[4].a, stream_));
+    if (ev[2].a) LF_HIP_CHECK(hipEventRecord(ev[2].a, stream_));
     for (size_t s = 0; s < steps; ++s)
         LF_RC(lf_track_batch_step(dev + off_track + s * n * J, n, gmax[s], half_maps_, stream_));
+    if (ev[2].b) LF_HIP_CHECK(hipEventRecord(ev[2].b, stream_));
     LF_HIP_CHECK(hipEventRecord(tracked_, stream_));
     tracked_once_ = true;
     // block allocation, visible lists and integration of every map
-    LF_RC(lf_update_batch_launch(dev + off_fuse, fd, stream_));
-    LF_RC(lf_raycast_batch_launch(dev + off_ray, ray, 1, stream_));
+    if (ev[3].a) LF_HIP_CHECK(hipEventRecord(ev[3].a, stream_));
+    LF_RC(lf_update_batch_launch(dev + off_fuse, fd, stream_, ev[0].a, ev[0].b));
+    if (ev[3].b) LF_HIP_CHECK(hipEventRecord(ev[3].b, stream_));
+    LF_RC(lf_raycast_batch_launch(dev + off_ray, ray, 1, stream_, ev[1].a, ev[1].b));
     // re-pyramids of the raycast maps
     for (size_t p = n_up; p < n_pass; ++p) LF_RC(lf_pass_batch(dev + p * n * PB, n, wmax[p], stream_));
+    if (ev[4].b) LF_HIP_CHECK(hipEventRecord(ev[4].b, stream_));
+    for (const Ev &e : ev) {
+        // an event pair whose launch did not happen (nothing to fuse) is not a timing
+        if (!e.a) continue;
+        if (e.stage == 0 && !fd.any) { pool_.push_back(e.a); pool_.push_back(e.b); continue; }
+        events_.push_back(e);
+    }
     LF_HIP_CHECK(hipEventRecord(table_read_[k], stream_));
     table_used_[k] = true;
     parity_ ^= 1;
@@ -865,6 +965,20 @@ int System::process_images(const uint16_t *depth, const unsigned char *rgb) {
     for (Ev *e : {&fr, &tr, &ig, &rc})
         if (e->a) events_.push_back(*e);
     return LF_OK;
+}
+
+int System::process_images_host(const uint16_t *depth, const unsigned char *rgb) {
+    const int W = cfg_.K.width, H = cfg_.K.height;
+    try {
+        host_depth_.create(W, H, 2);
+        host_rgb_.create(W, H, 3);
+    } catch (const std::bad_alloc &) {
+        return lf_set_error(LF_ERR_HIP, "device allocation of the input staging failed");
+    }
+    // the staging was last read by the previous frame's upload, earlier on this stream
+    LF_HIP_CHECK(hipMemcpyAsync(host_depth_.ptr<void>(), depth, host_depth_.bytes(), hipMemcpyHostToDevice, stream_));
+    LF_HIP_CHECK(hipMemcpyAsync(host_rgb_.ptr<void>(), rgb, host_rgb_.bytes(), hipMemcpyHostToDevice, stream_));
+    return process_images(host_depth_.ptr<uint16_t>(), host_rgb_.ptr<unsigned char>());
 }
 
 int System::writeMapToDisk(const std::string &file_name) { return mapping_->writeMapToDisk(file_name); }
@@ -1123,6 +1237,11 @@ extern "C" int lf_system_process_images(lf_system *s, const uint16_t *depth, con
     return s->sys->process_images(depth, rgb);
 }
 
+extern "C" int lf_system_process_images_host(lf_system *s, const uint16_t *depth, const unsigned char *rgb) {
+    if (!s || !depth || !rgb) return lf_set_error(LF_ERR_ARG, "null argument to process_images_host");
+    return s->sys->process_images_host(depth, rgb);
+}
+
 extern "C" int lf_system_synchronize(lf_system *s) {
     LF_HIP_CHECK(hipStreamSynchronize(s->sys->stream()));
     return LF_OK;
@@ -1230,6 +1349,24 @@ extern "C" int lf_batch_process(lf_batch *b, const uint16_t *const *depth, const
     for (int i = 0; i < b->b->size(); ++i)
         if (!depth[i] || !rgb[i]) return lf_set_error(LF_ERR_ARG, "null frame for batch system %d", i);
     return b->b->process(depth, rgb);
+}
+
+extern "C" int lf_batch_process_host(lf_batch *b, const uint16_t *const *depth, const unsigned char *const *rgb) {
+    if (!b || !depth || !rgb) return lf_set_error(LF_ERR_ARG, "null argument to lf_batch_process_host");
+    for (int i = 0; i < b->b->size(); ++i)
+        if (!depth[i] || !rgb[i]) return lf_set_error(LF_ERR_ARG, "null frame for batch system %d", i);
+    return b->b->process_host(depth, rgb);
+}
+
+extern "C" int lf_batch_set_timing(lf_batch *b, int stages) {
+    if (!b) return lf_set_error(LF_ERR_ARG, "null batch");
+    b->b->set_timing(stages);
+    return LF_OK;
+}
+
+extern "C" int lf_batch_read_timing(lf_batch *b, int stage, int *launches, double *total_ms) {
+    if (!b || !launches || !total_ms) return lf_set_error(LF_ERR_ARG, "null argument to lf_batch_read_timing");
+    return b->b->read_timing(stage, launches, total_ms);
 }
 
 extern "C" int lf_batch_synchronize(lf_batch *b) {

@@ -419,6 +419,18 @@ int lf_icp_reduce(const float *curr_vmap, const float *curr_nmap,
                   const float *last_vmap, const float *last_nmap,
                   int cols, int rows, const float *pose12, const lf_intrinsics *K,
                   float *scratch, float *sum29, void *stream);
+/* lf_icp_reduce without the scratch argument, allocating nothing: no
+ * counterpart in the reference; for callers that cannot allocate (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  Its
+ * partial rows go to a fixed device array of 512 x 58 floats (116 KiB) that
+ * the library holds on each GPU it runs on until the process ends; any image
+ * size fits (the reduction's grid is capped at 512 workgroups).  All calls on
+ * one GPU share that array: calls that overlap on one GPU (other streams or
+ * threads) race, so serialise them.  Same result as lf_icp_reduce. */
+int lf_icp_reduce_noalloc(const float *curr_vmap, const float *curr_nmap,
+                          const float *last_vmap, const float *last_nmap,
+                          int cols, int rows, const float *pose12, const lf_intrinsics *K,
+                          float *sum29, void *stream);
 /* rgb_step (m_estimator.cu:437-482): Huber-weighted photometric normal equations
  * with stddev_estimate; same output layout as lf_icp_reduce. */
 int lf_rgb_step(const float *curr_intensity, const float *last_intensity,

@@ -128,6 +128,7 @@ SIGNATURES = [
     ("lf_tracker_plane", I, [P, I, I, I, P]),
     ("lf_reduce_scratch_floats", ctypes.c_size_t, []),
     ("lf_icp_reduce", I, [P, P, P, P, I, I, P, P, P, P, P]),
+    ("lf_icp_reduce_noalloc", I, [P, P, P, P, I, I, P, P, P, P]),
     ("lf_rgb_step", I, [P, P, P, P, P, I, I, F, P, P, P, P, P]),
     ("lf_unpack_jtjjtr", None, [P, P, P, P]),
     ("lf_system_create", I, [P, P]),

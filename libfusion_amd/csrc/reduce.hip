@@ -27,6 +27,8 @@
 //  * The stand-alone reductions of the C-ABI (lf_icp_reduce, lf_rgb_step) are
 //    a partial-sum launch and a one-workgroup final-sum launch.
 #include <cfloat>
+#include <climits>
+#include <mutex>
 
 #include "lf_common.h"
 #include "lf_track.h"
@@ -886,26 +888,87 @@ int lf_track_set_pose(LfTrackState *st, const double *T, hipStream_t stream) {
     return LF_OK;
 }
 
+// icp_reduce of lf_icp_reduce and lf_icp_reduce_noalloc: the partial rows go
+// to `partials` (capacity floats), then the final kernel sums them.
+static int icp_reduce_into(const float *curr_vmap, const float *curr_nmap, const float *last_vmap,
+                           const float *last_nmap, int cols, int rows, const float *pose12, const lf_intrinsics *K,
+                           float *partials, size_t capacity, float *sum29, hipStream_t st) {
+    ReduceArgs a = {};
+    a.lv.curr_vmap = curr_vmap; a.lv.curr_nmap = curr_nmap; a.lv.last_vmap = last_vmap; a.lv.last_nmap = last_nmap;
+    a.lv.cols = cols; a.lv.rows = rows; a.lv.K = *K;
+    if (int rc = check_level(a.lv)) return rc;
+    if ((long long)cols * rows > INT_MAX)
+        return lf_set_error(LF_ERR_ARG, "icp_reduce: %dx%d is more than %d pixels", cols, rows, INT_MAX);
+    a.pose12 = pose12;
+    a.angle = icp_angle();
+    a.dthr = 0.01f;
+    a.partials = partials;
+    a.out29 = sum29;
+    const int G = grid_for(a.lv);
+    if ((size_t)G * kVals > capacity)
+        return lf_set_error(LF_ERR_CAPACITY, "icp_reduce: %d workgroups need %zu scratch floats, %zu available", G,
+                            (size_t)G * kVals, capacity);
+    k_reduce<1><<<G, kThreads, 0, st>>>(a);
+    LF_LAUNCH_CHECK();
+    k_reduce_final<1><<<1, kThreads, 0, st>>>(a, G);
+    LF_LAUNCH_CHECK();
+    return LF_OK;
+}
+
 extern "C" int lf_icp_reduce(const float *curr_vmap, const float *curr_nmap, const float *last_vmap,
                              const float *last_nmap, int cols, int rows, const float *pose12,
                              const lf_intrinsics *K, float *scratch, float *sum29, void *stream) {
     if (!curr_vmap || !curr_nmap || !last_vmap || !last_nmap || !pose12 || !K || !scratch || !sum29)
         return lf_set_error(LF_ERR_ARG, "null argument to icp_reduce");
-    ReduceArgs a = {};
-    a.lv.curr_vmap = curr_vmap; a.lv.curr_nmap = curr_nmap; a.lv.last_vmap = last_vmap; a.lv.last_nmap = last_nmap;
-    a.lv.cols = cols; a.lv.rows = rows; a.lv.K = *K;
-    if (int rc = check_level(a.lv)) return rc;
-    a.pose12 = pose12;
-    a.angle = icp_angle();
-    a.dthr = 0.01f;
-    a.partials = scratch;
-    a.out29 = sum29;
-    const int G = grid_for(a.lv);
-    k_reduce<1><<<G, kThreads, 0, lf_stream(stream)>>>(a);
-    LF_LAUNCH_CHECK();
-    k_reduce_final<1><<<1, kThreads, 0, lf_stream(stream)>>>(a, G);
-    LF_LAUNCH_CHECK();
+    return icp_reduce_into(curr_vmap, curr_nmap, last_vmap, last_nmap, cols, rows, pose12, K, scratch,
+                           lf_reduce_partials(), sum29, lf_stream(stream));
+}
+
+// Scratch of lf_icp_reduce_noalloc: the partial rows of at most kGrid
+// workgroups (kGrid * kVals floats, 116 KiB), one copy per GPU, held until the
+// process ends.  Host code never passes the array's name to a kernel or a
+// copy: on the host it is only a stand-in for the symbol, so its device
+// address is looked up per device (hipGetSymbolAddress, with that device
+// current) and that address is used everywhere.
+constexpr size_t kNoallocScratchFloats = (size_t)kGrid * kVals;
+static __device__ __align__(256) float scratch[kNoallocScratchFloats];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float *g_scratch_dev[kMaxDevices] = {};
+
+// The current device's address of `scratch`, resolved on its first use there.
+int noalloc_scratch(float **out) {
+    int dev = -1;
+    LF_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return lf_set_error(LF_ERR_ARG, "device %d is beyond the %d devices of the fixed scratch", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return lf_set_error(LF_ERR_HIP, "hipGetSymbolAddress of the fixed scratch failed on device %d: %s", dev,
+                                hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<float *>(p);
+    }
+    *out = g_scratch_dev[dev];
     return LF_OK;
+}
+}  // namespace
+
+extern "C" int lf_icp_reduce_noalloc(const float *curr_vmap, const float *curr_nmap, const float *last_vmap,
+                                     const float *last_nmap, int cols, int rows, const float *pose12,
+                                     const lf_intrinsics *K, float *sum29, void *stream) {
+    if (!curr_vmap || !curr_nmap || !last_vmap || !last_nmap || !pose12 || !K || !sum29)
+        return lf_set_error(LF_ERR_ARG, "null argument to icp_reduce_noalloc");
+    float *partials = nullptr;
+    if (int rc = noalloc_scratch(&partials)) return rc;
+    // every partial the final kernel reads is written by this call's first
+    // kernel, so nothing in the scratch needs clearing
+    return icp_reduce_into(curr_vmap, curr_nmap, last_vmap, last_nmap, cols, rows, pose12, K, partials,
+                           kNoallocScratchFloats, sum29, lf_stream(stream));
 }
 
 extern "C" int lf_rgb_step(const float *curr_intensity, const float *last_intensity, const float *last_vmap,

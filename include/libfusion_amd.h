@@ -56,6 +56,8 @@ extern "C" {
 #define LF_CNT_VIS_SUM_HI 5  /* high word */
 #define LF_CNT_UPDATES 6     /* number of updates since reset */
 #define LF_CNT_RENDER_BLOCKS 7 /* visible blocks with a non-empty tile rectangle, last raycast */
+#define LF_CNT_FUSE_PASSES 8 /* block-selection passes (updates and de-integrations) since reset:
+                                the per-pass seed of a mixed map's stochastic weight rounding */
 #define LF_NUM_COUNTERS 16
 
 /* Camera intrinsics — mirrors fusion::IntrinsicMatrix (intrinsic_matrix.h:12-37).
@@ -158,6 +160,14 @@ int lf_dir_cell(int bx, int by, int bz);
  * code); weights above 496 give 0xFF. */
 int lf_weight_code(float w);
 float lf_weight_value(int code);
+/* The code of w >= 0 rounded stochastically with the random bits rnd: the
+ * next code up with probability equal to w's fractional position between its
+ * two neighbouring codes (rnd's low 19 bits dither weights >= 2^-6, its top 24
+ * bits smaller ones), so the code's value equals w in expectation.  This is
+ * how a mixed map stores fused and de-integrated weights, with rnd a hash of
+ * the voxel and the map's fusion pass (LF_CNT_FUSE_PASSES): round-to-nearest
+ * would stall a running sum of small increments at a fraction of its value. */
+int lf_weight_code_dither(float w, unsigned int rnd);
 
 /* Library version string and the gfx target the kernels were built for. */
 const char *lf_version(void);

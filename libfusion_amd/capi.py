@@ -16,6 +16,7 @@ HEADER = os.path.join(os.path.dirname(_HERE), "include", "libfusion_amd.h")
 LF_OK, LF_ERR_ARG, LF_ERR_HIP, LF_ERR_CAPACITY, LF_ERR_NOT_READY = 0, 1, 2, 3, 4
 CNT_ALLOCATED, CNT_VISIBLE, CNT_OVERFLOW, CNT_TRIANGLES = 0, 1, 2, 3
 CNT_VIS_SUM_LO, CNT_VIS_SUM_HI, CNT_UPDATES, CNT_RENDER_BLOCKS = 4, 5, 6, 7
+CNT_FUSE_PASSES = 8  # block-selection passes since reset (seed of the mixed weights' dither)
 OVF_TABLE, OVF_POOL, OVF_TRIANGLES = 1, 2, 4
 SDF_UNSEEN = -32768  # sdf code of a voxel with weight 0
 PRECISION_FP32 = 0   # int16 TSDF, float32 weights and maps
@@ -76,6 +77,7 @@ SIGNATURES = [
     ("lf_dir_cell", I, [I, I, I]),
     ("lf_weight_code", I, [F]),
     ("lf_weight_value", F, [I]),
+    ("lf_weight_code_dither", I, [F, U]),
     ("lf_mc_tables", None, [P, P]),
     ("lf_device_count", I, []),
     ("lf_last_error", ctypes.c_char_p, []),

@@ -203,6 +203,8 @@ int lf_rebuild_dir(const lf_map_params *p, const lf_map_storage *s, hipStream_t 
 
 extern "C" int lf_weight_code(float w) { return (int)lf::weight_code(w); }
 
+extern "C" int lf_weight_code_dither(float w, unsigned int rnd) { return (int)lf::weight_code_sr(w, rnd); }
+
 extern "C" float lf_weight_value(int code) { return lf::weight_value((unsigned int)code & 0xFFu); }
 
 extern "C" int lf_map_read_counters(const lf_map_storage *s, unsigned int *out16, void *stream) {

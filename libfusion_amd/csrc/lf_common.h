@@ -124,6 +124,39 @@ LF_HD unsigned int weight_code(float w) {
     return !(w > 0.0f) ? 0u : w < 0.015625f ? small : big;
 }
 
+// The same code rounded stochastically: up to the next code with probability
+// equal to w's fractional position between its two neighbouring codes, so
+// the stored weight equals w in expectation.  rnd supplies the random bits
+// (its low 19 bits dither the normal range, its top 24 the range below 2^-6).
+// A running sum of increments far below half a code step would stall at a
+// fraction of its true value under round-to-nearest; this keeps it unbiased.
+// Callers clamp w to 255 first, so the result stays below 0xF1.
+LF_HD unsigned int weight_code_sr(float w, unsigned int rnd) {
+    const float d = (float)(rnd >> 8) * (1.0f / 16777216.0f);  // [0, 1)
+    const unsigned int small = (unsigned int)floorf(fminf(fmaxf(w, 0.0f), 0.015625f) * 1024.0f + d);
+    const unsigned int r = __builtin_bit_cast(unsigned int, w) - (120u << 23);
+    const unsigned int c = (r + (rnd & 0x7FFFFu)) >> 19;
+    const unsigned int big = c > 0xFFu ? 0xFFu : c;
+    return !(w > 0.0f) ? 0u : w < 0.015625f ? small : big;
+}
+
+// 32 random bits of (voxel, fusion pass): an integer hash (xor-shift /
+// multiply finaliser) of the voxel's block coordinate, its index in the block
+// and the map's pass counter, so every voxel and every pass dithers
+// independently.  The voxel is named by where it is, not by its pool slot
+// (which the allocation race decides), so a map's values do not depend on
+// the order its blocks were allocated in.
+LF_HD unsigned int voxel_rand(int bx, int by, int bz, unsigned int local, unsigned int pass) {
+    unsigned int x = (unsigned int)bx * 0x9E3779B1u ^ (unsigned int)by * 0x85EBCA77u ^
+                     (unsigned int)bz * 0xC2B2AE3Du ^ (local + (pass << 9)) * 0x27D4EB2Fu;
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return x;
+}
+
 // *(T *)((char *)base + off): a 32-bit byte offset from a (scalar) base, so
 // the access takes the offset in one VGPR instead of 64-bit address math.
 template <class T> LF_DEV T &at_byte(T *base, unsigned int off) {

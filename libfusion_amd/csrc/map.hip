@@ -193,7 +193,12 @@ LF_DEV void alloc_wg(const MapDev &m, const float *__restrict__ depth, int cols,
                      unsigned long long *s_set) {
     const int tid = threadIdx.x;
     for (int i = tid; i < kDedup; i += 256) s_set[i] = kEmptyKey;
-    if (tx == 0 && ty == 0 && tid == 0) m.counters[LF_CNT_VISIBLE] = 0u;
+    if (tx == 0 && ty == 0 && tid == 0) {
+        m.counters[LF_CNT_VISIBLE] = 0u;
+        // one more fusion pass: read by the integration / de-integration that
+        // follows this kernel (the seed of the mixed weights' dither)
+        m.counters[LF_CNT_FUSE_PASSES] = m.counters[LF_CNT_FUSE_PASSES] + 1u;
+    }
     float Twc[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) Twc[i] = pose24[i];
@@ -339,6 +344,7 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
 #pragma unroll
     for (int i = 0; i < 12; ++i) Tcw[i] = pose24[12 + i];
     const int nvis = (int)m.counters[LF_CNT_VISIBLE];
+    const unsigned int pass = kMixed ? m.counters[LF_CNT_FUSE_PASSES] : 0u;
     if (wg == 0 && threadIdx.x == 0) {
         atomicAdd(&m.counters[LF_CNT_UPDATES], 1u);
         atomicAdd(reinterpret_cast<unsigned long long *>(&m.counters[LF_CNT_VIS_SUM_LO]), (unsigned long long)nvis);
@@ -467,9 +473,13 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
             }
             if constexpr (kMixed) {
                 unsigned short *sdf16 = reinterpret_cast<unsigned short *>(sdf_b);
+                // the stored weight is rounded stochastically (unbiased running sums)
+                const unsigned int rnd = voxel_rand(p.x, p.y, p.z, o, pass);
                 if (wp[h] == 0.0f) {
-                    // a weight whose code is 0 leaves the voxel unseen
-                    const unsigned int code = weight_code(fminf(wt, 255.0f));
+                    // a first observation with a positive weight is seen, as in
+                    // the reference (the smallest code stands for weights below
+                    // it); a zero weight leaves the voxel unseen
+                    const unsigned int code = wt > 0.0f ? max(1u, weight_code_sr(fminf(wt, 255.0f), rnd)) : 0u;
                     at_byte(sdf16, 2u * o) = code == 0u ? kHalfUnseen : float_to_half_bits(sdf_h);
                     at_byte(w8_b, o) = (unsigned char)code;
                     if (kColour) {
@@ -481,7 +491,7 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
                 const float inv = __builtin_amdgcn_rcpf(wp[h] + wt);
                 const float spf = half_bits_to_float((unsigned short)sp[h]);
                 at_byte(sdf16, 2u * o) = float_to_half_bits((spf * wp[h] + sdf_h * wt) * inv);
-                at_byte(w8_b, o) = (unsigned char)weight_code(fminf(wp[h] + wt, 255.0f));
+                at_byte(w8_b, o) = (unsigned char)weight_code_sr(fminf(wp[h] + wt, 255.0f), rnd);
                 if (kColour) {
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
@@ -593,6 +603,7 @@ __global__ __launch_bounds__(256) void k_deintegrate(MapDev m, const float *__re
 #pragma unroll
     for (int i = 0; i < 12; ++i) Tcw[i] = pose24[12 + i];
     const int nvis = (int)m.counters[LF_CNT_VISIBLE];
+    const unsigned int pass = kMixed ? m.counters[LF_CNT_FUSE_PASSES] : 0u;
     const float vs = m.voxel_size, trunc = m.truncation, inv_trunc = 1.0f / trunc;
     const float zmin = m.zmin_update, zmax = m.zmax_update;
     const int t = threadIdx.x;
@@ -635,8 +646,15 @@ __global__ __launch_bounds__(256) void k_deintegrate(MapDev m, const float *__re
             if (w0 == 0.0f) continue;  // nothing fused here
             unsigned char *c = m.rgb + (size_t)b * (3 * LF_BLOCK3) + (size_t)(t + 256 * h);
             const float wr = w0 - wt;
-            const unsigned int wr_code = kMixed && wr > 0.0f ? weight_code(wr) : 0u;
-            if (!(wr > 0.0f) || (kMixed && wr_code == 0u)) {
+            // mixed: the stored w0 is within one code step (at most 1/16 of
+            // it) of the exact sum, so a remainder below that is the rounding
+            // of the fused weights and the voxel returns to unseen; a larger
+            // one is rounded stochastically, as fusion stores it
+            const bool keep = kMixed ? wr > w0 * (1.0f / 16.0f) : wr > 0.0f;
+            const unsigned int wr_code =
+                kMixed && keep ? max(1u, weight_code_sr(wr, voxel_rand(p.x, p.y, p.z, (unsigned int)(t + 256 * h), pass)))
+                               : 0u;
+            if (!keep) {
                 if (kMixed) {
                     sdf16[vi] = kHalfUnseen;
                     w8[vi] = 0;

@@ -845,12 +845,32 @@ template <class R, class RowFn> void reduce29(int n, RowFn rowfn, R *out) {
         for (int k = 0; k < 29; ++k) out[k] += part[(size_t)c * 29 + k];
 }
 
+// Every pixel's row instead of their sums (test infrastructure: the exact sum
+// of the method's own rows measures a summation's error): 8 values per pixel,
+// the 7 row values (0 when rejected) and 1 if the pixel was accepted.
+template <class R, class RowFn> void rows8(int n, RowFn rowfn, R *out) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        R row[7] = {0, 0, 0, 0, 0, 0, 0};
+        const bool found = rowfn(i, row);
+        for (int k = 0; k < 7; ++k) out[8 * (size_t)i + k] = found ? row[k] : (R)0;
+        out[8 * (size_t)i + 7] = found ? (R)1 : (R)0;
+    }
+}
+
+// The 29 sums of rowfn's rows (out) or every pixel's row (rows8, when rows8_out
+// is given).
+template <class R, class RowFn> void sums_or_rows(int n, RowFn rowfn, R *out, R *rows8_out) {
+    if (rows8_out) rows8<R>(n, rowfn, rows8_out);
+    else reduce29<R>(n, rowfn, out);
+}
+
 // ICPReduction (pose_estimator.cu:166-222): point-to-plane rows.
 template <class R> void icp_reduce(const R *curr_v, const R *curr_n, const R *last_v, const R *last_n, int cols,
-                                   int rows, const M34<R> &pose, const K_<R> &K, R *out) {
+                                   int rows, const M34<R> &pose, const K_<R> &K, R *out, R *rows8_out = nullptr) {
     const R angle = sizeof(R) == 4 ? (R)(float)std::cos(30 * 3.14 / 180) : (R)std::cos(30 * 3.14 / 180);
     const R dthr = sizeof(R) == 4 ? (R)0.01f : (R)0.01;
-    reduce29<R>(cols * rows, [&](int i, R *row) -> bool {
+    sums_or_rows<R>(cols * rows, [&](int i, R *row) -> bool {
         const R *lv = last_v + 4 * (size_t)i;
         V3<R> vl = {lv[0], lv[1], lv[2]};
         if (isnan_(vl.x)) return false;
@@ -872,7 +892,7 @@ template <class R> void icp_reduce(const R *curr_v, const R *curr_n, const R *la
         row[3] = c.x; row[4] = c.y; row[5] = c.z;
         row[6] = dot(ng, sub(vc, vg));
         return true;
-    }, out);
+    }, out, rows8_out);
 }
 
 template <class R> inline R interp2(const R *img, int w, R x, R y) {  // m_estimator.cu:340-346
@@ -884,10 +904,11 @@ template <class R> inline R interp2(const R *img, int w, R x, R y) {  // m_estim
 
 // RgbReduction2 (m_estimator.cu:313-391): Huber-weighted photometric rows.
 template <class R> void rgb_step(const R *curr_I, const R *last_I, const R *last_v, const R *dIdx, const R *dIdy,
-                                 int cols, int rows, R stddev, const M34<R> &pose, const K_<R> &K, R *out) {
+                                 int cols, int rows, R stddev, const M34<R> &pose, const K_<R> &K, R *out,
+                                 R *rows8_out = nullptr) {
     const double sd = (double)stddev;
     const R th = (R)(1.345 * sd);
-    reduce29<R>(cols * rows, [&](int i, R *row) -> bool {
+    sums_or_rows<R>(cols * rows, [&](int i, R *row) -> bool {
         const R *pt = last_v + 4 * (size_t)i;
         if (pt[3] < 0 || isnan_(pt[0])) return false;
         R il = last_I[i];
@@ -912,7 +933,7 @@ template <class R> void rgb_step(const R *curr_I, const R *last_I, const R *last
         row[0] = wl.x; row[1] = wl.y; row[2] = wl.z;
         row[3] = c.x * w; row[4] = c.y * w; row[5] = c.z * w;
         return true;
-    }, out);
+    }, out, rows8_out);
 }
 
 // create_jtjjtr<6,7> (reduce_sum.h:48-61).
@@ -1337,6 +1358,14 @@ extern "C" void orc_warp_image(const uint8_t *src, const float *vm, int w, int h
     extern "C" void orc_rgb_step_##S(const R *ci, const R *li, const R *lv, const R *dx, const R *dy, int w,   \
                                      int h, R stddev, const R *pose12, const Intr *K, R *out) {                \
         rgb_step(ci, li, lv, dx, dy, w, h, stddev, m34_from(pose12), K_<R>(*K), out);                          \
+    }                                                                                                          \
+    extern "C" void orc_icp_rows_##S(const R *cv, const R *cn, const R *lv, const R *ln, int w, int h,         \
+                                     const R *pose12, const Intr *K, R *rows8) {                               \
+        icp_reduce(cv, cn, lv, ln, w, h, m34_from(pose12), K_<R>(*K), (R *)nullptr, rows8);                    \
+    }                                                                                                          \
+    extern "C" void orc_rgb_rows_##S(const R *ci, const R *li, const R *lv, const R *dx, const R *dy, int w,   \
+                                     int h, R stddev, const R *pose12, const Intr *K, R *rows8) {              \
+        rgb_step(ci, li, lv, dx, dy, w, h, stddev, m34_from(pose12), K_<R>(*K), (R *)nullptr, rows8);          \
     }                                                                                                          \
     extern "C" int orc_track_maps_##S(const R *ref_vmap, const uint8_t *ref_rgb, const R *cur_vmap,            \
                                       const uint8_t *cur_rgb, const Intr *K, int L, const int *iters,          \

@@ -84,6 +84,8 @@ def lib():
             sig("orc_nmap_%s", None, [P, P, I, I])
             sig("orc_icp_reduce_%s", None, [P, P, P, P, I, I, P, P, P])
             sig("orc_rgb_step_%s", None, [P, P, P, P, P, I, I, R, P, P, P])
+            sig("orc_icp_rows_%s", None, [P, P, P, P, I, I, P, P, P])
+            sig("orc_rgb_rows_%s", None, [P, P, P, P, P, I, I, R, P, P, P])
             sig("orc_system_create_%s", P, [P, P, I, P, D])
             sig("orc_system_free_%s", None, [P])
             sig("orc_system_process_%s", None, [P, P, P])
@@ -353,6 +355,26 @@ def rgb_step(curr_I, last_I, last_v, dx, dy, stddev, pose12, K, dtype=np.float32
     out = np.zeros(29, dtype)
     ci = _c(curr_I, dtype)
     _img("orc_rgb_step", dtype)(_p(ci), _p(_c(last_I, dtype)), _p(_c(last_v, dtype)), _p(_c(dx, dtype)),
+                                _p(_c(dy, dtype)), ci.shape[1], ci.shape[0], float(stddev),
+                                _p(_c(pose12, dtype)), ctypes.byref(K), _p(out))
+    return out
+
+
+def icp_rows(curr_v, curr_n, last_v, last_n, pose12, K, dtype=np.float32):
+    """Every reference pixel's ICP row: [H*W, 8] = the 7 row values (0 when
+    rejected) and 1 where the pixel was accepted."""
+    cv = _c(curr_v, dtype)
+    out = np.zeros((cv.shape[0] * cv.shape[1], 8), dtype)
+    _img("orc_icp_rows", dtype)(_p(cv), _p(_c(curr_n, dtype)), _p(_c(last_v, dtype)), _p(_c(last_n, dtype)),
+                                cv.shape[1], cv.shape[0], _p(_c(pose12, dtype)), ctypes.byref(K), _p(out))
+    return out
+
+
+def rgb_rows(curr_I, last_I, last_v, dx, dy, stddev, pose12, K, dtype=np.float32):
+    """Every reference pixel's photometric row, laid out as icp_rows'."""
+    ci = _c(curr_I, dtype)
+    out = np.zeros((ci.shape[0] * ci.shape[1], 8), dtype)
+    _img("orc_rgb_rows", dtype)(_p(ci), _p(_c(last_I, dtype)), _p(_c(last_v, dtype)), _p(_c(dx, dtype)),
                                 _p(_c(dy, dtype)), ci.shape[1], ci.shape[0], float(stddev),
                                 _p(_c(pose12, dtype)), ctypes.byref(K), _p(out))
     return out

@@ -25,10 +25,11 @@ pytestmark = pytest.mark.gpu
 
 MIXED = capi.PRECISION_MIXED
 # TSDF RMS bar (truncation units).  The mixed error is dominated by the 8-bit
-# weight codes: a stored weight is within 2^-5 (3.1%) of the exact running sum,
-# which moves each fused mean by that fraction of the spread of its
-# observations (up to ~1 truncation unit of depth noise at 3 m).  Measured:
-# 0.020 at 320x240 / 10 mm and 0.037 at 640x480 / 5 mm.
+# weight codes, rounded stochastically: a stored weight is one of the two codes
+# around the exact running sum (within 1/16 of it), which moves each fused mean
+# by up to that fraction of the spread of its observations.  Measured on
+# tracked sequences (the poses differ slightly too): 0.014 at 320x240 / 10 mm
+# and 0.022 at 640x480 / 5 mm; at fixed true poses over 120 frames, 0.003.
 MIXED_TSDF_RMS = 0.05
 
 
@@ -184,7 +185,8 @@ def test_mixed_full_resolution_sequence_vs_fp64_oracle():
     """The metric's configuration (640x480, 5 mm voxels, 15 mm truncation,
     3-level ICP + RGB {10, 5, 3}) for 20 frames in mixed precision: every
     frame's pose within 0.5 mm / 0.05 deg of the FP64 oracle, tracking
-    success equal, TSDF RMS within MIXED_TSDF_RMS truncation units."""
+    success equal, TSDF RMS within MIXED_TSDF_RMS truncation units.  The plain
+    FP32 system on the same frames is held to the same per-frame pose bar."""
     from libfusion_amd import configs
     sc, K, _ = configs.c1()
     p16 = capi.map_params(2 ** 20, 1 << 17, 0.005, 0.015, precision=MIXED)
@@ -192,7 +194,7 @@ def test_mixed_full_resolution_sequence_vs_fp64_oracle():
     mixed = capi.System(K, p16, configs.C1_LEVELS, configs.C1_ITERATIONS, sc.depth_scale)
     plain = capi.System(K, p32, configs.C1_LEVELS, configs.C1_ITERATIONS, sc.depth_scale)
     osys = O.System(o_params(p32), o_K(K), configs.C1_LEVELS, configs.C1_ITERATIONS, sc.depth_scale, np.float64)
-    worst = (0.0, 0.0)
+    worst, worst32 = (0.0, 0.0), (0.0, 0.0)
     for f in range(20):
         d16, rgb = sc.frame(f)
         t_d, t_rgb = dev(d16), dev(rgb)
@@ -205,7 +207,12 @@ def test_mixed_full_resolution_sequence_vs_fp64_oracle():
         assert mixed.tracking()[0] == osys.success(), f
         assert dt <= 5e-4 and dr <= 0.05, (f, dt, dr)
         worst = (max(worst[0], dt), max(worst[1], dr))
+        pt, pr = pose_err(plain.pose(), osys.pose())
+        assert plain.tracking()[0] == osys.success(), f
+        assert pt <= 5e-4 and pr <= 0.05, ("plain", f, pt, pr)
+        worst32 = (max(worst32[0], pt), max(worst32[1], pr))
     om = osys.map().voxels()
+    print("plain FP32 vs FP64 at 640x480 / 5 mm, 20 frames: worst pose %.2e m %.4f deg" % worst32)
     rms, n = tsdf_rms(capi.read_tsdf(*mixed.map_storage()), om)
     rms32, n32 = tsdf_rms(capi.read_tsdf(*plain.map_storage()), om)
     print("mixed vs FP64 at 640x480 / 5 mm, 20 frames: worst pose %.2e m %.4f deg; TSDF RMS %.4f over %d voxels "
@@ -215,19 +222,48 @@ def test_mixed_full_resolution_sequence_vs_fp64_oracle():
     plain.close()
 
 
+def fp64_noise_spread(sc, p, K, frames, draws=2):
+    """How far the FP64 method's own pose moves when the sensor noise is drawn
+    afresh on the same trajectory: the largest (metres, degrees) difference at
+    the last frame between the oracle run on the stream's frames and runs on
+    renders of the same poses with other noise seeds.  On a well-conditioned
+    stream it is far below the pose bar; on an ill-conditioned one it shows
+    that the pose is only determined to within it."""
+    runs = []
+    for k in range(draws + 1):
+        osys = O.System(o_params(p), o_K(K), 3, (10, 5, 3), sc.depth_scale, np.float64)
+        for f in range(frames):
+            if k == 0:
+                d16, rgb = sc.frame(f)
+            else:
+                d16, rgb = sc.render(sc.pose(f), noise=True, frame_id=f + 1000 * k)
+            osys.process(d16, rgb)
+        runs.append(osys.pose())
+    errs = [pose_err(T, runs[0]) for T in runs[1:]]
+    return max(e[0] for e in errs), max(e[1] for e in errs)
+
+
 def test_c3_batched_uneven_streams_vs_fp64_oracle():
     """The batched configuration: 16 independent mixed-precision streams
     (seeds 100-115) at 160x120, 320x240, 640x480 and 848x480, voxels of 4, 5,
     8 and 10 mm (truncation 3 voxels), 2^18-2^20 buckets, stepped together in
-    one lf_batch for 6 frames.  Every stream's tracking verdict equals the FP64
-    oracle's.  Every tracked stream with voxels of 5 mm or more is within
-    0.5 mm / 0.05 deg of the oracle's pose.  At 4 mm voxels these small rooms
-    are ill-conditioned: the FP32 method itself (the oracle's FP32
-    instantiation, and this library's FP32 path) lands 0.4-0.55 mm from FP64,
-    and the mixed weights' 3% quantization about doubles that, so those
-    streams are held to 2 mm / 0.2 deg.  Errors are printed beside those of
-    the same stream run alone."""
-    frames = 6
+    one lf_batch for 8 frames.  Every stream's tracking verdict equals the FP64
+    oracle's, and every tracked stream's pose is within 0.5 mm / 0.05 deg of
+    the oracle's, except where the FP64 pose itself is not determined that
+    finely.
+
+    Some of these small rooms are ill-conditioned: on streams 2 and 3 (4 mm)
+    the FP64 oracle's own pose moves by 0.1-1.5 deg between independent draws
+    of the sensor noise on the same trajectory, on stream 5 (5 mm) by 2 mm, and
+    a perturbation of the volume at the level of any rounding moves it as far
+    (the FP32 path, the reference's method, is itself 0.5-0.7 mm from FP64 on
+    some of them).  No precision meets 0.5 mm / 0.05 deg there against one
+    noise draw.  So every tracked stream is held to the bar or, per component,
+    to that noise spread where the spread is larger: the mixed path must stay
+    within the method's own resolution.  At least 8 streams must be
+    well-conditioned (held to the bar itself).  The FP32 path's error on the
+    same frames is printed beside the mixed one for every stream."""
+    frames = 8
     runs = []
     for i in range(16):
         sc = Scene(3, 100 + i)
@@ -238,19 +274,22 @@ def test_c3_batched_uneven_streams_vs_fp64_oracle():
         runs.append((sc, K, p, host, [tuple(dev(a) for a in fr) for fr in host]))
     torch.cuda.synchronize()
     assert sorted({(r[1].width, r[1].height) for r in runs}) == [(160, 120), (320, 240), (640, 480), (848, 480)]
-    alone = []
+    alone, plain = [], []
     for sc, K, p, _, fs in runs:
-        s = capi.System(K, p, 3, (10, 5, 3), sc.depth_scale)
-        for d, c in fs:
-            s.process(d, c)
-        s.synchronize()
-        alone.append(s.pose())
-        s.close()
+        p32 = capi.map_params(p.num_buckets, p.num_blocks, p.voxel_size, p.truncation)
+        for prm, out in ((p, alone), (p32, plain)):
+            s = capi.System(K, prm, 3, (10, 5, 3), sc.depth_scale)
+            for d, c in fs:
+                s.process(d, c)
+            s.synchronize()
+            out.append(s.pose())
+            s.close()
     batch = capi.Batch([capi.system_config(K, p, 3, (10, 5, 3), sc.depth_scale) for sc, K, p, _, _ in runs])
     for f in range(frames):
         batch.process([r[4][f][0] for r in runs], [r[4][f][1] for r in runs])
     batch.synchronize()
-    tracked = 0
+    BAR = (5e-4, 0.05)
+    tracked = well_conditioned = 0
     for i, (sc, K, p, host, _) in enumerate(runs):
         osys = O.System(o_params(p), o_K(K), 3, (10, 5, 3), sc.depth_scale, np.float64)
         for d16, rgb in host:
@@ -259,13 +298,116 @@ def test_c3_batched_uneven_streams_vs_fp64_oracle():
         assert ok == osys.success(), i
         eb = pose_err(batch.pose(i), osys.pose())
         ea = pose_err(alone[i], osys.pose())
-        print("C3 stream %d %dx%d %g mm: tracked %s; vs FP64 batch %.2e m %.4f deg, alone %.2e m %.4f deg"
-              % (i, K.width, K.height, p.voxel_size * 1e3, ok, eb[0], eb[1], ea[0], ea[1]))
+        e32 = pose_err(plain[i], osys.pose())
         c = batch.counters(i)
         assert c[capi.CNT_UPDATES] == frames and c[capi.CNT_OVERFLOW] == 0, i
-        if ok:
-            tracked += 1
-            bar = (5e-4, 0.05) if p.voxel_size >= 0.005 else (2e-3, 0.2)
-            assert eb[0] <= bar[0] and eb[1] <= bar[1], (i, eb)
-    assert tracked >= 12
+        if not ok:
+            continue
+        tracked += 1
+        spread = fp64_noise_spread(sc, p, K, frames)
+        bar = (max(BAR[0], spread[0]), max(BAR[1], spread[1]))
+        held = bar == BAR
+        well_conditioned += held
+        print("C3 stream %d %dx%d %.0f mm vs FP64: mixed batch %.2e m %.4f deg, mixed alone %.2e m %.4f deg, FP32 "
+              "%.2e m %.4f deg; FP64 noise spread %.2e m %.4f deg%s"
+              % (i, K.width, K.height, p.voxel_size * 1e3, eb[0], eb[1], ea[0], ea[1], e32[0], e32[1], spread[0],
+                 spread[1], "" if held else " (ill-conditioned: held to the spread)"))
+        assert eb[0] <= bar[0] and eb[1] <= bar[1], (i, eb, bar)
+    assert tracked >= 12 and well_conditioned >= 8, (tracked, well_conditioned)
     batch.close()
+
+
+def fused_maps(sc, K, frames, params_list, deint=()):
+    """Fuse the rendered frames at their true poses into one map per params
+    (lf_update_image from a DeviceImage), then de-integrate the frames in
+    deint; also the FP64 oracle's map of the same inputs.  Returns the GPU maps
+    and the oracle's voxels."""
+    maps = [capi.Map(p, K.width, K.height) for p in params_list]
+    om = O.Map(o_params(params_list[0]), np.float64)
+    img = capi.Image(K, 1)
+    oK = o_K(K)
+    for f, remove in [(f, False) for f in frames] + [(f, True) for f in deint]:
+        T = sc.pose(f)
+        d16, rgb = sc.render(T, K.width, K.height, K.fx, K.fy, K.cx, K.cy, noise=True, frame_id=f)
+        img.upload(dev(d16), dev(rgb), sc.depth_scale)
+        pose = dev(capi.pose24(T))
+        for m in maps:
+            if remove:
+                m.deintegrate_image(img, pose)
+            else:
+                m.update_image(img, pose)
+        if not remove:
+            depth = d16.astype(np.float32) * np.float32(sc.depth_scale)
+            om.update_weighted(depth, O.nmap(O.back_project(O.bilateral(depth), oK)), rgb, T, oK)
+        torch.cuda.synchronize()
+    img.close()
+    return maps, om.voxels()
+
+
+def matched(a, b):
+    """Indices into two pools (blocks [n,3]) of the blocks they share."""
+    index = {tuple(p): i for i, p in enumerate(b.tolist())}
+    pairs = [(i, index[tuple(p)]) for i, p in enumerate(a.tolist()) if tuple(p) in index]
+    return np.array([i for i, _ in pairs]), np.array([j for _, j in pairs])
+
+
+def test_mixed_weights_stay_unbiased_over_120_frames():
+    """120 frames of the orbit fused at their true poses (320x240, 10 mm): the
+    mixed map's 8-bit weights, rounded stochastically, stay unbiased running
+    sums: over the voxels both maps saw, the stored weight over the FP32 map's
+    exact sum averages within 3% of 1 (round-to-nearest had stalled well
+    below), with a median deviation under 15%; and the mixed TSDF stays within
+    MIXED_TSDF_RMS truncation units RMS of the FP64 oracle's."""
+    sc = Scene(1, 1)
+    K = half_res(sc)
+    p16 = capi.map_params(2 ** 18, 1 << 17, 0.01, 0.03, precision=MIXED)
+    p32 = capi.map_params(2 ** 18, 1 << 17, 0.01, 0.03)
+    (m16, m32), om = fused_maps(sc, K, list(range(120)), [p16, p32])
+    pos16, t16, w16 = capi.read_tsdf(m16.params, m16.storage)
+    pos32, t32, w32 = capi.read_tsdf(m32.params, m32.storage)
+    i, j = matched(pos16, pos32)
+    a, e = w16[i], w32[j]
+    use = (e > 0.5) & (e < 250) & (a > 0)
+    ratio = a[use] / e[use]
+    print("120 frames: stored / exact weight: mean %.4f, median |dev| %.4f over %d voxels; exact weights up to %.1f"
+          % (ratio.mean(), np.median(np.abs(ratio - 1)), use.sum(), e[use].max()))
+    assert use.sum() > 100000 and e[use].max() > 20
+    assert abs(ratio.mean() - 1) < 0.03 and np.median(np.abs(ratio - 1)) < 0.15
+    rms16, n16 = tsdf_rms((pos16, t16, w16), om)
+    rms32, n32 = tsdf_rms((pos32, t32, w32), om)
+    print("TSDF RMS vs FP64: mixed %.4f over %d, FP32 %.5f over %d" % (rms16, n16, rms32, n32))
+    assert n16 > 0.95 * n32 and rms16 <= MIXED_TSDF_RMS
+
+
+def test_mixed_deintegration_round_trip():
+    """Mixed map: fuse frames A (six views) and B, de-integrate B: the volume
+    is that of A alone within the 8-bit weight codes' rounding: weight ratio
+    mean within 3% of 1, TSDF RMS within MIXED_TSDF_RMS truncation units, the
+    seen sets equal up to 0.5%; voxels only B saw keep at most a code step of
+    its weight."""
+    sc = Scene(1, 1)
+    K = half_res(sc)
+    p16 = capi.map_params(2 ** 18, 1 << 17, 0.01, 0.03, precision=MIXED)
+    A, B = [0, 2, 4, 6, 8, 10], [5]
+    (rt,), _ = fused_maps(sc, K, A + B, [p16], deint=B)
+    (alone,), _ = fused_maps(sc, K, A, [p16])
+    (only_b,), _ = fused_maps(sc, K, B, [p16])
+    pr, tr, wr = capi.read_tsdf(rt.params, rt.storage)
+    pa, ta, wa = capi.read_tsdf(alone.params, alone.storage)
+    pb, tb, wb = capi.read_tsdf(only_b.params, only_b.storage)
+    i, j = matched(pr, pa)
+    assert len(j) == len(pa)  # every block of A alone is in the round trip's pool
+    seen_r, seen_a = wr[i] > 0, wa[j] > 0
+    assert np.mean(seen_r != seen_a) < 0.005
+    both = seen_r & seen_a
+    ratio = wr[i][both] / wa[j][both]
+    d = tr[i][both].astype(np.float64) - ta[j][both]
+    print("round trip vs A alone: weight ratio mean %.4f, TSDF RMS %.4f over %d voxels"
+          % (ratio.mean(), np.sqrt(np.mean(d * d)), both.sum()))
+    assert abs(ratio.mean() - 1) < 0.03 and np.sqrt(np.mean(d * d)) <= MIXED_TSDF_RMS
+    # blocks only B allocated: what is left is below a code step of B's weight
+    extra = np.setdiff1d(np.arange(len(pr)), i)
+    assert len(extra) > 0
+    ib, jb = matched(pr[extra], pb)
+    left = wr[extra][ib]
+    assert np.all(left <= 0.07 * wb[jb] + 2.0 ** -10)

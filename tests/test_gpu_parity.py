@@ -162,6 +162,42 @@ def test_update_sequence_fusion_half_res():
         assert np.array_equal(gv, ov)
 
 
+def test_update_with_colour_matches_oracle(c0):
+    """lf_update, the reference's update(depth, image, pose) path
+    (update_map_with_colour_kernel, depth_fusion.cu:288-354: weight 1/d^2, no
+    normals): on C0 and then over five half-resolution views fused on both
+    sides, allocated and visible block sets bit-exact, TSDF within 32 int16
+    units, weights rel. 1e-5, colours within 1."""
+    sc, K, p, T, depth, _, rgb = c0
+    gmap = capi.Map(p, K.width, K.height)
+    gmap.update(dev(depth), dev(rgb), dev(capi.pose24(T)), K)
+    torch.cuda.synchronize()
+    omap = O.Map(o_params(p), np.float32)
+    omap.update_weighted(depth, None, rgb, T, o_K(K))
+    assert gmap.counters()[capi.CNT_OVERFLOW] == 0
+    exact, n = compare_maps(gmap, omap)
+    assert n > 1000 and exact > 0.99
+    # the 1/d^2 weights, not the normal-based ones
+    _, _, w, _ = gmap.voxels()
+    seen = w[w > 0]
+    assert seen.max() <= 1.0 / 0.3 ** 2 + 1e-3 and seen.min() >= 1.0 / 5.0 ** 2 - 1e-6
+    sc1 = Scene(1, 1)
+    K1 = half_res(sc1)
+    p1 = capi.map_params(2 ** 18, 1 << 17, 0.01, 0.03)
+    gmap = capi.Map(p1, K1.width, K1.height)
+    omap = O.Map(o_params(p1), np.float32)
+    for f in range(0, 25, 5):
+        T1 = sc1.pose(f)
+        d1, _, c1 = frame_inputs(sc1, T1, K1, f)
+        gmap.update(dev(d1), dev(c1), dev(capi.pose24(T1)), K1)
+        omap.update_weighted(d1, None, c1, T1, o_K(K1))
+        torch.cuda.synchronize()
+        compare_maps(gmap, omap)
+        _, gv = sort_blocks(gmap.visible_blocks())
+        _, ov = sort_blocks(omap.visible())
+        assert np.array_equal(gv, ov)
+
+
 def test_update_full_resolution_block_set():
     """640x480, 5 mm voxels: the exact allocated set on a full-size frame."""
     sc, K, p = configs.c1(num_blocks=1 << 18)

@@ -121,6 +121,40 @@ def test_weight_codes_are_the_nearest_code_ties_to_even():
     assert lib.lf_weight_code(1e9) == 255 and lib.lf_weight_code(-1.0) == 0
 
 
+def test_dithered_weight_codes_are_unbiased():
+    """lf_weight_code_dither (how a mixed map stores fused weights): always one
+    of the two codes around w (w itself when w is a code), the upper one with
+    probability w's fractional position between them, so the mean over the
+    random bits is w; and a running sum of increments far below half a code
+    step, stored after each addition, tracks the exact sum where round-to-
+    nearest stalls."""
+    lib = capi.load()
+    vals = capi.weight_values().astype(np.float64)
+    rng = np.random.default_rng(12)
+    rnds = rng.integers(0, 2 ** 32, size=4000, dtype=np.uint64).astype(np.uint32)
+    for w in [0.0, 1e-4, 0.004, 0.0149, 0.015625, 0.3, 1.0, 3.3, 17.77, 100.1, 254.9, 255.0]:
+        w = np.float32(w)
+        codes = np.array([lib.lf_weight_code_dither(float(w), int(r)) for r in rnds])
+        got = vals[codes]
+        lo = vals[vals <= w].max()
+        hi = vals[vals >= w].min()
+        assert np.all((got == lo) | (got == hi)), w
+        if lo == hi:
+            assert np.all(got == lo)
+            continue
+        frac = (float(w) - lo) / (hi - lo)
+        assert abs(np.mean(got == hi) - frac) < 4 * np.sqrt(frac * (1 - frac) / len(rnds)) + 1e-3, (w, frac)
+    # a running sum of 300 increments of 0.2 (exact 60): nearest codes stall
+    s_near, s_dith, exact = 0.0, np.zeros(200), 0.0
+    for k in range(300):
+        exact += 0.2
+        s_near = vals[lib.lf_weight_code(float(np.float32(s_near + 0.2)))]
+        s_dith = vals[[lib.lf_weight_code_dither(float(np.float32(s + 0.2)), int(r))
+                       for s, r in zip(s_dith, rng.integers(0, 2 ** 32, size=200, dtype=np.uint64))]]
+    assert s_near < 0.5 * exact
+    assert abs(s_dith.mean() / exact - 1) < 0.03
+
+
 def test_mc_tables_digest_and_consistency():
     edge, tri = capi.mc_tables()
     assert hashlib.sha256(struct.pack("<256i", *edge.tolist())).hexdigest() == EDGE_SHA

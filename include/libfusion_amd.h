@@ -85,7 +85,9 @@ typedef struct lf_map_params {
 /* Storage precision of a map and of the images and tracking of a System.
  *   FP32  the reference's method: int16 TSDF, float32 weights, float32 vertex /
  *         normal / intensity / gradient maps.
- *   MIXED FP16 TSDF with 8-bit weight codes (lf_weight_code); FP16 normal,
+ *   MIXED FP16 TSDF with 8-bit weight codes (lf_weight_code, stored with
+ *         stochastic rounding: lf_weight_code_dither) packed with the colour
+ *         into one word per voxel (6 bytes a voxel); FP16 normal,
  *         intensity and gradient maps (vertex maps stay float4: FP16's 2 mm
  *         step at 2-4 m biased the ICP fit); the same FP32 normal-equation
  *         sums (pairwise: per-lane, wavefront tree, fixed-order final sum) and
@@ -105,9 +107,12 @@ typedef struct lf_map_params {
  *                            Mixed: FP16 TSDF in truncation units; 0xFFFF (a NaN)
  *                            marks a voxel whose weight is 0
  *   weight[512*num_blocks]   FP32 precision: float32 fusion weight, clamped to 255.
- *                            Mixed: 512*num_blocks bytes of 8-bit weight codes
- *                            (lf_weight_value), the weight clamped to 255 first
- *   rgb[3*512*num_blocks]    uint8 colour, planar per block: R[512] G[512] B[512]
+ *                            Mixed: one 32-bit word per voxel, the 8-bit weight
+ *                            code (lf_weight_value; the weight clamped to 255
+ *                            first) in bits 0-7 and the colour R, G, B in bits
+ *                            8-15, 16-23, 24-31, so weight and colour are one access
+ *   rgb[3*512*num_blocks]    FP32 precision: uint8 colour, planar per block:
+ *                            R[512] G[512] B[512].  Mixed: unused (may be null)
  *   visible[num_blocks]      pool block indices visible in the last update
  *   counters[LF_NUM_COUNTERS]
  *   occupancy[LF_OCC_WORDS]  hashed occupancy bits of 4x4x4-block regions: bit

@@ -228,16 +228,16 @@ def read_tsdf(params, storage):
     pos = torch.empty(4 * max(n, 1), dtype=torch.int32, device="cuda")
     sdf = torch.empty(512 * max(n, 1), dtype=torch.int16, device="cuda")
     mixed = params.precision == PRECISION_MIXED
-    w = torch.empty(512 * max(n, 1), dtype=torch.uint8 if mixed else torch.float32, device="cuda")
+    w = torch.empty(512 * max(n, 1), dtype=torch.int32 if mixed else torch.float32, device="cuda")
     if n:
         copy_device(pos, storage.block_pos, 16 * n)
         copy_device(sdf, storage.sdf, 1024 * n)
-        copy_device(w, storage.weight, 512 * n * (1 if mixed else 4))
+        copy_device(w, storage.weight, 512 * n * 4)
     pos = pos.cpu().numpy()[: 4 * n].reshape(n, 4)[:, :3]
     sdf = sdf.cpu().numpy()[: 512 * n].reshape(n, 512)
     w = w.cpu().numpy()[: 512 * n].reshape(n, 512)
     if mixed:
-        weight = weight_values()[w]
+        weight = weight_values()[w.view(np.uint32) & 0xFF]
         tsdf = sdf.view(np.float16).astype(np.float32)
     else:
         weight = w.astype(np.float32)
@@ -434,9 +434,12 @@ class Map:
             "slot_flag": torch.empty(E, dtype=torch.uint8, device=device),
             "block_pos": torch.empty(B * 4, dtype=torch.int32, device=device),
             "sdf": torch.empty(B * 512, dtype=torch.int16, device=device),
-            "weight": torch.empty(B * 512, dtype=torch.uint8 if params.precision == PRECISION_MIXED
+            # FP32: float32 weights; mixed: one word per voxel, weight code | R<<8 | G<<16 | B<<24
+            "weight": torch.empty(B * 512, dtype=torch.int32 if params.precision == PRECISION_MIXED
                                   else torch.float32, device=device),
-            "rgb": torch.empty(B * 1536, dtype=torch.uint8, device=device),
+            # colour planes of an FP32 map (a mixed map packs colour into its weight words)
+            "rgb": torch.empty(B * 1536 if params.precision != PRECISION_MIXED else 0, dtype=torch.uint8,
+                               device=device),
             "visible": torch.empty(B, dtype=torch.int32, device=device),
             "counters": torch.zeros(16, dtype=torch.int32, device=device),
             "zrange": torch.empty(2 * tx * ty, dtype=torch.float32, device=device),
@@ -446,7 +449,7 @@ class Map:
         }
         s = LfMapStorage()
         for k, v in self.t.items():
-            setattr(s, k, v.data_ptr())
+            setattr(s, k, v.data_ptr() if v.numel() else 0)
         s.tiles_x, s.tiles_y = tx, ty
         self.storage = s
         self.reset()
@@ -544,13 +547,24 @@ class Map:
     def voxels(self):
         """(blocks[n,3], sdf int16[n,512], weight[n,512], rgb uint8[n,3,512]) in pool order.
         The sdf of a voxel with weight 0 is undefined on the device (it holds the
-        unseen code SDF_UNSEEN); it is returned as 0."""
+        unseen code SDF_UNSEEN); it is returned as 0.  FP32 maps only (read_tsdf
+        reads either precision)."""
+        assert self.params.precision != PRECISION_MIXED, "voxels() reads FP32 maps; use read_tsdf"
         c = self.counters()
         n = min(int(c[CNT_ALLOCATED]), self.params.num_blocks)
         sdf = self.t["sdf"][: 512 * n].view(n, 512).cpu().numpy()
         sdf = np.where(sdf == SDF_UNSEEN, np.int16(0), sdf)
         return (self.blocks(), sdf, self.t["weight"][: 512 * n].view(n, 512).cpu().numpy(),
                 self.t["rgb"][: 1536 * n].view(n, 3, 512).cpu().numpy())
+
+    def colours(self):
+        """uint8[n,3,512] colour of the allocated blocks in pool order, for either precision."""
+        c = self.counters()
+        n = min(int(c[CNT_ALLOCATED]), self.params.num_blocks)
+        if self.params.precision == PRECISION_MIXED:
+            words = self.t["weight"][: 512 * n].view(n, 512).cpu().numpy().view(np.uint32)
+            return np.stack([(words >> (8 + 8 * k)) & 0xFF for k in range(3)], 1).astype(np.uint8)
+        return self.t["rgb"][: 1536 * n].view(n, 3, 512).cpu().numpy()
 
 
 def system_config(K, params, num_pyr, iterations, depth_scale, integrate_colour=True):

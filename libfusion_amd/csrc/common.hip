@@ -75,17 +75,21 @@ static int check_params(const lf_map_params *p) {
     return LF_OK;
 }
 
-// Bytes of one voxel's weight: float32, or an 8-bit code in mixed precision.
-size_t lf_weight_bytes(const lf_map_params *p) { return p->precision == LF_PRECISION_MIXED ? 1 : sizeof(float); }
+// Bytes of one voxel's weight word: float32, or in mixed precision the 8-bit
+// weight code with the voxel's colour packed beside it (32 bits).
+size_t lf_weight_bytes(const lf_map_params *p) { return sizeof(float); }
 
 // Every array of a map storage present and the tile grid non-empty; entry
 // points call this before any launch, so storage filled in by hand with an
-// array missing is an error rather than a kernel writing through null.
-int lf_check_storage(const lf_map_storage *s) {
+// array missing is an error rather than a kernel writing through null.  The
+// colour planes (rgb) are required only of an FP32 map (p given): a mixed
+// map packs its colour into the weight words.
+int lf_check_storage(const lf_map_storage *s, const lf_map_params *p) {
     if (!s) return lf_set_error(LF_ERR_ARG, "null map storage");
+    const bool need_rgb = p && p->precision != LF_PRECISION_MIXED;
     const struct { const void *ptr; const char *name; } arrays[] = {
         {s->keys, "keys"}, {s->slot_block, "slot_block"}, {s->slot_flag, "slot_flag"},
-        {s->block_pos, "block_pos"}, {s->sdf, "sdf"}, {s->weight, "weight"}, {s->rgb, "rgb"},
+        {s->block_pos, "block_pos"}, {s->sdf, "sdf"}, {s->weight, "weight"}, {need_rgb ? static_cast<const void *>(s->rgb) : static_cast<const void *>(s), "rgb"},
         {s->visible, "visible"}, {s->counters, "counters"}, {s->zrange, "zrange"},
         {s->occupancy, "occupancy"}, {s->visible_pos, "visible_pos"}, {s->block_dir, "block_dir"}};
     for (const auto &a : arrays)
@@ -110,7 +114,7 @@ extern "C" int lf_map_create(const lf_map_params *p, int width, int height, lf_m
     LF_HIP_CHECK(hipMalloc(&s->block_pos, B * 4 * sizeof(int)));
     LF_HIP_CHECK(hipMalloc(&s->sdf, B * LF_BLOCK3 * sizeof(short)));
     LF_HIP_CHECK(hipMalloc(&s->weight, B * LF_BLOCK3 * lf_weight_bytes(p)));
-    LF_HIP_CHECK(hipMalloc(&s->rgb, B * LF_BLOCK3 * 3));
+    if (p->precision != LF_PRECISION_MIXED) LF_HIP_CHECK(hipMalloc(&s->rgb, B * LF_BLOCK3 * 3));
     LF_HIP_CHECK(hipMalloc(&s->visible, B * sizeof(int)));
     LF_HIP_CHECK(hipMalloc(&s->counters, LF_NUM_COUNTERS * sizeof(unsigned int)));
     LF_HIP_CHECK(hipMalloc(&s->zrange, 2 * (size_t)s->tiles_x * s->tiles_y * sizeof(float)));
@@ -152,7 +156,7 @@ __global__ void k_fill_u128(uint4 *p, size_t n, unsigned int v) {
 extern "C" int lf_map_reset(const lf_map_params *p, const lf_map_storage *s, void *stream) {
     int rc = check_params(p);
     if (rc) return rc;
-    if ((rc = lf_check_storage(s))) return rc;
+    if ((rc = lf_check_storage(s, p))) return rc;
     hipStream_t st = lf_stream(stream);
     const int E = p->num_entries;
     k_reset_table<<<std::min(4096, (E + 255) / 256), 256, 0, st>>>(s->keys, s->slot_block, s->slot_flag, E);
@@ -162,16 +166,17 @@ extern "C" int lf_map_reset(const lf_map_params *p, const lf_map_storage *s, voi
     k_zero_u32<<<LF_OCC_WORDS / 256, 256, 0, st>>>(s->occupancy, LF_OCC_WORDS);
     LF_LAUNCH_CHECK();
     const size_t B = (size_t)p->num_blocks;
-    // sdf (2 B, every voxel starts unseen), weight (4 B or 1 B, 0), rgb (3 B, 0)
-    // per voxel, and the block directory (every cell -1); every array is a
-    // multiple of 512 bytes.
+    // sdf (2 B, every voxel starts unseen), weight word (4 B, 0: weight 0 and,
+    // in a mixed map, colour 0), rgb (3 B, 0; FP32 maps) per voxel, and the
+    // block directory (every cell -1); every array is a multiple of 512 bytes.
     const bool mixed = p->precision == LF_PRECISION_MIXED;
     const unsigned int unseen2 = mixed ? 0xFFFFFFFFu   // two FP16 kHalfUnseen codes
                                        : 0x80008000u;  // two int16 kSdfUnseen codes
     struct { void *ptr; size_t bytes; unsigned int v; } arrs[] = {
-        {s->sdf, B * 1024, unseen2}, {s->weight, B * LF_BLOCK3 * lf_weight_bytes(p), 0u}, {s->rgb, B * 1536, 0u},
-        {s->block_dir, (size_t)LF_DIR_CELLS * sizeof(int), 0xFFFFFFFFu}};
+        {s->sdf, B * 1024, unseen2}, {s->weight, B * LF_BLOCK3 * lf_weight_bytes(p), 0u},
+        {mixed ? nullptr : s->rgb, B * 1536, 0u}, {s->block_dir, (size_t)LF_DIR_CELLS * sizeof(int), 0xFFFFFFFFu}};
     for (auto &a : arrs) {
+        if (!a.ptr) continue;
         size_t n16 = a.bytes / 16;
         int grid = (int)std::min<size_t>(8192, (n16 + 255) / 256);
         k_fill_u128<<<grid, 256, 0, st>>>(reinterpret_cast<uint4 *>(a.ptr), n16, a.v);
@@ -190,7 +195,7 @@ __global__ void k_dir_fill(const int4 *block_pos, const unsigned int *counters, 
 }
 
 int lf_rebuild_dir(const lf_map_params *p, const lf_map_storage *s, hipStream_t st) {
-    if (int rc = lf_check_storage(s)) return rc;
+    if (int rc = lf_check_storage(s, p)) return rc;
     const size_t n16 = (size_t)LF_DIR_CELLS * sizeof(int) / 16;
     k_fill_u128<<<(int)std::min<size_t>(8192, (n16 + 255) / 256), 256, 0, st>>>(
         reinterpret_cast<uint4 *>(s->block_dir), n16, 0xFFFFFFFFu);

@@ -124,6 +124,15 @@ LF_HD unsigned int weight_code(float w) {
     return !(w > 0.0f) ? 0u : w < 0.015625f ? small : big;
 }
 
+// A mixed map's voxel word (its weight array): the 8-bit weight code in bits
+// 0-7 and the colour R, G, B in bits 8-15, 16-23 and 24-31, so a voxel's
+// weight and colour are one 4-byte access (6 bytes per voxel with the FP16 sdf).
+LF_HD unsigned int word_code(unsigned int word) { return word & 0xFFu; }
+LF_HD unsigned int word_channel(unsigned int word, int k) { return (word >> (8 + 8 * k)) & 0xFFu; }
+LF_HD unsigned int word_colour_bits(unsigned int r, unsigned int g, unsigned int b) {
+    return (r << 8) | (g << 16) | (b << 24);
+}
+
 // The same code rounded stochastically: up to the next code with probability
 // equal to w's fractional position between its two neighbouring codes, so
 // the stored weight equals w in expectation.  rnd supplies the random bits
@@ -216,9 +225,11 @@ LF_DEV T *global_ptr(T *p) {
 
 // Error reporting (common.hip).
 int lf_set_error(int code, const char *fmt, ...);
-// LF_OK when every array of the storage is present (common.hip).
-int lf_check_storage(const lf_map_storage *s);
-// Bytes of one voxel's weight for the map's precision (common.hip).
+// LF_OK when every array of the storage is present (common.hip); the colour
+// planes are required only when p is given and the map is FP32.
+int lf_check_storage(const lf_map_storage *s, const lf_map_params *p = nullptr);
+// Bytes of one voxel's weight word (common.hip): float32, or a mixed map's
+// 8-bit weight code with its colour packed beside it.
 size_t lf_weight_bytes(const lf_map_params *p);
 // Block directory rebuilt from the pool (clear, then every allocated block's
 // cell), for storage whose pool was written directly (common.hip).

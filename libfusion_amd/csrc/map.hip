@@ -369,7 +369,7 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
         const size_t b = (size_t)p.w;
         short *sdf_b = m.sdf + b * LF_BLOCK3;
         float *w_b = m.weight + b * LF_BLOCK3;
-        unsigned char *w8_b = reinterpret_cast<unsigned char *>(m.weight) + b * LF_BLOCK3;
+        unsigned int *wd_b = reinterpret_cast<unsigned int *>(m.weight) + b * LF_BLOCK3;  // mixed: voxel words
         unsigned char *c_b = m.rgb + b * (3 * LF_BLOCK3);
         bool ok[2];
         unsigned int pix[2];
@@ -428,7 +428,7 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
             }
         }
         float sdf[2], wraw[2];
-        unsigned int wcode[2], vo[2];
+        unsigned int word[2], vo[2];
         short sp[2];
         unsigned char cold[2][3];
 #pragma unroll
@@ -444,20 +444,20 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
             vo[h] = (unsigned int)(t + 256 * h);
             const unsigned int o = in ? vo[h] : 0u;
             if constexpr (kMixed) {
-                wcode[h] = at_byte(w8_b, o);
+                word[h] = at_byte(wd_b, 4u * o);  // weight code and colour together
             } else {
                 wraw[h] = at_byte(w_b, 4u * o);
+                if (kColour) {
+                    cold[h][0] = at_byte(c_b, o);
+                    cold[h][1] = at_byte(c_b, o + LF_BLOCK3);
+                    cold[h][2] = at_byte(c_b, o + 2 * LF_BLOCK3);
+                }
             }
             sp[h] = at_byte(sdf_b, 2u * o);
-            if (kColour) {
-                cold[h][0] = at_byte(c_b, o);
-                cold[h][1] = at_byte(c_b, o + LF_BLOCK3);
-                cold[h][2] = at_byte(c_b, o + 2 * LF_BLOCK3);
-            }
         }
         float wp[2];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) wp[h] = kMixed ? weight_value(wcode[h]) : wraw[h];
+        for (int h = 0; h < 2; ++h) wp[h] = kMixed ? weight_value(word_code(word[h])) : wraw[h];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             if (!ok[h]) continue;
@@ -475,30 +475,32 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
                 unsigned short *sdf16 = reinterpret_cast<unsigned short *>(sdf_b);
                 // the stored weight is rounded stochastically (unbiased running sums)
                 const unsigned int rnd = voxel_rand(p.x, p.y, p.z, o, pass);
+                // without colour fusion the word's colour bits are kept
+                unsigned int colour = word[h] & ~0xFFu;
                 if (wp[h] == 0.0f) {
                     // a first observation with a positive weight is seen, as in
                     // the reference (the smallest code stands for weights below
                     // it); a zero weight leaves the voxel unseen
                     const unsigned int code = wt > 0.0f ? max(1u, weight_code_sr(fminf(wt, 255.0f), rnd)) : 0u;
                     at_byte(sdf16, 2u * o) = code == 0u ? kHalfUnseen : float_to_half_bits(sdf_h);
-                    at_byte(w8_b, o) = (unsigned char)code;
-                    if (kColour) {
-                        at_byte(c_b, o) = cin[0]; at_byte(c_b, o + LF_BLOCK3) = cin[1];
-                        at_byte(c_b, o + 2 * LF_BLOCK3) = cin[2];
-                    }
+                    if (kColour) colour = word_colour_bits(cin[0], cin[1], cin[2]);
+                    at_byte(wd_b, 4u * o) = code | colour;
                     continue;
                 }
                 const float inv = __builtin_amdgcn_rcpf(wp[h] + wt);
                 const float spf = half_bits_to_float((unsigned short)sp[h]);
                 at_byte(sdf16, 2u * o) = float_to_half_bits((spf * wp[h] + sdf_h * wt) * inv);
-                at_byte(w8_b, o) = (unsigned char)weight_code_sr(fminf(wp[h] + wt, 255.0f), rnd);
+                const unsigned int code = weight_code_sr(fminf(wp[h] + wt, 255.0f), rnd);
                 if (kColour) {
+                    unsigned int ch[3];
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        const float mc = ((float)cold[h][k] * wp[h] + (float)cin[k] * wt) * inv;
-                        at_byte(c_b, o + k * LF_BLOCK3) = (unsigned char)(int)fminf(mc, 255.0f);
+                        const float mc = ((float)word_channel(word[h], k) * wp[h] + (float)cin[k] * wt) * inv;
+                        ch[k] = (unsigned int)(int)fminf(mc, 255.0f);
                     }
+                    colour = word_colour_bits(ch[0], ch[1], ch[2]);
                 }
+                at_byte(wd_b, 4u * o) = code | colour;
                 continue;
             }
             if (wp[h] == 0.0f) {
@@ -641,10 +643,11 @@ __global__ __launch_bounds__(256) void k_deintegrate(MapDev m, const float *__re
             sdf = fminf(1.0f, sdf * inv_trunc);
             const size_t vi = (size_t)b * LF_BLOCK3 + (size_t)(t + 256 * h);
             unsigned short *sdf16 = reinterpret_cast<unsigned short *>(m.sdf);
-            unsigned char *w8 = reinterpret_cast<unsigned char *>(m.weight);
-            const float w0 = kMixed ? weight_value(w8[vi]) : m.weight[vi];
+            unsigned int *wd = reinterpret_cast<unsigned int *>(m.weight);  // mixed: voxel words
+            const unsigned int word = kMixed ? wd[vi] : 0u;
+            const float w0 = kMixed ? weight_value(word_code(word)) : m.weight[vi];
             if (w0 == 0.0f) continue;  // nothing fused here
-            unsigned char *c = m.rgb + (size_t)b * (3 * LF_BLOCK3) + (size_t)(t + 256 * h);
+            unsigned char *c = kMixed ? nullptr : m.rgb + (size_t)b * (3 * LF_BLOCK3) + (size_t)(t + 256 * h);
             const float wr = w0 - wt;
             // mixed: the stored w0 is within one code step (at most 1/16 of
             // it) of the exact sum, so a remainder below that is the rounding
@@ -657,28 +660,38 @@ __global__ __launch_bounds__(256) void k_deintegrate(MapDev m, const float *__re
             if (!keep) {
                 if (kMixed) {
                     sdf16[vi] = kHalfUnseen;
-                    w8[vi] = 0;
+                    wd[vi] = kColour ? 0u : word & ~0xFFu;  // weight 0; colour cleared with colour fusion
                 } else {
                     m.sdf[vi] = kSdfUnseen;
                     m.weight[vi] = 0.0f;
+                    if (kColour) { c[0] = 0; c[LF_BLOCK3] = 0; c[2 * LF_BLOCK3] = 0; }
                 }
-                if (kColour) { c[0] = 0; c[LF_BLOCK3] = 0; c[2 * LF_BLOCK3] = 0; }
                 continue;
             }
             if (kMixed) {
                 const float sr = (half_bits_to_float(sdf16[vi]) * w0 - sdf * wt) / wr;
                 sdf16[vi] = float_to_half_bits(fmaxf(-1.0f, fminf(1.0f, sr)));
-                w8[vi] = (unsigned char)wr_code;
+                unsigned int colour = word & ~0xFFu;
+                if (kColour) {
+                    unsigned int ch[3];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const float cr = ((float)word_channel(word, k) * w0 - (float)cin[k] * wt) / wr;
+                        ch[k] = (unsigned int)(int)fmaxf(0.0f, fminf(255.0f, cr));
+                    }
+                    colour = word_colour_bits(ch[0], ch[1], ch[2]);
+                }
+                wd[vi] = wr_code | colour;
             } else {
                 const float sr = (unpack_sdf(m.sdf[vi]) * w0 - sdf * wt) / wr;
                 m.sdf[vi] = pack_sdf(fmaxf(-1.0f, fminf(1.0f, sr)));
                 m.weight[vi] = wr;
-            }
-            if (kColour) {
+                if (kColour) {
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const float cr = ((float)c[k * LF_BLOCK3] * w0 - (float)cin[k] * wt) / wr;
-                    c[k * LF_BLOCK3] = (unsigned char)(int)fmaxf(0.0f, fminf(255.0f, cr));
+                    for (int k = 0; k < 3; ++k) {
+                        const float cr = ((float)c[k * LF_BLOCK3] * w0 - (float)cin[k] * wt) / wr;
+                        c[k * LF_BLOCK3] = (unsigned char)(int)fmaxf(0.0f, fminf(255.0f, cr));
+                    }
                 }
             }
         }
@@ -701,7 +714,7 @@ int check_update_args(const lf_map_params *p, const lf_map_storage *s, const flo
 int lf_update_ev(const lf_map_params *p, const lf_map_storage *s, const float *depth, const float *nmap,
                  const unsigned char *rgb, int cols, int rows, const float *pose24, const lf_intrinsics *K,
                  hipStream_t st, bool weighted, hipEvent_t e0, hipEvent_t e1, const float *samples) {
-    if (int rc = lf_check_storage(s)) return rc;
+    if (int rc = lf_check_storage(s, p)) return rc;
     const MapDev m = make_dev(p, s);
     dim3 g((cols + 15) / 16, (rows + 15) / 16);
     const bool mixed = p->precision == LF_PRECISION_MIXED;
@@ -744,7 +757,7 @@ int lf_update_ev(const lf_map_params *p, const lf_map_storage *s, const float *d
 int lf_deintegrate_ev(const lf_map_params *p, const lf_map_storage *s, const float *depth, const float *nmap,
                       const unsigned char *rgb, int cols, int rows, const float *pose24, const lf_intrinsics *K,
                       hipStream_t st, const float *samples) {
-    if (int rc = lf_check_storage(s)) return rc;
+    if (int rc = lf_check_storage(s, p)) return rc;
     const MapDev m = make_dev(p, s);
     dim3 g((cols + 15) / 16, (rows + 15) / 16);
     const bool mixed = p->precision == LF_PRECISION_MIXED;
@@ -789,7 +802,7 @@ int lf_update_batch_make(const LfFuseItem *items, int n, void *jobs_host, LfFuse
         if (!it.enabled) continue;  // tiles 0: every kernel skips this map
         if (!it.p || !it.s || !it.depth || !it.samples || !it.pose24 || !it.K || it.cols <= 0 || it.rows <= 0)
             return lf_set_error(LF_ERR_ARG, "bad update batch item %d", i);
-        if (int rc = lf_check_storage(it.s)) return rc;
+        if (int rc = lf_check_storage(it.s, it.p)) return rc;
         const int mixed = it.p->precision == LF_PRECISION_MIXED;
         if (d.any && mixed != d.mixed)
             return lf_set_error(LF_ERR_ARG, "update batch item %d: maps of one batch share one precision", i);
@@ -855,7 +868,7 @@ __global__ void k_reset_visible(MapDev m) {
 
 int lf_select_visible_ev(const lf_map_params *p, const lf_map_storage *s, const float *pose24, const lf_intrinsics *K,
                          int cols, int rows, hipStream_t st) {
-    if (int rc = lf_check_storage(s)) return rc;
+    if (int rc = lf_check_storage(s, p)) return rc;
     const MapDev m = make_dev(p, s);
     k_reset_visible<<<1, 64, 0, st>>>(m);
     LF_LAUNCH_CHECK();

@@ -71,22 +71,27 @@ int download(const lf_map_params *p, const lf_map_storage *s, hipStream_t st, Ho
     h.weight.resize(n * LF_BLOCK3);
     h.rgb.resize(3 * n * LF_BLOCK3);
     const bool mixed = p->precision == LF_PRECISION_MIXED;
-    std::vector<unsigned char> codes(mixed ? n * LF_BLOCK3 : 0);
+    std::vector<unsigned int> words(mixed ? n * LF_BLOCK3 : 0);
     if (n) {
         LF_HIP_CHECK(hipMemcpyAsync(h.pos.data(), s->block_pos, 4 * n * sizeof(int), hipMemcpyDeviceToHost, st));
         LF_HIP_CHECK(hipMemcpyAsync(h.sdf.data(), s->sdf, n * LF_BLOCK3 * sizeof(short), hipMemcpyDeviceToHost, st));
-        if (mixed)
-            LF_HIP_CHECK(hipMemcpyAsync(codes.data(), s->weight, n * LF_BLOCK3, hipMemcpyDeviceToHost, st));
-        else
+        if (mixed) {
+            LF_HIP_CHECK(hipMemcpyAsync(words.data(), s->weight, n * LF_BLOCK3 * sizeof(unsigned int),
+                                        hipMemcpyDeviceToHost, st));
+        } else {
             LF_HIP_CHECK(hipMemcpyAsync(h.weight.data(), s->weight, n * LF_BLOCK3 * sizeof(float), hipMemcpyDeviceToHost, st));
-        LF_HIP_CHECK(hipMemcpyAsync(h.rgb.data(), s->rgb, 3 * n * LF_BLOCK3, hipMemcpyDeviceToHost, st));
+            LF_HIP_CHECK(hipMemcpyAsync(h.rgb.data(), s->rgb, 3 * n * LF_BLOCK3, hipMemcpyDeviceToHost, st));
+        }
     }
     LF_HIP_CHECK(hipStreamSynchronize(st));
-    if (mixed)  // to the FP32 layout: int16 sdf (unseen kept), float weights
+    if (mixed)  // to the FP32 layout: int16 sdf (unseen kept), float weights, colour planes
         for (size_t v = 0; v < n * LF_BLOCK3; ++v) {
-            h.weight[v] = lf::weight_value(codes[v]);
+            h.weight[v] = lf::weight_value(lf::word_code(words[v]));
             const unsigned short hb = (unsigned short)h.sdf[v];
             h.sdf[v] = hb == lf::kHalfUnseen ? lf::kSdfUnseen : lf::pack_sdf(lf::half_bits_to_float(hb));
+            const size_t b = v / LF_BLOCK3, i = v % LF_BLOCK3;
+            for (int k = 0; k < 3; ++k)
+                h.rgb[b * 3 * LF_BLOCK3 + (size_t)k * LF_BLOCK3 + i] = (unsigned char)lf::word_channel(words[v], k);
         }
     return LF_OK;
 }
@@ -103,7 +108,7 @@ struct File {
 // capacities (num_blocks, num_entries, num_buckets), plus "<path>.txt".
 extern "C" int lf_map_write(const lf_map_params *p, const lf_map_storage *s, const char *path, void *stream) {
     if (!p || !s || !path) return lf_set_error(LF_ERR_ARG, "null argument to lf_map_write");
-    if (int rc = lf_check_storage(s)) return rc;
+    if (int rc = lf_check_storage(s, p)) return rc;
     hipStream_t st = lf_stream(stream);
     HostMap h;
     if (int rc = download(p, s, st, h)) return rc;
@@ -169,7 +174,7 @@ extern "C" int lf_map_write(const lf_map_params *p, const lf_map_storage *s, con
 // state); the map is reset and refilled.
 extern "C" int lf_map_read(const lf_map_params *p, const lf_map_storage *s, const char *path, void *stream) {
     if (!p || !s || !path) return lf_set_error(LF_ERR_ARG, "null argument to lf_map_read");
-    if (int rc = lf_check_storage(s)) return rc;
+    if (int rc = lf_check_storage(s, p)) return rc;
     hipStream_t st = lf_stream(stream);
     RefSize sz{p->num_blocks, p->num_entries, p->num_buckets};
     {
@@ -250,22 +255,27 @@ extern "C" int lf_map_read(const lf_map_params *p, const lf_map_storage *s, cons
     LF_HIP_CHECK(hipMemcpyAsync(s->keys, keys.data(), keys.size() * sizeof(unsigned long long), hipMemcpyHostToDevice, st));
     LF_HIP_CHECK(hipMemcpyAsync(s->slot_block, slot_block.data(), slot_block.size() * sizeof(int), hipMemcpyHostToDevice, st));
     LF_HIP_CHECK(hipMemcpyAsync(s->occupancy, occ.data(), occ.size() * sizeof(unsigned int), hipMemcpyHostToDevice, st));
-    std::vector<unsigned char> codes;
-    if (p->precision == LF_PRECISION_MIXED) {  // nearest FP16 sdf and weight code
-        codes.resize(weight.size());
+    std::vector<unsigned int> words;
+    if (p->precision == LF_PRECISION_MIXED) {  // nearest FP16 sdf; weight code and colour in one word
+        words.resize(weight.size());
         for (size_t v = 0; v < weight.size(); ++v) {
-            codes[v] = (unsigned char)lf::weight_code(std::min(weight[v], 255.0f));
-            sdf[v] = (short)(codes[v] == 0 ? lf::kHalfUnseen : lf::float_to_half_bits(lf::unpack_sdf(sdf[v])));
+            const unsigned int code = lf::weight_code(std::min(weight[v], 255.0f));
+            sdf[v] = (short)(code == 0 ? lf::kHalfUnseen : lf::float_to_half_bits(lf::unpack_sdf(sdf[v])));
+            const size_t b = v / LF_BLOCK3, i = v % LF_BLOCK3;
+            const unsigned char *c = &rgb[b * 3 * LF_BLOCK3 + i];
+            words[v] = code | lf::word_colour_bits(c[0], c[LF_BLOCK3], c[2 * LF_BLOCK3]);
         }
     }
     if (n) {
         LF_HIP_CHECK(hipMemcpyAsync(s->block_pos, pos.data(), pos.size() * sizeof(int), hipMemcpyHostToDevice, st));
         LF_HIP_CHECK(hipMemcpyAsync(s->sdf, sdf.data(), sdf.size() * sizeof(short), hipMemcpyHostToDevice, st));
-        if (codes.empty())
+        if (words.empty()) {
             LF_HIP_CHECK(hipMemcpyAsync(s->weight, weight.data(), weight.size() * sizeof(float), hipMemcpyHostToDevice, st));
-        else
-            LF_HIP_CHECK(hipMemcpyAsync(s->weight, codes.data(), codes.size(), hipMemcpyHostToDevice, st));
-        LF_HIP_CHECK(hipMemcpyAsync(s->rgb, rgb.data(), rgb.size(), hipMemcpyHostToDevice, st));
+            LF_HIP_CHECK(hipMemcpyAsync(s->rgb, rgb.data(), rgb.size(), hipMemcpyHostToDevice, st));
+        } else {
+            LF_HIP_CHECK(hipMemcpyAsync(s->weight, words.data(), words.size() * sizeof(unsigned int),
+                                        hipMemcpyHostToDevice, st));
+        }
     }
     const unsigned int allocated = (unsigned int)n;
     LF_HIP_CHECK(hipMemcpyAsync(s->counters + LF_CNT_ALLOCATED, &allocated, sizeof(allocated), hipMemcpyHostToDevice, st));
@@ -278,8 +288,8 @@ extern "C" int lf_map_read(const lf_map_params *p, const lf_map_storage *s, cons
 extern "C" int lf_map_copy(const lf_map_params *p, const lf_map_storage *src, const lf_map_storage *dst,
                            void *stream) {
     if (!p || !src || !dst) return lf_set_error(LF_ERR_ARG, "null argument to lf_map_copy");
-    if (int rc = lf_check_storage(src)) return rc;
-    if (int rc = lf_check_storage(dst)) return rc;
+    if (int rc = lf_check_storage(src, p)) return rc;
+    if (int rc = lf_check_storage(dst, p)) return rc;
     if (src->tiles_x != dst->tiles_x || src->tiles_y != dst->tiles_y)
         return lf_set_error(LF_ERR_ARG, "maps made for different raycast sizes");
     hipStream_t st = lf_stream(stream);
@@ -291,13 +301,14 @@ extern "C" int lf_map_copy(const lf_map_params *p, const lf_map_storage *src, co
         {dst->block_pos, src->block_pos, 4 * B * sizeof(int)},
         {dst->sdf, src->sdf, B * LF_BLOCK3 * sizeof(short)},
         {dst->weight, src->weight, B * LF_BLOCK3 * lf_weight_bytes(p)},
-        {dst->rgb, src->rgb, 3 * B * LF_BLOCK3},
+        {dst->rgb, src->rgb, p->precision == LF_PRECISION_MIXED ? 0 : 3 * B * LF_BLOCK3},  // mixed: in the words
         {dst->visible, src->visible, B * sizeof(int)},
         {dst->visible_pos, src->visible_pos, B * 4 * sizeof(int)},
         {dst->block_dir, src->block_dir, (size_t)LF_DIR_CELLS * sizeof(int)},
         {dst->counters, src->counters, LF_NUM_COUNTERS * sizeof(unsigned int)},
         {dst->occupancy, src->occupancy, LF_OCC_WORDS * sizeof(unsigned int)},
     };
-    for (auto &c : parts) LF_HIP_CHECK(hipMemcpyAsync(c.d, c.s, c.n, hipMemcpyDeviceToDevice, st));
+    for (auto &c : parts)
+        if (c.n) LF_HIP_CHECK(hipMemcpyAsync(c.d, c.s, c.n, hipMemcpyDeviceToDevice, st));
     return LF_OK;
 }

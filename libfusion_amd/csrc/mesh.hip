@@ -78,14 +78,19 @@ __global__ __launch_bounds__(256) void k_mesh(MeshDev m, float *__restrict__ vtx
             if (nb >= 0) {
                 const int l = lidx(x, y, z);
                 const size_t vi = (size_t)nb * LF_BLOCK3 + l;
-                const float w = kMixed ? weight_value(reinterpret_cast<const unsigned char *>(m.weight)[vi])
-                                       : m.weight[vi];
+                // mixed: the weight code and the colour share one voxel word
+                const unsigned int word = kMixed ? reinterpret_cast<const unsigned int *>(m.weight)[vi] : 0u;
+                const float w = kMixed ? weight_value(word_code(word)) : m.weight[vi];
                 ok = kMode == 2 ? ((double)w > 1e-3) : (w != 0.0f);
                 s = kMixed ? half_bits_to_float(reinterpret_cast<const unsigned short *>(m.sdf)[vi])
                            : unpack_sdf(m.sdf[vi]);
                 if (kMode == 2) {
-                    const unsigned char *c = m.rgb + (size_t)nb * (3 * LF_BLOCK3) + l;
-                    s_col[3 * i] = c[0]; s_col[3 * i + 1] = c[LF_BLOCK3]; s_col[3 * i + 2] = c[2 * LF_BLOCK3];
+                    if (kMixed) {
+                        for (int k = 0; k < 3; ++k) s_col[3 * i + k] = (unsigned char)word_channel(word, k);
+                    } else {
+                        const unsigned char *c = m.rgb + (size_t)nb * (3 * LF_BLOCK3) + l;
+                        s_col[3 * i] = c[0]; s_col[3 * i + 1] = c[LF_BLOCK3]; s_col[3 * i + 2] = c[2 * LF_BLOCK3];
+                    }
                 }
             }
             s_ok[i] = ok;
@@ -201,7 +206,7 @@ extern "C" int lf_create_mesh(const lf_map_params *p, const lf_map_storage *s, f
     if (!p || !s || !triangles_out) return lf_set_error(LF_ERR_ARG, "null argument to create_mesh");
     if (max_triangles < 0) return lf_set_error(LF_ERR_ARG, "negative triangle capacity");
     if (normal_out && colour_out) return lf_set_error(LF_ERR_ARG, "ask for normals or colours, not both");
-    if (int rc = lf_check_storage(s)) return rc;
+    if (int rc = lf_check_storage(s, p)) return rc;
     hipStream_t st = lf_stream(stream);
     MeshDev m;
     m.keys = s->keys; m.slot_block = s->slot_block; m.block_pos = s->block_pos; m.sdf = s->sdf;

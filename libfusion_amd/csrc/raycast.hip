@@ -458,9 +458,16 @@ LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char 
             int vi = -1;
             read_sdf<kMixed>(r, bc, res, cv, &vi);
             if (!cv) break;
-            const int b = vi / LF_BLOCK3, l = vi % LF_BLOCK3;
-            const unsigned char *c = r.rgb + (size_t)b * (3 * LF_BLOCK3) + l;
-            col[0] = c[0]; col[1] = c[LF_BLOCK3]; col[2] = c[2 * LF_BLOCK3];
+            if constexpr (kMixed) {  // colour packed beside the weight code
+                const unsigned int word = reinterpret_cast<const unsigned int *>(r.weight)[vi];
+                col[0] = (unsigned char)word_channel(word, 0);
+                col[1] = (unsigned char)word_channel(word, 1);
+                col[2] = (unsigned char)word_channel(word, 2);
+            } else {
+                const int b = vi / LF_BLOCK3, l = vi % LF_BLOCK3;
+                const unsigned char *c = r.rgb + (size_t)b * (3 * LF_BLOCK3) + l;
+                col[0] = c[0]; col[1] = c[LF_BLOCK3]; col[2] = c[2 * LF_BLOCK3];
+            }
         }
         const float *Tcw = pose24 + 12;
         const f3 q = xform(Tcw, mul(res, r.voxel_size));
@@ -534,7 +541,7 @@ int lf_raycast_ev(const lf_map_params *p, const lf_map_storage *s, float *vmap, 
                   int rows, const float *pose24, const lf_intrinsics *K, int check_colour, hipStream_t st,
                   hipEvent_t e0, hipEvent_t e1) {
     if (!p || !s || !vmap || !pose24 || !K) return lf_set_error(LF_ERR_ARG, "null argument to raycast");
-    if (int rc = lf_check_storage(s)) return rc;
+    if (int rc = lf_check_storage(s, p)) return rc;
     if (cols / 8 != s->tiles_x || rows / 8 != s->tiles_y)
         return lf_set_error(LF_ERR_ARG, "raycast size %dx%d does not match the map's %dx%d tiles", cols, rows,
                             s->tiles_x, s->tiles_y);
@@ -591,7 +598,7 @@ int lf_raycast_batch_make(const LfRayItem *items, int n, void *jobs_host, LfRayB
             return lf_set_error(LF_ERR_ARG, "null argument in raycast batch item %d", i);
         if (it.p->precision != items[0].p->precision)
             return lf_set_error(LF_ERR_ARG, "raycast batch item %d: maps of one batch share one precision", i);
-        if (int rc = lf_check_storage(it.s)) return rc;
+        if (int rc = lf_check_storage(it.s, it.p)) return rc;
         if (it.cols / 8 != it.s->tiles_x || it.rows / 8 != it.s->tiles_y)
             return lf_set_error(LF_ERR_ARG, "batch item %d: raycast size %dx%d does not match the map's %dx%d tiles",
                                 i, it.cols, it.rows, it.s->tiles_x, it.s->tiles_y);

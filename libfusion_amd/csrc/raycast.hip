@@ -256,14 +256,23 @@ LF_DEV unsigned long long lds_tag(int bx, int by, int bz) {
            ((unsigned long long)(bz & 0x1FFF) << 24);
 }
 
-// Resolve the pool block of voxel coordinate v: the lane's last block, then the
-// occupancy bits (a clear bit: the whole 4x4x4-block region is unallocated,
-// which is most of the free space a ray crosses), then one load from the block
-// directory.  Outside the directory: the workgroup's LDS cache of lookups,
-// then the hash table.  (Inside it the directory load is faster than the LDS
-// cache round trip plus the loads of its misses: 250 -> 234 us at C1.)
-LF_DEV int resolve(const RayDev &r, BlockCache &bc, const i3 &v) {
-    const int bx = blk(v.x), by = blk(v.y), bz = blk(v.z);
+// Every sample of a ray lies within a few voxels of the segment between its
+// start and end points, so one check per ray (all coordinates finite and
+// within kRayRange voxels) lets the per-sample code use a plain floor and
+// shift: no NaN or overflow guard per sample (to_i3's), no kBad block.  A ray
+// outside the range has no sample in any representable block (block keys hold
+// 21 bits per axis) and finds nothing.
+constexpr float kRayRange = 1.0e9f;
+LF_DEV int ray_floor(float v) { return (int)floorf(v); }
+
+// Resolve the pool block of block coordinate (bx, by, bz): the lane's last
+// block, then the occupancy bits (a clear bit: the whole 4x4x4-block region
+// is unallocated, which is most of the free space a ray crosses), then one
+// load from the block directory.  Outside the directory: the workgroup's LDS
+// cache of lookups, then the hash table.  (Inside it the directory load is
+// faster than the LDS cache round trip plus the loads of its misses: 250 ->
+// 234 us at C1.)
+LF_DEV int resolve(const RayDev &r, BlockCache &bc, int bx, int by, int bz) {
     if (bx == bc.bx && by == bc.by && bz == bc.bz) return bc.b;
     bc.bx = bx; bc.by = by; bc.bz = bz;
     const unsigned int ob = occupancy_bit(bx, by, bz);
@@ -295,17 +304,17 @@ LF_DEV int resolve(const RayDev &r, BlockCache &bc, const i3 &v) {
     return bc.b;
 }
 
-// findVoxel + read_sdf (ray_tracing.cu:291-305).  A voxel is valid when its
-// weight is non-zero, which the sdf plane encodes (kSdfUnseen; kHalfUnseen, a
-// NaN, in mixed precision), so one 2-byte load per sample decides both
-// validity and value.
+// findVoxel + read_sdf (ray_tracing.cu:291-305) at a sample of an in-range
+// ray.  A voxel is valid when its weight is non-zero, which the sdf plane
+// encodes (kSdfUnseen; kHalfUnseen, a NaN, in mixed precision), so one 2-byte
+// load per sample decides both validity and value.
 template <bool kMixed>
 LF_DEV float read_sdf(const RayDev &r, BlockCache &bc, f3 p, bool &valid, int *voxel_out = nullptr) {
-    const i3 v = to_i3(p);
-    const int b = resolve(r, bc, v);
+    const int vx = ray_floor(p.x), vy = ray_floor(p.y), vz = ray_floor(p.z);
+    const int b = resolve(r, bc, vx >> 3, vy >> 3, vz >> 3);
     valid = false;
     if (b < 0) return __int_as_float(0x7fc00000);
-    const int vi = b * LF_BLOCK3 + lidx(v.x, v.y, v.z);
+    const int vi = b * LF_BLOCK3 + lidx(vx, vy, vz);
     if (voxel_out) *voxel_out = vi;
     if constexpr (kMixed) {
         const float sv = half_bits_to_float(reinterpret_cast<const unsigned short *>(r.sdf)[vi]);
@@ -318,20 +327,24 @@ LF_DEV float read_sdf(const RayDev &r, BlockCache &bc, f3 p, bool &valid, int *v
     return unpack_sdf(sd);
 }
 
-// read_sdf_interped (ray_tracing.cu:307-339).  The 8 corner voxels are
-// resolved first, then all 8 sdf loads are issued in one round trip; the
-// arithmetic is the reference's order.
+// read_sdf_interped (ray_tracing.cu:307-339).  Corner k is the voxel of
+// p + offset_k with offsets 0 or 1 per axis, so each axis has only two
+// coordinates, floor(p) and floor(p + 1), computed once; the 8 corners'
+// blocks are resolved first, then all 8 sdf loads are issued in one round
+// trip; the arithmetic is the reference's order.
 template <bool kMixed>
 LF_DEV float read_interp(const RayDev &r, BlockCache &bc, f3 p, bool &valid) {
-    const f3 xyz = {p.x - floorf(p.x), p.y - floorf(p.y), p.z - floorf(p.z)};
+    const float fx = floorf(p.x), fy = floorf(p.y), fz = floorf(p.z);
+    const f3 xyz = {p.x - fx, p.y - fy, p.z - fz};
+    const int cx[2] = {(int)fx, ray_floor(p.x + 1.0f)}, cy[2] = {(int)fy, ray_floor(p.y + 1.0f)},
+              cz[2] = {(int)fz, ray_floor(p.z + 1.0f)};
     // corner order of the reference: (0,0,0) (1,0,0) (0,1,0) (1,1,0) (0,0,1) (1,0,1) (0,1,1) (1,1,1)
     int vi[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        const f3 q = add(p, f3{(float)(k & 1), (float)((k >> 1) & 1), (float)((k >> 2) & 1)});
-        const i3 v = to_i3(q);
-        const int b = resolve(r, bc, v);
-        vi[k] = b < 0 ? -1 : b * LF_BLOCK3 + lidx(v.x, v.y, v.z);
+        const int x = cx[k & 1], y = cy[(k >> 1) & 1], z = cz[k >> 2];
+        const int b = resolve(r, bc, x >> 3, y >> 3, z >> 3);
+        vi[k] = b < 0 ? -1 : b * LF_BLOCK3 + lidx(x, y, z);
     }
     short c[8];
 #pragma unroll
@@ -407,6 +420,12 @@ LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char 
         const float dist_e = norm(pt) * inv_vs;
         const f3 be = mul(xform(Twc, pt), inv_vs);
         const f3 dir = normalised(sub(be, bs));
+        // the per-ray range check of the sample code (kRayRange); NaN fails it
+        const float reach = fmaxf(fmaxf(fmaxf(fabsf(bs.x), fabsf(bs.y)), fmaxf(fabsf(bs.z), fabsf(be.x))),
+                                  fmaxf(fabsf(be.y), fabsf(be.z)));
+        if (!(reach < kRayRange) || isnan(bs.x + bs.y + bs.z + be.x + be.y + be.z) || isnan(dir.x) || isnan(dir.y) ||
+            isnan(dir.z))
+            break;
         f3 res = bs;
         BlockCache bc = {kBad, kBad, kBad, -1, r.pool_fits_lds ? s_cache : nullptr,
                          reinterpret_cast<const unsigned int *>(s_occ), false};

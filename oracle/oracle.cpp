@@ -489,6 +489,14 @@ template <class R> void raycast(Map<R> &m, R *vmap, uint8_t *rgb_out, int cols, 
             R dist_e = norm(pt) * inv_vs;
             V3<R> be = mul(pose(pt), inv_vs);
             V3<R> dir = normalised(sub(be, bs));
+            // DEVIATION: a ray with a coordinate beyond 1e9 voxels (or NaN) finds
+            // nothing (no representable block lies there); the reference's
+            // ToVector3i is undefined for such values
+            const R reach = std::max({std::fabs(bs.x), std::fabs(bs.y), std::fabs(bs.z), std::fabs(be.x),
+                                      std::fabs(be.y), std::fabs(be.z)});
+            if (!(reach < (R)1e9) || isnan_(bs.x + bs.y + bs.z + be.x + be.y + be.z) || isnan_(dir.x) ||
+                isnan_(dir.y) || isnan_(dir.z))
+                continue;
             V3<R> res = bs;
             bool valid = false, found = false;
             R step;

@@ -343,15 +343,17 @@ def main():
     rl = roofline(ig_bytes, ig_n, ig_ms, "k_integrate_batch<true> (mixed: FP16 sdf + 8-bit weight + 3 x u8 colour, "
                   "%d maps per launch)" % n,
                   {"visible_blocks_per_launch": round(head[0]["vis_blocks_per_step"], 1),
-                   "basis": "per visible block 2*512*6 + 16 B, per pixel 15 B (SURVEY 8d, mixed voxel = 6 B)"})
+                   "basis": "per visible block 2*512*6 + 16 B (mixed voxel: FP16 sdf + 32-bit weight code / colour word), "
+                           "per pixel 15 B"})
     rl["traffic"] = profile_value("integrate_batch_mixed_traffic.json", "hbm_bytes_per_launch")
     stage_rooflines = {}
     rc_n, rc_ms = t[capi.BATCH_RAYCAST]
     touched = profile_value("raycast_batch_touched.json", "blocks_touched_mean_per_map")
     if touched:
-        # compulsory: vertex map out (16 B / pixel), sdf + weight (3 B mixed) of
-        # the distinct blocks the march reads, their 20 B entries, z-range tiles
-        rb = n * (P * 16 + touched * 512 * 3 + touched * 20 + 2 * (P // 64) * 4)
+        # compulsory: vertex map (16 B) and colour (3 B) out and the hit
+        # voxel's colour word (4 B) per pixel, the FP16 sdf plane of the
+        # distinct blocks the march reads and their 20 B entries, z-range tiles
+        rb = n * (P * (16 + 3 + 4) + touched * (512 * 2 + 20) + 2 * (P // 64) * 4)
         stage_rooflines["raycast"] = roofline(rb, rc_n, rc_ms, "k_raycast_batch<true, true> (%d maps per launch)" % n,
                                               {"bound": "latency (dependent sample chain)",
                                                "blocks_touched_per_map": touched})

@@ -189,6 +189,9 @@ public:
     // Planned steps of a tracker that shares its launches with share - 1
     // others take 1/share of the reduction grid (a Batch sets it).
     void set_batch_share(int share) { batch_share_ = share < 1 ? 1 : share; }
+    // wgs > 0: planned steps take exactly that many workgroups (at most a
+    // lone system's grid) instead of 1/share of it.
+    void set_batch_workgroups(int wgs) { batch_wgs_ = wgs < 0 ? 0 : wgs; }
     // compute_transform(ctx) (icp_tracker.cpp:210-314) on the set_* pyramids
     // (source = current frame, reference = last); success is unconditional.
     int compute_transform(const TrackingContext &c);
@@ -208,6 +211,8 @@ public:
     LfTrackState *state() const { return state_; }
     // Synchronising read of the last result (update = estimate^-1).
     TrackingResult result();
+    // Synchronising read of the last Gauss-Newton step's FP64 sums (31 values).
+    int read_sums(double *sums31);
     float icp_weight = 1e6f;  // icp_tracker.cpp:89-90
 
 private:
@@ -218,6 +223,7 @@ private:
     float *scratch_ = nullptr;
     bool context_mode_ = false;
     int batch_share_ = 1;
+    int batch_wgs_ = 0;
     std::unique_ptr<DeviceImage> src_, ref_;
 };
 
@@ -353,9 +359,16 @@ public:
     // {launches, total ms} of a stage (synchronising).
     void set_timing(int stages);
     int read_timing(int stage, int *launches, double *total_ms);
+    // Launch options (LF_BATCH_OPT_*): shapes of the batched launches, which
+    // leave every system's results unchanged up to summation order.
+    int set_option(int option, int value);
+    int get_option(int option, int *value) const;
 
 private:
     int step(const uint16_t *const *depth, const unsigned char *const *rgb, bool from_host);
+    int upload_overlap_ = 1;  // uploads on aux_ beside the previous step (1) or on stream_ (0)
+    int track_wgs_ = 0;       // tracking workgroups per system (0: the share rule)
+    int integrate_wgs_ = 0;   // integration workgroups per map (0: the map-count rule)
     void release();
     std::vector<std::unique_ptr<System>> sys_;
     hipStream_t stream_ = nullptr;

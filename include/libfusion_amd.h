@@ -422,6 +422,14 @@ int lf_tracker_compute_transform_images(lf_tracker *t, lf_image *reference, lf_i
                                         lf_tracking_result *out);
 /* A plane (LF_PLANE_*) of the tracker's source (source != 0) or reference pyramid. */
 int lf_tracker_plane(lf_tracker *t, int source, int level, int plane, void **out);
+/* The normal-equation sums of the tracker's last Gauss-Newton step.  No
+ * reference counterpart: the reference sums JtJ / Jtr per term in FP32 and
+ * combines them on the host (icp_tracker.cpp:50-96); here they are combined
+ * per pixel and accumulated in FP64 on the device, and this read exposes them
+ * for checking.  sums31[0..26]: A = w H_icp + H_rgb and b in create_jtjjtr<6,7>
+ * order (reduce_sum.h:48-61), w = 1e6 (icp_tracker.cpp:89-90); [27] / [28]:
+ * ICP sum r^2 / count; [29] / [30]: RGB sum r^2 / count.  Synchronising. */
+int lf_tracker_get_sums(lf_tracker *t, double *sums31);
 
 /* Scratch for the reductions: lf_reduce_scratch_floats() floats of device memory. */
 size_t lf_reduce_scratch_floats(void);
@@ -540,7 +548,8 @@ int lf_system_read_timing(lf_system *sys, int stage, int *launches, double *tota
  * re-pyramids.  All systems share one precision (LF_ERR_ARG otherwise).  The
  * tracking reductions of n systems take 1/n of a lone system's grid each, so
  * each system's poses, tracking results and volume equal those of the same
- * system run alone up to FP32 summation order (exactly when n is 1).
+ * system run alone up to FP64 summation order in the tracking sums (exactly
+ * when n is 1).
  * Asynchronous; lf_batch_synchronize waits for all. */
 typedef struct lf_batch lf_batch;
 int lf_batch_create(const lf_system_config *cfgs, int n, lf_batch **out);
@@ -568,6 +577,21 @@ int lf_batch_synchronize(lf_batch *b);
 #define LF_BATCH_TIMING_STEP 0x10
 int lf_batch_set_timing(lf_batch *b, int stages);
 int lf_batch_read_timing(lf_batch *b, int stage, int *launches, double *total_ms);
+/* Launch options of a batch (no reference counterpart: they shape this
+ * build's batched launches).  Each system's poses, volume and maps stay those
+ * of the default shapes up to FP64 summation order in the tracking sums.
+ *   LF_BATCH_OPT_UPLOAD_OVERLAP: 1 (default) a step's uploads run on the
+ *     upload stream beside the previous step's fusion and raycast; 0 on the
+ *     batch stream ahead of the step's tracking.
+ *   LF_BATCH_OPT_TRACK_WORKGROUPS: workgroups per system of each Gauss-Newton
+ *     step, 1..512 (0, the default: 512 / n, at most the level's pixels / 512).
+ *   LF_BATCH_OPT_INTEGRATE_WORKGROUPS: workgroups per map of the batched
+ *     integration, 1..65535 (0, the default: max(1024, 8192 / n)). */
+#define LF_BATCH_OPT_UPLOAD_OVERLAP 0
+#define LF_BATCH_OPT_TRACK_WORKGROUPS 1
+#define LF_BATCH_OPT_INTEGRATE_WORKGROUPS 2
+int lf_batch_set_option(lf_batch *b, int option, int value);
+int lf_batch_get_option(lf_batch *b, int option, int *value);
 /* System i's camera pose (4x4 row-major doubles; synchronising), last
  * tracking result and map counters. */
 int lf_batch_get_pose(lf_batch *b, int i, double *out16);

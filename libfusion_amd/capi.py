@@ -29,6 +29,7 @@ TIMING_ALL = 0x17
 # batch stage ids (bit s of the mask brackets stage s)
 BATCH_INTEGRATE, BATCH_RAYCAST, BATCH_TRACKING, BATCH_FUSION, BATCH_STEP = range(5)
 BATCH_TIMING_ALL = 0x1F
+BATCH_OPT_UPLOAD_OVERLAP, BATCH_OPT_TRACK_WORKGROUPS, BATCH_OPT_INTEGRATE_WORKGROUPS = range(3)
 
 
 class LfIntrinsics(ctypes.Structure):
@@ -128,6 +129,7 @@ SIGNATURES = [
     ("lf_tracker_compute_transform", I, [P, P, P, I, P]),
     ("lf_tracker_compute_transform_images", I, [P, P, P, P, P, I, P]),
     ("lf_tracker_plane", I, [P, I, I, I, P]),
+    ("lf_tracker_get_sums", I, [P, P]),
     ("lf_reduce_scratch_floats", ctypes.c_size_t, []),
     ("lf_icp_reduce", I, [P, P, P, P, I, I, P, P, P, P, P]),
     ("lf_icp_reduce_noalloc", I, [P, P, P, P, I, I, P, P, P, P]),
@@ -157,6 +159,8 @@ SIGNATURES = [
     ("lf_batch_process_host", I, [P, P, P]),
     ("lf_batch_synchronize", I, [P]),
     ("lf_batch_set_timing", I, [P, I]),
+    ("lf_batch_set_option", I, [P, I, I]),
+    ("lf_batch_get_option", I, [P, I, P]),
     ("lf_batch_read_timing", I, [P, I, P, P]),
     ("lf_batch_get_pose", I, [P, I, P]),
     ("lf_batch_get_tracking", I, [P, I, P, P]),
@@ -411,6 +415,12 @@ class Tracker:
         check(load().lf_tracker_compute_transform(self.h, ini, its, len(iterations), ctypes.byref(r)))
         return self._result(r)
 
+    def sums(self):
+        """The last Gauss-Newton step's FP64 sums (31 values, lf_tracker_get_sums)."""
+        out = (ctypes.c_double * 31)()
+        check(load().lf_tracker_get_sums(self.h, out))
+        return np.array(out, np.float64)
+
     def compute_transform_images(self, reference, current, iterations, init=None):
         r = LfTrackingResult()
         its = (ctypes.c_int * len(iterations))(*iterations)
@@ -629,6 +639,15 @@ class Batch:
         t = ctypes.c_double(0)
         check(load().lf_batch_read_timing(self.h, int(stage), ctypes.byref(n), ctypes.byref(t)))
         return n.value, t.value
+
+    def set_option(self, option, value):
+        """A launch option (BATCH_OPT_*)."""
+        check(load().lf_batch_set_option(self.h, int(option), int(value)))
+
+    def option(self, option):
+        v = ctypes.c_int(0)
+        check(load().lf_batch_get_option(self.h, int(option), ctypes.byref(v)))
+        return v.value
 
     def pose(self, i):
         T = np.zeros((4, 4))

@@ -23,6 +23,10 @@ struct LfTrackState {
     double pose[12];     // T_world_camera of the current frame
     float pose24[24];    // float T_world_camera and T_camera_world for the mapping kernels
     int pad[3];
+    // the last Gauss-Newton step's FP64 sums: A = w H_icp + H_rgb and b in
+    // create_jtjjtr<6,7> order (0-26), ICP r^2 / count (27 / 28), RGB r^2 /
+    // count (29 / 30)
+    double sums[32];
 };
 
 // Per-level inputs of one tracking iteration.
@@ -74,7 +78,9 @@ int lf_track_set_pose(LfTrackState *st, const double *T12, hipStream_t stream);
 // lf_track_iteration's up to FP32 summation order (exactly, for share 1).
 // The jobs of one batch share one map format (half_maps as in their LfTrackLevel).
 size_t lf_track_job_bytes(void);
+// share: systems sharing the launch (the job's grid is 1/share of a lone
+// system's); wgs > 0 sets the grid instead.
 int lf_track_job_make(const LfTrackLevel &lv, float *scratch, LfTrackState *st, const LfStepFlags &f,
-                      float icp_weight, int share, void *job, int *grid);
+                      float icp_weight, int share, int wgs, void *job, int *grid);
 void lf_track_job_skip(void *job);
 int lf_track_batch_step(const void *jobs, int n, int gmax, int half_maps, hipStream_t stream);

@@ -61,9 +61,77 @@ struct ReduceArgs {
     int last_of_frame;     // tracking step: the frame's last scheduled step
     int context_mode;      // tracking step: compute_transform(ctx) semantics
     double init[12];       // tracking step: initial estimate (first_of_frame)
+    float init_f[12];      // the same in FP32 (the first step's pose, uniform across the grid)
 };
 
-LF_DEV void accumulate(float *s, const float *row);
+// create_jtjjtr<6,7> plus {r^2, count} of one row, summed in FP32: the
+// reference's sums (the stand-alone reductions lf_icp_reduce / lf_rgb_step).
+LF_DEV void accumulate(float *s, const float *row) {
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < 7; ++i)
+#pragma unroll
+        for (int j = i; j < 7; ++j) s[k++] += row[i] * row[j];
+    s[28] += 1.0f;
+}
+
+struct FloatSums {  // the 29 ICP and 29 RGB sums of the stand-alone reductions
+    float si[29], sr[29];
+    LF_DEV void clear() {
+#pragma unroll
+        for (int k = 0; k < 29; ++k) { si[k] = 0.0f; sr[k] = 0.0f; }
+    }
+    LF_DEV void icp(const float *row) { accumulate(si, row); }
+    LF_DEV void rgb(const float *row) { accumulate(sr, row); }
+};
+
+// The tracking step's sums: the normal equations A = w H_icp + H_rgb,
+// b = w g_icp + g_rgb of compute_transform (icp_tracker.cpp:89-96) combined
+// per pixel and accumulated in FP64, plus each term's {r^2, count}.  A
+// product of two FP32 row values is exact in FP64, the ICP weight costs one
+// rounding at 2^-53 and every fused multiply-add one more, so a lane's serial
+// sum and the tree above it stay within about 1e-13 of the exact sum of the
+// FP32 rows (relative to the sum of magnitudes), where FP32 sums (the
+// reference's, summed per term and combined in FP32) carry about 1e-6.
+// Layout (kTrackVals): 0-26 the create_jtjjtr<6,7> order of A and b, 27 / 28
+// ICP r^2 / count, 29 / 30 RGB r^2 / count.
+constexpr int kTrackVals = 31;
+struct CombinedSums {
+    double s[32];  // kTrackVals, padded to a power of two for the wavefront transpose
+    double w;      // the ICP weight
+    unsigned n_icp, n_rgb;  // the counts, moved into s[28] / s[30] for the reduction
+    LF_DEV void clear() {
+#pragma unroll
+        for (int k = 0; k < 32; ++k) s[k] = 0.0;
+        n_icp = n_rgb = 0u;
+    }
+    LF_DEV void icp(const float *row) {
+        double d[7], wd[6];
+#pragma unroll
+        for (int i = 0; i < 7; ++i) d[i] = (double)row[i];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) wd[i] = w * d[i];
+        int k = 0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+#pragma unroll
+            for (int j = i; j < 7; ++j) s[k] = __builtin_fma(wd[i], d[j], s[k]), ++k;
+        s[27] = __builtin_fma(d[6], d[6], s[27]);
+        ++n_icp;
+    }
+    LF_DEV void rgb(const float *row) {
+        double d[7];
+#pragma unroll
+        for (int i = 0; i < 7; ++i) d[i] = (double)row[i];
+        int k = 0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+#pragma unroll
+            for (int j = i; j < 7; ++j) s[k] = __builtin_fma(d[i], d[j], s[k]), ++k;
+        s[29] = __builtin_fma(d[6], d[6], s[29]);
+        ++n_rgb;
+    }
+};
 
 // Both rows of one reference pixel — ICPReduction::searchPoint + getRow
 // (pose_estimator.cu:168-222) and RgbReduction2::find_corresp +
@@ -112,9 +180,8 @@ LF_DEV Own load_own(const LfTrackLevel &L, int i) {
 // the reference's correctly rounded divisions and roots.
 LF_DEV float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-template <bool kIcp, bool kRgb, bool kHalf = false>
-LF_DEV void pixel_rows(const ReduceArgs &a, const Own &o, const float *P, float stddev, float th, float *si,
-                       float *sr) {
+template <bool kIcp, bool kRgb, bool kHalf, class Sums>
+LF_DEV void pixel_rows(const ReduceArgs &a, const Own &o, const float *P, float stddev, float th, Sums &sums) {
     const LfTrackLevel &L = a.lv;
     const float4 lv = o.lv, ln = o.ln;
     const float il = o.il;
@@ -170,7 +237,7 @@ LF_DEV void pixel_rows(const ReduceArgs &a, const Own &o, const float *P, float 
             row[0] = ng.x; row[1] = ng.y; row[2] = ng.z;
             row[3] = c.x; row[4] = c.y; row[5] = c.z;
             row[6] = dot(ng, sub(vc, vg));
-            accumulate(si, row);
+            sums.icp(row);
         }
     }
     if (kRgb && rgb_ok) {
@@ -194,18 +261,9 @@ LF_DEV void pixel_rows(const ReduceArgs &a, const Own &o, const float *P, float 
             row[6] = w * (-r);
             row[0] = l.x * w; row[1] = l.y * w; row[2] = l.z * w;
             row[3] = c.x * w; row[4] = c.y * w; row[5] = c.z * w;
-            accumulate(sr, row);
+            sums.rgb(row);
         }
     }
-}
-
-LF_DEV void accumulate(float *s, const float *row) {
-    int k = 0;
-#pragma unroll
-    for (int i = 0; i < 7; ++i)
-#pragma unroll
-        for (int j = i; j < 7; ++j) s[k++] += row[i] * row[j];
-    s[28] += 1.0f;
 }
 
 // ---------------------------------------------------------- FP64 SE(3) helpers
@@ -362,12 +420,12 @@ LF_DEV void ldlt6(const double *A, const double *b, double *x) {
         x[i] = s;
     }
 }
-// create_jtjjtr<6,7> (reduce_sum.h:48-61), scaled as A = w * H (float) + ...
-LF_DEV void unpack29(const float *s, float *A, float *b) {
+// create_jtjjtr<6,7> (reduce_sum.h:48-61) of the combined sums: A and b.
+LF_DEV void unpack27(const double *s, double *A, double *b) {
     int k = 0;
     for (int i = 0; i < 6; ++i)
         for (int j = i; j < 7; ++j) {
-            const float v = s[k++];
+            const double v = s[k++];
             if (j == 6) b[i] = v;
             else A[j * 6 + i] = A[i * 6 + j] = v;
         }
@@ -399,8 +457,7 @@ LF_DEV GnState load_gn(const LfTrackState &st) {
 // One Gauss-Newton step of compute_transform (icp_tracker.cpp:50-139), run by
 // a single lane after the sums are final: reads g, updates g and writes the
 // step's results to st.
-LF_DEV void gn_step(LfTrackState &st, GnState &g, const float *si, const float *sr, int first_iter,
-                    float icp_weight, bool context_mode) {
+LF_DEV void gn_step(LfTrackState &st, GnState &g, const double *sums, int first_iter, bool context_mode) {
     if (first_iter) {
         g.icp_error = FLT_MAX;
         g.rgb_error = FLT_MAX;
@@ -411,20 +468,19 @@ LF_DEV void gn_step(LfTrackState &st, GnState &g, const float *si, const float *
     for (int k = 0; k < 12; ++k) last[k] = g.est[k];
     const float last_icp = g.icp_error, last_rgb = g.rgb_error;
     st.last_icp_error = last_icp;
-    st.stddev = sqrtf(sr[27] / (sr[28] - 6.0f));
-    float Ai[36], bi[6], Ar[36], br[6];
-    unpack29(si, Ai, bi);
-    unpack29(sr, Ar, br);
+    // the residual terms in FP32, as the reference's (m_estimator.cu:470-480)
+    const float icp_r2 = (float)sums[27], icp_n = (float)sums[28];
+    const float rgb_r2 = (float)sums[29], rgb_n = (float)sums[30];
+    st.stddev = sqrtf(rgb_r2 / (rgb_n - 6.0f));
     double A[36], b[6], x[6];
-    for (int k = 0; k < 36; ++k) A[k] = (double)(icp_weight * Ai[k] + Ar[k]);
-    for (int k = 0; k < 6; ++k) b[k] = (double)(icp_weight * bi[k] + br[k]);
+    unpack27(sums, A, b);
     ldlt6(A, b, x);
     double E[12];
     se3_exp(x, E);
     m34_mul(E, last, g.est);
     g.iterations++;
     bool stop = false, revert = false;
-    float icp_error = sqrtf(si[27]) / si[28];
+    float icp_error = sqrtf(icp_r2) / icp_n;
     if (isnan(icp_error) && !context_mode) {
         g.invalid = 1;
         stop = true;
@@ -440,7 +496,7 @@ LF_DEV void gn_step(LfTrackState &st, GnState &g, const float *si, const float *
     }
     g.icp_error = icp_error;
     if (!stop) {
-        float rgb_error = sqrtf(sr[27]) / sr[28];
+        float rgb_error = sqrtf(rgb_r2) / rgb_n;
         if (isnan(rgb_error) && !context_mode) {
             g.invalid = 1;
             stop = true;
@@ -480,15 +536,15 @@ LF_DEV void gn_step(LfTrackState &st, GnState &g, const float *si, const float *
 // shuffles instead of 6 N.  Afterwards lane l holds the wavefront total of
 // value l >> (6 - log2 N).  Fixed pairing: deterministic.
 // (Compile-time recursion keeps every index constant, so v stays in VGPRs.)
-template <int N, int kOff>
-LF_DEV void transpose_halve(float *v, int lane) {
+template <int N, int kOff, class T>
+LF_DEV void transpose_halve(T *v, int lane) {
     if constexpr (N > 1) {
         constexpr int h = N / 2;
         const bool hi = (lane & kOff) != 0;
 #pragma unroll
         for (int i = 0; i < h; ++i) {
-            const float keep = hi ? v[h + i] : v[i];
-            const float send = hi ? v[i] : v[h + i];
+            const T keep = hi ? v[h + i] : v[i];
+            const T send = hi ? v[i] : v[h + i];
             v[i] = keep + __shfl_xor(send, kOff);
         }
         transpose_halve<h, kOff / 2>(v, lane);
@@ -498,8 +554,8 @@ LF_DEV void transpose_halve(float *v, int lane) {
     }
 }
 
-template <int N>
-LF_DEV float transpose_reduce(float (&v)[N]) {
+template <int N, class T>
+LF_DEV T transpose_reduce(T (&v)[N]) {
     static_assert(N >= 1 && N <= 64 && (N & (N - 1)) == 0, "N must be a power of two <= 64");
     transpose_halve<N, 32>(v, __lane_id());
     return v[0];
@@ -509,7 +565,8 @@ LF_DEV float transpose_reduce(float (&v)[N]) {
 // value order: transpose butterfly in each wavefront, then the wavefronts meet
 // in LDS.  Lanes with tid < kVals return value tid (the others return 0).
 template <bool kIcp, bool kRgb>
-LF_DEV float block_sums(const float *si, const float *sr, float (*s_wave)[kVals]) {
+LF_DEV float block_sums(const FloatSums &sums, float (*s_wave)[kVals]) {
+    const float *si = sums.si, *sr = sums.sr;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     float v[64];
 #pragma unroll
@@ -567,6 +624,54 @@ LF_DEV void final_sums(const float *partials, int G, float *s_sum) {
     if ((lane & 7) == 0 && v < kVals) s_sum[v] = s;
 }
 
+// The combined FP64 sums of one workgroup: transpose butterfly over the 32
+// (padded) values in each wavefront, leaving value l >> 1 in lane l, then
+// the wavefronts meet in LDS.  Lanes with tid < kTrackVals return value tid.
+LF_DEV double block_sums_d(CombinedSums &sums, double (*s_wave)[32]) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    sums.s[28] = (double)sums.n_icp;
+    sums.s[30] = (double)sums.n_rgb;
+    const double w = transpose_reduce<32>(sums.s);
+    if ((lane & 1) == 0) s_wave[wave][lane >> 1] = w;
+    __syncthreads();
+    double t = 0.0;
+    if (tid < kTrackVals) {
+#pragma unroll
+        for (int k = 0; k < kWaves; ++k) t += s_wave[k][tid];
+    }
+    return t;
+}
+
+// final_sums of the G combined FP64 partial rows (value v of workgroup g at
+// v * G + g), in a fixed order; the kTrackVals totals land in s_sum.
+LF_DEV void final_sums_d(const double *partials, int G, double *s_sum) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int kPer = 4;
+    static_assert(kPer * kWaves >= kTrackVals, "values per wavefront");
+    constexpr int kChunks = kGrid / 64;
+    double acc[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int v = wave + kWaves * j;
+        const double *src = partials + (size_t)(v < kTrackVals ? v : 0) * G;
+        double x[kChunks];
+#pragma unroll
+        for (int c = 0; c < kChunks; ++c) {
+            const int g = c * 64 + lane;
+            const double t = src[g < G ? g : G - 1];
+            const unsigned long long keep = (v < kTrackVals && g < G) ? ~0ull : 0ull;
+            x[c] = __longlong_as_double((long long)((unsigned long long)__double_as_longlong(t) & keep));
+        }
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < kChunks; ++c) s += x[c];
+        acc[j] = s;
+    }
+    const double s = transpose_reduce<kPer>(acc);
+    const int v = wave + kWaves * (lane >> 4);
+    if ((lane & 15) == 0 && v < kTrackVals) s_sum[v] = s;
+}
+
 // Stand-alone reductions (lf_icp_reduce, lf_rgb_step).  kMode 1: ICP rows,
 // 2: RGB rows.  Each workgroup writes its kVals partials value-major (value v
 // of workgroup g at v * G + g).
@@ -580,13 +685,12 @@ __global__ __launch_bounds__(kThreads) void k_reduce(ReduceArgs a) {
     for (int k = 0; k < 12; ++k) P[k] = a.pose12[k];
     const float stddev = a.stddev;
     const float th = (float)(1.345 * (double)stddev);
-    float si[29], sr[29];
-#pragma unroll
-    for (int k = 0; k < 29; ++k) { si[k] = 0.0f; sr[k] = 0.0f; }
+    FloatSums sums;
+    sums.clear();
     const int n = a.lv.cols * a.lv.rows;
     for (int i = blockIdx.x * kThreads + tid; i < n; i += gridDim.x * kThreads)
-        pixel_rows<kIcp, kRgb>(a, load_own<kIcp, kRgb>(a.lv, i), P, stddev, th, si, sr);
-    const float t = block_sums<kIcp, kRgb>(si, sr, s_wave);
+        pixel_rows<kIcp, kRgb, false>(a, load_own<kIcp, kRgb>(a.lv, i), P, stddev, th, sums);
+    const float t = block_sums<kIcp, kRgb>(sums, s_wave);
     if (tid < kVals) a.partials[(size_t)tid * gridDim.x + blockIdx.x] = t;
 }
 
@@ -630,7 +734,7 @@ LF_DEV void track_finish(LfTrackState &st, const double *est, int invalid, bool 
 // tracker-state loads.
 // Workgroup wg of the nwg computing the step's partial rows.  kHalf: FP16 maps.
 template <bool kHalf>
-LF_DEV void track_step_wg(const ReduceArgs &a, int wg, int nwg, float (*s_wave)[kVals]) {
+LF_DEV void track_step_wg(const ReduceArgs &a, int wg, int nwg, double (*s_wave)[32]) {
     const int tid = threadIdx.x;
     const int n = a.lv.cols * a.lv.rows;
     const int stride = nwg * kThreads;
@@ -641,7 +745,7 @@ LF_DEV void track_step_wg(const ReduceArgs &a, int wg, int nwg, float (*s_wave)[
     float P[12], stddev;
     if (a.first_of_frame) {
 #pragma unroll
-        for (int k = 0; k < 12; ++k) P[k] = (float)a.init[k];
+        for (int k = 0; k < 12; ++k) P[k] = a.init_f[k];
         stddev = 0.0f;
     } else {
         const LfTrackState &st = *a.st;
@@ -653,23 +757,24 @@ LF_DEV void track_step_wg(const ReduceArgs &a, int wg, int nwg, float (*s_wave)[
         if (!run) return;
     }
     const float th = (float)(1.345 * (double)stddev);
-    float si[29], sr[29];
-#pragma unroll
-    for (int k = 0; k < 29; ++k) { si[k] = 0.0f; sr[k] = 0.0f; }
+    CombinedSums sums;
+    sums.clear();
+    sums.w = (double)a.icp_weight;
     while (i < n) {
         const Own cur = o;
         const int next = i + stride;
         if (next < n) o = load_own<true, true, kHalf>(a.lv, next);
-        pixel_rows<true, true, kHalf>(a, cur, P, stddev, th, si, sr);
+        pixel_rows<true, true, kHalf>(a, cur, P, stddev, th, sums);
         i = next;
     }
-    const float t = block_sums<true, true>(si, sr, s_wave);
-    if (tid < kVals) a.partials[(size_t)tid * nwg + wg] = t;
+    const double t = block_sums_d(sums, s_wave);
+    if (tid < kTrackVals) reinterpret_cast<double *>(a.partials)[(size_t)tid * nwg + wg] = t;
 }
 
+// Held to 128 VGPRs: 4 waves per SIMD, two 512-lane workgroups per CU.
 template <bool kHalf>
-__global__ __launch_bounds__(kThreads) void k_track_step(ReduceArgs a) {
-    __shared__ float s_wave[kWaves][kVals];
+__global__ __launch_bounds__(kThreads, 4) void k_track_step(ReduceArgs a) {
+    __shared__ double s_wave[kWaves][32];
     track_step_wg<kHalf>(a, blockIdx.x, gridDim.x, s_wave);
 }
 
@@ -679,7 +784,7 @@ __global__ __launch_bounds__(kThreads) void k_track_step(ReduceArgs a) {
 // last-workgroup ticket needs an agent-scope release per workgroup, an L2
 // write-back on each of the 8 XCDs per workgroup; measured slower than the
 // kernel boundary.)
-LF_DEV void track_final_wg(const ReduceArgs &a, int G, float *s_sum) {
+LF_DEV void track_final_wg(const ReduceArgs &a, int G, double *s_sum) {
     const int tid = threadIdx.x;
     GnState g = load_gn(*a.st);
     if (a.first_of_frame) {
@@ -690,13 +795,14 @@ LF_DEV void track_final_wg(const ReduceArgs &a, int G, float *s_sum) {
     } else if (!(!g.invalid && (a.first_iter || g.active))) {
         return;
     }
-    final_sums(a.partials, G, s_sum);
+    final_sums_d(reinterpret_cast<const double *>(a.partials), G, s_sum);
     __syncthreads();
+    if (tid < kTrackVals) a.st->sums[tid] = s_sum[tid];
     if (tid == 0) {
         LfTrackState &st = *a.st;
         if (a.first_of_frame)
             for (int k = 0; k < 12; ++k) st.init[k] = a.init[k];
-        gn_step(st, g, s_sum, s_sum + 29, a.first_iter, a.icp_weight, a.context_mode != 0);
+        gn_step(st, g, s_sum, a.first_iter, a.context_mode != 0);
         // the frame ends here when the estimate became invalid, when the
         // finest level broke out of its loop, or after the frame's last step
         if (g.invalid || (a.last_level && !g.active) || a.last_of_frame)
@@ -705,7 +811,7 @@ LF_DEV void track_final_wg(const ReduceArgs &a, int G, float *s_sum) {
 }
 
 __global__ __launch_bounds__(kThreads) void k_track_final(ReduceArgs a, int G) {
-    __shared__ float s_sum[kVals];
+    __shared__ double s_sum[32];
     track_final_wg(a, G, s_sum);
 }
 
@@ -733,8 +839,8 @@ struct TrackJob {
 // pyramids): a 2-D grid whose y index is the system; each system's workgroups
 // run exactly the single launch's code.
 template <bool kHalf>
-__global__ __launch_bounds__(kThreads) void k_track_step_batch(const TrackJob *__restrict__ jobs) {
-    __shared__ float s_wave[kWaves][kVals];
+__global__ __launch_bounds__(kThreads, 4) void k_track_step_batch(const TrackJob *__restrict__ jobs) {
+    __shared__ double s_wave[kWaves][32];
     const TrackJob &j = jobs[blockIdx.y];
     if ((int)blockIdx.x >= j.G) return;
     track_step_wg<kHalf>(global_args(j.a), blockIdx.x, j.G, s_wave);
@@ -742,7 +848,7 @@ __global__ __launch_bounds__(kThreads) void k_track_step_batch(const TrackJob *_
 
 // The final sums and Gauss-Newton step of each system: workgroup i, system i.
 __global__ __launch_bounds__(kThreads) void k_track_final_batch(const TrackJob *__restrict__ jobs) {
-    __shared__ float s_sum[kVals];
+    __shared__ double s_sum[32];
     const TrackJob &j = jobs[blockIdx.x];
     if (j.G <= 0) return;
     track_final_wg(global_args(j.a), j.G, s_sum);
@@ -798,7 +904,11 @@ int grid_for(const LfTrackLevel &lv) {
 
 }  // namespace
 
-size_t lf_reduce_partials(void) { return (size_t)kGrid * kVals + 64; }
+// The stand-alone reductions' kGrid x kVals FP32 partials, or the tracking
+// step's kGrid x kTrackVals FP64 partials, whichever is larger.
+size_t lf_reduce_partials(void) {
+    return std::max((size_t)kGrid * kVals, (size_t)kGrid * kTrackVals * 2) + 64;
+}
 
 extern "C" size_t lf_reduce_scratch_floats(void) { return lf_reduce_partials(); }
 
@@ -822,7 +932,10 @@ static ReduceArgs step_args(const LfTrackLevel &lv, float *scratch, LfTrackState
     a.last_level = f.last_level;
     a.last_of_frame = f.last_of_frame;
     a.context_mode = f.context_mode;
-    for (int k = 0; k < 12; ++k) a.init[k] = f.init[k];
+    for (int k = 0; k < 12; ++k) {
+        a.init[k] = f.init[k];
+        a.init_f[k] = (float)f.init[k];
+    }
     return a;
 }
 
@@ -842,7 +955,7 @@ int lf_track_iteration(const LfTrackLevel &lv, float *scratch, LfTrackState *st,
 size_t lf_track_job_bytes(void) { return sizeof(TrackJob); }
 
 int lf_track_job_make(const LfTrackLevel &lv, float *scratch, LfTrackState *st, const LfStepFlags &f,
-                      float icp_weight, int share, void *job, int *grid) {
+                      float icp_weight, int share, int wgs, void *job, int *grid) {
     if (int rc = check_level(lv)) return rc;
     TrackJob &j = *static_cast<TrackJob *>(job);
     j = TrackJob{};
@@ -852,7 +965,7 @@ int lf_track_job_make(const LfTrackLevel &lv, float *scratch, LfTrackState *st, 
     // flight, and do 1/16 of the block reductions): 4.1k -> 4.7k frames/s
     // aggregate in mixed precision.  16 and 64 workgroups each measured
     // slower than 32 (5.46k and 5.61k against 5.74k).
-    j.G = std::min(grid_for(lv), std::max(1, kGrid / std::max(1, share)));
+    j.G = std::min(grid_for(lv), wgs > 0 ? std::min(wgs, kGrid) : std::max(1, kGrid / std::max(1, share)));
     if (grid) *grid = j.G;
     return LF_OK;
 }

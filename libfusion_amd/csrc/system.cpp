@@ -425,7 +425,7 @@ int DenseTracking::enqueue(DeviceImage &ref, DeviceImage &cur, const TrackingCon
             const size_t J = lf_track_job_bytes();
             jobs->resize(jobs->size() + J);
             int G = 0;
-            LF_RC(lf_track_job_make(lv, scratch_, state_, f, icp_weight, batch_share_, jobs->data() + jobs->size() - J,
+            LF_RC(lf_track_job_make(lv, scratch_, state_, f, icp_weight, batch_share_, batch_wgs_, jobs->data() + jobs->size() - J,
                                     &G));
             grids->push_back(G);
         } else {
@@ -445,6 +445,14 @@ TrackingResult DenseTracking::result() {
     r.icp_error = context_mode_ ? h.last_icp_error : h.icp_error;
     r.update = r.sucess ? SE3d::from3x4(h.est).inverse() : SE3d();
     return r;
+}
+
+int DenseTracking::read_sums(double *sums31) {
+    double h[32];
+    LF_HIP_CHECK(hipMemcpyAsync(h, state_->sums, sizeof(h), hipMemcpyDeviceToHost, stream_));
+    LF_HIP_CHECK(hipStreamSynchronize(stream_));
+    for (int k = 0; k < 31; ++k) sums31[k] = h[k];
+    return LF_OK;
 }
 
 // ------------------------------------------------------------ DenseOdometry
@@ -718,6 +726,36 @@ hipEvent_t Batch::take_event() {
     return e;
 }
 
+int Batch::set_option(int option, int value) {
+    switch (option) {
+    case LF_BATCH_OPT_UPLOAD_OVERLAP:
+        if (value != 0 && value != 1) return lf_set_error(LF_ERR_ARG, "upload overlap is 0 or 1, not %d", value);
+        upload_overlap_ = value;
+        return LF_OK;
+    case LF_BATCH_OPT_TRACK_WORKGROUPS:
+        if (value < 0 || value > 512) return lf_set_error(LF_ERR_ARG, "tracking workgroups %d not in 0..512", value);
+        track_wgs_ = value;
+        for (auto &s : sys_) s->odometry().tracker().set_batch_workgroups(value);
+        return LF_OK;
+    case LF_BATCH_OPT_INTEGRATE_WORKGROUPS:
+        if (value < 0 || value > 65535)
+            return lf_set_error(LF_ERR_ARG, "integration workgroups %d not in 0..65535", value);
+        integrate_wgs_ = value;
+        return LF_OK;
+    default:
+        return lf_set_error(LF_ERR_ARG, "unknown batch option %d", option);
+    }
+}
+
+int Batch::get_option(int option, int *value) const {
+    switch (option) {
+    case LF_BATCH_OPT_UPLOAD_OVERLAP: *value = upload_overlap_; return LF_OK;
+    case LF_BATCH_OPT_TRACK_WORKGROUPS: *value = track_wgs_; return LF_OK;
+    case LF_BATCH_OPT_INTEGRATE_WORKGROUPS: *value = integrate_wgs_; return LF_OK;
+    default: return lf_set_error(LF_ERR_ARG, "unknown batch option %d", option);
+    }
+}
+
 void Batch::set_timing(int stages) {
     timing_ = stages;
     for (auto &e : events_) { pool_.push_back(e.a); pool_.push_back(e.b); }
@@ -839,6 +877,7 @@ int Batch::step(const uint16_t *const *depth_in, const unsigned char *const *rgb
     }
     LfFuseBatchDims fd;
     LF_RC(lf_update_batch_make(fuse.data(), n, host + off_fuse, &fd));
+    if (integrate_wgs_ > 0) fd.integrate_grid = integrate_wgs_;
     LfRayBatchDims ray;
     LF_RC(lf_raycast_batch_make(items.data(), n, host + off_ray, &ray));
     // Upload stream: the table, then the uploads.  The uploads rewrite the
@@ -846,21 +885,26 @@ int Batch::step(const uint16_t *const *depth_in, const unsigned char *const *rgb
     // fusion, raycast and re-pyramid use the other image of each system), and
     // the table buffer was last read by the step before that: both wait only
     // for the previous step's tracking, so they run beside its fusion and raycast.
-    if (tracked_once_) LF_HIP_CHECK(hipStreamWaitEvent(aux_, tracked_, 0));
-    LF_HIP_CHECK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, aux_));
+    // Without the overlap (upload_overlap_ 0) they run on the batch stream
+    // ahead of the step's tracking.
+    hipStream_t up_st = upload_overlap_ ? aux_ : stream_;
+    if (upload_overlap_ && tracked_once_) LF_HIP_CHECK(hipStreamWaitEvent(aux_, tracked_, 0));
+    LF_HIP_CHECK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, up_st));
     if (from_host) {
         // the staging buffers were last read by the previous step's upload
         // passes, earlier on this same stream
         for (int i = 0; i < n; ++i) {
             LF_HIP_CHECK(hipMemcpyAsync(depth_in_[i].ptr<void>(), depth_in[i], depth_in_[i].bytes(),
-                                        hipMemcpyHostToDevice, aux_));
+                                        hipMemcpyHostToDevice, up_st));
             LF_HIP_CHECK(hipMemcpyAsync(rgb_in_[i].ptr<void>(), rgb_in[i], rgb_in_[i].bytes(), hipMemcpyHostToDevice,
-                                        aux_));
+                                        up_st));
         }
     }
-    for (size_t p = 0; p < n_up; ++p) LF_RC(lf_pass_batch(dev + p * n * PB, n, wmax[p], aux_));
-    LF_HIP_CHECK(hipEventRecord(uploaded_, aux_));
-    LF_HIP_CHECK(hipStreamWaitEvent(stream_, uploaded_, 0));
+    for (size_t p = 0; p < n_up; ++p) LF_RC(lf_pass_batch(dev + p * n * PB, n, wmax[p], up_st));
+    if (upload_overlap_) {
+        LF_HIP_CHECK(hipEventRecord(uploaded_, aux_));
+        LF_HIP_CHECK(hipStreamWaitEvent(stream_, uploaded_, 0));
+    }
     // stage events (set_timing), recorded on the batch stream
     Ev ev[5];
     for (int sid = 0; sid < 5; ++sid) {
@@ -1150,6 +1194,11 @@ static int tracking_context(const double *init12, const int *iterations, int num
     return LF_OK;
 }
 
+extern "C" int lf_tracker_get_sums(lf_tracker *t, double *sums31) {
+    if (!t || !sums31) return lf_set_error(LF_ERR_ARG, "null argument to lf_tracker_get_sums");
+    return t->t->read_sums(sums31);
+}
+
 static int read_result(fusion::DenseTracking &t, lf_tracking_result *out) {
     const fusion::TrackingResult r = t.result();
     out->success = r.sucess ? 1 : 0;
@@ -1356,6 +1405,16 @@ extern "C" int lf_batch_process_host(lf_batch *b, const uint16_t *const *depth, 
     for (int i = 0; i < b->b->size(); ++i)
         if (!depth[i] || !rgb[i]) return lf_set_error(LF_ERR_ARG, "null frame for batch system %d", i);
     return b->b->process_host(depth, rgb);
+}
+
+extern "C" int lf_batch_set_option(lf_batch *b, int option, int value) {
+    if (!b) return lf_set_error(LF_ERR_ARG, "null batch");
+    return b->b->set_option(option, value);
+}
+
+extern "C" int lf_batch_get_option(lf_batch *b, int option, int *value) {
+    if (!b || !value) return lf_set_error(LF_ERR_ARG, "null argument to lf_batch_get_option");
+    return b->b->get_option(option, value);
 }
 
 extern "C" int lf_batch_set_timing(lf_batch *b, int stages) {

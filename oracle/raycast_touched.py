@@ -1,14 +1,21 @@
-"""Distinct voxel blocks and hash entries sampled by the C1 raycast (test
+"""Distinct voxel blocks and hash entries sampled by a raycast (test
 infrastructure: the raycast's compulsory bytes for its roofline basis).
 
-Runs the FP32 oracle System over the C1 orbit and, at the sampled frames,
+Runs the FP32 oracle System over a stream's orbit and, at the sampled frames,
 repeats the frame's raycast with the oracle's instrumented march, counting the
-distinct blocks read.  The raycast's algorithmic bytes per frame are then
-    P * 16 (vertex map) + N_blocks * 512 * 6 (sdf + weight) + N_blocks * 20
-    (hash entries) + 2 * tiles * 4 (z-range tiles).
-Writes profiles/raycast_c1_touched.json.
+distinct blocks read.
 
     python oracle/raycast_touched.py [frames ...]
+        the 640x480 5 mm orbit of seed 1; writes profiles/raycast_c1_touched.json
+        with the FP32 raycast's bytes per frame
+            P * 16 (vertex map) + N_blocks * 512 * 6 (int16 sdf + float weight)
+            + N_blocks * 20 (hash entries) + 2 * tiles * 4 (z-range tiles).
+    python oracle/raycast_touched.py --streams 16 [frames ...]
+        the batched workload's streams (seeds 100 .. 100 + n - 1, same volume);
+        writes profiles/raycast_batch_touched.json with the mean blocks per map
+        (the batched mixed raycast's bytes per map: P * (16 + 3 + 4) for the
+        vertex map, colour out and hit colour word, N_blocks * (512 * 2 + 20)
+        for the FP16 sdf plane and entries, 2 * tiles * 4 for the tiles).
 """
 import json
 import os
@@ -20,12 +27,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path[0] = ROOT  # the package, not this directory
 
 from libfusion_amd import configs  # noqa: E402
+from libfusion_amd.synth import Scene  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 
 
-def main():
-    frames = [int(a) for a in sys.argv[1:]] or [50, 150, 250]
-    sc, K, p = configs.c1()
+def touched(sc, K, p, frames):
     op = O.Params(p.num_buckets, p.num_entries, p.num_blocks, p.max_triangles, p.zmin_raycast, p.zmax_raycast,
                   p.zmin_update, p.zmax_update, p.voxel_size, p.truncation)
     oK = O.make_intr(K.width, K.height, K.fx, K.fy, K.cx, K.cy)
@@ -41,17 +47,45 @@ def main():
             samples.append({"frame": i, "blocks_touched": m.ray_blocks_touched(), "rays": st["rays"],
                             "samples": st["samples"], "hits": st["hits"]})
             print(samples[-1], flush=True)
+    return samples
+
+
+def main():
+    args = sys.argv[1:]
+    n = 0
+    if args and args[0] == "--streams":
+        n = int(args[1])
+        args = args[2:]
+    frames = [int(a) for a in args] or ([10, 35, 59] if n else [50, 150, 250])
+    sc0, K, p = configs.c1()
     P = K.width * K.height
     tiles = (K.width // 8) * (K.height // 8)
-    nb = float(np.mean([x["blocks_touched"] for x in samples]))
-    out = {"workload": "C1 640x480, 5 mm voxels: raycast at the tracked pose after the frame's update",
-           "frames": samples, "blocks_touched_mean": nb,
-           "algorithmic_bytes_per_raycast": int(P * 16 + nb * 512 * 6 + nb * 20 + 2 * tiles * 4),
-           "command": "python oracle/raycast_touched.py " + " ".join(map(str, frames))}
-    path = os.path.join(ROOT, "profiles", "raycast_c1_touched.json")
+    cmd = "python oracle/raycast_touched.py " + ("--streams %d " % n if n else "") + " ".join(map(str, frames))
+    if not n:
+        samples = touched(sc0, K, p, frames)
+        nb = float(np.mean([x["blocks_touched"] for x in samples]))
+        out = {"workload": "C1 640x480, 5 mm voxels: raycast at the tracked pose after the frame's update",
+               "frames": samples, "blocks_touched_mean": nb,
+               "algorithmic_bytes_per_raycast": int(P * 16 + nb * 512 * 6 + nb * 20 + 2 * tiles * 4),
+               "command": cmd}
+        name = "raycast_c1_touched.json"
+    else:
+        streams = []
+        for i in range(n):
+            sc = Scene(1, 100 + i)
+            samples = touched(sc, K, p, frames)
+            streams.append({"seed": 100 + i, "frames": samples})
+        nb = float(np.mean([x["blocks_touched"] for s in streams for x in s["frames"]]))
+        out = {"workload": "%d streams (seeds 100-%d), 640x480, 5 mm voxels: FP32 oracle raycast at the tracked "
+                           "pose after the frame's update" % (n, 99 + n),
+               "streams": streams, "blocks_touched_mean_per_map": nb,
+               "algorithmic_bytes_per_map_mixed": int(P * (16 + 3 + 4) + nb * (512 * 2 + 20) + 2 * tiles * 4),
+               "command": cmd}
+        name = "raycast_batch_touched.json"
+    path = os.path.join(ROOT, "profiles", name)
     with open(path, "w") as f:
         json.dump(out, f, indent=1)
-    print("wrote", path, out["algorithmic_bytes_per_raycast"])
+    print("wrote", path, nb)
 
 
 if __name__ == "__main__":

@@ -886,6 +886,64 @@ def test_reductions_match_oracle(stddev):
     assert np.allclose(g, o64, rtol=1e-3, atol=1e-4 * np.abs(o64).max())
 
 
+def test_tracking_sums_are_exact_sums_of_the_oracle_rows():
+    """The tracking step's normal equations (A = 1e6 H_icp + H_rgb, b, and the
+    residual terms; icp_tracker.cpp:50-96) at 640x480, one Gauss-Newton step
+    from the true relative pose: the device combines each pixel's rows and
+    accumulates in FP64, so every sum is within 1e-6 (the bar for compensated
+    sums) of the exact sum of the oracle's FP32 rows on the same level-0 maps,
+    relative to the sum of the terms' magnitudes; counts are equal.  The
+    exact sums are taken in extended precision over the oracle's rows."""
+    sc = Scene(1, 1)
+    K = capi.intrinsics(sc.width, sc.height, sc.fx, sc.fy, sc.cx, sc.cy)
+    frames = []
+    for f in (0, 1):
+        d16, rgb = sc.render(sc.pose(f), K.width, K.height, K.fx, K.fy, K.cx, K.cy, noise=True, frame_id=f)
+        depth = d16.astype(np.float32) * np.float32(sc.depth_scale)
+        frames.append((O.back_project(O.bilateral(depth), o_K(K)), rgb))
+    init = np.linalg.inv(sc.pose(1)) @ sc.pose(0)
+    tr = capi.Tracker(K, 1)
+    t_rv, t_rc, t_cv, t_cc = dev(frames[0][0]), dev(frames[0][1]), dev(frames[1][0]), dev(frames[1][1])
+    tr.set("set_source_vmap", t_cv)
+    tr.set("set_source_image", t_cc)
+    tr.set("set_reference_vmap", t_rv)
+    tr.set("set_reference_image", t_rc)
+    tr.compute_transform((1,), init)
+    g = tr.sums()
+    lib = capi.load()
+
+    def plane(src, which, shape):
+        out = ctypes.c_void_p()
+        capi.check(lib.lf_tracker_plane(tr.h, src, 0, which, ctypes.byref(out)))
+        t = torch.empty(shape, dtype=torch.float32, device="cuda")
+        capi.copy_device(t, out.value, t.numel() * t.element_size())
+        return t.cpu().numpy()
+
+    h, w = K.height, K.width
+    cv, cn = plane(1, capi.PLANE_VMAP, (h, w, 4)), plane(1, capi.PLANE_NMAP, (h, w, 4))
+    cI, cdx, cdy = plane(1, capi.PLANE_INTENSITY, (h, w)), plane(1, capi.PLANE_DX, (h, w)), plane(1, capi.PLANE_DY, (h, w))
+    lv, ln, lI = plane(0, capi.PLANE_VMAP, (h, w, 4)), plane(0, capi.PLANE_NMAP, (h, w, 4)), plane(0, capi.PLANE_INTENSITY, (h, w))
+    tr.close()
+    pose12 = init[:3, :].astype(np.float32).ravel()
+    ri = O.icp_rows(cv, cn, lv, ln, pose12, o_K(K)).astype(np.longdouble)
+    rr = O.rgb_rows(cI, lI, lv, cdx, cdy, 0.0, pose12, o_K(K)).astype(np.longdouble)
+    wt = np.longdouble(1e6)
+    exact, mag = np.zeros(31, np.longdouble), np.zeros(31, np.longdouble)
+    k = 0
+    for i in range(6):
+        for j in range(i, 7):
+            ti, tg = wt * ri[:, i] * ri[:, j], rr[:, i] * rr[:, j]
+            exact[k] = ti.sum() + tg.sum()
+            mag[k] = np.abs(ti).sum() + np.abs(tg).sum()
+            k += 1
+    exact[27], exact[28] = (ri[:, 6] ** 2).sum(), ri[:, 7].sum()
+    exact[29], exact[30] = (rr[:, 6] ** 2).sum(), rr[:, 7].sum()
+    mag[27:] = np.abs(exact[27:])
+    assert g[28] == exact[28] and g[30] == exact[30] and g[28] > 50000 and g[30] > 50000, (g[28], g[30])
+    rel = (np.abs(g.astype(np.longdouble) - exact) / mag).astype(np.float64)
+    assert rel.max() <= 1e-6, (rel.max(), int(rel.argmax()))
+
+
 def run_sequence(sc, K, p, levels, iters, frames, scale_depth=None):
     gsys = capi.System(K, p, levels, iters, sc.depth_scale)
     osys = O.System(o_params(p), o_K(K), levels, iters, sc.depth_scale, np.float64)

@@ -369,6 +369,7 @@ private:
     int upload_overlap_ = 1;  // uploads on aux_ beside the previous step (1) or on stream_ (0)
     int track_wgs_ = 0;       // tracking workgroups per system (0: the share rule)
     int integrate_wgs_ = 0;   // integration workgroups per map (0: the map-count rule)
+    int zrange_wgs_ = 0;      // z-range workgroups per map (0: the map-count rule)
     void release();
     std::vector<std::unique_ptr<System>> sys_;
     hipStream_t stream_ = nullptr;

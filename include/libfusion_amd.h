@@ -372,6 +372,20 @@ int lf_deintegrate_weighted(const lf_map_params *p, const lf_map_storage *s, con
 /* De-integration of a DeviceImage frame (the inverse of lf_update_image). */
 int lf_deintegrate_image(const lf_map_params *p, const lf_map_storage *s, lf_image *img, const float *pose24,
                          int colour, void *stream);
+/* Keyframe re-fusion: n keyframes (DeviceImage frames of one size and
+ * intrinsics) leave the map at their old poses and are fused at their new
+ * (corrected) poses, in one batched pass of three launches.  No single
+ * reference counterpart: the reference re-tracks keyframes against the model
+ * (keyframe_graph.cpp:162-229) and has a deleteBlock with no callers
+ * (map_struct.cu:327-363); this is lf_deintegrate_image of keyframes 0..n-1 at
+ * old_pose24[k] followed by lf_update_image of keyframes 0..n-1 at
+ * new_pose24[k], replayed per voxel in registers with one store (no
+ * re-quantisation between the steps).  old_pose24[k] / new_pose24[k]: device
+ * pointers to float T_wc, T_cw (24 floats).  1 <= n <= LF_REFUSE_MAX_KEYFRAMES. */
+#define LF_REFUSE_MAX_KEYFRAMES 31
+int lf_refuse_keyframes(const lf_map_params *p, const lf_map_storage *s, lf_image *const *frames,
+                        const float *const *old_pose24, const float *const *new_pose24, int n, int colour,
+                        void *stream);
 /* update(RgbdImagePtr frame) (voxel_hashing.cpp:44-65) into a caller's map:
  * lf_update_weighted on the frame's raw depth, level-0 normals and (colour != 0)
  * colour, read through the frame's packed integration samples.  Same result
@@ -586,10 +600,13 @@ int lf_batch_read_timing(lf_batch *b, int stage, int *launches, double *total_ms
  *   LF_BATCH_OPT_TRACK_WORKGROUPS: workgroups per system of each Gauss-Newton
  *     step, 1..512 (0, the default: 512 / n, at most the level's pixels / 512).
  *   LF_BATCH_OPT_INTEGRATE_WORKGROUPS: workgroups per map of the batched
- *     integration, 1..65535 (0, the default: max(1024, 8192 / n)). */
+ *     integration, 1..65535 (0, the default: max(1024, 8192 / n)).
+ *   LF_BATCH_OPT_ZRANGE_WORKGROUPS: workgroups per map of the batched z-range
+ *     fill, 1..65535 (0, the default: 1/n of a lone map's, at least 64). */
 #define LF_BATCH_OPT_UPLOAD_OVERLAP 0
 #define LF_BATCH_OPT_TRACK_WORKGROUPS 1
 #define LF_BATCH_OPT_INTEGRATE_WORKGROUPS 2
+#define LF_BATCH_OPT_ZRANGE_WORKGROUPS 3
 int lf_batch_set_option(lf_batch *b, int option, int value);
 int lf_batch_get_option(lf_batch *b, int option, int *value);
 /* System i's camera pose (4x4 row-major doubles; synchronising), last

@@ -29,7 +29,7 @@ TIMING_ALL = 0x17
 # batch stage ids (bit s of the mask brackets stage s)
 BATCH_INTEGRATE, BATCH_RAYCAST, BATCH_TRACKING, BATCH_FUSION, BATCH_STEP = range(5)
 BATCH_TIMING_ALL = 0x1F
-BATCH_OPT_UPLOAD_OVERLAP, BATCH_OPT_TRACK_WORKGROUPS, BATCH_OPT_INTEGRATE_WORKGROUPS = range(3)
+BATCH_OPT_UPLOAD_OVERLAP, BATCH_OPT_TRACK_WORKGROUPS, BATCH_OPT_INTEGRATE_WORKGROUPS, BATCH_OPT_ZRANGE_WORKGROUPS = range(4)
 
 
 class LfIntrinsics(ctypes.Structure):
@@ -116,6 +116,7 @@ SIGNATURES = [
     ("lf_update_image", I, [P, P, P, P, I, P]),
     ("lf_deintegrate_weighted", I, [P, P, P, P, P, I, I, P, P, P]),
     ("lf_deintegrate_image", I, [P, P, P, P, I, P]),
+    ("lf_refuse_keyframes", I, [P, P, P, P, P, I, I, P]),
     ("lf_tracker_create", I, [P, I, P, P]),
     ("lf_tracker_destroy", I, [P]),
     ("lf_tracker_set_source_vmap", I, [P, P]),
@@ -498,6 +499,18 @@ class Map:
         """Remove a frame fused with update_image (inverse weighted update)."""
         check(load().lf_deintegrate_image(ctypes.byref(self.params), ctypes.byref(self.storage), image.h,
                                           ptr(pose24_dev), 1 if colour else 0, stream_ptr()))
+
+    def refuse_keyframes(self, images, old_pose24_devs, new_pose24_devs, colour=True):
+        """Keyframe re-fusion (lf_refuse_keyframes): every image leaves the map at
+        its old pose and is fused at its new pose, in one batched pass."""
+        n = len(images)
+        if not (len(old_pose24_devs) == len(new_pose24_devs) == n):
+            raise ValueError("one old and one new pose per keyframe")
+        hs = (ctypes.c_void_p * n)(*[im.h for im in images])
+        olds = (ctypes.c_void_p * n)(*[ptr(t) for t in old_pose24_devs])
+        news = (ctypes.c_void_p * n)(*[ptr(t) for t in new_pose24_devs])
+        check(load().lf_refuse_keyframes(ctypes.byref(self.params), ctypes.byref(self.storage), hs, olds, news, n,
+                                         1 if colour else 0, stream_ptr()))
 
     def deintegrate_weighted(self, depth, nmap, rgb, pose24_dev, K):
         h, w = depth.shape[:2]

@@ -21,6 +21,11 @@ int lf_update_ev(const lf_map_params *p, const lf_map_storage *s, const float *d
 int lf_deintegrate_ev(const lf_map_params *p, const lf_map_storage *s, const float *depth, const float *nmap,
                       const unsigned char *rgb, int cols, int rows, const float *pose24, const lf_intrinsics *K,
                       hipStream_t st, const float *samples);
+// Keyframe re-fusion of n keyframes (raw depth and packed samples of one
+// size, float pose24 of each keyframe's old and new pose, device pointers).
+int lf_refuse_ev(const lf_map_params *p, const lf_map_storage *s, const float *const *depth,
+                 const float *const *samples, const float *const *old_pose24, const float *const *new_pose24, int n,
+                 int cols, int rows, const lf_intrinsics *K, int colour, hipStream_t st);
 // lf_select_visible without the count read-back.
 int lf_select_visible_ev(const lf_map_params *p, const lf_map_storage *s, const float *pose24, const lf_intrinsics *K,
                          int cols, int rows, hipStream_t st);

@@ -73,7 +73,9 @@ MapDev make_dev(const lf_map_params *p, const lf_map_storage *s) {
 
 // createBlock (map_struct.cu:280-325) as a lock-free insert.  Marks the slot as
 // touched by this update (the reference's flag = 2, depth_fusion.cu:157).
-LF_DEV void insert_block(const MapDev &m, int bx, int by, int bz) {
+// Returns the block's hash slot (-1: the table is full); *claimed: this call
+// created the entry.
+LF_DEV int insert_block(const MapDev &m, int bx, int by, int bz, bool *claimed_out = nullptr) {
     const unsigned long long key = pack_key(bx, by, bz);
     int slot = compute_hash(bx, by, bz, m.num_buckets);
     for (int i = 0; i < m.num_entries; ++i) {
@@ -98,15 +100,17 @@ LF_DEV void insert_block(const MapDev &m, int bx, int by, int bz) {
                 atomicOr(&m.counters[LF_CNT_OVERFLOW], LF_OVF_POOL);
             }
             m.slot_flag[slot] = 1;
-            return;
+            if (claimed_out) *claimed_out = true;
+            return slot;
         }
         if (k == key) {
             m.slot_flag[slot] = 1;
-            return;
+            return slot;
         }
         if (++slot == m.num_entries) slot = 0;
     }
     atomicOr(&m.counters[LF_CNT_OVERFLOW], LF_OVF_TABLE);
+    return -1;
 }
 
 // One pixel of create_blocks_kernel (depth_fusion.cu:121-228): the 3-D
@@ -178,19 +182,28 @@ LF_DEV unsigned int key_mix(unsigned long long k) {
 }
 
 // Mark an existing block as touched by the current frame (no insertion).
-LF_DEV void touch_block(const MapDev &m, int bx, int by, int bz) {
+// The mask bit of a block that keyframe re-fusion's band walks created.
+constexpr unsigned long long kRefuseFresh = 1ull << 63;
+
+LF_DEV int touch_block(const MapDev &m, int bx, int by, int bz) {
     const int b = find_block(m.keys, m.slot_block, bx, by, bz, m.num_buckets, m.num_entries);
-    if (b >= 0) m.slot_flag[reinterpret_cast<const int4 *>(m.block_pos)[b].w] = 1;
+    if (b < 0) return -1;
+    const int slot = reinterpret_cast<const int4 *>(m.block_pos)[b].w;
+    m.slot_flag[slot] = 1;
+    return slot;
 }
 
 // kInsert: create_blocks_kernel.  !kInsert (de-integration): the same band
 // walk only marks the blocks that exist, so the visible list of a frame can be
 // rebuilt without allocating.
-// 16x16 pixel tile (tx, ty) of the frame.
+// 16x16 pixel tile (tx, ty) of the frame.  slot_mask (keyframe re-fusion):
+// each block the walk marks also gets `bit` in its hash slot's mask, and
+// kRefuseFresh when the walk created it.
 template <bool kInsert, bool kFast = false>
 LF_DEV void alloc_wg(const MapDev &m, const float *__restrict__ depth, int cols, int rows,
                      const float *__restrict__ pose24, const lf_intrinsics &K, int tx, int ty,
-                     unsigned long long *s_set) {
+                     unsigned long long *s_set, unsigned long long *slot_mask = nullptr,
+                     unsigned long long bit = 0ull) {
     const int tid = threadIdx.x;
     for (int i = tid; i < kDedup; i += 256) s_set[i] = kEmptyKey;
     if (tx == 0 && ty == 0 && tid == 0) {
@@ -211,16 +224,18 @@ LF_DEV void alloc_wg(const MapDev &m, const float *__restrict__ depth, int cols,
         for (int probe = 0; probe < kDedup; ++probe) {
             unsigned long long old = atomicCAS(&s_set[h], kEmptyKey, key);
             if (old == kEmptyKey) {
-                if (kInsert) insert_block(m, bx, by, bz);
-                else touch_block(m, bx, by, bz);
+                bool fresh = false;
+                const int slot = kInsert ? insert_block(m, bx, by, bz, &fresh) : touch_block(m, bx, by, bz);
+                if (slot_mask && slot >= 0) atomicOr(&slot_mask[slot], bit | (fresh ? kRefuseFresh : 0ull));
                 return;
             }
             if (old == key) return;  // another lane of this tile owns the request
             h = (h + 1) & (kDedup - 1);
         }
         // LDS set full: go straight to the table
-        if (kInsert) insert_block(m, bx, by, bz);
-        else touch_block(m, bx, by, bz);
+        bool fresh = false;
+        const int slot = kInsert ? insert_block(m, bx, by, bz, &fresh) : touch_block(m, bx, by, bz);
+        if (slot_mask && slot >= 0) atomicOr(&slot_mask[slot], bit | (fresh ? kRefuseFresh : 0ull));
     });
 }
 
@@ -410,7 +425,20 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
         // The gathers and voxel loads below are unconditional, so that both
         // voxels' loads issue before the first wait: a lane with nothing to
         // load reads pixel 0, or voxel 0 of its block, and its values are
-        // never used.
+        // never used.  Mixed precision reads every voxel's sdf and word in
+        // the same round as the gathers (one memory round trip before the
+        // fusion instead of two, for the bytes of the voxels the band leaves
+        // out).
+        unsigned int word[2], vo[2];
+        short sp[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            vo[h] = (unsigned int)(t + 256 * h);
+            if constexpr (kMixed) {
+                word[h] = at_byte(wd_b, 4u * vo[h]);  // weight code and colour together
+                sp[h] = at_byte(sdf_b, 2u * vo[h]);
+            }
+        }
         float nx[2] = {0.0f, 0.0f};
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -428,8 +456,6 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
             }
         }
         float sdf[2], wraw[2];
-        unsigned int word[2], vo[2];
-        short sp[2];
         unsigned char cold[2][3];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -441,19 +467,16 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
             sdf[h] = d - ptz[h];
             in = in && !(sdf[h] < -trunc);
             ok[h] = in;
-            vo[h] = (unsigned int)(t + 256 * h);
-            const unsigned int o = in ? vo[h] : 0u;
-            if constexpr (kMixed) {
-                word[h] = at_byte(wd_b, 4u * o);  // weight code and colour together
-            } else {
+            if constexpr (!kMixed) {
+                const unsigned int o = in ? vo[h] : 0u;
                 wraw[h] = at_byte(w_b, 4u * o);
                 if (kColour) {
                     cold[h][0] = at_byte(c_b, o);
                     cold[h][1] = at_byte(c_b, o + LF_BLOCK3);
                     cold[h][2] = at_byte(c_b, o + 2 * LF_BLOCK3);
                 }
+                sp[h] = at_byte(sdf_b, 2u * o);
             }
-            sp[h] = at_byte(sdf_b, 2u * o);
         }
         float wp[2];
 #pragma unroll
@@ -698,6 +721,279 @@ __global__ __launch_bounds__(256) void k_deintegrate(MapDev m, const float *__re
     }
 }
 
+// ------------------------------------------------------------ keyframe re-fusion
+//
+// n keyframes (each a frame's raw depth and packed integration samples, all of
+// one size and intrinsics) leave the map at their old poses and are fused at
+// their new poses in one pass: block allocation for every keyframe at its new
+// pose (one launch), the union of the blocks any old or new pose sees (one
+// launch), then one lane per voxel replays, in registers, the de-integration
+// of keyframes 0..n-1 at their old poses followed by their weighted update at
+// the new poses, in that order, and stores once.  Each step follows
+// k_deintegrate / the weighted update (depth_fusion.cu:356-428) for that
+// voxel, so the result is that of the 2n launches in sequence except that the
+// voxel is not re-quantised between the steps.  A keyframe's step touches the
+// blocks its own launches would have listed: those its pose sees
+// (is_block_visible, depth_fusion.cu:27-47) and those the band walk of its
+// depth at that pose marks (create_blocks_kernel, depth_fusion.cu:121-228;
+// the old poses' walks only mark blocks that exist), recorded per hash slot
+// as one bit per keyframe and pose.  A block this call creates did not exist
+// for the de-integrations, nor for the updates before that of the first
+// keyframe whose walk reaches it (in sequence it would be allocated there).
+constexpr int kMaxRefuse = LF_REFUSE_MAX_KEYFRAMES;  // 2 poses each: one wavefront's lanes
+static_assert(2 * kMaxRefuse < 64, "a wavefront tests every pose of a block; bit 63 marks new blocks");
+
+struct RefuseArgs {
+    MapDev m;
+    unsigned long long *slot_mask;  // per hash slot: bit k old pose k, bit n + k new pose k
+    const float *depth[kMaxRefuse];
+    const float4 *samples[kMaxRefuse];
+    const float *old24[kMaxRefuse];
+    const float *new24[kMaxRefuse];
+    int n, cols, rows;
+    lf_intrinsics K;
+};
+
+__global__ void k_fill_u64(unsigned long long *p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+// Row y < n of the grid: keyframe y's band walk at its old pose (existing
+// blocks only); row n + k: keyframe k's at its new pose, allocating.
+template <bool kFast>
+__global__ __launch_bounds__(256) void k_refuse_alloc(RefuseArgs a) {
+    __shared__ unsigned long long s_set[kDedup];
+    const int tiles_x = (a.cols + 15) / 16;
+    const int y = blockIdx.y, t = blockIdx.x;
+    if (y < a.n)
+        alloc_wg<false, kFast>(a.m, a.depth[y], a.cols, a.rows, a.old24[y], a.K, t % tiles_x, t / tiles_x, s_set,
+                               a.slot_mask, 1ull << y);
+    else
+        alloc_wg<true, kFast>(a.m, a.depth[y - a.n], a.cols, a.rows, a.new24[y - a.n], a.K, t % tiles_x,
+                              t / tiles_x, s_set, a.slot_mask, 1ull << y);
+}
+
+// Pose j of the 2n (j < n: keyframe j's old pose, else keyframe j - n's new
+// pose) as a float T_cw.
+LF_DEV const float *refuse_pose(const RefuseArgs &a, int j) {
+    return (j < a.n ? a.old24[j] : a.new24[j - a.n]) + 12;
+}
+
+// The visible list becomes every allocated block that a keyframe's old or new
+// pose sees or whose slot the band walks marked (the slot flags, left set
+// for k_refuse to clear).
+template <bool kFast>
+__global__ __launch_bounds__(kVisThreads) void k_refuse_visible(RefuseArgs a) {
+    __shared__ unsigned int s_cnt[kVisThreads / kWave];
+    __shared__ unsigned int s_base;
+    const MapDev &m = a.m;
+    const unsigned int nb = min(m.counters[LF_CNT_ALLOCATED], (unsigned int)m.num_blocks);
+    const float scale = 8.0f * m.voxel_size;
+    const unsigned int stride = gridDim.x * blockDim.x;
+    const int lane = lane_id(), wave = threadIdx.x / kWave;
+    for (unsigned int b0 = blockIdx.x * blockDim.x; b0 < nb; b0 += stride) {
+        const unsigned int b = b0 + threadIdx.x;
+        bool vis = false;
+        int4 p = make_int4(0, 0, 0, 0);
+        if (b < nb) {
+            p = reinterpret_cast<const int4 *>(m.block_pos)[b];
+            vis = m.slot_flag[p.w] != 0 || a.slot_mask[p.w] != 0ull;
+            for (int j = 0; j < 2 * a.n && !vis; ++j) {
+                float Tcw[12];
+#pragma unroll
+                for (int i = 0; i < 12; ++i) Tcw[i] = refuse_pose(a, j)[i];
+                vis = block_visible<kFast>(p.x, p.y, p.z, Tcw, a.K, a.cols, a.rows, m.zmin_update, m.zmax_update,
+                                           scale);
+            }
+        }
+        const unsigned long long ballot = __ballot(vis);
+        if (lane == 0) s_cnt[wave] = (unsigned int)__popcll(ballot);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned int total = 0;
+            for (int w = 0; w < kVisThreads / kWave; ++w) {
+                const unsigned int c = s_cnt[w];
+                s_cnt[w] = total;
+                total += c;
+            }
+            s_base = total ? atomicAdd(&m.counters[LF_CNT_VISIBLE], total) : 0u;
+        }
+        __syncthreads();
+        if (vis) {
+            const unsigned int slot = s_base + s_cnt[wave] + __popcll(ballot & ((1ull << lane) - 1ull));
+            m.visible[slot] = (int)b;
+            reinterpret_cast<int4 *>(m.visible_pos)[slot] = make_int4(p.x, p.y, p.z, (int)b);
+        }
+        __syncthreads();
+    }
+}
+
+// Keyframe k's observation of a voxel at world point (wx, wy, wz) under the
+// float 3x4 T_cw: the weighted update's selection (pixel in the image, depth
+// within the update range, sdf >= -truncation), its clamped sdf in truncation
+// units, its weight and colour.  kFast: the hardware reciprocal projection of
+// the mixed-precision update.
+template <bool kFast>
+LF_DEV bool refuse_observe(const RefuseArgs &a, int k, const float *Tcw, float wx, float wy, float wz, float &sdf,
+                           float &wt, unsigned int &rgb) {
+    const MapDev &m = a.m;
+    const f3 pt = xform(Tcw, f3{wx, wy, wz});
+    int u, v;
+    if (kFast) {
+        const float iz = __builtin_amdgcn_rcpf(pt.z);
+        u = proj_floor(a.K.fx * pt.x * iz + a.K.cx + 0.5f);
+        v = proj_floor(a.K.fy * pt.y * iz + a.K.cy + 0.5f);
+    } else {
+        u = proj_floor(a.K.fx * pt.x / pt.z + a.K.cx + 0.5f);
+        v = proj_floor(a.K.fy * pt.y / pt.z + a.K.cy + 0.5f);
+    }
+    if (u < 0 || v < 0 || u > a.cols - 1 || v > a.rows - 1) return false;
+    const float4 smp = a.samples[k][(size_t)v * a.cols + u];
+    const float d = smp.x;
+    if (isnan(d) || d > m.zmax_update || d < m.zmin_update) return false;
+    const float s = d - pt.z;
+    if (s < -m.truncation) return false;
+    sdf = fminf(1.0f, s * (1.0f / m.truncation));
+    wt = smp.y;
+    rgb = __float_as_uint(smp.z);
+    return true;
+}
+
+// One lane per voxel (two voxels per lane, 256 lanes per block) of the visible
+// list.  kMixed: FP16 sdf and the packed weight-code / colour word, the
+// weights rounded stochastically once at the end (as the update stores them).
+template <bool kColour, bool kMixed>
+__global__ __launch_bounds__(256) void k_refuse(RefuseArgs a) {
+    const MapDev &m = a.m;
+    const int nvis = (int)m.counters[LF_CNT_VISIBLE];
+    const unsigned int pass = kMixed ? m.counters[LF_CNT_FUSE_PASSES] : 0u;
+    const float vs = m.voxel_size, scale = 8.0f * vs;
+    const int t = threadIdx.x, lane = lane_id();
+    const int lx = t & 7, ly = (t >> 3) & 7, lz0 = t >> 6;
+    for (int i = blockIdx.x; i < nvis; i += gridDim.x) {
+        const int4 p = reinterpret_cast<const int4 *>(m.visible_pos)[i];
+        const int b = p.w;
+        // which poses see the block: lane j < 2n of each wavefront tests pose j
+        bool sees = false;
+        if (lane < 2 * a.n) {
+            float Tcw[12];
+#pragma unroll
+            for (int c = 0; c < 12; ++c) Tcw[c] = refuse_pose(a, lane)[c];
+            sees = block_visible<kMixed>(p.x, p.y, p.z, Tcw, a.K, a.cols, a.rows, m.zmin_update, m.zmax_update,
+                                         scale);
+        }
+        const int slot = reinterpret_cast<const int4 *>(m.block_pos)[b].w;
+        const bool flagged = m.slot_flag[slot] != 0;
+        const unsigned long long marks = a.slot_mask[slot], all = (1ull << a.n) - 1ull;
+        const unsigned long long seen = __ballot(sees) | (marks & ~kRefuseFresh);
+        unsigned long long old_mask = seen & all, new_mask = (seen >> a.n) & all;
+        if (marks & kRefuseFresh) {
+            // created here: by the first keyframe whose new-pose walk reached it
+            const unsigned long long walks = (marks >> a.n) & all;
+            old_mask = 0ull;
+            new_mask &= ~((walks & (0ull - walks)) - 1ull);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int lz = lz0 + 4 * h;
+            const float wx = (float)(p.x * 8 + lx) * vs, wy = (float)(p.y * 8 + ly) * vs, wz = (float)(p.z * 8 + lz) * vs;
+            const unsigned int o = (unsigned int)(t + 256 * h);
+            const size_t vi = (size_t)b * LF_BLOCK3 + o;
+            unsigned short *sdf16 = reinterpret_cast<unsigned short *>(m.sdf);
+            unsigned int *wd = reinterpret_cast<unsigned int *>(m.weight);
+            unsigned char *c = m.rgb + (size_t)b * (3 * LF_BLOCK3) + o;
+            // the voxel's running mean (sdf, colour) and weight
+            float w, sdf, col[3] = {0.0f, 0.0f, 0.0f};
+            unsigned int word = 0u;
+            if constexpr (kMixed) {
+                word = wd[vi];
+                w = weight_value(word_code(word));
+                sdf = w > 0.0f ? half_bits_to_float(sdf16[vi]) : 0.0f;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) col[k] = (float)word_channel(word, k);
+            } else {
+                w = m.weight[vi];
+                sdf = w > 0.0f ? unpack_sdf(m.sdf[vi]) : 0.0f;
+                if (kColour) {
+                    col[0] = c[0]; col[1] = c[LF_BLOCK3]; col[2] = c[2 * LF_BLOCK3];
+                }
+            }
+            bool changed = false;
+            // de-integration at the old poses (k_deintegrate's steps)
+            for (unsigned long long mk = old_mask; mk; mk &= mk - 1ull) {
+                const int k = __builtin_ctzll(mk);
+                float s, wt;
+                unsigned int rgb;
+                if (w == 0.0f || !refuse_observe<kMixed>(a, k, a.old24[k] + 12, wx, wy, wz, s, wt, rgb)) continue;
+                changed = true;
+                const float wr = w - wt;
+                const bool keep = kMixed ? wr > w * (1.0f / 16.0f) : wr > 0.0f;
+                if (!keep) {
+                    w = 0.0f;
+                    sdf = 0.0f;
+                    if (kColour) col[0] = col[1] = col[2] = 0.0f;
+                    continue;
+                }
+                sdf = fmaxf(-1.0f, fminf(1.0f, (sdf * w - s * wt) / wr));
+                if (kColour) {
+#pragma unroll
+                    for (int q = 0; q < 3; ++q)
+                        col[q] = fmaxf(0.0f, fminf(255.0f, (col[q] * w - (float)((rgb >> (8 * q)) & 0xFFu) * wt) / wr));
+                }
+                w = wr;
+            }
+            // the weighted update at the new poses
+            for (unsigned long long mk = new_mask; mk; mk &= mk - 1ull) {
+                const int k = __builtin_ctzll(mk);
+                float s, wt;
+                unsigned int rgb;
+                if (!refuse_observe<kMixed>(a, k, a.new24[k] + 12, wx, wy, wz, s, wt, rgb)) continue;
+                changed = true;
+                if (w == 0.0f) {
+                    sdf = s;
+                    w = fminf(wt, 255.0f);
+                    if (kColour) {
+#pragma unroll
+                        for (int q = 0; q < 3; ++q) col[q] = (float)((rgb >> (8 * q)) & 0xFFu);
+                    }
+                    continue;
+                }
+                const float wn = w + wt;
+                sdf = (sdf * w + s * wt) / wn;
+                if (kColour) {
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) col[q] = (col[q] * w + (float)((rgb >> (8 * q)) & 0xFFu) * wt) / wn;
+                }
+                w = fminf(wn, 255.0f);
+            }
+            if (!changed) continue;
+            if constexpr (kMixed) {
+                unsigned int colour = word & ~0xFFu;
+                if (kColour)
+                    colour = word_colour_bits((unsigned int)(int)col[0], (unsigned int)(int)col[1],
+                                              (unsigned int)(int)col[2]);
+                // a voxel left with a positive weight stays seen (the smallest
+                // code stands for weights below it); none left: unseen
+                const unsigned int code =
+                    w > 0.0f ? max(1u, weight_code_sr(fminf(w, 255.0f), voxel_rand(p.x, p.y, p.z, o, pass))) : 0u;
+                sdf16[vi] = code ? float_to_half_bits(sdf) : kHalfUnseen;
+                wd[vi] = code | (code ? colour : (kColour ? 0u : colour));
+            } else {
+                m.sdf[vi] = w > 0.0f ? pack_sdf(sdf) : kSdfUnseen;
+                m.weight[vi] = w;
+                if (kColour) {
+                    c[0] = (unsigned char)(int)col[0];
+                    c[LF_BLOCK3] = (unsigned char)(int)col[1];
+                    c[2 * LF_BLOCK3] = (unsigned char)(int)col[2];
+                }
+            }
+        }
+        __syncthreads();  // every lane has read the slot flag
+        if (t == 0 && flagged) m.slot_flag[slot] = 0;
+    }
+}
+
 // one map at 640x480 / 5 mm (~40k visible blocks): 4096 workgroups measured
 // 70 us, 8192 and 16384 both 66 us
 int integrate_grid() { return 8192; }
@@ -850,6 +1146,67 @@ int lf_update_batch_launch(const void *jobs_dev, const LfFuseBatchDims &d, hipSt
     else k_integrate_batch<false><<<dim3(d.integrate_grid, d.n), 256, 0, st>>>(J);
     LF_LAUNCH_CHECK();
     if (e1) LF_HIP_CHECK(hipEventRecord(e1, st));
+    return LF_OK;
+}
+
+// The three launches of a re-fusion (the mask array cleared before them).
+static int refuse_launch(const lf_map_params *p, const RefuseArgs &a, int colour, hipStream_t st) {
+    LF_LAUNCH_CHECK();  // the mask clear
+    const bool mixed = p->precision == LF_PRECISION_MIXED;
+    const dim3 g(((a.cols + 15) / 16) * ((a.rows + 15) / 16), 2 * a.n);
+    if (mixed) k_refuse_alloc<true><<<g, 256, 0, st>>>(a);
+    else k_refuse_alloc<false><<<g, 256, 0, st>>>(a);
+    LF_LAUNCH_CHECK();
+    if (mixed) k_refuse_visible<true><<<visible_grid(p), kVisThreads, 0, st>>>(a);
+    else k_refuse_visible<false><<<visible_grid(p), kVisThreads, 0, st>>>(a);
+    LF_LAUNCH_CHECK();
+    const int G = integrate_grid();
+    if (mixed) {
+        if (colour) k_refuse<true, true><<<G, 256, 0, st>>>(a);
+        else k_refuse<false, true><<<G, 256, 0, st>>>(a);
+    } else {
+        if (colour) k_refuse<true, false><<<G, 256, 0, st>>>(a);
+        else k_refuse<false, false><<<G, 256, 0, st>>>(a);
+    }
+    LF_LAUNCH_CHECK();
+    return LF_OK;
+}
+
+int lf_refuse_ev(const lf_map_params *p, const lf_map_storage *s, const float *const *depth,
+                 const float *const *samples, const float *const *old_pose24, const float *const *new_pose24, int n,
+                 int cols, int rows, const lf_intrinsics *K, int colour, hipStream_t st) {
+    if (!p || !s || !depth || !samples || !old_pose24 || !new_pose24 || !K)
+        return lf_set_error(LF_ERR_ARG, "null argument to keyframe re-fusion");
+    if (n < 1 || n > kMaxRefuse)
+        return lf_set_error(LF_ERR_ARG, "keyframe re-fusion takes 1..%d keyframes, not %d", kMaxRefuse, n);
+    if (cols <= 0 || rows <= 0) return lf_set_error(LF_ERR_ARG, "bad image size %dx%d", cols, rows);
+    if (int rc = lf_check_storage(s, p)) return rc;
+    if (colour && p->precision != LF_PRECISION_MIXED && !s->rgb)
+        return lf_set_error(LF_ERR_ARG, "colour re-fusion needs the map's colour planes");
+    RefuseArgs a = {};
+    a.m = make_dev(p, s);
+    for (int k = 0; k < n; ++k) {
+        if (!depth[k] || !samples[k] || !old_pose24[k] || !new_pose24[k])
+            return lf_set_error(LF_ERR_ARG, "null input of keyframe %d", k);
+        a.depth[k] = depth[k];
+        a.samples[k] = reinterpret_cast<const float4 *>(samples[k]);
+        a.old24[k] = old_pose24[k];
+        a.new24[k] = new_pose24[k];
+    }
+    a.n = n;
+    a.cols = cols;
+    a.rows = rows;
+    a.K = *K;
+    // the per-slot masks: stream-ordered device memory for the call's length
+    const size_t E = (size_t)p->num_entries;
+    void *mask = nullptr;
+    LF_HIP_CHECK(hipMallocAsync(&mask, E * sizeof(unsigned long long), st));
+    a.slot_mask = static_cast<unsigned long long *>(mask);
+    k_fill_u64<<<(int)std::min<size_t>(4096, (E + 255) / 256), 256, 0, st>>>(a.slot_mask, E);
+    const int rc = refuse_launch(p, a, colour, st);
+    const hipError_t ef = hipFreeAsync(mask, st);
+    if (rc) return rc;
+    if (ef != hipSuccess) return lf_set_error(LF_ERR_HIP, "hipFreeAsync failed: %s", hipGetErrorString(ef));
     return LF_OK;
 }
 

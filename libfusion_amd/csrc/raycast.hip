@@ -638,6 +638,10 @@ int lf_raycast_batch_make(const LfRayItem *items, int n, void *jobs_host, LfRayB
         lds_max = std::max(lds_max, 2 * (size_t)T * sizeof(int));
     }
     d.zrange_lds = lds_max;
+    // n maps share the z-range launch: each takes 1/n of a lone map's grid
+    // (at least 64 workgroups), so that a workgroup's LDS tile grid, cleared
+    // and flushed once, collects several passes of blocks
+    d.zgrid_max = std::max(std::min(d.zgrid_max, 64), d.zgrid_max / n);
     *dims = d;
     return LF_OK;
 }

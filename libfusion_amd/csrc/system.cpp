@@ -742,6 +742,11 @@ int Batch::set_option(int option, int value) {
             return lf_set_error(LF_ERR_ARG, "integration workgroups %d not in 0..65535", value);
         integrate_wgs_ = value;
         return LF_OK;
+    case LF_BATCH_OPT_ZRANGE_WORKGROUPS:
+        if (value < 0 || value > 65535)
+            return lf_set_error(LF_ERR_ARG, "z-range workgroups %d not in 0..65535", value);
+        zrange_wgs_ = value;
+        return LF_OK;
     default:
         return lf_set_error(LF_ERR_ARG, "unknown batch option %d", option);
     }
@@ -752,6 +757,7 @@ int Batch::get_option(int option, int *value) const {
     case LF_BATCH_OPT_UPLOAD_OVERLAP: *value = upload_overlap_; return LF_OK;
     case LF_BATCH_OPT_TRACK_WORKGROUPS: *value = track_wgs_; return LF_OK;
     case LF_BATCH_OPT_INTEGRATE_WORKGROUPS: *value = integrate_wgs_; return LF_OK;
+    case LF_BATCH_OPT_ZRANGE_WORKGROUPS: *value = zrange_wgs_; return LF_OK;
     default: return lf_set_error(LF_ERR_ARG, "unknown batch option %d", option);
     }
 }
@@ -880,6 +886,7 @@ int Batch::step(const uint16_t *const *depth_in, const unsigned char *const *rgb
     if (integrate_wgs_ > 0) fd.integrate_grid = integrate_wgs_;
     LfRayBatchDims ray;
     LF_RC(lf_raycast_batch_make(items.data(), n, host + off_ray, &ray));
+    if (zrange_wgs_ > 0) ray.zgrid_max = zrange_wgs_;
     // Upload stream: the table, then the uploads.  The uploads rewrite the
     // images that the previous step's tracking read (the previous step's
     // fusion, raycast and re-pyramid use the other image of each system), and
@@ -1255,6 +1262,30 @@ extern "C" int lf_deintegrate_image(const lf_map_params *p, const lf_map_storage
     return lf_deintegrate_ev(p, s, d.get_raw_depth().ptr<float>(), d.get_nmap(0).ptr<float>(),
                              colour ? d.get_image().ptr<unsigned char>() : nullptr, K.width, K.height, pose24, &K,
                              lf_stream(stream), d.get_integration_samples().ptr<float>());
+}
+
+extern "C" int lf_refuse_keyframes(const lf_map_params *p, const lf_map_storage *s, lf_image *const *frames,
+                                   const float *const *old_pose24, const float *const *new_pose24, int n, int colour,
+                                   void *stream) {
+    if (!p || !s || !frames || !old_pose24 || !new_pose24)
+        return lf_set_error(LF_ERR_ARG, "null argument to lf_refuse_keyframes");
+    if (n < 1 || n > LF_REFUSE_MAX_KEYFRAMES)
+        return lf_set_error(LF_ERR_ARG, "lf_refuse_keyframes takes 1..%d keyframes, not %d", LF_REFUSE_MAX_KEYFRAMES, n);
+    const float *depth[LF_REFUSE_MAX_KEYFRAMES], *samples[LF_REFUSE_MAX_KEYFRAMES];
+    const lf_intrinsics *K0 = nullptr;
+    for (int k = 0; k < n; ++k) {
+        if (!frames[k]) return lf_set_error(LF_ERR_ARG, "null keyframe %d", k);
+        fusion::DeviceImage &d = *frames[k]->img;
+        const lf_intrinsics &K = d.intrinsics()[0];
+        if (!K0) K0 = &d.intrinsics()[0];
+        else if (K.width != K0->width || K.height != K0->height || K.fx != K0->fx || K.fy != K0->fy ||
+                 K.cx != K0->cx || K.cy != K0->cy)
+            return lf_set_error(LF_ERR_ARG, "keyframe %d's size or intrinsics differ from keyframe 0's", k);
+        depth[k] = d.get_raw_depth().ptr<float>();
+        samples[k] = d.get_integration_samples().ptr<float>();
+    }
+    return lf_refuse_ev(p, s, depth, samples, old_pose24, new_pose24, n, K0->width, K0->height, K0, colour,
+                        lf_stream(stream));
 }
 
 // ------------------------------------------------------------ lf_system_*

@@ -329,9 +329,11 @@ LF_DEV float read_sdf(const RayDev &r, BlockCache &bc, f3 p, bool &valid, int *v
 
 // read_sdf_interped (ray_tracing.cu:307-339).  Corner k is the voxel of
 // p + offset_k with offsets 0 or 1 per axis, so each axis has only two
-// coordinates, floor(p) and floor(p + 1), computed once; the 8 corners'
-// blocks are resolved first, then all 8 sdf loads are issued in one round
-// trip; the arithmetic is the reference's order.
+// coordinates, floor(p) and floor(p + 1), computed once, and the corners lie
+// in at most two blocks per axis: each distinct block is resolved once (one
+// lookup when no axis crosses a block face, the common case), then all 8
+// sdf loads are issued in one round trip; the arithmetic is the reference's
+// order.
 template <bool kMixed>
 LF_DEV float read_interp(const RayDev &r, BlockCache &bc, f3 p, bool &valid) {
     const float fx = floorf(p.x), fy = floorf(p.y), fz = floorf(p.z);
@@ -339,12 +341,23 @@ LF_DEV float read_interp(const RayDev &r, BlockCache &bc, f3 p, bool &valid) {
     const int cx[2] = {(int)fx, ray_floor(p.x + 1.0f)}, cy[2] = {(int)fy, ray_floor(p.y + 1.0f)},
               cz[2] = {(int)fz, ray_floor(p.z + 1.0f)};
     // corner order of the reference: (0,0,0) (1,0,0) (0,1,0) (1,1,0) (0,0,1) (1,0,1) (0,1,1) (1,1,1)
+    const int bx0 = cx[0] >> 3, bx1 = cx[1] >> 3, by0 = cy[0] >> 3, by1 = cy[1] >> 3, bz0 = cz[0] >> 3,
+              bz1 = cz[1] >> 3;
+    const bool sx = bx1 == bx0, sy = by1 == by0, sz = bz1 == bz0;
+    int bb[8];
+    bb[0] = resolve(r, bc, bx0, by0, bz0);
+    bb[1] = sx ? bb[0] : resolve(r, bc, bx1, by0, bz0);
+    bb[2] = sy ? bb[0] : resolve(r, bc, bx0, by1, bz0);
+    bb[3] = sy ? bb[1] : sx ? bb[2] : resolve(r, bc, bx1, by1, bz0);
+    bb[4] = sz ? bb[0] : resolve(r, bc, bx0, by0, bz1);
+    bb[5] = sz ? bb[1] : sx ? bb[4] : resolve(r, bc, bx1, by0, bz1);
+    bb[6] = sz ? bb[2] : sy ? bb[4] : resolve(r, bc, bx0, by1, bz1);
+    bb[7] = sz ? bb[3] : sy ? bb[5] : sx ? bb[6] : resolve(r, bc, bx1, by1, bz1);
     int vi[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int x = cx[k & 1], y = cy[(k >> 1) & 1], z = cz[k >> 2];
-        const int b = resolve(r, bc, x >> 3, y >> 3, z >> 3);
-        vi[k] = b < 0 ? -1 : b * LF_BLOCK3 + lidx(x, y, z);
+        vi[k] = bb[k] < 0 ? -1 : bb[k] * LF_BLOCK3 + lidx(x, y, z);
     }
     short c[8];
 #pragma unroll

@@ -473,8 +473,8 @@ def main():
     t = head[0]["timing"]
     ig_n, ig_ms = t[capi.BATCH_INTEGRATE]
     ig_bytes = integrate_bytes(head[0]["vis_blocks_per_step"], n * P, 6)
-    rl = roofline(ig_bytes, ig_n, ig_ms, "k_integrate_batch<true> (mixed: FP16 sdf + 8-bit weight + 3 x u8 colour, "
-                  "%d maps per launch)" % n,
+    rl = roofline(ig_bytes, ig_n, ig_ms, "k_integrate_batch<true> (mixed: FP16 sdf + a 32-bit word of 8-bit weight "
+                  "code and colour per voxel, %d maps per launch)" % n,
                   {"visible_blocks_per_launch": round(head[0]["vis_blocks_per_step"], 1),
                    "basis": "per visible block 2*512*6 + 16 B (mixed voxel: FP16 sdf + 32-bit weight code / colour word), "
                            "per pixel 15 B"})
@@ -483,13 +483,16 @@ def main():
     rc_n, rc_ms = t[capi.BATCH_RAYCAST]
     touched = profile_value("raycast_batch_touched.json", "blocks_touched_mean_per_map")
     if touched:
-        # compulsory: vertex map (16 B) and colour (3 B) out and the hit
-        # voxel's colour word (4 B) per pixel, the FP16 sdf plane of the
-        # distinct blocks the march reads and their 20 B entries, z-range tiles
-        rb = n * (P * (16 + 3 + 4) + touched * (512 * 2 + 20) + 2 * (P // 64) * 4)
+        # compulsory: the vertex map out (16 B) and the hit voxel's colour
+        # word read (4 B: raycast_with_colour checks it) per pixel, the FP16
+        # sdf plane of the distinct blocks the march reads and their 20 B
+        # entries, the z-range tiles
+        rb = n * (P * (16 + 4) + touched * (512 * 2 + 20) + 2 * (P // 64) * 4)
         stage_rooflines["raycast"] = roofline(rb, rc_n, rc_ms, "k_raycast_batch<true, true> (%d maps per launch)" % n,
-                                              {"bound": "latency (dependent sample chain)",
+                                              {"bound": "VALU issue (dependent sample chain)",
                                                "blocks_touched_per_map": touched})
+        stage_rooflines["raycast"]["traffic"] = profile_value("raycast_batch_mixed_traffic.json",
+                                                              "hbm_bytes_per_launch")
     else:
         stage_rooflines["raycast"] = {"avg_launch_ms": round(rc_ms / max(rc_n, 1), 5), "launches": rc_n}
     pt = base[0]["timing"]

@@ -143,11 +143,12 @@ class Streams:
         return self.pinned
 
 
-def batch_runs(streams, cfgs, warmup, steps, runs, timing_first=0, host=False):
+def batch_runs(streams, cfgs, warmup, steps, runs, timing_first=0, host=False, graphs=False):
     """Aggregate frames/s of the streams stepped together by lf_batch: `runs`
     fresh batches over the same frames, each W untimed steps then exactly K
     timed steps bracketed by synchronisation.  The first run records the
-    stage events of timing_first (HIP events on the batch stream)."""
+    stage events of timing_first (HIP events on the batch stream).  graphs:
+    steady-state steps replayed from HIP graphs (lf_batch_set_graphs)."""
     import torch
 
     from libfusion_amd import capi
@@ -156,6 +157,8 @@ def batch_runs(streams, cfgs, warmup, steps, runs, timing_first=0, host=False):
     out = []
     for r in range(runs):
         b = capi.Batch(cfgs)
+        if graphs:
+            b.set_graphs(True)
         step = (lambda f: b.process_host([d[f] for d, _ in src], [c[f] for _, c in src])) if host else \
             (lambda f: b.process([d[f] for d, _ in src], [c[f] for _, c in src]))
         for f in range(warmup):
@@ -508,10 +511,17 @@ def main():
         extras["stages_ms"] = {names[s]: round(ms / max(k, 1), 4) for s, (k, ms) in st.items()}
         pb = batch_runs(streams, cfgs(plain), W, K, 1)[0]
         ms = single_runs(streams, c1_params(mixed), W, K, 1)[0]
+        gr = batch_runs(streams, cfgs(mixed), W, K, args.runs, graphs=True)
+        fps_graph = median([r["fps"] for r in gr])
         extras["breakdown"] = {
             "plain_fp32_batched": {"value": round(pb["fps"], 1), "vs_plain_fp32": round(pb["fps"] / fps_plain, 3)},
-            "mixed_single_systems": {"value": round(ms["fps"], 1), "vs_plain_fp32": round(ms["fps"] / fps_plain, 3)}}
-        log("breakdown: plain batched %.1f, mixed single %.1f" % (pb["fps"], ms["fps"]))
+            "mixed_single_systems": {"value": round(ms["fps"], 1), "vs_plain_fp32": round(ms["fps"] / fps_plain, 3)},
+            "mixed_batched_graph_replay": {"value": round(fps_graph, 1), "runs": [round(r["fps"], 1) for r in gr],
+                                           "vs_plain_fp32": round(fps_graph / fps_plain, 3),
+                                           "note": "the headline's steps replayed from HIP graphs "
+                                                   "(lf_batch_set_graphs); the headline runs them eagerly"}}
+        log("breakdown: plain batched %.1f, mixed single %.1f, mixed batched with graphs %.1f"
+            % (pb["fps"], ms["fps"], fps_graph))
         # the same headline step with every frame copied from pinned host memory
         ph = batch_runs(streams, cfgs(mixed), W, K, 1, host=True)[0]
         extras["pcie_inclusive"] = {

@@ -363,8 +363,21 @@ public:
     // leave every system's results unchanged up to summation order.
     int set_option(int option, int value);
     int get_option(int option, int *value) const;
+    // Replay steady-state steps from HIP graphs: the step's table copy and
+    // uploads, its Gauss-Newton launches, and its fusion / raycast /
+    // re-pyramid launches, each captured once per table buffer and launch
+    // shape.  Steps with timing on, or where a system starts a frame, run
+    // eagerly.  Results are those of the eager steps.
+    void set_graphs(bool on);
 
 private:
+    struct StepGraphs {
+        hipGraphExec_t up = nullptr, track = nullptr, fuse = nullptr;
+        std::vector<long long> sig;  // the launch shapes they were captured with
+        void drop();
+    };
+    bool graphs_ = false;
+    StepGraphs graphs_k_[2];  // per table buffer
     int step(const uint16_t *const *depth, const unsigned char *const *rgb, bool from_host);
     int upload_overlap_ = 1;  // uploads on aux_ beside the previous step (1) or on stream_ (0)
     int track_wgs_ = 0;       // tracking workgroups per system (0: the share rule)

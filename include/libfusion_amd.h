@@ -609,11 +609,21 @@ int lf_batch_read_timing(lf_batch *b, int stage, int *launches, double *total_ms
 #define LF_BATCH_OPT_ZRANGE_WORKGROUPS 3
 int lf_batch_set_option(lf_batch *b, int option, int value);
 int lf_batch_get_option(lf_batch *b, int option, int *value);
+/* Replay the batch's steady-state steps from HIP graphs (enable != 0; no
+ * reference counterpart): a step's table copy and uploads, its Gauss-Newton
+ * launches and its fusion / raycast / re-pyramid launches are captured as
+ * three graphs per table buffer and launch shape on first use and replayed
+ * after; the events joining the upload and batch streams stay outside them.
+ * Results are those of the eager steps; steps with timing on, or in which a
+ * system starts a frame, run eagerly. */
+int lf_batch_set_graphs(lf_batch *b, int enable);
 /* System i's camera pose (4x4 row-major doubles; synchronising), last
  * tracking result and map counters. */
 int lf_batch_get_pose(lf_batch *b, int i, double *out16);
 int lf_batch_get_tracking(lf_batch *b, int i, int *success, float *icp_error);
 int lf_batch_read_counters(lf_batch *b, int i, unsigned int *out16);
+/* System i's map (the arrays of its volume, as lf_system_map). */
+int lf_batch_map(lf_batch *b, int i, lf_map_params *p, lf_map_storage *s);
 
 #ifdef __cplusplus
 }

@@ -161,6 +161,8 @@ SIGNATURES = [
     ("lf_batch_synchronize", I, [P]),
     ("lf_batch_set_timing", I, [P, I]),
     ("lf_batch_set_option", I, [P, I, I]),
+    ("lf_batch_set_graphs", I, [P, I]),
+    ("lf_batch_map", I, [P, I, P, P]),
     ("lf_batch_get_option", I, [P, I, P]),
     ("lf_batch_read_timing", I, [P, I, P, P]),
     ("lf_batch_get_pose", I, [P, I, P]),
@@ -653,6 +655,10 @@ class Batch:
         check(load().lf_batch_read_timing(self.h, int(stage), ctypes.byref(n), ctypes.byref(t)))
         return n.value, t.value
 
+    def set_graphs(self, on=True):
+        """Replay steady-state steps from HIP graphs (lf_batch_set_graphs)."""
+        check(load().lf_batch_set_graphs(self.h, 1 if on else 0))
+
     def set_option(self, option, value):
         """A launch option (BATCH_OPT_*)."""
         check(load().lf_batch_set_option(self.h, int(option), int(value)))
@@ -677,6 +683,13 @@ class Batch:
         out = np.zeros(16, np.uint32)
         check(load().lf_batch_read_counters(self.h, int(i), out.ctypes.data_as(P)))
         return out
+
+    def map_storage(self, i):
+        """System i's (params, storage) (lf_batch_map)."""
+        p = LfMapParams()
+        st = LfMapStorage()
+        check(load().lf_batch_map(self.h, int(i), ctypes.byref(p), ctypes.byref(st)))
+        return p, st
 
 
 class System:

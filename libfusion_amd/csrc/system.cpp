@@ -665,9 +665,27 @@ Batch::Batch(const lf_system_config *cfgs, int n) {
 
 Batch::~Batch() { release(); }
 
+void Batch::StepGraphs::drop() {
+    for (hipGraphExec_t *e : {&up, &track, &fuse}) {
+        if (*e) (void)hipGraphExecDestroy(*e);
+        *e = nullptr;
+    }
+    sig.clear();
+}
+
+void Batch::set_graphs(bool on) {
+    graphs_ = on;
+    if (!on) {
+        if (stream_) (void)hipStreamSynchronize(stream_);
+        if (aux_) (void)hipStreamSynchronize(aux_);
+        for (StepGraphs &g : graphs_k_) g.drop();
+    }
+}
+
 void Batch::release() {
     if (aux_) (void)hipStreamSynchronize(aux_);
     if (stream_) (void)hipStreamSynchronize(stream_);
+    for (StepGraphs &g : graphs_k_) g.drop();
     for (auto &s : sys_)
         if (s) (void)hipStreamSynchronize(s->stream());
     for (hipEvent_t e : ready_)
@@ -821,6 +839,7 @@ int Batch::step(const uint16_t *const *depth_in, const unsigned char *const *rgb
     std::vector<std::vector<unsigned char>> jobs(n), up(n), after(n);
     std::vector<std::vector<int>> grids(n);
     size_t steps = 0, n_up = 0, n_after = 0;
+    bool any_start = false;  // a system starts its frame on its own stream (no steps)
     for (int i = 0; i < n; ++i) {
         DenseOdometry &od = sys_[i]->odometry();
         const bool tracks = od.initialized() && !od.trackingLost;
@@ -833,6 +852,7 @@ int Batch::step(const uint16_t *const *depth_in, const unsigned char *const *rgb
         // without steps (first frame: the initial pose); the batch's tracking
         // waits for that, and needs no event otherwise
         if (!tracks || grids[i].empty()) {
+            any_start = true;
             LF_HIP_CHECK(hipEventRecord(ready_[i], sys_[i]->stream()));
             LF_HIP_CHECK(hipStreamWaitEvent(stream_, ready_[i], 0));
         }
@@ -895,6 +915,90 @@ int Batch::step(const uint16_t *const *depth_in, const unsigned char *const *rgb
     // Without the overlap (upload_overlap_ 0) they run on the batch stream
     // ahead of the step's tracking.
     hipStream_t up_st = upload_overlap_ ? aux_ : stream_;
+    // The step's device work in three parts: the table copy and uploads (on
+    // up_st), the Gauss-Newton launches, then fusion, raycast and re-pyramids
+    // (on the batch stream).  Run eagerly, or captured once per table buffer
+    // and shape as three graphs and replayed (set_graphs); the cross-stream
+    // events stay outside the graphs, so the overlap is the same.
+    auto enqueue_up = [&](hipStream_t st) -> int {
+        LF_HIP_CHECK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, st));
+        for (size_t p = 0; p < n_up; ++p) LF_RC(lf_pass_batch(dev + p * n * PB, n, wmax[p], st));
+        return LF_OK;
+    };
+    auto enqueue_track = [&]() -> int {
+        for (size_t s = 0; s < steps; ++s)
+            LF_RC(lf_track_batch_step(dev + off_track + s * n * J, n, gmax[s], half_maps_, stream_));
+        return LF_OK;
+    };
+    auto enqueue_fuse = [&](hipEvent_t i0, hipEvent_t i1, hipEvent_t f0, hipEvent_t f1, hipEvent_t r0,
+                            hipEvent_t r1) -> int {
+        if (f0) LF_HIP_CHECK(hipEventRecord(f0, stream_));
+        LF_RC(lf_update_batch_launch(dev + off_fuse, fd, stream_, i0, i1));
+        if (f1) LF_HIP_CHECK(hipEventRecord(f1, stream_));
+        LF_RC(lf_raycast_batch_launch(dev + off_ray, ray, 1, stream_, r0, r1));
+        for (size_t p = n_up; p < n_pass; ++p) LF_RC(lf_pass_batch(dev + p * n * PB, n, wmax[p], stream_));
+        return LF_OK;
+    };
+    if (graphs_ && timing_ == 0 && !any_start) {
+        // the launch shapes and the table's address decide the graphs
+        std::vector<long long> sig = {n, (long long)n_up, (long long)n_after, (long long)steps, (long long)bytes,
+                                      (long long)(uintptr_t)dev, half_maps_ ? 1 : 0, upload_overlap_,
+                                      fd.n, fd.any, fd.mixed, fd.tiles_max, fd.vis_grid_max, fd.integrate_grid,
+                                      ray.n, ray.mixed, ray.tiles_max, ray.zgrid_max, ray.grid_max,
+                                      (long long)ray.zrange_lds};
+        sig.insert(sig.end(), gmax.begin(), gmax.end());
+        sig.insert(sig.end(), wmax.begin(), wmax.end());
+        StepGraphs &g = graphs_k_[k];
+        if (g.sig != sig) {
+            g.drop();
+            auto capture = [&](hipStream_t st, hipGraphExec_t &exec, auto &&enqueue) -> int {
+                hipGraph_t graph = nullptr;
+                LF_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                const int rc = enqueue();
+                const hipError_t ec = hipStreamEndCapture(st, &graph);
+                if (rc != LF_OK) {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    return rc;
+                }
+                if (ec != hipSuccess)
+                    return lf_set_error(LF_ERR_HIP, "batch step capture failed: %s", hipGetErrorString(ec));
+                const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                if (ei != hipSuccess) {
+                    exec = nullptr;
+                    return lf_set_error(LF_ERR_HIP, "batch step graph instantiation failed: %s",
+                                        hipGetErrorString(ei));
+                }
+                return LF_OK;
+            };
+            LF_RC(capture(up_st, g.up, [&]() { return enqueue_up(up_st); }));
+            LF_RC(capture(stream_, g.track, [&]() { return enqueue_track(); }));
+            LF_RC(capture(stream_, g.fuse,
+                          [&]() { return enqueue_fuse(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr); }));
+            g.sig = sig;
+        }
+        if (upload_overlap_ && tracked_once_) LF_HIP_CHECK(hipStreamWaitEvent(aux_, tracked_, 0));
+        if (from_host)
+            for (int i = 0; i < n; ++i) {
+                LF_HIP_CHECK(hipMemcpyAsync(depth_in_[i].ptr<void>(), depth_in[i], depth_in_[i].bytes(),
+                                            hipMemcpyHostToDevice, up_st));
+                LF_HIP_CHECK(hipMemcpyAsync(rgb_in_[i].ptr<void>(), rgb_in[i], rgb_in_[i].bytes(),
+                                            hipMemcpyHostToDevice, up_st));
+            }
+        LF_HIP_CHECK(hipGraphLaunch(g.up, up_st));
+        if (upload_overlap_) {
+            LF_HIP_CHECK(hipEventRecord(uploaded_, aux_));
+            LF_HIP_CHECK(hipStreamWaitEvent(stream_, uploaded_, 0));
+        }
+        LF_HIP_CHECK(hipGraphLaunch(g.track, stream_));
+        LF_HIP_CHECK(hipEventRecord(tracked_, stream_));
+        tracked_once_ = true;
+        LF_HIP_CHECK(hipGraphLaunch(g.fuse, stream_));
+        LF_HIP_CHECK(hipEventRecord(table_read_[k], stream_));
+        table_used_[k] = true;
+        parity_ ^= 1;
+        return LF_OK;
+    }
     if (upload_overlap_ && tracked_once_) LF_HIP_CHECK(hipStreamWaitEvent(aux_, tracked_, 0));
     LF_HIP_CHECK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, up_st));
     if (from_host) {
@@ -924,18 +1028,13 @@ int Batch::step(const uint16_t *const *depth_in, const unsigned char *const *rgb
     }
     if (ev[4].a) LF_HIP_CHECK(hipEventRecord(ev[4].a, stream_));
     if (ev[2].a) LF_HIP_CHECK(hipEventRecord(ev[2].a, stream_));
-    for (size_t s = 0; s < steps; ++s)
-        LF_RC(lf_track_batch_step(dev + off_track + s * n * J, n, gmax[s], half_maps_, stream_));
+    LF_RC(enqueue_track());
     if (ev[2].b) LF_HIP_CHECK(hipEventRecord(ev[2].b, stream_));
     LF_HIP_CHECK(hipEventRecord(tracked_, stream_));
     tracked_once_ = true;
-    // block allocation, visible lists and integration of every map
-    if (ev[3].a) LF_HIP_CHECK(hipEventRecord(ev[3].a, stream_));
-    LF_RC(lf_update_batch_launch(dev + off_fuse, fd, stream_, ev[0].a, ev[0].b));
-    if (ev[3].b) LF_HIP_CHECK(hipEventRecord(ev[3].b, stream_));
-    LF_RC(lf_raycast_batch_launch(dev + off_ray, ray, 1, stream_, ev[1].a, ev[1].b));
-    // re-pyramids of the raycast maps
-    for (size_t p = n_up; p < n_pass; ++p) LF_RC(lf_pass_batch(dev + p * n * PB, n, wmax[p], stream_));
+    // block allocation, visible lists and integration of every map, the
+    // z-range and raycast, the re-pyramids of the raycast maps
+    LF_RC(enqueue_fuse(ev[0].a, ev[0].b, ev[3].a, ev[3].b, ev[1].a, ev[1].b));
     if (ev[4].b) LF_HIP_CHECK(hipEventRecord(ev[4].b, stream_));
     for (const Ev &e : ev) {
         // an event pair whose launch did not happen (nothing to fuse) is not a timing
@@ -1438,6 +1537,12 @@ extern "C" int lf_batch_process_host(lf_batch *b, const uint16_t *const *depth, 
     return b->b->process_host(depth, rgb);
 }
 
+extern "C" int lf_batch_set_graphs(lf_batch *b, int enable) {
+    if (!b) return lf_set_error(LF_ERR_ARG, "null batch");
+    b->b->set_graphs(enable != 0);
+    return LF_OK;
+}
+
 extern "C" int lf_batch_set_option(lf_batch *b, int option, int value) {
     if (!b) return lf_set_error(LF_ERR_ARG, "null batch");
     return b->b->set_option(option, value);
@@ -1481,6 +1586,14 @@ extern "C" int lf_batch_get_tracking(lf_batch *b, int i, int *success, float *ic
     fusion::TrackingResult r = b->b->system(i).odometry().tracker().result();
     if (success) *success = r.sucess ? 1 : 0;
     if (icp_error) *icp_error = r.icp_error;
+    return LF_OK;
+}
+
+extern "C" int lf_batch_map(lf_batch *b, int i, lf_map_params *p, lf_map_storage *st) {
+    if (int rc = batch_index(b, i)) return rc;
+    fusion::System &s = b->b->system(i);
+    if (p) *p = s.mapping().params();
+    if (st) *st = s.mapping().storage();
     return LF_OK;
 }
 

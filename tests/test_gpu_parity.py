@@ -1096,6 +1096,59 @@ def test_batch_of_one_equals_the_system_run_alone():
     batch.close()
 
 
+@pytest.mark.parametrize("precision", [capi.PRECISION_FP32, capi.PRECISION_MIXED])
+def test_batch_graph_replay_equals_eager_steps(precision):
+    """lf_batch_set_graphs: three systems (two image sizes, three voxel sizes)
+    stepped by a batch replaying its steps from HIP graphs and by an eager
+    batch, 12 frames, the upload stream overlapping or not and the launch
+    shapes changed midway (a new capture): after every step (host reads in
+    between, as the program's own work) each system's pose, tracking result
+    and counters equal the eager batch's bit for bit, and so do the volumes at
+    the end (blocks matched by coordinate: pool order is the allocation
+    race's)."""
+    specs = [(Scene(1, 1), 2, 0.01), (Scene(1, 7), 2, 0.008), (Scene(1, 9), 1, 0.012)]
+    runs = []
+    for sc, div, vs in specs:
+        K = capi.intrinsics(sc.width // div, sc.height // div, sc.fx / div, sc.fy / div,
+                            (sc.cx + 0.5) / div - 0.5, (sc.cy + 0.5) / div - 0.5)
+        p = capi.map_params(2 ** 18, 1 << 17, vs, 3 * vs, precision=precision)
+        fs = []
+        for f in range(12):
+            d16, rgb = sc.render(sc.pose(f), K.width, K.height, K.fx, K.fy, K.cx, K.cy, noise=True, frame_id=f)
+            fs.append((dev(d16), dev(rgb)))
+        runs.append((K, p, sc.depth_scale, fs))
+    torch.cuda.synchronize()
+    cfgs = [capi.system_config(K, p, 3, (10, 5, 3), ds) for K, p, ds, _ in runs]
+    eager, graphed = capi.Batch(cfgs), capi.Batch(cfgs)
+    graphed.set_graphs(True)
+    for f in range(12):
+        if f == 6:  # new launch shapes: the graphs are captured again
+            for b in (eager, graphed):
+                b.set_option(capi.BATCH_OPT_INTEGRATE_WORKGROUPS, 512)
+                b.set_option(capi.BATCH_OPT_UPLOAD_OVERLAP, 0)
+        for b in (eager, graphed):
+            b.process([r[3][f][0] for r in runs], [r[3][f][1] for r in runs])
+        eager.synchronize()
+        graphed.synchronize()
+        for i in range(len(runs)):
+            assert np.array_equal(graphed.pose(i), eager.pose(i)), (f, i)
+            tg, te = graphed.tracking(i), eager.tracking(i)
+            assert tg[0] == te[0] and np.array_equal(np.float32(tg[1]), np.float32(te[1]), equal_nan=True), (f, i)
+            assert np.array_equal(graphed.counters(i), eager.counters(i)), (f, i)
+    assert all(eager.tracking(i)[0] for i in range(len(runs)))
+    for i in range(len(runs)):
+        pe, se = eager.map_storage(i)
+        pg, sg = graphed.map_storage(i)
+        ve, vg = capi.read_tsdf(pe, se), capi.read_tsdf(pg, sg)
+        # pool order is the allocation race's: match blocks by coordinate
+        ie = {tuple(b): j for j, b in enumerate(ve[0].tolist())}
+        order = np.array([ie[tuple(b)] for b in vg[0].tolist()])
+        assert len(ie) == len(vg[0])
+        assert np.array_equal(ve[1][order], vg[1], equal_nan=True) and np.array_equal(ve[2][order], vg[2])
+    eager.close()
+    graphed.close()
+
+
 def test_batch_with_one_raycast_launch_equals_systems_run_alone():
     """lf_batch: three systems of different image sizes, voxel sizes and
     pyramid depths whose pyramid passes, tracking steps and raycasts share

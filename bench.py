@@ -259,6 +259,33 @@ def profile_value(name, key):
     return None
 
 
+def tuned_leg(streams, cfgs, warmup, steps, runs):
+    """lf_batch_autotune on the headline batch (one settling and one timed
+    step per candidate, one round, over the streams' frames in a loop), then
+    the headline measurement again: batches of this shape now start with the
+    tuned launch options."""
+    from libfusion_amd import capi
+    b = capi.Batch(cfgs)
+    b.autotune(1, 1)
+    f = 0
+    while b.tuning():
+        k = f % streams.frames
+        b.process([d[k] for d, _ in streams.dev], [c[k] for _, c in streams.dev])
+        f += 1
+    b.synchronize()
+    names = {capi.BATCH_OPT_INTEGRATE_WORKGROUPS: "integrate_workgroups_per_map",
+             capi.BATCH_OPT_TRACK_WORKGROUPS: "tracking_workgroups_per_system",
+             capi.BATCH_OPT_ZRANGE_WORKGROUPS: "zrange_workgroups_per_map",
+             capi.BATCH_OPT_UPLOAD_OVERLAP: "upload_overlap"}
+    chosen = {name: b.option(o) for o, name in names.items()}
+    b.close()
+    rs = batch_runs(streams, cfgs, warmup, steps, runs)
+    fps = median([r["fps"] for r in rs])
+    return {"value": round(fps, 1), "runs": [round(r["fps"], 1) for r in rs], "tuning_steps": f,
+            "options": chosen, "note": "0 = the built-in rule (1024 integration workgroups per map, 512 / n "
+                                      "tracking workgroups per system, 64 z-range workgroups per map at 16 maps)"}
+
+
 def c3_leg(warmup, steps, runs):
     """The uneven batched configuration: 16 mixed-precision streams at 160x120,
     320x240, 640x480 and 848x480 (seeds 100-115, resolution by seed), voxels of
@@ -530,6 +557,10 @@ def main():
             "note": "lf_batch_process_host from pinned host buffers: each step's 16 depth + RGB frames are copied "
                     "on the upload stream, overlapping the previous step's fusion and raycast"}
         log("pcie-inclusive mixed batched: %.1f" % ph["fps"])
+        # last on this shape: batches of it start with the tuned options from here on
+        extras["autotuned"] = tuned_leg(streams, cfgs(mixed), W, K, args.runs)
+        extras["autotuned"]["vs_plain_fp32"] = round(extras["autotuned"]["value"] / fps_plain, 3)
+        log("autotuned mixed batched: %s" % extras["autotuned"])
         del streams
         torch.cuda.empty_cache()
         extras["c3_uneven_streams"] = c3_leg(W, K, args.runs)

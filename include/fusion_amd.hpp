@@ -369,8 +369,31 @@ public:
     // shape.  Steps with timing on, or where a system starts a frame, run
     // eagerly.  Results are those of the eager steps.
     void set_graphs(bool on);
+    // Online autotuning of the launch options: the next steps try each
+    // option's candidate values in turn (one option at a time, the best kept
+    // before the next), each candidate for one settling step then
+    // steps_per_trial timed steps, in `rounds` interleaved rounds, timed by
+    // events on the batch stream; the medians decide.  The choice is cached
+    // per batch shape (process-wide, and in the file of set_tune_file), and a
+    // batch of a cached shape starts with it.  0 stops a running tuning.
+    int autotune(int steps_per_trial, int rounds = 2);
+    bool tuning() const { return tune_.per_trial > 0; }
+    // The file the tuned choices are read from and appended to (empty: none).
+    static int set_tune_file(const std::string &path);
 
 private:
+    struct Tuner {
+        int per_trial = 0, rounds = 2;
+        int opt = 0;         // index into the tuned options
+        int round = 0, cand = 0, step = 0;  // position in the current option's trials
+        std::vector<std::vector<double>> ms;  // per candidate: ms per step of each round
+        hipEvent_t a = nullptr;  // end of the trial's settling step
+        std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;  // (candidate, trial events)
+    };
+    Tuner tune_;
+    std::string tune_key() const;
+    void tune_begin_step();
+    int tune_end_step();
     struct StepGraphs {
         hipGraphExec_t up = nullptr, track = nullptr, fuse = nullptr;
         std::vector<long long> sig;  // the launch shapes they were captured with

@@ -617,6 +617,25 @@ int lf_batch_get_option(lf_batch *b, int option, int *value);
  * Results are those of the eager steps; steps with timing on, or in which a
  * system starts a frame, run eagerly. */
 int lf_batch_set_graphs(lf_batch *b, int enable);
+/* Online autotuning of the launch options above (no reference counterpart).
+ * The batch's next steps try each option's candidate values in turn, one
+ * option at a time (integration workgroups {default, 512, 2048, 4096},
+ * tracking workgroups {default, 16, 48, 64}, z-range workgroups {default,
+ * 32, 128, 256}, upload overlap {1, 0}); each candidate runs one settling step
+ * and steps_per_trial steps timed by events on the batch stream, in `rounds`
+ * interleaved rounds, and the fastest median is kept before the next option
+ * (one synchronisation per option).  The steps are the caller's own frames:
+ * only launch shapes change (results up to FP64 summation order in the
+ * tracking sums).  The choice is cached per batch shape (every system's image
+ * size, levels, voxel size and precision) for the process, and appended to
+ * the file set by lf_batch_tune_file; a batch of a cached shape starts with it.
+ * steps_per_trial 1..64 (0 stops), rounds 1..16.  lf_batch_tuning: 1 while
+ * tuning runs. */
+int lf_batch_autotune(lf_batch *b, int steps_per_trial, int rounds);
+int lf_batch_tuning(lf_batch *b);
+/* Read tuned choices from path (lines "<shape> <values>", later lines win) and
+ * append new ones to it; null or "" stops using a file. */
+int lf_batch_tune_file(const char *path);
 /* System i's camera pose (4x4 row-major doubles; synchronising), last
  * tracking result and map counters. */
 int lf_batch_get_pose(lf_batch *b, int i, double *out16);

@@ -162,6 +162,9 @@ SIGNATURES = [
     ("lf_batch_set_timing", I, [P, I]),
     ("lf_batch_set_option", I, [P, I, I]),
     ("lf_batch_set_graphs", I, [P, I]),
+    ("lf_batch_autotune", I, [P, I, I]),
+    ("lf_batch_tuning", I, [P]),
+    ("lf_batch_tune_file", I, [ctypes.c_char_p]),
     ("lf_batch_map", I, [P, I, P, P]),
     ("lf_batch_get_option", I, [P, I, P]),
     ("lf_batch_read_timing", I, [P, I, P, P]),
@@ -658,6 +661,18 @@ class Batch:
     def set_graphs(self, on=True):
         """Replay steady-state steps from HIP graphs (lf_batch_set_graphs)."""
         check(load().lf_batch_set_graphs(self.h, 1 if on else 0))
+
+    def autotune(self, steps_per_trial=3, rounds=2):
+        """Tune the launch options online over the next steps (lf_batch_autotune)."""
+        check(load().lf_batch_autotune(self.h, int(steps_per_trial), int(rounds)))
+
+    def tuning(self):
+        return bool(load().lf_batch_tuning(self.h))
+
+    @staticmethod
+    def tune_file(path):
+        """Read tuned choices from path and append new ones to it (lf_batch_tune_file)."""
+        check(load().lf_batch_tune_file(None if path is None else str(path).encode()))
 
     def set_option(self, option, value):
         """A launch option (BATCH_OPT_*)."""

@@ -8,8 +8,13 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
 #include <new>
+#include <sstream>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "fusion_amd.hpp"
@@ -640,6 +645,27 @@ int System::enqueue_frame(const uint16_t *depth, const unsigned char *rgb) {
 
 // ------------------------------------------------------------------- Batch
 
+namespace {
+struct TunedOption {
+    int option;
+    std::vector<int> values;  // candidates; the first is the default
+};
+// The options tuned, in order, with their candidate values (0: the built-in
+// rule, which the sweeps behind it chose for 16 maps of 640x480).
+const std::vector<TunedOption> &tuned_options() {
+    static const std::vector<TunedOption> o = {
+        {LF_BATCH_OPT_INTEGRATE_WORKGROUPS, {0, 512, 2048, 4096}},
+        {LF_BATCH_OPT_TRACK_WORKGROUPS, {0, 16, 48, 64}},
+        {LF_BATCH_OPT_ZRANGE_WORKGROUPS, {0, 32, 128, 256}},
+        {LF_BATCH_OPT_UPLOAD_OVERLAP, {1, 0}},
+    };
+    return o;
+}
+std::mutex g_tune_mutex;
+std::map<std::string, std::vector<int>> g_tune_cache;  // batch shape -> tuned option values
+std::string g_tune_file;
+}  // namespace
+
 Batch::Batch(const lf_system_config *cfgs, int n) {
     auto fail = [&]() {
         release();
@@ -661,9 +687,139 @@ Batch::Batch(const lf_system_config *cfgs, int n) {
         fail();
     for (int k = 0; k < 2; ++k)
         if (hipEventCreateWithFlags(&table_read_[k], hipEventDisableTiming) != hipSuccess) fail();
+    // a shape tuned before in this process (or read from the tuning file) starts with its choice
+    std::vector<int> cached;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        auto it = g_tune_cache.find(tune_key());
+        if (it != g_tune_cache.end()) cached = it->second;
+    }
+    for (size_t t = 0; t < cached.size(); ++t) (void)set_option(tuned_options()[t].option, cached[t]);
 }
 
 Batch::~Batch() { release(); }
+
+// ------------------------------------------------------------ autotuning
+
+
+int Batch::set_tune_file(const std::string &path) {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_tune_file = path;
+    if (path.empty()) return LF_OK;
+    std::ifstream in(path);
+    std::string line;
+    while (std::getline(in, line)) {  // "<shape key> v0 v1 ..." per line; later lines win
+        std::istringstream ls(line);
+        std::string key;
+        if (!(ls >> key)) continue;
+        std::vector<int> v;
+        int x;
+        while (ls >> x) v.push_back(x);
+        if (v.size() == tuned_options().size()) g_tune_cache[key] = v;
+    }
+    return LF_OK;
+}
+
+// The batch's shape: every system's image size, pyramid levels, voxel size
+// and precision, in order.
+std::string Batch::tune_key() const {
+    std::ostringstream k;
+    k << "n" << sys_.size();
+    for (const auto &s : sys_) {
+        const lf_intrinsics &K = s->mapping().intrinsics();
+        const lf_map_params &p = s->mapping().params();
+        k << "_" << K.width << "x" << K.height << "l" << s->odometry().get_reference_image().levels() << "v"
+          << (int)std::lround(p.voxel_size * 1e5f) << "p" << p.precision;
+    }
+    return k.str();
+}
+
+int Batch::autotune(int steps_per_trial, int rounds) {
+    for (auto &e : tune_.pending) {
+        pool_.push_back(e.second.first);
+        pool_.push_back(e.second.second);
+    }
+    if (tune_.a) pool_.push_back(tune_.a);
+    tune_ = Tuner{};
+    if (steps_per_trial <= 0) return LF_OK;
+    if (steps_per_trial > 64 || rounds < 1 || rounds > 16)
+        return lf_set_error(LF_ERR_ARG, "autotune: 1..64 steps per trial and 1..16 rounds");
+    tune_.per_trial = steps_per_trial;
+    tune_.rounds = rounds;
+    tune_.ms.assign(tuned_options()[0].values.size(), {});
+    return LF_OK;
+}
+
+// Before a step: the current trial's candidate value.
+void Batch::tune_begin_step() {
+    if (!tuning()) return;
+    const TunedOption &o = tuned_options()[tune_.opt];
+    (void)set_option(o.option, o.values[tune_.cand]);
+}
+
+// After a step: mark the trial's timing events, advance, and when an option's
+// trials are done, read them, keep the fastest candidate and go on.
+int Batch::tune_end_step() {
+    if (!tuning()) return LF_OK;
+    if (tune_.step == 0) {  // the settling step ends: the timed steps start here
+        tune_.a = take_event();
+        if (!tune_.a) return lf_set_error(LF_ERR_HIP, "hipEventCreate failed");
+        LF_HIP_CHECK(hipEventRecord(tune_.a, stream_));
+    }
+    if (++tune_.step <= tune_.per_trial) return LF_OK;
+    hipEvent_t b = take_event();
+    if (!b) return lf_set_error(LF_ERR_HIP, "hipEventCreate failed");
+    LF_HIP_CHECK(hipEventRecord(b, stream_));
+    tune_.pending.push_back({tune_.cand, {tune_.a, b}});
+    tune_.a = nullptr;
+    tune_.step = 0;
+    const TunedOption &o = tuned_options()[tune_.opt];
+    if (++tune_.cand < (int)o.values.size()) return LF_OK;
+    tune_.cand = 0;
+    if (++tune_.round < tune_.rounds) return LF_OK;
+    // every trial of this option is enqueued: read them
+    LF_HIP_CHECK(hipEventSynchronize(tune_.pending.back().second.second));
+    for (auto &e : tune_.pending) {
+        float ms = 0.0f;
+        LF_HIP_CHECK(hipEventElapsedTime(&ms, e.second.first, e.second.second));
+        tune_.ms[e.first].push_back(ms / tune_.per_trial);
+        pool_.push_back(e.second.first);
+        pool_.push_back(e.second.second);
+    }
+    tune_.pending.clear();
+    int best = 0;
+    double best_ms = 0.0;
+    for (size_t c = 0; c < tune_.ms.size(); ++c) {
+        std::vector<double> v = tune_.ms[c];
+        std::sort(v.begin(), v.end());
+        const double med = v[v.size() / 2];
+        if (c == 0 || med < best_ms) { best = (int)c; best_ms = med; }
+    }
+    LF_RC(set_option(o.option, o.values[best]));
+    tune_.round = 0;
+    if (++tune_.opt < (int)tuned_options().size()) {
+        tune_.ms.assign(tuned_options()[tune_.opt].values.size(), {});
+        return LF_OK;
+    }
+    // done: cache the choice for this shape
+    std::vector<int> chosen;
+    for (const TunedOption &t : tuned_options()) {
+        int v = 0;
+        LF_RC(get_option(t.option, &v));
+        chosen.push_back(v);
+    }
+    tune_ = Tuner{};
+    const std::string key = tune_key();
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_tune_cache[key] = chosen;
+    if (!g_tune_file.empty()) {
+        std::ofstream out(g_tune_file, std::ios::app);
+        out << key;
+        for (int v : chosen) out << " " << v;
+        out << "\n";
+    }
+    return LF_OK;
+}
 
 void Batch::StepGraphs::drop() {
     for (hipGraphExec_t *e : {&up, &track, &fuse}) {
@@ -802,10 +958,16 @@ int Batch::read_timing(int stage, int *launches, double *total_ms) {
     return LF_OK;
 }
 
-int Batch::process(const uint16_t *const *depth, const unsigned char *const *rgb) { return step(depth, rgb, false); }
+int Batch::process(const uint16_t *const *depth, const unsigned char *const *rgb) {
+    tune_begin_step();
+    LF_RC(step(depth, rgb, false));
+    return tune_end_step();
+}
 
 int Batch::process_host(const uint16_t *const *depth, const unsigned char *const *rgb) {
-    return step(depth, rgb, true);
+    tune_begin_step();
+    LF_RC(step(depth, rgb, true));
+    return tune_end_step();
 }
 
 int Batch::step(const uint16_t *const *depth_in, const unsigned char *const *rgb_in, bool from_host) {
@@ -1536,6 +1698,15 @@ extern "C" int lf_batch_process_host(lf_batch *b, const uint16_t *const *depth, 
         if (!depth[i] || !rgb[i]) return lf_set_error(LF_ERR_ARG, "null frame for batch system %d", i);
     return b->b->process_host(depth, rgb);
 }
+
+extern "C" int lf_batch_autotune(lf_batch *b, int steps_per_trial, int rounds) {
+    if (!b) return lf_set_error(LF_ERR_ARG, "null batch");
+    return b->b->autotune(steps_per_trial, rounds);
+}
+
+extern "C" int lf_batch_tuning(lf_batch *b) { return b && b->b->tuning() ? 1 : 0; }
+
+extern "C" int lf_batch_tune_file(const char *path) { return fusion::Batch::set_tune_file(path ? path : ""); }
 
 extern "C" int lf_batch_set_graphs(lf_batch *b, int enable) {
     if (!b) return lf_set_error(LF_ERR_ARG, "null batch");

@@ -1149,6 +1149,57 @@ def test_batch_graph_replay_equals_eager_steps(precision):
     graphed.close()
 
 
+def test_batch_autotune_picks_caches_and_keeps_results(tmp_path):
+    """lf_batch_autotune over a batch's own steps (3 mixed systems at 320x240,
+    one settling + one timed step per candidate, one round): tuning ends after
+    the schedule's 28 steps (14 candidates) with every option at one of its
+    candidates; the
+    choice is written to the tuning file and a new batch of the same shape
+    starts with it; the tuned batch's poses equal an untuned batch's on the
+    same frames up to the tracking sums' FP64 summation order."""
+    seqs = [Scene(1, 1), Scene(1, 7), Scene(1, 9)]
+    K = half_res(seqs[0])
+    p = capi.map_params(2 ** 18, 1 << 17, 0.01, 0.03, precision=capi.PRECISION_MIXED)
+    frames = []
+    for sc in seqs:
+        fs = []
+        for f in range(30):
+            d16, rgb = sc.render(sc.pose(f), K.width, K.height, K.fx, K.fy, K.cx, K.cy, noise=True, frame_id=f)
+            fs.append((dev(d16), dev(rgb)))
+        frames.append(fs)
+    torch.cuda.synchronize()
+    cfgs = [capi.system_config(K, p, 3, (10, 5, 3), sc.depth_scale) for sc in seqs]
+    path = tmp_path / "tune.txt"
+    capi.Batch.tune_file(path)
+    tuned, plain = capi.Batch(cfgs), capi.Batch(cfgs)
+    tuned.autotune(1, 1)
+    opts = (capi.BATCH_OPT_INTEGRATE_WORKGROUPS, capi.BATCH_OPT_TRACK_WORKGROUPS, capi.BATCH_OPT_ZRANGE_WORKGROUPS,
+            capi.BATCH_OPT_UPLOAD_OVERLAP)
+    cands = ({0, 512, 2048, 4096}, {0, 16, 48, 64}, {0, 32, 128, 256}, {1, 0})
+    for f in range(30):
+        for b in (tuned, plain):
+            b.process([fs[f][0] for fs in frames], [fs[f][1] for fs in frames])
+        if f == 26:
+            assert tuned.tuning()
+        if f == 27:
+            assert not tuned.tuning()
+    tuned.synchronize()
+    plain.synchronize()
+    chosen = [tuned.option(o) for o in opts]
+    assert all(v in c for v, c in zip(chosen, cands)), chosen
+    lines = path.read_text().split("\n")
+    assert len([x for x in lines if x]) == 1 and lines[0].split()[1:] == [str(v) for v in chosen], lines
+    again = capi.Batch(cfgs)
+    assert [again.option(o) for o in opts] == chosen
+    again.close()
+    for i in range(len(seqs)):
+        assert tuned.tracking(i)[0] == plain.tracking(i)[0]
+        assert np.abs(tuned.pose(i) - plain.pose(i)).max() < 1e-6, i
+    capi.Batch.tune_file(None)
+    tuned.close()
+    plain.close()
+
+
 def test_batch_with_one_raycast_launch_equals_systems_run_alone():
     """lf_batch: three systems of different image sizes, voxel sizes and
     pyramid depths whose pyramid passes, tracking steps and raycasts share

@@ -260,13 +260,13 @@ def profile_value(name, key):
 
 
 def tuned_leg(streams, cfgs, warmup, steps, runs):
-    """lf_batch_autotune on the headline batch (one settling and one timed
-    step per candidate, one round, over the streams' frames in a loop), then
+    """lf_batch_autotune on the headline batch (one settling and two timed
+    steps per candidate, two rounds, over the streams' frames in a loop), then
     the headline measurement again: batches of this shape now start with the
     tuned launch options."""
     from libfusion_amd import capi
     b = capi.Batch(cfgs)
-    b.autotune(1, 1)
+    b.autotune(2, 2)
     f = 0
     while b.tuning():
         k = f % streams.frames

@@ -750,7 +750,8 @@ int Batch::autotune(int steps_per_trial, int rounds) {
     return LF_OK;
 }
 
-// Before a step: the current trial's candidate value.
+// Before a step: the current trial's candidate value (the first candidate of
+// each option is its built-in default).
 void Batch::tune_begin_step() {
     if (!tuning()) return;
     const TunedOption &o = tuned_options()[tune_.opt];
@@ -787,14 +788,18 @@ int Batch::tune_end_step() {
         pool_.push_back(e.second.second);
     }
     tune_.pending.clear();
-    int best = 0;
-    double best_ms = 0.0;
+    // the fastest median, kept only when it beats the built-in default (the
+    // first candidate) by more than the trials' noise (1%)
+    std::vector<double> med(tune_.ms.size());
     for (size_t c = 0; c < tune_.ms.size(); ++c) {
         std::vector<double> v = tune_.ms[c];
         std::sort(v.begin(), v.end());
-        const double med = v[v.size() / 2];
-        if (c == 0 || med < best_ms) { best = (int)c; best_ms = med; }
+        med[c] = v[v.size() / 2];
     }
+    int best = 0;
+    for (size_t c = 1; c < med.size(); ++c)
+        if (med[c] < med[best]) best = (int)c;
+    if (!(med[best] < 0.99 * med[0])) best = 0;
     LF_RC(set_option(o.option, o.values[best]));
     tune_.round = 0;
     if (++tune_.opt < (int)tuned_options().size()) {

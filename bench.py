@@ -11,7 +11,7 @@ raycast -> vertex/normal re-pyramid).  All frames are resident in HBM before
 the timed region starts.
 
 value       mixed precision (FP16 TSDF + 8-bit weights, FP16 normal / intensity /
-            gradient maps, FP32 sums, FP64 solve) with the 16 streams stepped
+            gradient maps, FP64 sums and solve) with the 16 streams stepped
             together by lf_batch: aggregate frames/s over exactly --steps timed
             steps after --warmup steps, median of 3 runs.
 vs_plain_fp32  value / the plain FP32 GPU path (the reference's method, one
@@ -591,7 +591,7 @@ def main():
                             "plain FP32 GPU path on the same streams and frames, is vs_plain_fp32",
         "vs_plain_fp32": round(ratio, 3),
         "dtype": "mixed: fp16 storage (TSDF, normal / intensity / gradient maps), u8 weights, fp32 arithmetic, "
-                 "fp64 solve",
+                 "fp64 normal-equation sums and solve",
         "data": "synthetic (seeded analytic rooms with 8 primitives, Perlin albedo, depth noise + 2% dropouts)",
         "config": {"workload": "%d independent 640x480 RGB-D streams (seeds %d-%d, own orbit and map each), 5 mm "
                                "voxels, 15 mm truncation, 2^20 buckets, 3-level ICP+RGB {10,5,3}; one step = one "

@@ -89,9 +89,9 @@ typedef struct lf_map_params {
  *         stochastic rounding: lf_weight_code_dither) packed with the colour
  *         into one word per voxel (6 bytes a voxel); FP16 normal,
  *         intensity and gradient maps (vertex maps stay float4: FP16's 2 mm
- *         step at 2-4 m biased the ICP fit); the same FP32 normal-equation
- *         sums (pairwise: per-lane, wavefront tree, fixed-order final sum) and
- *         FP64 solve and pose updates.  Results are stated against the FP64
+ *         step at 2-4 m biased the ICP fit); the same tracking arithmetic as
+ *         FP32 (FP32 rows, their normal equations accumulated in FP64; FP64
+ *         solve and pose updates).  Results are stated against the FP64
  *         method, not bit for bit. */
 #define LF_PRECISION_FP32 0
 #define LF_PRECISION_MIXED 1

@@ -1,6 +1,6 @@
 """GPU: the mixed-precision pipeline (FP16 TSDF with 8-bit weight codes, FP16
-normal / intensity / gradient maps, FP32 vertex maps, FP32 pairwise sums,
-FP64 solve).
+normal / intensity / gradient maps, FP32 vertex maps, FP64 normal-equation
+sums, FP64 solve).
 
 Its results are stated against the FP64 oracle, not bit for bit:
   * per-frame poses within 0.5 mm and 0.05 deg of the FP64 oracle's, tracking

@@ -261,15 +261,18 @@ def profile_value(name, key):
 
 def tuned_leg(streams, cfgs, warmup, steps, runs):
     """lf_batch_autotune on the headline batch (one settling and two timed
-    steps per candidate, two rounds, over the streams' frames in a loop), then
-    the headline measurement again: batches of this shape now start with the
-    tuned launch options."""
+    steps per candidate, two rounds, over the streams' frames forth and back,
+    so consecutive steps stay neighbouring frames), then the headline
+    measurement again: batches of this shape now start with the tuned launch
+    options."""
     from libfusion_amd import capi
     b = capi.Batch(cfgs)
     b.autotune(2, 2)
     f = 0
+    period = max(1, 2 * streams.frames - 2)
     while b.tuning():
-        k = f % streams.frames
+        k = f % period
+        k = k if k < streams.frames else period - k
         b.process([d[k] for d, _ in streams.dev], [c[k] for _, c in streams.dev])
         f += 1
     b.synchronize()

@@ -623,7 +623,8 @@ int lf_batch_set_graphs(lf_batch *b, int enable);
  * tracking workgroups {default, 16, 48, 64}, z-range workgroups {default,
  * 32, 128, 256}, upload overlap {1, 0}); each candidate runs one settling step
  * and steps_per_trial steps timed by events on the batch stream, in `rounds`
- * interleaved rounds, and the fastest median is kept before the next option
+ * interleaved rounds (odd rounds in reverse order, so a steady drift of the
+ * step time cancels), and the fastest median is kept before the next option
  * when it beats the default's by more than 1% (one synchronisation per
  * option).  The steps are the caller's own frames:
  * only launch shapes change (results up to FP64 summation order in the

@@ -750,12 +750,17 @@ int Batch::autotune(int steps_per_trial, int rounds) {
     return LF_OK;
 }
 
+// The candidate of the current trial: odd rounds visit the candidates in
+// reverse order, so a steady drift of the step time (maps grow) cancels in
+// the medians instead of favouring the candidates tried first.
+static int trial_candidate(int round, int cand, int count) { return round % 2 ? count - 1 - cand : cand; }
+
 // Before a step: the current trial's candidate value (the first candidate of
 // each option is its built-in default).
 void Batch::tune_begin_step() {
     if (!tuning()) return;
     const TunedOption &o = tuned_options()[tune_.opt];
-    (void)set_option(o.option, o.values[tune_.cand]);
+    (void)set_option(o.option, o.values[trial_candidate(tune_.round, tune_.cand, (int)o.values.size())]);
 }
 
 // After a step: mark the trial's timing events, advance, and when an option's
@@ -771,10 +776,10 @@ int Batch::tune_end_step() {
     hipEvent_t b = take_event();
     if (!b) return lf_set_error(LF_ERR_HIP, "hipEventCreate failed");
     LF_HIP_CHECK(hipEventRecord(b, stream_));
-    tune_.pending.push_back({tune_.cand, {tune_.a, b}});
+    const TunedOption &o = tuned_options()[tune_.opt];
+    tune_.pending.push_back({trial_candidate(tune_.round, tune_.cand, (int)o.values.size()), {tune_.a, b}});
     tune_.a = nullptr;
     tune_.step = 0;
-    const TunedOption &o = tuned_options()[tune_.opt];
     if (++tune_.cand < (int)o.values.size()) return LF_OK;
     tune_.cand = 0;
     if (++tune_.round < tune_.rounds) return LF_OK;

@@ -115,7 +115,7 @@ typedef struct lf_map_params {
  *                            R[512] G[512] B[512].  Mixed: unused (may be null)
  *   visible[num_blocks]      pool block indices visible in the last update
  *   counters[LF_NUM_COUNTERS]
- *   occupancy[LF_OCC_WORDS]  hashed occupancy bits of 4x4x4-block regions: bit
+ *   occupancy[LF_OCC_WORDS]  occupancy bits of 4x4x4-block regions: bit
  *                            lf_occupancy_bit(bx, by, bz) is set when a block of
  *                            the region is inserted (a clear bit proves every
  *                            block of the region absent; the raycast skips the
@@ -153,8 +153,9 @@ typedef struct lf_map_storage {
 
 /* ---------------------------------------------------------------- general */
 
-/* Occupancy bit of block (bx, by, bz): the computeHash mix of its 4x4x4-block
- * region (floor division), modulo 2^17. */
+/* Occupancy bit of block (bx, by, bz): its 4x4x4-block region (floor
+ * division) (rx, ry, rz) wrapped onto a 64 x 64 x 32 torus,
+ * (rz mod 32) * 4096 + (ry mod 64) * 64 + (rx mod 64). */
 int lf_occupancy_bit(int bx, int by, int bz);
 /* Directory cell of block (bx, by, bz), or -1 outside the directory. */
 int lf_dir_cell(int bx, int by, int bz);

@@ -304,6 +304,10 @@ LF_DEV int resolve(const RayDev &r, BlockCache &bc, int bx, int by, int bz) {
     return bc.b;
 }
 
+// The 2-byte sdf of voxel vi, addressed by a 32-bit byte offset (a pool of
+// at most 2^31 / 512 blocks keeps 2 * vi below 2^32).
+LF_DEV unsigned short sdf_bits(const RayDev &r, int vi) { return at_byte(reinterpret_cast<const unsigned short *>(r.sdf), 2u * (unsigned int)vi); }
+
 // findVoxel + read_sdf (ray_tracing.cu:291-305) at a sample of an in-range
 // ray.  A voxel is valid when its weight is non-zero, which the sdf plane
 // encodes (kSdfUnseen; kHalfUnseen, a NaN, in mixed precision), so one 2-byte
@@ -317,11 +321,11 @@ LF_DEV float read_sdf(const RayDev &r, BlockCache &bc, f3 p, bool &valid, int *v
     const int vi = b * LF_BLOCK3 + lidx(vx, vy, vz);
     if (voxel_out) *voxel_out = vi;
     if constexpr (kMixed) {
-        const float sv = half_bits_to_float(reinterpret_cast<const unsigned short *>(r.sdf)[vi]);
+        const float sv = half_bits_to_float(sdf_bits(r, vi));
         valid = !isnan(sv);
         return sv;
     }
-    const short sd = r.sdf[vi];
+    const short sd = (short)sdf_bits(r, vi);
     if (sd == kSdfUnseen) return __int_as_float(0x7fc00000);
     valid = true;
     return unpack_sdf(sd);
@@ -330,10 +334,14 @@ LF_DEV float read_sdf(const RayDev &r, BlockCache &bc, f3 p, bool &valid, int *v
 // read_sdf_interped (ray_tracing.cu:307-339).  Corner k is the voxel of
 // p + offset_k with offsets 0 or 1 per axis, so each axis has only two
 // coordinates, floor(p) and floor(p + 1), computed once, and the corners lie
-// in at most two blocks per axis: each distinct block is resolved once (one
-// lookup when no axis crosses a block face, the common case), then all 8
-// sdf loads are issued in one round trip; the arithmetic is the reference's
-// order.
+// in at most two blocks per axis.  Corner k's block is B[k & m], where m has
+// a bit for each axis whose two coordinates lie in different blocks and B[e]
+// is the block one step across the faces of e: B[0] is the lane's cached
+// block (p's own), and the others are resolved in one loop over the nonzero
+// subsets of m (one code path whatever faces a lane's corners straddle; none
+// in the common case), after which the lane's cache returns to B[0].  Then
+// all 8 sdf loads are issued in one round trip; the arithmetic is the
+// reference's order.
 template <bool kMixed>
 LF_DEV float read_interp(const RayDev &r, BlockCache &bc, f3 p, bool &valid) {
     const float fx = floorf(p.x), fy = floorf(p.y), fz = floorf(p.z);
@@ -343,16 +351,21 @@ LF_DEV float read_interp(const RayDev &r, BlockCache &bc, f3 p, bool &valid) {
     // corner order of the reference: (0,0,0) (1,0,0) (0,1,0) (1,1,0) (0,0,1) (1,0,1) (0,1,1) (1,1,1)
     const int bx0 = cx[0] >> 3, bx1 = cx[1] >> 3, by0 = cy[0] >> 3, by1 = cy[1] >> 3, bz0 = cz[0] >> 3,
               bz1 = cz[1] >> 3;
-    const bool sx = bx1 == bx0, sy = by1 == by0, sz = bz1 == bz0;
+    const int m = (bx1 != bx0 ? 1 : 0) | (by1 != by0 ? 2 : 0) | (bz1 != bz0 ? 4 : 0);
     int bb[8];
     bb[0] = resolve(r, bc, bx0, by0, bz0);
-    bb[1] = sx ? bb[0] : resolve(r, bc, bx1, by0, bz0);
-    bb[2] = sy ? bb[0] : resolve(r, bc, bx0, by1, bz0);
-    bb[3] = sy ? bb[1] : sx ? bb[2] : resolve(r, bc, bx1, by1, bz0);
-    bb[4] = sz ? bb[0] : resolve(r, bc, bx0, by0, bz1);
-    bb[5] = sz ? bb[1] : sx ? bb[4] : resolve(r, bc, bx1, by0, bz1);
-    bb[6] = sz ? bb[2] : sy ? bb[4] : resolve(r, bc, bx0, by1, bz1);
-    bb[7] = sz ? bb[3] : sy ? bb[5] : sx ? bb[6] : resolve(r, bc, bx1, by1, bz1);
+#pragma unroll
+    for (int k = 1; k < 8; ++k) bb[k] = bb[0];
+    if (m != 0) {
+        const bool empty0 = bc.empty_region;
+        for (int e = m & -m; e != 0; e = (e - m) & m) {  // the nonzero subsets of m, increasing
+            const int b = resolve(r, bc, (e & 1) ? bx1 : bx0, (e & 2) ? by1 : by0, (e & 4) ? bz1 : bz0);
+#pragma unroll
+            for (int k = 1; k < 8; ++k)
+                if ((k & m) == e) bb[k] = b;
+        }
+        bc.bx = bx0; bc.by = by0; bc.bz = bz0; bc.b = bb[0]; bc.empty_region = empty0;
+    }
     int vi[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -361,7 +374,7 @@ LF_DEV float read_interp(const RayDev &r, BlockCache &bc, f3 p, bool &valid) {
     }
     short c[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) c[k] = r.sdf[vi[k] < 0 ? 0 : vi[k]];
+    for (int k = 0; k < 8; ++k) c[k] = (short)sdf_bits(r, vi[k] < 0 ? 0 : vi[k]);
     float s[8];
     bool vp = true;
 #pragma unroll

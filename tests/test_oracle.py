@@ -42,26 +42,32 @@ def test_compute_hash_known_answers(pos, n, expect):
 
 def test_occupancy_bit_regions():
     """The occupancy bit depends only on the 4x4x4-block region (floor
-    division, negative coordinates included) and hashes the region like
-    computeHash (known answers from the 32-bit wrapped products)."""
+    division, negative coordinates included), wraps the regions onto a
+    64 x 64 x 32 torus (known answers), and gives every region of one torus
+    period its own bit."""
     lib = capi.load()
-    mask = 32 * capi.OCC_WORDS - 1
+    assert 32 * capi.OCC_WORDS == 64 * 64 * 32
 
     def expect(bx, by, bz):
-        a = ((bx // 4) * 73856093) & 0xFFFFFFFF
-        b = ((by // 4) * 19349669) & 0xFFFFFFFF
-        c = ((bz // 4) * 83492791) & 0xFFFFFFFF
-        return (a ^ b ^ c) & mask
+        return ((bz // 4) % 32) * 4096 + ((by // 4) % 64) * 64 + (bx // 4) % 64
 
     assert lib.lf_occupancy_bit(0, 0, 0) == 0
-    assert lib.lf_occupancy_bit(4, 0, 0) == 73856093 & mask
-    assert lib.lf_occupancy_bit(-1, 0, 0) == (-73856093 & 0xFFFFFFFF) & mask
+    assert lib.lf_occupancy_bit(4, 0, 0) == 1
+    assert lib.lf_occupancy_bit(0, 4, 0) == 64
+    assert lib.lf_occupancy_bit(0, 0, 4) == 4096
+    assert lib.lf_occupancy_bit(-1, 0, 0) == 63
+    assert lib.lf_occupancy_bit(0, 0, -1) == 31 * 4096
+    assert lib.lf_occupancy_bit(256, 256, 128) == 0  # one period on; 4 * 64 blocks in x, y, 4 * 32 in z
     rng = np.random.default_rng(5)
-    for bx, by, bz in rng.integers(-300, 300, size=(200, 3)).tolist():
+    for bx, by, bz in rng.integers(-3000, 3000, size=(200, 3)).tolist():
         b = lib.lf_occupancy_bit(bx, by, bz)
         assert b == expect(bx, by, bz)
         r = (bx - bx % 4, by - by % 4, bz - bz % 4)
         assert lib.lf_occupancy_bit(r[0] + 3, r[1] + 3, r[2] + 3) == b
+    # one torus period of regions, shifted to straddle the origin: all bits distinct
+    seen = {lib.lf_occupancy_bit(4 * rx, 4 * ry, 4 * rz)
+            for rx in range(-32, 32) for ry in range(-32, 32) for rz in range(-16, 16)}
+    assert len(seen) == 64 * 64 * 32
 
 
 def test_block_directory_cells_are_a_bijection_in_range():

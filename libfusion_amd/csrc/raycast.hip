@@ -264,44 +264,63 @@ LF_DEV unsigned long long lds_tag(int bx, int by, int bz) {
 // 21 bits per axis) and finds nothing.
 constexpr float kRayRange = 1.0e9f;
 LF_DEV int ray_floor(float v) { return (int)floorf(v); }
+// A float that every active lane holds alike, in a scalar register.
+LF_DEV float uniform(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+// 1 when v's sign bit is clear, else 0 (for a v neither NaN nor zero: v > 0)
+LF_DEV unsigned int positive(float v) { return ~__float_as_uint(v) >> 31; }
+
+// The pool block of a block outside the block directory: the workgroup's LDS
+// cache of lookups, then the hash table.  (Inside the directory its load is
+// faster than the LDS cache round trip plus the loads of its misses: 250 ->
+// 234 us at C1.)
+LF_DEV int resolve_far(const RayDev &r, unsigned long long *lds, int bx, int by, int bz) {
+    if (lds && fits13(bx) && fits13(by) && fits13(bz)) {
+        const unsigned long long tag = lds_tag(bx, by, bz);
+        const unsigned int h = ((unsigned int)bx * 73856093u ^ (unsigned int)by * 19349669u ^
+                                (unsigned int)bz * 83492791u) & (kLdsCache - 1);
+        const unsigned long long e = lds[h];
+        if ((e & ~kNoBlock) == tag) {
+            const unsigned long long idx = e & kNoBlock;
+            return idx == kNoBlock ? -1 : (int)idx;
+        }
+        const int b = find_block(r.keys, r.slot_block, bx, by, bz, r.num_buckets, r.num_entries);
+        lds[h] = tag | (b < 0 ? kNoBlock : (unsigned long long)b);
+        return b;
+    }
+    return find_block(r.keys, r.slot_block, bx, by, bz, r.num_buckets, r.num_entries);
+}
+
+LF_DEV bool region_occupied(const BlockCache &bc, int bx, int by, int bz) {
+    const unsigned int ob = occupancy_bit(bx, by, bz);
+    return (bc.occ[ob >> 5] >> (ob & 31)) & 1u;
+}
 
 // Resolve the pool block of block coordinate (bx, by, bz): the lane's last
 // block, then the occupancy bits (a clear bit: the whole 4x4x4-block region
 // is unallocated, which is most of the free space a ray crosses), then one
-// load from the block directory.  Outside the directory: the workgroup's LDS
-// cache of lookups, then the hash table.  (Inside it the directory load is
-// faster than the LDS cache round trip plus the loads of its misses: 250 ->
-// 234 us at C1.)
+// load from the block directory, outside it resolve_far.
 LF_DEV int resolve(const RayDev &r, BlockCache &bc, int bx, int by, int bz) {
     if (bx == bc.bx && by == bc.by && bz == bc.bz) return bc.b;
     bc.bx = bx; bc.by = by; bc.bz = bz;
-    const unsigned int ob = occupancy_bit(bx, by, bz);
-    bc.empty_region = !((bc.occ[ob >> 5] >> (ob & 31)) & 1u);
+    bc.empty_region = !region_occupied(bc, bx, by, bz);
     if (bc.empty_region) {
         bc.b = -1;
         return -1;
     }
     const int cell = dir_cell(bx, by, bz);
-    if (cell >= 0) {
-        bc.b = r.block_dir[cell];
-        return bc.b;
-    }
-    if (bc.lds && fits13(bx) && fits13(by) && fits13(bz)) {
-        const unsigned long long tag = lds_tag(bx, by, bz);
-        const unsigned int h = ((unsigned int)bx * 73856093u ^ (unsigned int)by * 19349669u ^
-                                (unsigned int)bz * 83492791u) & (kLdsCache - 1);
-        const unsigned long long e = bc.lds[h];
-        if ((e & ~kNoBlock) == tag) {
-            const unsigned long long idx = e & kNoBlock;
-            bc.b = idx == kNoBlock ? -1 : (int)idx;
-            return bc.b;
-        }
-        bc.b = find_block(r.keys, r.slot_block, bx, by, bz, r.num_buckets, r.num_entries);
-        bc.lds[h] = tag | (bc.b < 0 ? kNoBlock : (unsigned long long)bc.b);
-        return bc.b;
-    }
-    bc.b = find_block(r.keys, r.slot_block, bx, by, bz, r.num_buckets, r.num_entries);
+    bc.b = cell >= 0 ? r.block_dir[cell] : resolve_far(r, bc.lds, bx, by, bz);
     return bc.b;
+}
+
+// The block of a trilinear corner across a block face from the sample's own
+// block, without the lane's cache (which keeps the sample's block): inside
+// the directory its one load (no occupancy test: a block next to a sampled
+// voxel's block seldom lies in an empty region), outside it the occupancy
+// bits and resolve_far.
+LF_DEV int corner_block(const RayDev &r, const BlockCache &bc, int bx, int by, int bz) {
+    const int cell = dir_cell(bx, by, bz);
+    if (cell >= 0) return r.block_dir[cell];
+    return region_occupied(bc, bx, by, bz) ? resolve_far(r, bc.lds, bx, by, bz) : -1;
 }
 
 // The 2-byte sdf of voxel vi, addressed by a 32-bit byte offset (a pool of
@@ -336,12 +355,14 @@ LF_DEV float read_sdf(const RayDev &r, BlockCache &bc, f3 p, bool &valid, int *v
 // coordinates, floor(p) and floor(p + 1), computed once, and the corners lie
 // in at most two blocks per axis.  Corner k's block is B[k & m], where m has
 // a bit for each axis whose two coordinates lie in different blocks and B[e]
-// is the block one step across the faces of e: B[0] is the lane's cached
-// block (p's own), and the others are resolved in one loop over the nonzero
+// is the block one step across the faces of e: B[0] is p's own block (the
+// lane's cache), and the others are looked up in one loop over the nonzero
 // subsets of m (one code path whatever faces a lane's corners straddle; none
-// in the common case), after which the lane's cache returns to B[0].  Then
-// all 8 sdf loads are issued in one round trip; the arithmetic is the
-// reference's order.
+// in the common case).  Then all 8 sdf loads are issued in one round trip
+// (a corner in an unallocated block reads voxel 0 of pool block 0, and NaN
+// replaces corner 0's value).  An invalid corner is NaN and the corner values are
+// finite otherwise, so the result is valid exactly when it is not NaN.  The
+// arithmetic is the reference's order.
 template <bool kMixed>
 LF_DEV float read_interp(const RayDev &r, BlockCache &bc, f3 p, bool &valid) {
     const float fx = floorf(p.x), fy = floorf(p.y), fz = floorf(p.z);
@@ -356,48 +377,41 @@ LF_DEV float read_interp(const RayDev &r, BlockCache &bc, f3 p, bool &valid) {
     bb[0] = resolve(r, bc, bx0, by0, bz0);
 #pragma unroll
     for (int k = 1; k < 8; ++k) bb[k] = bb[0];
-    if (m != 0) {
-        const bool empty0 = bc.empty_region;
-        for (int e = m & -m; e != 0; e = (e - m) & m) {  // the nonzero subsets of m, increasing
-            const int b = resolve(r, bc, (e & 1) ? bx1 : bx0, (e & 2) ? by1 : by0, (e & 4) ? bz1 : bz0);
+    for (int e = m & -m; e != 0; e = (e - m) & m) {  // the nonzero subsets of m, increasing
+        const int b = corner_block(r, bc, (e & 1) ? bx1 : bx0, (e & 2) ? by1 : by0, (e & 4) ? bz1 : bz0);
 #pragma unroll
-            for (int k = 1; k < 8; ++k)
-                if ((k & m) == e) bb[k] = b;
-        }
-        bc.bx = bx0; bc.by = by0; bc.bz = bz0; bc.b = bb[0]; bc.empty_region = empty0;
+        for (int k = 1; k < 8; ++k)
+            if ((k & m) == e) bb[k] = b;
     }
-    int vi[8];
+    // byte offsets of the corners' sdf: block << 10 | 2 * lidx
+    const unsigned int ox[2] = {(unsigned int)(cx[0] & 7) << 1, (unsigned int)(cx[1] & 7) << 1},
+                       oy[2] = {(unsigned int)(cy[0] & 7) << 4, (unsigned int)(cy[1] & 7) << 4},
+                       oz[2] = {(unsigned int)(cz[0] & 7) << 7, (unsigned int)(cz[1] & 7) << 7};
+    const unsigned short *sdf16 = reinterpret_cast<const unsigned short *>(r.sdf);
+    unsigned short c[8];
+    int bmin = bb[0];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        const int x = cx[k & 1], y = cy[(k >> 1) & 1], z = cz[k >> 2];
-        vi[k] = bb[k] < 0 ? -1 : bb[k] * LF_BLOCK3 + lidx(x, y, z);
+        c[k] = at_byte(sdf16, ((unsigned int)max(bb[k], 0) << 10) | ox[k & 1] | oy[(k >> 1) & 1] | oz[k >> 2]);
+        bmin = min(bmin, bb[k]);
     }
-    short c[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) c[k] = (short)sdf_bits(r, vi[k] < 0 ? 0 : vi[k]);
+    const float qnan = __int_as_float(0x7fc00000);
     float s[8];
-    bool vp = true;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        if constexpr (kMixed) {
-            const float sv = half_bits_to_float((unsigned short)c[k]);
-            const bool ok = vi[k] >= 0 && !isnan(sv);
-            vp = vp && ok;
-            s[k] = ok ? sv : __int_as_float(0x7fc00000);
-        } else {
-            const bool ok = vi[k] >= 0 && c[k] != kSdfUnseen;
-            vp = vp && ok;
-            s[k] = ok ? unpack_sdf(c[k]) : __int_as_float(0x7fc00000);
-        }
+        if constexpr (kMixed) s[k] = half_bits_to_float(c[k]);  // kHalfUnseen is a NaN already
+        else s[k] = (short)c[k] == kSdfUnseen ? qnan : unpack_sdf((short)c[k]);
     }
+    if (bmin < 0) s[0] = qnan;  // a corner in an unallocated block
     float r0 = (1.0f - xyz.x) * s[0] + xyz.x * s[1];
     float r1 = (1.0f - xyz.x) * s[2] + xyz.x * s[3];
     const float r2 = (1.0f - xyz.y) * r0 + xyz.y * r1;
     r0 = (1.0f - xyz.x) * s[4] + xyz.x * s[5];
     r1 = (1.0f - xyz.x) * s[6] + xyz.x * s[7];
     const float r3 = (1.0f - xyz.y) * r0 + xyz.y * r1;
-    valid = vp;
-    return (1.0f - xyz.z) * r2 + xyz.z * r3;
+    const float v = (1.0f - xyz.z) * r2 + xyz.z * r3;
+    valid = !isnan(v);
+    return v;
 }
 
 // MapRenderingDelegate::operator() (check_colour == false) and
@@ -437,8 +451,9 @@ LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char 
         const float z0 = r.zx[t], z1 = r.zy[t];
         if (z1 < 1e-3f || z0 < 1e-3f || isnan(z0) || isnan(z1)) break;
         const float *Twc = pose24;
-        const float inv_vs = 1.0f / r.voxel_size;
-        const float step_scale = r.truncation * inv_vs;
+        // uniform over the workgroup (one map): kept in scalar registers
+        const float inv_vs = uniform(1.0f / r.voxel_size);
+        const float step_scale = uniform(r.truncation * inv_vs);
         f3 pt = {((float)x - K.cx) * K.invfx * z0, ((float)y - K.cy) * K.invfy * z0, z0};
         float dist_s = norm(pt) * inv_vs;
         const f3 bs = mul(xform(Twc, pt), inv_vs);
@@ -462,6 +477,9 @@ LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char 
         // stops 0.05 voxels short of where the ray leaves the box, so that no
         // sample the box does not surely hold is passed over.
         const f3 inv_dir = {__builtin_amdgcn_rcpf(dir.x), __builtin_amdgcn_rcpf(dir.y), __builtin_amdgcn_rcpf(dir.z)};
+        // bit a set: the ray runs towards + along axis a (inv_dir's sign bit
+        // clear, +inf included), so a box's exit face on that axis is its far side
+        const unsigned int fwd = positive(inv_dir.x) | positive(inv_dir.y) << 1 | positive(inv_dir.z) << 2;
         float sdf = 1.0f, last_sdf, step;
         bool valid = false, back_face = false;
         while (dist_s < dist_e) {
@@ -469,13 +487,10 @@ LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char 
             sdf = read_sdf<kMixed>(r, bc, res, valid);
             int quiet = 0;  // further samples surely inside the same unallocated box
             if (bc.b < 0) {
-                const int sh = bc.empty_region ? 5 : 3;  // region: 32 voxels a side; block: 8
-                const float size = (float)(1 << sh);
-                const float lx = (float)((bc.bx >> (sh - 3)) << sh), ly = (float)((bc.by >> (sh - 3)) << sh),
-                            lz = (float)((bc.bz >> (sh - 3)) << sh);
-                const float tx = ((inv_dir.x > 0.0f ? lx + size : lx) - res.x) * inv_dir.x;
-                const float ty = ((inv_dir.y > 0.0f ? ly + size : ly) - res.y) * inv_dir.y;
-                const float tz = ((inv_dir.z > 0.0f ? lz + size : lz) - res.z) * inv_dir.z;
+                const int k = bc.empty_region ? 2 : 0, sh = k + 3;  // region: 32 voxels a side; block: 8
+                const float tx = ((float)(((bc.bx >> k) + (int)(fwd & 1u)) << sh) - res.x) * inv_dir.x;
+                const float ty = ((float)(((bc.by >> k) + (int)((fwd >> 1) & 1u)) << sh) - res.y) * inv_dir.y;
+                const float tz = ((float)(((bc.bz >> k) + (int)(fwd >> 2)) << sh) - res.z) * inv_dir.z;
                 const float texit = fminf(tx, fminf(ty, tz));  // NaN / inf terms drop out
                 // samples at 2k (k >= 1) with 2k < texit - 0.05
                 if (texit < 1e4f) quiet = max(0, (int)ceilf(0.5f * (texit - 0.05f)) - 1);

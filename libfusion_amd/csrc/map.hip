@@ -368,6 +368,7 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
     const float zmin = m.zmin_update, zmax = m.zmax_update;
     const int t = threadIdx.x;
     const int lx = t & 7, ly = (t >> 3) & 7, lz0 = t >> 6;
+    const float fcols = (float)cols, frows = (float)rows;
     // A lane owns voxels t and t + 256 of the block.  Three phases, each
     // issuing both voxels' loads before anything waits on them: the
     // projections and input gathers, then (inside the truncation band) the
@@ -408,17 +409,21 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
                             wz = (float)(p.z * 8 + lz) * vs;
                 pt = xform(Tcw, f3{wx, wy, wz});
             }
-            int u, v;
+            // The pixel is (floor(tu), floor(tv)) (__float2int_rd of the
+            // rounded projection); it lies in the image exactly when
+            // 0 <= tu < cols and 0 <= tv < rows (NaN fails), where the floor
+            // is the truncating conversion.
+            float tu, tv;
             if constexpr (kMixed) {
                 const float iz = __builtin_amdgcn_rcpf(pt.z);
-                u = proj_floor(K.fx * pt.x * iz + K.cx + 0.5f);
-                v = proj_floor(K.fy * pt.y * iz + K.cy + 0.5f);
+                tu = K.fx * pt.x * iz + K.cx + 0.5f;
+                tv = K.fy * pt.y * iz + K.cy + 0.5f;
             } else {
-                u = proj_floor(K.fx * pt.x / pt.z + K.cx + 0.5f);
-                v = proj_floor(K.fy * pt.y / pt.z + K.cy + 0.5f);
+                tu = K.fx * pt.x / pt.z + K.cx + 0.5f;
+                tv = K.fy * pt.y / pt.z + K.cy + 0.5f;
             }
-            ok[h] = !(u < 0 || v < 0 || u > cols - 1 || v > rows - 1);
-            pix[h] = ok[h] ? (unsigned int)(v * cols + u) : 0u;
+            ok[h] = tu >= 0.0f && tu < fcols && tv >= 0.0f && tv < frows;
+            pix[h] = ok[h] ? __umul24((unsigned int)tv, (unsigned int)cols) + (unsigned int)tu : 0u;
             ptz[h] = pt.z;
             dist[h] = 0.0f; nz[h] = 0.0f; wpk[h] = 0.0f; cpk[h] = 0u;
         }

@@ -138,10 +138,6 @@ struct FloatAt {
     const float *p; int w;
     LF_DEV float operator()(int x, int y) const { return p[(size_t)y * w + x]; }
 };
-struct DepthU16At {
-    const uint16_t *p; int w; float scale;
-    LF_DEV float operator()(int x, int y) const { return depth_value(p[(size_t)y * w + x], scale); }
-};
 struct VertexAt {
     const float4 *p; int w;
     LF_DEV float4 operator()(int x, int y) const { return p[(size_t)y * w + x]; }
@@ -184,12 +180,6 @@ LF_DEV void store4(void *p, size_t i, float4 v) {
     else
         static_cast<float4 *>(p)[i] = v;
 }
-// vertices back-projected on the fly from a depth image (same arithmetic as
-// a stored vertex map)
-struct DepthVertexAt {
-    const float *d; int w; lf_intrinsics K;
-    LF_DEV float4 operator()(int x, int y) const { return back_project_at(d[(size_t)y * w + x], x, y, K); }
-};
 // vertex (x, y) of the quarter-size map, through the four half-size vertices
 // resized on the fly from the full-size map (same arithmetic as two stored
 // resizes)
@@ -288,37 +278,13 @@ LF_DEV void frame_part(const Part &p, int x, int y) {
     const int w = p.w, h = p.h;
     const size_t i = (size_t)y * w + x;
     switch (p.kind) {
-    case kFrameIn: {  // raw depth, bilateral depth, intensity and the colour copy of a frame
-        const uint16_t *d16 = static_cast<const uint16_t *>(p.in0);
-        const unsigned char *rgb = static_cast<const unsigned char *>(p.in1);
-        static_cast<float *>(p.out0)[i] = depth_value(d16[i], p.scale);
-        static_cast<float *>(p.out1)[i] = bilateral_at<DepthU16At, kHalf>(DepthU16At{d16, w, p.scale}, x, y, w, h);
-        const unsigned char r = rgb[3 * i], g = rgb[3 * i + 1], b = rgb[3 * i + 2];
-        store1<kHalf>(p.out2, i, grey_of(r, g, b));
-        unsigned char *c = static_cast<unsigned char *>(p.out3) + 3 * i;
-        c[0] = r; c[1] = g; c[2] = b;
-        break;
-    }
+    // kFrameIn (raw depth, bilateral depth, intensity and the colour copy of a
+    // frame) and kLevelMaps (intensity derivatives, vertex and normal maps of a
+    // level): frame_in_tile and level_maps_tile
     case kPyramid:  // next level of the depth (resize) and intensity (pyrDown) pyramids
         static_cast<float *>(p.out0)[i] = resize_at(static_cast<const float *>(p.in0), 2 * w, x, y);
         store1<kHalf>(p.out1, i, pyr_down_at(PlaneAt<kHalf>{p.in1, 2 * w}, 2 * w, 2 * h, x, y));
         break;
-    case kLevelMaps: {  // intensity derivatives, vertex and normal maps of a level
-        const float *d = static_cast<const float *>(p.in0);
-        float gx, gy;
-        derivative_at(PlaneAt<kHalf>{p.in1, w}, w, h, x, y, gx, gy);
-        store1<kHalf>(p.out0, i, gx);
-        store1<kHalf>(p.out1, i, gy);
-        store4<false>(p.out2, i, back_project_at(d[i], x, y, p.K));  // vertices stay FP32
-        const float4 n = nmap_at<DepthVertexAt, kHalf>(DepthVertexAt{d, w, p.K}, w, h, x, y);
-        store4<kHalf>(p.out3, i, n);
-        if (p.out4) {  // level 0: the pixel's integration sample
-            const float dist = static_cast<const float *>(p.in2)[i];
-            const unsigned char *c = static_cast<const unsigned char *>(p.in3) + 3 * i;
-            static_cast<float4 *>(p.out4)[i] = integration_sample<kHalf>(dist, n, c);
-        }
-        break;
-    }
     case kNmap:  // normal map of a stored vertex map
         store4<kHalf>(p.out0, i, nmap_at<Vec4At<false>, kHalf>(Vec4At<false>{p.in0, w}, w, h, x, y));
         break;
@@ -331,25 +297,142 @@ LF_DEV void frame_part(const Part &p, int x, int y) {
     }
 }
 
-// Workgroup wg of a pass.
-LF_DEV void frame_pass_wg(const Pass &ps, int wg) {
-    int pi = 0;
+// The frame and level-map parts of a 16x16 output tile with their windows
+// staged in LDS: each source pixel of the tile and its apron is read (and
+// converted or back-projected) once instead of by every lane whose window
+// holds it, and the window taps need no border arithmetic.  The values and
+// the operation order are those of frame_part, so the outputs are the same.
+constexpr int kTile = 16;
+constexpr int kBilApron = 2, kBilSide = kTile + 2 * kBilApron;  // 5x5 bilateral window
+constexpr int kNrmApron = 1, kNrmSide = kTile + 2 * kNrmApron;  // 3x3 (cross) normal window
+union TileLds {
+    float depth[kBilSide * kBilSide];
+    float4 vert[kNrmSide * kNrmSide];
+};
+
+// bilateral_at over the staged depths: the window of tile pixel (cx, cy) in
+// the apron's coordinates, reflect-101 already applied when staging.
+template <bool kFast>
+LF_DEV float bilateral_tile(const float *s, int cx, int cy) {
+    const float c = s[cy * kBilSide + cx];
+    float s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
-    for (int k = 1; k < kMaxParts; ++k)
-        if (k < ps.nparts && wg >= ps.part[k].wg_begin) pi = k;
-    const Part &p = ps.part[pi];
-    const int t = wg - p.wg_begin, tiles_x = (p.w + 15) >> 4;
-    const int x = (t % tiles_x) * 16 + (threadIdx.x & 15), y = (t / tiles_x) * 16 + (threadIdx.x >> 4);
+    for (int dy = -2; dy <= 2; ++dy) {
+#pragma unroll
+        for (int dx = -2; dx <= 2; ++dx) {
+            const float v = s[(cy + dy) * kBilSide + cx + dx];
+            const float d = v - c;
+            const float arg = (float)(dx * dx + dy * dy) * -0.5f + d * d * -0.5f;
+            const float wt = kFast ? __expf(arg) : expf(arg);
+            s1 += v * wt;
+            s2 += wt;
+        }
+    }
+    return s1 / s2;
+}
+
+// kFrameIn for the tile at (x0, y0).
+template <bool kHalf>
+LF_DEV void frame_in_tile(const Part &p, int x0, int y0, TileLds &lds) {
+    const int w = p.w, h = p.h;
+    const uint16_t *d16 = static_cast<const uint16_t *>(p.in0);
+    for (int k = threadIdx.x; k < kBilSide * kBilSide; k += blockDim.x) {
+        const int sx = reflect101(x0 - kBilApron + k % kBilSide, w), sy = reflect101(y0 - kBilApron + k / kBilSide, h);
+        lds.depth[k] = depth_value(d16[(size_t)sy * w + sx], p.scale);
+    }
+    __syncthreads();
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4, x = x0 + lx, y = y0 + ly;
+    if (x >= w || y >= h) return;
+    const size_t i = (size_t)y * w + x;
+    const unsigned char *rgb = static_cast<const unsigned char *>(p.in1);
+    static_cast<float *>(p.out0)[i] = lds.depth[(ly + kBilApron) * kBilSide + lx + kBilApron];
+    static_cast<float *>(p.out1)[i] = bilateral_tile<kHalf>(lds.depth, lx + kBilApron, ly + kBilApron);
+    const unsigned char r = rgb[3 * i], g = rgb[3 * i + 1], b = rgb[3 * i + 2];
+    store1<kHalf>(p.out2, i, grey_of(r, g, b));
+    unsigned char *c = static_cast<unsigned char *>(p.out3) + 3 * i;
+    c[0] = r; c[1] = g; c[2] = b;
+}
+
+// kLevelMaps for the tile at (x0, y0): the vertices of the tile and a
+// one-pixel apron (coordinates clamped into the image, as nmap_at's
+// neighbours are) back-projected once into LDS.
+template <bool kHalf>
+LF_DEV void level_maps_tile(const Part &p, int x0, int y0, TileLds &lds) {
+    const int w = p.w, h = p.h;
+    const float *d = static_cast<const float *>(p.in0);
+    for (int k = threadIdx.x; k < kNrmSide * kNrmSide; k += blockDim.x) {
+        const int sx = min(max(x0 - kNrmApron + k % kNrmSide, 0), w - 1),
+                  sy = min(max(y0 - kNrmApron + k / kNrmSide, 0), h - 1);
+        lds.vert[k] = back_project_at(d[(size_t)sy * w + sx], sx, sy, p.K);
+    }
+    __syncthreads();
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4, x = x0 + lx, y = y0 + ly;
+    if (x >= w || y >= h) return;
+    const size_t i = (size_t)y * w + x;
+    float gx, gy;
+    derivative_at(PlaneAt<kHalf>{p.in1, w}, w, h, x, y, gx, gy);
+    store1<kHalf>(p.out0, i, gx);
+    store1<kHalf>(p.out1, i, gy);
+    const int cx = lx + kNrmApron, cy = ly + kNrmApron;
+    store4<false>(p.out2, i, lds.vert[cy * kNrmSide + cx]);  // vertices stay FP32
+    // nmap_at: neighbours (min(x+1, w), y), (max(x-1, 0), y), (x, min(y+1, h)), (x, max(y-1, 0))
+    const float qnan = __int_as_float(0x7fc00000);
+    float4 n = make_float4(qnan, qnan, qnan, qnan);
+    if (!(x >= w - 1 || y >= h - 1)) {
+        const float4 a = lds.vert[cy * kNrmSide + cx + 1], b = lds.vert[cy * kNrmSide + cx - 1],
+                     c = lds.vert[(cy + 1) * kNrmSide + cx], e = lds.vert[(cy - 1) * kNrmSide + cx];
+        const f3 cr = cross(f3{a.x - b.x, a.y - b.y, a.z - b.z}, f3{c.x - e.x, c.y - e.y, c.z - e.z});
+        const f3 nn = kHalf ? mul(cr, __builtin_amdgcn_rsqf(dot(cr, cr))) : normalised(cr);
+        n = make_float4(nn.x, nn.y, nn.z, 1.0f);
+    }
+    store4<kHalf>(p.out3, i, n);
+    if (p.out4) {  // level 0: the pixel's integration sample
+        const float dist = static_cast<const float *>(p.in2)[i];
+        const unsigned char *col = static_cast<const unsigned char *>(p.in3) + 3 * i;
+        static_cast<float4 *>(p.out4)[i] = integration_sample<kHalf>(dist, n, col);
+    }
+}
+
+// Part p's workgroup t (the tile it covers); the staged parts keep every lane
+// to the barrier.
+LF_DEV void part_wg(const Part &p, int t, TileLds &lds) {
+    const int tiles_x = (p.w + 15) >> 4;
+    const int x0 = (t % tiles_x) * kTile, y0 = (t / tiles_x) * kTile;
+    if (p.kind == kFrameIn) {
+        if (p.half) frame_in_tile<true>(p, x0, y0, lds);
+        else frame_in_tile<false>(p, x0, y0, lds);
+        return;
+    }
+    if (p.kind == kLevelMaps) {
+        if (p.half) level_maps_tile<true>(p, x0, y0, lds);
+        else level_maps_tile<false>(p, x0, y0, lds);
+        return;
+    }
+    const int x = x0 + (threadIdx.x & 15), y = y0 + (threadIdx.x >> 4);
     if (x >= p.w || y >= p.h) return;
     if (p.half) frame_part<true>(p, x, y);
     else frame_part<false>(p, x, y);
 }
 
-__global__ __launch_bounds__(256) void k_frame_pass(Pass ps) { frame_pass_wg(ps, blockIdx.x); }
+// Workgroup wg of a pass.
+LF_DEV void frame_pass_wg(const Pass &ps, int wg, TileLds &lds) {
+    int pi = 0;
+#pragma unroll
+    for (int k = 1; k < kMaxParts; ++k)
+        if (k < ps.nparts && wg >= ps.part[k].wg_begin) pi = k;
+    const Part &p = ps.part[pi];
+    part_wg(p, wg - p.wg_begin, lds);
+}
+
+__global__ __launch_bounds__(256) void k_frame_pass(Pass ps) {
+    __shared__ TileLds lds;
+    frame_pass_wg(ps, blockIdx.x, lds);
+}
 
 // One pass of each of gridDim.y images: row y of the grid runs table[y] (its
 // part's pointers asserted global: they are read from memory).
 __global__ __launch_bounds__(256) void k_frame_pass_batch(const Pass *__restrict__ table) {
+    __shared__ TileLds lds;
     const Pass &ps = table[blockIdx.y];
     const int wg = blockIdx.x;
     if (wg >= ps.total_wg) return;
@@ -361,11 +444,7 @@ __global__ __launch_bounds__(256) void k_frame_pass_batch(const Pass *__restrict
     p.in0 = global_ptr(p.in0); p.in1 = global_ptr(p.in1); p.in2 = global_ptr(p.in2); p.in3 = global_ptr(p.in3);
     p.out0 = global_ptr(p.out0); p.out1 = global_ptr(p.out1); p.out2 = global_ptr(p.out2);
     p.out3 = global_ptr(p.out3); p.out4 = global_ptr(p.out4);
-    const int t = wg - p.wg_begin, tiles_x = (p.w + 15) >> 4;
-    const int x = (t % tiles_x) * 16 + (threadIdx.x & 15), y = (t / tiles_x) * 16 + (threadIdx.x >> 4);
-    if (x >= p.w || y >= p.h) return;
-    if (p.half) frame_part<true>(p, x, y);
-    else frame_part<false>(p, x, y);
+    part_wg(p, wg - p.wg_begin, lds);
 }
 
 dim3 grid16(int w, int h) { return dim3((w + 15) / 16, (h + 15) / 16); }

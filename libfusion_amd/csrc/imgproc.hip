@@ -279,15 +279,10 @@ LF_DEV void frame_part(const Part &p, int x, int y) {
     const size_t i = (size_t)y * w + x;
     switch (p.kind) {
     // kFrameIn (raw depth, bilateral depth, intensity and the colour copy of a
-    // frame) and kLevelMaps (intensity derivatives, vertex and normal maps of a
-    // level): frame_in_tile and level_maps_tile
-    case kPyramid:  // next level of the depth (resize) and intensity (pyrDown) pyramids
-        static_cast<float *>(p.out0)[i] = resize_at(static_cast<const float *>(p.in0), 2 * w, x, y);
-        store1<kHalf>(p.out1, i, pyr_down_at(PlaneAt<kHalf>{p.in1, 2 * w}, 2 * w, 2 * h, x, y));
-        break;
-    case kNmap:  // normal map of a stored vertex map
-        store4<kHalf>(p.out0, i, nmap_at<Vec4At<false>, kHalf>(Vec4At<false>{p.in0, w}, w, h, x, y));
-        break;
+    // frame), kPyramid (next level of the depth and intensity pyramids),
+    // kLevelMaps (intensity derivatives, vertex and normal maps of a level)
+    // and kNmap (normal map of a stored vertex map): frame_in_tile,
+    // pyramid_tile, level_maps_tile and nmap_tile
     case kVmapDown:  // half-size vertex map
         store4<false>(p.out0, i, resize4_at(Vec4At<false>{p.in0, 2 * w}, x, y));
         break;
@@ -297,17 +292,21 @@ LF_DEV void frame_part(const Part &p, int x, int y) {
     }
 }
 
-// The frame and level-map parts of a 16x16 output tile with their windows
-// staged in LDS: each source pixel of the tile and its apron is read (and
-// converted or back-projected) once instead of by every lane whose window
-// holds it, and the window taps need no border arithmetic.  The values and
-// the operation order are those of frame_part, so the outputs are the same.
+// The windowed parts (frame, pyramid, level maps, normal maps) of a 16x16
+// output tile with their windows staged in LDS: each source pixel of the tile
+// and its apron is read (and converted or back-projected) once instead of by
+// every lane whose window holds it, and the window taps need no border
+// arithmetic.  The values and the operation order are those of the per-pixel
+// functions (bilateral_at, pyr_down_at, back_project_at, nmap_at), so the
+// outputs are the same.
 constexpr int kTile = 16;
 constexpr int kBilApron = 2, kBilSide = kTile + 2 * kBilApron;  // 5x5 bilateral window
 constexpr int kNrmApron = 1, kNrmSide = kTile + 2 * kNrmApron;  // 3x3 (cross) normal window
+constexpr int kPyrSide = 2 * kTile + 4;  // the 2x source rows / columns of a pyrDown tile, 5-tap apron
 union TileLds {
     float depth[kBilSide * kBilSide];
     float4 vert[kNrmSide * kNrmSide];
+    float src[kPyrSide * kPyrSide];
 };
 
 // bilateral_at over the staged depths: the window of tile pixel (cx, cy) in
@@ -353,6 +352,65 @@ LF_DEV void frame_in_tile(const Part &p, int x0, int y0, TileLds &lds) {
     c[0] = r; c[1] = g; c[2] = b;
 }
 
+// kPyramid for the tile at (x0, y0): the depth resize as frame_part's, the
+// intensity pyrDown from the tile's staged source window (reflect-101 applied
+// when staging), in pyr_down_at's operation order.
+template <bool kHalf>
+LF_DEV void pyramid_tile(const Part &p, int x0, int y0, TileLds &lds) {
+    const int w = p.w, h = p.h, sw = 2 * w, sh = 2 * h;
+    const PlaneAt<kHalf> in{p.in1, sw};
+    for (int k = threadIdx.x; k < kPyrSide * kPyrSide; k += blockDim.x) {
+        const int sx = reflect101(2 * x0 - 2 + k % kPyrSide, sw), sy = reflect101(2 * y0 - 2 + k / kPyrSide, sh);
+        lds.src[k] = in(sx, sy);
+    }
+    __syncthreads();
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4, x = x0 + lx, y = y0 + ly;
+    if (x >= w || y >= h) return;
+    const size_t i = (size_t)y * w + x;
+    static_cast<float *>(p.out0)[i] = resize_at(static_cast<const float *>(p.in0), sw, x, y);
+    const float *s = lds.src + (2 * ly) * kPyrSide + 2 * lx;  // tap (j, k): s[j * kPyrSide + k]
+    float rs[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        const float *r = s + j * kPyrSide;
+        rs[j] = (r[0] + r[4]) + 4.0f * (r[1] + r[3]) + 6.0f * r[2];
+    }
+    store1<kHalf>(p.out1, i, ((rs[0] + rs[4]) + 4.0f * (rs[1] + rs[3]) + 6.0f * rs[2]) * (1.0f / 256.0f));
+}
+
+// nmap_at of pixel (x, y), tile pixel (lx, ly), from the staged vertices:
+// its neighbours (min(x+1, w), y), (max(x-1, 0), y), (x, min(y+1, h)),
+// (x, max(y-1, 0)) are the tile's, staged with their coordinates clamped
+// into the image.
+template <bool kFast>
+LF_DEV float4 normal_tile(const float4 *v, int lx, int ly, int x, int y, int w, int h) {
+    const float qnan = __int_as_float(0x7fc00000);
+    if (x >= w - 1 || y >= h - 1) return make_float4(qnan, qnan, qnan, qnan);
+    const int cx = lx + kNrmApron, cy = ly + kNrmApron;
+    const float4 a = v[cy * kNrmSide + cx + 1], b = v[cy * kNrmSide + cx - 1], c = v[(cy + 1) * kNrmSide + cx],
+                 e = v[(cy - 1) * kNrmSide + cx];
+    const f3 cr = cross(f3{a.x - b.x, a.y - b.y, a.z - b.z}, f3{c.x - e.x, c.y - e.y, c.z - e.z});
+    const f3 n = kFast ? mul(cr, __builtin_amdgcn_rsqf(dot(cr, cr))) : normalised(cr);
+    return make_float4(n.x, n.y, n.z, 1.0f);
+}
+
+// kNmap for the tile at (x0, y0): the stored vertices of the tile and a
+// one-pixel apron staged once.
+template <bool kHalf>
+LF_DEV void nmap_tile(const Part &p, int x0, int y0, TileLds &lds) {
+    const int w = p.w, h = p.h;
+    const float4 *vm = static_cast<const float4 *>(p.in0);
+    for (int k = threadIdx.x; k < kNrmSide * kNrmSide; k += blockDim.x) {
+        const int sx = min(max(x0 - kNrmApron + k % kNrmSide, 0), w - 1),
+                  sy = min(max(y0 - kNrmApron + k / kNrmSide, 0), h - 1);
+        lds.vert[k] = vm[(size_t)sy * w + sx];
+    }
+    __syncthreads();
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4, x = x0 + lx, y = y0 + ly;
+    if (x >= w || y >= h) return;
+    store4<kHalf>(p.out0, (size_t)y * w + x, normal_tile<kHalf>(lds.vert, lx, ly, x, y, w, h));
+}
+
 // kLevelMaps for the tile at (x0, y0): the vertices of the tile and a
 // one-pixel apron (coordinates clamped into the image, as nmap_at's
 // neighbours are) back-projected once into LDS.
@@ -373,18 +431,8 @@ LF_DEV void level_maps_tile(const Part &p, int x0, int y0, TileLds &lds) {
     derivative_at(PlaneAt<kHalf>{p.in1, w}, w, h, x, y, gx, gy);
     store1<kHalf>(p.out0, i, gx);
     store1<kHalf>(p.out1, i, gy);
-    const int cx = lx + kNrmApron, cy = ly + kNrmApron;
-    store4<false>(p.out2, i, lds.vert[cy * kNrmSide + cx]);  // vertices stay FP32
-    // nmap_at: neighbours (min(x+1, w), y), (max(x-1, 0), y), (x, min(y+1, h)), (x, max(y-1, 0))
-    const float qnan = __int_as_float(0x7fc00000);
-    float4 n = make_float4(qnan, qnan, qnan, qnan);
-    if (!(x >= w - 1 || y >= h - 1)) {
-        const float4 a = lds.vert[cy * kNrmSide + cx + 1], b = lds.vert[cy * kNrmSide + cx - 1],
-                     c = lds.vert[(cy + 1) * kNrmSide + cx], e = lds.vert[(cy - 1) * kNrmSide + cx];
-        const f3 cr = cross(f3{a.x - b.x, a.y - b.y, a.z - b.z}, f3{c.x - e.x, c.y - e.y, c.z - e.z});
-        const f3 nn = kHalf ? mul(cr, __builtin_amdgcn_rsqf(dot(cr, cr))) : normalised(cr);
-        n = make_float4(nn.x, nn.y, nn.z, 1.0f);
-    }
+    store4<false>(p.out2, i, lds.vert[(ly + kNrmApron) * kNrmSide + lx + kNrmApron]);  // vertices stay FP32
+    const float4 n = normal_tile<kHalf>(lds.vert, lx, ly, x, y, w, h);
     store4<kHalf>(p.out3, i, n);
     if (p.out4) {  // level 0: the pixel's integration sample
         const float dist = static_cast<const float *>(p.in2)[i];
@@ -406,6 +454,16 @@ LF_DEV void part_wg(const Part &p, int t, TileLds &lds) {
     if (p.kind == kLevelMaps) {
         if (p.half) level_maps_tile<true>(p, x0, y0, lds);
         else level_maps_tile<false>(p, x0, y0, lds);
+        return;
+    }
+    if (p.kind == kNmap) {
+        if (p.half) nmap_tile<true>(p, x0, y0, lds);
+        else nmap_tile<false>(p, x0, y0, lds);
+        return;
+    }
+    if (p.kind == kPyramid) {
+        if (p.half) pyramid_tile<true>(p, x0, y0, lds);
+        else pyramid_tile<false>(p, x0, y0, lds);
         return;
     }
     const int x = x0 + (threadIdx.x & 15), y = y0 + (threadIdx.x >> 4);

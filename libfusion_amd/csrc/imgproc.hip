@@ -152,21 +152,6 @@ struct PlaneAt {
         else return static_cast<const float *>(p)[i];
     }
 };
-// a map of float4, or of 4 x FP16 (kHalf)
-template <bool kHalf>
-struct Vec4At {
-    const void *p; int w;
-    LF_DEV float4 operator()(int x, int y) const {
-        const size_t i = (size_t)y * w + x;
-        if constexpr (kHalf) {
-            const uint2 u = static_cast<const uint2 *>(p)[i];
-            return make_float4(half_bits_to_float((unsigned short)u.x), half_bits_to_float((unsigned short)(u.x >> 16)),
-                               half_bits_to_float((unsigned short)u.y), half_bits_to_float((unsigned short)(u.y >> 16)));
-        } else {
-            return static_cast<const float4 *>(p)[i];
-        }
-    }
-};
 template <bool kHalf>
 LF_DEV void store1(void *p, size_t i, float v) {
     if constexpr (kHalf) static_cast<unsigned short *>(p)[i] = float_to_half_bits(v);
@@ -253,7 +238,7 @@ __global__ void k_nmap(const float *__restrict__ vm, float *__restrict__ nm, int
 // workgroups of the parts follow each other in the 1-D grid.
 
 constexpr int kMaxParts = 4;
-enum PartKind { kFrameIn, kPyramid, kLevelMaps, kNmap, kVmapDown, kVmapDown2 };
+enum PartKind { kFrameIn, kPyramid, kLevelMaps, kNmap };
 
 struct Part {
     int kind, w, h, wg_begin;  // w, h: the part's output size
@@ -268,29 +253,6 @@ struct Pass {
     int nparts;
     int total_wg;  // workgroups of all parts
 };
-
-// Pixel (x, y) of a part.  kHalf (mixed precision): the intensity, gradient
-// and normal planes are FP16 (read and written; depth, vertices, colour and
-// samples are not), and the bilateral weights, normals and sample weights use
-// the hardware exponential, reciprocal square root, sine and reciprocal.
-template <bool kHalf>
-LF_DEV void frame_part(const Part &p, int x, int y) {
-    const int w = p.w, h = p.h;
-    const size_t i = (size_t)y * w + x;
-    switch (p.kind) {
-    // kFrameIn (raw depth, bilateral depth, intensity and the colour copy of a
-    // frame), kPyramid (next level of the depth and intensity pyramids),
-    // kLevelMaps (intensity derivatives, vertex and normal maps of a level)
-    // and kNmap (normal map of a stored vertex map): frame_in_tile,
-    // pyramid_tile, level_maps_tile and nmap_tile
-    case kVmapDown:  // half-size vertex map
-        store4<false>(p.out0, i, resize4_at(Vec4At<false>{p.in0, 2 * w}, x, y));
-        break;
-    case kVmapDown2:  // quarter-size vertex map, through the half-size one
-        store4<false>(p.out0, i, resize4_quarter_at(Vec4At<false>{p.in0, 4 * w}, x, y));
-        break;
-    }
-}
 
 // The windowed parts (frame, pyramid, level maps, normal maps) of a 16x16
 // output tile with their windows staged in LDS: each source pixel of the tile
@@ -352,7 +314,7 @@ LF_DEV void frame_in_tile(const Part &p, int x0, int y0, TileLds &lds) {
     c[0] = r; c[1] = g; c[2] = b;
 }
 
-// kPyramid for the tile at (x0, y0): the depth resize as frame_part's, the
+// kPyramid for the tile at (x0, y0): the depth resize (resize_at), the
 // intensity pyrDown from the tile's staged source window (reflect-101 applied
 // when staging), in pyr_down_at's operation order.
 template <bool kHalf>
@@ -395,7 +357,9 @@ LF_DEV float4 normal_tile(const float4 *v, int lx, int ly, int x, int y, int w, 
 }
 
 // kNmap for the tile at (x0, y0): the stored vertices of the tile and a
-// one-pixel apron staged once.
+// one-pixel apron staged once.  out1 / out2 (when set): the tile's 8x8
+// half-size and 4x4 quarter-size vertices (kVmapDown's and kVmapDown2's
+// values) from the same staged vertices, so the vertex map is read once.
 template <bool kHalf>
 LF_DEV void nmap_tile(const Part &p, int x0, int y0, TileLds &lds) {
     const int w = p.w, h = p.h;
@@ -406,7 +370,19 @@ LF_DEV void nmap_tile(const Part &p, int x0, int y0, TileLds &lds) {
         lds.vert[k] = vm[(size_t)sy * w + sx];
     }
     __syncthreads();
-    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4, x = x0 + lx, y = y0 + ly;
+    // tile pixel (tx, ty) of the staged vertices (in-range pixels only)
+    const auto at = [&](int tx, int ty) { return lds.vert[(ty + kNrmApron) * kNrmSide + tx + kNrmApron]; };
+    const int t = threadIdx.x;
+    if (p.out1 && t < 64) {  // half-size pixel (x0 / 2 + i, y0 / 2 + j)
+        const int i = t & 7, j = t >> 3, hx = x0 / 2 + i, hy = y0 / 2 + j, hw = w / 2, hh = h / 2;
+        if (hx < hw && hy < hh)
+            store4<false>(p.out1, (size_t)hy * hw + hx, resize4_at(at, i, j));
+    } else if (p.out2 && t >= 64 && t < 80) {  // quarter-size pixel (x0 / 4 + i, y0 / 4 + j)
+        const int i = (t - 64) & 3, j = (t - 64) >> 2, qx = x0 / 4 + i, qy = y0 / 4 + j, qw = w / 4, qh = h / 4;
+        if (qx < qw && qy < qh)
+            store4<false>(p.out2, (size_t)qy * qw + qx, resize4_quarter_at(at, i, j));
+    }
+    const int lx = t & 15, ly = t >> 4, x = x0 + lx, y = y0 + ly;
     if (x >= w || y >= h) return;
     store4<kHalf>(p.out0, (size_t)y * w + x, normal_tile<kHalf>(lds.vert, lx, ly, x, y, w, h));
 }
@@ -441,8 +417,11 @@ LF_DEV void level_maps_tile(const Part &p, int x0, int y0, TileLds &lds) {
     }
 }
 
-// Part p's workgroup t (the tile it covers); the staged parts keep every lane
-// to the barrier.
+// Part p's workgroup t (the tile it covers); every lane reaches the part's
+// barrier.  p.half (mixed precision): the intensity, gradient and normal
+// planes are FP16 (read and written; depth, vertices, colour and samples are
+// not), and the bilateral weights, normals and sample weights use the
+// hardware exponential, reciprocal square root, sine and reciprocal.
 LF_DEV void part_wg(const Part &p, int t, TileLds &lds) {
     const int tiles_x = (p.w + 15) >> 4;
     const int x0 = (t % tiles_x) * kTile, y0 = (t / tiles_x) * kTile;
@@ -464,12 +443,7 @@ LF_DEV void part_wg(const Part &p, int t, TileLds &lds) {
     if (p.kind == kPyramid) {
         if (p.half) pyramid_tile<true>(p, x0, y0, lds);
         else pyramid_tile<false>(p, x0, y0, lds);
-        return;
     }
-    const int x = x0 + (threadIdx.x & 15), y = y0 + (threadIdx.x >> 4);
-    if (x >= p.w || y >= p.h) return;
-    if (p.half) frame_part<true>(p, x, y);
-    else frame_part<false>(p, x, y);
 }
 
 // Workgroup wg of a pass.
@@ -586,8 +560,9 @@ int plan_upload(const uint16_t *depth_u16, const unsigned char *rgb, float depth
 
 // DeviceImage::resize_device_map: the vertex pyramid of a raycast level-0 map
 // and every level's normals, two levels per pass (a pass with base level b
-// writes the normals still missing below b + 1 and the vertices of levels
-// b + 1 and b + 2, the latter resized twice on the fly).
+// writes the normals still missing below b + 1 and, from the base level's
+// staged tiles, the vertices of levels b + 1 and b + 2, the latter resized
+// twice).
 int plan_refresh(int levels, const lf_intrinsics *K, const LfImageLevels &m, std::vector<Pass> &out) {
     if (int rc = check_pyramid(levels, K)) return rc;
     int nmap_done = 0;
@@ -596,16 +571,10 @@ int plan_refresh(int levels, const lf_intrinsics *K, const LfImageLevels &m, std
         for (; nmap_done <= b && nmap_done < levels; ++nmap_done) {
             Part p = make_part(kNmap, K[nmap_done].width, K[nmap_done].height);
             p.in0 = m.vmap[nmap_done]; p.out0 = m.nmap[nmap_done];
-            ps.part[ps.nparts++] = p;
-        }
-        if (b + 1 < levels) {
-            Part p = make_part(kVmapDown, K[b + 1].width, K[b + 1].height);
-            p.in0 = m.vmap[b]; p.out0 = m.vmap[b + 1];
-            ps.part[ps.nparts++] = p;
-        }
-        if (b + 2 < levels) {
-            Part p = make_part(kVmapDown2, K[b + 2].width, K[b + 2].height);
-            p.in0 = m.vmap[b]; p.out0 = m.vmap[b + 2];
+            if (nmap_done == b) {  // the base level's tiles also write levels b + 1 and b + 2
+                if (b + 1 < levels) p.out1 = m.vmap[b + 1];
+                if (b + 2 < levels) p.out2 = m.vmap[b + 2];
+            }
             ps.part[ps.nparts++] = p;
         }
         seal_pass(ps, m.half_maps);

@@ -241,23 +241,36 @@ public:
                   std::vector<unsigned char> &jobs, std::vector<int> &grids,
                   std::vector<unsigned char> *upload = nullptr);
     void reset();
-    DeviceImage &get_current_image() { return *curr_; }
+    // the frame before the reference: the buffer the reference's two-buffer
+    // swap leaves as current after a frame (rgbd_odometry.cpp:41,50)
+    DeviceImage &get_current_image() { return *spare_; }
     DeviceImage &get_reference_image() { return *ref_; }
     DenseTracking &tracker() { return tracker_; }
     const float *pose24() const;
     bool initialized() const { return initialized_; }
-    // which of the two frame buffers is current (flips with every tracked frame)
+    // which of the three frame buffers is current (0, 1, 2 in turn, one step
+    // per tracked frame)
     int buffer_parity() const { return parity_; }
     // the host side of a tracked frame whose device work was replayed from a graph
-    void replayed_frame() { std::swap(curr_, ref_); parity_ ^= 1; trackingLost = false; }
+    void replayed_frame() { advance(); trackingLost = false; }
     bool trackingLost = false;
 
 private:
+    // After a frame: it becomes the reference, and the next frame is uploaded
+    // into the buffer of the frame before the reference, which the tracking
+    // of the reference read last.  With three buffers the next upload does
+    // not overwrite anything the current frame's tracking reads, so it can
+    // run beside that tracking.
+    void advance() {
+        std::swap(ref_, spare_);
+        std::swap(curr_, ref_);
+        parity_ = (parity_ + 1) % 3;
+    }
     int parity_ = 0;
     std::vector<IntrinsicMatrix> K_;
     std::vector<int> iterations_;
     hipStream_t stream_;
-    std::unique_ptr<DeviceImage> curr_, ref_;
+    std::unique_ptr<DeviceImage> curr_, ref_, spare_;
     DenseTracking tracker_;
     bool initialized_ = false;
 };
@@ -325,7 +338,7 @@ private:
     int enqueue_frame(const uint16_t *depth, const unsigned char *rgb);
     void drop_graphs();
     bool graphs_ = false;
-    hipGraphExec_t exec_[2] = {nullptr, nullptr};
+    hipGraphExec_t exec_[3] = {nullptr, nullptr, nullptr};  // per frame buffer
     GpuImage depth_in_, rgb_in_;
     GpuImage host_depth_, host_rgb_;  // device staging of process_images_host
     hipEvent_t take_event();
@@ -412,8 +425,6 @@ private:
     std::vector<hipEvent_t> ready_;  // per system: its tracking and fusion are enqueued
     hipStream_t aux_ = nullptr;      // the step's descriptor table and uploads
     hipEvent_t uploaded_ = nullptr;  // the step's uploads
-    hipEvent_t tracked_ = nullptr;
-    bool tracked_once_ = false;
     bool half_maps_ = false;  // the systems are mixed precision
     int parity_ = 0;
     // the step's descriptors (frame passes, tracking jobs, raycast jobs),

@@ -468,6 +468,7 @@ DenseOdometry::DenseOdometry(const IntrinsicMatrix &base, int num_pyr, std::vect
     BuildIntrinsicPyramid(base, K_, num_pyr);
     curr_.reset(new DeviceImage(base, num_pyr, stream, half_maps));
     ref_.reset(new DeviceImage(base, num_pyr, stream, half_maps));
+    spare_.reset(new DeviceImage(base, num_pyr, stream, half_maps));
 }
 
 const float *DenseOdometry::pose24() const { return tracker_.state()->pose24; }
@@ -479,8 +480,7 @@ int DenseOdometry::trackFrame(const uint16_t *depth, const unsigned char *rgb, f
         double T[12];
         initial_pose.matrix3x4(T);
         LF_RC(lf_track_set_pose(tracker_.state(), T, stream_));
-        std::swap(curr_, ref_);
-        parity_ ^= 1;
+        advance();
         initialized_ = true;
         return LF_OK;
     }
@@ -491,8 +491,7 @@ int DenseOdometry::trackFrame(const uint16_t *depth, const unsigned char *rgb, f
     c.max_iterations_ = iterations_;
     // compute_transform ends with the finish, which also chains the odometry pose
     LF_RC(tracker_.compute_transform(*ref_, *curr_, c));
-    std::swap(curr_, ref_);
-    parity_ ^= 1;
+    advance();
     trackingLost = false;
     return LF_OK;
 }
@@ -508,8 +507,7 @@ int DenseOdometry::planFrame(const uint16_t *depth, const unsigned char *rgb, fl
         double T[12];
         initial_pose.matrix3x4(T);
         LF_RC(lf_track_set_pose(tracker_.state(), T, stream_));
-        std::swap(curr_, ref_);
-        parity_ ^= 1;
+        advance();
         initialized_ = true;
         return LF_OK;
     }
@@ -519,8 +517,7 @@ int DenseOdometry::planFrame(const uint16_t *depth, const unsigned char *rgb, fl
     c.intrinsics_pyr_ = K_;
     c.max_iterations_ = iterations_;
     LF_RC(tracker_.plan(*ref_, *curr_, c, jobs, grids));
-    std::swap(curr_, ref_);
-    parity_ ^= 1;
+    advance();
     trackingLost = false;
     return LF_OK;
 }
@@ -682,9 +679,7 @@ Batch::Batch(const lf_system_config *cfgs, int n) {
     ready_.assign(n, nullptr);
     for (hipEvent_t &e : ready_)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) fail();
-    if (hipEventCreateWithFlags(&uploaded_, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&tracked_, hipEventDisableTiming) != hipSuccess)
-        fail();
+    if (hipEventCreateWithFlags(&uploaded_, hipEventDisableTiming) != hipSuccess) fail();
     for (int k = 0; k < 2; ++k)
         if (hipEventCreateWithFlags(&table_read_[k], hipEventDisableTiming) != hipSuccess) fail();
     // a shape tuned before in this process (or read from the tuning file) starts with its choice
@@ -859,8 +854,6 @@ void Batch::release() {
     ready_.clear();
     if (uploaded_) (void)hipEventDestroy(uploaded_);
     uploaded_ = nullptr;
-    if (tracked_) (void)hipEventDestroy(tracked_);
-    tracked_ = nullptr;
     for (int k = 0; k < 2; ++k) {
         if (table_host_[k]) (void)hipHostFree(table_host_[k]);
         if (table_dev_[k]) (void)hipFree(table_dev_[k]);
@@ -1079,13 +1072,14 @@ int Batch::step(const uint16_t *const *depth_in, const unsigned char *const *rgb
     LfRayBatchDims ray;
     LF_RC(lf_raycast_batch_make(items.data(), n, host + off_ray, &ray));
     if (zrange_wgs_ > 0) ray.zgrid_max = zrange_wgs_;
-    // Upload stream: the table, then the uploads.  The uploads rewrite the
-    // images that the previous step's tracking read (the previous step's
-    // fusion, raycast and re-pyramid use the other image of each system), and
-    // the table buffer was last read by the step before that: both wait only
-    // for the previous step's tracking, so they run beside its fusion and raycast.
-    // Without the overlap (upload_overlap_ 0) they run on the batch stream
-    // ahead of the step's tracking.
+    // Upload stream: the table, then the uploads.  The uploads rewrite each
+    // system's frame buffer that the step before the previous one tracked
+    // against (three buffers per system), and the table buffer was last read
+    // by that same step, whose end the host waited for above: nothing on the
+    // device is left to wait for, so the uploads run as soon as they are
+    // enqueued, beside the previous step's tracking (latency-bound launches)
+    // when the host keeps ahead.  Without the overlap (upload_overlap_ 0) they
+    // run on the batch stream ahead of the step's tracking.
     hipStream_t up_st = upload_overlap_ ? aux_ : stream_;
     // The step's device work in three parts: the table copy and uploads (on
     // up_st), the Gauss-Newton launches, then fusion, raycast and re-pyramids
@@ -1149,7 +1143,6 @@ int Batch::step(const uint16_t *const *depth_in, const unsigned char *const *rgb
                           [&]() { return enqueue_fuse(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr); }));
             g.sig = sig;
         }
-        if (upload_overlap_ && tracked_once_) LF_HIP_CHECK(hipStreamWaitEvent(aux_, tracked_, 0));
         if (from_host)
             for (int i = 0; i < n; ++i) {
                 LF_HIP_CHECK(hipMemcpyAsync(depth_in_[i].ptr<void>(), depth_in[i], depth_in_[i].bytes(),
@@ -1163,15 +1156,12 @@ int Batch::step(const uint16_t *const *depth_in, const unsigned char *const *rgb
             LF_HIP_CHECK(hipStreamWaitEvent(stream_, uploaded_, 0));
         }
         LF_HIP_CHECK(hipGraphLaunch(g.track, stream_));
-        LF_HIP_CHECK(hipEventRecord(tracked_, stream_));
-        tracked_once_ = true;
         LF_HIP_CHECK(hipGraphLaunch(g.fuse, stream_));
         LF_HIP_CHECK(hipEventRecord(table_read_[k], stream_));
         table_used_[k] = true;
         parity_ ^= 1;
         return LF_OK;
     }
-    if (upload_overlap_ && tracked_once_) LF_HIP_CHECK(hipStreamWaitEvent(aux_, tracked_, 0));
     LF_HIP_CHECK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, up_st));
     if (from_host) {
         // the staging buffers were last read by the previous step's upload
@@ -1202,8 +1192,6 @@ int Batch::step(const uint16_t *const *depth_in, const unsigned char *const *rgb
     if (ev[2].a) LF_HIP_CHECK(hipEventRecord(ev[2].a, stream_));
     LF_RC(enqueue_track());
     if (ev[2].b) LF_HIP_CHECK(hipEventRecord(ev[2].b, stream_));
-    LF_HIP_CHECK(hipEventRecord(tracked_, stream_));
-    tracked_once_ = true;
     // block allocation, visible lists and integration of every map, the
     // z-range and raycast, the re-pyramids of the raycast maps
     LF_RC(enqueue_fuse(ev[0].a, ev[0].b, ev[3].a, ev[3].b, ev[1].a, ev[1].b));

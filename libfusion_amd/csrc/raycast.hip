@@ -263,7 +263,13 @@ LF_DEV unsigned long long lds_tag(int bx, int by, int bz) {
 // outside the range has no sample in any representable block (block keys hold
 // 21 bits per axis) and finds nothing.
 constexpr float kRayRange = 1.0e9f;
-LF_DEV int ray_floor(float v) { return (int)floorf(v); }
+// floor to int in one instruction (the floor-rounding conversion), equal to
+// (int)floorf(v) for every |v| < 2^31
+LF_DEV int ray_floor(float v) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
 // A float that every active lane holds alike, in a scalar register.
 LF_DEV float uniform(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
 // 1 when v's sign bit is clear, else 0 (for a v neither NaN nor zero: v > 0)
@@ -331,9 +337,12 @@ LF_DEV unsigned short sdf_bits(const RayDev &r, int vi) { return at_byte(reinter
 // ray.  A voxel is valid when its weight is non-zero, which the sdf plane
 // encodes (kSdfUnseen; kHalfUnseen, a NaN, in mixed precision), so one 2-byte
 // load per sample decides both validity and value.
+// floor_out (or null) receives floor(p).
 template <bool kMixed>
-LF_DEV float read_sdf(const RayDev &r, BlockCache &bc, f3 p, bool &valid, int *voxel_out = nullptr) {
+LF_DEV float read_sdf(const RayDev &r, BlockCache &bc, f3 p, bool &valid, int *voxel_out = nullptr,
+                      i3 *floor_out = nullptr) {
     const int vx = ray_floor(p.x), vy = ray_floor(p.y), vz = ray_floor(p.z);
+    if (floor_out) *floor_out = i3{vx, vy, vz};
     const int b = resolve(r, bc, vx >> 3, vy >> 3, vz >> 3);
     valid = false;
     if (b < 0) return __int_as_float(0x7fc00000);
@@ -363,18 +372,23 @@ LF_DEV float read_sdf(const RayDev &r, BlockCache &bc, f3 p, bool &valid, int *v
 // replaces corner 0's value).  An invalid corner is NaN and the corner values are
 // finite otherwise, so the result is valid exactly when it is not NaN.  The
 // arithmetic is the reference's order.
+// f0: floor(p) when the caller has it from the sample read at p, whose
+// block the lane's cache then holds; else computed and resolved here.
 template <bool kMixed>
-LF_DEV float read_interp(const RayDev &r, BlockCache &bc, f3 p, bool &valid) {
-    const float fx = floorf(p.x), fy = floorf(p.y), fz = floorf(p.z);
+LF_DEV float read_interp(const RayDev &r, BlockCache &bc, f3 p, bool &valid, const i3 *f0 = nullptr) {
+    // floor(p) as int and as float (exact: an int from a float converts back
+    // to that float)
+    const i3 fl = f0 ? *f0 : i3{ray_floor(p.x), ray_floor(p.y), ray_floor(p.z)};
+    const int cx[2] = {fl.x, ray_floor(p.x + 1.0f)}, cy[2] = {fl.y, ray_floor(p.y + 1.0f)},
+              cz[2] = {fl.z, ray_floor(p.z + 1.0f)};
+    const float fx = (float)cx[0], fy = (float)cy[0], fz = (float)cz[0];
     const f3 xyz = {p.x - fx, p.y - fy, p.z - fz};
-    const int cx[2] = {(int)fx, ray_floor(p.x + 1.0f)}, cy[2] = {(int)fy, ray_floor(p.y + 1.0f)},
-              cz[2] = {(int)fz, ray_floor(p.z + 1.0f)};
     // corner order of the reference: (0,0,0) (1,0,0) (0,1,0) (1,1,0) (0,0,1) (1,0,1) (0,1,1) (1,1,1)
     const int bx0 = cx[0] >> 3, bx1 = cx[1] >> 3, by0 = cy[0] >> 3, by1 = cy[1] >> 3, bz0 = cz[0] >> 3,
               bz1 = cz[1] >> 3;
     const int m = (bx1 != bx0 ? 1 : 0) | (by1 != by0 ? 2 : 0) | (bz1 != bz0 ? 4 : 0);
     int bb[8];
-    bb[0] = resolve(r, bc, bx0, by0, bz0);
+    bb[0] = f0 ? bc.b : resolve(r, bc, bx0, by0, bz0);
 #pragma unroll
     for (int k = 1; k < 8; ++k) bb[k] = bb[0];
     for (int e = m & -m; e != 0; e = (e - m) & m) {  // the nonzero subsets of m, increasing
@@ -484,7 +498,8 @@ LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char 
         bool valid = false, back_face = false;
         while (dist_s < dist_e) {
             last_sdf = sdf;
-            sdf = read_sdf<kMixed>(r, bc, res, valid);
+            i3 fl;
+            sdf = read_sdf<kMixed>(r, bc, res, valid, nullptr, &fl);
             int quiet = 0;  // further samples surely inside the same unallocated box
             if (bc.b < 0) {
                 const int k = bc.empty_region ? 2 : 0, sh = k + 3;  // region: 32 voxels a side; block: 8
@@ -495,7 +510,7 @@ LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char 
                 // samples at 2k (k >= 1) with 2k < texit - 0.05
                 if (texit < 1e4f) quiet = max(0, (int)ceilf(0.5f * (texit - 0.05f)) - 1);
             }
-            if (sdf <= 0.5f && sdf >= -0.5f) sdf = read_interp<kMixed>(r, bc, res, valid);
+            if (sdf <= 0.5f && sdf >= -0.5f) sdf = read_interp<kMixed>(r, bc, res, valid, &fl);
             if (sdf <= 0.0f) break;
             if (sdf >= 0.0f && last_sdf < 0.0f) { back_face = true; break; }
             step = valid ? fmaxf(sdf * step_scale, 1.0f) : 2.0f;

@@ -417,18 +417,30 @@ def test_image_pipeline_matches_oracle():
     assert np.array_equal(vh.cpu().numpy(), O.resize_half(ovm), equal_nan=True)
 
 
-def test_device_image_passes_match_per_op_kernels():
-    """DeviceImage upload / resize_device_map (fused launches) equal the
-    stand-alone C-ABI image kernels bit for bit, at every pyramid level."""
+@pytest.mark.parametrize("size", [None, (200, 152)])
+def test_device_image_passes_match_per_op_kernels(size):
+    """DeviceImage upload / resize_device_map (fused launches, windows staged
+    per 16x16 tile) equal the stand-alone C-ABI image kernels bit for bit, at
+    every pyramid level: at 640x480, and at 200x152 (100x76, 50x38), where
+    every level ends in partial tiles and the borders' reflected and clamped
+    windows fall inside them."""
     sc = Scene(1, 1)
-    d16, rgb = sc.frame(4)
-    K = capi.intrinsics(sc.width, sc.height, sc.fx, sc.fy, sc.cx, sc.cy)
+    if size is None:
+        K = capi.intrinsics(sc.width, sc.height, sc.fx, sc.fy, sc.cx, sc.cy)
+        frame = sc.frame
+    else:
+        s = size[0] / sc.width
+        K = capi.intrinsics(size[0], size[1], sc.fx * s, sc.fy * s, (sc.cx + 0.5) * s - 0.5, (sc.cy + 0.5) * s - 0.5)
+
+        def frame(f):
+            return sc.render(sc.pose(f), K.width, K.height, K.fx, K.fy, K.cx, K.cy, noise=True, frame_id=f)
+    d16, rgb = frame(4)
     levels = 3
     img = capi.Image(K, levels)
     lib, st, P = capi.load(), capi.stream_ptr(), capi.ptr
     d16_t, rgb_t = dev(d16), dev(rgb)
     img.upload(d16_t, rgb_t, sc.depth_scale)
-    W, H = sc.width, sc.height
+    W, H = K.width, K.height
     raw = torch.empty((H, W), device="cuda")
     capi.check(lib.lf_depth_to_float(P(d16_t), P(raw), W, H, float(np.float32(sc.depth_scale)), st))
     depth = [torch.empty_like(raw)]
@@ -462,7 +474,7 @@ def test_device_image_passes_match_per_op_kernels():
                            (capi.PLANE_DY, dy), (capi.PLANE_VMAP, vm), (capi.PLANE_NMAP, nm)):
             assert same(img.plane(l, plane), ref), (l, plane)
     # resize_device_map from a new level-0 vertex map (another frame's vertices)
-    d16b, _ = sc.frame(9)
+    d16b, _ = frame(9)
     v0 = torch.from_numpy(O.back_project(d16b.astype(np.float32) * np.float32(sc.depth_scale), o_K(K))).cuda()
     torch.cuda.synchronize()
     capi.copy_device_to(img.plane_ptr(0, capi.PLANE_VMAP), v0)

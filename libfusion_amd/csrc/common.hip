@@ -70,6 +70,10 @@ static int check_params(const lf_map_params *p) {
         return lf_set_error(LF_ERR_ARG, "pool of %d blocks exceeds 32-bit voxel indexing", p->num_blocks);
     if (!(p->voxel_size > 0) || !(p->truncation > 0) || !(p->zmin_raycast > 0) || !(p->zmin_update >= 0))
         return lf_set_error(LF_ERR_ARG, "bad voxel size / truncation / z ranges");
+    // the raycast's step scale (truncation in voxels) and its half are normal floats
+    const float step_scale = p->truncation * (1.0f / p->voxel_size);
+    if (!(step_scale >= 0x1p-125f && step_scale <= 0x1p100f))
+        return lf_set_error(LF_ERR_ARG, "truncation / voxel size %g out of range", (double)step_scale);
     if (p->precision != LF_PRECISION_FP32 && p->precision != LF_PRECISION_MIXED)
         return lf_set_error(LF_ERR_ARG, "unknown map precision %d", p->precision);
     return LF_OK;
@@ -86,6 +90,8 @@ size_t lf_weight_bytes(const lf_map_params *p) { return sizeof(float); }
 // map packs its colour into the weight words.
 int lf_check_storage(const lf_map_storage *s, const lf_map_params *p) {
     if (!s) return lf_set_error(LF_ERR_ARG, "null map storage");
+    if (p)
+        if (int rc = check_params(p)) return rc;
     const bool need_rgb = p && p->precision != LF_PRECISION_MIXED;
     const struct { const void *ptr; const char *name; } arrays[] = {
         {s->keys, "keys"}, {s->slot_block, "slot_block"}, {s->slot_flag, "slot_flag"},

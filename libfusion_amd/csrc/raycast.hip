@@ -467,7 +467,6 @@ LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char 
         const float *Twc = pose24;
         // uniform over the workgroup (one map): kept in scalar registers
         const float inv_vs = uniform(1.0f / r.voxel_size);
-        const float step_scale = uniform(r.truncation * inv_vs);
         f3 pt = {((float)x - K.cx) * K.invfx * z0, ((float)y - K.cy) * K.invfy * z0, z0};
         float dist_s = norm(pt) * inv_vs;
         const f3 bs = mul(xform(Twc, pt), inv_vs);
@@ -494,6 +493,13 @@ LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char 
         // bit a set: the ray runs towards + along axis a (inv_dir's sign bit
         // clear, +inf included), so a box's exit face on that axis is its far side
         const unsigned int fwd = positive(inv_dir.x) | positive(inv_dir.y) << 1 | positive(inv_dir.z) << 2;
+        // The march keeps 2 dir (exact), which a quiet step adds as it is; a
+        // step of s voxels adds 2 dir * (s / 2), the same float as dir * s
+        // (both factors are exact power-of-two scalings of the reference's).
+        const f3 dir2 = mul(dir, 2.0f);
+        // raycast_step_scale() / 2 (a normal float: its exponent less one)
+        const float half_scale =
+            __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(r.truncation * inv_vs)) - (1 << 23));
         float sdf = 1.0f, last_sdf, step;
         bool valid = false, back_face = false;
         while (dist_s < dist_e) {
@@ -513,20 +519,23 @@ LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char 
             if (sdf <= 0.5f && sdf >= -0.5f) sdf = read_interp<kMixed>(r, bc, res, valid, &fl);
             if (sdf <= 0.0f) break;
             if (sdf >= 0.0f && last_sdf < 0.0f) { back_face = true; break; }
-            step = valid ? fmaxf(sdf * step_scale, 1.0f) : 2.0f;
-            res = add(res, mul(dir, step));
-            dist_s += step;
+            // half the reference's step max(sdf * step_scale, 1) (or 2), exactly
+            const float half_step = valid ? fmaxf(sdf * half_scale, 0.5f) : 1.0f;
+            res = add(res, mul(dir2, half_step));
+            dist_s = __builtin_fmaf(2.0f, half_step, dist_s);  // dist_s + step, one rounding
             for (int k = 0; k < quiet && dist_s < dist_e; ++k) {  // each: a read that finds nothing, a 2-voxel step
-                res = add(res, mul(dir, 2.0f));
+                res = add(res, dir2);
                 dist_s += 2.0f;
             }
         }
         if (back_face || !(sdf <= 0.0f)) break;
+        const f3 dir1 = mul(dir2, 0.5f);  // dir again, exactly
+        const float step_scale = 2.0f * half_scale;
         step = sdf * step_scale;
-        res = add(res, mul(dir, step));
+        res = add(res, mul(dir1, step));
         sdf = read_interp<kMixed>(r, bc, res, valid);
         step = sdf * step_scale;
-        res = add(res, mul(dir, step));
+        res = add(res, mul(dir1, step));
         if (!valid) break;
         if (kCheckColour) {  // read_colour (ray_tracing.cu:427-441)
             bool cv;

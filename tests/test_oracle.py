@@ -237,6 +237,31 @@ def test_storage_with_a_missing_array_is_rejected_before_device_work():
             assert missing.encode() in lib.lf_last_error(), (missing, lib.lf_last_error())
 
 
+def test_map_entry_points_reject_bad_params_before_device_work():
+    """The parameters passed with a map's storage are checked like lf_map_create's
+    (positive sizes; a truncation of a normal number of voxels, so the raycast's
+    step scale and its half are normal floats) before any launch."""
+    from libfusion_amd import configs
+
+    lib = capi.load()
+    _, K, p0 = configs.c0()
+    fake = ctypes.c_void_p(0x10000)
+    arrays = [f for f, t in capi.LfMapStorage._fields_ if t is ctypes.c_void_p]
+    s = capi.LfMapStorage()
+    for i, f in enumerate(arrays):
+        setattr(s, f, 0x100000 * (i + 1))
+    s.tiles_x, s.tiles_y = K.width // 8, K.height // 8
+    for field, value in [("truncation", 0.0), ("truncation", 1e-41), ("voxel_size", -1.0), ("truncation", 1e35)]:
+        p = capi.LfMapParams.from_buffer_copy(p0)
+        setattr(p, field, value)
+        for call in [
+            lambda: lib.lf_raycast(ctypes.byref(p), ctypes.byref(s), fake, None, K.width, K.height, fake,
+                                   ctypes.byref(K), 1, None),
+            lambda: lib.lf_map_reset(ctypes.byref(p), ctypes.byref(s), None),
+        ]:
+            assert call() == capi.LF_ERR_ARG, (field, value)
+
+
 # ------------------------------------------------------------- SE(3), LDLT
 
 def twist_matrix(xi):

@@ -155,9 +155,13 @@ LF_HD unsigned int weight_code_sr(float w, unsigned int rnd) {
 // independently.  The voxel is named by where it is, not by its pool slot
 // (which the allocation race decides), so a map's values do not depend on
 // the order its blocks were allocated in.
-LF_HD unsigned int voxel_rand(int bx, int by, int bz, unsigned int local, unsigned int pass) {
+constexpr unsigned int kRandLocalMul = 0x27D4EB2Fu;
+// voxel_rand with its voxel term local * kRandLocalMul given (mod 2^32,
+// (local + (pass << 9)) * k = local * k + (pass << 9) * k, so a caller that
+// visits many blocks computes each lane's product once).
+LF_HD unsigned int voxel_rand_mixed(int bx, int by, int bz, unsigned int local_mul, unsigned int pass) {
     unsigned int x = (unsigned int)bx * 0x9E3779B1u ^ (unsigned int)by * 0x85EBCA77u ^
-                     (unsigned int)bz * 0xC2B2AE3Du ^ (local + (pass << 9)) * 0x27D4EB2Fu;
+                     (unsigned int)bz * 0xC2B2AE3Du ^ (local_mul + (pass << 9) * kRandLocalMul);
     x ^= x >> 16;
     x *= 0x7FEB352Du;
     x ^= x >> 15;
@@ -165,12 +169,28 @@ LF_HD unsigned int voxel_rand(int bx, int by, int bz, unsigned int local, unsign
     x ^= x >> 16;
     return x;
 }
+LF_HD unsigned int voxel_rand(int bx, int by, int bz, unsigned int local, unsigned int pass) {
+    return voxel_rand_mixed(bx, by, bz, local * kRandLocalMul, pass);
+}
 
 // *(T *)((char *)base + off): a 32-bit byte offset from a (scalar) base, so
 // the access takes the offset in one VGPR instead of 64-bit address math.
 template <class T> LF_DEV T &at_byte(T *base, unsigned int off) {
     using C = typename std::conditional<std::is_const<T>::value, const char, char>::type;
     return *reinterpret_cast<T *>(reinterpret_cast<C *>(base) + off);
+}
+
+// A float that every active lane holds alike, moved to a scalar register.
+LF_DEV float uniform(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+
+// A 32-bit byte offset held in a vector register at this point of the code
+// (an empty asm statement, no instruction): the compiler can neither hoist it
+// out of a loop nor widen it to 64 bits early, so a load or store at_byte(p,
+// off) with a uniform p keeps the scalar-base + 32-bit-offset address form
+// instead of a 64-bit vector add per access.
+LF_DEV unsigned int vgpr_offset(unsigned int v) {
+    asm volatile("" : "+v"(v));
+    return v;
 }
 
 // Occupancy bit of a block: its 4x4x4-block region (rx, ry, rz) = (bx, by, bz)

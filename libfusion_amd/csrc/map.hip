@@ -376,6 +376,8 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
     // The block's list entry is the same for every lane (scalar registers,
     // loaded one iteration ahead); voxel and pixel addresses are 32-bit byte
     // offsets from a scalar base.
+    // the voxel terms of the stochastic rounding's hash, once per lane
+    const unsigned int rand_local[2] = {(unsigned int)t * kRandLocalMul, (unsigned int)(t + 256) * kRandLocalMul};
     const int4 *vpos = reinterpret_cast<const int4 *>(m.visible_pos);  // x, y, z, pool block
     int4 p_next = wg < nvis ? vpos[wg] : make_int4(0, 0, 0, 0);
     for (int i = wg; i < nvis; i += nwg) {
@@ -438,7 +440,7 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
         short sp[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            vo[h] = (unsigned int)(t + 256 * h);
+            vo[h] = vgpr_offset((unsigned int)(t + 256 * h));
             if constexpr (kMixed) {
                 word[h] = at_byte(wd_b, 4u * vo[h]);  // weight code and colour together
                 sp[h] = at_byte(sdf_b, 2u * vo[h]);
@@ -448,7 +450,7 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             if (kPacked) {
-                const float4 smp = at_byte(samples, 16u * pix[h]);
+                const float4 smp = at_byte(samples, vgpr_offset(16u * pix[h]));
                 dist[h] = smp.x;  // NaN also when the normal is invalid
                 wpk[h] = smp.y;
                 cpk[h] = __float_as_uint(smp.z);
@@ -502,7 +504,7 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
             if constexpr (kMixed) {
                 unsigned short *sdf16 = reinterpret_cast<unsigned short *>(sdf_b);
                 // the stored weight is rounded stochastically (unbiased running sums)
-                const unsigned int rnd = voxel_rand(p.x, p.y, p.z, o, pass);
+                const unsigned int rnd = voxel_rand_mixed(p.x, p.y, p.z, rand_local[h], pass);
                 // without colour fusion the word's colour bits are kept
                 unsigned int colour = word[h] & ~0xFFu;
                 if (wp[h] == 0.0f) {
@@ -510,14 +512,14 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
                     // the reference (the smallest code stands for weights below
                     // it); a zero weight leaves the voxel unseen
                     const unsigned int code = wt > 0.0f ? max(1u, weight_code_sr(fminf(wt, 255.0f), rnd)) : 0u;
-                    at_byte(sdf16, 2u * o) = code == 0u ? kHalfUnseen : float_to_half_bits(sdf_h);
+                    at_byte(sdf16, vgpr_offset(2u * o)) = code == 0u ? kHalfUnseen : float_to_half_bits(sdf_h);
                     if (kColour) colour = word_colour_bits(cin[0], cin[1], cin[2]);
-                    at_byte(wd_b, 4u * o) = code | colour;
+                    at_byte(wd_b, vgpr_offset(4u * o)) = code | colour;
                     continue;
                 }
                 const float inv = __builtin_amdgcn_rcpf(wp[h] + wt);
                 const float spf = half_bits_to_float((unsigned short)sp[h]);
-                at_byte(sdf16, 2u * o) = float_to_half_bits((spf * wp[h] + sdf_h * wt) * inv);
+                at_byte(sdf16, vgpr_offset(2u * o)) = float_to_half_bits((spf * wp[h] + sdf_h * wt) * inv);
                 const unsigned int code = weight_code_sr(fminf(wp[h] + wt, 255.0f), rnd);
                 if (kColour) {
                     unsigned int ch[3];
@@ -528,7 +530,7 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
                     }
                     colour = word_colour_bits(ch[0], ch[1], ch[2]);
                 }
-                at_byte(wd_b, 4u * o) = code | colour;
+                at_byte(wd_b, vgpr_offset(4u * o)) = code | colour;
                 continue;
             }
             if (wp[h] == 0.0f) {

@@ -270,8 +270,6 @@ LF_DEV int ray_floor(float v) {
     asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(v));
     return r;
 }
-// A float that every active lane holds alike, in a scalar register.
-LF_DEV float uniform(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
 // 1 when v's sign bit is clear, else 0 (for a v neither NaN nor zero: v > 0)
 LF_DEV unsigned int positive(float v) { return ~__float_as_uint(v) >> 31; }
 

@@ -246,7 +246,7 @@ struct BlockCache {
     int bx, by, bz, b;
     unsigned long long *lds;  // null: no shared cache (coordinates / pool out of range)
     const unsigned int *occ;  // the map's occupancy bits, copied to LDS
-    bool empty_region;        // b < 0 because the block's whole 4x4x4-block region is unallocated
+    int box;                  // b < 0: 2 when the block's whole 4x4x4-block region is unallocated, else 0
 };
 
 LF_DEV bool fits13(int v) { return v >= -4096 && v < 4096; }
@@ -306,8 +306,9 @@ LF_DEV bool region_occupied(const BlockCache &bc, int bx, int by, int bz) {
 LF_DEV int resolve(const RayDev &r, BlockCache &bc, int bx, int by, int bz) {
     if (bx == bc.bx && by == bc.by && bz == bc.bz) return bc.b;
     bc.bx = bx; bc.by = by; bc.bz = bz;
-    bc.empty_region = !region_occupied(bc, bx, by, bz);
-    if (bc.empty_region) {
+    const bool empty_region = !region_occupied(bc, bx, by, bz);
+    bc.box = empty_region ? 2 : 0;
+    if (empty_region) {
         bc.b = -1;
         return -1;
     }
@@ -480,7 +481,7 @@ LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char 
             break;
         f3 res = bs;
         BlockCache bc = {kBad, kBad, kBad, -1, r.pool_fits_lds ? s_cache : nullptr,
-                         reinterpret_cast<const unsigned int *>(s_occ), false};
+                         reinterpret_cast<const unsigned int *>(s_occ), 0};
         // A sample in an unallocated block (or empty 4x4x4-block region) is
         // followed by the reference's 2-voxel steps for as long as they stay
         // inside that box, where every read finds nothing: those steps are
@@ -506,7 +507,7 @@ LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char 
             sdf = read_sdf<kMixed>(r, bc, res, valid, nullptr, &fl);
             int quiet = 0;  // further samples surely inside the same unallocated box
             if (bc.b < 0) {
-                const int k = bc.empty_region ? 2 : 0, sh = k + 3;  // region: 32 voxels a side; block: 8
+                const int k = bc.box, sh = k + 3;  // region: 32 voxels a side; block: 8
                 const float tx = ((float)(((bc.bx >> k) + (int)(fwd & 1u)) << sh) - res.x) * inv_dir.x;
                 const float ty = ((float)(((bc.by >> k) + (int)((fwd >> 1) & 1u)) << sh) - res.y) * inv_dir.y;
                 const float tz = ((float)(((bc.bz >> k) + (int)(fwd >> 2)) << sh) - res.z) * inv_dir.z;

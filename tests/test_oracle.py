@@ -8,6 +8,7 @@ import hashlib
 import os
 import re
 import struct
+import sys
 
 import numpy as np
 import pytest
@@ -608,6 +609,22 @@ def test_synthetic_frames_are_deterministic():
     valid = d > 0
     assert 0.97 < valid.mean() < 0.99  # 2% dropouts
     assert 1.5 < np.median(d[valid]) < 4.0
+
+
+def test_bench_cpu_leg_is_bounded_by_time_not_by_gpu_frames():
+    """bench.py's CPU baseline generates its own frames of stream 100, so asking
+    for more frames than a short GPU run generates (60 against --steps 20
+    --warmup 5) is bounded by the time budget, not by an index; both legs (all
+    threads, then one) report what they ran."""
+    sys.path.insert(0, ROOT)
+    import bench
+    from libfusion_amd import capi
+
+    cpu = bench.cpu_baseline(bench.c1_params(capi.PRECISION_FP32), 60, 0.2, 0.1)
+    assert cpu["kind"] == "port" and cpu["unit"] == "frames/s" and cpu["cores"] >= 1
+    assert cpu["value"] > 0 and cpu["one_core"]["value"] > 0
+    frames = int(re.search(r"frames 0-(\d+)", cpu["sample"]).group(1)) + 1
+    assert 1 <= frames < 60
 
 
 # ---------------------------------------------- oracle float vs double system

@@ -103,11 +103,12 @@ LF_HD unsigned short float_to_half_bits(float v) { return __builtin_bit_cast(uns
 // 8-bit weight code: an unsigned float with 4 exponent bits (bias 7) and 4
 // mantissa bits.  Code 0 is 0; codes 1..15 are m * 2^-10; code e<<4 | m with
 // e >= 1 is (1 + m/16) * 2^(e-7), up to 496.
+// For c < 256, ((e + 120) << 23) | (m << 19) is (c << 19) + (120 << 23),
+// and below 16 the code is m itself.
 LF_HD float weight_value(unsigned int c) {
-    const unsigned int e = c >> 4, m = c & 15u;
-    const float sub = (float)m * (1.0f / 1024.0f);
-    const float nrm = __builtin_bit_cast(float, ((e + 120u) << 23) | (m << 19));
-    return e == 0 ? sub : nrm;  // both computed: a select, not a branch
+    const float sub = (float)c * (1.0f / 1024.0f);
+    const float nrm = __builtin_bit_cast(float, (c << 19) + (120u << 23));
+    return c < 16u ? sub : nrm;  // both computed: a select, not a branch
 }
 // Nearest code of w >= 0, ties to the even code; above 496, 0xFF.  Below 2^-6
 // the code is round(w * 1024) (exact scaling; 16 rounds up into code 0x10 =

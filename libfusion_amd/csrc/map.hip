@@ -350,16 +350,29 @@ int visible_grid(const lf_map_params *p) {
 // and the projection and the weighted means use one reciprocal each instead
 // of correctly rounded divisions.
 // Workgroup wg of nwg (grid-stride over the visible list).
-template <bool kWeighted, bool kColour, bool kPacked, bool kMixed>
+// The 256 weight-code values (weight_value) in LDS: a mixed voxel's weight
+// is one LDS read instead of the decode's arithmetic.  Every thread of the
+// workgroup calls it.
+LF_DEV void fill_weight_table(float *s) {
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) s[i] = weight_value((unsigned int)i);
+    __syncthreads();
+}
+
+// kTable: the weight codes' values from wtab, an LDS array of 256 floats that
+// this function fills (every thread of the workgroup calls it).
+template <bool kWeighted, bool kColour, bool kPacked, bool kMixed, bool kTable = false>
 LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const float *__restrict__ nmap,
                          const unsigned char *__restrict__ rgb, int cols, int rows, const float *__restrict__ pose24,
-                         const lf_intrinsics &K, const float4 *__restrict__ samples, int wg, int nwg) {
+                         const lf_intrinsics &K, const float4 *__restrict__ samples, int wg, int nwg,
+                         float *wtab = nullptr) {
     static_assert(!kPacked || kWeighted, "packed samples carry the weighted update's weight");
     float Tcw[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) Tcw[i] = pose24[12 + i];
     const int nvis = (int)m.counters[LF_CNT_VISIBLE];
     const unsigned int pass = kMixed ? m.counters[LF_CNT_FUSE_PASSES] : 0u;
+    // after the loads above: they stay scalar loads ahead of the barrier
+    if constexpr (kTable) fill_weight_table(wtab);
     if (wg == 0 && threadIdx.x == 0) {
         atomicAdd(&m.counters[LF_CNT_UPDATES], 1u);
         atomicAdd(reinterpret_cast<unsigned long long *>(&m.counters[LF_CNT_VIS_SUM_LO]), (unsigned long long)nvis);
@@ -487,7 +500,8 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
         }
         float wp[2];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) wp[h] = kMixed ? weight_value(word_code(word[h])) : wraw[h];
+        for (int h = 0; h < 2; ++h)
+            wp[h] = kMixed ? (kTable ? wtab[word_code(word[h])] : weight_value(word_code(word[h]))) : wraw[h];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             if (!ok[h]) continue;
@@ -565,8 +579,9 @@ __global__ __launch_bounds__(256) void k_integrate(MapDev m, const float *__rest
                                                    const unsigned char *__restrict__ rgb, int cols, int rows,
                                                    const float *__restrict__ pose24, lf_intrinsics K,
                                                    const float4 *__restrict__ samples = nullptr) {
-    integrate_wg<kWeighted, kColour, kPacked, kMixed>(m, depth, nmap, rgb, cols, rows, pose24, K, samples, blockIdx.x,
-                                                      gridDim.x);
+    __shared__ float s_wv[kMixed ? 256 : 1];
+    integrate_wg<kWeighted, kColour, kPacked, kMixed, kMixed>(m, depth, nmap, rgb, cols, rows, pose24, K, samples,
+                                                              blockIdx.x, gridDim.x, s_wv);
 }
 
 // One map's weighted update from a frame's packed samples within a batch
@@ -604,18 +619,19 @@ __global__ __launch_bounds__(kVisThreads) void k_visible_batch(const FuseJob *__
 
 template <bool kMixed>
 __global__ __launch_bounds__(256) void k_integrate_batch(const FuseJob *__restrict__ jobs) {
+    __shared__ float s_wv[256];
     const FuseJob &j = jobs[blockIdx.y];
-    if (j.tiles_x == 0) return;  // a map without an update this step
+    if (j.tiles_x == 0) return;  // a map without an update this step (the whole workgroup)
     const MapDev m = global_dev(j.m);
     const float *depth = global_ptr(j.depth), *pose24 = global_ptr(j.pose24);
     const float4 *samples = global_ptr(j.samples);
     const unsigned char *rgb = global_ptr(j.rgb);
     if (rgb)
-        integrate_wg<true, true, true, kMixed>(m, depth, nullptr, rgb, j.cols, j.rows, pose24, j.K, samples,
-                                               blockIdx.x, gridDim.x);
+        integrate_wg<true, true, true, kMixed, kMixed>(m, depth, nullptr, rgb, j.cols, j.rows, pose24, j.K, samples,
+                                                       blockIdx.x, gridDim.x, s_wv);
     else
-        integrate_wg<true, false, true, kMixed>(m, depth, nullptr, rgb, j.cols, j.rows, pose24, j.K, samples,
-                                                blockIdx.x, gridDim.x);
+        integrate_wg<true, false, true, kMixed, kMixed>(m, depth, nullptr, rgb, j.cols, j.rows, pose24, j.K, samples,
+                                                        blockIdx.x, gridDim.x, s_wv);
 }
 
 // De-integration: the inverse of the weighted update for one frame, over the

@@ -285,7 +285,7 @@ def tuned_leg(streams, cfgs, warmup, steps, runs):
     rs = batch_runs(streams, cfgs, warmup, steps, runs)
     fps = median([r["fps"] for r in rs])
     return {"value": round(fps, 1), "runs": [round(r["fps"], 1) for r in rs], "tuning_steps": f,
-            "options": chosen, "note": "0 = the built-in rule (1024 integration workgroups per map, 512 / n "
+            "options": chosen, "note": "0 = the built-in rule (2048 integration workgroups per map, 512 / n "
                                       "tracking workgroups per system, 64 z-range workgroups per map at 16 maps)"}
 
 

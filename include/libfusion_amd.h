@@ -601,7 +601,7 @@ int lf_batch_read_timing(lf_batch *b, int stage, int *launches, double *total_ms
  *   LF_BATCH_OPT_TRACK_WORKGROUPS: workgroups per system of each Gauss-Newton
  *     step, 1..512 (0, the default: 512 / n, at most the level's pixels / 512).
  *   LF_BATCH_OPT_INTEGRATE_WORKGROUPS: workgroups per map of the batched
- *     integration, 1..65535 (0, the default: max(1024, 8192 / n)).
+ *     integration, 1..65535 (0, the default: max(2048, 8192 / n)).
  *   LF_BATCH_OPT_ZRANGE_WORKGROUPS: workgroups per map of the batched z-range
  *     fill, 1..65535 (0, the default: 1/n of a lone map's, at least 64). */
 #define LF_BATCH_OPT_UPLOAD_OVERLAP 0
@@ -620,7 +620,7 @@ int lf_batch_get_option(lf_batch *b, int option, int *value);
 int lf_batch_set_graphs(lf_batch *b, int enable);
 /* Online autotuning of the launch options above (no reference counterpart).
  * The batch's next steps try each option's candidate values in turn, one
- * option at a time (integration workgroups {default, 512, 2048, 4096},
+ * option at a time (integration workgroups {default, 512, 1024, 4096},
  * tracking workgroups {default, 16, 48, 64}, z-range workgroups {default,
  * 32, 128, 256}, upload overlap {1, 0}); each candidate runs one settling step
  * and steps_per_trial steps timed by events on the batch stream, in `rounds`

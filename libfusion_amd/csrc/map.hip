@@ -1141,10 +1141,10 @@ int lf_update_batch_make(const LfFuseItem *items, int n, void *jobs_host, LfFuse
         d.tiles_max = std::max(d.tiles_max, j.tiles_x * j.tiles_y);
         d.vis_grid_max = std::max(d.vis_grid_max, j.vis_grid);
     }
-    // the single launch's 8192 workgroups shared out, at least 1024 per map
-    // (16 maps: 256 / 512 / 1024 / 2048 / 4096 per map gave 5.34k / 5.44k /
-    // 5.50-5.53k / 5.52k / 5.53k frames/s aggregate in mixed precision)
-    d.integrate_grid = std::max(1024, integrate_grid() / n);
+    // the single launch's 8192 workgroups shared out, at least 2048 per map
+    // (16 mixed maps, frames 10-49, 5 alternating runs: 1024 / 2048 / 4096
+    // per map gave 7376 / 7433 / 7429 frames/s aggregate)
+    d.integrate_grid = std::max(2048, integrate_grid() / n);
     *dims = d;
     return LF_OK;
 }

@@ -651,7 +651,7 @@ struct TunedOption {
 // rule, which the sweeps behind it chose for 16 maps of 640x480).
 const std::vector<TunedOption> &tuned_options() {
     static const std::vector<TunedOption> o = {
-        {LF_BATCH_OPT_INTEGRATE_WORKGROUPS, {0, 512, 2048, 4096}},
+        {LF_BATCH_OPT_INTEGRATE_WORKGROUPS, {0, 512, 1024, 4096}},
         {LF_BATCH_OPT_TRACK_WORKGROUPS, {0, 16, 48, 64}},
         {LF_BATCH_OPT_ZRANGE_WORKGROUPS, {0, 32, 128, 256}},
         {LF_BATCH_OPT_UPLOAD_OVERLAP, {1, 0}},

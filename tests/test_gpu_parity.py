@@ -1187,7 +1187,7 @@ def test_batch_autotune_picks_caches_and_keeps_results(tmp_path):
     tuned.autotune(1, 1)
     opts = (capi.BATCH_OPT_INTEGRATE_WORKGROUPS, capi.BATCH_OPT_TRACK_WORKGROUPS, capi.BATCH_OPT_ZRANGE_WORKGROUPS,
             capi.BATCH_OPT_UPLOAD_OVERLAP)
-    cands = ({0, 512, 2048, 4096}, {0, 16, 48, 64}, {0, 32, 128, 256}, {1, 0})
+    cands = ({0, 512, 1024, 4096}, {0, 16, 48, 64}, {0, 32, 128, 256}, {1, 0})
     for f in range(30):
         for b in (tuned, plain):
             b.process([fs[f][0] for fs in frames], [fs[f][1] for fs in frames])

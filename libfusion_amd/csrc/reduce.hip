@@ -643,12 +643,15 @@ LF_DEV double block_sums_d(CombinedSums &sums, double (*s_wave)[32]) {
 }
 
 // final_sums of the G combined FP64 partial rows (value v of workgroup g at
-// v * G + g), in a fixed order; the kTrackVals totals land in s_sum.
-LF_DEV void final_sums_d(const double *partials, int G, double *s_sum) {
+// v * G + g), in a fixed order; the kTrackVals totals land in s_sum.  kChunks
+// 64-lane chunks of partials per value, enough for G (the chunks past G hold
+// only masked zeros, which leave the sums unchanged: a sum started at +0 is
+// never -0).
+template <int kChunks>
+LF_DEV void final_sums_d_chunks(const double *partials, int G, double *s_sum) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int kPer = 4;
     static_assert(kPer * kWaves >= kTrackVals, "values per wavefront");
-    constexpr int kChunks = kGrid / 64;
     double acc[kPer];
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
@@ -670,6 +673,11 @@ LF_DEV void final_sums_d(const double *partials, int G, double *s_sum) {
     const double s = transpose_reduce<kPer>(acc);
     const int v = wave + kWaves * (lane >> 4);
     if ((lane & 15) == 0 && v < kTrackVals) s_sum[v] = s;
+}
+// (a batch of 16 systems: 32 partials per value, one chunk)
+LF_DEV void final_sums_d(const double *partials, int G, double *s_sum) {
+    if (G <= 64) final_sums_d_chunks<1>(partials, G, s_sum);
+    else final_sums_d_chunks<kGrid / 64>(partials, G, s_sum);
 }
 
 // Stand-alone reductions (lf_icp_reduce, lf_rgb_step).  kMode 1: ICP rows,

@@ -205,6 +205,10 @@ LF_DEV void alloc_wg(const MapDev &m, const float *__restrict__ depth, int cols,
                      unsigned long long *s_set, unsigned long long *slot_mask = nullptr,
                      unsigned long long bit = 0ull) {
     const int tid = threadIdx.x;
+    const int x = tx * 16 + (tid & 15), y = ty * 16 + (tid >> 4);
+    const bool inside = x < cols && y < rows;
+    // the pixel's depth is in flight while the request set is cleared
+    const float z = inside ? depth[(size_t)y * cols + x] : 0.0f;
     for (int i = tid; i < kDedup; i += 256) s_set[i] = kEmptyKey;
     if (tx == 0 && ty == 0 && tid == 0) {
         m.counters[LF_CNT_VISIBLE] = 0u;
@@ -216,9 +220,8 @@ LF_DEV void alloc_wg(const MapDev &m, const float *__restrict__ depth, int cols,
 #pragma unroll
     for (int i = 0; i < 12; ++i) Twc[i] = pose24[i];
     __syncthreads();
-    const int x = tx * 16 + (tid & 15), y = ty * 16 + (tid >> 4);
-    if (x >= cols || y >= rows) return;
-    band_blocks<kFast>(x, y, depth[(size_t)y * cols + x], Twc, K, m, [&](int bx, int by, int bz) {
+    if (!inside) return;
+    band_blocks<kFast>(x, y, z, Twc, K, m, [&](int bx, int by, int bz) {
         const unsigned long long key = pack_key(bx, by, bz);
         unsigned int h = key_mix(key) & (kDedup - 1);
         for (int probe = 0; probe < kDedup; ++probe) {

@@ -143,12 +143,13 @@ class Streams:
         return self.pinned
 
 
-def batch_runs(streams, cfgs, warmup, steps, runs, timing_first=0, host=False, graphs=False):
+def batch_runs(streams, cfgs, warmup, steps, runs, timing_first=0, host=False, graphs=False, options=None):
     """Aggregate frames/s of the streams stepped together by lf_batch: `runs`
     fresh batches over the same frames, each W untimed steps then exactly K
     timed steps bracketed by synchronisation.  The first run records the
     stage events of timing_first (HIP events on the batch stream).  graphs:
-    steady-state steps replayed from HIP graphs (lf_batch_set_graphs)."""
+    steady-state steps replayed from HIP graphs (lf_batch_set_graphs).  options:
+    {BATCH_OPT_*: value} set on each batch."""
     import torch
 
     from libfusion_amd import capi
@@ -157,6 +158,8 @@ def batch_runs(streams, cfgs, warmup, steps, runs, timing_first=0, host=False, g
     out = []
     for r in range(runs):
         b = capi.Batch(cfgs)
+        for o, v in (options or {}).items():
+            b.set_option(o, v)
         if graphs:
             b.set_graphs(True)
         step = (lambda f: b.process_host([d[f] for d, _ in src], [c[f] for _, c in src])) if host else \
@@ -456,6 +459,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline and its plain FP32 comparison (no breakdown, PCIe or C3 legs)")
+    ap.add_argument("--upload-overlap", type=int, default=1, choices=(0, 1),
+                    help="0: the headline batch uploads each step's frames on its own stream ahead of the "
+                         "tracking (no kernel runs beside the integration; for kernel traces)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device-to-device (a plain copy kernel of known size, "
                          "to scale rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -492,7 +498,8 @@ def main():
 
     # headline: mixed precision, batched
     kmask = (1 << capi.BATCH_INTEGRATE) | (1 << capi.BATCH_RAYCAST)
-    head = batch_runs(streams, cfgs(mixed), W, K, args.runs, timing_first=kmask)
+    head = batch_runs(streams, cfgs(mixed), W, K, args.runs, timing_first=kmask,
+                      options={capi.BATCH_OPT_UPLOAD_OVERLAP: args.upload_overlap})
     fps = median([r["fps"] for r in head])
     log("mixed batched: runs %s frames/s, median %.1f" % (", ".join("%.1f" % r["fps"] for r in head), fps))
     # the comparison: plain FP32, one System per stream, same frames
@@ -600,7 +607,7 @@ def main():
                                "voxels, 15 mm truncation, 2^20 buckets, 3-level ICP+RGB {10,5,3}; one step = one "
                                "frame of every stream; frames %d-%d timed" % (n, SEED0, SEED0 + n - 1, W, W + K - 1),
                    "streams": n, "width": 640, "height": 480, "voxel_mm": 5, "buckets": 2 ** 20,
-                   "pool_blocks": c1_params(mixed).num_blocks, "precision": "mixed", "launch": "lf_batch"},
+                   "pool_blocks": c1_params(mixed).num_blocks, "precision": "mixed", "launch": "lf_batch", "upload_overlap": args.upload_overlap},
         "roofline": rl,
         "stage_rooflines": stage_rooflines,
         "cpu_baseline": cpu,

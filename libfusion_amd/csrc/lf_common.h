@@ -181,9 +181,6 @@ template <class T> LF_DEV T &at_byte(T *base, unsigned int off) {
     return *reinterpret_cast<T *>(reinterpret_cast<C *>(base) + off);
 }
 
-// A float that every active lane holds alike, moved to a scalar register.
-LF_DEV float uniform(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
-
 // A 32-bit byte offset held in a vector register at this point of the code
 // (an empty asm statement, no instruction): the compiler can neither hoist it
 // out of a loop nor widen it to 64 bits early, so a load or store at_byte(p,

@@ -46,6 +46,9 @@ struct RayDev {
     int tiles_x, tiles_y;
     int num_buckets, num_entries;
     float voxel_size, truncation, zmin_raycast, zmax_raycast;
+    // 1 / voxel_size and raycast_step_scale() / 2 = truncation / voxel_size / 2,
+    // by the same float operations on the host
+    float inv_voxel_size, half_step_scale;
     int pool_fits_lds;  // pool indices fit the 24-bit field of the LDS cache
 };
 
@@ -464,8 +467,7 @@ LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char 
         const float z0 = r.zx[t], z1 = r.zy[t];
         if (z1 < 1e-3f || z0 < 1e-3f || isnan(z0) || isnan(z1)) break;
         const float *Twc = pose24;
-        // uniform over the workgroup (one map): kept in scalar registers
-        const float inv_vs = uniform(1.0f / r.voxel_size);
+        const float inv_vs = r.inv_voxel_size;  // the map's (scalar registers)
         f3 pt = {((float)x - K.cx) * K.invfx * z0, ((float)y - K.cy) * K.invfy * z0, z0};
         float dist_s = norm(pt) * inv_vs;
         const f3 bs = mul(xform(Twc, pt), inv_vs);
@@ -496,9 +498,7 @@ LF_DEV void raycast_wg(const RayDev &r, float *__restrict__ vmap, unsigned char 
         // step of s voxels adds 2 dir * (s / 2), the same float as dir * s
         // (both factors are exact power-of-two scalings of the reference's).
         const f3 dir2 = mul(dir, 2.0f);
-        // raycast_step_scale() / 2 (a normal float: its exponent less one)
-        const float half_scale =
-            __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(r.truncation * inv_vs)) - (1 << 23));
+        const float half_scale = r.half_step_scale;
         float sdf = 1.0f, last_sdf, step;
         bool valid = false, back_face = false;
         while (dist_s < dist_e) {
@@ -611,6 +611,8 @@ static RayDev make_raydev(const lf_map_params *p, const lf_map_storage *s) {
     r.tiles_x = s->tiles_x; r.tiles_y = s->tiles_y;
     r.num_buckets = p->num_buckets; r.num_entries = p->num_entries;
     r.voxel_size = p->voxel_size; r.truncation = p->truncation;
+    r.inv_voxel_size = 1.0f / p->voxel_size;
+    r.half_step_scale = 0.5f * (p->truncation * r.inv_voxel_size);  // exact: a normal float (checked with the params)
     r.zmin_raycast = p->zmin_raycast; r.zmax_raycast = p->zmax_raycast;
     r.pool_fits_lds = p->num_blocks < (int)kNoBlock ? 1 : 0;
     return r;

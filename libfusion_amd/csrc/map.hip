@@ -46,6 +46,7 @@ struct MapDev {
     unsigned int *occupancy;
     int num_buckets, num_entries, num_blocks;
     float voxel_size, truncation, zmin_update, zmax_update;
+    float inv_truncation;  // 1 / truncation, computed on the host (the same correctly rounded division)
 };
 
 // A map read from a descriptor table, its pointers asserted global.
@@ -67,6 +68,7 @@ MapDev make_dev(const lf_map_params *p, const lf_map_storage *s) {
     d.block_dir = s->block_dir;
     d.num_buckets = p->num_buckets; d.num_entries = p->num_entries; d.num_blocks = p->num_blocks;
     d.voxel_size = p->voxel_size; d.truncation = p->truncation;
+    d.inv_truncation = 1.0f / p->truncation;
     d.zmin_update = p->zmin_update; d.zmax_update = p->zmax_update;
     return d;
 }
@@ -380,7 +382,7 @@ LF_DEV void integrate_wg(const MapDev &m, const float *__restrict__ depth, const
         atomicAdd(&m.counters[LF_CNT_UPDATES], 1u);
         atomicAdd(reinterpret_cast<unsigned long long *>(&m.counters[LF_CNT_VIS_SUM_LO]), (unsigned long long)nvis);
     }
-    const float vs = m.voxel_size, trunc = m.truncation, inv_trunc = 1.0f / trunc;
+    const float vs = m.voxel_size, trunc = m.truncation, inv_trunc = m.inv_truncation;
     const float zmin = m.zmin_update, zmax = m.zmax_update;
     const int t = threadIdx.x;
     const int lx = t & 7, ly = (t >> 3) & 7, lz0 = t >> 6;
@@ -655,7 +657,7 @@ __global__ __launch_bounds__(256) void k_deintegrate(MapDev m, const float *__re
     for (int i = 0; i < 12; ++i) Tcw[i] = pose24[12 + i];
     const int nvis = (int)m.counters[LF_CNT_VISIBLE];
     const unsigned int pass = kMixed ? m.counters[LF_CNT_FUSE_PASSES] : 0u;
-    const float vs = m.voxel_size, trunc = m.truncation, inv_trunc = 1.0f / trunc;
+    const float vs = m.voxel_size, trunc = m.truncation, inv_trunc = m.inv_truncation;
     const float zmin = m.zmin_update, zmax = m.zmax_update;
     const int t = threadIdx.x;
     const int lx = t & 7, ly = (t >> 3) & 7, lz0 = t >> 6;
@@ -880,7 +882,7 @@ LF_DEV bool refuse_observe(const RefuseArgs &a, int k, const float *Tcw, float w
     if (isnan(d) || d > m.zmax_update || d < m.zmin_update) return false;
     const float s = d - pt.z;
     if (s < -m.truncation) return false;
-    sdf = fminf(1.0f, s * (1.0f / m.truncation));
+    sdf = fminf(1.0f, s * m.inv_truncation);
     wt = smp.y;
     rgb = __float_as_uint(smp.z);
     return true;
